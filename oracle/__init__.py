@@ -1,0 +1,263 @@
+"""ctypes front end of the oracle (test infrastructure only).
+
+Two libraries live here:
+
+* ``libnrl_oracle.so`` -- the C restatement (nrl_oracle.c) of the reference's
+  per-k path; the checker for the HIP product and the CPU baseline.
+* ``_ref/libnrlref.so`` -- the reference's own Fortran (NRLtb/ex1 sources,
+  compiled where they lie) behind the bind(C) driver ref/ref_driver.f.  It
+  exists only where the reference tree was present at build time and is
+  used to pin the restatement.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg
+import this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "libnrl_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libnrlref.so")
+
+_d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_c = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle not built: {ORACLE_SO} (run make -C oracle)")
+        L = C.CDLL(ORACLE_SO)
+        L.orc_search2.argtypes = [C.c_int, _d, _d, _i, C.c_int, _i, _d, _d, C.c_int,
+                                  C.POINTER(C.c_int), _d, _d, _d, _d, _d, _i, _i,
+                                  C.POINTER(C.c_double)]
+        L.orc_setpar.argtypes = [C.c_int, _i, C.c_int, _d, C.c_int, C.c_int, C.c_int,
+                                 C.c_int, C.c_int, _d, _d, _i, _i, _d, _d, _d, _d]
+        L.orc_setparv.argtypes = [C.c_int, _i, C.c_int, _d, C.c_int, C.c_int, C.c_int,
+                                  C.c_int, C.c_double, C.c_int, _d, _d, _d, _i, _i, _d,
+                                  _d, _d, _d]
+        L.orc_setham.argtypes = [C.c_int, _d, C.c_int, _i, _i, _i, _d, C.c_int, _d, _d,
+                                 _i, _i, _d, _d, _d, C.c_int, C.c_int, _c, _c]
+        L.orc_setham.restype = None
+        L.orc_unpack.argtypes = [C.c_int, _c, _c]
+        L.orc_unpack.restype = None
+        L.orc_zhegv.argtypes = [C.c_int, _c, _c, _d, C.c_void_p]
+        L.orc_perturb.argtypes = [C.c_int, _c, _c, _c, _d, C.c_double,
+                                  C.POINTER(C.c_int), _d]
+        L.orc_fermi.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _d, _d,
+                                C.c_double, C.c_double]
+        L.orc_fermi.restype = C.c_double
+        L.orc_bande_sums.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _d,
+                                     _d, _d, _i, C.c_double, C.c_double,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]
+        L.orc_bande_sums.restype = None
+        L.orc_bande.argtypes = [C.c_int, C.c_int, C.c_int, _i, _i, _i, _d, C.c_int, _d,
+                                _d, _i, _i, _d, _d, _d, _d, _d, _d, C.c_int, _d, _d,
+                                C.c_double, C.c_double, C.c_double, C.c_int,
+                                C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    """The reference's own routines (only where it was built)."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError("reference build not present (make -C oracle ref)")
+        L = C.CDLL(REF_SO)
+        ip = C.POINTER(C.c_int)
+        dp = C.POINTER(C.c_double)
+        L.ref_setup.argtypes = [ip, _d, _d, _d, _i, _i, ip, _i, _d, _d, _d, _d, ip, ip,
+                                ip, ip, dp, ip]
+        L.ref_get_pairs.argtypes = [_d, _d, _d, _d, _d, _i, _i]
+        L.ref_setpar.argtypes = [dp, _d, _d, _d, _d, _d, _d, _d, ip]
+        L.ref_setham.argtypes = [_d, ip, _c, _c]
+        L.ref_diag.argtypes = [ip, _c, _c, _d, _c, ip]
+        L.ref_perturb.argtypes = [_c, _c, _d, dp, ip, _d]
+        L.ref_fermi.argtypes = [ip, dp, _d, _d, dp, dp]
+        L.ref_fermi.restype = C.c_double
+        L.ref_bande.argtypes = [ip, ip, _d, _d, dp, dp, dp, dp, dp, dp, dp, ip]
+        _ref = L
+    return _ref
+
+
+class Setup:
+    """Pair list + SK parameters of one structure, from the C restatement."""
+
+    def __init__(self, st, maxpairs: Optional[int] = None):
+        L = lib()
+        p = st.params
+        nat = st.natoms
+        maxpairs = maxpairs or max(1000, 400 * nat)
+        self.st = st
+        self.tt = np.zeros(3 * maxpairs); self.dlv = np.zeros(3 * maxpairs)
+        self.dist = np.zeros(maxpairs); self.screen = np.zeros(maxpairs)
+        self.dscrn = np.zeros(maxpairs)
+        self.jkind = np.zeros(2 * maxpairs, np.int32); self.jatm = np.zeros(2 * maxpairs, np.int32)
+        npairs = C.c_int(0); dmin = C.c_double(0)
+        ierr = L.orc_search2(nat, np.ascontiguousarray(st.lat.ravel()),
+                             np.ascontiguousarray(st.posn.ravel()), st.kkind, p.kinds,
+                             np.array(st.nd, np.int32), np.array(p.dnn, float),
+                             np.array(p.screenl, float), maxpairs, C.byref(npairs),
+                             self.tt, self.dlv, self.dist, self.screen, self.dscrn,
+                             self.jkind, self.jatm, C.byref(dmin))
+        if ierr:
+            raise RuntimeError("pair overflow")
+        self.npairs = npairs.value
+        self.dismin = dmin.value
+        n = self.npairs
+        self.den = np.zeros(nat * p.kinds); self.paros = np.zeros(4 * nat)
+        self.hop = np.zeros(14 * max(n, 1)); self.over = np.zeros(14 * max(n, 1))
+        self.dparos = np.zeros(4 * nat)
+        self.dhop = np.zeros(14 * max(n, 1)); self.dover = np.zeros(14 * max(n, 1))
+        args = (nat, st.kkind, p.kinds, np.ascontiguousarray(p.param, float), p.nltype,
+                int(p.realov), int(p.harrison), int(p.jsover))
+        L.orc_setpar(*args, n, self.dist, self.screen, self.jkind, self.jatm,
+                     self.den, self.paros, self.hop, self.over)
+        L.orc_setparv(*args, st.volume, n, self.dist, self.screen, self.dscrn,
+                      self.jkind, self.jatm, self.den, self.dparos, self.dhop, self.dover)
+        self.kbas = np.array(p.kbas, np.int32)
+        self.ksk = st.ksk
+        self.nv = st.nv
+        self.posq = np.ascontiguousarray(st.posq.ravel())
+
+    def hs(self, ak, deriv: bool = False):
+        """Full Hermitian H(k), S(k) (column-major as Fortran, returned as
+        numpy arrays indexed [row, col])."""
+        L = lib()
+        nv = self.nv
+        np_ = nv * (nv + 1) // 2
+        hp = np.zeros(np_, np.complex128); sp = np.zeros(np_, np.complex128)
+        if deriv:
+            L.orc_setham(nv, np.asarray(ak, float), self.st.natoms, self.st.kkind, self.ksk,
+                         self.kbas, self.posq, self.npairs, self.tt, self.dlv, self.jkind,
+                         self.jatm, self.dhop, self.dover, self.dparos, 1, 0, hp, sp)
+        else:
+            L.orc_setham(nv, np.asarray(ak, float), self.st.natoms, self.st.kkind, self.ksk,
+                         self.kbas, self.posq, self.npairs, self.tt, self.dlv, self.jkind,
+                         self.jatm, self.hop, self.over, self.paros, 1, 1, hp, sp)
+        return hp, sp
+
+    @staticmethod
+    def unpack(n, ap):
+        a = np.zeros(n * n, np.complex128)
+        lib().orc_unpack(n, np.ascontiguousarray(ap), a)
+        return a.reshape(n, n).T.copy()   # column-major -> [row, col]
+
+    def bande(self, kpts, weights, mode=4, tkb=0.005, eigcut=10.0, nthreads=1,
+              want_eig=False):
+        L = lib()
+        nk = len(kpts)
+        ef, e1, e2 = C.c_double(0), C.c_double(0), C.c_double(0)
+        eig = np.zeros(nk * self.nv) if want_eig else None
+        desk = np.zeros(nk * self.nv) if want_eig else None
+        err = L.orc_bande(mode, self.nv, self.st.natoms, self.st.kkind, self.ksk, self.kbas,
+                          self.posq, self.npairs, self.tt, self.dlv, self.jkind, self.jatm,
+                          self.hop, self.over, self.paros, self.dhop, self.dover,
+                          self.dparos, nk, np.ascontiguousarray(np.asarray(kpts, float).ravel()),
+                          np.ascontiguousarray(np.asarray(weights, float)), self.st.zfill,
+                          tkb, eigcut, nthreads, C.byref(ef), C.byref(e1), C.byref(e2),
+                          eig.ctypes.data if want_eig else None,
+                          desk.ctypes.data if want_eig else None)
+        out = dict(err=err, efermi=ef.value, earg1=e1.value, earg2=e2.value)
+        if want_eig:
+            out["eig"] = eig.reshape(nk, self.nv)
+            out["desk"] = desk.reshape(nk, self.nv)
+        return out
+
+
+def zhegv(h, s, vectors=True):
+    """Oracle generalized eigensolve of full Hermitian h, s ([row, col])."""
+    n = h.shape[0]
+    hc = np.ascontiguousarray(h.T, np.complex128).ravel()   # column-major
+    sc = np.ascontiguousarray(s.T, np.complex128).ravel()
+    w = np.zeros(n)
+    z = np.zeros(n * n, np.complex128) if vectors else None
+    info = lib().orc_zhegv(n, hc, sc, w, z.ctypes.data if vectors else None)
+    return info, w, (z.reshape(n, n).T.copy() if vectors else None)
+
+
+class RefSetup:
+    """The same structure pushed through the reference's own routines."""
+
+    def __init__(self, st):
+        L = ref()
+        p = st.params
+        ci = lambda v: C.byref(C.c_int(int(v)))
+        lat = np.ascontiguousarray(st.lat.ravel())       # lat(c,v) column-major
+        npo = C.c_int(0); dmin = C.c_double(0); ierr = C.c_int(0)
+        L.ref_setup(ci(st.natoms), lat, np.ascontiguousarray(st.posn.ravel()),
+                    np.ascontiguousarray(st.posq.ravel()), st.kkind,
+                    np.array(st.nd, np.int32), ci(p.kinds), np.array(p.kbas, np.int32),
+                    np.array(p.valence, float), np.array(p.dnn, float),
+                    np.array(p.screenl, float), np.ascontiguousarray(p.param, float),
+                    ci(len(p.param)), ci(p.nltype), ci(int(p.jsover)), C.byref(npo),
+                    C.byref(dmin), C.byref(ierr))
+        if ierr.value:
+            raise RuntimeError(f"ref_setup ierr={ierr.value}")
+        self.st = st
+        self.npairs = npo.value
+        self.dismin = dmin.value
+        n = self.npairs
+        self.tt = np.zeros(3 * n); self.dlv = np.zeros(3 * n); self.dist = np.zeros(n)
+        self.screen = np.zeros(n); self.dscrn = np.zeros(n)
+        self.jkind = np.zeros(2 * n, np.int32); self.jatm = np.zeros(2 * n, np.int32)
+        L.ref_get_pairs(self.tt, self.dlv, self.dist, self.screen, self.dscrn,
+                        self.jkind, self.jatm)
+        nat = st.natoms
+        self.paros = np.zeros(4 * nat); self.dparos = np.zeros(4 * nat)
+        self.hop = np.zeros(14 * n); self.over = np.zeros(14 * n)
+        self.dhop = np.zeros(14 * n); self.dover = np.zeros(14 * n)
+        self.den = np.zeros(2 * nat)
+        L.ref_setpar(C.byref(C.c_double(st.volume)), self.paros, self.hop, self.over,
+                     self.dparos, self.dhop, self.dover, self.den, C.byref(ierr))
+        self.nv = st.nv
+
+    def hs(self, ak, deriv=False):
+        np_ = self.nv * (self.nv + 1) // 2
+        hp = np.zeros(np_, np.complex128); sp = np.zeros(np_, np.complex128)
+        ref().ref_setham(np.asarray(ak, float), C.byref(C.c_int(int(deriv))), hp, sp)
+        return hp, sp
+
+    def diag(self, hp, sp, mode=4):
+        n = self.nv
+        e = np.zeros(n); z = np.zeros(n * n, np.complex128); io = C.c_int(0)
+        ref().ref_diag(C.byref(C.c_int(mode)), np.ascontiguousarray(hp),
+                       np.ascontiguousarray(sp), e, z, C.byref(io))
+        return io.value, e, z.reshape(n, n).T.copy()
+
+    def perturb(self, dhp, dsp, eig, eigcut=10.0):
+        d = np.zeros(self.nv); nt = C.c_int(0)
+        ref().ref_perturb(np.ascontiguousarray(dhp), np.ascontiguousarray(dsp),
+                          np.ascontiguousarray(eig), C.byref(C.c_double(eigcut)),
+                          C.byref(nt), d)
+        return nt.value, d
+
+    def bande(self, kpts, weights, mode=4, tkb=0.005, eigcut=10.0):
+        ef, e1, e2 = C.c_double(0), C.c_double(0), C.c_double(0)
+        ib = C.c_int(0)
+        kp = np.ascontiguousarray(np.asarray(kpts, float).ravel())
+        ref().ref_bande(C.byref(C.c_int(mode)), C.byref(C.c_int(len(kpts))), kp,
+                        np.ascontiguousarray(np.asarray(weights, float)),
+                        C.byref(C.c_double(self.st.volume)),
+                        C.byref(C.c_double(self.st.zfill)), C.byref(C.c_double(tkb)),
+                        C.byref(C.c_double(eigcut)), C.byref(ef), C.byref(e1),
+                        C.byref(e2), C.byref(ib))
+        return dict(err=ib.value, efermi=ef.value, earg1=e1.value, earg2=e2.value)

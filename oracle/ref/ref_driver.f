@@ -1,0 +1,350 @@
+c     ref_driver.f -- bind(C) entry points that fill the reference's
+c     COMMON blocks and call its own routines (search2, setpar, setparv,
+c     setham, sethamv, diag, perturb, fermi, bande) unchanged.
+c
+c     TEST INFRASTRUCTURE ONLY: linked into oracle/_ref/libnrlref.so
+c     together with the reference's sources compiled where they lie
+c     (NRLtb/ex1, whose P1/P2/P3 give matom=20, mbas=9, mpress=1).
+c     Used to write golden vectors and to pin the C restatement.
+c
+c     Layouts seen from C (all arrays by pointer):
+c       lat(3,3)   lat(c,v) = component c of lattice vector v
+c       pos(3,nat) Cartesian positions (bohr); pq(3,nat) phase origins
+c       packed H/S: complex*16 upper packed, nv*(nv+1)/2 entries
+c
+      subroutine ref_setup(nat,lat,pos,pq,kk,nd,nkinds,kb,val,
+     $     dnnin,scrin,par,npar,nlt,jsov,npout,dmin,ierr)
+     $     bind(C,name='ref_setup')
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      integer nat,nkinds,npar,nlt,jsov,npout,ierr
+      integer kk(nat),nd(3),kb(nkinds)
+      real*8 lat(3,3),pos(3,nat),pq(3,nat),val(nkinds)
+      real*8 dnnin(*),scrin(*),par(npar),dmin
+      logical master,multiprc
+      common /MPIcomm/ MPISIZE,MPIRANK,master,multiprc
+      common /codes/posn(matom,3),kkind(matom),natoms
+      common /latt1/plv(3,3)
+      common /parinfo/ valence(mkind),kinds,kbas(mkind)
+      common /cutcom/ dnnsq(mint),dnn(mint),
+     $     screenl(mint),scrninv(mint)
+      integer nkdarry(6)
+      common/disply/ nkdarry
+      common /kstuff/ ksk(matom)
+      common /types/ npkind(mptype)
+      common /fazon/posq(matom,3)
+      common /struc/ nv
+      logical jsover
+      common /lia/ npts,jsover
+      common /params/ param(npard,mspin)
+      logical realov,harrison
+      common /parcom/ nltype,realov,harrison
+      ierr = 0
+      if(nat.gt.matom.or.nkinds.gt.mkind.or.npar.gt.npard) then
+         ierr = 10
+         return
+      end if
+      MPISIZE = 1
+      MPIRANK = 0
+      master = .true.
+      multiprc = .false.
+      npkind(1) = 4
+      npkind(2) = 10
+      npkind(3) = 14
+      natoms = nat
+      kinds = nkinds
+      do i = 1,3
+         do j = 1,3
+            plv(j,i) = lat(i,j)
+         end do
+      end do
+      do i = 1,nkinds
+         kbas(i) = kb(i)
+         valence(i) = val(i)
+      end do
+      do i = 1,nkinds*(nkinds+1)/2
+         dnn(i) = dnnin(i)
+         screenl(i) = scrin(i)
+         dnnsq(i) = dnn(i)*dnn(i)
+         scrninv(i) = 1d0/screenl(i)
+      end do
+      nv = 0
+      do ia = 1,nat
+         kkind(ia) = kk(ia)
+         do j = 1,3
+            posn(ia,j) = pos(j,ia)
+            posq(ia,j) = pq(j,ia)
+         end do
+         ksk(ia) = nv
+         nv = nv + kbas(kkind(ia))
+      end do
+      do i = 1,3
+         nkdarry(i) = nd(i)
+         nkdarry(i+3) = 2*nd(i)+1
+      end do
+      do i = 1,npard
+         param(i,1) = 0d0
+      end do
+      do i = 1,npar
+         param(i,1) = par(i)
+      end do
+      nltype = nlt
+      realov = (nlt.eq.1).or.(nlt.eq.3)
+      harrison = (nlt.eq.2).or.(nlt.eq.3)
+      jsover = jsov.ne.0
+      call search2(npout,dmin,ierr)
+      return
+      end
+c
+      subroutine ref_get_pairs(tto,dlvo,disto,scro,dscro,jko,jao)
+     $     bind(C,name='ref_get_pairs')
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      real*8 tto(3,*),dlvo(3,*),disto(*),scro(*),dscro(*)
+      integer jko(2,*),jao(2,*)
+      common /pairs/tt_list(3,mpair),
+     $     dlv_list(3,mpair),dist_list(mpair),
+     $     screen_list(mpair),
+     $     jkind_list(mpair,2),
+     $     jatm_list(mpair,2),
+     $     npairs
+      common /neigh/ dscrn(mpair)
+      do ip = 1,npairs
+         do j = 1,3
+            tto(j,ip) = tt_list(j,ip)
+            dlvo(j,ip) = dlv_list(j,ip)
+         end do
+         disto(ip) = dist_list(ip)
+         scro(ip) = screen_list(ip)
+         dscro(ip) = dscrn(ip)
+         jko(1,ip) = jkind_list(ip,1)
+         jko(2,ip) = jkind_list(ip,2)
+         jao(1,ip) = jatm_list(ip,1)
+         jao(2,ip) = jatm_list(ip,2)
+      end do
+      return
+      end
+c
+c     Per-structure parameters; results kept here for the per-k calls.
+c
+      subroutine ref_setpar(thisvol,dpar,hpo,opo,dparo,dhpo,dopo,
+     $     deno,ierr) bind(C,name='ref_setpar')
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      integer ierr
+      real*8 dpar(4,*),hpo(14,*),opo(14,*),dparo(4,*),dhpo(14,*)
+      real*8 dopo(14,*),deno(mkind,*)
+      common /codes/posn(matom,3),kkind(matom),natoms
+      common /pairs/tt_list(3,mpair),
+     $     dlv_list(3,mpair),dist_list(mpair),
+     $     screen_list(mpair),
+     $     jkind_list(mpair,2),
+     $     jatm_list(mpair,2),
+     $     npairs
+      logical jsover
+      common /lia/ npts,jsover
+      common /params/ param(npard,mspin)
+      common /refstate/ paros(4,matom),hop(mpkind,mpair),
+     $     over(mpkind,mpair),dparos(4,matom),dhop(mpkind,mpair),
+     $     dover(mpkind,mpair),den(mkind,matom,mspin)
+      real*8 spinos(4,matom),dspnos(4,matom)
+      logical flipos(matom)
+      do i = 1,matom
+         flipos(i) = .false.
+      end do
+      do ip = 1,npairs
+         do j = 1,mpkind
+            hop(j,ip) = 0d0
+            over(j,ip) = 0d0
+            dhop(j,ip) = 0d0
+            dover(j,ip) = 0d0
+         end do
+      end do
+      call setpar(1,param,jsover,den,flipos,.false.,
+     $     paros,spinos,hop,over,.false.,ierr)
+      if(ierr.ne.0) return
+      call setparv(1,param,thisvol,jsover,den,flipos,.false.,
+     $     dparos,dspnos,dhop,dover,.false.,ierr)
+      do ia = 1,natoms
+         do k = 1,4
+            dpar(k,ia) = paros(k,ia)
+            dparo(k,ia) = dparos(k,ia)
+         end do
+         do k = 1,mkind
+            deno(k,ia) = den(k,ia,1)
+         end do
+      end do
+      do ip = 1,npairs
+         do j = 1,14
+            hpo(j,ip) = hop(j,ip)
+            opo(j,ip) = over(j,ip)
+            dhpo(j,ip) = dhop(j,ip)
+            dopo(j,ip) = dover(j,ip)
+         end do
+      end do
+      return
+      end
+c
+c     Packed H(k), S(k) (ideriv=0) or dH/dV, dS/dV (ideriv=1).
+c
+      subroutine ref_setham(ak,ideriv,hpk,spk)
+     $     bind(C,name='ref_setham')
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      real*8 ak(3)
+      integer ideriv
+      complex*16 hpk(*),spk(*)
+      common /struc/ nv
+      logical jsover
+      common /lia/ npts,jsover
+      common /refstate/ paros(4,matom),hop(mpkind,mpair),
+     $     over(mpkind,mpair),dparos(4,matom),dhop(mpkind,mpair),
+     $     dover(mpkind,mpair),den(mkind,matom,mspin)
+      complex*16 hmat(mh*(mh+1)/2),smat(mh*(mh+1)/2)
+      if(ideriv.eq.0) then
+         call setham(jsover,ak,paros,hop,over,hmat,smat,.false.)
+      else
+         call sethamv(jsover,ak,dparos,dhop,dover,hmat,smat,.false.)
+      end if
+      do i = 1,nv*(nv+1)/2
+         hpk(i) = hmat(i)
+         spk(i) = smat(i)
+      end do
+      return
+      end
+c
+c     diag with packed inputs (copied); eigenvectors out as z(nv,nv).
+c
+      subroutine ref_diag(mode,hpk,spk,eo,zo,ioverr)
+     $     bind(C,name='ref_diag')
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      include 'P2'
+      integer mode,ioverr
+      complex*16 hpk(*),spk(*),zo(*)
+      real*8 eo(*)
+      common /struc/ nv
+      logical jsover
+      common /lia/ npts,jsover
+      common /codes/posn(matom,3),kkind(matom),natoms
+      common /parinfo/ valence(mkind),kinds,kbas(mkind)
+      common /kstuff/ ksk(matom)
+      common /refzeig/ zeig(mhe,mhe)
+      complex*16 zeig
+      complex*16 hmat(mh*(mh+1)/2),smat(mh*(mh+1)/2)
+      real*8 en(mh),spev(3,msdir*(matom-1)+1,msdir*(mh-1)+1),ak(3)
+      do i = 1,nv*(nv+1)/2
+         hmat(i) = hpk(i)
+         smat(i) = spk(i)
+      end do
+      ak(1) = 0d0
+      ak(2) = 0d0
+      ak(3) = 0d0
+      call diag(mode,.false.,hmat,smat,en,.false.,spev,nv,zeig,
+     $     jsover,ak,1d3,1d0,ioverr,natoms,kkind,kbas,ksk)
+      do i = 1,nv
+         eo(i) = en(i)
+      end do
+      do j = 1,nv
+         do i = 1,nv
+            zo(i+(j-1)*nv) = zeig(i,j)
+         end do
+      end do
+      return
+      end
+c
+c     perturb on the eigenvectors of the last ref_diag call.
+c
+      subroutine ref_perturb(dhpk,dspk,eig,eigcut,ntot,deig)
+     $     bind(C,name='ref_perturb')
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      include 'P2'
+      complex*16 dhpk(*),dspk(*)
+      real*8 eig(*),deig(*),eigcut
+      integer ntot
+      common /struc/ nv
+      common /refzeig/ zeig(mhe,mhe)
+      complex*16 zeig
+      complex*16 hmat(mh*(mh+1)/2),smat(mh*(mh+1)/2),work(mhp)
+      real*8 en(mh),de(mh)
+      do i = 1,nv*(nv+1)/2
+         hmat(i) = dhpk(i)
+         smat(i) = dspk(i)
+      end do
+      do i = 1,nv
+         en(i) = eig(i)
+      end do
+      call perturb(mhp,nv,hmat,smat,zeig,en,eigcut,ntot,de,work)
+      do i = 1,nv
+         deig(i) = de(i)
+      end do
+      return
+      end
+c
+c     fermi on eigen(nv,nk) with weights w(nk).
+c
+      real*8 function ref_fermi(nk,el,w,eig,tkb,zfill)
+     $     bind(C,name='ref_fermi')
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      include 'P3'
+      integer nk
+      real*8 w(*),eig(*),el,tkb,zfill
+      logical master,multiprc
+      common /MPIcomm/ MPISIZE,MPIRANK,master,multiprc
+      common /struc/ nv
+      common /refeig/ esk(mh,nkdproc),wt(nkd)
+      real*8 fermi
+      external fermi
+      do k = 1,nk
+         wt(k) = w(k)
+         do i = 1,nv
+            esk(i,k) = eig(i+(k-1)*nv)
+         end do
+      end do
+      ref_fermi = fermi(mh,nkd,nkdproc,1,nk,nv,1,el,wt,esk,tkb,zfill)
+      return
+      end
+c
+c     The whole bande k-loop on an explicit k list (fractional
+c     reciprocal-lattice units) with weights.
+c
+      subroutine ref_bande(mode,nk,kp,w,thisvol,zfill,tkb,eigcut,
+     $     efermi,earg1,earg2,iberr) bind(C,name='ref_bande')
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      include 'P3'
+      integer mode,nk,iberr
+      real*8 kp(3,*),w(*),thisvol,zfill,tkb,eigcut
+      real*8 efermi,earg1,earg2
+      logical master,multiprc
+      common /MPIcomm/ MPISIZE,MPIRANK,master,multiprc
+      logical jsover
+      common /lia/ npts,jsover
+      common /cd1/ x(nkd),y(nkd),z(nkd)
+      common /refeig/ esk(mh,nkdproc),wt(nkd)
+      logical flipos(matom)
+      real*8 spinat(3,matom),dspth(3,matom),dspphi(3,matom)
+      character*20 label
+      iberr = 0
+      if(nk.gt.nkd) then
+         iberr = 10
+         return
+      end if
+      do i = 1,matom
+         flipos(i) = .false.
+      end do
+      npts = nk
+      do k = 1,nk
+         x(k) = kp(1,k)
+         y(k) = kp(2,k)
+         z(k) = kp(3,k)
+         wt(k) = w(k)
+      end do
+      label = 'ref_bande'
+      call bande(mode,1,flipos,.false.,spinat,.false.,dspth,dspphi,
+     $     .false.,thisvol,zfill,tkb,eigcut,wt,efermi,earg1,earg2,
+     $     smag,.false.,.false.,.false.,0,label,iberr)
+      return
+      end

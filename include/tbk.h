@@ -1,0 +1,196 @@
+/*
+ * tbk.h -- C ABI of the MI355X-native NRL tight-binding k-point engine.
+ *
+ * The engine replaces the per-k-point hot path of the NRL program `static`
+ * (boiled-greggs/Tight-Binding, NRLtb/src): the neighbour list and
+ * k-independent Slater-Koster parameters feeding it, the per-k H(k)/S(k)
+ * build, the generalized Hermitian eigensolve, the first-order pressure
+ * derivatives, the Fermi search and the band-energy/pressure sums.
+ *
+ * Conventions (all entry points):
+ *   - plain pointers and sizes; no allocation is visible to the caller;
+ *   - lengths in bohr, energies in Ry, pressures in Ry/bohr^3, k-points in
+ *     reciprocal-lattice units (the x,y,z of COMMON /cd1/, kptin.f:349-366);
+ *   - lat[3*v+c]  = component c of primitive vector v   (plv(v,c))
+ *     posn[3*a+c] = Cartesian coordinate c of atom a     (posn(a,c))
+ *     posq[3*a+c] = phase origin of atom a, lattice units (posq(a,c))
+ *     kkind[a]    = 1-based kind of atom a;
+ *   - complex matrices are complex128 interleaved (re, im), column-major;
+ *   - every function returns TBK_OK or an error code; none aborts.
+ */
+#ifndef TBK_H
+#define TBK_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes.  The reference reports the same conditions through iberr,
+ * ioverr, ierr or STOP (bande.f:987-995, diag.f:429-455, search2.f:180-188,
+ * perturb.f:106, fermi.f:165); here each is a return value. */
+#define TBK_OK            0
+#define TBK_ERR_ARG       1   /* invalid argument or size                 */
+#define TBK_ERR_PAIRS     2   /* pair list exceeds maxpairs (search2)      */
+#define TBK_ERR_OVERLAP   3   /* S(k) not positive definite at some k      */
+#define TBK_ERR_CONVERGE  4   /* tridiagonal eigensolver did not converge  */
+#define TBK_ERR_ZFILL     5   /* zfill <= 0 (fermi.f:165)                  */
+#define TBK_ERR_DEGEN     6   /* degenerate group larger than 9 (perturb.f:106) */
+#define TBK_ERR_HIP       10  /* HIP runtime error                         */
+#define TBK_ERR_NOMEM     11  /* device allocation failed                  */
+#define TBK_ERR_NODEVICE  12  /* no usable GPU                             */
+
+/* Precision of a plan (BASELINE north star "plain" / "mixed"). */
+#define TBK_FP64          0   /* all-FP64 plain method                     */
+#define TBK_MIXED         1   /* FP32 tridiagonalisation + back-transform,
+                                 FP64 Cholesky, Fermi sums and refinement  */
+
+/* bande modes supported (bande.f:99-123). */
+#define TBK_MODE_E0_ET    1   /* earg1 = E(0) (Gillan), earg2 = E(T)       */
+#define TBK_MODE_E0_P0    2   /* earg1 = E(0), earg2 = P(0)                */
+#define TBK_MODE_ET       3   /* earg1 = E(T)                              */
+#define TBK_MODE_ET_PT    4   /* earg1 = E(T), earg2 = P(T)                */
+
+/* ---------------------------------------------------------------------
+ * Host set-up (k-independent).  Bit-exact restatements.
+ * --------------------------------------------------------------------- */
+
+/* Neighbour list.  Replaces search2(npfound,dismin,ierr), search2.f:1
+ * (pair order iat, jat >= iat, ix, iy, iz; cutoff test search2.f:168-175).
+ * nd[3]: search half-widths (input1.f:488).  dnn, screenl: per interaction
+ * type.  Outputs sized maxpairs: tt[3p+c] direction cosines, dlv[3p+c]
+ * lattice translation, dist[p], screen[p], dscrn[p], jkind[2p+i],
+ * jatm[2p+i] (1-based).  Returns TBK_ERR_PAIRS on overflow. */
+int tbk_search2(int natoms, const double *lat, const double *posn,
+                const int *kkind, int kinds, const int *nd,
+                const double *dnn, const double *screenl, int maxpairs,
+                int *npairs, double *tt, double *dlv, double *dist,
+                double *screen, double *dscrn, int *jkind, int *jatm,
+                double *dismin);
+
+/* Density-dependent on-site terms and bond/overlap integrals.  Replaces
+ * setpar(jspin,param,jsover,den,flipos,lnoncol,paros,spinos,hop,over,
+ * lprint,ierr), setpar.f:1 (collinear, jspin = 1).  param: one spin channel
+ * (input.f:327-359 order).  Outputs den[a*kinds+k], paros[4a+j],
+ * hop[14p+j], over[14p+j]. */
+int tbk_setpar(int natoms, const int *kkind, int kinds, const double *param,
+               int nltype, int jsover, int npairs, const double *dist,
+               const double *screen, const int *jkind, const int *jatm,
+               double *den, double *paros, double *hop, double *over);
+
+/* Volume derivatives.  Replaces setparv(...), setparv.f:1. */
+int tbk_setparv(int natoms, const int *kkind, int kinds, const double *param,
+                int nltype, int jsover, double thisvol, int npairs,
+                const double *dist, const double *screen, const double *dscrn,
+                const int *jkind, const int *jatm, const double *den,
+                double *dparos, double *dhop, double *dover);
+
+/* ---------------------------------------------------------------------
+ * Device plans: one structure, its k-mesh, device-resident H/S/Z state.
+ * --------------------------------------------------------------------- */
+typedef struct tbk_plan tbk_plan;
+
+/* Structure description passed to tbk_plan_create. */
+typedef struct tbk_structure_desc {
+    int natoms;
+    const double *lat;        /* [9]            */
+    const double *posn;       /* [3*natoms]     */
+    const double *posq;       /* [3*natoms]     */
+    const int *kkind;         /* [natoms]       */
+    int nd[3];
+    int kinds;
+    const int *kbas;          /* [kinds]        */
+    const double *valence;    /* [kinds]        */
+    const double *dnn;        /* [kinds*(kinds+1)/2] */
+    const double *screenl;    /* [kinds*(kinds+1)/2] */
+    const double *param;      /* [nparam]       */
+    int nparam;
+    int nltype;
+    int jsover;
+    double thisvol;           /* cell volume (setlat.f:997)            */
+    double zfill;             /* electrons (input1.f:733, 834)         */
+} tbk_structure_desc;
+
+/* Build the pair list and SK tables on the host, upload them, and size the
+ * device state for nk k-points solved `batch` at a time (batch <= 0: all).
+ * precision: TBK_FP64 or TBK_MIXED.  device: HIP device ordinal. */
+int tbk_plan_create(int device, const tbk_structure_desc *st, int nk,
+                    const double *kpts, const double *weights, int batch,
+                    int precision, tbk_plan **out);
+
+/* Run the whole bande k-loop + Fermi + sums on the device.  Replaces
+ * bande(mode,...,efermi,earg1,earg2,...,iberr), bande.f:1, for the
+ * collinear non-magnetic case.  Blocks until results are on the host. */
+int tbk_plan_run(tbk_plan *plan, int mode, double tkb, double eigcut,
+                 double *efermi, double *earg1, double *earg2);
+
+/* Same, enqueued on the plan's stream without the final synchronisation;
+ * tbk_plan_wait collects the results. */
+int tbk_plan_launch(tbk_plan *plan, int mode, double tkb, double eigcut);
+int tbk_plan_wait(tbk_plan *plan, double *efermi, double *earg1,
+                  double *earg2);
+
+/* Results of the last run: eigenvalues esk[k*nv+i] (ascending per k) and
+ * first-order derivatives desk[k*nv+i] (pressure modes), host copies. */
+int tbk_plan_eigenvalues(tbk_plan *plan, double *esk, double *desk);
+
+/* Eigenvectors of k-point k of the last run when the plan holds it
+ * (batch covering k), complex128 column-major nv x nv, Z^H S Z = I. */
+int tbk_plan_eigenvectors(tbk_plan *plan, int k, double *z);
+
+/* H(k), S(k) of k-point k as built on the device (full Hermitian,
+ * column-major complex128), for parity checks of the build.  deriv = 1
+ * returns dH/dV, dS/dV (sethamv.f). */
+int tbk_plan_hs(tbk_plan *plan, int k, int deriv, double *h, double *s);
+
+int tbk_plan_nv(const tbk_plan *plan);
+int tbk_plan_npairs(const tbk_plan *plan);
+int tbk_plan_destroy(tbk_plan *plan);
+
+/* Kernel-level timing of the last run (ms per stage, HIP events on the
+ * plan's stream): names[i] and ms[i] for i < *n (at most cap entries). */
+int tbk_plan_stage_times(const tbk_plan *plan, int cap, const char **names,
+                         double *ms, int *n);
+
+/* Enable (on = 1) or disable per-stage HIP event timing of later runs. */
+int tbk_plan_set_timing(tbk_plan *plan, int on);
+
+/* ---------------------------------------------------------------------
+ * Single-call drop-ins on host arrays.
+ * --------------------------------------------------------------------- */
+
+/* Generalized Hermitian eigensolve of one pair of full column-major
+ * complex matrices on the device.  Replaces zhpgv(1,'V','U',...) as called
+ * from diag.f:396-412 (diag_ for the collinear case).  z may be NULL.
+ * Returns TBK_ERR_OVERLAP when S is not positive definite (diag.f:429). */
+int tbk_zhegv(int device, int n, const double *h, const double *s,
+              double *w, double *z);
+
+/* Fermi level.  Replaces fermi(ntbd,nkptd,nkloc,istart,ifin,nv,jspins,el,
+ * weight,eigen,tkb,zfill), fermi.f:1 (one process, jspins = 1).
+ * eigen[k*ldn+i].  Evaluated on the device. */
+int tbk_fermi(int device, int ldn, int nk, int nv, double el,
+              const double *weight, const double *eigen, double tkb,
+              double zfill, double *efermi);
+
+const char *tbk_error_string(int code);
+
+/* ---------------------------------------------------------------------
+ * Diagnostics (no counterpart in the reference): stage-by-stage outputs of
+ * the device eigensolver for one matrix pair, and one device GEMM.  Any
+ * output pointer may be NULL.  uinv, c, refl, z: n x n complex128; vt:
+ * n x n real (row-major, vt[i*n+j] = component i of tridiagonal
+ * eigenvector j); d, e, w: n reals.
+ * --------------------------------------------------------------------- */
+int tbk_debug_stages(int device, int n, const double *h, const double *s,
+                     double *uinv, double *c, double *refl, double *d,
+                     double *e, double *w, double *vt, double *z);
+int tbk_debug_zgemm(int device, int m, int n, int k, int opa, int opb,
+                    int tri, const double *a, int lda, const double *b,
+                    int ldb, double *c, int ldc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

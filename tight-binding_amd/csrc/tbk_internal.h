@@ -1,0 +1,40 @@
+// tbk_internal.h -- host-side types shared by the set-up, the device
+// driver and the C ABI.
+#pragma once
+
+#include <vector>
+
+namespace tbk {
+
+// One entry of the neighbour list (COMMON /pairs/, /neigh/, search2.f:72-81).
+struct Pair {
+    double cosines[3];   // tt_list: direction cosines of R_j - R_i
+    double dlv[3];       // dlv_list: lattice translation of atom j
+    double dist;         // dist_list
+    double screen;       // screen_list: smooth cutoff F(R)
+    double dscrn;        // dF/dR = dscrn * F
+    int kind[2];         // jkind_list (1-based)
+    int atom[2];         // jatm_list (0-based here)
+};
+
+struct PairList {
+    std::vector<Pair> pairs;
+    int n = 0;
+    double dismin = 0.0;
+    void clear() { pairs.clear(); n = 0; dismin = 0.0; }
+    void push(const Pair &p) { pairs.push_back(p); n = (int)pairs.size(); }
+};
+
+int search2(int natoms, const double *lat, const double *posn, const int *kkind,
+            int kinds, const int *nd, const double *dnn, const double *screenl,
+            int maxpairs, PairList &out);
+int setpar(int natoms, const int *kkind, int kinds, const double *param, int nltype,
+           int jsover, const PairList &pl, double *den, double *paros, double *hop,
+           double *over);
+int setparv(int natoms, const int *kkind, int kinds, const double *param, int nltype,
+            int jsover, double thisvol, const PairList &pl, const double *den,
+            double *dparos, double *dhop, double *dover);
+void sk_block(double l, double m, double n, const double *p, bool unlike, double *e);
+void arrange_integrals(const double *stored, int k1, int k2, double *out);
+
+}  // namespace tbk

@@ -1,0 +1,76 @@
+// tbk_kernels.h -- argument blocks and launchers of the device kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace tbk {
+
+struct BuildArgs {
+    int n, natoms, onsite_s;
+    size_t stride;                 // elements between k-point slabs
+    const int *row_atom, *row_orb, *ksk, *nbas, *blk_ptr;
+    const double *dvec;            // [3*npairs] dlv - posq_j + posq_i
+    const double *eh, *es;         // [81*npairs] SK blocks e(a,b), row-major
+    const double *onsite;          // [4*natoms] paros or dparos
+    const double *kpts;            // [3*nb]
+    double2 *h, *s;
+};
+
+struct GemmArgs {
+    int m, n, k;
+    double2 alpha, beta;
+    const double2 *a; int lda; size_t sa;
+    const double2 *b; int ldb; size_t sb;
+    double2 *c; int ldc; size_t sc;
+    int tri;
+};
+
+struct HetrdArgs {
+    int n;
+    size_t stride;
+    double2 *a, *vp, *wp, *t;      // vp, wp, t: [nb][n*32]
+    double *d, *e;                 // [nb][n]
+};
+
+struct SteinArgs {
+    int n;
+    double ortol_rel;              // group threshold relative to ||T||_1
+    const double *d, *e;
+    double *w, *v, *scratch;       // w [nb][n]; v [nb][n*n]; scratch [nb][5*n*256]
+    int *info;
+};
+
+struct PerturbArgs {
+    int n, nb, k0;
+    size_t stride;
+    double eigcut;
+    const double *w;
+    const double2 *px, *py, *z, *x, *y;
+    double *desk;
+    int *ncut, *err;
+};
+
+struct FermiArgs {
+    int nk, nv, ld, mode;
+    double el, tkb, zfill;
+    const double *esk, *desk, *w;
+    const int *ncut;
+    double *out;                   // efermi, earg1, earg2, esum, entropy, iters
+};
+
+void launch_build_hs(const BuildArgs &a, int natoms, int nb, hipStream_t st);
+void launch_potrf_trtri(int n, double2 *s, size_t stride, double2 *work, int *info, int nb,
+                        hipStream_t st);
+void launch_zgemm(const GemmArgs &g, int opa, int opb, int nb, hipStream_t st);
+void launch_hetrd(const HetrdArgs &h, int nb, hipStream_t st);
+void launch_stein(const SteinArgs &s, int nb, hipStream_t st);
+void launch_real_to_complex_t(int n, const double *vt, double2 *y, size_t stride, int nb,
+                              hipStream_t st);
+void launch_coldot(int n, const double2 *z, const double2 *x, const double2 *y, size_t stride,
+                   double2 *px, double2 *py, int nb, hipStream_t st);
+void launch_perturb(const PerturbArgs &p, hipStream_t st);
+void launch_fermi_sums(const FermiArgs &f, hipStream_t st);
+
+}  // namespace tbk

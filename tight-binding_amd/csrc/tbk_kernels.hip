@@ -1,0 +1,1231 @@
+// tbk_kernels.hip -- CDNA4 (gfx950) kernels of the k-point engine.
+//
+// Every matrix is complex128 (double2: x = re, y = im), column-major with
+// leading dimension N, one N*N slab per k-point of the batch.
+//
+// Kernels, in pipeline order:
+//   k_build_hs      Bloch sums of precomputed Slater-Koster blocks -> H, S
+//                   (setham.f + rotate.f; sethamv.f for dH/dV, dS/dV)
+//   k_potrf_trtri   blocked Cholesky S = U^H U and U^{-1}, one WG per k
+//                   (zpptrf, lapack.f:4466; the inverse feeds the GEMMs)
+//   k_zgemm         batched complex GEMM on the FP64 MFMA units
+//   k_hetrd         blocked Householder tridiagonalisation (zhptrd,
+//                   lapack.f:3218), WY factors for the back-transform
+//   k_stein         bisection eigenvalues + inverse-iteration eigenvectors
+//                   of the real tridiagonal (replaces dsterf / zsteqr)
+//   k_coldot, k_perturb   first-order derivatives (perturb.f:87-221)
+//   k_fermi_sums    Fermi level + band-energy / pressure sums
+//                   (fermi.f:163-433, bande.f:1170-1470)
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+
+#include "tbk_kernels.h"
+
+namespace tbk {
+
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 zsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 zmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// conj(a) * b
+__device__ __forceinline__ double2 zcmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 zscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+__device__ __forceinline__ double2 zzero() { return make_double2(0.0, 0.0); }
+
+// Block-wide sum of one double (blockDim.x multiple of 64, <= 1024).
+__device__ double block_sum(double v, double *red)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    const int nw = blockDim.x >> 6;
+    double s = 0.0;
+    for (int i = 0; i < nw; ++i) s += red[i];   // fixed order: deterministic
+    __syncthreads();
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// H(k), S(k) build
+// ---------------------------------------------------------------------------
+// One workgroup per (column atom ja, k-point).  Thread r owns matrix row r
+// and the kb(ja) columns of atom ja.  For rows of atoms ia <= ja the block
+// is the Bloch sum over the pairs (ia, ja) of the reference's pair list, in
+// list order, of e(a,b) * exp(2 pi i k.d) (rotate.f:144-149, 260-326); rows
+// of atoms ia > ja take the conjugate of block (ja, ia) (Hermitian).  The
+// accumulation order and the absence of FMA contraction reproduce the
+// reference's rounding except for the ulp of sin/cos.
+__global__ __launch_bounds__(256) void k_build_hs(BuildArgs a)
+{
+#pragma clang fp contract(off)
+    const int ja = blockIdx.x;
+    const int kb = blockIdx.y;
+    const int N = a.n;
+    const double *ak = a.kpts + 3 * (size_t)kb;
+    const double k0 = ak[0], k1 = ak[1], k2 = ak[2];
+    const double tpi = 6.283185307179586;
+    const int c0 = a.ksk[ja], nc = a.nbas[ja];
+    double2 *H = a.h + (size_t)kb * a.stride;
+    double2 *S = a.s + (size_t)kb * a.stride;
+    for (int r = threadIdx.x; r < N; r += blockDim.x) {
+        const int ia = a.row_atom[r], o = a.row_orb[r];
+        double hu[9][2], su[9][2], hl[9][2], sl[9][2];
+        for (int b = 0; b < 9; ++b) {
+            hu[b][0] = hu[b][1] = su[b][0] = su[b][1] = 0.0;
+            hl[b][0] = hl[b][1] = sl[b][0] = sl[b][1] = 0.0;
+        }
+        if (ia <= ja) {            // block (ia, ja): row o of e
+            const int p0 = a.blk_ptr[ia * a.natoms + ja], p1 = a.blk_ptr[ia * a.natoms + ja + 1];
+            for (int p = p0; p < p1; ++p) {
+                const double *d = a.dvec + 3 * (size_t)p;
+                const double dot = k0 * d[0] + k1 * d[1] + k2 * d[2];
+                double sn, cs;
+                sincos(tpi * dot, &sn, &cs);
+                const double *eh = a.eh + 81 * (size_t)p + 9 * o;
+                const double *es = a.es + 81 * (size_t)p + 9 * o;
+                for (int b = 0; b < nc; ++b) {
+                    const double e1 = eh[b], e2 = es[b];
+                    hu[b][0] = hu[b][0] + e1 * cs; hu[b][1] = hu[b][1] + e1 * sn;
+                    su[b][0] = su[b][0] + e2 * cs; su[b][1] = su[b][1] + e2 * sn;
+                }
+            }
+        }
+        if (ia >= ja) {            // block (ja, ia): column o of e, conjugated
+            const int p0 = a.blk_ptr[ja * a.natoms + ia], p1 = a.blk_ptr[ja * a.natoms + ia + 1];
+            for (int p = p0; p < p1; ++p) {
+                const double *d = a.dvec + 3 * (size_t)p;
+                const double dot = k0 * d[0] + k1 * d[1] + k2 * d[2];
+                double sn, cs;
+                sincos(tpi * dot, &sn, &cs);
+                const double *eh = a.eh + 81 * (size_t)p + o;
+                const double *es = a.es + 81 * (size_t)p + o;
+                for (int b = 0; b < nc; ++b) {
+                    const double e1 = eh[9 * b], e2 = es[9 * b];
+                    hl[b][0] = hl[b][0] + e1 * cs; hl[b][1] = hl[b][1] + e1 * sn;
+                    sl[b][0] = sl[b][0] + e2 * cs; sl[b][1] = sl[b][1] + e2 * sn;
+                }
+            }
+        }
+        for (int b = 0; b < nc; ++b) {
+            double2 hv, sv;
+            if (ia < ja || (ia == ja && b > o)) {
+                hv = make_double2(hu[b][0], hu[b][1]);
+                sv = make_double2(su[b][0], su[b][1]);
+            } else if (ia > ja || b < o) {
+                hv = make_double2(hl[b][0], -hl[b][1]);
+                sv = make_double2(sl[b][0], -sl[b][1]);
+            } else {                 // diagonal: real, plus on-site terms
+                const int cls = o == 0 ? 0 : (o <= 3 ? 1 : (o <= 6 ? 2 : 3));
+                hv = make_double2(hu[b][0] + a.onsite[4 * ia + cls], 0.0);
+                sv = make_double2(su[b][0] + (a.onsite_s ? 1.0 : 0.0), 0.0);
+            }
+            H[(size_t)(c0 + b) * N + r] = hv;
+            S[(size_t)(c0 + b) * N + r] = sv;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cholesky S = U^H U and the triangular inverse, one workgroup per k
+// ---------------------------------------------------------------------------
+constexpr int NB = 32;
+
+// In-place upper Cholesky of a w x w block held in LDS (row stride 33).
+__device__ bool lds_potrf(double2 (*D)[NB + 1], int w)
+{
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int j = 0; j < w; ++j) {
+        if (threadIdx.x == 0) {
+            const double d = D[j][j].x;
+            if (!(d > 0.0)) { bad = 1; D[j][j] = make_double2(1.0, 0.0); }
+            else D[j][j] = make_double2(sqrt(d), 0.0);
+        }
+        __syncthreads();
+        const double inv = 1.0 / D[j][j].x;
+        for (int c = j + 1 + threadIdx.x; c < w; c += blockDim.x) D[j][c] = zscale(inv, D[j][c]);
+        __syncthreads();
+        const int m = w - j - 1;
+        for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
+            const int r = j + 1 + t % m, c = j + 1 + t / m;
+            if (r <= c) D[r][c] = zsub(D[r][c], zcmul(D[j][r], D[j][c]));
+        }
+        __syncthreads();
+    }
+    const int b = bad;
+    __syncthreads();
+    return b == 0;
+}
+
+// In-place inverse of a w x w upper triangular block in LDS (ztrti2).
+__device__ void lds_trtri(double2 (*D)[NB + 1], int w)
+{
+    for (int j = 0; j < w; ++j) {
+        // column j above the diagonal: x = -inv(D[j][j]) * Dinv[0:j,0:j] * D[0:j, j]
+        __shared__ double2 col[NB];
+        if (threadIdx.x < j) col[threadIdx.x] = D[threadIdx.x][j];
+        __syncthreads();
+        const double djj = 1.0 / D[j][j].x;
+        if (threadIdx.x < j) {
+            const int i = threadIdx.x;
+            double2 acc = zzero();
+            for (int l = i; l < j; ++l) acc = zadd(acc, zmul(D[i][l], col[l]));
+            D[i][j] = zscale(-djj, acc);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) D[j][j] = make_double2(djj, 0.0);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_potrf_trtri(int N, double2 *S, size_t stride,
+                                                     double2 *work, int *info)
+{
+    __shared__ double2 D[NB][NB + 1];
+    __shared__ double2 Ta[NB][64 + 1];   // staging of U12 column blocks
+    __shared__ double2 Tb[NB][64 + 1];
+    const int kb = blockIdx.x;
+    double2 *A = S + (size_t)kb * stride;
+    double2 *Wk = work + (size_t)kb * stride;
+    const int tid = threadIdx.x;
+    // ---------------- Cholesky, right-looking, panel NB ----------------
+    for (int j0 = 0; j0 < N; j0 += NB) {
+        const int w = min(NB, N - j0);
+        for (int t = tid; t < w * w; t += blockDim.x) {
+            const int r = t % w, c = t / w;
+            D[r][c] = A[(size_t)(j0 + c) * N + j0 + r];
+        }
+        __syncthreads();
+        if (!lds_potrf(D, w)) {
+            if (tid == 0) info[kb] = j0 + 1;
+            return;
+        }
+        for (int t = tid; t < w * w; t += blockDim.x) {
+            const int r = t % w, c = t / w;
+            if (r <= c) A[(size_t)(j0 + c) * N + j0 + r] = D[r][c];
+        }
+        // U12 = U11^{-H} S12, one thread per column
+        for (int c = j0 + w + tid; c < N; c += blockDim.x) {
+            double2 x[NB];
+            for (int i = 0; i < w; ++i) x[i] = A[(size_t)c * N + j0 + i];
+            for (int i = 0; i < w; ++i) {
+                double2 v = x[i];
+                for (int l = 0; l < i; ++l) v = zsub(v, zcmul(D[l][i], x[l]));
+                x[i] = zscale(1.0 / D[i][i].x, v);
+            }
+            for (int i = 0; i < w; ++i) A[(size_t)c * N + j0 + i] = x[i];
+        }
+        __syncthreads();
+        // trailing update, upper triangle: A22 -= U12^H U12, 64x64 tiles
+        const int s0 = j0 + w;
+        const int m = N - s0;
+        const int nt = (m + 63) / 64;
+        for (int tc = 0; tc < nt; ++tc) {
+            const int cb = s0 + 64 * tc;
+            for (int t = tid; t < w * 64; t += blockDim.x) {
+                const int l = t % w, c = t / w;
+                Tb[l][c] = (cb + c < N) ? A[(size_t)(cb + c) * N + j0 + l] : zzero();
+            }
+            for (int tr = 0; tr <= tc; ++tr) {
+                const int rb = s0 + 64 * tr;
+                __syncthreads();
+                for (int t = tid; t < w * 64; t += blockDim.x) {
+                    const int l = t % w, r = t / w;
+                    Ta[l][r] = (rb + r < N) ? A[(size_t)(rb + r) * N + j0 + l] : zzero();
+                }
+                __syncthreads();
+                // thread: rows 4*tx..+3, cols ty + 16*jj
+                const int tx = tid & 15, ty = tid >> 4;
+                double2 acc[4][4];
+                for (int i = 0; i < 4; ++i) for (int jj = 0; jj < 4; ++jj) acc[i][jj] = zzero();
+                for (int l = 0; l < w; ++l) {
+                    double2 av[4], bv[4];
+                    for (int i = 0; i < 4; ++i) av[i] = Ta[l][4 * tx + i];
+                    for (int jj = 0; jj < 4; ++jj) bv[jj] = Tb[l][ty + 16 * jj];
+                    for (int i = 0; i < 4; ++i)
+                        for (int jj = 0; jj < 4; ++jj) acc[i][jj] = zadd(acc[i][jj], zcmul(av[i], bv[jj]));
+                }
+                for (int i = 0; i < 4; ++i)
+                    for (int jj = 0; jj < 4; ++jj) {
+                        const int r = rb + 4 * tx + i, c = cb + ty + 16 * jj;
+                        if (r < N && c < N && r <= c) {
+                            double2 *p = A + (size_t)c * N + r;
+                            *p = zsub(*p, acc[i][jj]);
+                        }
+                    }
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+    }
+    // ---------------- triangular inverse, blocked (ztrtri upper) --------
+    for (int j0 = 0; j0 < N; j0 += NB) {
+        const int w = min(NB, N - j0);
+        for (int t = tid; t < w * w; t += blockDim.x) {
+            const int r = t % w, c = t / w;
+            D[r][c] = (r <= c) ? A[(size_t)(j0 + c) * N + j0 + r] : zzero();
+        }
+        __syncthreads();
+        if (j0 > 0) {
+            // Wk[0:j0, 0:w] = Uinv[0:j0,0:j0] * U[0:j0, J]  (trmm, upper)
+            for (int i0 = 0; i0 < j0; i0 += blockDim.x) {
+                const int i = i0 + tid;
+                double2 acc[NB];
+                for (int c = 0; c < w; ++c) acc[c] = zzero();
+                for (int l0 = i0; l0 < j0; l0 += 64) {
+                    const int lw = min(64, j0 - l0);
+                    __syncthreads();
+                    for (int t = tid; t < lw * w; t += blockDim.x) {
+                        const int l = t % lw, c = t / lw;
+                        Ta[c][l] = A[(size_t)(j0 + c) * N + l0 + l];
+                    }
+                    __syncthreads();
+                    if (i < j0) {
+                        for (int l = 0; l < lw; ++l) {
+                            if (l0 + l < i) continue;
+                            const double2 u = A[(size_t)(l0 + l) * N + i];
+                            for (int c = 0; c < w; ++c) acc[c] = zadd(acc[c], zmul(u, Ta[c][l]));
+                        }
+                    }
+                }
+                if (i < j0)
+                    for (int c = 0; c < w; ++c) Wk[(size_t)c * N + i] = acc[c];
+            }
+            __syncthreads();
+            // A[0:j0, J] = -Wk * U[J,J]^{-1}, row by row
+            for (int i = tid; i < j0; i += blockDim.x) {
+                double2 x[NB];
+                for (int c = 0; c < w; ++c) {
+                    double2 v = zscale(-1.0, Wk[(size_t)c * N + i]);
+                    for (int l = 0; l < c; ++l) v = zsub(v, zmul(x[l], D[l][c]));
+                    x[c] = zscale(1.0 / D[c][c].x, v);
+                }
+                for (int c = 0; c < w; ++c) A[(size_t)(j0 + c) * N + i] = x[c];
+            }
+            __syncthreads();
+        }
+        lds_trtri(D, w);
+        for (int t = tid; t < w * w; t += blockDim.x) {
+            const int r = t % w, c = t / w;
+            A[(size_t)(j0 + c) * N + j0 + r] = (r <= c) ? D[r][c] : zzero();
+        }
+        __syncthreads();
+    }
+    // zero the strict lower triangle (the GEMMs read whole tiles)
+    for (size_t t = tid; t < (size_t)N * N; t += blockDim.x) {
+        const int r = (int)(t % N), c = (int)(t / N);
+        if (r > c) A[t] = zzero();
+    }
+    if (tid == 0) info[kb] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batched complex GEMM on the FP64 matrix cores
+// ---------------------------------------------------------------------------
+// C = alpha * op(A) * op(B) + beta * C.  64x64 tile per workgroup of 256
+// threads (4 waves, 32x32 each = 2x2 v_mfma_f64_16x16x4 tiles), K staged
+// through LDS 16 at a time.  A complex product is four real MFMAs
+// (re*re - im*im, re*im + im*re).  tri: 1 = op(A) upper triangular (skip
+// k < row tile), 2 = op(A) lower (skip k beyond the row tile), 3 = op(B)
+// upper (skip k beyond the column tile).
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <int OPA, int OPB>
+__global__ __launch_bounds__(256) void k_zgemm(GemmArgs g)
+{
+    __shared__ double As[2][16][64 + 1];  // [re/im][k][m]
+    __shared__ double Bs[2][16][64 + 1];  // [re/im][k][n]
+    const int bm = blockIdx.x * 64, bn = blockIdx.y * 64, kb = blockIdx.z;
+    const double2 *A = g.a + (size_t)kb * g.sa;
+    const double2 *B = g.b + (size_t)kb * g.sb;
+    double2 *C = g.c + (size_t)kb * g.sc;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;
+    int kbeg = 0, kend = g.k;
+    if (g.tri == 1) kbeg = bm;
+    if (g.tri == 2) kend = min(g.k, bm + 64);
+    if (g.tri == 3) kend = min(g.k, bn + 64);
+    kbeg = (kbeg / 16) * 16;
+    v4d accr[2][2], acci[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) { accr[i][j] = v4d{0, 0, 0, 0}; acci[i][j] = v4d{0, 0, 0, 0}; }
+    for (int k0 = kbeg; k0 < kend; k0 += 16) {
+        __syncthreads();
+        for (int t = tid; t < 1024; t += 256) {
+            int m, kk;
+            if (OPA == 0) { m = t & 63; kk = t >> 6; } else { kk = t & 15; m = t >> 4; }
+            const int gm = bm + m, gk = k0 + kk;
+            double2 v = zzero();
+            if (gm < g.m && gk < kend) {
+                if (OPA == 0) v = A[(size_t)gk * g.lda + gm];
+                else { v = A[(size_t)gm * g.lda + gk]; if (OPA == 1) v.y = -v.y; }
+            }
+            As[0][kk][m] = v.x; As[1][kk][m] = v.y;
+        }
+        for (int t = tid; t < 1024; t += 256) {
+            int n, kk;
+            if (OPB == 0) { kk = t & 15; n = t >> 4; } else { n = t & 63; kk = t >> 6; }
+            const int gn = bn + n, gk = k0 + kk;
+            double2 v = zzero();
+            if (gn < g.n && gk < kend) {
+                if (OPB == 0) v = B[(size_t)gn * g.ldb + gk];
+                else { v = B[(size_t)gk * g.ldb + gn]; if (OPB == 1) v.y = -v.y; }
+            }
+            Bs[0][kk][n] = v.x; Bs[1][kk][n] = v.y;
+        }
+        __syncthreads();
+        for (int ks = 0; ks < 16; ks += 4) {
+            const int kk = ks + (lane >> 4);
+            double ar[2], ai[2], br[2], bi[2];
+            for (int i = 0; i < 2; ++i) {
+                ar[i] = As[0][kk][wm + 16 * i + (lane & 15)];
+                ai[i] = As[1][kk][wm + 16 * i + (lane & 15)];
+            }
+            for (int j = 0; j < 2; ++j) {
+                br[j] = Bs[0][kk][wn + 16 * j + (lane & 15)];
+                bi[j] = Bs[1][kk][wn + 16 * j + (lane & 15)];
+            }
+            for (int i = 0; i < 2; ++i)
+                for (int j = 0; j < 2; ++j) {
+                    accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accr[i][j], 0, 0, 0);
+                    accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], accr[i][j], 0, 0, 0);
+                    acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], acci[i][j], 0, 0, 0);
+                    acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], acci[i][j], 0, 0, 0);
+                }
+        }
+    }
+    // D layout of v_mfma_f64_16x16x4_f64: item q of lane l holds row
+    // (l / 16) + 4 q, column l % 16 of the tile.
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int q = 0; q < 4; ++q) {
+                const int r = bm + wm + 16 * i + (lane >> 4) + 4 * q;
+                const int c = bn + wn + 16 * j + (lane & 15);
+                if (r < g.m && c < g.n) {
+                    const double2 v = make_double2(accr[i][j][q], acci[i][j][q]);
+                    double2 out = zmul(g.alpha, v);
+                    double2 *p = C + (size_t)c * g.ldc + r;
+                    if (g.beta.x != 0.0 || g.beta.y != 0.0) out = zadd(out, zmul(g.beta, *p));
+                    *p = out;
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Householder tridiagonalisation (lower, blocked), one workgroup per k
+// ---------------------------------------------------------------------------
+// On exit: d[N], e[N-1] (real tridiagonal), A holds the reflectors
+// v_j (A[j+1, j] = 1, A[r, j] = v_r below, zero at and above the diagonal),
+// T holds the upper triangular WY factor of every panel (zlarft forward
+// columnwise): the panel's Q_p = I - V_p T_p V_p^H.
+__global__ __launch_bounds__(256) void k_hetrd(HetrdArgs h)
+{
+    __shared__ double red[16];
+    __shared__ double2 u1[NB], u2[NB];
+    __shared__ double2 tauS;
+    __shared__ double betaS;
+    __shared__ double2 yacc[4][64];
+    __shared__ double2 vloc[1280];       // v of the current column (N <= 1280)
+    __shared__ double2 cw[NB][16], cv[NB][16];
+    const int N = h.n;
+    const int kb = blockIdx.x;
+    double2 *A = h.a + (size_t)kb * h.stride;
+    double2 *Vp = h.vp + (size_t)kb * N * NB;
+    double2 *Wp = h.wp + (size_t)kb * N * NB;
+    double2 *Tf = h.t + (size_t)kb * N * NB;
+    double *d = h.d + (size_t)kb * N;
+    double *e = h.e + (size_t)kb * N;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int i0 = 0; i0 < N - 1; i0 += NB) {
+        const int pw = min(NB, N - 1 - i0);
+        for (int t = tid; t < N * NB; t += blockDim.x) { Vp[t] = zzero(); Wp[t] = zzero(); }
+        __syncthreads();
+        for (int i = 0; i < pw; ++i) {
+            const int j = i0 + i;
+            // (1) bring column j up to date with the panel's earlier reflectors
+            if (i > 0) {
+                for (int l = tid; l < i; l += blockDim.x) {
+                    u1[l] = zconj(Wp[(size_t)l * N + j]);
+                    u2[l] = zconj(Vp[(size_t)l * N + j]);
+                }
+                __syncthreads();
+                for (int r = j + tid; r < N; r += blockDim.x) {
+                    double2 a = A[(size_t)j * N + r];
+                    for (int l = 0; l < i; ++l) {
+                        a = zsub(a, zmul(Vp[(size_t)l * N + r], u1[l]));
+                        a = zsub(a, zmul(Wp[(size_t)l * N + r], u2[l]));
+                    }
+                    if (r == j) a.y = 0.0;
+                    A[(size_t)j * N + r] = a;
+                }
+                __syncthreads();
+            }
+            // (2) reflector for x = A[j+1:N, j] (zlarfg)
+            double xn = 0.0;
+            for (int r = j + 2 + tid; r < N; r += blockDim.x) {
+                const double2 x = A[(size_t)j * N + r];
+                xn += x.x * x.x + x.y * x.y;
+            }
+            xn = block_sum(xn, red);
+            if (tid == 0) {
+                const double2 al = A[(size_t)j * N + j + 1];
+                double2 tau = zzero();
+                double beta = al.x;
+                if (!(xn == 0.0 && al.y == 0.0)) {
+                    beta = -copysign(sqrt(al.x * al.x + al.y * al.y + xn), al.x);
+                    tau = make_double2((beta - al.x) / beta, -al.y / beta);
+                }
+                tauS = tau; betaS = beta;
+                d[j] = A[(size_t)j * N + j].x;
+                e[j] = beta;
+            }
+            __syncthreads();
+            const double2 tau = tauS;
+            const double beta = betaS;
+            {
+                const double2 al = A[(size_t)j * N + j + 1];
+                const double2 dv = make_double2(al.x - beta, al.y);
+                const double den = dv.x * dv.x + dv.y * dv.y;
+                const double2 sc = (tau.x != 0.0 || tau.y != 0.0) ? make_double2(dv.x / den, -dv.y / den)
+                                                                  : make_double2(1.0, 0.0);
+                __syncthreads();
+                for (int r = tid; r < N; r += blockDim.x) {
+                    double2 v;
+                    if (r <= j) v = zzero();
+                    else if (r == j + 1) v = make_double2(1.0, 0.0);
+                    else v = zmul(A[(size_t)j * N + r], sc);
+                    A[(size_t)j * N + r] = v;          // reflector stored in place
+                    Vp[(size_t)i * N + r] = v;
+                    vloc[r] = v;
+                }
+            }
+            __syncthreads();
+            // (3) y = A_t v over the trailing block (rows/cols j+1..N-1):
+            //     wave wv takes columns c = j+1+wv, +4, ...; lanes take rows
+            const int t0 = j + 1;
+            for (int rb = t0; rb < N; rb += 64) {
+                const int r = rb + lane;
+                double2 acc = zzero();
+                if (r < N)
+                    for (int c = t0 + wv; c < N; c += 4) acc = zadd(acc, zmul(A[(size_t)c * N + r], vloc[c]));
+                yacc[wv][lane] = acc;
+                __syncthreads();
+                if (wv == 0 && r < N) {
+                    double2 y = zadd(zadd(yacc[0][lane], yacc[1][lane]), zadd(yacc[2][lane], yacc[3][lane]));
+                    Wp[(size_t)i * N + r] = y;          // raw A_t v, corrected below
+                }
+                __syncthreads();
+            }
+            // u1 = W^H v, u2 = V^H v over rows j+1..N-1 (l < i)
+            for (int l = 0; l < i; ++l) {
+                double s1r = 0, s1i = 0, s2r = 0, s2i = 0;
+                for (int r = t0 + tid; r < N; r += blockDim.x) {
+                    const double2 v = vloc[r];
+                    const double2 a1 = zcmul(Wp[(size_t)l * N + r], v);
+                    const double2 a2 = zcmul(Vp[(size_t)l * N + r], v);
+                    s1r += a1.x; s1i += a1.y; s2r += a2.x; s2i += a2.y;
+                }
+                s1r = block_sum(s1r, red); s1i = block_sum(s1i, red);
+                s2r = block_sum(s2r, red); s2i = block_sum(s2i, red);
+                if (tid == 0) { u1[l] = make_double2(s1r, s1i); u2[l] = make_double2(s2r, s2i); }
+            }
+            __syncthreads();
+            // w = tau (y - V u1 - W u2); then w += alpha v, alpha = -tau/2 (w^H v)
+            double pr = 0.0, pi = 0.0;
+            for (int r = t0 + tid; r < N; r += blockDim.x) {
+                double2 y = Wp[(size_t)i * N + r];
+                for (int l = 0; l < i; ++l) {
+                    y = zsub(y, zmul(Vp[(size_t)l * N + r], u1[l]));
+                    y = zsub(y, zmul(Wp[(size_t)l * N + r], u2[l]));
+                }
+                const double2 w = zmul(tau, y);
+                Wp[(size_t)i * N + r] = w;
+                const double2 q = zcmul(w, vloc[r]);
+                pr += q.x; pi += q.y;
+            }
+            pr = block_sum(pr, red); pi = block_sum(pi, red);
+            const double2 alpha = zscale(-0.5, zmul(tau, make_double2(pr, pi)));
+            for (int r = t0 + tid; r < N; r += blockDim.x)
+                Wp[(size_t)i * N + r] = zadd(Wp[(size_t)i * N + r], zmul(alpha, vloc[r]));
+            // WY factor column i: T[0:i, i] = -tau T[0:i,0:i] (V^H v), T[i][i] = tau
+            for (int l = 0; l < i; ++l) {
+                double sr = 0.0, si = 0.0;
+                for (int r = t0 + tid; r < N; r += blockDim.x) {
+                    const double2 q = zcmul(Vp[(size_t)l * N + r], vloc[r]);
+                    sr += q.x; si += q.y;
+                }
+                sr = block_sum(sr, red); si = block_sum(si, red);
+                if (tid == 0) u2[l] = make_double2(sr, si);
+            }
+            __syncthreads();
+            double2 *Tp = Tf + (size_t)(i0 / NB) * NB * NB;   // NB x NB, column-major
+            if (tid < i) {
+                double2 acc = zzero();
+                for (int l = tid; l < i; ++l) acc = zadd(acc, zmul(Tp[(size_t)l * NB + tid], u2[l]));
+                Tp[(size_t)i * NB + tid] = zmul(make_double2(-tau.x, -tau.y), acc);
+            }
+            if (tid == 0) Tp[(size_t)i * NB + i] = tau;
+            __syncthreads();
+        }
+        // trailing her2k: A[s:, s:] -= V W^H + W V^H over rows/cols s..N-1
+        const int s = i0 + pw;
+        for (int cb = s; cb < N; cb += 16) {
+            const int nc = min(16, N - cb);
+            __syncthreads();
+            for (int t = tid; t < pw * 16; t += blockDim.x) {
+                const int l = t / 16, c = t % 16;
+                cw[l][c] = (c < nc) ? zconj(Wp[(size_t)l * N + cb + c]) : zzero();
+                cv[l][c] = (c < nc) ? zconj(Vp[(size_t)l * N + cb + c]) : zzero();
+            }
+            __syncthreads();
+            for (int r = s + tid; r < N; r += blockDim.x) {
+                double2 acc[16];
+                for (int c = 0; c < 16; ++c) acc[c] = zzero();
+                for (int l = 0; l < pw; ++l) {
+                    const double2 vr = Vp[(size_t)l * N + r], wr = Wp[(size_t)l * N + r];
+                    for (int c = 0; c < 16; ++c) acc[c] = zadd(acc[c], zadd(zmul(vr, cw[l][c]), zmul(wr, cv[l][c])));
+                }
+                for (int c = 0; c < nc; ++c) {
+                    double2 *p = A + (size_t)(cb + c) * N + r;
+                    double2 v = zsub(*p, acc[c]);
+                    if (r == cb + c) v.y = 0.0;
+                    *p = v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        d[N - 1] = A[(size_t)(N - 1) * N + N - 1].x;
+        e[N - 1] = 0.0;
+    }
+    __syncthreads();
+    // the last column holds no reflector
+    for (int r = tid; r < N; r += blockDim.x) A[(size_t)(N - 1) * N + r] = zzero();
+}
+
+// ---------------------------------------------------------------------------
+// Tridiagonal eigenpairs: bisection + inverse iteration, one WG per k
+// ---------------------------------------------------------------------------
+__device__ int sturm_count(const double *d, const double *e2, int n, double x, double pivmin)
+{
+    int cnt = 0;
+    double q = d[0] - x;
+    if (fabs(q) <= pivmin) q = -pivmin;
+    cnt += q < 0.0;
+    for (int i = 1; i < n; ++i) {
+        q = d[i] - x - e2[i - 1] / q;
+        if (fabs(q) <= pivmin) q = -pivmin;
+        cnt += q < 0.0;
+    }
+    return cnt;
+}
+
+// Deterministic uniform(-1,1) start vector entry (inverse iteration).
+__device__ __forceinline__ double start_entry(int i, int j)
+{
+    uint32_t x = 0x9E3779B9u * (uint32_t)(i + 1) ^ (0x85EBCA6Bu * (uint32_t)(j + 7));
+    x ^= x >> 16; x *= 0x7FEB352Du; x ^= x >> 15; x *= 0x846CA68Bu; x ^= x >> 16;
+    return (double)x * (2.0 / 4294967296.0) - 1.0;
+}
+
+__global__ __launch_bounds__(256) void k_stein(SteinArgs s)
+{
+    __shared__ double d[1280], e[1280], e2[1280], w[1280];
+    __shared__ int cstart[1281];
+    __shared__ int ncl;
+    __shared__ double red[16];
+    const int N = s.n;
+    const int kb = blockIdx.x;
+    const int tid = threadIdx.x;
+    const double *dg = s.d + (size_t)kb * N;
+    const double *eg = s.e + (size_t)kb * N;
+    double *Vt = s.v + (size_t)kb * N * N;            // Vt[i*N + j] = v_j(i)
+    double *scr = s.scratch + (size_t)kb * 5 * N * blockDim.x;
+    for (int i = tid; i < N; i += blockDim.x) {
+        d[i] = dg[i];
+        e[i] = (i < N - 1) ? eg[i] : 0.0;
+        e2[i] = e[i] * e[i];
+    }
+    __syncthreads();
+    // Gershgorin bounds and the 1-norm (dstein's onenrm)
+    double lo = DBL_MAX, hi = -DBL_MAX, nrm = 0.0, emax = 0.0;
+    for (int i = tid; i < N; i += blockDim.x) {
+        const double r = fabs(e[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0);
+        lo = fmin(lo, d[i] - r); hi = fmax(hi, d[i] + r);
+        nrm = fmax(nrm, fabs(d[i]) + r);
+        emax = fmax(emax, e2[i]);
+    }
+    // block min / max via the sum helper on transformed values is not
+    // order-free; use shared atomics-free reduction instead
+    __shared__ double rlo[256], rhi[256], rn[256], re[256];
+    rlo[tid] = lo; rhi[tid] = hi; rn[tid] = nrm; re[tid] = emax;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (tid < st) {
+            rlo[tid] = fmin(rlo[tid], rlo[tid + st]); rhi[tid] = fmax(rhi[tid], rhi[tid + st]);
+            rn[tid] = fmax(rn[tid], rn[tid + st]); re[tid] = fmax(re[tid], re[tid + st]);
+        }
+        __syncthreads();
+    }
+    const double onenrm = rn[0];
+    const double pivmin = DBL_MIN * fmax(1.0, re[0]);
+    const double eps = DBL_EPSILON * 0.5;                 // dlamch('E')
+    const double gl = rlo[0] - 2.0 * eps * onenrm - pivmin;
+    const double gu = rhi[0] + 2.0 * eps * onenrm + pivmin;
+    // bisection for eigenvalue index j (ascending)
+    for (int j = tid; j < N; j += blockDim.x) {
+        double a = gl, b = gu;
+        for (int it = 0; it < 200; ++it) {
+            const double mid = 0.5 * (a + b);
+            const double tol = 2.0 * eps * fmax(fabs(a), fabs(b)) + pivmin;
+            if (b - a <= tol || mid <= a || mid >= b) break;
+            if (sturm_count(d, e2, N, mid, pivmin) >= j + 1) b = mid; else a = mid;
+        }
+        w[j] = 0.5 * (a + b);
+    }
+    __syncthreads();
+    for (int j = tid; j < N; j += blockDim.x) s.w[(size_t)kb * N + j] = w[j];
+    // orthogonalisation groups: consecutive eigenvalues closer than ortol
+    const double ortol = s.ortol_rel * onenrm;
+    if (tid == 0) {
+        int nc = 0;
+        for (int j = 0; j < N; ++j)
+            if (j == 0 || fabs(w[j] - w[j - 1]) > ortol) cstart[nc++] = j;
+        cstart[nc] = N;
+        ncl = nc;
+    }
+    __syncthreads();
+    // per-thread LU workspace: columns a, b, c, dd, y of length N
+    const int nthr = blockDim.x;
+    double *wa = scr + 0 * (size_t)N * nthr, *wb = scr + 1 * (size_t)N * nthr;
+    double *wc = scr + 2 * (size_t)N * nthr, *wd = scr + 3 * (size_t)N * nthr;
+    double *wy = scr + 4 * (size_t)N * nthr;
+#define AT(arr, i) arr[(size_t)(i) * nthr + tid]
+    const double dtpcrt = sqrt(0.1 / N);
+    int fails = 0;
+    for (int cl = tid; cl < ncl; cl += nthr) {
+        const int j0 = cstart[cl], j1 = cstart[cl + 1];
+        double xjm = 0.0;
+        for (int j = j0; j < j1; ++j) {
+            double xj = w[j];
+            if (j > j0) {
+                const double pertol = 10.0 * fabs(eps * xj);
+                if (xj - xjm < pertol) xj = xjm + pertol;
+            }
+            // LU of T - xj I with partial pivoting (dlagtf)
+            for (int i = 0; i < N; ++i) {
+                AT(wa, i) = d[i] - xj;
+                AT(wb, i) = e[i];
+                AT(wc, i) = e[i];
+                AT(wd, i) = 0.0;
+            }
+            unsigned int piv_lo = 0;   // pivot flags packed per 32 rows
+            (void)piv_lo;
+            double scale1 = fabs(AT(wa, 0)) + (N > 1 ? fabs(AT(wb, 0)) : 0.0);
+            // pivot flags stored as sign of an extra encoding in wd's slot n-1..:
+            // use wy temporarily as flag storage (0 / 1)
+            for (int k = 0; k < N - 1; ++k) {
+                double scale2 = fabs(AT(wc, k)) + fabs(AT(wa, k + 1));
+                if (k < N - 2) scale2 += fabs(AT(wb, k + 1));
+                const double ak = AT(wa, k);
+                const double piv1 = (ak == 0.0) ? 0.0 : fabs(ak) / scale1;
+                const double ck = AT(wc, k);
+                if (ck == 0.0) {
+                    AT(wy, k) = 0.0;
+                    scale1 = scale2;
+                    if (k < N - 2) AT(wd, k) = 0.0;
+                } else {
+                    const double piv2 = fabs(ck) / scale2;
+                    if (piv2 <= piv1) {
+                        AT(wy, k) = 0.0;
+                        scale1 = scale2;
+                        const double mult = ck / ak;
+                        AT(wc, k) = mult;
+                        AT(wa, k + 1) = AT(wa, k + 1) - mult * AT(wb, k);
+                        if (k < N - 2) AT(wd, k) = 0.0;
+                    } else {
+                        AT(wy, k) = 1.0;
+                        const double mult = ak / ck;
+                        AT(wa, k) = ck;
+                        const double temp = AT(wa, k + 1);
+                        AT(wa, k + 1) = AT(wb, k) - mult * temp;
+                        if (k < N - 2) {
+                            AT(wd, k) = AT(wb, k + 1);
+                            AT(wb, k + 1) = -mult * AT(wd, k);
+                        }
+                        AT(wb, k) = temp;
+                        AT(wc, k) = mult;
+                    }
+                }
+            }
+            // pivot flags now in wy[0..N-2]; keep them in registers-free form:
+            // move into the sign bit of a spare column (wd keeps d2); we copy
+            // flags to Vt-independent storage: reuse wy until the solve, then
+            // the iterate lives in the output column of Vt.
+            double tolp = fabs(AT(wa, 0));
+            if (N > 1) tolp = fmax(tolp, fmax(fabs(AT(wa, 1)), fabs(AT(wb, 0))));
+            for (int k = 2; k < N; ++k)
+                tolp = fmax(tolp, fmax(fabs(AT(wa, k)), fmax(fabs(AT(wb, k - 1)), fabs(AT(wd, k - 2)))));
+            tolp *= eps;
+            if (tolp == 0.0) tolp = eps;
+            // start vector in the output column of Vt
+            for (int i = 0; i < N; ++i) Vt[(size_t)i * N + j] = start_entry(i, j);
+            int its = 0, nrmchk = 0;
+            bool ok = false;
+            while (its < 5) {
+                ++its;
+                double asum = 0.0;
+                for (int i = 0; i < N; ++i) asum += fabs(Vt[(size_t)i * N + j]);
+                const double scl = N * onenrm * fmax(eps, fabs(AT(wa, N - 1))) / fmax(asum, DBL_MIN);
+                // forward: apply P and L
+                double yprev = Vt[j] * scl;
+                Vt[j] = yprev;
+                for (int k = 1; k < N; ++k) {
+                    double yk = Vt[(size_t)k * N + j] * scl;
+                    if (AT(wy, k - 1) == 0.0) {
+                        yk = yk - AT(wc, k - 1) * yprev;
+                    } else {
+                        const double t = yprev;
+                        Vt[(size_t)(k - 1) * N + j] = yk;
+                        yk = t - AT(wc, k - 1) * yk;
+                    }
+                    Vt[(size_t)k * N + j] = yk;
+                    yprev = yk;
+                }
+                // back substitution with perturbed tiny pivots (dlagts, job -1)
+                for (int k = N - 1; k >= 0; --k) {
+                    double temp = Vt[(size_t)k * N + j];
+                    if (k <= N - 3)
+                        temp = temp - AT(wb, k) * Vt[(size_t)(k + 1) * N + j] - AT(wd, k) * Vt[(size_t)(k + 2) * N + j];
+                    else if (k == N - 2)
+                        temp = temp - AT(wb, k) * Vt[(size_t)(k + 1) * N + j];
+                    double ak = AT(wa, k);
+                    double pert = copysign(tolp, ak);
+                    for (int g = 0; g < 64; ++g) {
+                        const double absak = fabs(ak);
+                        if (absak < 1.0) {
+                            if (absak < DBL_MIN) {
+                                if (absak == 0.0 || fabs(temp) * DBL_MIN > absak) { ak += pert; pert *= 2.0; continue; }
+                                temp *= 1.0 / DBL_MIN; ak *= 1.0 / DBL_MIN;
+                            } else if (fabs(temp) > absak * (1.0 / DBL_MIN)) {
+                                ak += pert; pert *= 2.0; continue;
+                            }
+                        }
+                        break;
+                    }
+                    Vt[(size_t)k * N + j] = temp / ak;
+                }
+                // reorthogonalise against earlier members of the group
+                for (int jp = j0; jp < j; ++jp) {
+                    double dot = 0.0;
+                    for (int i = 0; i < N; ++i) dot += Vt[(size_t)i * N + j] * Vt[(size_t)i * N + jp];
+                    for (int i = 0; i < N; ++i) Vt[(size_t)i * N + j] -= dot * Vt[(size_t)i * N + jp];
+                }
+                double amax = 0.0;
+                for (int i = 0; i < N; ++i) amax = fmax(amax, fabs(Vt[(size_t)i * N + j]));
+                if (amax < dtpcrt) continue;
+                ++nrmchk;
+                if (nrmchk < 3) continue;
+                ok = true;
+                break;
+            }
+            if (!ok) ++fails;
+            double ss = 0.0, amax = 0.0;
+            int imax = 0;
+            for (int i = 0; i < N; ++i) {
+                const double v = Vt[(size_t)i * N + j];
+                ss += v * v;
+                if (fabs(v) > amax) { amax = fabs(v); imax = i; }
+            }
+            double scl = 1.0 / sqrt(ss);
+            if (Vt[(size_t)imax * N + j] < 0.0) scl = -scl;
+            for (int i = 0; i < N; ++i) Vt[(size_t)i * N + j] *= scl;
+            xjm = xj;
+        }
+    }
+#undef AT
+    const double f = block_sum((double)fails, red);
+    if (tid == 0) s.info[kb] = (int)f;
+}
+
+// Complex copy of the real row-major eigenvectors: Y[:, j] = v_j.
+__global__ void k_real_to_complex_t(int N, const double *Vt, double2 *Y, size_t stride)
+{
+    const int kb = blockIdx.y;
+    const double *V = Vt + (size_t)kb * N * N;
+    double2 *Yk = Y + (size_t)kb * stride;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < (size_t)N * N;
+         t += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(t % N), c = (int)(t / N);
+        Yk[t] = make_double2(V[(size_t)r * N + c], 0.0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// First-order perturbation (perturb.f:87-221)
+// ---------------------------------------------------------------------------
+// Column dots: px[j] = z_j^H X[:, j], py[j] = z_j^H Y[:, j].
+__global__ __launch_bounds__(256) void k_coldot(int N, const double2 *Z, const double2 *X,
+                                                const double2 *Y, size_t stride, double2 *px,
+                                                double2 *py)
+{
+    const int kb = blockIdx.y;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int j = blockIdx.x * 4 + wv;
+    if (j >= N) return;
+    const double2 *z = Z + (size_t)kb * stride + (size_t)j * N;
+    const double2 *x = X + (size_t)kb * stride + (size_t)j * N;
+    const double2 *y = Y + (size_t)kb * stride + (size_t)j * N;
+    double ar = 0, ai = 0, br = 0, bi = 0;
+    for (int i = lane; i < N; i += 64) {
+        const double2 zi = z[i];
+        const double2 p = zcmul(zi, x[i]), q = zcmul(zi, y[i]);
+        ar += p.x; ai += p.y; br += q.x; bi += q.y;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ar += __shfl_down(ar, off, 64); ai += __shfl_down(ai, off, 64);
+        br += __shfl_down(br, off, 64); bi += __shfl_down(bi, off, 64);
+    }
+    if (lane == 0) {
+        px[(size_t)kb * N + j] = make_double2(ar, ai);
+        py[(size_t)kb * N + j] = make_double2(br, bi);
+    }
+}
+
+// Eigenvalues of a small Hermitian matrix (n <= 9) by cyclic Jacobi on its
+// real symmetric 2n x 2n embedding; every eigenvalue appears twice.
+__device__ void small_heev(int n, const double2 *hd, double *out)
+{
+    double b[18][18];
+    const int m = 2 * n;
+    for (int c = 0; c < n; ++c)
+        for (int r = 0; r < n; ++r) {
+            const double2 v = hd[c * 9 + r];
+            b[r][c] = v.x; b[r + n][c + n] = v.x;
+            b[r + n][c] = v.y; b[r][c + n] = -v.y;
+        }
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0;
+        for (int r = 0; r < m; ++r) for (int c = 0; c < m; ++c) if (r != c) off += b[r][c] * b[r][c];
+        if (off < 1e-300) break;
+        for (int p = 0; p < m - 1; ++p)
+            for (int q = p + 1; q < m; ++q) {
+                const double apq = b[p][q];
+                if (apq == 0.0) continue;
+                const double th = (b[q][q] - b[p][p]) / (2.0 * apq);
+                const double t = copysign(1.0, th) / (fabs(th) + sqrt(th * th + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < m; ++k) {
+                    const double x = b[k][p], y = b[k][q];
+                    b[k][p] = c * x - s * y; b[k][q] = s * x + c * y;
+                }
+                for (int k = 0; k < m; ++k) {
+                    const double x = b[p][k], y = b[q][k];
+                    b[p][k] = c * x - s * y; b[q][k] = s * x + c * y;
+                }
+            }
+    }
+    double ev[18];
+    for (int i = 0; i < m; ++i) ev[i] = b[i][i];
+    for (int i = 1; i < m; ++i) {
+        const double x = ev[i];
+        int k = i - 1;
+        while (k >= 0 && ev[k] > x) { ev[k + 1] = ev[k]; --k; }
+        ev[k + 1] = x;
+    }
+    for (int i = 0; i < n; ++i) out[i] = 0.5 * (ev[2 * i] + ev[2 * i + 1]);
+}
+
+// One thread per k-point: degenerate groups (|e_j - e_start| <= 1e-8, at
+// most 9) get the eigenvalues of <a|dH - e_start dS|b>; isolated states
+// take the diagonal element.  ncut counts states up to eigcut.
+__global__ void k_perturb(PerturbArgs p)
+{
+    const int kb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (kb >= p.nb) return;
+    const int N = p.n;
+    const double *w = p.w + (size_t)kb * N;
+    const double2 *px = p.px + (size_t)kb * N, *py = p.py + (size_t)kb * N;
+    double *desk = p.desk + (size_t)kb * N;
+    const double2 *Z = p.z + (size_t)kb * p.stride;
+    const double2 *X = p.x + (size_t)kb * p.stride;
+    const double2 *Y = p.y + (size_t)kb * p.stride;
+    int start = 0, err = 0, ncut = N;
+    while (start < N) {
+        if (w[start] > p.eigcut) { ncut = start; break; }
+        int num = N - start;
+        for (int j = start + 1; j < N; ++j)
+            if (fabs(w[j] - w[start]) > 1e-8) { num = j - start; break; }
+        if (num > 9) { err = 1; ncut = start; break; }
+        const double es = w[start];
+        if (num == 1) {
+            desk[start] = px[start].x - es * py[start].x;
+        } else {
+            double2 hd[81];
+            for (int b = 0; b < num; ++b)
+                for (int a = 0; a < num; ++a) {
+                    const int ja = start + a, jb = start + b;
+                    double2 v;
+                    if (a == b) {
+                        v = make_double2(px[ja].x - es * py[ja].x, 0.0);
+                    } else {
+                        double sr = 0, si = 0;
+                        for (int i = 0; i < N; ++i) {
+                            const double2 z = Z[(size_t)ja * N + i];
+                            const double2 q = zsub(X[(size_t)jb * N + i], zscale(es, Y[(size_t)jb * N + i]));
+                            const double2 t = zcmul(z, q);
+                            sr += t.x; si += t.y;
+                        }
+                        v = make_double2(sr, si);
+                    }
+                    hd[b * 9 + a] = v;
+                }
+            if (num == 2) {
+                const double h1 = hd[0].x, h3 = hd[10].x;
+                const double2 h2 = hd[9];
+                const double sq = sqrt((h1 - h3) * (h1 - h3) + 4.0 * (h2.x * h2.x + h2.y * h2.y));
+                desk[start] = 0.5 * ((h1 + h3) - sq);
+                desk[start + 1] = 0.5 * ((h1 + h3) + sq);
+            } else {
+                small_heev(num, hd, desk + start);
+            }
+        }
+        start += num;
+    }
+    for (int j = ncut; j < N; ++j) desk[j] = 0.0;
+    p.ncut[p.k0 + kb] = ncut;
+    if (err) atomicOr(p.err, 2);
+}
+
+// ---------------------------------------------------------------------------
+// Fermi level and sums over all k-points, one workgroup
+// ---------------------------------------------------------------------------
+// Reductions run over a fixed thread -> k mapping and a fixed tree, so the
+// result does not depend on how the k-points were batched.
+__device__ void block_min_max(double &mn, double &mx, double *sm, double *sx)
+{
+    const int tid = threadIdx.x;
+    sm[tid] = mn; sx[tid] = mx;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (tid < st) { sm[tid] = fmin(sm[tid], sm[tid + st]); sx[tid] = fmax(sx[tid], sx[tid + st]); }
+        __syncthreads();
+    }
+    mn = sm[0]; mx = sx[0];
+    __syncthreads();
+}
+
+__device__ double block_sum_tree(double v, double *sbuf)
+{
+    const int tid = threadIdx.x;
+    sbuf[tid] = v;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (tid < st) sbuf[tid] = sbuf[tid] + sbuf[tid + st];
+        __syncthreads();
+    }
+    const double r = sbuf[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(1024) void k_fermi_sums(FermiArgs f)
+{
+    __shared__ double s1[1024], s2[1024];
+    __shared__ double mu_s;
+    const int tid = threadIdx.x;
+    const int nk = f.nk, nv = f.nv, ld = f.ld;
+    const double el = f.el, tkb = f.tkb, zfill = f.zfill;
+    const double nearz = 1e-5, tol = 1e-8, twenty = 20.0;
+    const double beta = (tkb < nearz) ? 1.0 / nearz : 1.0 / tkb;
+    int nfill = (int)(zfill / el + nearz);
+    const bool linst = fabs(zfill / el - (double)nfill) < tol;
+    if (nfill < 1) nfill = 1;
+    if (nfill > nv - 1) nfill = nv > 1 ? nv - 1 : 1;
+    // bracket (fermi.f:197-227)
+    double stlow = -DBL_MAX, sthigh = DBL_MAX, mulow = DBL_MAX, muhigh = -DBL_MAX;
+    for (int k = tid; k < nk; k += blockDim.x) {
+        const double *ev = f.esk + (size_t)k * ld;
+        stlow = fmax(stlow, ev[nfill - 1]);
+        sthigh = fmin(sthigh, ev[nfill]);
+        mulow = fmin(mulow, ev[0]);
+        muhigh = fmax(muhigh, ev[nv - 1]);
+    }
+    {
+        double a = -stlow, b = muhigh;
+        block_min_max(a, b, s1, s2);       // max(stlow) = -min(-stlow)
+        stlow = -a; muhigh = b;
+        double c = mulow, d = -sthigh;
+        block_min_max(c, d, s1, s2);
+        mulow = c; sthigh = -d;
+    }
+    double mu;
+    bool done = false;
+    if (linst && (sthigh - stlow > twenty * tkb)) { mu = 0.5 * (stlow + sthigh); done = true; }
+    else mu = 0.5 * (stlow + sthigh);
+    int iters = 0;
+    while (!done && iters < 1000) {
+        ++iters;
+        const double mum20 = mu - tkb * twenty, mup20 = mu + tkb * twenty;
+        const double btmu = -beta * mu;
+        double ns = 0.0, dns = 0.0;
+        for (int k = tid; k < nk; k += blockDim.x) {
+            const double *ev = f.esk + (size_t)k * ld;
+            double a = 0.0, b = 0.0;
+            for (int i = 0; i < nv; ++i) {
+                const double x = ev[i];
+                if (x < mum20) a = a + 1.0;
+                else if (x < mup20) {
+                    const double ex = exp(beta * x + btmu);
+                    const double fx = 1.0 / (1.0 + ex);
+                    a = a + fx;
+                    b = b + ex * fx * fx;
+                } else break;
+            }
+            ns = ns + f.w[k] * a;
+            dns = dns + f.w[k] * b;
+        }
+        double nsum = block_sum_tree(ns, s1);
+        double dnsum = block_sum_tree(dns, s2);
+        nsum = el * nsum;
+        dnsum = beta * el * dnsum;
+        if (fabs(nsum - zfill) < tol) break;
+        if (nsum < zfill) mulow = mu; else muhigh = mu;
+        double mu1;
+        if (dnsum < tol) mu1 = 0.5 * (mulow + muhigh);
+        else {
+            mu1 = mu + (zfill - nsum) / dnsum;
+            if (mu1 > muhigh) mu1 = 0.5 * (mu + muhigh);
+            else if (mu1 < mulow) mu1 = 0.5 * (mu + mulow);
+        }
+        if (muhigh - mulow > tol) mu = mu1;
+        else break;
+    }
+    if (tid == 0) mu_s = mu;
+    __syncthreads();
+    const double efermi = mu_s;
+    // sums (bande.f:1170-1321)
+    const double tkbi = (tkb < nearz) ? 1.0 / nearz : 1.0 / tkb;
+    const double etol = twenty * tkb, efp = efermi + etol, efm = efermi - etol;
+    const bool lpress = f.mode == 2 || f.mode == 4;
+    const bool lentro = f.mode < 3;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // esum desum1 desum2 desum3 dfbsum dferm entropy dent1 dent2
+    for (int k = tid; k < nk; k += blockDim.x) {
+        const double *ev = f.esk + (size_t)k * ld;
+        const double *dv = f.desk + (size_t)k * ld;
+        const double wt = el * f.w[k];
+        const int ni = lpress ? f.ncut[k] : nv;
+        for (int i = 0; i < ni; ++i) {
+            const double x = ev[i];
+            if (!(x < efp)) continue;
+            const double ewt = wt * x;
+            if (x < efm) {
+                acc[0] = acc[0] + ewt;
+                if (lpress) acc[1] = acc[1] + wt * dv[i];
+            } else {
+                const double fbe = 1.0 / (1.0 + exp(tkbi * (x - efermi)));
+                acc[0] = acc[0] + ewt * fbe;
+                if (lentro) acc[6] = acc[6] - wt * (fbe * log(fbe) + (1.0 - fbe) * log(1.0 - fbe));
+                if (lpress) {
+                    const double fbm = 1.0 / (1.0 + exp(tkbi * (-(x - efermi))));
+                    const double dfbe = -tkbi * fbe * fbm;
+                    acc[4] = acc[4] + dfbe * wt;
+                    acc[3] = acc[3] + dfbe * ewt;
+                    const double dewt = wt * dv[i];
+                    acc[1] = acc[1] + dewt * fbe;
+                    acc[5] = acc[5] + dfbe * dewt;
+                    acc[2] = acc[2] + dfbe * ewt * dv[i];
+                    if (lentro) {
+                        const double con1 = dfbe * wt * (x - efermi);
+                        acc[7] = acc[7] + con1 * dv[i];
+                        acc[8] = acc[8] + con1;
+                    }
+                }
+            }
+        }
+    }
+    double tot[9];
+    for (int q = 0; q < 9; ++q) tot[q] = block_sum_tree(acc[q], s1);
+    if (tid == 0) {
+        const double esum = tot[0];
+        double e1, e2 = 0.0;
+        if (lentro) { const double fr = esum - tkb * tot[6]; e1 = 0.5 * (esum + fr); e2 = esum; }
+        else e1 = esum;
+        if (lpress) {
+            const double dfermi = (fabs(tot[4]) > 1e-10) ? tot[5] / tot[4] : 0.0;
+            const double press = -(tot[1] + (tot[2] - dfermi * tot[3]));
+            if (lentro) { const double pent = dfermi * tot[8] - tot[7]; e2 = 0.5 * (press + (press - pent)); }
+            else e2 = press;
+        }
+        f.out[0] = efermi; f.out[1] = e1; f.out[2] = e2;
+        f.out[3] = esum; f.out[4] = tot[6];
+        f.out[5] = (double)iters;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_build_hs(const BuildArgs &a, int natoms, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_build_hs, dim3(natoms, nb), dim3(256), 0, st, a);
+}
+void launch_potrf_trtri(int N, double2 *S, size_t stride, double2 *work, int *info, int nb,
+                        hipStream_t st)
+{
+    hipLaunchKernelGGL(k_potrf_trtri, dim3(nb), dim3(256), 0, st, N, S, stride, work, info);
+}
+void launch_zgemm(const GemmArgs &g, int opa, int opb, int nb, hipStream_t st)
+{
+    dim3 grid((g.m + 63) / 64, (g.n + 63) / 64, nb);
+    if (opa == 0 && opb == 0) hipLaunchKernelGGL((k_zgemm<0, 0>), grid, dim3(256), 0, st, g);
+    else if (opa == 1 && opb == 0) hipLaunchKernelGGL((k_zgemm<1, 0>), grid, dim3(256), 0, st, g);
+    else if (opa == 0 && opb == 1) hipLaunchKernelGGL((k_zgemm<0, 1>), grid, dim3(256), 0, st, g);
+    else hipLaunchKernelGGL((k_zgemm<1, 1>), grid, dim3(256), 0, st, g);
+}
+void launch_hetrd(const HetrdArgs &h, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_hetrd, dim3(nb), dim3(256), 0, st, h);
+}
+void launch_stein(const SteinArgs &s, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_stein, dim3(nb), dim3(256), 0, st, s);
+}
+void launch_real_to_complex_t(int N, const double *Vt, double2 *Y, size_t stride, int nb,
+                              hipStream_t st)
+{
+    const int blocks = (int)std::min<size_t>(((size_t)N * N + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_real_to_complex_t, dim3(blocks, nb), dim3(256), 0, st, N, Vt, Y, stride);
+}
+void launch_coldot(int N, const double2 *Z, const double2 *X, const double2 *Y, size_t stride,
+                   double2 *px, double2 *py, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_coldot, dim3((N + 3) / 4, nb), dim3(256), 0, st, N, Z, X, Y, stride, px, py);
+}
+void launch_perturb(const PerturbArgs &p, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_perturb, dim3((p.nb + 63) / 64), dim3(64), 0, st, p);
+}
+void launch_fermi_sums(const FermiArgs &f, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fermi_sums, dim3(1), dim3(1024), 0, st, f);
+}
+
+}  // namespace tbk

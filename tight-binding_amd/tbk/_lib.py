@@ -1,0 +1,96 @@
+"""ctypes binding of lib/libtbk.so (the C ABI declared in include/tbk.h).
+
+Loading fails loudly when the HIP library is missing: there is no CPU
+fallback anywhere in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libtbk.so")
+
+TBK_OK, TBK_ERR_ARG, TBK_ERR_PAIRS, TBK_ERR_OVERLAP = 0, 1, 2, 3
+TBK_ERR_CONVERGE, TBK_ERR_ZFILL, TBK_ERR_DEGEN = 4, 5, 6
+TBK_ERR_HIP, TBK_ERR_NOMEM, TBK_ERR_NODEVICE = 10, 11, 12
+TBK_FP64, TBK_MIXED = 0, 1
+
+# Every symbol include/tbk.h declares (checked by the CPU test suite).
+EXPORTS = [
+    "tbk_search2", "tbk_setpar", "tbk_setparv", "tbk_plan_create", "tbk_plan_run",
+    "tbk_plan_launch", "tbk_plan_wait", "tbk_plan_eigenvalues", "tbk_plan_eigenvectors",
+    "tbk_plan_hs", "tbk_plan_nv", "tbk_plan_npairs", "tbk_plan_destroy",
+    "tbk_plan_stage_times", "tbk_plan_set_timing", "tbk_zhegv", "tbk_fermi",
+    "tbk_error_string", "tbk_debug_stages", "tbk_debug_zgemm",
+]
+
+
+class TbkError(RuntimeError):
+    def __init__(self, code: int, what: str = ""):
+        self.code = code
+        msg = _lib.tbk_error_string(code).decode() if _lib is not None else str(code)
+        super().__init__(f"{what}: {msg} (code {code})" if what else f"{msg} (code {code})")
+
+
+class StructureDesc(C.Structure):
+    _fields_ = [
+        ("natoms", C.c_int), ("lat", C.POINTER(C.c_double)), ("posn", C.POINTER(C.c_double)),
+        ("posq", C.POINTER(C.c_double)), ("kkind", C.POINTER(C.c_int)), ("nd", C.c_int * 3),
+        ("kinds", C.c_int), ("kbas", C.POINTER(C.c_int)), ("valence", C.POINTER(C.c_double)),
+        ("dnn", C.POINTER(C.c_double)), ("screenl", C.POINTER(C.c_double)),
+        ("param", C.POINTER(C.c_double)), ("nparam", C.c_int), ("nltype", C.c_int),
+        ("jsover", C.c_int), ("thisvol", C.c_double), ("zfill", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        i = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        ip = C.POINTER(C.c_int)
+        dp = C.POINTER(C.c_double)
+        vp = C.c_void_p
+        L.tbk_search2.argtypes = [C.c_int, d, d, i, C.c_int, i, d, d, C.c_int, ip, d, d, d, d,
+                                  d, i, i, dp]
+        L.tbk_setpar.argtypes = [C.c_int, i, C.c_int, d, C.c_int, C.c_int, C.c_int, d, d, i, i,
+                                 d, d, d, d]
+        L.tbk_setparv.argtypes = [C.c_int, i, C.c_int, d, C.c_int, C.c_int, C.c_double, C.c_int,
+                                  d, d, d, i, i, d, d, d, d]
+        L.tbk_plan_create.argtypes = [C.c_int, C.POINTER(StructureDesc), C.c_int, d, d, C.c_int,
+                                      C.c_int, C.POINTER(vp)]
+        L.tbk_plan_run.argtypes = [vp, C.c_int, C.c_double, C.c_double, dp, dp, dp]
+        L.tbk_plan_launch.argtypes = [vp, C.c_int, C.c_double, C.c_double]
+        L.tbk_plan_wait.argtypes = [vp, dp, dp, dp]
+        L.tbk_plan_eigenvalues.argtypes = [vp, vp, vp]
+        L.tbk_plan_eigenvectors.argtypes = [vp, C.c_int, vp]
+        L.tbk_plan_hs.argtypes = [vp, C.c_int, C.c_int, vp, vp]
+        L.tbk_plan_nv.argtypes = [vp]
+        L.tbk_plan_npairs.argtypes = [vp]
+        L.tbk_plan_destroy.argtypes = [vp]
+        L.tbk_plan_stage_times.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), dp, ip]
+        L.tbk_plan_set_timing.argtypes = [vp, C.c_int]
+        L.tbk_zhegv.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp]
+        L.tbk_fermi.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, d, d, C.c_double,
+                                C.c_double, dp]
+        L.tbk_debug_stages.argtypes = [C.c_int, C.c_int] + [vp] * 10
+        L.tbk_debug_zgemm.argtypes = [C.c_int] * 7 + [vp, C.c_int, vp, C.c_int, vp, C.c_int]
+        L.tbk_error_string.argtypes = [C.c_int]
+        L.tbk_error_string.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = ""):
+    if rc != TBK_OK:
+        raise TbkError(rc, what)

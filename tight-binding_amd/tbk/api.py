@@ -1,0 +1,189 @@
+"""Host-side mirror of the reference's per-structure interface.
+
+``bande(structure, kpts, weights, mode, tkb, eigcut)`` plays the role of
+``bande(mode,...,efermi,earg1,earg2,...,iberr)`` (NRLtb/src/bande.f:1-4):
+it returns E_F, earg1 and earg2 with the meaning of bande.f:99-123 and
+raises :class:`TbkError` where the reference sets iberr / ioverr or stops.
+Everything below runs in lib/libtbk.so (HIP kernels on the GPU); this
+module only marshals arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import TBK_FP64, TBK_MIXED, TbkError, check, lib
+from .structures import Structure
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class HostSetup:
+    """Neighbour list and SK parameters from the library's host set-up
+    (search2.f, setpar.f, setparv.f restated bit-exactly)."""
+
+    def __init__(self, st: Structure, maxpairs: Optional[int] = None):
+        L = lib()
+        p = st.params
+        nat = st.natoms
+        maxpairs = maxpairs or max(4096, 4000 * nat)
+        self.tt = np.zeros(3 * maxpairs); self.dlv = np.zeros(3 * maxpairs)
+        self.dist = np.zeros(maxpairs); self.screen = np.zeros(maxpairs)
+        self.dscrn = np.zeros(maxpairs)
+        self.jkind = np.zeros(2 * maxpairs, np.int32); self.jatm = np.zeros(2 * maxpairs, np.int32)
+        npairs = C.c_int(0); dmin = C.c_double(0.0)
+        rc = L.tbk_search2(nat, np.ascontiguousarray(st.lat.ravel()),
+                           np.ascontiguousarray(st.posn.ravel()), st.kkind, p.kinds,
+                           np.array(st.nd, np.int32), np.array(p.dnn, float),
+                           np.array(p.screenl, float), maxpairs, C.byref(npairs), self.tt,
+                           self.dlv, self.dist, self.screen, self.dscrn, self.jkind, self.jatm,
+                           C.byref(dmin))
+        check(rc, "tbk_search2")
+        n = self.npairs = npairs.value
+        self.dismin = dmin.value
+        for nm in ("tt", "dlv"):
+            setattr(self, nm, getattr(self, nm)[:3 * n].copy())
+        for nm in ("dist", "screen", "dscrn"):
+            setattr(self, nm, getattr(self, nm)[:n].copy())
+        self.jkind = self.jkind[:2 * n].copy(); self.jatm = self.jatm[:2 * n].copy()
+        self.den = np.zeros(nat * p.kinds); self.paros = np.zeros(4 * nat)
+        self.hop = np.zeros(14 * max(n, 1)); self.over = np.zeros(14 * max(n, 1))
+        param = np.ascontiguousarray(p.param, float)
+        check(L.tbk_setpar(nat, st.kkind, p.kinds, param, p.nltype, int(p.jsover), n, self.dist,
+                           self.screen, self.jkind, self.jatm, self.den, self.paros, self.hop,
+                           self.over), "tbk_setpar")
+        self.dparos = np.zeros(4 * nat)
+        self.dhop = np.zeros(14 * max(n, 1)); self.dover = np.zeros(14 * max(n, 1))
+        check(L.tbk_setparv(nat, st.kkind, p.kinds, param, p.nltype, int(p.jsover), st.volume, n,
+                            self.dist, self.screen, self.dscrn, self.jkind, self.jatm, self.den,
+                            self.dparos, self.dhop, self.dover), "tbk_setparv")
+
+
+class Plan:
+    """A structure, its k-mesh and device-resident H/S/Z state."""
+
+    def __init__(self, st: Structure, kpts, weights, batch: int = 0,
+                 precision: int = TBK_FP64, device: int = 0, timing: bool = False):
+        L = lib()
+        p = st.params
+        self.st = st
+        self._keep = dict(
+            lat=np.ascontiguousarray(st.lat.ravel(), float),
+            posn=np.ascontiguousarray(st.posn.ravel(), float),
+            posq=np.ascontiguousarray(st.posq.ravel(), float),
+            kkind=np.ascontiguousarray(st.kkind, np.int32),
+            kbas=np.array(p.kbas, np.int32), valence=np.array(p.valence, float),
+            dnn=np.array(p.dnn, float), screenl=np.array(p.screenl, float),
+            param=np.ascontiguousarray(p.param, float))
+        k = self._keep
+        desc = _lib.StructureDesc()
+        desc.natoms = st.natoms
+        desc.lat, desc.posn, desc.posq = _dp(k["lat"]), _dp(k["posn"]), _dp(k["posq"])
+        desc.kkind = _ip(k["kkind"])
+        desc.nd = (C.c_int * 3)(*[int(x) for x in st.nd])
+        desc.kinds = p.kinds
+        desc.kbas, desc.valence = _ip(k["kbas"]), _dp(k["valence"])
+        desc.dnn, desc.screenl = _dp(k["dnn"]), _dp(k["screenl"])
+        desc.param, desc.nparam = _dp(k["param"]), len(k["param"])
+        desc.nltype, desc.jsover = p.nltype, int(p.jsover)
+        desc.thisvol, desc.zfill = st.volume, st.zfill
+        self.kpts = np.ascontiguousarray(np.asarray(kpts, float).reshape(-1, 3))
+        self.weights = np.ascontiguousarray(np.asarray(weights, float))
+        self.nk = len(self.weights)
+        h = C.c_void_p()
+        check(L.tbk_plan_create(device, C.byref(desc), self.nk, self.kpts.ravel(), self.weights,
+                                int(batch), int(precision), C.byref(h)), "tbk_plan_create")
+        self.h = h
+        self.nv = L.tbk_plan_nv(h)
+        self.npairs = L.tbk_plan_npairs(h)
+        if timing:
+            L.tbk_plan_set_timing(h, 1)
+
+    def run(self, mode: int = 4, tkb: float = 0.005, eigcut: float = 10.0) -> Dict[str, float]:
+        ef, e1, e2 = C.c_double(), C.c_double(), C.c_double()
+        check(lib().tbk_plan_run(self.h, mode, tkb, eigcut, C.byref(ef), C.byref(e1),
+                                 C.byref(e2)), "tbk_plan_run")
+        return dict(efermi=ef.value, earg1=e1.value, earg2=e2.value)
+
+    def launch(self, mode: int = 4, tkb: float = 0.005, eigcut: float = 10.0):
+        check(lib().tbk_plan_launch(self.h, mode, tkb, eigcut), "tbk_plan_launch")
+
+    def wait(self) -> Dict[str, float]:
+        ef, e1, e2 = C.c_double(), C.c_double(), C.c_double()
+        check(lib().tbk_plan_wait(self.h, C.byref(ef), C.byref(e1), C.byref(e2)), "tbk_plan_wait")
+        return dict(efermi=ef.value, earg1=e1.value, earg2=e2.value)
+
+    def eigenvalues(self):
+        e = np.zeros((self.nk, self.nv)); d = np.zeros((self.nk, self.nv))
+        check(lib().tbk_plan_eigenvalues(self.h, e.ctypes.data, d.ctypes.data))
+        return e, d
+
+    def eigenvectors(self, k: int) -> np.ndarray:
+        z = np.zeros(self.nv * self.nv, np.complex128)
+        check(lib().tbk_plan_eigenvectors(self.h, int(k), z.ctypes.data), "tbk_plan_eigenvectors")
+        return z.reshape(self.nv, self.nv).T.copy()          # [row, col]
+
+    def hs(self, k: int, deriv: bool = False):
+        n = self.nv
+        h = np.zeros(n * n, np.complex128); s = np.zeros(n * n, np.complex128)
+        check(lib().tbk_plan_hs(self.h, int(k), int(deriv), h.ctypes.data, s.ctypes.data))
+        return h.reshape(n, n).T.copy(), s.reshape(n, n).T.copy()
+
+    def stage_times(self) -> Dict[str, float]:
+        names = (C.c_char_p * 32)(); ms = np.zeros(32); n = C.c_int(0)
+        check(lib().tbk_plan_stage_times(self.h, 32, names, _dp(ms), C.byref(n)))
+        return {names[i].decode(): float(ms[i]) for i in range(n.value)}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tbk_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def bande(st: Structure, kpts, weights, mode: int = 4, tkb: float = 0.005,
+          eigcut: float = 10.0, batch: int = 0, precision: int = TBK_FP64,
+          device: int = 0) -> Dict[str, float]:
+    """One structure through the whole device pipeline (bande.f:1)."""
+    plan = Plan(st, kpts, weights, batch=batch, precision=precision, device=device)
+    try:
+        return plan.run(mode, tkb, eigcut)
+    finally:
+        plan.close()
+
+
+def zhegv(h: np.ndarray, s: np.ndarray, vectors: bool = True, device: int = 0):
+    """Device generalized eigensolve of full Hermitian h, s ([row, col])."""
+    n = h.shape[0]
+    hc = np.ascontiguousarray(h.T, np.complex128)
+    sc = np.ascontiguousarray(s.T, np.complex128)
+    w = np.zeros(n)
+    z = np.zeros(n * n, np.complex128) if vectors else None
+    check(lib().tbk_zhegv(device, n, hc.ctypes.data, sc.ctypes.data, w.ctypes.data,
+                          z.ctypes.data if vectors else None), "tbk_zhegv")
+    return w, (z.reshape(n, n).T.copy() if vectors else None)
+
+
+def fermi(eig: np.ndarray, weights, zfill: float, tkb: float = 0.005, el: float = 2.0,
+          device: int = 0) -> float:
+    """Device Fermi level (fermi.f:1) of eigenvalues eig[k, i]."""
+    eig = np.ascontiguousarray(eig, float)
+    nk, nv = eig.shape
+    ef = C.c_double()
+    check(lib().tbk_fermi(device, nv, nk, nv, el, np.ascontiguousarray(weights, float),
+                          eig.ravel(), tkb, zfill, C.byref(ef)), "tbk_fermi")
+    return ef.value

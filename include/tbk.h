@@ -148,12 +148,14 @@ int tbk_plan_nv(const tbk_plan *plan);
 int tbk_plan_npairs(const tbk_plan *plan);
 int tbk_plan_destroy(tbk_plan *plan);
 
-/* Kernel-level timing of the last run (ms per stage, HIP events on the
- * plan's stream): names[i] and ms[i] for i < *n (at most cap entries). */
+/* Stage timing (ms per stage, HIP events on the plan's stream) summed over
+ * the runs since timing was last enabled: names[i] and ms[i] for i < *n
+ * (at most cap entries). */
 int tbk_plan_stage_times(const tbk_plan *plan, int cap, const char **names,
                          double *ms, int *n);
 
-/* Enable (on = 1) or disable per-stage HIP event timing of later runs. */
+/* Enable (on = 1) or disable per-stage HIP event timing of later runs;
+ * either call clears the accumulated times. */
 int tbk_plan_set_timing(tbk_plan *plan, int on);
 
 /* ---------------------------------------------------------------------

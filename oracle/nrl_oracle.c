@@ -536,6 +536,8 @@ void orc_unpack(int n, const double *ap, double *a)
             a[2 * ((size_t)i * n + j)] = ap[2 * k];
             a[2 * ((size_t)i * n + j) + 1] = -ap[2 * k + 1];
         }
+    /* zhpgv reads only the real part of diagonal entries */
+    for (int j = 0; j < n; ++j) a[2 * ((size_t)j * n + j) + 1] = 0.0;
 }
 
 /* ------------------------------------------------------------------ */

@@ -30,8 +30,9 @@ struct GemmArgs {
 struct HetrdArgs {
     int n;
     size_t stride;
-    double2 *a, *vp, *wp, *t;      // vp, wp, t: [nb][n*32]
+    double2 *a, *vp, *wp, *t;      // vp, wp: [nb][n*32]; t: [nb][tstride]
     double *d, *e;                 // [nb][n]
+    size_t tstride;                // ceil(n/32) panels of 32x32 WY factors
 };
 
 struct SteinArgs {

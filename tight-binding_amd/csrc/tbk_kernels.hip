@@ -447,13 +447,15 @@ __global__ __launch_bounds__(256) void k_hetrd(HetrdArgs h)
     double2 *A = h.a + (size_t)kb * h.stride;
     double2 *Vp = h.vp + (size_t)kb * N * NB;
     double2 *Wp = h.wp + (size_t)kb * N * NB;
-    double2 *Tf = h.t + (size_t)kb * N * NB;
+    double2 *Tf = h.t + (size_t)kb * h.tstride;
     double *d = h.d + (size_t)kb * N;
     double *e = h.e + (size_t)kb * N;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int i0 = 0; i0 < N - 1; i0 += NB) {
         const int pw = min(NB, N - 1 - i0);
         for (int t = tid; t < N * NB; t += blockDim.x) { Vp[t] = zzero(); Wp[t] = zzero(); }
+        // the panel's WY factor is upper triangular: clear the whole block
+        for (int t = tid; t < NB * NB; t += blockDim.x) Tf[(size_t)(i0 / NB) * NB * NB + t] = zzero();
         __syncthreads();
         for (int i = 0; i < pw; ++i) {
             const int j = i0 + i;

@@ -47,6 +47,8 @@ struct Workspace {
     double *d = nullptr, *e = nullptr, *Vt = nullptr, *scratch = nullptr, *w = nullptr;
     int *info = nullptr, *sinfo = nullptr;
     size_t stride() const { return (size_t)n * n; }
+    // WY factors: one kNB x kNB block per panel, ceil(n / kNB) panels
+    size_t tstride() const { return (size_t)((n + kNB - 1) / kNB) * kNB * kNB; }
     int alloc(int n_, int b_)
     {
         n = n_; b = b_;
@@ -54,7 +56,7 @@ struct Workspace {
         int rc = TBK_OK;
         rc |= dalloc(&A, nn); rc |= dalloc(&S, nn); rc |= dalloc(&Y, nn);
         rc |= dalloc(&Z, nn); rc |= dalloc(&X, nn);
-        rc |= dalloc(&Vp, np); rc |= dalloc(&Wp, np); rc |= dalloc(&Tf, np);
+        rc |= dalloc(&Vp, np); rc |= dalloc(&Wp, np); rc |= dalloc(&Tf, tstride() * b);
         rc |= dalloc(&px, (size_t)n * b); rc |= dalloc(&py, (size_t)n * b);
         rc |= dalloc(&d, (size_t)n * b); rc |= dalloc(&e, (size_t)n * b);
         rc |= dalloc(&w, (size_t)n * b);
@@ -143,7 +145,7 @@ void solve_batch(Workspace &ws, int nb, double *w_out, hipStream_t st, StageTime
     tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 2), 1, 0, nb, st);
     tm.end();
     tm.begin("hetrd");
-    tbk::HetrdArgs h{n, ss, ws.A, ws.Vp, ws.Wp, ws.Tf, ws.d, ws.e};
+    tbk::HetrdArgs h{n, ss, ws.A, ws.Vp, ws.Wp, ws.Tf, ws.d, ws.e, ws.tstride()};
     tbk::launch_hetrd(h, nb, st);
     tm.end();
     tm.begin("stein");
@@ -152,7 +154,7 @@ void solve_batch(Workspace &ws, int nb, double *w_out, hipStream_t st, StageTime
     tm.end();
     tm.begin("backtransform");
     tbk::launch_real_to_complex_t(n, ws.Vt, ws.Y, ss, nb, st);
-    const size_t sp = (size_t)n * kNB;
+    const size_t sp = ws.tstride();
     for (int i0 = ((n - 2) / kNB) * kNB; i0 >= 0; i0 -= kNB) {
         const int pw = std::min(kNB, n - 1 - i0);
         if (pw <= 0) continue;
@@ -326,7 +328,6 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
     hipStream_t st = p->stream;
     HIP_TRY(hipMemsetAsync(p->err, 0, sizeof(int), st));
     HIP_TRY(hipMemsetAsync(p->desk, 0, sizeof(double) * (size_t)p->nk * n, st));
-    p->timer.reset();
     p->last_mode = mode;
     for (int k0 = 0; k0 < p->nk; k0 += p->batch) {
         const int nb = std::min(p->batch, p->nk - k0);
@@ -446,6 +447,8 @@ extern "C" int tbk_plan_stage_times(const tbk_plan *p, int cap, const char **nam
 extern "C" int tbk_plan_set_timing(tbk_plan *p, int on)
 {
     if (!p) return TBK_ERR_ARG;
+    p->timer.collect();
+    p->timer.reset();
     p->timer.on = on != 0;
     return TBK_OK;
 }
@@ -563,7 +566,7 @@ extern "C" int tbk_debug_stages(int device, int n, const double *h, const double
     tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 2), 1, 0, 1, st);
     (void)hipDeviceSynchronize();
     get(c, ws.A, ss * sizeof(double2));
-    tbk::HetrdArgs ha{n, ss, ws.A, ws.Vp, ws.Wp, ws.Tf, ws.d, ws.e};
+    tbk::HetrdArgs ha{n, ss, ws.A, ws.Vp, ws.Wp, ws.Tf, ws.d, ws.e, ws.tstride()};
     tbk::launch_hetrd(ha, 1, st);
     (void)hipDeviceSynchronize();
     get(refl, ws.A, ss * sizeof(double2));
@@ -574,8 +577,17 @@ extern "C" int tbk_debug_stages(int device, int n, const double *h, const double
     (void)hipDeviceSynchronize();
     get(w, ws.w, n * sizeof(double));
     get(vt, ws.Vt, ss * sizeof(double));
+    int fails = 0;
+    get(&fails, ws.sinfo, sizeof(int));
+    if (rc == TBK_OK && fails) rc = -fails;      // inverse-iteration failures
+    // full pipeline again from the inputs for Z
+    if (hipMemcpy(ws.A, h, ss * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(ws.S, s, ss * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
+        ws.release();
+        return TBK_ERR_HIP;
+    }
     StageTimer tm;
-    solve_batch(ws, 1, ws.w, st, tm);   // full pipeline again for Z
+    solve_batch(ws, 1, ws.w, st, tm);
     (void)hipDeviceSynchronize();
     get(z, ws.Z, ss * sizeof(double2));
     ws.release();

@@ -138,6 +138,10 @@ class Plan:
         check(lib().tbk_plan_hs(self.h, int(k), int(deriv), h.ctypes.data, s.ctypes.data))
         return h.reshape(n, n).T.copy(), s.reshape(n, n).T.copy()
 
+    def set_timing(self, on: bool = True):
+        """Enable per-stage HIP event timing (clears accumulated times)."""
+        check(lib().tbk_plan_set_timing(self.h, int(on)))
+
     def stage_times(self) -> Dict[str, float]:
         names = (C.c_char_p * 32)(); ms = np.zeros(32); n = C.c_int(0)
         check(lib().tbk_plan_stage_times(self.h, 32, names, _dp(ms), C.byref(n)))

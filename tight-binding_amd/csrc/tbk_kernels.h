@@ -30,7 +30,7 @@ struct GemmArgs {
 struct HetrdArgs {
     int n;
     size_t stride;
-    double2 *a, *vp, *wp, *t;      // vp, wp: [nb][n*32]; t: [nb][tstride]
+    double2 *a, *vp, *wp, *t;      // vp: [nb][n*96] panel [V|W|V]; t: [nb][tstride]
     double *d, *e;                 // [nb][n]
     size_t tstride;                // ceil(n/32) panels of 32x32 WY factors
 };
@@ -62,10 +62,13 @@ struct FermiArgs {
 };
 
 void launch_build_hs(const BuildArgs &a, int natoms, int nb, hipStream_t st);
-void launch_potrf_trtri(int n, double2 *s, size_t stride, double2 *work, int *info, int nb,
+void launch_potrf_panel(int n, double2 *s, size_t stride, int j0, int *info, int nb,
                         hipStream_t st);
+void launch_trtri_panel(int n, double2 *s, size_t stride, const double2 *wk, int j0,
+                        const int *info, int nb, hipStream_t st);
 void launch_zgemm(const GemmArgs &g, int opa, int opb, int nb, hipStream_t st);
-void launch_hetrd(const HetrdArgs &h, int nb, hipStream_t st);
+void launch_hetrd_panel(const HetrdArgs &h, int i0, int nb, hipStream_t st);
+void launch_hetrd_finish(const HetrdArgs &h, int nb, hipStream_t st);
 void launch_stein(const SteinArgs &s, int nb, hipStream_t st);
 void launch_real_to_complex_t(int n, const double *vt, double2 *y, size_t stride, int nb,
                               hipStream_t st);

@@ -6,13 +6,13 @@
 // Kernels, in pipeline order:
 //   k_build_hs      Bloch sums of precomputed Slater-Koster blocks -> H, S
 //                   (setham.f + rotate.f; sethamv.f for dH/dV, dS/dV)
-//   k_potrf_trtri   blocked Cholesky S = U^H U and U^{-1}, one WG per k
-//                   (zpptrf, lapack.f:4466; the inverse feeds the GEMMs)
+//   k_potrf_panel,  blocked Cholesky S = U^H U (zpptrf, lapack.f:4466) and
+//   k_trtri_panel   U^{-1} by panels; O(n^3) updates run as GEMMs
 //   k_zgemm         batched complex GEMM on the FP64 MFMA units
-//   k_hetrd         blocked Householder tridiagonalisation (zhptrd,
-//                   lapack.f:3218), WY factors for the back-transform
-//   k_stein         bisection eigenvalues + inverse-iteration eigenvectors
-//                   of the real tridiagonal (replaces dsterf / zsteqr)
+//   k_hetrd_panel   blocked Householder tridiagonalisation (zhptrd,
+//                   lapack.f:3218) by panels; WY factors for the back-transform
+//   k_stein         bisection eigenvalues; eigenvectors by twisted
+//                   factorisation (isolated) or inverse iteration (groups)
 //   k_coldot, k_perturb   first-order derivatives (perturb.f:87-221)
 //   k_fermi_sums    Fermi level + band-energy / pressure sums
 //                   (fermi.f:163-433, bande.f:1170-1470)
@@ -193,145 +193,84 @@ __device__ void lds_trtri(double2 (*D)[NB + 1], int w)
     }
 }
 
-__global__ __launch_bounds__(256) void k_potrf_trtri(int N, double2 *S, size_t stride,
-                                                     double2 *work, int *info)
+// Cholesky, right-looking by panels of NB columns: this kernel factors the
+// diagonal block (LDS) and forms the panel row U12 = U11^{-H} S12; the
+// caller applies the trailing update S22 -= U12^H U12 as a GEMM.  The strict
+// lower triangle of the panel's columns is cleared (the inverse stage and
+// the GEMMs read whole tiles).  info[k] = first failing column + 1.
+__global__ __launch_bounds__(256) void k_potrf_panel(int N, double2 *S, size_t stride, int j0,
+                                                     int *info)
 {
     __shared__ double2 D[NB][NB + 1];
-    __shared__ double2 Ta[NB][64 + 1];   // staging of U12 column blocks
-    __shared__ double2 Tb[NB][64 + 1];
-    const int kb = blockIdx.x;
+    const int kb = blockIdx.x, tid = threadIdx.x;
     double2 *A = S + (size_t)kb * stride;
-    double2 *Wk = work + (size_t)kb * stride;
-    const int tid = threadIdx.x;
-    // ---------------- Cholesky, right-looking, panel NB ----------------
-    for (int j0 = 0; j0 < N; j0 += NB) {
-        const int w = min(NB, N - j0);
-        for (int t = tid; t < w * w; t += blockDim.x) {
-            const int r = t % w, c = t / w;
-            D[r][c] = A[(size_t)(j0 + c) * N + j0 + r];
-        }
-        __syncthreads();
-        if (!lds_potrf(D, w)) {
-            if (tid == 0) info[kb] = j0 + 1;
-            return;
-        }
-        for (int t = tid; t < w * w; t += blockDim.x) {
-            const int r = t % w, c = t / w;
-            if (r <= c) A[(size_t)(j0 + c) * N + j0 + r] = D[r][c];
-        }
-        // U12 = U11^{-H} S12, one thread per column
-        for (int c = j0 + w + tid; c < N; c += blockDim.x) {
-            double2 x[NB];
-            for (int i = 0; i < w; ++i) x[i] = A[(size_t)c * N + j0 + i];
-            for (int i = 0; i < w; ++i) {
-                double2 v = x[i];
-                for (int l = 0; l < i; ++l) v = zsub(v, zcmul(D[l][i], x[l]));
-                x[i] = zscale(1.0 / D[i][i].x, v);
-            }
-            for (int i = 0; i < w; ++i) A[(size_t)c * N + j0 + i] = x[i];
-        }
-        __syncthreads();
-        // trailing update, upper triangle: A22 -= U12^H U12, 64x64 tiles
-        const int s0 = j0 + w;
-        const int m = N - s0;
-        const int nt = (m + 63) / 64;
-        for (int tc = 0; tc < nt; ++tc) {
-            const int cb = s0 + 64 * tc;
-            for (int t = tid; t < w * 64; t += blockDim.x) {
-                const int l = t % w, c = t / w;
-                Tb[l][c] = (cb + c < N) ? A[(size_t)(cb + c) * N + j0 + l] : zzero();
-            }
-            for (int tr = 0; tr <= tc; ++tr) {
-                const int rb = s0 + 64 * tr;
-                __syncthreads();
-                for (int t = tid; t < w * 64; t += blockDim.x) {
-                    const int l = t % w, r = t / w;
-                    Ta[l][r] = (rb + r < N) ? A[(size_t)(rb + r) * N + j0 + l] : zzero();
-                }
-                __syncthreads();
-                // thread: rows 4*tx..+3, cols ty + 16*jj
-                const int tx = tid & 15, ty = tid >> 4;
-                double2 acc[4][4];
-                for (int i = 0; i < 4; ++i) for (int jj = 0; jj < 4; ++jj) acc[i][jj] = zzero();
-                for (int l = 0; l < w; ++l) {
-                    double2 av[4], bv[4];
-                    for (int i = 0; i < 4; ++i) av[i] = Ta[l][4 * tx + i];
-                    for (int jj = 0; jj < 4; ++jj) bv[jj] = Tb[l][ty + 16 * jj];
-                    for (int i = 0; i < 4; ++i)
-                        for (int jj = 0; jj < 4; ++jj) acc[i][jj] = zadd(acc[i][jj], zcmul(av[i], bv[jj]));
-                }
-                for (int i = 0; i < 4; ++i)
-                    for (int jj = 0; jj < 4; ++jj) {
-                        const int r = rb + 4 * tx + i, c = cb + ty + 16 * jj;
-                        if (r < N && c < N && r <= c) {
-                            double2 *p = A + (size_t)c * N + r;
-                            *p = zsub(*p, acc[i][jj]);
-                        }
-                    }
-            }
-            __syncthreads();
-        }
-        __syncthreads();
+    if (j0 == 0) {
+        if (tid == 0) info[kb] = 0;
+    } else if (info[kb] != 0) {
+        return;                                      // uniform per block
     }
-    // ---------------- triangular inverse, blocked (ztrtri upper) --------
-    for (int j0 = 0; j0 < N; j0 += NB) {
-        const int w = min(NB, N - j0);
-        for (int t = tid; t < w * w; t += blockDim.x) {
-            const int r = t % w, c = t / w;
-            D[r][c] = (r <= c) ? A[(size_t)(j0 + c) * N + j0 + r] : zzero();
-        }
-        __syncthreads();
-        if (j0 > 0) {
-            // Wk[0:j0, 0:w] = Uinv[0:j0,0:j0] * U[0:j0, J]  (trmm, upper)
-            for (int i0 = 0; i0 < j0; i0 += blockDim.x) {
-                const int i = i0 + tid;
-                double2 acc[NB];
-                for (int c = 0; c < w; ++c) acc[c] = zzero();
-                for (int l0 = i0; l0 < j0; l0 += 64) {
-                    const int lw = min(64, j0 - l0);
-                    __syncthreads();
-                    for (int t = tid; t < lw * w; t += blockDim.x) {
-                        const int l = t % lw, c = t / lw;
-                        Ta[c][l] = A[(size_t)(j0 + c) * N + l0 + l];
-                    }
-                    __syncthreads();
-                    if (i < j0) {
-                        for (int l = 0; l < lw; ++l) {
-                            if (l0 + l < i) continue;
-                            const double2 u = A[(size_t)(l0 + l) * N + i];
-                            for (int c = 0; c < w; ++c) acc[c] = zadd(acc[c], zmul(u, Ta[c][l]));
-                        }
-                    }
-                }
-                if (i < j0)
-                    for (int c = 0; c < w; ++c) Wk[(size_t)c * N + i] = acc[c];
-            }
-            __syncthreads();
-            // A[0:j0, J] = -Wk * U[J,J]^{-1}, row by row
-            for (int i = tid; i < j0; i += blockDim.x) {
-                double2 x[NB];
-                for (int c = 0; c < w; ++c) {
-                    double2 v = zscale(-1.0, Wk[(size_t)c * N + i]);
-                    for (int l = 0; l < c; ++l) v = zsub(v, zmul(x[l], D[l][c]));
-                    x[c] = zscale(1.0 / D[c][c].x, v);
-                }
-                for (int c = 0; c < w; ++c) A[(size_t)(j0 + c) * N + i] = x[c];
-            }
-            __syncthreads();
-        }
-        lds_trtri(D, w);
-        for (int t = tid; t < w * w; t += blockDim.x) {
-            const int r = t % w, c = t / w;
-            A[(size_t)(j0 + c) * N + j0 + r] = (r <= c) ? D[r][c] : zzero();
-        }
-        __syncthreads();
+    const int w = min(NB, N - j0);
+    for (int t = tid; t < w * w; t += blockDim.x) {
+        const int r = t % w, c = t / w;
+        D[r][c] = A[(size_t)(j0 + c) * N + j0 + r];
     }
-    // zero the strict lower triangle (the GEMMs read whole tiles)
-    for (size_t t = tid; t < (size_t)N * N; t += blockDim.x) {
-        const int r = (int)(t % N), c = (int)(t / N);
-        if (r > c) A[t] = zzero();
+    __syncthreads();
+    if (!lds_potrf(D, w)) {
+        if (tid == 0) info[kb] = j0 + 1;
+        return;
     }
-    if (tid == 0) info[kb] = 0;
+    for (int t = tid; t < w * N; t += blockDim.x) {   // write U11, clear below
+        const int r = t % N, c = t / N;
+        if (r >= j0 && r - j0 <= c) {
+            if (r - j0 < w) A[(size_t)(j0 + c) * N + r] = D[r - j0][c];
+        } else if (r > j0 + c) {
+            A[(size_t)(j0 + c) * N + r] = zzero();
+        }
+    }
+    for (int c = j0 + w + tid; c < N; c += blockDim.x) {
+        double2 x[NB];
+        for (int i = 0; i < w; ++i) x[i] = A[(size_t)c * N + j0 + i];
+        for (int i = 0; i < w; ++i) {
+            double2 v = x[i];
+            for (int l = 0; l < i; ++l) v = zsub(v, zcmul(D[l][i], x[l]));
+            x[i] = zscale(1.0 / D[i][i].x, v);
+        }
+        for (int i = 0; i < w; ++i) A[(size_t)c * N + j0 + i] = x[i];
+    }
+}
+
+// Triangular inverse, left to right by block columns J: the caller forms
+// X = Uinv[0:j0, 0:j0] U[0:j0, J] (GEMM) into Wk; this kernel sets
+// Uinv[0:j0, J] = -X U_JJ^{-1} and inverts the diagonal block in LDS.
+__global__ __launch_bounds__(256) void k_trtri_panel(int N, double2 *S, size_t stride,
+                                                     const double2 *Wk, int j0, const int *info)
+{
+    __shared__ double2 D[NB][NB + 1];
+    const int kb = blockIdx.x, tid = threadIdx.x;
+    if (info[kb] != 0) return;
+    double2 *A = S + (size_t)kb * stride;
+    const double2 *X = Wk + (size_t)kb * stride;
+    const int w = min(NB, N - j0);
+    for (int t = tid; t < w * w; t += blockDim.x) {
+        const int r = t % w, c = t / w;
+        D[r][c] = (r <= c) ? A[(size_t)(j0 + c) * N + j0 + r] : zzero();
+    }
+    __syncthreads();
+    for (int i = tid; i < j0; i += blockDim.x) {
+        double2 x[NB];
+        for (int c = 0; c < w; ++c) {
+            double2 v = zscale(-1.0, X[(size_t)c * N + i]);
+            for (int l = 0; l < c; ++l) v = zsub(v, zmul(x[l], D[l][c]));
+            x[c] = zscale(1.0 / D[c][c].x, v);
+        }
+        for (int c = 0; c < w; ++c) A[(size_t)(j0 + c) * N + i] = x[c];
+    }
+    __syncthreads();
+    lds_trtri(D, w);
+    for (int t = tid; t < w * w; t += blockDim.x) {
+        const int r = t % w, c = t / w;
+        A[(size_t)(j0 + c) * N + j0 + r] = (r <= c) ? D[r][c] : zzero();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -427,198 +366,177 @@ __global__ __launch_bounds__(256) void k_zgemm(GemmArgs g)
 }
 
 // ---------------------------------------------------------------------------
-// Householder tridiagonalisation (lower, blocked), one workgroup per k
+// Householder tridiagonalisation (lower, blocked): one panel per launch
 // ---------------------------------------------------------------------------
-// On exit: d[N], e[N-1] (real tridiagonal), A holds the reflectors
-// v_j (A[j+1, j] = 1, A[r, j] = v_r below, zero at and above the diagonal),
-// T holds the upper triangular WY factor of every panel (zlarft forward
-// columnwise): the panel's Q_p = I - V_p T_p V_p^H.
-__global__ __launch_bounds__(256) void k_hetrd(HetrdArgs h)
+// k_hetrd_panel reduces columns i0 .. i0+pw-1 of every matrix of the batch
+// (one workgroup of 8 waves per k), following zlatrd (delayed update of the
+// trailing matrix through V and W).  It leaves
+//   P = [V | W | V]   (n x 96, column-major) for the trailing update
+//       A[s:, s:] -= V W^H + W V^H, which the caller runs as one GEMM,
+//   the reflectors in A (A[j+1, j] = 1, v below, zero at and above j),
+//   the panel's upper triangular WY factor T (Q_p = I - V T V^H),
+//   d[j], e[j] (the real tridiagonal).
+// The symmetric matrix-vector product y = A_t v is evaluated as dot products
+// y_r = sum_c conj(A[c, r]) v_c over contiguous columns (A is stored in full).
+constexpr int HT = 512;            // threads of the panel kernel
+constexpr int HW = HT / 64;        // waves
+
+__device__ __forceinline__ double2 wave_sum2(double2 v)
 {
-    __shared__ double red[16];
-    __shared__ double2 u1[NB], u2[NB];
-    __shared__ double2 tauS;
-    __shared__ double betaS;
-    __shared__ double2 yacc[4][64];
-    __shared__ double2 vloc[1280];       // v of the current column (N <= 1280)
-    __shared__ double2 cw[NB][16], cv[NB][16];
+    for (int off = 32; off > 0; off >>= 1) {
+        v.x += __shfl_xor(v.x, off, 64);
+        v.y += __shfl_xor(v.y, off, 64);
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(HT) void k_hetrd_panel(HetrdArgs h, int i0)
+{
+    __shared__ double2 vloc[1280];     // reflector of the current column
+    __shared__ double2 ybuf[1280];     // A_t v
+    __shared__ double2 udot[2 * NB];   // W^H v (0..31) and V^H v (32..63)
+    __shared__ double2 uw[NB], uv[NB]; // conj(W[j, l]), conj(V[j, l])
+    __shared__ double red[2 * HW];
+    __shared__ double2 scal[2];        // tau, (beta, 0)
     const int N = h.n;
     const int kb = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     double2 *A = h.a + (size_t)kb * h.stride;
-    double2 *Vp = h.vp + (size_t)kb * N * NB;
-    double2 *Wp = h.wp + (size_t)kb * N * NB;
-    double2 *Tf = h.t + (size_t)kb * h.tstride;
+    double2 *P = h.vp + (size_t)kb * N * 96;
+    double2 *Tp = h.t + (size_t)kb * h.tstride + (size_t)(i0 / NB) * NB * NB;
     double *d = h.d + (size_t)kb * N;
     double *e = h.e + (size_t)kb * N;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int i0 = 0; i0 < N - 1; i0 += NB) {
-        const int pw = min(NB, N - 1 - i0);
-        for (int t = tid; t < N * NB; t += blockDim.x) { Vp[t] = zzero(); Wp[t] = zzero(); }
-        // the panel's WY factor is upper triangular: clear the whole block
-        for (int t = tid; t < NB * NB; t += blockDim.x) Tf[(size_t)(i0 / NB) * NB * NB + t] = zzero();
+    const int pw = min(NB, N - 1 - i0);
+    double2 *V = P, *W = P + (size_t)NB * N, *V2 = P + (size_t)2 * NB * N;
+    for (int t = tid; t < N * 96; t += HT) P[t] = zzero();
+    for (int t = tid; t < NB * NB; t += HT) Tp[t] = zzero();
+    __syncthreads();
+    for (int i = 0; i < pw; ++i) {
+        const int j = i0 + i, t0 = j + 1, m = N - t0;
+        // (1) column j brought up to date: a_r -= V[r,:] conj(W[j,:]) + W[r,:] conj(V[j,:])
+        if (tid < i) { uw[tid] = zconj(W[(size_t)tid * N + j]); uv[tid] = zconj(V[(size_t)tid * N + j]); }
         __syncthreads();
-        for (int i = 0; i < pw; ++i) {
-            const int j = i0 + i;
-            // (1) bring column j up to date with the panel's earlier reflectors
-            if (i > 0) {
-                for (int l = tid; l < i; l += blockDim.x) {
-                    u1[l] = zconj(Wp[(size_t)l * N + j]);
-                    u2[l] = zconj(Vp[(size_t)l * N + j]);
-                }
-                __syncthreads();
-                for (int r = j + tid; r < N; r += blockDim.x) {
-                    double2 a = A[(size_t)j * N + r];
-                    for (int l = 0; l < i; ++l) {
-                        a = zsub(a, zmul(Vp[(size_t)l * N + r], u1[l]));
-                        a = zsub(a, zmul(Wp[(size_t)l * N + r], u2[l]));
-                    }
-                    if (r == j) a.y = 0.0;
-                    A[(size_t)j * N + r] = a;
-                }
-                __syncthreads();
+        double xs = 0.0;
+        for (int r = j + tid; r < N; r += HT) {
+            double2 a = A[(size_t)j * N + r];
+            for (int l = 0; l < i; ++l) {
+                a = zsub(a, zmul(V[(size_t)l * N + r], uw[l]));
+                a = zsub(a, zmul(W[(size_t)l * N + r], uv[l]));
             }
-            // (2) reflector for x = A[j+1:N, j] (zlarfg)
+            if (r == j) a.y = 0.0;
+            vloc[r] = a;
+            if (r >= j + 2) xs += a.x * a.x + a.y * a.y;
+        }
+        for (int off = 32; off > 0; off >>= 1) xs += __shfl_xor(xs, off, 64);
+        if (lane == 0) red[wv] = xs;
+        __syncthreads();
+        // (2) reflector (zlarfg) for x = a[j+1:]
+        if (tid == 0) {
             double xn = 0.0;
-            for (int r = j + 2 + tid; r < N; r += blockDim.x) {
-                const double2 x = A[(size_t)j * N + r];
-                xn += x.x * x.x + x.y * x.y;
+            for (int q = 0; q < HW; ++q) xn += red[q];
+            const double2 al = vloc[j + 1];
+            double2 tau = zzero();
+            double beta = al.x;
+            if (!(xn == 0.0 && al.y == 0.0)) {
+                beta = -copysign(sqrt(al.x * al.x + al.y * al.y + xn), al.x);
+                tau = make_double2((beta - al.x) / beta, -al.y / beta);
             }
-            xn = block_sum(xn, red);
-            if (tid == 0) {
-                const double2 al = A[(size_t)j * N + j + 1];
-                double2 tau = zzero();
-                double beta = al.x;
-                if (!(xn == 0.0 && al.y == 0.0)) {
-                    beta = -copysign(sqrt(al.x * al.x + al.y * al.y + xn), al.x);
-                    tau = make_double2((beta - al.x) / beta, -al.y / beta);
-                }
-                tauS = tau; betaS = beta;
-                d[j] = A[(size_t)j * N + j].x;
-                e[j] = beta;
-            }
-            __syncthreads();
-            const double2 tau = tauS;
-            const double beta = betaS;
-            {
-                const double2 al = A[(size_t)j * N + j + 1];
-                const double2 dv = make_double2(al.x - beta, al.y);
-                const double den = dv.x * dv.x + dv.y * dv.y;
-                const double2 sc = (tau.x != 0.0 || tau.y != 0.0) ? make_double2(dv.x / den, -dv.y / den)
-                                                                  : make_double2(1.0, 0.0);
-                __syncthreads();
-                for (int r = tid; r < N; r += blockDim.x) {
-                    double2 v;
-                    if (r <= j) v = zzero();
-                    else if (r == j + 1) v = make_double2(1.0, 0.0);
-                    else v = zmul(A[(size_t)j * N + r], sc);
-                    A[(size_t)j * N + r] = v;          // reflector stored in place
-                    Vp[(size_t)i * N + r] = v;
-                    vloc[r] = v;
-                }
-            }
-            __syncthreads();
-            // (3) y = A_t v over the trailing block (rows/cols j+1..N-1):
-            //     wave wv takes columns c = j+1+wv, +4, ...; lanes take rows
-            const int t0 = j + 1;
-            for (int rb = t0; rb < N; rb += 64) {
-                const int r = rb + lane;
-                double2 acc = zzero();
-                if (r < N)
-                    for (int c = t0 + wv; c < N; c += 4) acc = zadd(acc, zmul(A[(size_t)c * N + r], vloc[c]));
-                yacc[wv][lane] = acc;
-                __syncthreads();
-                if (wv == 0 && r < N) {
-                    double2 y = zadd(zadd(yacc[0][lane], yacc[1][lane]), zadd(yacc[2][lane], yacc[3][lane]));
-                    Wp[(size_t)i * N + r] = y;          // raw A_t v, corrected below
-                }
-                __syncthreads();
-            }
-            // u1 = W^H v, u2 = V^H v over rows j+1..N-1 (l < i)
-            for (int l = 0; l < i; ++l) {
-                double s1r = 0, s1i = 0, s2r = 0, s2i = 0;
-                for (int r = t0 + tid; r < N; r += blockDim.x) {
-                    const double2 v = vloc[r];
-                    const double2 a1 = zcmul(Wp[(size_t)l * N + r], v);
-                    const double2 a2 = zcmul(Vp[(size_t)l * N + r], v);
-                    s1r += a1.x; s1i += a1.y; s2r += a2.x; s2i += a2.y;
-                }
-                s1r = block_sum(s1r, red); s1i = block_sum(s1i, red);
-                s2r = block_sum(s2r, red); s2i = block_sum(s2i, red);
-                if (tid == 0) { u1[l] = make_double2(s1r, s1i); u2[l] = make_double2(s2r, s2i); }
-            }
-            __syncthreads();
-            // w = tau (y - V u1 - W u2); then w += alpha v, alpha = -tau/2 (w^H v)
-            double pr = 0.0, pi = 0.0;
-            for (int r = t0 + tid; r < N; r += blockDim.x) {
-                double2 y = Wp[(size_t)i * N + r];
-                for (int l = 0; l < i; ++l) {
-                    y = zsub(y, zmul(Vp[(size_t)l * N + r], u1[l]));
-                    y = zsub(y, zmul(Wp[(size_t)l * N + r], u2[l]));
-                }
-                const double2 w = zmul(tau, y);
-                Wp[(size_t)i * N + r] = w;
-                const double2 q = zcmul(w, vloc[r]);
-                pr += q.x; pi += q.y;
-            }
-            pr = block_sum(pr, red); pi = block_sum(pi, red);
-            const double2 alpha = zscale(-0.5, zmul(tau, make_double2(pr, pi)));
-            for (int r = t0 + tid; r < N; r += blockDim.x)
-                Wp[(size_t)i * N + r] = zadd(Wp[(size_t)i * N + r], zmul(alpha, vloc[r]));
-            // WY factor column i: T[0:i, i] = -tau T[0:i,0:i] (V^H v), T[i][i] = tau
-            for (int l = 0; l < i; ++l) {
-                double sr = 0.0, si = 0.0;
-                for (int r = t0 + tid; r < N; r += blockDim.x) {
-                    const double2 q = zcmul(Vp[(size_t)l * N + r], vloc[r]);
-                    sr += q.x; si += q.y;
-                }
-                sr = block_sum(sr, red); si = block_sum(si, red);
-                if (tid == 0) u2[l] = make_double2(sr, si);
-            }
-            __syncthreads();
-            double2 *Tp = Tf + (size_t)(i0 / NB) * NB * NB;   // NB x NB, column-major
-            if (tid < i) {
-                double2 acc = zzero();
-                for (int l = tid; l < i; ++l) acc = zadd(acc, zmul(Tp[(size_t)l * NB + tid], u2[l]));
-                Tp[(size_t)i * NB + tid] = zmul(make_double2(-tau.x, -tau.y), acc);
-            }
-            if (tid == 0) Tp[(size_t)i * NB + i] = tau;
-            __syncthreads();
+            scal[0] = tau; scal[1] = make_double2(beta, 0.0);
+            d[j] = vloc[j].x;
+            e[j] = beta;
         }
-        // trailing her2k: A[s:, s:] -= V W^H + W V^H over rows/cols s..N-1
-        const int s = i0 + pw;
-        for (int cb = s; cb < N; cb += 16) {
-            const int nc = min(16, N - cb);
+        __syncthreads();
+        const double2 tau = scal[0];
+        const double beta = scal[1].x;
+        {
+            const double2 al = vloc[j + 1];
+            const double2 dv = make_double2(al.x - beta, al.y);
+            const double den = dv.x * dv.x + dv.y * dv.y;
+            const bool refl = tau.x != 0.0 || tau.y != 0.0;
+            const double2 sc = refl ? make_double2(dv.x / den, -dv.y / den) : make_double2(1.0, 0.0);
             __syncthreads();
-            for (int t = tid; t < pw * 16; t += blockDim.x) {
-                const int l = t / 16, c = t % 16;
-                cw[l][c] = (c < nc) ? zconj(Wp[(size_t)l * N + cb + c]) : zzero();
-                cv[l][c] = (c < nc) ? zconj(Vp[(size_t)l * N + cb + c]) : zzero();
-            }
-            __syncthreads();
-            for (int r = s + tid; r < N; r += blockDim.x) {
-                double2 acc[16];
-                for (int c = 0; c < 16; ++c) acc[c] = zzero();
-                for (int l = 0; l < pw; ++l) {
-                    const double2 vr = Vp[(size_t)l * N + r], wr = Wp[(size_t)l * N + r];
-                    for (int c = 0; c < 16; ++c) acc[c] = zadd(acc[c], zadd(zmul(vr, cw[l][c]), zmul(wr, cv[l][c])));
-                }
-                for (int c = 0; c < nc; ++c) {
-                    double2 *p = A + (size_t)(cb + c) * N + r;
-                    double2 v = zsub(*p, acc[c]);
-                    if (r == cb + c) v.y = 0.0;
-                    *p = v;
-                }
+            for (int r = tid; r < N; r += HT) {
+                double2 v;
+                if (r <= j) v = zzero();
+                else if (r == j + 1) v = make_double2(1.0, 0.0);
+                else v = zmul(vloc[r], sc);
+                vloc[r] = v;
+                A[(size_t)j * N + r] = v;
+                V[(size_t)i * N + r] = v;
+                V2[(size_t)i * N + r] = v;
             }
         }
+        __syncthreads();
+        // (3) dot products, one per wave task: rows r (y_r = col_r^H v) and
+        //     the 2i panel dots W[:,l]^H v, V[:,l]^H v over rows >= t0
+        const int ntask = m + 2 * i;
+        for (int task = wv; task < ntask; task += HW) {
+            const double2 *col;
+            if (task < m) col = A + (size_t)(t0 + task) * N;
+            else if (task < m + i) col = W + (size_t)(task - m) * N;
+            else col = V + (size_t)(task - m - i) * N;
+            double2 acc = zzero(), acc2 = zzero();
+            int c = t0 + lane;
+            for (; c + 64 < N; c += 128) {
+                const double2 a0 = col[c], a1 = col[c + 64];
+                acc = zadd(acc, zcmul(a0, vloc[c]));
+                acc2 = zadd(acc2, zcmul(a1, vloc[c + 64]));
+            }
+            if (c < N) acc = zadd(acc, zcmul(col[c], vloc[c]));
+            acc = wave_sum2(zadd(acc, acc2));
+            if (lane == 0) {
+                if (task < m) ybuf[t0 + task] = acc;
+                else udot[task - m + (task < m + i ? 0 : NB - i)] = acc;
+            }
+        }
+        __syncthreads();
+        // (4) w = tau (y - V u1 - W u2), u1 = W^H v, u2 = V^H v; then the
+        //     correction w += alpha v with alpha = -tau/2 (w^H v)
+        double pr = 0.0, pi = 0.0;
+        for (int r = t0 + tid; r < N; r += HT) {
+            double2 y = ybuf[r];
+            for (int l = 0; l < i; ++l) {
+                y = zsub(y, zmul(V[(size_t)l * N + r], udot[l]));
+                y = zsub(y, zmul(W[(size_t)l * N + r], udot[NB + l]));
+            }
+            const double2 w = zmul(tau, y);
+            ybuf[r] = w;
+            const double2 q = zcmul(w, vloc[r]);
+            pr += q.x; pi += q.y;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            pr += __shfl_xor(pr, off, 64);
+            pi += __shfl_xor(pi, off, 64);
+        }
+        __syncthreads();
+        if (lane == 0) { red[wv] = pr; red[HW + wv] = pi; }
+        __syncthreads();
+        double sr = 0.0, si = 0.0;
+        for (int q = 0; q < HW; ++q) { sr += red[q]; si += red[HW + q]; }
+        const double2 alpha = zscale(-0.5, zmul(tau, make_double2(sr, si)));
+        for (int r = t0 + tid; r < N; r += HT) W[(size_t)i * N + r] = zadd(ybuf[r], zmul(alpha, vloc[r]));
+        // (5) WY factor: T[0:i, i] = -tau T[0:i, 0:i] (V^H v), T[i, i] = tau
+        if (tid < i) {
+            double2 acc = zzero();
+            for (int l = tid; l < i; ++l) acc = zadd(acc, zmul(Tp[(size_t)l * NB + tid], udot[NB + l]));
+            Tp[(size_t)i * NB + tid] = zmul(make_double2(-tau.x, -tau.y), acc);
+        }
+        if (tid == 0) Tp[(size_t)i * NB + i] = tau;
         __syncthreads();
     }
-    if (tid == 0) {
-        d[N - 1] = A[(size_t)(N - 1) * N + N - 1].x;
-        e[N - 1] = 0.0;
+}
+
+// After the last panel's trailing update: d[n-1] and the empty last column.
+__global__ void k_hetrd_finish(HetrdArgs h)
+{
+    const int N = h.n, kb = blockIdx.x;
+    double2 *A = h.a + (size_t)kb * h.stride;
+    if (threadIdx.x == 0) {
+        h.d[(size_t)kb * N + N - 1] = A[(size_t)(N - 1) * N + N - 1].x;
+        h.e[(size_t)kb * N + N - 1] = 0.0;
     }
     __syncthreads();
-    // the last column holds no reflector
-    for (int r = tid; r < N; r += blockDim.x) A[(size_t)(N - 1) * N + r] = zzero();
+    for (int r = threadIdx.x; r < N; r += blockDim.x) A[(size_t)(N - 1) * N + r] = zzero();
 }
 
 // ---------------------------------------------------------------------------
@@ -725,6 +643,55 @@ __global__ __launch_bounds__(256) void k_stein(SteinArgs s)
         const int j0 = cstart[cl], j1 = cstart[cl + 1];
         double xjm = 0.0;
         for (int j = j0; j < j1; ++j) {
+            if (j == j0 || w[j] - w[j - 1] > 1e-10 * onenrm) {
+                // Twisted factorisation of T - lambda I: L D L^T from the
+                // top (stationary qd), U D U^T from the bottom; the twist
+                // index r minimises |gamma_r| and the vector follows from
+                // z_r = 1 by the two bidiagonal recurrences.
+                const double lam = w[j];
+                double *wl = wa, *wp = wb, *wu = wc;     // L, D+, U per thread
+                double dp = d[0] - lam;
+                if (fabs(dp) < pivmin) dp = -pivmin;
+                AT(wp, 0) = dp;
+                for (int i = 0; i < N - 1; ++i) {
+                    const double l = e[i] / dp;
+                    AT(wl, i) = l;
+                    dp = d[i + 1] - lam - l * e[i];
+                    if (fabs(dp) < pivmin) dp = -pivmin;
+                    AT(wp, i + 1) = dp;
+                }
+                double dm = d[N - 1] - lam;
+                if (fabs(dm) < pivmin) dm = -pivmin;
+                double gbest = fabs(AT(wp, N - 1) + dm - (d[N - 1] - lam));
+                int r = N - 1;
+                for (int i = N - 2; i >= 0; --i) {
+                    const double u = e[i] / dm;
+                    AT(wu, i) = u;
+                    dm = d[i] - lam - u * e[i];
+                    if (fabs(dm) < pivmin) dm = -pivmin;
+                    const double g = fabs(AT(wp, i) + dm - (d[i] - lam));
+                    if (g < gbest) { gbest = g; r = i; }
+                }
+                double z = 1.0;
+                Vt[(size_t)r * N + j] = 1.0;
+                for (int i = r - 1; i >= 0; --i) { z = -AT(wl, i) * z; Vt[(size_t)i * N + j] = z; }
+                z = 1.0;
+                for (int i = r; i < N - 1; ++i) { z = -AT(wu, i) * z; Vt[(size_t)(i + 1) * N + j] = z; }
+                // orthogonalise against the group's earlier vectors
+                for (int jp = j0; jp < j; ++jp) {
+                    double dot = 0.0;
+                    for (int i = 0; i < N; ++i) dot += Vt[(size_t)i * N + j] * Vt[(size_t)i * N + jp];
+                    for (int i = 0; i < N; ++i) Vt[(size_t)i * N + j] -= dot * Vt[(size_t)i * N + jp];
+                }
+                double ss = 0.0;
+                for (int i = 0; i < N; ++i) ss += Vt[(size_t)i * N + j] * Vt[(size_t)i * N + j];
+                const double scl = 1.0 / sqrt(ss);
+                for (int i = 0; i < N; ++i) Vt[(size_t)i * N + j] *= scl;
+                xjm = lam;
+                continue;
+            }
+            // near-exact degeneracy: inverse iteration from a distinct start
+            // vector with the group's reorthogonalisation (dstein)
             double xj = w[j];
             if (j > j0) {
                 const double pertol = 10.0 * fabs(eps * xj);
@@ -737,11 +704,8 @@ __global__ __launch_bounds__(256) void k_stein(SteinArgs s)
                 AT(wc, i) = e[i];
                 AT(wd, i) = 0.0;
             }
-            unsigned int piv_lo = 0;   // pivot flags packed per 32 rows
-            (void)piv_lo;
             double scale1 = fabs(AT(wa, 0)) + (N > 1 ? fabs(AT(wb, 0)) : 0.0);
-            // pivot flags stored as sign of an extra encoding in wd's slot n-1..:
-            // use wy temporarily as flag storage (0 / 1)
+            // pivot flags (0 / 1) in wy
             for (int k = 0; k < N - 1; ++k) {
                 double scale2 = fabs(AT(wc, k)) + fabs(AT(wa, k + 1));
                 if (k < N - 2) scale2 += fabs(AT(wb, k + 1));
@@ -776,10 +740,7 @@ __global__ __launch_bounds__(256) void k_stein(SteinArgs s)
                     }
                 }
             }
-            // pivot flags now in wy[0..N-2]; keep them in registers-free form:
-            // move into the sign bit of a spare column (wd keeps d2); we copy
-            // flags to Vt-independent storage: reuse wy until the solve, then
-            // the iterate lives in the output column of Vt.
+            // the iterate lives in the output column of Vt
             double tolp = fabs(AT(wa, 0));
             if (N > 1) tolp = fmax(tolp, fmax(fabs(AT(wa, 1)), fabs(AT(wb, 0))));
             for (int k = 2; k < N; ++k)
@@ -1189,10 +1150,15 @@ void launch_build_hs(const BuildArgs &a, int natoms, int nb, hipStream_t st)
 {
     hipLaunchKernelGGL(k_build_hs, dim3(natoms, nb), dim3(256), 0, st, a);
 }
-void launch_potrf_trtri(int N, double2 *S, size_t stride, double2 *work, int *info, int nb,
+void launch_potrf_panel(int n, double2 *s, size_t stride, int j0, int *info, int nb,
                         hipStream_t st)
 {
-    hipLaunchKernelGGL(k_potrf_trtri, dim3(nb), dim3(256), 0, st, N, S, stride, work, info);
+    hipLaunchKernelGGL(k_potrf_panel, dim3(nb), dim3(256), 0, st, n, s, stride, j0, info);
+}
+void launch_trtri_panel(int n, double2 *s, size_t stride, const double2 *wk, int j0,
+                        const int *info, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_trtri_panel, dim3(nb), dim3(256), 0, st, n, s, stride, wk, j0, info);
 }
 void launch_zgemm(const GemmArgs &g, int opa, int opb, int nb, hipStream_t st)
 {
@@ -1202,9 +1168,13 @@ void launch_zgemm(const GemmArgs &g, int opa, int opb, int nb, hipStream_t st)
     else if (opa == 0 && opb == 1) hipLaunchKernelGGL((k_zgemm<0, 1>), grid, dim3(256), 0, st, g);
     else hipLaunchKernelGGL((k_zgemm<1, 1>), grid, dim3(256), 0, st, g);
 }
-void launch_hetrd(const HetrdArgs &h, int nb, hipStream_t st)
+void launch_hetrd_panel(const HetrdArgs &h, int i0, int nb, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_hetrd, dim3(nb), dim3(256), 0, st, h);
+    hipLaunchKernelGGL(k_hetrd_panel, dim3(nb), dim3(HT), 0, st, h, i0);
+}
+void launch_hetrd_finish(const HetrdArgs &h, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_hetrd_finish, dim3(nb), dim3(256), 0, st, h);
 }
 void launch_stein(const SteinArgs &s, int nb, hipStream_t st)
 {

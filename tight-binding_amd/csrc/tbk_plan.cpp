@@ -56,7 +56,7 @@ struct Workspace {
         int rc = TBK_OK;
         rc |= dalloc(&A, nn); rc |= dalloc(&S, nn); rc |= dalloc(&Y, nn);
         rc |= dalloc(&Z, nn); rc |= dalloc(&X, nn);
-        rc |= dalloc(&Vp, np); rc |= dalloc(&Wp, np); rc |= dalloc(&Tf, tstride() * b);
+        rc |= dalloc(&Vp, (size_t)n * 96 * b); rc |= dalloc(&Wp, np); rc |= dalloc(&Tf, tstride() * b);
         rc |= dalloc(&px, (size_t)n * b); rc |= dalloc(&py, (size_t)n * b);
         rc |= dalloc(&d, (size_t)n * b); rc |= dalloc(&e, (size_t)n * b);
         rc |= dalloc(&w, (size_t)n * b);
@@ -130,6 +130,49 @@ struct StageTimer {
     void reset() { for (auto &m : ms) m = 0.0; }
 };
 
+// Blocked tridiagonalisation: per panel one k_hetrd_panel launch and one
+// GEMM for the trailing update A[s:, s:] -= [V W] [W V]^H.
+void hetrd(Workspace &ws, int nb, hipStream_t st)
+{
+    const int n = ws.n;
+    const size_t ss = ws.stride(), sp = (size_t)n * 96;
+    tbk::HetrdArgs h{n, ss, ws.A, ws.Vp, ws.Wp, ws.Tf, ws.d, ws.e, ws.tstride()};
+    for (int i0 = 0; i0 < n - 1; i0 += kNB) {
+        const int pw = std::min(kNB, n - 1 - i0);
+        tbk::launch_hetrd_panel(h, i0, nb, st);
+        const int s = i0 + pw, m = n - s;
+        if (m > 0)
+            tbk::launch_zgemm(gemm(m, m, 2 * kNB, ws.Vp + s, n, sp, ws.Vp + (size_t)kNB * n + s, n, sp,
+                                   ws.A + (size_t)s * n + s, n, ss, 0, double2{-1.0, 0.0},
+                                   double2{1.0, 0.0}), 0, 1, nb, st);
+    }
+    tbk::launch_hetrd_finish(h, nb, st);
+}
+
+// S = U^H U and U^{-1} in place (lower triangle cleared): panel kernels for
+// the diagonal blocks, GEMMs for the O(n^3) updates.
+void potrf_trtri(Workspace &ws, int nb, hipStream_t st)
+{
+    const int n = ws.n;
+    const size_t ss = ws.stride();
+    for (int j0 = 0; j0 < n; j0 += kNB) {
+        const int w = std::min(kNB, n - j0), s0 = j0 + w, m = n - s0;
+        tbk::launch_potrf_panel(n, ws.S, ss, j0, ws.info, nb, st);
+        if (m > 0) {   // S22 -= U12^H U12
+            const double2 *u12 = ws.S + (size_t)s0 * n + j0;
+            tbk::launch_zgemm(gemm(m, m, w, u12, n, ss, u12, n, ss, ws.S + (size_t)s0 * n + s0, n, ss,
+                                   0, double2{-1.0, 0.0}, double2{1.0, 0.0}), 1, 0, nb, st);
+        }
+    }
+    for (int j0 = 0; j0 < n; j0 += kNB) {
+        const int w = std::min(kNB, n - j0);
+        if (j0 > 0)    // Y = Uinv[0:j0, 0:j0] U[0:j0, J]
+            tbk::launch_zgemm(gemm(j0, w, j0, ws.S, n, ss, ws.S + (size_t)j0 * n, n, ss, ws.Y, n, ss, 1),
+                              0, 0, nb, st);
+        tbk::launch_trtri_panel(n, ws.S, ss, ws.Y, j0, ws.info, nb, st);
+    }
+}
+
 // Stages 2-9 of the pipeline on ws.A = H, ws.S = S for nb k-points:
 // eigenvalues into w_out[nb*n], eigenvectors into ws.Z.
 void solve_batch(Workspace &ws, int nb, double *w_out, hipStream_t st, StageTimer &tm)
@@ -137,7 +180,7 @@ void solve_batch(Workspace &ws, int nb, double *w_out, hipStream_t st, StageTime
     const int n = ws.n;
     const size_t ss = ws.stride();
     tm.begin("potrf_trtri");
-    tbk::launch_potrf_trtri(n, ws.S, ss, ws.Y, ws.info, nb, st);
+    potrf_trtri(ws, nb, st);
     tm.end();
     tm.begin("reduce_gemm");
     // Y = H Uinv (Uinv upper) ; A = Uinv^H Y
@@ -145,8 +188,7 @@ void solve_batch(Workspace &ws, int nb, double *w_out, hipStream_t st, StageTime
     tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 2), 1, 0, nb, st);
     tm.end();
     tm.begin("hetrd");
-    tbk::HetrdArgs h{n, ss, ws.A, ws.Vp, ws.Wp, ws.Tf, ws.d, ws.e, ws.tstride()};
-    tbk::launch_hetrd(h, nb, st);
+    hetrd(ws, nb, st);
     tm.end();
     tm.begin("stein");
     tbk::SteinArgs sa{n, 1e-4, ws.d, ws.e, w_out, ws.Vt, ws.scratch, ws.sinfo};
@@ -559,15 +601,14 @@ extern "C" int tbk_debug_stages(int device, int n, const double *h, const double
         ws.release();
         return TBK_ERR_HIP;
     }
-    tbk::launch_potrf_trtri(n, ws.S, ss, ws.Y, ws.info, 1, st);
+    potrf_trtri(ws, 1, st);
     (void)hipDeviceSynchronize();
     get(uinv, ws.S, ss * sizeof(double2));
     tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.S, n, ss, ws.Y, n, ss, 3), 0, 0, 1, st);
     tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 2), 1, 0, 1, st);
     (void)hipDeviceSynchronize();
     get(c, ws.A, ss * sizeof(double2));
-    tbk::HetrdArgs ha{n, ss, ws.A, ws.Vp, ws.Wp, ws.Tf, ws.d, ws.e, ws.tstride()};
-    tbk::launch_hetrd(ha, 1, st);
+    hetrd(ws, 1, st);
     (void)hipDeviceSynchronize();
     get(refl, ws.A, ss * sizeof(double2));
     get(d, ws.d, n * sizeof(double));

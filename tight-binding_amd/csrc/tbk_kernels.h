@@ -8,13 +8,13 @@
 namespace tbk {
 
 struct BuildArgs {
-    int n, natoms, onsite_s;
+    int n, natoms, onsite_s, npairs;
     size_t stride;                 // elements between k-point slabs
     const int *row_atom, *row_orb, *ksk, *nbas, *blk_ptr;
     const double *dvec;            // [3*npairs] dlv - posq_j + posq_i
     const double *eh, *es;         // [81*npairs] SK blocks e(a,b), row-major
     const double *onsite;          // [4*natoms] paros or dparos
-    const double *kpts;            // [3*nb]
+    const double2 *ph;             // [nb][npairs] Bloch phases
     double2 *h, *s;
 };
 
@@ -61,6 +61,8 @@ struct FermiArgs {
     double *out;                   // efermi, earg1, earg2, esum, entropy, iters
 };
 
+void launch_phases(int npairs, const double *kpts, const double *dvec, double2 *ph, int nb,
+                   hipStream_t st);
 void launch_build_hs(const BuildArgs &a, int natoms, int nb, hipStream_t st);
 void launch_potrf_panel(int n, double2 *s, size_t stride, int j0, int *info, int nb,
                         hipStream_t st);

@@ -18,6 +18,7 @@
 //                   (fermi.f:163-433, bande.f:1170-1470)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -62,11 +63,30 @@ __device__ double block_sum(double v, double *red)
 // ---------------------------------------------------------------------------
 // H(k), S(k) build
 // ---------------------------------------------------------------------------
-// One workgroup per (column atom ja, k-point).  Thread r owns matrix row r
-// and the kb(ja) columns of atom ja.  For rows of atoms ia <= ja the block
-// is the Bloch sum over the pairs (ia, ja) of the reference's pair list, in
-// list order, of e(a,b) * exp(2 pi i k.d) (rotate.f:144-149, 260-326); rows
-// of atoms ia > ja take the conjugate of block (ja, ia) (Hermitian).  The
+// Bloch phases exp(2 pi i k.d_p) for every (k-point, pair), d_p = dlv -
+// posq_j + posq_i (rotate.f:144-149), written once per batch.
+__global__ __launch_bounds__(256) void k_phases(int npairs, const double *kpts, const double *dvec,
+                                                double2 *ph)
+{
+#pragma clang fp contract(off)
+    const int kb = blockIdx.y;
+    const double *ak = kpts + 3 * (size_t)kb;
+    const double k0 = ak[0], k1 = ak[1], k2 = ak[2];
+    const double tpi = 6.283185307179586;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += gridDim.x * blockDim.x) {
+        const double *d = dvec + 3 * (size_t)p;
+        const double dot = k0 * d[0] + k1 * d[1] + k2 * d[2];
+        double sn, cs;
+        sincos(tpi * dot, &sn, &cs);
+        ph[(size_t)kb * npairs + p] = make_double2(cs, sn);
+    }
+}
+
+// One workgroup per (column atom ja, k-point); one thread per element of
+// the nc(ja) columns.  Elements of blocks (ia <= ja) are the Bloch sum over
+// the pairs (ia, ja) of the reference's pair list, in list order, of
+// e(a,b) * phase (rotate.f:260-326); elements of blocks ia > ja (and below
+// the diagonal of ia = ja) are the conjugates of their upper partners.  The
 // accumulation order and the absence of FMA contraction reproduce the
 // reference's rounding except for the ulp of sin/cos.
 __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a)
@@ -75,67 +95,41 @@ __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a)
     const int ja = blockIdx.x;
     const int kb = blockIdx.y;
     const int N = a.n;
-    const double *ak = a.kpts + 3 * (size_t)kb;
-    const double k0 = ak[0], k1 = ak[1], k2 = ak[2];
-    const double tpi = 6.283185307179586;
     const int c0 = a.ksk[ja], nc = a.nbas[ja];
+    const double2 *ph = a.ph + (size_t)kb * a.npairs;
     double2 *H = a.h + (size_t)kb * a.stride;
     double2 *S = a.s + (size_t)kb * a.stride;
-    for (int r = threadIdx.x; r < N; r += blockDim.x) {
+    for (int t = threadIdx.x; t < N * nc; t += blockDim.x) {
+        const int r = t % N, b = t / N;
         const int ia = a.row_atom[r], o = a.row_orb[r];
-        double hu[9][2], su[9][2], hl[9][2], sl[9][2];
-        for (int b = 0; b < 9; ++b) {
-            hu[b][0] = hu[b][1] = su[b][0] = su[b][1] = 0.0;
-            hl[b][0] = hl[b][1] = sl[b][0] = sl[b][1] = 0.0;
+        const bool upper = ia < ja || (ia == ja && b >= o);
+        int p0, p1, idx;
+        if (upper) {   // e(o, b) over pairs (ia, ja)
+            p0 = a.blk_ptr[ia * a.natoms + ja]; p1 = a.blk_ptr[ia * a.natoms + ja + 1];
+            idx = 9 * o + b;
+        } else {       // conj of e(b, o) over pairs (ja, ia)
+            p0 = a.blk_ptr[ja * a.natoms + ia]; p1 = a.blk_ptr[ja * a.natoms + ia + 1];
+            idx = 9 * b + o;
         }
-        if (ia <= ja) {            // block (ia, ja): row o of e
-            const int p0 = a.blk_ptr[ia * a.natoms + ja], p1 = a.blk_ptr[ia * a.natoms + ja + 1];
-            for (int p = p0; p < p1; ++p) {
-                const double *d = a.dvec + 3 * (size_t)p;
-                const double dot = k0 * d[0] + k1 * d[1] + k2 * d[2];
-                double sn, cs;
-                sincos(tpi * dot, &sn, &cs);
-                const double *eh = a.eh + 81 * (size_t)p + 9 * o;
-                const double *es = a.es + 81 * (size_t)p + 9 * o;
-                for (int b = 0; b < nc; ++b) {
-                    const double e1 = eh[b], e2 = es[b];
-                    hu[b][0] = hu[b][0] + e1 * cs; hu[b][1] = hu[b][1] + e1 * sn;
-                    su[b][0] = su[b][0] + e2 * cs; su[b][1] = su[b][1] + e2 * sn;
-                }
-            }
+        double hr = 0.0, hi = 0.0, sr = 0.0, si = 0.0;
+        for (int p = p0; p < p1; ++p) {
+            const double2 z = ph[p];
+            const double e1 = a.eh[81 * (size_t)p + idx], e2 = a.es[81 * (size_t)p + idx];
+            hr = hr + e1 * z.x; hi = hi + e1 * z.y;
+            sr = sr + e2 * z.x; si = si + e2 * z.y;
         }
-        if (ia >= ja) {            // block (ja, ia): column o of e, conjugated
-            const int p0 = a.blk_ptr[ja * a.natoms + ia], p1 = a.blk_ptr[ja * a.natoms + ia + 1];
-            for (int p = p0; p < p1; ++p) {
-                const double *d = a.dvec + 3 * (size_t)p;
-                const double dot = k0 * d[0] + k1 * d[1] + k2 * d[2];
-                double sn, cs;
-                sincos(tpi * dot, &sn, &cs);
-                const double *eh = a.eh + 81 * (size_t)p + o;
-                const double *es = a.es + 81 * (size_t)p + o;
-                for (int b = 0; b < nc; ++b) {
-                    const double e1 = eh[9 * b], e2 = es[9 * b];
-                    hl[b][0] = hl[b][0] + e1 * cs; hl[b][1] = hl[b][1] + e1 * sn;
-                    sl[b][0] = sl[b][0] + e2 * cs; sl[b][1] = sl[b][1] + e2 * sn;
-                }
-            }
+        double2 hv, sv;
+        if (ia == ja && b == o) {            // diagonal: real, plus on-site terms
+            const int cls = o == 0 ? 0 : (o <= 3 ? 1 : (o <= 6 ? 2 : 3));
+            hv = make_double2(hr + a.onsite[4 * ia + cls], 0.0);
+            sv = make_double2(sr + (a.onsite_s ? 1.0 : 0.0), 0.0);
+        } else if (upper) {
+            hv = make_double2(hr, hi); sv = make_double2(sr, si);
+        } else {
+            hv = make_double2(hr, -hi); sv = make_double2(sr, -si);
         }
-        for (int b = 0; b < nc; ++b) {
-            double2 hv, sv;
-            if (ia < ja || (ia == ja && b > o)) {
-                hv = make_double2(hu[b][0], hu[b][1]);
-                sv = make_double2(su[b][0], su[b][1]);
-            } else if (ia > ja || b < o) {
-                hv = make_double2(hl[b][0], -hl[b][1]);
-                sv = make_double2(sl[b][0], -sl[b][1]);
-            } else {                 // diagonal: real, plus on-site terms
-                const int cls = o == 0 ? 0 : (o <= 3 ? 1 : (o <= 6 ? 2 : 3));
-                hv = make_double2(hu[b][0] + a.onsite[4 * ia + cls], 0.0);
-                sv = make_double2(su[b][0] + (a.onsite_s ? 1.0 : 0.0), 0.0);
-            }
-            H[(size_t)(c0 + b) * N + r] = hv;
-            S[(size_t)(c0 + b) * N + r] = sv;
-        }
+        H[(size_t)(c0 + b) * N + r] = hv;
+        S[(size_t)(c0 + b) * N + r] = sv;
     }
 }
 
@@ -1146,6 +1140,12 @@ __global__ __launch_bounds__(1024) void k_fermi_sums(FermiArgs f)
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+void launch_phases(int npairs, const double *kpts, const double *dvec, double2 *ph, int nb,
+                   hipStream_t st)
+{
+    const int gx = std::max(1, std::min((npairs + 255) / 256, 64));
+    hipLaunchKernelGGL(k_phases, dim3(gx, nb), dim3(256), 0, st, npairs, kpts, dvec, ph);
+}
 void launch_build_hs(const BuildArgs &a, int natoms, int nb, hipStream_t st)
 {
     hipLaunchKernelGGL(k_build_hs, dim3(natoms, nb), dim3(256), 0, st, a);

@@ -230,6 +230,7 @@ struct tbk_plan {
     double *paros = nullptr, *dparos = nullptr, *kp = nullptr, *wts = nullptr;
     // results
     double *esk = nullptr, *desk = nullptr, *out = nullptr;
+    double2 *ph = nullptr;          // Bloch phases of the current batch
     int *ncut = nullptr, *err = nullptr;
     std::vector<int> info_host;
     Workspace ws;
@@ -244,7 +245,7 @@ static void plan_free(tbk_plan *p)
     dfree(p->row_atom); dfree(p->row_orb); dfree(p->ksk); dfree(p->nbas); dfree(p->blk_ptr);
     dfree(p->dvec); dfree(p->eh); dfree(p->es); dfree(p->edh); dfree(p->eds);
     dfree(p->paros); dfree(p->dparos); dfree(p->kp); dfree(p->wts);
-    dfree(p->esk); dfree(p->desk); dfree(p->out); dfree(p->ncut); dfree(p->err);
+    dfree(p->esk); dfree(p->desk); dfree(p->out); dfree(p->ncut); dfree(p->err); dfree(p->ph);
     p->ws.release();
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
@@ -337,6 +338,7 @@ extern "C" int tbk_plan_create(int device, const tbk_structure_desc *st, int nk,
     rc |= upload(&p->kp, p->kpts); rc |= upload(&p->wts, p->weights);
     rc |= dalloc(&p->esk, (size_t)nk * nv); rc |= dalloc(&p->desk, (size_t)nk * nv);
     rc |= dalloc(&p->ncut, (size_t)nk); rc |= dalloc(&p->err, 1); rc |= dalloc(&p->out, 8);
+    if (!rc) rc = dalloc(&p->ph, (size_t)p->batch * std::max(np, 1));
     if (!rc) rc = p->ws.alloc(nv, p->batch);
     if (rc) { plan_free(p); return rc == TBK_ERR_HIP ? TBK_ERR_HIP : TBK_ERR_NOMEM; }
     p->timer.st = p->stream;
@@ -348,13 +350,13 @@ extern "C" int tbk_plan_create(int device, const tbk_structure_desc *st, int nk,
 static tbk::BuildArgs build_args(tbk_plan *p, int k0, bool deriv)
 {
     tbk::BuildArgs a;
-    a.n = p->nv; a.natoms = p->natoms; a.onsite_s = deriv ? 0 : 1;
+    a.n = p->nv; a.natoms = p->natoms; a.onsite_s = deriv ? 0 : 1; a.npairs = p->npairs;
+    a.ph = p->ph;
     a.stride = p->ws.stride();
     a.row_atom = p->row_atom; a.row_orb = p->row_orb; a.ksk = p->ksk; a.nbas = p->nbas;
     a.blk_ptr = p->blk_ptr; a.dvec = p->dvec;
     a.eh = deriv ? p->edh : p->eh; a.es = deriv ? p->eds : p->es;
     a.onsite = deriv ? p->dparos : p->paros;
-    a.kpts = p->kp + 3 * (size_t)k0;
     a.h = p->ws.A; a.s = p->ws.S;
     return a;
 }
@@ -374,6 +376,7 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
     for (int k0 = 0; k0 < p->nk; k0 += p->batch) {
         const int nb = std::min(p->batch, p->nk - k0);
         p->timer.begin("build_hs");
+        tbk::launch_phases(p->npairs, p->kp + 3 * (size_t)k0, p->dvec, p->ph, nb, st);
         tbk::launch_build_hs(build_args(p, k0, false), p->natoms, nb, st);
         p->timer.end();
         solve_batch(p->ws, nb, p->esk + (size_t)k0 * n, st, p->timer);
@@ -459,6 +462,7 @@ extern "C" int tbk_plan_hs(tbk_plan *p, int k, int deriv, double *h, double *s)
 {
     if (!p || k < 0 || k >= p->nk) return TBK_ERR_ARG;
     HIP_TRY(hipSetDevice(p->device));
+    tbk::launch_phases(p->npairs, p->kp + 3 * (size_t)k, p->dvec, p->ph, 1, p->stream);
     tbk::launch_build_hs(build_args(p, k, deriv != 0), p->natoms, 1, p->stream);
     HIP_TRY(hipStreamSynchronize(p->stream));
     const size_t ss = p->ws.stride();

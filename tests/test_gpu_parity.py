@@ -187,3 +187,40 @@ def test_indefinite_overlap_structure_is_an_error():
     with pytest.raises(_lib.TbkError) as ei:
         Plan(st, [[0.0, 0.0, 0.0]], [1.0]).run(4)
     assert ei.value.code == _lib.TBK_ERR_OVERLAP
+
+
+# Mixed precision (FP32 tridiagonalisation and back-transform, one FP64
+# Ogita-Aishima refinement): stated bounds |dE_F|, |dE_tot|/atom <= 1e-6 Ry;
+# eigenvalues 1e-8 Ry; pressure 1e-3 kbar; residual |HZ - SZ diag(e)| 1e-9.
+# One refinement step leaves |Z^H S Z - I| of order |R|^2 with R the FP32
+# orthogonality defect (about N * 2^-24): bound 1e-6.
+MIX_EF, MIX_E, MIX_EIG, MIX_P_KBAR, MIX_ORTH = 1e-6, 1e-6, 1e-8, 1e-3, 1e-6
+
+
+@pytest.mark.parametrize("cfg", ["c1", "c2"])
+def test_mixed_precision_error_vs_oracle(cfg):
+    if cfg == "c1":
+        st, k = S.config_c1(), S.monkhorst_pack((6, 6, 6))
+    else:
+        st, k = S.config_c2(), S.monkhorst_pack((8, 8, 8))[::16]
+    w = np.full(len(k), 1.0 / len(k))
+    print(f"mixed case {cfg}: N={st.nv} nk={len(k)}", flush=True)
+    plan = Plan(st, k, w, precision=_lib.TBK_MIXED)
+    r = plan.run(4, 0.005, 10.0)
+    e, d = plan.eigenvalues()
+    z0 = plan.eigenvectors(0)
+    h0, s0 = plan.hs(0)
+    plan.close()
+    ref = oracle.Setup(st).bande(k, w, mode=4, nthreads=16, want_eig=True)
+    de = float(np.abs(e - ref["eig"]).max())
+    def_ = abs(r["efermi"] - ref["efermi"])
+    dE = abs(r["earg1"] - ref["earg1"]) / st.natoms
+    dP = abs(r["earg2"] - ref["earg2"]) * S.RYBOHR3_KBAR
+    orth = float(np.abs(z0.conj().T @ s0 @ z0 - np.eye(st.nv)).max())
+    res = float(np.abs(h0 @ z0 - s0 @ z0 * e[0]).max())
+    print(f"  max|de|={de:.2e} dEF={def_:.2e} dE/atom={dE:.2e} dP={dP:.2e} kbar "
+          f"orth={orth:.2e} res={res:.2e}", flush=True)
+    assert def_ < MIX_EF and dE < MIX_E
+    assert de < MIX_EIG
+    assert dP < MIX_P_KBAR
+    assert orth < MIX_ORTH and res < 1e-9

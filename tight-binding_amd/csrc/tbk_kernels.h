@@ -18,22 +18,33 @@ struct BuildArgs {
     double2 *h, *s;
 };
 
-struct GemmArgs {
+template <class R> struct cplx_t;
+template <> struct cplx_t<double> { typedef double2 type; };
+template <> struct cplx_t<float> { typedef float2 type; };
+
+template <class R>
+struct GemmArgsT {
+    typedef typename cplx_t<R>::type C;
     int m, n, k;
-    double2 alpha, beta;
-    const double2 *a; int lda; size_t sa;
-    const double2 *b; int ldb; size_t sb;
-    double2 *c; int ldc; size_t sc;
+    C alpha, beta;
+    const C *a; int lda; size_t sa;
+    const C *b; int ldb; size_t sb;
+    C *c; int ldc; size_t sc;
     int tri;
 };
+typedef GemmArgsT<double> GemmArgs;
 
-struct HetrdArgs {
+template <class R>
+struct HetrdArgsT {
+    typedef typename cplx_t<R>::type C;
     int n;
     size_t stride;
-    double2 *a, *vp, *wp, *t;      // vp: [nb][n*96] panel [V|W|V]; t: [nb][tstride]
+    C *a, *vp, *wp, *t;            // vp: [nb][n*96] panel [V|W|V]; t: [nb][tstride]
+    C *vt;                         // [nb][n*n]: V T^H of every panel (columns i0..)
     double *d, *e;                 // [nb][n]
     size_t tstride;                // ceil(n/32) panels of 32x32 WY factors
 };
+typedef HetrdArgsT<double> HetrdArgs;
 
 struct SteinArgs {
     int n;
@@ -69,11 +80,20 @@ void launch_potrf_panel(int n, double2 *s, size_t stride, int j0, int *info, int
 void launch_trtri_panel(int n, double2 *s, size_t stride, const double2 *wk, int j0,
                         const int *info, int nb, hipStream_t st);
 void launch_zgemm(const GemmArgs &g, int opa, int opb, int nb, hipStream_t st);
+void launch_cgemm(const GemmArgsT<float> &g, int opa, int opb, int nb, hipStream_t st);
 void launch_hetrd_panel(const HetrdArgs &h, int i0, int nb, hipStream_t st);
 void launch_hetrd_finish(const HetrdArgs &h, int nb, hipStream_t st);
+void launch_hetrd_panel(const HetrdArgsT<float> &h, int i0, int nb, hipStream_t st);
+void launch_hetrd_finish(const HetrdArgsT<float> &h, int nb, hipStream_t st);
 void launch_stein(const SteinArgs &s, int nb, hipStream_t st);
 void launch_real_to_complex_t(int n, const double *vt, double2 *y, size_t stride, int nb,
                               hipStream_t st);
+void launch_c64_to_c32(size_t count, const double2 *x, float2 *y, hipStream_t st);
+void launch_c32_to_c64(size_t count, const float2 *x, double2 *y, hipStream_t st);
+void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int nb, hipStream_t st);
+int oa_max_n();                  // largest N of launch_oa_update
+void launch_oa_update(int n, const double2 *t, const double2 *g, double2 *f, size_t stride, double *w,
+                      int nb, hipStream_t st);
 void launch_coldot(int n, const double2 *z, const double2 *x, const double2 *y, size_t stride,
                    double2 *px, double2 *py, int nb, hipStream_t st);
 void launch_perturb(const PerturbArgs &p, hipStream_t st);

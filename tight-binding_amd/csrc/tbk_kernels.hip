@@ -44,6 +44,29 @@ __device__ __forceinline__ double2 zcmul(double2 a, double2 b)
 __device__ __forceinline__ double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
 __device__ __forceinline__ double2 zscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
 __device__ __forceinline__ double2 zzero() { return make_double2(0.0, 0.0); }
+__device__ __forceinline__ float2 zadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 zsub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 zmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 zcmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float2 zconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 zscale(float s, float2 a) { return make_float2(s * a.x, s * a.y); }
+
+// complex type of a real type
+template <class R> struct cplx;
+template <> struct cplx<double> { typedef double2 type; };
+template <> struct cplx<float> { typedef float2 type; };
+template <class C> __device__ __forceinline__ C czero();
+template <> __device__ __forceinline__ double2 czero<double2>() { return make_double2(0.0, 0.0); }
+template <> __device__ __forceinline__ float2 czero<float2>() { return make_float2(0.f, 0.f); }
+template <class C, class R> __device__ __forceinline__ C cmake(R x, R y);
+template <> __device__ __forceinline__ double2 cmake<double2, double>(double x, double y) { return make_double2(x, y); }
+template <> __device__ __forceinline__ float2 cmake<float2, float>(float x, float y) { return make_float2(x, y); }
 
 // Block-wide sum of one double (blockDim.x multiple of 64, <= 1024).
 __device__ double block_sum(double v, double *red)
@@ -277,16 +300,32 @@ __global__ __launch_bounds__(256) void k_trtri_panel(int N, double2 *S, size_t s
 // k < row tile), 2 = op(A) lower (skip k beyond the row tile), 3 = op(B)
 // upper (skip k beyond the column tile).
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
-template <int OPA, int OPB>
-__global__ __launch_bounds__(256) void k_zgemm(GemmArgs g)
+__device__ __forceinline__ v4d mfma16(double a, double b, v4d c)
 {
-    __shared__ double As[2][16][64 + 1];  // [re/im][k][m]
-    __shared__ double Bs[2][16][64 + 1];  // [re/im][k][n]
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mfma16(float a, float b, v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+// Row of accumulator item q in lane l of a 16x16 tile: the f64 MFMA holds
+// rows l/16 + 4q, the f32 MFMA rows 4(l/16) + q (column l % 16 for both).
+__device__ __forceinline__ int mfma_row(double, int lane, int q) { return (lane >> 4) + 4 * q; }
+__device__ __forceinline__ int mfma_row(float, int lane, int q) { return 4 * (lane >> 4) + q; }
+
+template <class R, int OPA, int OPB>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
+{
+    typedef typename cplx<R>::type C;
+    typedef R v4 __attribute__((ext_vector_type(4)));
+    __shared__ R As[2][16][64 + 1];  // [re/im][k][m]
+    __shared__ R Bs[2][16][64 + 1];  // [re/im][k][n]
     const int bm = blockIdx.x * 64, bn = blockIdx.y * 64, kb = blockIdx.z;
-    const double2 *A = g.a + (size_t)kb * g.sa;
-    const double2 *B = g.b + (size_t)kb * g.sb;
-    double2 *C = g.c + (size_t)kb * g.sc;
+    const C *A = g.a + (size_t)kb * g.sa;
+    const C *B = g.b + (size_t)kb * g.sb;
+    C *Cm = g.c + (size_t)kb * g.sc;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;
     int kbeg = 0, kend = g.k;
@@ -294,16 +333,16 @@ __global__ __launch_bounds__(256) void k_zgemm(GemmArgs g)
     if (g.tri == 2) kend = min(g.k, bm + 64);
     if (g.tri == 3) kend = min(g.k, bn + 64);
     kbeg = (kbeg / 16) * 16;
-    v4d accr[2][2], acci[2][2];
+    v4 accr[2][2], acci[2][2];
     for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) { accr[i][j] = v4d{0, 0, 0, 0}; acci[i][j] = v4d{0, 0, 0, 0}; }
+        for (int j = 0; j < 2; ++j) { accr[i][j] = v4{0, 0, 0, 0}; acci[i][j] = v4{0, 0, 0, 0}; }
     for (int k0 = kbeg; k0 < kend; k0 += 16) {
         __syncthreads();
         for (int t = tid; t < 1024; t += 256) {
             int m, kk;
             if (OPA == 0) { m = t & 63; kk = t >> 6; } else { kk = t & 15; m = t >> 4; }
             const int gm = bm + m, gk = k0 + kk;
-            double2 v = zzero();
+            C v = czero<C>();
             if (gm < g.m && gk < kend) {
                 if (OPA == 0) v = A[(size_t)gk * g.lda + gm];
                 else { v = A[(size_t)gm * g.lda + gk]; if (OPA == 1) v.y = -v.y; }
@@ -314,7 +353,7 @@ __global__ __launch_bounds__(256) void k_zgemm(GemmArgs g)
             int n, kk;
             if (OPB == 0) { kk = t & 15; n = t >> 4; } else { n = t & 63; kk = t >> 6; }
             const int gn = bn + n, gk = k0 + kk;
-            double2 v = zzero();
+            C v = czero<C>();
             if (gn < g.n && gk < kend) {
                 if (OPB == 0) v = B[(size_t)gn * g.ldb + gk];
                 else { v = B[(size_t)gk * g.ldb + gn]; if (OPB == 1) v.y = -v.y; }
@@ -324,7 +363,7 @@ __global__ __launch_bounds__(256) void k_zgemm(GemmArgs g)
         __syncthreads();
         for (int ks = 0; ks < 16; ks += 4) {
             const int kk = ks + (lane >> 4);
-            double ar[2], ai[2], br[2], bi[2];
+            R ar[2], ai[2], br[2], bi[2];
             for (int i = 0; i < 2; ++i) {
                 ar[i] = As[0][kk][wm + 16 * i + (lane & 15)];
                 ai[i] = As[1][kk][wm + 16 * i + (lane & 15)];
@@ -335,25 +374,23 @@ __global__ __launch_bounds__(256) void k_zgemm(GemmArgs g)
             }
             for (int i = 0; i < 2; ++i)
                 for (int j = 0; j < 2; ++j) {
-                    accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accr[i][j], 0, 0, 0);
-                    accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], accr[i][j], 0, 0, 0);
-                    acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], acci[i][j], 0, 0, 0);
-                    acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], acci[i][j], 0, 0, 0);
+                    accr[i][j] = mfma16(ar[i], br[j], accr[i][j]);
+                    accr[i][j] = mfma16(-ai[i], bi[j], accr[i][j]);
+                    acci[i][j] = mfma16(ar[i], bi[j], acci[i][j]);
+                    acci[i][j] = mfma16(ai[i], br[j], acci[i][j]);
                 }
         }
     }
-    // D layout of v_mfma_f64_16x16x4_f64: item q of lane l holds row
-    // (l / 16) + 4 q, column l % 16 of the tile.
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
             for (int q = 0; q < 4; ++q) {
-                const int r = bm + wm + 16 * i + (lane >> 4) + 4 * q;
+                const int r = bm + wm + 16 * i + mfma_row(R(0), lane, q);
                 const int c = bn + wn + 16 * j + (lane & 15);
                 if (r < g.m && c < g.n) {
-                    const double2 v = make_double2(accr[i][j][q], acci[i][j][q]);
-                    double2 out = zmul(g.alpha, v);
-                    double2 *p = C + (size_t)c * g.ldc + r;
-                    if (g.beta.x != 0.0 || g.beta.y != 0.0) out = zadd(out, zmul(g.beta, *p));
+                    const C v = cmake<C, R>(accr[i][j][q], acci[i][j][q]);
+                    C out = zmul(g.alpha, v);
+                    C *p = Cm + (size_t)c * g.ldc + r;
+                    if (g.beta.x != R(0) || g.beta.y != R(0)) out = zadd(out, zmul(g.beta, *p));
                     *p = out;
                 }
             }
@@ -384,40 +421,42 @@ __device__ __forceinline__ double2 wave_sum2(double2 v)
     return v;
 }
 
-__global__ __launch_bounds__(HT) void k_hetrd_panel(HetrdArgs h, int i0)
+template <class R>
+__global__ __launch_bounds__(HT) void k_hetrd_panel(HetrdArgsT<R> h, int i0)
 {
-    __shared__ double2 vloc[1280];     // reflector of the current column
-    __shared__ double2 ybuf[1280];     // A_t v
-    __shared__ double2 udot[2 * NB];   // W^H v (0..31) and V^H v (32..63)
-    __shared__ double2 uw[NB], uv[NB]; // conj(W[j, l]), conj(V[j, l])
-    __shared__ double red[2 * HW];
-    __shared__ double2 scal[2];        // tau, (beta, 0)
+    typedef typename cplx<R>::type C;
+    __shared__ C vloc[1280];           // reflector of the current column
+    __shared__ C ybuf[1280];           // A_t v
+    __shared__ C udot[2 * NB];         // W^H v (0..31) and V^H v (32..63)
+    __shared__ C uw[NB], uv[NB];       // conj(W[j, l]), conj(V[j, l])
+    __shared__ R red[2 * HW];
+    __shared__ C scal[2];              // tau, (beta, 0)
     const int N = h.n;
     const int kb = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    double2 *A = h.a + (size_t)kb * h.stride;
-    double2 *P = h.vp + (size_t)kb * N * 96;
-    double2 *Tp = h.t + (size_t)kb * h.tstride + (size_t)(i0 / NB) * NB * NB;
+    C *A = h.a + (size_t)kb * h.stride;
+    C *P = h.vp + (size_t)kb * N * 96;
+    C *Tp = h.t + (size_t)kb * h.tstride + (size_t)(i0 / NB) * NB * NB;
     double *d = h.d + (size_t)kb * N;
     double *e = h.e + (size_t)kb * N;
     const int pw = min(NB, N - 1 - i0);
-    double2 *V = P, *W = P + (size_t)NB * N, *V2 = P + (size_t)2 * NB * N;
-    for (int t = tid; t < N * 96; t += HT) P[t] = zzero();
-    for (int t = tid; t < NB * NB; t += HT) Tp[t] = zzero();
+    C *V = P, *W = P + (size_t)NB * N, *V2 = P + (size_t)2 * NB * N;
+    for (int t = tid; t < N * 96; t += HT) P[t] = czero<C>();
+    for (int t = tid; t < NB * NB; t += HT) Tp[t] = czero<C>();
     __syncthreads();
     for (int i = 0; i < pw; ++i) {
         const int j = i0 + i, t0 = j + 1, m = N - t0;
         // (1) column j brought up to date: a_r -= V[r,:] conj(W[j,:]) + W[r,:] conj(V[j,:])
         if (tid < i) { uw[tid] = zconj(W[(size_t)tid * N + j]); uv[tid] = zconj(V[(size_t)tid * N + j]); }
         __syncthreads();
-        double xs = 0.0;
+        R xs = 0;
         for (int r = j + tid; r < N; r += HT) {
-            double2 a = A[(size_t)j * N + r];
+            C a = A[(size_t)j * N + r];
             for (int l = 0; l < i; ++l) {
                 a = zsub(a, zmul(V[(size_t)l * N + r], uw[l]));
                 a = zsub(a, zmul(W[(size_t)l * N + r], uv[l]));
             }
-            if (r == j) a.y = 0.0;
+            if (r == j) a.y = 0;
             vloc[r] = a;
             if (r >= j + 2) xs += a.x * a.x + a.y * a.y;
         }
@@ -426,33 +465,33 @@ __global__ __launch_bounds__(HT) void k_hetrd_panel(HetrdArgs h, int i0)
         __syncthreads();
         // (2) reflector (zlarfg) for x = a[j+1:]
         if (tid == 0) {
-            double xn = 0.0;
+            R xn = 0;
             for (int q = 0; q < HW; ++q) xn += red[q];
-            const double2 al = vloc[j + 1];
-            double2 tau = zzero();
-            double beta = al.x;
-            if (!(xn == 0.0 && al.y == 0.0)) {
+            const C al = vloc[j + 1];
+            C tau = czero<C>();
+            R beta = al.x;
+            if (!(xn == R(0) && al.y == R(0))) {
                 beta = -copysign(sqrt(al.x * al.x + al.y * al.y + xn), al.x);
-                tau = make_double2((beta - al.x) / beta, -al.y / beta);
+                tau = cmake<C, R>((beta - al.x) / beta, -al.y / beta);
             }
-            scal[0] = tau; scal[1] = make_double2(beta, 0.0);
-            d[j] = vloc[j].x;
-            e[j] = beta;
+            scal[0] = tau; scal[1] = cmake<C, R>(beta, R(0));
+            d[j] = (double)vloc[j].x;
+            e[j] = (double)beta;
         }
         __syncthreads();
-        const double2 tau = scal[0];
-        const double beta = scal[1].x;
+        const C tau = scal[0];
+        const R beta = scal[1].x;
         {
-            const double2 al = vloc[j + 1];
-            const double2 dv = make_double2(al.x - beta, al.y);
-            const double den = dv.x * dv.x + dv.y * dv.y;
-            const bool refl = tau.x != 0.0 || tau.y != 0.0;
-            const double2 sc = refl ? make_double2(dv.x / den, -dv.y / den) : make_double2(1.0, 0.0);
+            const C al = vloc[j + 1];
+            const C dv = cmake<C, R>(al.x - beta, al.y);
+            const R den = dv.x * dv.x + dv.y * dv.y;
+            const bool refl = tau.x != R(0) || tau.y != R(0);
+            const C sc = refl ? cmake<C, R>(dv.x / den, -dv.y / den) : cmake<C, R>(R(1), R(0));
             __syncthreads();
             for (int r = tid; r < N; r += HT) {
-                double2 v;
-                if (r <= j) v = zzero();
-                else if (r == j + 1) v = make_double2(1.0, 0.0);
+                C v;
+                if (r <= j) v = czero<C>();
+                else if (r == j + 1) v = cmake<C, R>(R(1), R(0));
                 else v = zmul(vloc[r], sc);
                 vloc[r] = v;
                 A[(size_t)j * N + r] = v;
@@ -465,19 +504,23 @@ __global__ __launch_bounds__(HT) void k_hetrd_panel(HetrdArgs h, int i0)
         //     the 2i panel dots W[:,l]^H v, V[:,l]^H v over rows >= t0
         const int ntask = m + 2 * i;
         for (int task = wv; task < ntask; task += HW) {
-            const double2 *col;
+            const C *col;
             if (task < m) col = A + (size_t)(t0 + task) * N;
             else if (task < m + i) col = W + (size_t)(task - m) * N;
             else col = V + (size_t)(task - m - i) * N;
-            double2 acc = zzero(), acc2 = zzero();
+            C acc = czero<C>(), acc2 = czero<C>();
             int c = t0 + lane;
             for (; c + 64 < N; c += 128) {
-                const double2 a0 = col[c], a1 = col[c + 64];
+                const C a0 = col[c], a1 = col[c + 64];
                 acc = zadd(acc, zcmul(a0, vloc[c]));
                 acc2 = zadd(acc2, zcmul(a1, vloc[c + 64]));
             }
             if (c < N) acc = zadd(acc, zcmul(col[c], vloc[c]));
-            acc = wave_sum2(zadd(acc, acc2));
+            acc = zadd(acc, acc2);
+            for (int off = 32; off > 0; off >>= 1) {
+                acc.x += __shfl_xor(acc.x, off, 64);
+                acc.y += __shfl_xor(acc.y, off, 64);
+            }
             if (lane == 0) {
                 if (task < m) ybuf[t0 + task] = acc;
                 else udot[task - m + (task < m + i ? 0 : NB - i)] = acc;
@@ -486,16 +529,16 @@ __global__ __launch_bounds__(HT) void k_hetrd_panel(HetrdArgs h, int i0)
         __syncthreads();
         // (4) w = tau (y - V u1 - W u2), u1 = W^H v, u2 = V^H v; then the
         //     correction w += alpha v with alpha = -tau/2 (w^H v)
-        double pr = 0.0, pi = 0.0;
+        R pr = 0, pi = 0;
         for (int r = t0 + tid; r < N; r += HT) {
-            double2 y = ybuf[r];
+            C y = ybuf[r];
             for (int l = 0; l < i; ++l) {
                 y = zsub(y, zmul(V[(size_t)l * N + r], udot[l]));
                 y = zsub(y, zmul(W[(size_t)l * N + r], udot[NB + l]));
             }
-            const double2 w = zmul(tau, y);
+            const C w = zmul(tau, y);
             ybuf[r] = w;
-            const double2 q = zcmul(w, vloc[r]);
+            const C q = zcmul(w, vloc[r]);
             pr += q.x; pi += q.y;
         }
         for (int off = 32; off > 0; off >>= 1) {
@@ -505,32 +548,44 @@ __global__ __launch_bounds__(HT) void k_hetrd_panel(HetrdArgs h, int i0)
         __syncthreads();
         if (lane == 0) { red[wv] = pr; red[HW + wv] = pi; }
         __syncthreads();
-        double sr = 0.0, si = 0.0;
+        R sr = 0, si = 0;
         for (int q = 0; q < HW; ++q) { sr += red[q]; si += red[HW + q]; }
-        const double2 alpha = zscale(-0.5, zmul(tau, make_double2(sr, si)));
+        const C alpha = zscale(R(-0.5), zmul(tau, cmake<C, R>(sr, si)));
         for (int r = t0 + tid; r < N; r += HT) W[(size_t)i * N + r] = zadd(ybuf[r], zmul(alpha, vloc[r]));
         // (5) WY factor: T[0:i, i] = -tau T[0:i, 0:i] (V^H v), T[i, i] = tau
         if (tid < i) {
-            double2 acc = zzero();
+            C acc = czero<C>();
             for (int l = tid; l < i; ++l) acc = zadd(acc, zmul(Tp[(size_t)l * NB + tid], udot[NB + l]));
-            Tp[(size_t)i * NB + tid] = zmul(make_double2(-tau.x, -tau.y), acc);
+            Tp[(size_t)i * NB + tid] = zmul(cmake<C, R>(-tau.x, -tau.y), acc);
         }
         if (tid == 0) Tp[(size_t)i * NB + i] = tau;
         __syncthreads();
     }
+    // V T^H of the panel (rows > i0), for the back-transform
+    // Q_p Y = Y - V ((V T^H)^H Y)
+    C *VT = h.vt + (size_t)kb * h.stride + (size_t)i0 * N;
+    for (int t = tid; t < N * NB; t += HT) {
+        const int r = t % N, c = t / N;
+        C acc = czero<C>();
+        if (c < pw && r > i0)
+            for (int l = c; l < pw; ++l) acc = zadd(acc, zmul(V[(size_t)l * N + r], zconj(Tp[(size_t)l * NB + c])));
+        if (i0 + c < N) VT[(size_t)c * N + r] = acc;
+    }
 }
 
 // After the last panel's trailing update: d[n-1] and the empty last column.
-__global__ void k_hetrd_finish(HetrdArgs h)
+template <class R>
+__global__ void k_hetrd_finish(HetrdArgsT<R> h)
 {
+    typedef typename cplx<R>::type C;
     const int N = h.n, kb = blockIdx.x;
-    double2 *A = h.a + (size_t)kb * h.stride;
+    C *A = h.a + (size_t)kb * h.stride;
     if (threadIdx.x == 0) {
-        h.d[(size_t)kb * N + N - 1] = A[(size_t)(N - 1) * N + N - 1].x;
+        h.d[(size_t)kb * N + N - 1] = (double)A[(size_t)(N - 1) * N + N - 1].x;
         h.e[(size_t)kb * N + N - 1] = 0.0;
     }
     __syncthreads();
-    for (int r = threadIdx.x; r < N; r += blockDim.x) A[(size_t)(N - 1) * N + r] = zzero();
+    for (int r = threadIdx.x; r < N; r += blockDim.x) A[(size_t)(N - 1) * N + r] = czero<C>();
 }
 
 // ---------------------------------------------------------------------------
@@ -832,6 +887,113 @@ __global__ void k_real_to_complex_t(int N, const double *Vt, double2 *Y, size_t 
         const int r = (int)(t % N), c = (int)(t / N);
         Yk[t] = make_double2(V[(size_t)r * N + c], 0.0);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: conversions and the Ogita-Aishima refinement step
+// ---------------------------------------------------------------------------
+__global__ void k_c64_to_c32(size_t count, const double2 *x, float2 *y)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (size_t)gridDim.x * blockDim.x)
+        y[t] = make_float2((float)x[t].x, (float)x[t].y);
+}
+__global__ void k_c32_to_c64(size_t count, const float2 *x, double2 *y)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (size_t)gridDim.x * blockDim.x)
+        y[t] = make_double2((double)x[t].x, (double)x[t].y);
+}
+// Y32[:, j] = v_j of the row-major real tridiagonal eigenvectors.
+__global__ void k_real_to_c32_t(int N, const double *Vt, float2 *Y, size_t stride)
+{
+    const int kb = blockIdx.y;
+    const double *V = Vt + (size_t)kb * N * N;
+    float2 *Yk = Y + (size_t)kb * stride;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < (size_t)N * N;
+         t += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(t % N), c = (int)(t / N);
+        Yk[t] = make_float2((float)V[(size_t)r * N + c], 0.f);
+    }
+}
+
+// One refinement step for the Hermitian eigenproblem of C with approximate
+// orthonormal eigenvectors Y (Ogita & Aishima, RefSyEv, extended to complex
+// Hermitian): on entry T = Y^H C Y and G = Y^H Y.  With R = I - G,
+//   lambda_i = T_ii / (1 - R_ii),
+//   delta    = 2 (||T - diag(lambda)||_F + max|lambda| ||R||_F),
+//   E_ij     = (T_ij + lambda_j R_ij) / (lambda_j - lambda_i)  if |lambda_i - lambda_j| > delta,
+//              R_ij / 2                                        otherwise,
+// and the refined vectors are Y (I + E).  The kernel writes F = I + E with
+// its columns in ascending order of lambda (F[:, j] = (I + E)[:, perm_j]), so
+// that Y F is the refined, sorted eigenvector matrix, and lambda sorted into w.
+constexpr int kOaMax = 1536;
+__global__ __launch_bounds__(256) void k_oa_update(int N, const double2 *T, const double2 *G,
+                                                   double2 *F, size_t stride, double *w)
+{
+    __shared__ double lam[kOaMax], lsort[kOaMax];
+    __shared__ int idx[kOaMax];
+    __shared__ double red[256];
+    const int kb = blockIdx.x, tid = threadIdx.x;
+    const double2 *Tk = T + (size_t)kb * stride;
+    const double2 *Gk = G + (size_t)kb * stride;
+    double2 *Fk = F + (size_t)kb * stride;
+    double lmax = 0.0;
+    for (int i = tid; i < N; i += blockDim.x) {
+        const double l = Tk[(size_t)i * N + i].x / Gk[(size_t)i * N + i].x;   // G_ii = 1 - R_ii
+        lam[i] = l; lsort[i] = l; idx[i] = i;
+        lmax = fmax(lmax, fabs(l));
+    }
+    red[tid] = lmax;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) { if (tid < st) red[tid] = fmax(red[tid], red[tid + st]); __syncthreads(); }
+    lmax = red[0];
+    __syncthreads();
+    if (tid == 0) {                  // stable insertion sort (the input is nearly sorted)
+        for (int i = 1; i < N; ++i) {
+            const double x = lsort[i];
+            const int xi = idx[i];
+            int k = i - 1;
+            while (k >= 0 && lsort[k] > x) { lsort[k + 1] = lsort[k]; idx[k + 1] = idx[k]; --k; }
+            lsort[k + 1] = x; idx[k + 1] = xi;
+        }
+    }
+    double ft = 0.0, fr = 0.0;
+    for (size_t t = tid; t < (size_t)N * N; t += blockDim.x) {
+        const int r = (int)(t % N), c = (int)(t / N);
+        double2 tv = Tk[t];
+        double2 rv = make_double2(-Gk[t].x, -Gk[t].y);
+        if (r == c) { tv.x -= lam[r]; rv.x += 1.0; }
+        ft += tv.x * tv.x + tv.y * tv.y;
+        fr += rv.x * rv.x + rv.y * rv.y;
+    }
+    __syncthreads();
+    red[tid] = ft;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) { if (tid < st) red[tid] += red[tid + st]; __syncthreads(); }
+    ft = sqrt(red[0]);
+    __syncthreads();
+    red[tid] = fr;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) { if (tid < st) red[tid] += red[tid + st]; __syncthreads(); }
+    fr = sqrt(red[0]);
+    const double delta = 2.0 * (ft + lmax * fr);
+    for (size_t t = tid; t < (size_t)N * N; t += blockDim.x) {
+        const int r = (int)(t % N), j = (int)(t / N);
+        const int c = idx[j];
+        const size_t src = (size_t)c * N + r;
+        double2 rv = make_double2(-Gk[src].x, -Gk[src].y);
+        if (r == c) rv.x += 1.0;
+        double2 ev;
+        const double gap = lam[c] - lam[r];
+        if (fabs(gap) > delta) {
+            const double2 tv = Tk[src];
+            ev = make_double2((tv.x + lam[c] * rv.x) / gap, (tv.y + lam[c] * rv.y) / gap);
+        } else {
+            ev = make_double2(0.5 * rv.x, 0.5 * rv.y);
+        }
+        if (r == c) ev.x += 1.0;
+        Fk[t] = ev;
+    }
+    for (int i = tid; i < N; i += blockDim.x) w[(size_t)kb * N + i] = lsort[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1160,22 +1322,37 @@ void launch_trtri_panel(int n, double2 *s, size_t stride, const double2 *wk, int
 {
     hipLaunchKernelGGL(k_trtri_panel, dim3(nb), dim3(256), 0, st, n, s, stride, wk, j0, info);
 }
-void launch_zgemm(const GemmArgs &g, int opa, int opb, int nb, hipStream_t st)
+template <class R>
+static void launch_gemm_t(const GemmArgsT<R> &g, int opa, int opb, int nb, hipStream_t st)
 {
     dim3 grid((g.m + 63) / 64, (g.n + 63) / 64, nb);
-    if (opa == 0 && opb == 0) hipLaunchKernelGGL((k_zgemm<0, 0>), grid, dim3(256), 0, st, g);
-    else if (opa == 1 && opb == 0) hipLaunchKernelGGL((k_zgemm<1, 0>), grid, dim3(256), 0, st, g);
-    else if (opa == 0 && opb == 1) hipLaunchKernelGGL((k_zgemm<0, 1>), grid, dim3(256), 0, st, g);
-    else hipLaunchKernelGGL((k_zgemm<1, 1>), grid, dim3(256), 0, st, g);
+    if (opa == 0 && opb == 0) hipLaunchKernelGGL((k_gemm<R, 0, 0>), grid, dim3(256), 0, st, g);
+    else if (opa == 1 && opb == 0) hipLaunchKernelGGL((k_gemm<R, 1, 0>), grid, dim3(256), 0, st, g);
+    else if (opa == 0 && opb == 1) hipLaunchKernelGGL((k_gemm<R, 0, 1>), grid, dim3(256), 0, st, g);
+    else hipLaunchKernelGGL((k_gemm<R, 1, 1>), grid, dim3(256), 0, st, g);
 }
-void launch_hetrd_panel(const HetrdArgs &h, int i0, int nb, hipStream_t st)
+void launch_zgemm(const GemmArgs &g, int opa, int opb, int nb, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_hetrd_panel, dim3(nb), dim3(HT), 0, st, h, i0);
+    launch_gemm_t<double>(g, opa, opb, nb, st);
 }
-void launch_hetrd_finish(const HetrdArgs &h, int nb, hipStream_t st)
+void launch_cgemm(const GemmArgsT<float> &g, int opa, int opb, int nb, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_hetrd_finish, dim3(nb), dim3(256), 0, st, h);
+    launch_gemm_t<float>(g, opa, opb, nb, st);
 }
+template <class R>
+static void launch_hetrd_panel_t(const HetrdArgsT<R> &h, int i0, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_hetrd_panel<R>), dim3(nb), dim3(HT), 0, st, h, i0);
+}
+template <class R>
+static void launch_hetrd_finish_t(const HetrdArgsT<R> &h, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_hetrd_finish<R>), dim3(nb), dim3(256), 0, st, h);
+}
+void launch_hetrd_panel(const HetrdArgs &h, int i0, int nb, hipStream_t st) { launch_hetrd_panel_t(h, i0, nb, st); }
+void launch_hetrd_finish(const HetrdArgs &h, int nb, hipStream_t st) { launch_hetrd_finish_t(h, nb, st); }
+void launch_hetrd_panel(const HetrdArgsT<float> &h, int i0, int nb, hipStream_t st) { launch_hetrd_panel_t(h, i0, nb, st); }
+void launch_hetrd_finish(const HetrdArgsT<float> &h, int nb, hipStream_t st) { launch_hetrd_finish_t(h, nb, st); }
 void launch_stein(const SteinArgs &s, int nb, hipStream_t st)
 {
     hipLaunchKernelGGL(k_stein, dim3(nb), dim3(256), 0, st, s);
@@ -1185,6 +1362,26 @@ void launch_real_to_complex_t(int N, const double *Vt, double2 *Y, size_t stride
 {
     const int blocks = (int)std::min<size_t>(((size_t)N * N + 255) / 256, 4096);
     hipLaunchKernelGGL(k_real_to_complex_t, dim3(blocks, nb), dim3(256), 0, st, N, Vt, Y, stride);
+}
+static int grid_for(size_t count) { return (int)std::min<size_t>((count + 255) / 256, 8192); }
+void launch_c64_to_c32(size_t count, const double2 *x, float2 *y, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_c64_to_c32, dim3(grid_for(count)), dim3(256), 0, st, count, x, y);
+}
+void launch_c32_to_c64(size_t count, const float2 *x, double2 *y, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_c32_to_c64, dim3(grid_for(count)), dim3(256), 0, st, count, x, y);
+}
+void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int nb, hipStream_t st)
+{
+    const int blocks = (int)std::min<size_t>(((size_t)n * n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_real_to_c32_t, dim3(blocks, nb), dim3(256), 0, st, n, vt, y, stride);
+}
+int oa_max_n() { return kOaMax; }
+void launch_oa_update(int n, const double2 *t, const double2 *g, double2 *f, size_t stride, double *w,
+                      int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_oa_update, dim3(nb), dim3(256), 0, st, n, t, g, f, stride, w);
 }
 void launch_coldot(int N, const double2 *Z, const double2 *X, const double2 *Y, size_t stride,
                    double2 *px, double2 *py, int nb, hipStream_t st)

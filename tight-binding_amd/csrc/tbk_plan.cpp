@@ -44,8 +44,12 @@ struct Workspace {
     int n = 0, b = 0;
     double2 *A = nullptr, *S = nullptr, *Y = nullptr, *Z = nullptr, *X = nullptr;
     double2 *Vp = nullptr, *Wp = nullptr, *Tf = nullptr, *px = nullptr, *py = nullptr;
+    double2 *VT = nullptr;
     double *d = nullptr, *e = nullptr, *Vt = nullptr, *scratch = nullptr, *w = nullptr;
     int *info = nullptr, *sinfo = nullptr;
+    // mixed precision: FP32 copies for the reduction and back-transform
+    float2 *A32 = nullptr, *P32 = nullptr, *T32 = nullptr, *VT32 = nullptr, *Y32 = nullptr,
+           *X32 = nullptr;
     size_t stride() const { return (size_t)n * n; }
     // WY factors: one kNB x kNB block per panel, ceil(n / kNB) panels
     size_t tstride() const { return (size_t)((n + kNB - 1) / kNB) * kNB * kNB; }
@@ -55,7 +59,7 @@ struct Workspace {
         const size_t nn = (size_t)n * n * b, np = (size_t)n * kNB * b;
         int rc = TBK_OK;
         rc |= dalloc(&A, nn); rc |= dalloc(&S, nn); rc |= dalloc(&Y, nn);
-        rc |= dalloc(&Z, nn); rc |= dalloc(&X, nn);
+        rc |= dalloc(&Z, nn); rc |= dalloc(&X, nn); rc |= dalloc(&VT, nn);
         rc |= dalloc(&Vp, (size_t)n * 96 * b); rc |= dalloc(&Wp, np); rc |= dalloc(&Tf, tstride() * b);
         rc |= dalloc(&px, (size_t)n * b); rc |= dalloc(&py, (size_t)n * b);
         rc |= dalloc(&d, (size_t)n * b); rc |= dalloc(&e, (size_t)n * b);
@@ -65,11 +69,20 @@ struct Workspace {
         rc |= dalloc(&info, (size_t)b); rc |= dalloc(&sinfo, (size_t)b);
         return rc ? TBK_ERR_NOMEM : TBK_OK;
     }
+    int alloc_mixed()
+    {
+        const size_t nn = (size_t)n * n * b;
+        int rc = TBK_OK;
+        rc |= dalloc(&A32, nn); rc |= dalloc(&VT32, nn); rc |= dalloc(&Y32, nn); rc |= dalloc(&X32, nn);
+        rc |= dalloc(&P32, (size_t)n * 96 * b); rc |= dalloc(&T32, tstride() * b);
+        return rc ? TBK_ERR_NOMEM : TBK_OK;
+    }
     void release()
     {
-        dfree(A); dfree(S); dfree(Y); dfree(Z); dfree(X); dfree(Vp); dfree(Wp); dfree(Tf);
+        dfree(A); dfree(S); dfree(Y); dfree(Z); dfree(X); dfree(VT); dfree(Vp); dfree(Wp); dfree(Tf);
         dfree(px); dfree(py); dfree(d); dfree(e); dfree(w); dfree(Vt); dfree(scratch);
         dfree(info); dfree(sinfo);
+        dfree(A32); dfree(P32); dfree(T32); dfree(VT32); dfree(Y32); dfree(X32);
     }
 };
 
@@ -136,7 +149,7 @@ void hetrd(Workspace &ws, int nb, hipStream_t st)
 {
     const int n = ws.n;
     const size_t ss = ws.stride(), sp = (size_t)n * 96;
-    tbk::HetrdArgs h{n, ss, ws.A, ws.Vp, ws.Wp, ws.Tf, ws.d, ws.e, ws.tstride()};
+    tbk::HetrdArgs h{n, ss, ws.A, ws.Vp, ws.Wp, ws.Tf, ws.VT, ws.d, ws.e, ws.tstride()};
     for (int i0 = 0; i0 < n - 1; i0 += kNB) {
         const int pw = std::min(kNB, n - 1 - i0);
         tbk::launch_hetrd_panel(h, i0, nb, st);
@@ -173,6 +186,76 @@ void potrf_trtri(Workspace &ws, int nb, hipStream_t st)
     }
 }
 
+tbk::GemmArgsT<float> cgemm(int m, int n, int k, const float2 *a, int lda, size_t sa, const float2 *b,
+                            int ldb, size_t sb, float2 *c, int ldc, size_t sc, int tri = 0,
+                            float2 alpha = {1.f, 0.f}, float2 beta = {0.f, 0.f})
+{
+    tbk::GemmArgsT<float> g;
+    g.m = m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
+    g.a = a; g.lda = lda; g.sa = sa; g.b = b; g.ldb = ldb; g.sb = sb;
+    g.c = c; g.ldc = ldc; g.sc = sc; g.tri = tri;
+    return g;
+}
+
+// Mixed-precision solve: FP64 Cholesky, inverse and reduction; FP32 panel
+// tridiagonalisation and back-transform on the FP32 MFMA units; FP64
+// bisection/eigenvectors of the tridiagonal; one FP64 Ogita-Aishima step on
+// the reduced problem C; FP64 Z = U^{-1} Y.
+void solve_batch_mixed(Workspace &ws, int nb, double *w_out, hipStream_t st, StageTimer &tm)
+{
+    const int n = ws.n;
+    const size_t ss = ws.stride(), sp = (size_t)n * 96;
+    tm.begin("potrf_trtri");
+    potrf_trtri(ws, nb, st);
+    tm.end();
+    tm.begin("reduce_gemm");
+    tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.S, n, ss, ws.Y, n, ss, 3), 0, 0, nb, st);
+    tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 2), 1, 0, nb, st);
+    tm.end();
+    tm.begin("hetrd");
+    tbk::launch_c64_to_c32(ss * nb, ws.A, ws.A32, st);
+    tbk::HetrdArgsT<float> h{n, ss, ws.A32, ws.P32, nullptr, ws.T32, ws.VT32, ws.d, ws.e, ws.tstride()};
+    for (int i0 = 0; i0 < n - 1; i0 += kNB) {
+        const int pw = std::min(kNB, n - 1 - i0);
+        tbk::launch_hetrd_panel(h, i0, nb, st);
+        const int s = i0 + pw, m = n - s;
+        if (m > 0)
+            tbk::launch_cgemm(cgemm(m, m, 2 * kNB, ws.P32 + s, n, sp, ws.P32 + (size_t)kNB * n + s, n, sp,
+                                    ws.A32 + (size_t)s * n + s, n, ss, 0, float2{-1.f, 0.f},
+                                    float2{1.f, 0.f}), 0, 1, nb, st);
+    }
+    tbk::launch_hetrd_finish(h, nb, st);
+    tm.end();
+    tm.begin("stein");
+    tbk::SteinArgs sa{n, 1e-4, ws.d, ws.e, ws.w, ws.Vt, ws.scratch, ws.sinfo};
+    tbk::launch_stein(sa, nb, st);
+    tm.end();
+    tm.begin("backtransform");
+    tbk::launch_real_to_c32_t(n, ws.Vt, ws.Y32, ss, nb, st);
+    for (int i0 = ((n - 2) / kNB) * kNB; i0 >= 0; i0 -= kNB) {
+        const int pw = std::min(kNB, n - 1 - i0);
+        if (pw <= 0) continue;
+        const int r0 = i0 + 1, m = n - r0;
+        const float2 *Vsub = ws.A32 + (size_t)i0 * n + r0;
+        const float2 *VTsub = ws.VT32 + (size_t)i0 * n + r0;
+        tbk::launch_cgemm(cgemm(pw, n, m, VTsub, n, ss, ws.Y32 + r0, n, ss, ws.X32, n, ss), 1, 0, nb, st);
+        tbk::launch_cgemm(cgemm(m, n, pw, Vsub, n, ss, ws.X32, n, ss, ws.Y32 + r0, n, ss, 0,
+                                float2{-1.f, 0.f}, float2{1.f, 0.f}), 0, 0, nb, st);
+    }
+    tbk::launch_c32_to_c64(ss * nb, ws.Y32, ws.Y, st);
+    tm.end();
+    tm.begin("refine");
+    // X = C Y, Z = Y^H X (= Y^H C Y), VT = Y^H Y
+    tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.Y, n, ss, ws.X, n, ss), 0, 0, nb, st);
+    tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.Z, n, ss), 1, 0, nb, st);
+    tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.Y, n, ss, ws.VT, n, ss), 1, 0, nb, st);
+    // X = (I + E) with sorted columns; VT = Y X; Z = U^{-1} VT
+    tbk::launch_oa_update(n, ws.Z, ws.VT, ws.X, ss, w_out, nb, st);
+    tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.VT, n, ss), 0, 0, nb, st);
+    tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.VT, n, ss, ws.Z, n, ss, 1), 0, 0, nb, st);
+    tm.end();
+}
+
 // Stages 2-9 of the pipeline on ws.A = H, ws.S = S for nb k-points:
 // eigenvalues into w_out[nb*n], eigenvectors into ws.Z.
 void solve_batch(Workspace &ws, int nb, double *w_out, hipStream_t st, StageTimer &tm)
@@ -196,19 +279,16 @@ void solve_batch(Workspace &ws, int nb, double *w_out, hipStream_t st, StageTime
     tm.end();
     tm.begin("backtransform");
     tbk::launch_real_to_complex_t(n, ws.Vt, ws.Y, ss, nb, st);
-    const size_t sp = ws.tstride();
     for (int i0 = ((n - 2) / kNB) * kNB; i0 >= 0; i0 -= kNB) {
         const int pw = std::min(kNB, n - 1 - i0);
         if (pw <= 0) continue;
         const int r0 = i0 + 1, m = n - r0;
         const double2 *Vsub = ws.A + (size_t)i0 * n + r0;
-        // X[0:pw, :] = V_p^H Y[r0:, :]
-        tbk::launch_zgemm(gemm(pw, n, m, Vsub, n, ss, ws.Y + r0, n, ss, ws.X, n, ss), 1, 0, nb, st);
-        // Z[0:pw, :] = T_p X[0:pw, :]
-        tbk::launch_zgemm(gemm(pw, n, pw, ws.Tf + (size_t)(i0 / kNB) * kNB * kNB, kNB, sp, ws.X, n, ss,
-                               ws.Z, n, ss), 0, 0, nb, st);
-        // Y[r0:, :] -= V_p Z[0:pw, :]
-        tbk::launch_zgemm(gemm(m, n, pw, Vsub, n, ss, ws.Z, n, ss, ws.Y + r0, n, ss, 0,
+        const double2 *VTsub = ws.VT + (size_t)i0 * n + r0;
+        // X[0:pw, :] = (V_p T_p^H)^H Y[r0:, :]
+        tbk::launch_zgemm(gemm(pw, n, m, VTsub, n, ss, ws.Y + r0, n, ss, ws.X, n, ss), 1, 0, nb, st);
+        // Y[r0:, :] -= V_p X[0:pw, :]
+        tbk::launch_zgemm(gemm(m, n, pw, Vsub, n, ss, ws.X, n, ss, ws.Y + r0, n, ss, 0,
                                double2{-1.0, 0.0}, double2{1.0, 0.0}), 0, 0, nb, st);
     }
     // Z = Uinv Y
@@ -340,6 +420,8 @@ extern "C" int tbk_plan_create(int device, const tbk_structure_desc *st, int nk,
     rc |= dalloc(&p->ncut, (size_t)nk); rc |= dalloc(&p->err, 1); rc |= dalloc(&p->out, 8);
     if (!rc) rc = dalloc(&p->ph, (size_t)p->batch * std::max(np, 1));
     if (!rc) rc = p->ws.alloc(nv, p->batch);
+    if (!rc && precision == TBK_MIXED && nv > tbk::oa_max_n()) rc = TBK_ERR_ARG;
+    if (!rc && precision == TBK_MIXED) rc = p->ws.alloc_mixed();
     if (rc) { plan_free(p); return rc == TBK_ERR_HIP ? TBK_ERR_HIP : TBK_ERR_NOMEM; }
     p->timer.st = p->stream;
     p->info_host.assign(nk, 0);
@@ -379,7 +461,10 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
         tbk::launch_phases(p->npairs, p->kp + 3 * (size_t)k0, p->dvec, p->ph, nb, st);
         tbk::launch_build_hs(build_args(p, k0, false), p->natoms, nb, st);
         p->timer.end();
-        solve_batch(p->ws, nb, p->esk + (size_t)k0 * n, st, p->timer);
+        if (p->precision == TBK_MIXED)
+            solve_batch_mixed(p->ws, nb, p->esk + (size_t)k0 * n, st, p->timer);
+        else
+            solve_batch(p->ws, nb, p->esk + (size_t)k0 * n, st, p->timer);
         HIP_TRY(hipMemcpyAsync(p->info_host.data() + k0, p->ws.info, sizeof(int) * nb,
                                hipMemcpyDeviceToHost, st));
         if (lpress) {

@@ -158,6 +158,16 @@ int tbk_plan_stage_times(const tbk_plan *plan, int cap, const char **names,
  * either call clears the accumulated times. */
 int tbk_plan_set_timing(tbk_plan *plan, int on);
 
+/* GEMM tile autotuner (no reference counterpart: the reference calls
+ * unblocked LAPACK).  Runs one mode-4 pass of the plan to collect its GEMM
+ * shapes, times every tile for each shape not tuned yet (reps launches
+ * each) and keeps the fastest for the rest of the process.  The plan's
+ * results are undefined until its next run.  *ntuned (may be NULL) receives
+ * the number of shapes tuned. */
+int tbk_plan_autotune(tbk_plan *plan, int reps, int *ntuned);
+/* Tile used for a GEMM shape: 0 = 64x64, 1 = 96x96, 2 = 128x64, 3 = 32x32. */
+int tbk_gemm_tile(int precision, int opa, int opb, int m, int n, int k, int tri, int nb);
+
 /* ---------------------------------------------------------------------
  * Single-call drop-ins on host arrays.
  * --------------------------------------------------------------------- */
@@ -188,9 +198,12 @@ const char *tbk_error_string(int code);
 int tbk_debug_stages(int device, int n, const double *h, const double *s,
                      double *uinv, double *c, double *refl, double *d,
                      double *e, double *w, double *vt, double *z);
-int tbk_debug_zgemm(int device, int m, int n, int k, int opa, int opb,
-                    int tri, const double *a, int lda, const double *b,
-                    int ldb, double *c, int ldc);
+/* C = op(A) op(B) of column-major complex matrices (complex128, or
+ * complex64 when precision is TBK_MIXED) with the library's batched GEMM;
+ * tile -1 = the tuned / modelled choice, else 0..3 (see tbk_gemm_tile). */
+int tbk_debug_zgemm(int device, int precision, int tile, int m, int n, int k,
+                    int opa, int opb, int tri, const void *a, int lda,
+                    const void *b, int ldb, void *c, int ldc);
 
 #ifdef __cplusplus
 }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <vector>
 
 namespace tbk {
 
@@ -30,7 +31,8 @@ struct GemmArgsT {
     const C *a; int lda; size_t sa;
     const C *b; int ldb; size_t sb;
     C *c; int ldc; size_t sc;
-    int tri;
+    int tri;                       // 0 full; 1 A upper triangular; 2 A lower triangular;
+                                   // 3 B upper triangular; 4 lower triangle of C only
 };
 typedef GemmArgsT<double> GemmArgs;
 
@@ -50,7 +52,7 @@ struct SteinArgs {
     int n;
     double ortol_rel;              // group threshold relative to ||T||_1
     const double *d, *e;
-    double *w, *v, *scratch;       // w [nb][n]; v [nb][n*n]; scratch [nb][5*n*256]
+    double *w, *v, *scratch;       // w [nb][n]; v [nb][n*n]; scratch [nb][5*n*stein_threads(n)]
     int *info;
 };
 
@@ -72,6 +74,22 @@ struct FermiArgs {
     double *out;                   // efermi, earg1, earg2, esum, entropy, iters
 };
 
+// GEMM tiles and their choice (tbk_tune.cpp)
+constexpr int kGemmTiles = 4;      // 0 = 64x64, 1 = 96x96, 2 = 128x64, 3 = 32x32
+struct GemmShape {
+    int prec, opa, opb, m, n, k, tri, nb;   // prec 0 = complex FP64, 1 = complex FP32
+    GemmArgsT<double> g64;                  // arguments of the first launch seen
+    GemmArgsT<float> g32;
+};
+int gemm_tile(int prec, int opa, int opb, int m, int n, int k, int tri, int nb);
+int gemm_tile_model(int m, int n, int nb);
+void gemm_force_tile(int tile);            // -1: table / model
+void gemm_store_tile(const GemmShape &s, int tile);
+bool gemm_tuned(const GemmShape &s);
+void gemm_record(bool on);                  // collect the shapes of launches
+void gemm_note(const GemmShape &s);
+std::vector<GemmShape> gemm_recorded();
+
 void launch_phases(int npairs, const double *kpts, const double *dvec, double2 *ph, int nb,
                    hipStream_t st);
 void launch_build_hs(const BuildArgs &a, int natoms, int nb, hipStream_t st);
@@ -81,10 +99,12 @@ void launch_trtri_panel(int n, double2 *s, size_t stride, const double2 *wk, int
                         const int *info, int nb, hipStream_t st);
 void launch_zgemm(const GemmArgs &g, int opa, int opb, int nb, hipStream_t st);
 void launch_cgemm(const GemmArgsT<float> &g, int opa, int opb, int nb, hipStream_t st);
+void launch_gemm_tile(const GemmShape &s, int tile, hipStream_t st);   // a recorded shape, given tile
 void launch_hetrd_panel(const HetrdArgs &h, int i0, int nb, hipStream_t st);
 void launch_hetrd_finish(const HetrdArgs &h, int nb, hipStream_t st);
 void launch_hetrd_panel(const HetrdArgsT<float> &h, int i0, int nb, hipStream_t st);
 void launch_hetrd_finish(const HetrdArgsT<float> &h, int nb, hipStream_t st);
+int stein_threads(int n);        // eigenvector threads per k-point
 void launch_stein(const SteinArgs &s, int nb, hipStream_t st);
 void launch_real_to_complex_t(int n, const double *vt, double2 *y, size_t stride, int nb,
                               hipStream_t st);
@@ -92,8 +112,9 @@ void launch_c64_to_c32(size_t count, const double2 *x, float2 *y, hipStream_t st
 void launch_c32_to_c64(size_t count, const float2 *x, double2 *y, hipStream_t st);
 void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int nb, hipStream_t st);
 int oa_max_n();                  // largest N of launch_oa_update
-void launch_oa_update(int n, const double2 *t, const double2 *g, double2 *f, size_t stride, double *w,
-                      int nb, hipStream_t st);
+// Ogita-Aishima step: lam [nb][n], perm [nb][n], delta [nb] are scratch
+void launch_oa_update(int n, const double2 *t, const double2 *g, double2 *f, size_t stride, double *lam,
+                      int *perm, double *delta, double *w, int nb, hipStream_t st);
 void launch_coldot(int n, const double2 *z, const double2 *x, const double2 *y, size_t stride,
                    double2 *px, double2 *py, int nb, hipStream_t st);
 void launch_perturb(const PerturbArgs &p, hipStream_t st);

@@ -11,7 +11,7 @@
 //   k_zgemm         batched complex GEMM on the FP64 MFMA units
 //   k_hetrd_panel   blocked Householder tridiagonalisation (zhptrd,
 //                   lapack.f:3218) by panels; WY factors for the back-transform
-//   k_stein         bisection eigenvalues; eigenvectors by twisted
+//   k_stebz, k_stein_vec   bisection eigenvalues; eigenvectors by twisted
 //                   factorisation (isolated) or inverse iteration (groups)
 //   k_coldot, k_perturb   first-order derivatives (perturb.f:87-221)
 //   k_fermi_sums    Fermi level + band-energy / pressure sums
@@ -293,12 +293,14 @@ __global__ __launch_bounds__(256) void k_trtri_panel(int N, double2 *S, size_t s
 // ---------------------------------------------------------------------------
 // Batched complex GEMM on the FP64 matrix cores
 // ---------------------------------------------------------------------------
-// C = alpha * op(A) * op(B) + beta * C.  64x64 tile per workgroup of 256
-// threads (4 waves, 32x32 each = 2x2 v_mfma_f64_16x16x4 tiles), K staged
-// through LDS 16 at a time.  A complex product is four real MFMAs
-// (re*re - im*im, re*im + im*re).  tri: 1 = op(A) upper triangular (skip
-// k < row tile), 2 = op(A) lower (skip k beyond the row tile), 3 = op(B)
-// upper (skip k beyond the column tile).
+// C = alpha * op(A) * op(B) + beta * C.  BM x BN tile per workgroup of 256
+// threads (4 waves in 2 x 2, each (BM/2) x (BN/2) = MI x NJ blocks of the
+// 16x16x4 MFMA), K staged through LDS 16 at a time with the next slab
+// prefetched into registers while the matrix cores work on the current one.
+// A complex product is four real MFMAs (re*re - im*im, re*im + im*re).
+// tri: 1 = op(A) upper triangular (skip k < row tile), 2 = op(A) lower (skip
+// k beyond the row tile), 3 = op(B) upper (skip k beyond the column tile),
+// 4 = only tiles touching the lower triangle of C are computed.
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -315,65 +317,98 @@ __device__ __forceinline__ v4f mfma16(float a, float b, v4f c)
 __device__ __forceinline__ int mfma_row(double, int lane, int q) { return (lane >> 4) + 4 * q; }
 __device__ __forceinline__ int mfma_row(float, int lane, int q) { return 4 * (lane >> 4) + q; }
 
-template <class R, int OPA, int OPB>
+template <class R, int OPA, int OPB, int BM, int BN>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
 {
     typedef typename cplx<R>::type C;
     typedef R v4 __attribute__((ext_vector_type(4)));
-    __shared__ R As[2][16][64 + 1];  // [re/im][k][m]
-    __shared__ R Bs[2][16][64 + 1];  // [re/im][k][n]
-    const int bm = blockIdx.x * 64, bn = blockIdx.y * 64, kb = blockIdx.z;
+    constexpr int BK = 16, MI = BM / 32, NJ = BN / 32;
+    constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
+    __shared__ R As[2][BK][BM + 1];  // [re/im][k][m]
+    __shared__ R Bs[2][BK][BN + 1];  // [re/im][k][n]
+    const int bm = blockIdx.x * BM, bn = blockIdx.y * BN, kb = blockIdx.z;
+    if (g.tri == 4 && bm + BM <= bn) return;
     const C *A = g.a + (size_t)kb * g.sa;
     const C *B = g.b + (size_t)kb * g.sb;
     C *Cm = g.c + (size_t)kb * g.sc;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;
+    const int wm = (wv & 1) * (BM / 2), wn = (wv >> 1) * (BN / 2);
     int kbeg = 0, kend = g.k;
     if (g.tri == 1) kbeg = bm;
-    if (g.tri == 2) kend = min(g.k, bm + 64);
-    if (g.tri == 3) kend = min(g.k, bn + 64);
-    kbeg = (kbeg / 16) * 16;
-    v4 accr[2][2], acci[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) { accr[i][j] = v4{0, 0, 0, 0}; acci[i][j] = v4{0, 0, 0, 0}; }
-    for (int k0 = kbeg; k0 < kend; k0 += 16) {
-        __syncthreads();
-        for (int t = tid; t < 1024; t += 256) {
+    if (g.tri == 2) kend = min(g.k, bm + BM);
+    if (g.tri == 3) kend = min(g.k, bn + BN);
+    kbeg = (kbeg / BK) * BK;
+    v4 accr[MI][NJ], acci[MI][NJ];
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) { accr[i][j] = v4{0, 0, 0, 0}; acci[i][j] = v4{0, 0, 0, 0}; }
+    C ra[LA], rb[LB];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < LA; ++u) {
+            const int t = tid + 256 * u;
             int m, kk;
-            if (OPA == 0) { m = t & 63; kk = t >> 6; } else { kk = t & 15; m = t >> 4; }
+            if (OPA == 0) { m = t % BM; kk = t / BM; } else { kk = t % BK; m = t / BK; }
             const int gm = bm + m, gk = k0 + kk;
             C v = czero<C>();
             if (gm < g.m && gk < kend) {
                 if (OPA == 0) v = A[(size_t)gk * g.lda + gm];
                 else { v = A[(size_t)gm * g.lda + gk]; if (OPA == 1) v.y = -v.y; }
             }
-            As[0][kk][m] = v.x; As[1][kk][m] = v.y;
+            ra[u] = v;
         }
-        for (int t = tid; t < 1024; t += 256) {
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+            const int t = tid + 256 * u;
             int n, kk;
-            if (OPB == 0) { kk = t & 15; n = t >> 4; } else { n = t & 63; kk = t >> 6; }
+            if (OPB == 0) { kk = t % BK; n = t / BK; } else { n = t % BN; kk = t / BN; }
             const int gn = bn + n, gk = k0 + kk;
             C v = czero<C>();
             if (gn < g.n && gk < kend) {
                 if (OPB == 0) v = B[(size_t)gn * g.ldb + gk];
                 else { v = B[(size_t)gk * g.ldb + gn]; if (OPB == 1) v.y = -v.y; }
             }
-            Bs[0][kk][n] = v.x; Bs[1][kk][n] = v.y;
+            rb[u] = v;
+        }
+    };
+    if (kbeg < kend) load(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < LA; ++u) {
+            const int t = tid + 256 * u;
+            int m, kk;
+            if (OPA == 0) { m = t % BM; kk = t / BM; } else { kk = t % BK; m = t / BK; }
+            As[0][kk][m] = ra[u].x; As[1][kk][m] = ra[u].y;
+        }
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+            const int t = tid + 256 * u;
+            int n, kk;
+            if (OPB == 0) { kk = t % BK; n = t / BK; } else { n = t % BN; kk = t / BN; }
+            Bs[0][kk][n] = rb[u].x; Bs[1][kk][n] = rb[u].y;
         }
         __syncthreads();
-        for (int ks = 0; ks < 16; ks += 4) {
+        if (k0 + BK < kend) load(k0 + BK);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
             const int kk = ks + (lane >> 4);
-            R ar[2], ai[2], br[2], bi[2];
-            for (int i = 0; i < 2; ++i) {
+            R ar[MI], ai[MI], br[NJ], bi[NJ];
+#pragma unroll
+            for (int i = 0; i < MI; ++i) {
                 ar[i] = As[0][kk][wm + 16 * i + (lane & 15)];
                 ai[i] = As[1][kk][wm + 16 * i + (lane & 15)];
             }
-            for (int j = 0; j < 2; ++j) {
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
                 br[j] = Bs[0][kk][wn + 16 * j + (lane & 15)];
                 bi[j] = Bs[1][kk][wn + 16 * j + (lane & 15)];
             }
-            for (int i = 0; i < 2; ++i)
-                for (int j = 0; j < 2; ++j) {
+#pragma unroll
+            for (int i = 0; i < MI; ++i)
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) {
                     accr[i][j] = mfma16(ar[i], br[j], accr[i][j]);
                     accr[i][j] = mfma16(-ai[i], bi[j], accr[i][j]);
                     acci[i][j] = mfma16(ar[i], bi[j], acci[i][j]);
@@ -381,8 +416,12 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
                 }
         }
     }
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j)
+    const bool usebeta = g.beta.x != R(0) || g.beta.y != R(0);
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+#pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int r = bm + wm + 16 * i + mfma_row(R(0), lane, q);
                 const int c = bn + wn + 16 * j + (lane & 15);
@@ -390,7 +429,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
                     const C v = cmake<C, R>(accr[i][j][q], acci[i][j][q]);
                     C out = zmul(g.alpha, v);
                     C *p = Cm + (size_t)c * g.ldc + r;
-                    if (g.beta.x != R(0) || g.beta.y != R(0)) out = zadd(out, zmul(g.beta, *p));
+                    if (usebeta) out = zadd(out, zmul(g.beta, *p));
                     *p = out;
                 }
             }
@@ -407,8 +446,11 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
 //   the reflectors in A (A[j+1, j] = 1, v below, zero at and above j),
 //   the panel's upper triangular WY factor T (Q_p = I - V T V^H),
 //   d[j], e[j] (the real tridiagonal).
-// The symmetric matrix-vector product y = A_t v is evaluated as dot products
-// y_r = sum_c conj(A[c, r]) v_c over contiguous columns (A is stored in full).
+// Only the lower triangle of the trailing matrix is read or updated: the
+// Hermitian product y = A_t v is formed column by column from A[r:, r], the
+// wave owning column r taking the dot conj(A[r:, r])^T v[r:] (-> y_r) and
+// accumulating A[r+1:, r] v_r into its private partial y (registers).  The
+// partials are combined in a fixed order, so results are deterministic.
 constexpr int HT = 512;            // threads of the panel kernel
 constexpr int HW = HT / 64;        // waves
 
@@ -421,14 +463,20 @@ __device__ __forceinline__ double2 wave_sum2(double2 v)
     return v;
 }
 
-template <class R>
-__global__ __launch_bounds__(HT) void k_hetrd_panel(HetrdArgsT<R> h, int i0)
+// dynamic LDS of k_hetrd_panel: vloc[N], ybuf[N], 4 partial-y slots [4][N]
+template <class R> constexpr size_t hetrd_lds_bytes(int n) { return (size_t)6 * n * 2 * sizeof(R); }
+
+template <class R, int MAXQ>
+__global__ __launch_bounds__(HT, (MAXQ <= 5 ? 4 : 2)) void k_hetrd_panel(HetrdArgsT<R> h, int i0)
 {
     typedef typename cplx<R>::type C;
-    __shared__ C vloc[1280];           // reflector of the current column
-    __shared__ C ybuf[1280];           // A_t v
+    extern __shared__ double4 hetrd_smem[];
+    C *vloc = reinterpret_cast<C *>(hetrd_smem);     // reflector of the current column
+    C *ybuf = vloc + h.n;                              // dot part of A_t v
+    C *slot = ybuf + h.n;                              // [4][N] partial sums of A_t v
     __shared__ C udot[2 * NB];         // W^H v (0..31) and V^H v (32..63)
-    __shared__ C uw[NB], uv[NB];       // conj(W[j, l]), conj(V[j, l])
+    __shared__ C uwn[NB], uvn[NB];     // conj(W[j+1, l]), conj(V[j+1, l])
+    __shared__ C Ts[NB][NB + 1];       // WY factor T of the panel, Ts[col][row]
     __shared__ R red[2 * HW];
     __shared__ C scal[2];              // tau, (beta, 0)
     const int N = h.n;
@@ -442,20 +490,24 @@ __global__ __launch_bounds__(HT) void k_hetrd_panel(HetrdArgsT<R> h, int i0)
     const int pw = min(NB, N - 1 - i0);
     C *V = P, *W = P + (size_t)NB * N, *V2 = P + (size_t)2 * NB * N;
     for (int t = tid; t < N * 96; t += HT) P[t] = czero<C>();
-    for (int t = tid; t < NB * NB; t += HT) Tp[t] = czero<C>();
+    for (int t = tid; t < NB * (NB + 1); t += HT) Ts[t / (NB + 1)][t % (NB + 1)] = czero<C>();
+    // Rows are owned by threads in a fixed map (r = tid + HT s); carry[s] is
+    // the delayed update V[r,:] conj(W[j,:]) + W[r,:] conj(V[j,:]) of the
+    // current column j, accumulated while the previous column's w was formed.
+    constexpr int RPT = (64 * MAXQ + HT - 1) / HT;   // rows per thread
+    C carry[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) carry[q] = czero<C>();
     __syncthreads();
     for (int i = 0; i < pw; ++i) {
         const int j = i0 + i, t0 = j + 1, m = N - t0;
-        // (1) column j brought up to date: a_r -= V[r,:] conj(W[j,:]) + W[r,:] conj(V[j,:])
-        if (tid < i) { uw[tid] = zconj(W[(size_t)tid * N + j]); uv[tid] = zconj(V[(size_t)tid * N + j]); }
-        __syncthreads();
+        // (1) column j brought up to date: a_r = A[r, j] - carry_r
         R xs = 0;
-        for (int r = j + tid; r < N; r += HT) {
-            C a = A[(size_t)j * N + r];
-            for (int l = 0; l < i; ++l) {
-                a = zsub(a, zmul(V[(size_t)l * N + r], uw[l]));
-                a = zsub(a, zmul(W[(size_t)l * N + r], uv[l]));
-            }
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = tid + HT * q;
+            if (r < j || r >= N) continue;
+            C a = zsub(A[(size_t)j * N + r], carry[q]);
             if (r == j) a.y = 0;
             vloc[r] = a;
             if (r >= j + 2) xs += a.x * a.x + a.y * a.y;
@@ -500,14 +552,63 @@ __global__ __launch_bounds__(HT) void k_hetrd_panel(HetrdArgsT<R> h, int i0)
             }
         }
         __syncthreads();
-        // (3) dot products, one per wave task: rows r (y_r = col_r^H v) and
-        //     the 2i panel dots W[:,l]^H v, V[:,l]^H v over rows >= t0
-        const int ntask = m + 2 * i;
-        for (int task = wv; task < ntask; task += HW) {
-            const C *col;
-            if (task < m) col = A + (size_t)(t0 + task) * N;
-            else if (task < m + i) col = W + (size_t)(task - m) * N;
-            else col = V + (size_t)(task - m - i) * N;
+        // (3) y = A_t v from the lower triangle (columns r >= t0), and the 2i
+        //     panel dots W[:,l]^H v, V[:,l]^H v over rows >= t0, one wave task each
+        C ypart[MAXQ];
+#pragma unroll
+        for (int q = 0; q < MAXQ; ++q) ypart[q] = czero<C>();
+        const int nq = (N + 63) >> 6;
+        // columns r = t0 + tb + HW g (g < HG) per wave pass: HG x QC loads in
+        // flight per lane and HG independent wave reductions
+        constexpr int HG = sizeof(R) == 8 ? 2 : 4, QC = sizeof(R) == 8 ? 3 : 4;
+        for (int tb = wv; tb < m; tb += HW * HG) {
+            C acc[HG], vr[HG];
+            int rg[HG];
+#pragma unroll
+            for (int g = 0; g < HG; ++g) {
+                rg[g] = t0 + tb + HW * g;
+                acc[g] = czero<C>();
+                vr[g] = rg[g] < N ? vloc[rg[g]] : czero<C>();
+            }
+#pragma unroll
+            for (int q0 = 0; q0 < MAXQ; q0 += QC) {
+                if (q0 < nq) {
+                    C a[HG][QC];
+#pragma unroll
+                    for (int g = 0; g < HG; ++g)
+#pragma unroll
+                        for (int u = 0; u < QC; ++u) {
+                            const int c = ((q0 + u) << 6) + lane;
+                            a[g][u] = (c >= rg[g] && c < N) ? A[(size_t)rg[g] * N + c] : czero<C>();
+                        }
+#pragma unroll
+                    for (int u = 0; u < QC; ++u) {
+                        const int c = ((q0 + u) << 6) + lane;
+                        if (q0 + u < MAXQ) {
+                            const C vc = c < N ? vloc[c] : czero<C>();
+#pragma unroll
+                            for (int g = 0; g < HG; ++g) {
+                                acc[g] = zadd(acc[g], zcmul(a[g][u], vc));
+                                if (c > rg[g]) ypart[q0 + u] = zadd(ypart[q0 + u], zmul(a[g][u], vr[g]));
+                            }
+                        }
+                    }
+                }
+            }
+            for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+                for (int g = 0; g < HG; ++g) {
+                    acc[g].x += __shfl_xor(acc[g].x, off, 64);
+                    acc[g].y += __shfl_xor(acc[g].y, off, 64);
+                }
+            if (lane == 0)
+#pragma unroll
+                for (int g = 0; g < HG; ++g)
+                    if (rg[g] < N) ybuf[rg[g]] = acc[g];
+        }
+        // panel dots W[:,l]^H v (udot[l]) and V[:,l]^H v (udot[NB + l])
+        for (int task = wv; task < 2 * i; task += HW) {
+            const C *col = task < i ? W + (size_t)task * N : V + (size_t)(task - i) * N;
             C acc = czero<C>(), acc2 = czero<C>();
             int c = t0 + lane;
             for (; c + 64 < N; c += 128) {
@@ -521,25 +622,57 @@ __global__ __launch_bounds__(HT) void k_hetrd_panel(HetrdArgsT<R> h, int i0)
                 acc.x += __shfl_xor(acc.x, off, 64);
                 acc.y += __shfl_xor(acc.y, off, 64);
             }
-            if (lane == 0) {
-                if (task < m) ybuf[t0 + task] = acc;
-                else udot[task - m + (task < m + i ? 0 : NB - i)] = acc;
+            if (lane == 0) udot[task < i ? task : NB + task - i] = acc;
+        }
+        if (tid < i && t0 < N) {
+            uwn[tid] = zconj(W[(size_t)tid * N + t0]);
+            uvn[tid] = zconj(V[(size_t)tid * N + t0]);
+        }
+        // partials: slot[w] = part_w + part_{w+4} for w < 4
+        if (wv >= 4) {
+#pragma unroll
+            for (int q = 0; q < MAXQ; ++q) {
+                const int c = (q << 6) + lane;
+                if (q < nq && c >= t0 && c < N) slot[(size_t)(wv - 4) * N + c] = ypart[q];
+            }
+        }
+        __syncthreads();
+        if (wv < 4) {
+#pragma unroll
+            for (int q = 0; q < MAXQ; ++q) {
+                const int c = (q << 6) + lane;
+                if (q < nq && c >= t0 && c < N) {
+                    const size_t o = (size_t)wv * N + c;
+                    slot[o] = zadd(ypart[q], slot[o]);
+                }
             }
         }
         __syncthreads();
         // (4) w = tau (y - V u1 - W u2), u1 = W^H v, u2 = V^H v; then the
-        //     correction w += alpha v with alpha = -tau/2 (w^H v)
+        //     correction w += alpha v with alpha = -tau/2 (w^H v).  The same
+        //     pass over V[r, 0:i], W[r, 0:i] accumulates the next column's
+        //     delayed update into carry.
         R pr = 0, pi = 0;
-        for (int r = t0 + tid; r < N; r += HT) {
-            C y = ybuf[r];
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = tid + HT * q;
+            if (r < t0 || r >= N) continue;
+            C y = zadd(zadd(zadd(zadd(ybuf[r], slot[r]), slot[(size_t)N + r]), slot[(size_t)2 * N + r]),
+                       slot[(size_t)3 * N + r]);
+            C nx = czero<C>();
+#pragma unroll 2
             for (int l = 0; l < i; ++l) {
-                y = zsub(y, zmul(V[(size_t)l * N + r], udot[l]));
-                y = zsub(y, zmul(W[(size_t)l * N + r], udot[NB + l]));
+                const C vl = V[(size_t)l * N + r], wl = W[(size_t)l * N + r];
+                y = zsub(y, zmul(vl, udot[l]));
+                y = zsub(y, zmul(wl, udot[NB + l]));
+                nx = zadd(nx, zmul(vl, uwn[l]));
+                nx = zadd(nx, zmul(wl, uvn[l]));
             }
+            carry[q] = nx;
             const C w = zmul(tau, y);
             ybuf[r] = w;
-            const C q = zcmul(w, vloc[r]);
-            pr += q.x; pi += q.y;
+            const C qv = zcmul(w, vloc[r]);
+            pr += qv.x; pi += qv.y;
         }
         for (int off = 32; off > 0; off >>= 1) {
             pr += __shfl_xor(pr, off, 64);
@@ -551,16 +684,26 @@ __global__ __launch_bounds__(HT) void k_hetrd_panel(HetrdArgsT<R> h, int i0)
         R sr = 0, si = 0;
         for (int q = 0; q < HW; ++q) { sr += red[q]; si += red[HW + q]; }
         const C alpha = zscale(R(-0.5), zmul(tau, cmake<C, R>(sr, si)));
-        for (int r = t0 + tid; r < N; r += HT) W[(size_t)i * N + r] = zadd(ybuf[r], zmul(alpha, vloc[r]));
+        // W[j+1, i] (v_{j+1} = 1) for the l = i term of the next column's update
+        const C wnext = t0 < N ? zadd(ybuf[t0], alpha) : czero<C>();
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = tid + HT * q;
+            if (r < t0 || r >= N) continue;
+            const C wr = zadd(ybuf[r], zmul(alpha, vloc[r]));
+            W[(size_t)i * N + r] = wr;
+            carry[q] = zadd(carry[q], zadd(zmul(vloc[r], zconj(wnext)), wr));
+        }
         // (5) WY factor: T[0:i, i] = -tau T[0:i, 0:i] (V^H v), T[i, i] = tau
         if (tid < i) {
             C acc = czero<C>();
-            for (int l = tid; l < i; ++l) acc = zadd(acc, zmul(Tp[(size_t)l * NB + tid], udot[NB + l]));
-            Tp[(size_t)i * NB + tid] = zmul(cmake<C, R>(-tau.x, -tau.y), acc);
+            for (int l = tid; l < i; ++l) acc = zadd(acc, zmul(Ts[l][tid], udot[NB + l]));
+            Ts[i][tid] = zmul(cmake<C, R>(-tau.x, -tau.y), acc);
         }
-        if (tid == 0) Tp[(size_t)i * NB + i] = tau;
+        if (tid == 0) Ts[i][i] = tau;
         __syncthreads();
     }
+    for (int t = tid; t < NB * NB; t += HT) Tp[t] = Ts[t / NB][t % NB];
     // V T^H of the panel (rows > i0), for the back-transform
     // Q_p Y = Y - V ((V T^H)^H Y)
     C *VT = h.vt + (size_t)kb * h.stride + (size_t)i0 * N;
@@ -568,7 +711,7 @@ __global__ __launch_bounds__(HT) void k_hetrd_panel(HetrdArgsT<R> h, int i0)
         const int r = t % N, c = t / N;
         C acc = czero<C>();
         if (c < pw && r > i0)
-            for (int l = c; l < pw; ++l) acc = zadd(acc, zmul(V[(size_t)l * N + r], zconj(Tp[(size_t)l * NB + c])));
+            for (int l = c; l < pw; ++l) acc = zadd(acc, zmul(V[(size_t)l * N + r], zconj(Ts[l][c])));
         if (i0 + c < N) VT[(size_t)c * N + r] = acc;
     }
 }
@@ -591,6 +734,16 @@ __global__ void k_hetrd_finish(HetrdArgsT<R> h)
 // ---------------------------------------------------------------------------
 // Tridiagonal eigenpairs: bisection + inverse iteration, one WG per k
 // ---------------------------------------------------------------------------
+// 1/x from the hardware reciprocal refined by one Newton step (relative
+// error a few ulp); x is never 0 here (pivots are clamped to -pivmin).
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+
+// Number of eigenvalues of the tridiagonal below x (Sturm count of the
+// LDL^T pivots, dstebz's dlaebz with pivmin clamping).
 __device__ int sturm_count(const double *d, const double *e2, int n, double x, double pivmin)
 {
     int cnt = 0;
@@ -598,7 +751,7 @@ __device__ int sturm_count(const double *d, const double *e2, int n, double x, d
     if (fabs(q) <= pivmin) q = -pivmin;
     cnt += q < 0.0;
     for (int i = 1; i < n; ++i) {
-        q = d[i] - x - e2[i - 1] / q;
+        q = d[i] - x - e2[i - 1] * rcp_nr(q);
         if (fabs(q) <= pivmin) q = -pivmin;
         cnt += q < 0.0;
     }
@@ -613,64 +766,105 @@ __device__ __forceinline__ double start_entry(int i, int j)
     return (double)x * (2.0 / 4294967296.0) - 1.0;
 }
 
-__global__ __launch_bounds__(256) void k_stein(SteinArgs s)
+// Gershgorin interval, 1-norm (dstein's onenrm) and pivmin of the
+// tridiagonal held in LDS (d, e2 = e^2, e), reduced over the block.
+struct TriBounds { double gl, gu, onenrm, pivmin; };
+__device__ TriBounds tri_bounds(const double *d, const double *e, const double *e2, int N, double *red4)
 {
-    __shared__ double d[1280], e[1280], e2[1280], w[1280];
-    __shared__ int cstart[1281];
-    __shared__ int ncl;
-    __shared__ double red[16];
-    const int N = s.n;
-    const int kb = blockIdx.x;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double lo = DBL_MAX, hi = -DBL_MAX, nrm = 0.0, emax = 0.0;
+    for (int i = tid; i < N; i += nt) {
+        const double r = fabs(e[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0);
+        lo = fmin(lo, d[i] - r); hi = fmax(hi, d[i] + r);
+        nrm = fmax(nrm, fabs(d[i]) + r);
+        emax = fmax(emax, e2[i]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, off, 64)); hi = fmax(hi, __shfl_xor(hi, off, 64));
+        nrm = fmax(nrm, __shfl_xor(nrm, off, 64)); emax = fmax(emax, __shfl_xor(emax, off, 64));
+    }
+    const int lane = tid & 63, wv = tid >> 6, nw = (nt + 63) >> 6;
+    __syncthreads();
+    if (lane == 0) { red4[4 * wv] = lo; red4[4 * wv + 1] = hi; red4[4 * wv + 2] = nrm; red4[4 * wv + 3] = emax; }
+    __syncthreads();
+    lo = red4[0]; hi = red4[1]; nrm = red4[2]; emax = red4[3];
+    for (int q = 1; q < nw; ++q) {
+        lo = fmin(lo, red4[4 * q]); hi = fmax(hi, red4[4 * q + 1]);
+        nrm = fmax(nrm, red4[4 * q + 2]); emax = fmax(emax, red4[4 * q + 3]);
+    }
+    __syncthreads();
+    TriBounds b;
+    b.onenrm = nrm;
+    b.pivmin = DBL_MIN * fmax(1.0, emax);
+    const double eps = DBL_EPSILON * 0.5;                 // dlamch('E')
+    b.gl = lo - 2.0 * eps * nrm - b.pivmin;
+    b.gu = hi + 2.0 * eps * nrm + b.pivmin;
+    return b;
+}
+
+// Eigenvalues of the tridiagonal by bisection on Sturm counts, one thread
+// per eigenvalue (dstebz with the absolute tolerance 2 eps max(|a|,|b|)).
+__global__ __launch_bounds__(256) void k_stebz(SteinArgs s)
+{
+    __shared__ double d[1280], e[1280], e2[1280];
+    __shared__ double red4[64];
+    const int N = s.n, kb = blockIdx.y, tid = threadIdx.x;
     const double *dg = s.d + (size_t)kb * N;
     const double *eg = s.e + (size_t)kb * N;
-    double *Vt = s.v + (size_t)kb * N * N;            // Vt[i*N + j] = v_j(i)
-    double *scr = s.scratch + (size_t)kb * 5 * N * blockDim.x;
     for (int i = tid; i < N; i += blockDim.x) {
         d[i] = dg[i];
         e[i] = (i < N - 1) ? eg[i] : 0.0;
         e2[i] = e[i] * e[i];
     }
     __syncthreads();
-    // Gershgorin bounds and the 1-norm (dstein's onenrm)
-    double lo = DBL_MAX, hi = -DBL_MAX, nrm = 0.0, emax = 0.0;
+    const TriBounds tb = tri_bounds(d, e, e2, N, red4);
+    const double eps = DBL_EPSILON * 0.5;
+    if (blockIdx.x == 0 && tid == 0) s.info[kb] = 0;
+    const int j = blockIdx.x * blockDim.x + tid;
+    if (j >= N) return;
+    double a = tb.gl, b = tb.gu;
+    for (int it = 0; it < 200; ++it) {
+        const double mid = 0.5 * (a + b);
+        const double tol = 2.0 * eps * fmax(fabs(a), fabs(b)) + tb.pivmin;
+        if (b - a <= tol || mid <= a || mid >= b) break;
+        if (sturm_count(d, e2, N, mid, tb.pivmin) >= j + 1) b = mid; else a = mid;
+    }
+    s.w[(size_t)kb * N + j] = 0.5 * (a + b);
+}
+
+// Eigenvectors of the tridiagonal, one thread per group of close
+// eigenvalues (consecutive gaps below ortol_rel * onenrm), 64 groups per
+// workgroup.  Isolated and well-separated eigenvalues: twisted
+// factorisation of T - lambda I (L D+ L^T from the top, U D- U^T from the
+// bottom, twist r at min |gamma_r|, z_r = 1 and the two bidiagonal
+// recurrences), orthogonalised within the group.  Near-exact degeneracies
+// (gap <= 1e-10 onenrm): inverse iteration from a distinct start vector
+// with reorthogonalisation (dstein).
+__global__ __launch_bounds__(64) void k_stein_vec(SteinArgs s)
+{
+    __shared__ double d[1280], e[1280], e2[1280], w[1280];
+    __shared__ int cstart[1281];
+    __shared__ int ncl;
+    __shared__ double red4[16];
+    const int N = s.n;
+    const int kb = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int nthr = gridDim.x * blockDim.x;                  // threads per k
+    const int gt = blockIdx.x * blockDim.x + tid;
+    const double *dg = s.d + (size_t)kb * N;
+    const double *eg = s.e + (size_t)kb * N;
+    double *Vt = s.v + (size_t)kb * N * N;            // Vt[i*N + j] = v_j(i)
+    double *scr = s.scratch + (size_t)kb * 5 * N * nthr;
     for (int i = tid; i < N; i += blockDim.x) {
-        const double r = fabs(e[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0);
-        lo = fmin(lo, d[i] - r); hi = fmax(hi, d[i] + r);
-        nrm = fmax(nrm, fabs(d[i]) + r);
-        emax = fmax(emax, e2[i]);
-    }
-    // block min / max via the sum helper on transformed values is not
-    // order-free; use shared atomics-free reduction instead
-    __shared__ double rlo[256], rhi[256], rn[256], re[256];
-    rlo[tid] = lo; rhi[tid] = hi; rn[tid] = nrm; re[tid] = emax;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if (tid < st) {
-            rlo[tid] = fmin(rlo[tid], rlo[tid + st]); rhi[tid] = fmax(rhi[tid], rhi[tid + st]);
-            rn[tid] = fmax(rn[tid], rn[tid + st]); re[tid] = fmax(re[tid], re[tid + st]);
-        }
-        __syncthreads();
-    }
-    const double onenrm = rn[0];
-    const double pivmin = DBL_MIN * fmax(1.0, re[0]);
-    const double eps = DBL_EPSILON * 0.5;                 // dlamch('E')
-    const double gl = rlo[0] - 2.0 * eps * onenrm - pivmin;
-    const double gu = rhi[0] + 2.0 * eps * onenrm + pivmin;
-    // bisection for eigenvalue index j (ascending)
-    for (int j = tid; j < N; j += blockDim.x) {
-        double a = gl, b = gu;
-        for (int it = 0; it < 200; ++it) {
-            const double mid = 0.5 * (a + b);
-            const double tol = 2.0 * eps * fmax(fabs(a), fabs(b)) + pivmin;
-            if (b - a <= tol || mid <= a || mid >= b) break;
-            if (sturm_count(d, e2, N, mid, pivmin) >= j + 1) b = mid; else a = mid;
-        }
-        w[j] = 0.5 * (a + b);
+        d[i] = dg[i];
+        e[i] = (i < N - 1) ? eg[i] : 0.0;
+        e2[i] = e[i] * e[i];
+        w[i] = s.w[(size_t)kb * N + i];
     }
     __syncthreads();
-    for (int j = tid; j < N; j += blockDim.x) s.w[(size_t)kb * N + j] = w[j];
-    // orthogonalisation groups: consecutive eigenvalues closer than ortol
+    const TriBounds tb = tri_bounds(d, e, e2, N, red4);
+    const double onenrm = tb.onenrm, pivmin = tb.pivmin;
+    const double eps = DBL_EPSILON * 0.5;
     const double ortol = s.ortol_rel * onenrm;
     if (tid == 0) {
         int nc = 0;
@@ -680,60 +874,65 @@ __global__ __launch_bounds__(256) void k_stein(SteinArgs s)
         ncl = nc;
     }
     __syncthreads();
-    // per-thread LU workspace: columns a, b, c, dd, y of length N
-    const int nthr = blockDim.x;
     double *wa = scr + 0 * (size_t)N * nthr, *wb = scr + 1 * (size_t)N * nthr;
     double *wc = scr + 2 * (size_t)N * nthr, *wd = scr + 3 * (size_t)N * nthr;
     double *wy = scr + 4 * (size_t)N * nthr;
-#define AT(arr, i) arr[(size_t)(i) * nthr + tid]
+#define AT(arr, i) arr[(size_t)(i) * nthr + gt]
     const double dtpcrt = sqrt(0.1 / N);
     int fails = 0;
-    for (int cl = tid; cl < ncl; cl += nthr) {
+    const int cl = gt;
+    if (cl < ncl) {
         const int j0 = cstart[cl], j1 = cstart[cl + 1];
         double xjm = 0.0;
         for (int j = j0; j < j1; ++j) {
             if (j == j0 || w[j] - w[j - 1] > 1e-10 * onenrm) {
-                // Twisted factorisation of T - lambda I: L D L^T from the
-                // top (stationary qd), U D U^T from the bottom; the twist
-                // index r minimises |gamma_r| and the vector follows from
-                // z_r = 1 by the two bidiagonal recurrences.
+                // D+ (top-down) into scratch; U_i = e_i / D-_{i+1} (bottom-up)
+                // into slot i+1 of the output column; z overwrites both.
                 const double lam = w[j];
-                double *wl = wa, *wp = wb, *wu = wc;     // L, D+, U per thread
                 double dp = d[0] - lam;
                 if (fabs(dp) < pivmin) dp = -pivmin;
-                AT(wp, 0) = dp;
+                AT(wa, 0) = dp;
                 for (int i = 0; i < N - 1; ++i) {
-                    const double l = e[i] / dp;
-                    AT(wl, i) = l;
+                    const double l = e[i] * rcp_nr(dp);
                     dp = d[i + 1] - lam - l * e[i];
                     if (fabs(dp) < pivmin) dp = -pivmin;
-                    AT(wp, i + 1) = dp;
+                    AT(wa, i + 1) = dp;
                 }
                 double dm = d[N - 1] - lam;
                 if (fabs(dm) < pivmin) dm = -pivmin;
-                double gbest = fabs(AT(wp, N - 1) + dm - (d[N - 1] - lam));
+                double gbest = fabs(AT(wa, N - 1) + dm - (d[N - 1] - lam));
                 int r = N - 1;
                 for (int i = N - 2; i >= 0; --i) {
-                    const double u = e[i] / dm;
-                    AT(wu, i) = u;
+                    const double u = e[i] * rcp_nr(dm);
+                    Vt[(size_t)(i + 1) * N + j] = u;
                     dm = d[i] - lam - u * e[i];
                     if (fabs(dm) < pivmin) dm = -pivmin;
-                    const double g = fabs(AT(wp, i) + dm - (d[i] - lam));
+                    const double g = fabs(AT(wa, i) + dm - (d[i] - lam));
                     if (g < gbest) { gbest = g; r = i; }
                 }
-                double z = 1.0;
+                double z = 1.0, ss = 1.0;
                 Vt[(size_t)r * N + j] = 1.0;
-                for (int i = r - 1; i >= 0; --i) { z = -AT(wl, i) * z; Vt[(size_t)i * N + j] = z; }
-                z = 1.0;
-                for (int i = r; i < N - 1; ++i) { z = -AT(wu, i) * z; Vt[(size_t)(i + 1) * N + j] = z; }
-                // orthogonalise against the group's earlier vectors
-                for (int jp = j0; jp < j; ++jp) {
-                    double dot = 0.0;
-                    for (int i = 0; i < N; ++i) dot += Vt[(size_t)i * N + j] * Vt[(size_t)i * N + jp];
-                    for (int i = 0; i < N; ++i) Vt[(size_t)i * N + j] -= dot * Vt[(size_t)i * N + jp];
+                for (int i = r; i < N - 1; ++i) {
+                    z = -Vt[(size_t)(i + 1) * N + j] * z;
+                    Vt[(size_t)(i + 1) * N + j] = z;
+                    ss += z * z;
                 }
-                double ss = 0.0;
-                for (int i = 0; i < N; ++i) ss += Vt[(size_t)i * N + j] * Vt[(size_t)i * N + j];
+                z = 1.0;
+                for (int i = r - 1; i >= 0; --i) {
+                    z = -(e[i] * rcp_nr(AT(wa, i))) * z;
+                    Vt[(size_t)i * N + j] = z;
+                    ss += z * z;
+                }
+                if (j > j0) {
+                    // orthogonalise against the group's earlier vectors
+                    for (int jp = j0; jp < j; ++jp) {
+                        double dot = 0.0;
+                        for (int i = 0; i < N; ++i) dot += Vt[(size_t)i * N + j] * Vt[(size_t)i * N + jp];
+                        for (int i = 0; i < N; ++i) Vt[(size_t)i * N + j] -= dot * Vt[(size_t)i * N + jp];
+                    }
+                    ss = 0.0;
+                    for (int i = 0; i < N; ++i) ss += Vt[(size_t)i * N + j] * Vt[(size_t)i * N + j];
+                }
                 const double scl = 1.0 / sqrt(ss);
                 for (int i = 0; i < N; ++i) Vt[(size_t)i * N + j] *= scl;
                 xjm = lam;
@@ -872,9 +1071,9 @@ __global__ __launch_bounds__(256) void k_stein(SteinArgs s)
         }
     }
 #undef AT
-    const double f = block_sum((double)fails, red);
-    if (tid == 0) s.info[kb] = (int)f;
+    if (fails) atomicAdd(s.info + kb, fails);
 }
+
 
 // Complex copy of the real row-major eigenvectors: Y[:, j] = v_j.
 __global__ void k_real_to_complex_t(int N, const double *Vt, double2 *Y, size_t stride)
@@ -922,32 +1121,61 @@ __global__ void k_real_to_c32_t(int N, const double *Vt, float2 *Y, size_t strid
 //   delta    = 2 (||T - diag(lambda)||_F + max|lambda| ||R||_F),
 //   E_ij     = (T_ij + lambda_j R_ij) / (lambda_j - lambda_i)  if |lambda_i - lambda_j| > delta,
 //              R_ij / 2                                        otherwise,
-// and the refined vectors are Y (I + E).  The kernel writes F = I + E with
-// its columns in ascending order of lambda (F[:, j] = (I + E)[:, perm_j]), so
-// that Y F is the refined, sorted eigenvector matrix, and lambda sorted into w.
+// and the refined vectors are Y (I + E).
+// k_oa_stats (one workgroup per k) forms lambda, delta and the ascending
+// order perm of lambda; k_oa_fmat writes F = I + E with its columns in that
+// order (F[:, j] = (I + E)[:, perm_j]), so that Y F is the refined, sorted
+// eigenvector matrix.
 constexpr int kOaMax = 1536;
-__global__ __launch_bounds__(256) void k_oa_update(int N, const double2 *T, const double2 *G,
-                                                   double2 *F, size_t stride, double *w)
+__global__ __launch_bounds__(1024) void k_oa_stats(int N, const double2 *T, const double2 *G, size_t stride,
+                                                   double *lam_out, int *perm, double *delta_out, double *w)
 {
     __shared__ double lam[kOaMax], lsort[kOaMax];
     __shared__ int idx[kOaMax];
-    __shared__ double red[256];
-    const int kb = blockIdx.x, tid = threadIdx.x;
+    __shared__ double red[3][16];
+    const int kb = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const double2 *Tk = T + (size_t)kb * stride;
     const double2 *Gk = G + (size_t)kb * stride;
-    double2 *Fk = F + (size_t)kb * stride;
     double lmax = 0.0;
     for (int i = tid; i < N; i += blockDim.x) {
         const double l = Tk[(size_t)i * N + i].x / Gk[(size_t)i * N + i].x;   // G_ii = 1 - R_ii
         lam[i] = l; lsort[i] = l; idx[i] = i;
         lmax = fmax(lmax, fabs(l));
     }
-    red[tid] = lmax;
     __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) { if (tid < st) red[tid] = fmax(red[tid], red[tid + st]); __syncthreads(); }
-    lmax = red[0];
+    double ft = 0.0, fr = 0.0;
+    const size_t nn = (size_t)N * N;
+    for (size_t t = tid; t < nn; t += 4 * blockDim.x) {
+        double2 tv[4], gv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const size_t tt = t + (size_t)u * blockDim.x;
+            tv[u] = tt < nn ? Tk[tt] : make_double2(0.0, 0.0);
+            gv[u] = tt < nn ? Gk[tt] : make_double2(1.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const size_t tt = t + (size_t)u * blockDim.x;
+            const int r = (int)(tt % N), c = (int)(tt / N);
+            double2 rv = make_double2(-gv[u].x, -gv[u].y);
+            if (tt < nn && r == c) { tv[u].x -= lam[r]; rv.x += 1.0; }
+            if (tt >= nn) rv.x += 1.0;
+            ft += tv[u].x * tv[u].x + tv[u].y * tv[u].y;
+            fr += rv.x * rv.x + rv.y * rv.y;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ft += __shfl_xor(ft, off, 64);
+        fr += __shfl_xor(fr, off, 64);
+        lmax = fmax(lmax, __shfl_xor(lmax, off, 64));
+    }
+    if (lane == 0) { red[0][wv] = ft; red[1][wv] = fr; red[2][wv] = lmax; }
     __syncthreads();
-    if (tid == 0) {                  // stable insertion sort (the input is nearly sorted)
+    if (tid == 0) {
+        double a = 0.0, b = 0.0, c = 0.0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) { a += red[0][q]; b += red[1][q]; c = fmax(c, red[2][q]); }
+        delta_out[kb] = 2.0 * (sqrt(a) + c * sqrt(b));
+        // stable insertion sort (the input is nearly sorted)
         for (int i = 1; i < N; ++i) {
             const double x = lsort[i];
             const int xi = idx[i];
@@ -956,44 +1184,38 @@ __global__ __launch_bounds__(256) void k_oa_update(int N, const double2 *T, cons
             lsort[k + 1] = x; idx[k + 1] = xi;
         }
     }
-    double ft = 0.0, fr = 0.0;
-    for (size_t t = tid; t < (size_t)N * N; t += blockDim.x) {
-        const int r = (int)(t % N), c = (int)(t / N);
-        double2 tv = Tk[t];
-        double2 rv = make_double2(-Gk[t].x, -Gk[t].y);
-        if (r == c) { tv.x -= lam[r]; rv.x += 1.0; }
-        ft += tv.x * tv.x + tv.y * tv.y;
-        fr += rv.x * rv.x + rv.y * rv.y;
+    __syncthreads();
+    for (int i = tid; i < N; i += blockDim.x) {
+        w[(size_t)kb * N + i] = lsort[i];
+        perm[(size_t)kb * N + i] = idx[i];
+        lam_out[(size_t)kb * N + i] = lam[i];
     }
-    __syncthreads();
-    red[tid] = ft;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) { if (tid < st) red[tid] += red[tid + st]; __syncthreads(); }
-    ft = sqrt(red[0]);
-    __syncthreads();
-    red[tid] = fr;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) { if (tid < st) red[tid] += red[tid + st]; __syncthreads(); }
-    fr = sqrt(red[0]);
-    const double delta = 2.0 * (ft + lmax * fr);
-    for (size_t t = tid; t < (size_t)N * N; t += blockDim.x) {
-        const int r = (int)(t % N), j = (int)(t / N);
-        const int c = idx[j];
-        const size_t src = (size_t)c * N + r;
-        double2 rv = make_double2(-Gk[src].x, -Gk[src].y);
-        if (r == c) rv.x += 1.0;
-        double2 ev;
-        const double gap = lam[c] - lam[r];
-        if (fabs(gap) > delta) {
-            const double2 tv = Tk[src];
-            ev = make_double2((tv.x + lam[c] * rv.x) / gap, (tv.y + lam[c] * rv.y) / gap);
-        } else {
-            ev = make_double2(0.5 * rv.x, 0.5 * rv.y);
-        }
-        if (r == c) ev.x += 1.0;
-        Fk[t] = ev;
+}
+
+__global__ __launch_bounds__(256) void k_oa_fmat(int N, const double2 *T, const double2 *G, double2 *F,
+                                                 size_t stride, const double *lamv, const int *perm,
+                                                 const double *delta)
+{
+    const int kb = blockIdx.y;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)N * N) return;
+    const int r = (int)(t % N), j = (int)(t / N);
+    const int c = perm[(size_t)kb * N + j];
+    const double *lam = lamv + (size_t)kb * N;
+    const size_t src = (size_t)kb * stride + (size_t)c * N + r;
+    const double2 gv = G[src];
+    double2 rv = make_double2(-gv.x, -gv.y);
+    if (r == c) rv.x += 1.0;
+    double2 ev;
+    const double gap = lam[c] - lam[r];
+    if (fabs(gap) > delta[kb]) {
+        const double2 tv = T[src];
+        ev = make_double2((tv.x + lam[c] * rv.x) / gap, (tv.y + lam[c] * rv.y) / gap);
+    } else {
+        ev = make_double2(0.5 * rv.x, 0.5 * rv.y);
     }
-    for (int i = tid; i < N; i += blockDim.x) w[(size_t)kb * N + i] = lsort[i];
+    if (r == c) ev.x += 1.0;
+    F[(size_t)kb * stride + t] = ev;
 }
 
 // ---------------------------------------------------------------------------
@@ -1322,14 +1544,48 @@ void launch_trtri_panel(int n, double2 *s, size_t stride, const double2 *wk, int
 {
     hipLaunchKernelGGL(k_trtri_panel, dim3(nb), dim3(256), 0, st, n, s, stride, wk, j0, info);
 }
+constexpr int kTileBM[kGemmTiles] = {64, 96, 128, 32};
+constexpr int kTileBN[kGemmTiles] = {64, 96, 64, 32};
+
+template <class R, int OPA, int OPB>
+static void launch_gemm_ops(const GemmArgsT<R> &g, int tile, int nb, hipStream_t st)
+{
+    dim3 grid((g.m + kTileBM[tile] - 1) / kTileBM[tile], (g.n + kTileBN[tile] - 1) / kTileBN[tile], nb);
+    switch (tile) {
+    case 1: hipLaunchKernelGGL((k_gemm<R, OPA, OPB, 96, 96>), grid, dim3(256), 0, st, g); break;
+    case 2: hipLaunchKernelGGL((k_gemm<R, OPA, OPB, 128, 64>), grid, dim3(256), 0, st, g); break;
+    case 3: hipLaunchKernelGGL((k_gemm<R, OPA, OPB, 32, 32>), grid, dim3(256), 0, st, g); break;
+    default: hipLaunchKernelGGL((k_gemm<R, OPA, OPB, 64, 64>), grid, dim3(256), 0, st, g); break;
+    }
+}
+static void store_args(GemmShape &s, const GemmArgsT<double> &g) { s.g64 = g; }
+static void store_args(GemmShape &s, const GemmArgsT<float> &g) { s.g32 = g; }
 template <class R>
 static void launch_gemm_t(const GemmArgsT<R> &g, int opa, int opb, int nb, hipStream_t st)
 {
-    dim3 grid((g.m + 63) / 64, (g.n + 63) / 64, nb);
-    if (opa == 0 && opb == 0) hipLaunchKernelGGL((k_gemm<R, 0, 0>), grid, dim3(256), 0, st, g);
-    else if (opa == 1 && opb == 0) hipLaunchKernelGGL((k_gemm<R, 1, 0>), grid, dim3(256), 0, st, g);
-    else if (opa == 0 && opb == 1) hipLaunchKernelGGL((k_gemm<R, 0, 1>), grid, dim3(256), 0, st, g);
-    else hipLaunchKernelGGL((k_gemm<R, 1, 1>), grid, dim3(256), 0, st, g);
+    const int prec = sizeof(R) == 8 ? 0 : 1;
+    GemmShape sh{prec, opa, opb, g.m, g.n, g.k, g.tri, nb, {}, {}};
+    store_args(sh, g);
+    gemm_note(sh);
+    const int tile = gemm_tile(prec, opa, opb, g.m, g.n, g.k, g.tri, nb);
+    if (opa == 0 && opb == 0) launch_gemm_ops<R, 0, 0>(g, tile, nb, st);
+    else if (opa == 1 && opb == 0) launch_gemm_ops<R, 1, 0>(g, tile, nb, st);
+    else if (opa == 0 && opb == 1) launch_gemm_ops<R, 0, 1>(g, tile, nb, st);
+    else launch_gemm_ops<R, 1, 1>(g, tile, nb, st);
+}
+void launch_gemm_tile(const GemmShape &s, int tile, hipStream_t st)
+{
+    if (s.prec == 0) {
+        if (s.opa == 0 && s.opb == 0) launch_gemm_ops<double, 0, 0>(s.g64, tile, s.nb, st);
+        else if (s.opa == 1 && s.opb == 0) launch_gemm_ops<double, 1, 0>(s.g64, tile, s.nb, st);
+        else if (s.opa == 0 && s.opb == 1) launch_gemm_ops<double, 0, 1>(s.g64, tile, s.nb, st);
+        else launch_gemm_ops<double, 1, 1>(s.g64, tile, s.nb, st);
+    } else {
+        if (s.opa == 0 && s.opb == 0) launch_gemm_ops<float, 0, 0>(s.g32, tile, s.nb, st);
+        else if (s.opa == 1 && s.opb == 0) launch_gemm_ops<float, 1, 0>(s.g32, tile, s.nb, st);
+        else if (s.opa == 0 && s.opb == 1) launch_gemm_ops<float, 0, 1>(s.g32, tile, s.nb, st);
+        else launch_gemm_ops<float, 1, 1>(s.g32, tile, s.nb, st);
+    }
 }
 void launch_zgemm(const GemmArgs &g, int opa, int opb, int nb, hipStream_t st)
 {
@@ -1339,10 +1595,26 @@ void launch_cgemm(const GemmArgsT<float> &g, int opa, int opb, int nb, hipStream
 {
     launch_gemm_t<float>(g, opa, opb, nb, st);
 }
+template <class R, int MAXQ>
+static void launch_hetrd_panel_q(const HetrdArgsT<R> &h, int i0, int nb, hipStream_t st)
+{
+    const size_t lds = hetrd_lds_bytes<R>(h.n);
+    if (lds > 65536) {
+        static bool set = false;     // per instantiation; the attribute is per function
+        if (!set) {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_hetrd_panel<R, MAXQ>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            set = true;
+        }
+    }
+    hipLaunchKernelGGL((k_hetrd_panel<R, MAXQ>), dim3(nb), dim3(HT), lds, st, h, i0);
+}
 template <class R>
 static void launch_hetrd_panel_t(const HetrdArgsT<R> &h, int i0, int nb, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_hetrd_panel<R>), dim3(nb), dim3(HT), 0, st, h, i0);
+    if (h.n <= 320) launch_hetrd_panel_q<R, 5>(h, i0, nb, st);
+    else if (h.n <= 640) launch_hetrd_panel_q<R, 10>(h, i0, nb, st);
+    else launch_hetrd_panel_q<R, 20>(h, i0, nb, st);
 }
 template <class R>
 static void launch_hetrd_finish_t(const HetrdArgsT<R> &h, int nb, hipStream_t st)
@@ -1353,9 +1625,11 @@ void launch_hetrd_panel(const HetrdArgs &h, int i0, int nb, hipStream_t st) { la
 void launch_hetrd_finish(const HetrdArgs &h, int nb, hipStream_t st) { launch_hetrd_finish_t(h, nb, st); }
 void launch_hetrd_panel(const HetrdArgsT<float> &h, int i0, int nb, hipStream_t st) { launch_hetrd_panel_t(h, i0, nb, st); }
 void launch_hetrd_finish(const HetrdArgsT<float> &h, int nb, hipStream_t st) { launch_hetrd_finish_t(h, nb, st); }
+int stein_threads(int n) { return (n + 63) / 64 * 64; }
 void launch_stein(const SteinArgs &s, int nb, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_stein, dim3(nb), dim3(256), 0, st, s);
+    hipLaunchKernelGGL(k_stebz, dim3((s.n + 255) / 256, nb), dim3(256), 0, st, s);
+    hipLaunchKernelGGL(k_stein_vec, dim3((s.n + 63) / 64, nb), dim3(64), 0, st, s);
 }
 void launch_real_to_complex_t(int N, const double *Vt, double2 *Y, size_t stride, int nb,
                               hipStream_t st)
@@ -1378,10 +1652,12 @@ void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int
     hipLaunchKernelGGL(k_real_to_c32_t, dim3(blocks, nb), dim3(256), 0, st, n, vt, y, stride);
 }
 int oa_max_n() { return kOaMax; }
-void launch_oa_update(int n, const double2 *t, const double2 *g, double2 *f, size_t stride, double *w,
-                      int nb, hipStream_t st)
+void launch_oa_update(int n, const double2 *t, const double2 *g, double2 *f, size_t stride, double *lam,
+                      int *perm, double *delta, double *w, int nb, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_oa_update, dim3(nb), dim3(256), 0, st, n, t, g, f, stride, w);
+    hipLaunchKernelGGL(k_oa_stats, dim3(nb), dim3(1024), 0, st, n, t, g, stride, lam, perm, delta, w);
+    hipLaunchKernelGGL(k_oa_fmat, dim3((unsigned)(((size_t)n * n + 255) / 256), nb), dim3(256), 0, st, n, t,
+                       g, f, stride, lam, perm, delta);
 }
 void launch_coldot(int N, const double2 *Z, const double2 *X, const double2 *Y, size_t stride,
                    double2 *px, double2 *py, int nb, hipStream_t st)

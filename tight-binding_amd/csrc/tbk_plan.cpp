@@ -50,6 +50,8 @@ struct Workspace {
     // mixed precision: FP32 copies for the reduction and back-transform
     float2 *A32 = nullptr, *P32 = nullptr, *T32 = nullptr, *VT32 = nullptr, *Y32 = nullptr,
            *X32 = nullptr;
+    double *lam = nullptr, *delta = nullptr;
+    int *perm = nullptr;
     size_t stride() const { return (size_t)n * n; }
     // WY factors: one kNB x kNB block per panel, ceil(n / kNB) panels
     size_t tstride() const { return (size_t)((n + kNB - 1) / kNB) * kNB * kNB; }
@@ -65,7 +67,7 @@ struct Workspace {
         rc |= dalloc(&d, (size_t)n * b); rc |= dalloc(&e, (size_t)n * b);
         rc |= dalloc(&w, (size_t)n * b);
         rc |= dalloc(&Vt, nn);
-        rc |= dalloc(&scratch, (size_t)5 * n * 256 * b);
+        rc |= dalloc(&scratch, (size_t)5 * n * tbk::stein_threads(n) * b);
         rc |= dalloc(&info, (size_t)b); rc |= dalloc(&sinfo, (size_t)b);
         return rc ? TBK_ERR_NOMEM : TBK_OK;
     }
@@ -75,6 +77,7 @@ struct Workspace {
         int rc = TBK_OK;
         rc |= dalloc(&A32, nn); rc |= dalloc(&VT32, nn); rc |= dalloc(&Y32, nn); rc |= dalloc(&X32, nn);
         rc |= dalloc(&P32, (size_t)n * 96 * b); rc |= dalloc(&T32, tstride() * b);
+        rc |= dalloc(&lam, (size_t)n * b); rc |= dalloc(&perm, (size_t)n * b); rc |= dalloc(&delta, (size_t)b);
         return rc ? TBK_ERR_NOMEM : TBK_OK;
     }
     void release()
@@ -82,7 +85,7 @@ struct Workspace {
         dfree(A); dfree(S); dfree(Y); dfree(Z); dfree(X); dfree(VT); dfree(Vp); dfree(Wp); dfree(Tf);
         dfree(px); dfree(py); dfree(d); dfree(e); dfree(w); dfree(Vt); dfree(scratch);
         dfree(info); dfree(sinfo);
-        dfree(A32); dfree(P32); dfree(T32); dfree(VT32); dfree(Y32); dfree(X32);
+        dfree(A32); dfree(P32); dfree(T32); dfree(VT32); dfree(Y32); dfree(X32); dfree(lam); dfree(perm); dfree(delta);
     }
 };
 
@@ -156,7 +159,7 @@ void hetrd(Workspace &ws, int nb, hipStream_t st)
         const int s = i0 + pw, m = n - s;
         if (m > 0)
             tbk::launch_zgemm(gemm(m, m, 2 * kNB, ws.Vp + s, n, sp, ws.Vp + (size_t)kNB * n + s, n, sp,
-                                   ws.A + (size_t)s * n + s, n, ss, 0, double2{-1.0, 0.0},
+                                   ws.A + (size_t)s * n + s, n, ss, 4, double2{-1.0, 0.0},
                                    double2{1.0, 0.0}), 0, 1, nb, st);
     }
     tbk::launch_hetrd_finish(h, nb, st);
@@ -221,7 +224,7 @@ void solve_batch_mixed(Workspace &ws, int nb, double *w_out, hipStream_t st, Sta
         const int s = i0 + pw, m = n - s;
         if (m > 0)
             tbk::launch_cgemm(cgemm(m, m, 2 * kNB, ws.P32 + s, n, sp, ws.P32 + (size_t)kNB * n + s, n, sp,
-                                    ws.A32 + (size_t)s * n + s, n, ss, 0, float2{-1.f, 0.f},
+                                    ws.A32 + (size_t)s * n + s, n, ss, 4, float2{-1.f, 0.f},
                                     float2{1.f, 0.f}), 0, 1, nb, st);
     }
     tbk::launch_hetrd_finish(h, nb, st);
@@ -250,7 +253,7 @@ void solve_batch_mixed(Workspace &ws, int nb, double *w_out, hipStream_t st, Sta
     tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.Z, n, ss), 1, 0, nb, st);
     tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.Y, n, ss, ws.VT, n, ss), 1, 0, nb, st);
     // X = (I + E) with sorted columns; VT = Y X; Z = U^{-1} VT
-    tbk::launch_oa_update(n, ws.Z, ws.VT, ws.X, ss, w_out, nb, st);
+    tbk::launch_oa_update(n, ws.Z, ws.VT, ws.X, ss, ws.lam, ws.perm, ws.delta, w_out, nb, st);
     tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.VT, n, ss), 0, 0, nb, st);
     tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.VT, n, ss, ws.Z, n, ss, 1), 0, 0, nb, st);
     tm.end();
@@ -520,6 +523,56 @@ extern "C" int tbk_plan_run(tbk_plan *p, int mode, double tkb, double eigcut, do
     return tbk_plan_wait(p, efermi, earg1, earg2);
 }
 
+// GEMM autotuner: one mode-4 pass records the plan's GEMM shapes, then each
+// shape not yet in the table is timed with every tile (one warm-up and
+// `reps` launches on the plan's stream) and the fastest tile is kept for the
+// rest of the process.  The timing launches overwrite workspace contents, so
+// the plan's results are undefined until its next run.
+extern "C" int tbk_plan_autotune(tbk_plan *p, int reps, int *ntuned)
+{
+    if (!p || reps < 1) return TBK_ERR_ARG;
+    HIP_TRY(hipSetDevice(p->device));
+    tbk::gemm_record(true);
+    int rc = tbk_plan_launch(p, 4, 0.005, 1e30);
+    if (!rc) rc = tbk_plan_wait(p, nullptr, nullptr, nullptr);
+    tbk::gemm_record(false);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int count = 0;
+    for (const tbk::GemmShape &s : tbk::gemm_recorded()) {
+        if (tbk::gemm_tuned(s)) continue;
+        int best = 0;
+        float bt = 3.4e38f;
+        for (int t = 0; t < tbk::kGemmTiles; ++t) {
+            tbk::launch_gemm_tile(s, t, p->stream);
+            (void)hipEventRecord(e0, p->stream);
+            for (int r = 0; r < reps; ++r) tbk::launch_gemm_tile(s, t, p->stream);
+            (void)hipEventRecord(e1, p->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) { rc = TBK_ERR_HIP; break; }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < bt) { bt = ms; best = t; }
+        }
+        if (rc) break;
+        tbk::gemm_store_tile(s, best);
+        ++count;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ntuned) *ntuned = count;
+    if (!rc && hipGetLastError() != hipSuccess) rc = TBK_ERR_HIP;
+    return rc;
+}
+
+// Tile of a GEMM shape (the autotuned entry, else the cost model); for
+// reporting which tiles the plan's launches use.
+extern "C" int tbk_gemm_tile(int precision, int opa, int opb, int m, int n, int k, int tri, int nb)
+{
+    return tbk::gemm_tile(precision == TBK_MIXED ? 1 : 0, opa, opb, m, n, k, tri, nb);
+}
+
 extern "C" int tbk_plan_eigenvalues(tbk_plan *p, double *esk, double *desk)
 {
     if (!p) return TBK_ERR_ARG;
@@ -724,24 +777,40 @@ extern "C" int tbk_debug_stages(int device, int n, const double *h, const double
     return rc;
 }
 
-extern "C" int tbk_debug_zgemm(int device, int m, int n, int k, int opa, int opb, int tri,
-                               const double *a, int lda, const double *b, int ldb, double *c,
-                               int ldc)
+template <class C, class GA>
+static int debug_gemm_t(int tile, int m, int n, int k, int opa, int opb, int tri, const void *a, int lda,
+                        const void *b, int ldb, void *c, int ldc)
 {
-    HIP_TRY(hipSetDevice(device));
     const size_t na = (size_t)lda * (opa ? m : k), nb = (size_t)ldb * (opb ? k : n),
                  nc = (size_t)ldc * n;
-    double2 *da = nullptr, *db = nullptr, *dc = nullptr;
+    C *da = nullptr, *db = nullptr, *dc = nullptr;
     int rc = dalloc(&da, na) | dalloc(&db, nb) | dalloc(&dc, nc);
     if (!rc) {
-        (void)hipMemcpy(da, a, na * sizeof(double2), hipMemcpyHostToDevice);
-        (void)hipMemcpy(db, b, nb * sizeof(double2), hipMemcpyHostToDevice);
-        (void)hipMemset(dc, 0, nc * sizeof(double2));
-        tbk::launch_zgemm(gemm(m, n, k, da, lda, 0, db, ldb, 0, dc, ldc, 0, tri), opa, opb, 1, nullptr);
-        if (hipMemcpy(c, dc, nc * sizeof(double2), hipMemcpyDeviceToHost) != hipSuccess) rc = TBK_ERR_HIP;
+        (void)hipMemcpy(da, a, na * sizeof(C), hipMemcpyHostToDevice);
+        (void)hipMemcpy(db, b, nb * sizeof(C), hipMemcpyHostToDevice);
+        (void)hipMemset(dc, 0, nc * sizeof(C));
+        GA g;
+        g.m = m; g.n = n; g.k = k; g.tri = tri;
+        g.alpha.x = 1; g.alpha.y = 0; g.beta.x = 0; g.beta.y = 0;
+        g.a = da; g.lda = lda; g.sa = 0; g.b = db; g.ldb = ldb; g.sb = 0; g.c = dc; g.ldc = ldc; g.sc = 0;
+        tbk::GemmShape sh{sizeof(C) == 16 ? 0 : 1, opa, opb, m, n, k, tri, 1, {}, {}};
+        if constexpr (sizeof(C) == 16) sh.g64 = g; else sh.g32 = g;
+        const int t = tile >= 0 ? tile : tbk::gemm_tile(sh.prec, opa, opb, m, n, k, tri, 1);
+        tbk::launch_gemm_tile(sh, t, nullptr);
+        if (hipMemcpy(c, dc, nc * sizeof(C), hipMemcpyDeviceToHost) != hipSuccess) rc = TBK_ERR_HIP;
     } else {
         rc = TBK_ERR_NOMEM;
     }
     dfree(da); dfree(db); dfree(dc);
     return rc;
+}
+
+extern "C" int tbk_debug_zgemm(int device, int precision, int tile, int m, int n, int k, int opa, int opb,
+                               int tri, const void *a, int lda, const void *b, int ldb, void *c, int ldc)
+{
+    if (tile >= tbk::kGemmTiles || opa < 0 || opa > 1 || opb < 0 || opb > 1) return TBK_ERR_ARG;
+    HIP_TRY(hipSetDevice(device));
+    if (precision == TBK_MIXED)
+        return debug_gemm_t<float2, tbk::GemmArgsT<float>>(tile, m, n, k, opa, opb, tri, a, lda, b, ldb, c, ldc);
+    return debug_gemm_t<double2, tbk::GemmArgsT<double>>(tile, m, n, k, opa, opb, tri, a, lda, b, ldb, c, ldc);
 }

@@ -23,7 +23,8 @@ EXPORTS = [
     "tbk_search2", "tbk_setpar", "tbk_setparv", "tbk_plan_create", "tbk_plan_run",
     "tbk_plan_launch", "tbk_plan_wait", "tbk_plan_eigenvalues", "tbk_plan_eigenvectors",
     "tbk_plan_hs", "tbk_plan_nv", "tbk_plan_npairs", "tbk_plan_destroy",
-    "tbk_plan_stage_times", "tbk_plan_set_timing", "tbk_zhegv", "tbk_fermi",
+    "tbk_plan_stage_times", "tbk_plan_set_timing", "tbk_plan_autotune", "tbk_gemm_tile",
+    "tbk_zhegv", "tbk_fermi",
     "tbk_error_string", "tbk_debug_stages", "tbk_debug_zgemm",
 ]
 
@@ -67,6 +68,8 @@ def lib():
                                  d, d, d, d]
         L.tbk_setparv.argtypes = [C.c_int, i, C.c_int, d, C.c_int, C.c_int, C.c_double, C.c_int,
                                   d, d, d, i, i, d, d, d, d]
+        L.tbk_plan_autotune.argtypes = [vp, C.c_int, ip]
+        L.tbk_gemm_tile.argtypes = [C.c_int] * 8
         L.tbk_plan_create.argtypes = [C.c_int, C.POINTER(StructureDesc), C.c_int, d, d, C.c_int,
                                       C.c_int, C.POINTER(vp)]
         L.tbk_plan_run.argtypes = [vp, C.c_int, C.c_double, C.c_double, dp, dp, dp]
@@ -84,7 +87,7 @@ def lib():
         L.tbk_fermi.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, d, d, C.c_double,
                                 C.c_double, dp]
         L.tbk_debug_stages.argtypes = [C.c_int, C.c_int] + [vp] * 10
-        L.tbk_debug_zgemm.argtypes = [C.c_int] * 7 + [vp, C.c_int, vp, C.c_int, vp, C.c_int]
+        L.tbk_debug_zgemm.argtypes = [C.c_int] * 9 + [vp, C.c_int, vp, C.c_int, vp, C.c_int]
         L.tbk_error_string.argtypes = [C.c_int]
         L.tbk_error_string.restype = C.c_char_p
         _lib = L

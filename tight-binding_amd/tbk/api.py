@@ -138,6 +138,13 @@ class Plan:
         check(lib().tbk_plan_hs(self.h, int(k), int(deriv), h.ctypes.data, s.ctypes.data))
         return h.reshape(n, n).T.copy(), s.reshape(n, n).T.copy()
 
+    def autotune(self, reps: int = 3) -> int:
+        """Time every GEMM tile for this plan's shapes and keep the fastest
+        (per process).  Results are undefined until the next run."""
+        n = C.c_int(0)
+        check(lib().tbk_plan_autotune(self.h, int(reps), C.byref(n)), "tbk_plan_autotune")
+        return n.value
+
     def set_timing(self, on: bool = True):
         """Enable per-stage HIP event timing (clears accumulated times)."""
         check(lib().tbk_plan_set_timing(self.h, int(on)))

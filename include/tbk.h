@@ -90,6 +90,7 @@ int tbk_setparv(int natoms, const int *kkind, int kinds, const double *param,
  * Device plans: one structure, its k-mesh, device-resident H/S/Z state.
  * --------------------------------------------------------------------- */
 typedef struct tbk_plan tbk_plan;
+typedef struct tbk_multi tbk_multi;
 
 /* Structure description passed to tbk_plan_create. */
 typedef struct tbk_structure_desc {
@@ -119,6 +120,19 @@ int tbk_plan_create(int device, const tbk_structure_desc *st, int nk,
                     const double *kpts, const double *weights, int batch,
                     int precision, tbk_plan **out);
 
+/* A plan over several structures of uneven orbital count, solved together
+ * in the same batched launches (no reference counterpart: the reference
+ * runs one structure per call of bande, static.f:1218-1222).  Structure i
+ * has nk[i] k-points; kpts [3*sum(nk)] and weights [sum(nk)] hold them
+ * structure after structure.  Items are padded to the largest orbital
+ * count N with decoupled rows (H = 1e4 Ry * I, S = I), which leaves each
+ * structure's eigenpairs, E_F and sums those of the structure alone.
+ * tbk_plan_create is the case nstruct = 1 (no padding). */
+int tbk_plan_create_group(int device, int nstruct,
+                          const tbk_structure_desc *st, const int *nk,
+                          const double *kpts, const double *weights,
+                          int batch, int precision, tbk_plan **out);
+
 /* Run the whole bande k-loop + Fermi + sums on the device.  Replaces
  * bande(mode,...,efermi,earg1,earg2,...,iberr), bande.f:1, for the
  * collinear non-magnetic case.  Blocks until results are on the host. */
@@ -131,12 +145,24 @@ int tbk_plan_launch(tbk_plan *plan, int mode, double tkb, double eigcut);
 int tbk_plan_wait(tbk_plan *plan, double *efermi, double *earg1,
                   double *earg2);
 
-/* Results of the last run: eigenvalues esk[k*nv+i] (ascending per k) and
- * first-order derivatives desk[k*nv+i] (pressure modes), host copies. */
-int tbk_plan_eigenvalues(tbk_plan *plan, double *esk, double *desk);
+/* Results of structure s after tbk_plan_wait: E_F, earg1, earg2 as in
+ * tbk_plan_run; returns the structure's own status (TBK_ERR_OVERLAP,
+ * TBK_ERR_DEGEN) so one failing structure does not hide the others.
+ * tbk_plan_wait returns the first failing status and structure 0's
+ * results. */
+int tbk_plan_result(const tbk_plan *plan, int s, double *efermi,
+                    double *earg1, double *earg2);
 
-/* Eigenvectors of k-point k of the last run when the plan holds it
- * (batch covering k), complex128 column-major nv x nv, Z^H S Z = I. */
+/* Results of the last run: eigenvalues esk[k*nv+i] (ascending per k) and
+ * first-order derivatives desk[k*nv+i] (pressure modes, else stale), host
+ * copies, of structure s (tbk_plan_eigenvalues: s = 0). */
+int tbk_plan_eigenvalues(tbk_plan *plan, double *esk, double *desk);
+int tbk_plan_structure_eigenvalues(tbk_plan *plan, int s, double *esk,
+                                   double *desk);
+
+/* Eigenvectors of item k (k-points numbered across structures) of the
+ * last run when the plan holds it (last batch), complex128 column-major
+ * nv x nv of the item's structure, Z^H S Z = I. */
 int tbk_plan_eigenvectors(tbk_plan *plan, int k, double *z);
 
 /* H(k), S(k) of k-point k as built on the device (full Hermitian,
@@ -158,6 +184,16 @@ int tbk_plan_stage_times(const tbk_plan *plan, int cap, const char **names,
  * either call clears the accumulated times. */
 int tbk_plan_set_timing(tbk_plan *plan, int on);
 
+/* Capture the launch sequence of a run into a hipGraph on first use and
+ * replay it while (mode, tkb, eigcut) stay the same (on = 1); stage timing
+ * and the autotuner run uncaptured.  Off by default. */
+int tbk_plan_set_graph(tbk_plan *plan, int on);
+
+/* Padded order N, number of structures, device bytes held by the plan. */
+int tbk_plan_order(const tbk_plan *plan);
+int tbk_plan_nstruct(const tbk_plan *plan);
+long long tbk_plan_device_bytes(const tbk_plan *plan);
+
 /* GEMM tile autotuner (no reference counterpart: the reference calls
  * unblocked LAPACK).  Runs one mode-4 pass of the plan to collect its GEMM
  * shapes, times every tile for each shape not tuned yet (reps launches
@@ -167,6 +203,32 @@ int tbk_plan_set_timing(tbk_plan *plan, int on);
 int tbk_plan_autotune(tbk_plan *plan, int reps, int *ntuned);
 /* Tile used for a GEMM shape: 0 = 64x64, 1 = 96x96, 2 = 128x64, 3 = 32x32. */
 int tbk_gemm_tile(int precision, int opa, int opb, int m, int n, int k, int tri, int nb);
+
+/* ---------------------------------------------------------------------
+ * Many structures of uneven orbital count (no reference counterpart: the
+ * reference solves one structure per bande call).  Structures are grouped
+ * by padded order (orbital count rounded up to `granularity`), each group
+ * is one group plan, and the groups run concurrently on their own streams.
+ * max_batch_bytes bounds the workspace of each group (items per launch);
+ * <= 0: all items of a group in one launch.
+ * --------------------------------------------------------------------- */
+int tbk_multi_create(int device, int nstruct, const tbk_structure_desc *st,
+                     const int *nk, const double *kpts,
+                     const double *weights, int precision, int granularity,
+                     long long max_batch_bytes, tbk_multi **out);
+int tbk_multi_launch(tbk_multi *m, int mode, double tkb, double eigcut);
+/* Waits for every group; returns the first failing structure status. */
+int tbk_multi_wait(tbk_multi *m);
+int tbk_multi_run(tbk_multi *m, int mode, double tkb, double eigcut);
+int tbk_multi_result(const tbk_multi *m, int s, double *efermi,
+                     double *earg1, double *earg2);
+int tbk_multi_eigenvalues(tbk_multi *m, int s, double *esk, double *desk);
+int tbk_multi_ngroups(const tbk_multi *m);
+int tbk_multi_group_order(const tbk_multi *m, int g);
+int tbk_multi_set_graph(tbk_multi *m, int on);
+int tbk_multi_autotune(tbk_multi *m, int reps);
+long long tbk_multi_device_bytes(const tbk_multi *m);
+int tbk_multi_destroy(tbk_multi *m);
 
 /* ---------------------------------------------------------------------
  * Single-call drop-ins on host arrays.

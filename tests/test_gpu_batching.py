@@ -1,0 +1,125 @@
+"""Batching of many structures of uneven orbital count (group plans padded
+to a common order, multi plans grouped by padded order) and hipGraph
+replay: each structure's results are the same alone and in the mix."""
+import numpy as np
+import pytest
+
+import oracle
+from tbk import _lib
+from tbk import structures as S
+from tbk.api import GroupPlan, MultiPlan, Plan
+
+pytestmark = pytest.mark.gpu
+
+EIG_TOL, EF_TOL, E_TOL, P_TOL_KBAR = 1e-10, 1e-9, 1e-9, 1e-6
+
+
+def pick(c4, orders):
+    out = []
+    for n in orders:
+        for st, q in c4:
+            if st.nv == n:
+                out.append((st, q))
+                break
+    return out
+
+
+@pytest.fixture(scope="module")
+def c4():
+    return S.config_c4(count=256)
+
+
+@pytest.mark.parametrize("prec", [_lib.TBK_FP64, _lib.TBK_MIXED])
+def test_group_same_alone_and_in_mix(c4, prec):
+    sel = pick(c4, [9, 18, 27, 36, 54])
+    ks = [S.monkhorst_pack((3, 3, 3)) for _ in sel]
+    ws = [np.full(len(k), 1.0 / len(k)) for k in ks]
+    print(f"group: orders {[st.nv for st, _ in sel]} precision {prec}", flush=True)
+    g = GroupPlan([st for st, _ in sel], ks, ws, precision=prec)
+    assert g.order == 54
+    res = g.run(4)
+    tol = 1e-12 if prec == _lib.TBK_FP64 else 1e-10
+    for i, (st, _) in enumerate(sel):
+        p = Plan(st, ks[i], ws[i], precision=prec)
+        r = p.run(4)
+        e_alone, _ = p.eigenvalues()
+        p.close()
+        e_mix, _ = g.eigenvalues(i)
+        de = float(np.abs(e_mix - e_alone).max())
+        d = {k: abs(res[i][k] - r[k]) for k in r}
+        print(f"  N={st.nv}: max|de|={de:.1e} dEF={d['efermi']:.1e} dE={d['earg1']:.1e} "
+              f"dP={d['earg2']:.1e}", flush=True)
+        assert de < tol and max(d.values()) < tol
+    g.close()
+
+
+def test_group_matches_oracle(c4):
+    sel = pick(c4, [18, 36, 72])
+    ks = [S.monkhorst_pack((2, 2, 2)) for _ in sel]
+    ws = [np.full(len(k), 1.0 / len(k)) for k in ks]
+    g = GroupPlan([st for st, _ in sel], ks, ws)
+    res = g.run(4)
+    for i, (st, _) in enumerate(sel):
+        ref = oracle.Setup(st).bande(ks[i], ws[i], mode=4, nthreads=16, want_eig=True)
+        e, _ = g.eigenvalues(i)
+        de = float(np.abs(e - ref["eig"]).max())
+        print(f"  N={st.nv}: max|de|={de:.1e}", flush=True)
+        assert de < EIG_TOL
+        assert abs(res[i]["efermi"] - ref["efermi"]) < EF_TOL
+        assert abs(res[i]["earg1"] - ref["earg1"]) / st.natoms < E_TOL
+        assert abs(res[i]["earg2"] - ref["earg2"]) * S.RYBOHR3_KBAR < P_TOL_KBAR
+    g.close()
+
+
+def test_graph_replay_matches_eager():
+    """Captured launches replayed several times, with eager runs between
+    the replays, reproduce the eager results bit for bit."""
+    st = S.config_c1()
+    k = S.monkhorst_pack((4, 4, 4))
+    w = np.full(len(k), 1.0 / len(k))
+    p = Plan(st, k, w)
+    ref = p.run(4)
+    e_ref, d_ref = p.eigenvalues()
+    p.set_graph(True)
+    for rep in range(3):
+        r = p.run(4)
+        e, d = p.eigenvalues()
+        print(f"replay {rep}: {r}", flush=True)
+        assert r == ref
+        assert np.array_equal(e, e_ref) and np.array_equal(d, d_ref)
+        p.set_graph(False)
+        assert p.run(4) == ref            # eager work between replays
+        p.set_graph(True)
+    # a changed kT re-captures
+    r2 = p.run(4, tkb=0.01)
+    p.set_graph(False)
+    assert p.run(4, tkb=0.01) == r2
+    p.close()
+
+
+@pytest.mark.parametrize("prec", [_lib.TBK_FP64, _lib.TBK_MIXED])
+def test_multi_uneven_subset_vs_oracle(c4, prec):
+    sel = [(st, q) for st, q in c4 if st.nv <= 108][:24]
+    ks = [S.monkhorst_pack((q, q, q)) for _, q in sel]
+    ws = [np.full(len(k), 1.0 / len(k)) for k in ks]
+    m = MultiPlan([st for st, _ in sel], ks, ws, precision=prec, graph=True)
+    print(f"multi: {len(sel)} structures, {m.ngroups} groups, {sum(len(k) for k in ks)} k, "
+          f"{m.device_bytes() / 2**30:.2f} GiB", flush=True)
+    m.run(4)
+    res = m.run(4)                            # graph replay
+    worst = {"ef": 0.0, "e": 0.0, "p": 0.0, "eig": 0.0}
+    for i, (st, q) in enumerate(sel):
+        ref = oracle.Setup(st).bande(ks[i], ws[i], mode=4, nthreads=16, want_eig=True)
+        e, _ = m.eigenvalues(i)
+        worst["eig"] = max(worst["eig"], float(np.abs(e - ref["eig"]).max()))
+        worst["ef"] = max(worst["ef"], abs(res[i]["efermi"] - ref["efermi"]))
+        worst["e"] = max(worst["e"], abs(res[i]["earg1"] - ref["earg1"]) / st.natoms)
+        worst["p"] = max(worst["p"], abs(res[i]["earg2"] - ref["earg2"]) * S.RYBOHR3_KBAR)
+    print(f"  worst vs oracle: {worst}", flush=True)
+    if prec == _lib.TBK_FP64:
+        assert worst["eig"] < EIG_TOL and worst["ef"] < EF_TOL and worst["e"] < E_TOL
+        assert worst["p"] < P_TOL_KBAR
+    else:
+        assert worst["ef"] < 1e-6 and worst["e"] < 1e-6 and worst["eig"] < 1e-8
+        assert worst["p"] < 1e-3
+    m.close()

@@ -8,14 +8,29 @@
 
 namespace tbk {
 
+// One structure of a plan (device pointers into the plan's concatenated
+// set-up arrays).
+struct StructDev {
+    int natoms, nv, npairs;
+    const int *row_atom, *row_orb;   // [nv] atom and orbital of each row
+    const int *ksk, *nbas;           // [natoms] first row, orbital count
+    const int *blk_ptr;              // [natoms*natoms+1] CSR of pairs by (i, j)
+    const double *dvec;              // [3*npairs] dlv - posq_j + posq_i
+    const double *eh, *es;           // [81*npairs] SK blocks e(a,b) of H, S
+    const double *edh, *eds;         // [81*npairs] volume derivatives
+    const double *paros, *dparos;    // [4*natoms] on-site terms and derivatives
+};
+
+// Build of H(k), S(k) (or dH, dS) for a batch of items (structure, k-point)
+// at the plan order n >= nv: rows/columns nv..n-1 are padding, decoupled
+// (H = pad * I, S = I; zero in the derivative).
 struct BuildArgs {
-    int n, natoms, onsite_s, npairs;
-    size_t stride;                 // elements between k-point slabs
-    const int *row_atom, *row_orb, *ksk, *nbas, *blk_ptr;
-    const double *dvec;            // [3*npairs] dlv - posq_j + posq_i
-    const double *eh, *es;         // [81*npairs] SK blocks e(a,b), row-major
-    const double *onsite;          // [4*natoms] paros or dparos
-    const double2 *ph;             // [nb][npairs] Bloch phases
+    int n, deriv, maxpairs;
+    size_t stride;                 // elements between item slabs
+    double pad;
+    const StructDev *sd;
+    const int *item_s;             // [nb] structure of each item
+    const double2 *ph;             // [nb][maxpairs] Bloch phases
     double2 *h, *s;
 };
 
@@ -51,6 +66,7 @@ typedef HetrdArgsT<double> HetrdArgs;
 struct SteinArgs {
     int n;
     double ortol_rel;              // group threshold relative to ||T||_1
+    const int *nact;               // [nb] active order of each item (null: n)
     const double *d, *e;
     double *w, *v, *scratch;       // w [nb][n]; v [nb][n*n]; scratch [nb][5*n*stein_threads(n)]
     int *info;
@@ -60,10 +76,11 @@ struct PerturbArgs {
     int n, nb, k0;
     size_t stride;
     double eigcut;
+    const int *nact;               // [nb] active order of each item (null: n)
     const double *w;
     const double2 *px, *py, *z, *x, *y;
     double *desk;
-    int *ncut, *err;
+    int *ncut, *err;               // ncut [k0+nb], err [k0+nb] (2: > 9 degenerate)
 };
 
 struct FermiArgs {
@@ -90,9 +107,9 @@ void gemm_record(bool on);                  // collect the shapes of launches
 void gemm_note(const GemmShape &s);
 std::vector<GemmShape> gemm_recorded();
 
-void launch_phases(int npairs, const double *kpts, const double *dvec, double2 *ph, int nb,
-                   hipStream_t st);
-void launch_build_hs(const BuildArgs &a, int natoms, int nb, hipStream_t st);
+void launch_phases(const StructDev *sd, const int *item_s, const double *kpts, int maxpairs,
+                   double2 *ph, int nb, hipStream_t st);
+void launch_build_hs(const BuildArgs &a, int maxnat, int nb, hipStream_t st);
 void launch_potrf_panel(int n, double2 *s, size_t stride, int j0, int *info, int nb,
                         hipStream_t st);
 void launch_trtri_panel(int n, double2 *s, size_t stride, const double2 *wk, int j0,
@@ -113,11 +130,13 @@ void launch_c32_to_c64(size_t count, const float2 *x, double2 *y, hipStream_t st
 void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int nb, hipStream_t st);
 int oa_max_n();                  // largest N of launch_oa_update
 // Ogita-Aishima step: lam [nb][n], perm [nb][n], delta [nb] are scratch
-void launch_oa_update(int n, const double2 *t, const double2 *g, double2 *f, size_t stride, double *lam,
-                      int *perm, double *delta, double *w, int nb, hipStream_t st);
+void launch_oa_update(int n, const int *nact, const double2 *t, const double2 *g, double2 *f, size_t stride,
+                      double *lam, int *perm, double *delta, double *w, int nb, hipStream_t st);
 void launch_coldot(int n, const double2 *z, const double2 *x, const double2 *y, size_t stride,
                    double2 *px, double2 *py, int nb, hipStream_t st);
 void launch_perturb(const PerturbArgs &p, hipStream_t st);
 void launch_fermi_sums(const FermiArgs &f, hipStream_t st);
+void launch_fermi_sums_group(const FermiArgs *f_dev, int nstruct, hipStream_t st);   // one WG each
+void launch_copy_int(const int *src, int *dst, int n, hipStream_t st);
 
 }  // namespace tbk

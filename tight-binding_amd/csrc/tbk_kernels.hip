@@ -86,66 +86,86 @@ __device__ double block_sum(double v, double *red)
 // ---------------------------------------------------------------------------
 // H(k), S(k) build
 // ---------------------------------------------------------------------------
-// Bloch phases exp(2 pi i k.d_p) for every (k-point, pair), d_p = dlv -
+// Bloch phases exp(2 pi i k.d_p) for every (item, pair), d_p = dlv -
 // posq_j + posq_i (rotate.f:144-149), written once per batch.
-__global__ __launch_bounds__(256) void k_phases(int npairs, const double *kpts, const double *dvec,
-                                                double2 *ph)
+__global__ __launch_bounds__(256) void k_phases(const StructDev *sd, const int *item_s, const double *kpts,
+                                                int maxpairs, double2 *ph)
 {
 #pragma clang fp contract(off)
     const int kb = blockIdx.y;
+    const StructDev &S = sd[item_s[kb]];
     const double *ak = kpts + 3 * (size_t)kb;
     const double k0 = ak[0], k1 = ak[1], k2 = ak[2];
     const double tpi = 6.283185307179586;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += gridDim.x * blockDim.x) {
-        const double *d = dvec + 3 * (size_t)p;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < S.npairs; p += gridDim.x * blockDim.x) {
+        const double *d = S.dvec + 3 * (size_t)p;
         const double dot = k0 * d[0] + k1 * d[1] + k2 * d[2];
         double sn, cs;
         sincos(tpi * dot, &sn, &cs);
-        ph[(size_t)kb * npairs + p] = make_double2(cs, sn);
+        ph[(size_t)kb * maxpairs + p] = make_double2(cs, sn);
     }
 }
 
-// One workgroup per (column atom ja, k-point); one thread per element of
-// the nc(ja) columns.  Elements of blocks (ia <= ja) are the Bloch sum over
+// One workgroup per (column atom ja, item); one thread per element of the
+// nc(ja) columns.  Elements of blocks (ia <= ja) are the Bloch sum over
 // the pairs (ia, ja) of the reference's pair list, in list order, of
 // e(a,b) * phase (rotate.f:260-326); elements of blocks ia > ja (and below
 // the diagonal of ia = ja) are the conjugates of their upper partners.  The
 // accumulation order and the absence of FMA contraction reproduce the
-// reference's rounding except for the ulp of sin/cos.
+// reference's rounding except for the ulp of sin/cos.  The workgroup with
+// ja = natoms writes the padding columns nv..n-1.
 __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a)
 {
 #pragma clang fp contract(off)
     const int ja = blockIdx.x;
     const int kb = blockIdx.y;
-    const int N = a.n;
-    const int c0 = a.ksk[ja], nc = a.nbas[ja];
-    const double2 *ph = a.ph + (size_t)kb * a.npairs;
+    const StructDev &SD = a.sd[a.item_s[kb]];
+    const int N = a.n, nat = SD.natoms, nv = SD.nv;
+    if (ja > nat) return;
     double2 *H = a.h + (size_t)kb * a.stride;
     double2 *S = a.s + (size_t)kb * a.stride;
+    if (ja == nat) {
+        const double hp = a.deriv ? 0.0 : a.pad, sp = a.deriv ? 0.0 : 1.0;
+        for (int t = threadIdx.x; t < N * (N - nv); t += blockDim.x) {
+            const int r = t % N, c = nv + t / N;
+            H[(size_t)c * N + r] = make_double2(r == c ? hp : 0.0, 0.0);
+            S[(size_t)c * N + r] = make_double2(r == c ? sp : 0.0, 0.0);
+        }
+        return;
+    }
+    const int c0 = SD.ksk[ja], nc = SD.nbas[ja];
+    const double2 *ph = a.ph + (size_t)kb * a.maxpairs;
+    const double *eh = a.deriv ? SD.edh : SD.eh, *es = a.deriv ? SD.eds : SD.es;
+    const double *onsite = a.deriv ? SD.dparos : SD.paros;
     for (int t = threadIdx.x; t < N * nc; t += blockDim.x) {
         const int r = t % N, b = t / N;
-        const int ia = a.row_atom[r], o = a.row_orb[r];
+        if (r >= nv) {
+            H[(size_t)(c0 + b) * N + r] = make_double2(0.0, 0.0);
+            S[(size_t)(c0 + b) * N + r] = make_double2(0.0, 0.0);
+            continue;
+        }
+        const int ia = SD.row_atom[r], o = SD.row_orb[r];
         const bool upper = ia < ja || (ia == ja && b >= o);
         int p0, p1, idx;
         if (upper) {   // e(o, b) over pairs (ia, ja)
-            p0 = a.blk_ptr[ia * a.natoms + ja]; p1 = a.blk_ptr[ia * a.natoms + ja + 1];
+            p0 = SD.blk_ptr[ia * nat + ja]; p1 = SD.blk_ptr[ia * nat + ja + 1];
             idx = 9 * o + b;
         } else {       // conj of e(b, o) over pairs (ja, ia)
-            p0 = a.blk_ptr[ja * a.natoms + ia]; p1 = a.blk_ptr[ja * a.natoms + ia + 1];
+            p0 = SD.blk_ptr[ja * nat + ia]; p1 = SD.blk_ptr[ja * nat + ia + 1];
             idx = 9 * b + o;
         }
         double hr = 0.0, hi = 0.0, sr = 0.0, si = 0.0;
         for (int p = p0; p < p1; ++p) {
             const double2 z = ph[p];
-            const double e1 = a.eh[81 * (size_t)p + idx], e2 = a.es[81 * (size_t)p + idx];
+            const double e1 = eh[81 * (size_t)p + idx], e2 = es[81 * (size_t)p + idx];
             hr = hr + e1 * z.x; hi = hi + e1 * z.y;
             sr = sr + e2 * z.x; si = si + e2 * z.y;
         }
         double2 hv, sv;
         if (ia == ja && b == o) {            // diagonal: real, plus on-site terms
             const int cls = o == 0 ? 0 : (o <= 3 ? 1 : (o <= 6 ? 2 : 3));
-            hv = make_double2(hr + a.onsite[4 * ia + cls], 0.0);
-            sv = make_double2(sr + (a.onsite_s ? 1.0 : 0.0), 0.0);
+            hv = make_double2(hr + onsite[4 * ia + cls], 0.0);
+            sv = make_double2(sr + (a.deriv ? 0.0 : 1.0), 0.0);
         } else if (upper) {
             hv = make_double2(hr, hi); sv = make_double2(sr, si);
         } else {
@@ -808,9 +828,10 @@ __global__ __launch_bounds__(256) void k_stebz(SteinArgs s)
 {
     __shared__ double d[1280], e[1280], e2[1280];
     __shared__ double red4[64];
-    const int N = s.n, kb = blockIdx.y, tid = threadIdx.x;
-    const double *dg = s.d + (size_t)kb * N;
-    const double *eg = s.e + (size_t)kb * N;
+    const int LD = s.n, kb = blockIdx.y, tid = threadIdx.x;
+    const int N = s.nact ? s.nact[kb] : LD;           // active order (padding after)
+    const double *dg = s.d + (size_t)kb * LD;
+    const double *eg = s.e + (size_t)kb * LD;
     for (int i = tid; i < N; i += blockDim.x) {
         d[i] = dg[i];
         e[i] = (i < N - 1) ? eg[i] : 0.0;
@@ -821,7 +842,8 @@ __global__ __launch_bounds__(256) void k_stebz(SteinArgs s)
     const double eps = DBL_EPSILON * 0.5;
     if (blockIdx.x == 0 && tid == 0) s.info[kb] = 0;
     const int j = blockIdx.x * blockDim.x + tid;
-    if (j >= N) return;
+    if (j >= LD) return;
+    if (j >= N) { s.w[(size_t)kb * LD + j] = dg[j]; return; }    // decoupled padding
     double a = tb.gl, b = tb.gu;
     for (int it = 0; it < 200; ++it) {
         const double mid = 0.5 * (a + b);
@@ -829,7 +851,7 @@ __global__ __launch_bounds__(256) void k_stebz(SteinArgs s)
         if (b - a <= tol || mid <= a || mid >= b) break;
         if (sturm_count(d, e2, N, mid, tb.pivmin) >= j + 1) b = mid; else a = mid;
     }
-    s.w[(size_t)kb * N + j] = 0.5 * (a + b);
+    s.w[(size_t)kb * LD + j] = 0.5 * (a + b);
 }
 
 // Eigenvectors of the tridiagonal, one thread per group of close
@@ -846,20 +868,21 @@ __global__ __launch_bounds__(64) void k_stein_vec(SteinArgs s)
     __shared__ int cstart[1281];
     __shared__ int ncl;
     __shared__ double red4[16];
-    const int N = s.n;
+    const int LD = s.n;                               // stride; order of the padded batch
     const int kb = blockIdx.y;
+    const int N = s.nact ? s.nact[kb] : LD;           // active order
     const int tid = threadIdx.x;
     const int nthr = gridDim.x * blockDim.x;                  // threads per k
     const int gt = blockIdx.x * blockDim.x + tid;
-    const double *dg = s.d + (size_t)kb * N;
-    const double *eg = s.e + (size_t)kb * N;
-    double *Vt = s.v + (size_t)kb * N * N;            // Vt[i*N + j] = v_j(i)
-    double *scr = s.scratch + (size_t)kb * 5 * N * nthr;
+    const double *dg = s.d + (size_t)kb * LD;
+    const double *eg = s.e + (size_t)kb * LD;
+    double *Vt = s.v + (size_t)kb * LD * LD;          // Vt[i*LD + j] = v_j(i)
+    double *scr = s.scratch + (size_t)kb * 5 * LD * nthr;
     for (int i = tid; i < N; i += blockDim.x) {
         d[i] = dg[i];
         e[i] = (i < N - 1) ? eg[i] : 0.0;
         e2[i] = e[i] * e[i];
-        w[i] = s.w[(size_t)kb * N + i];
+        w[i] = s.w[(size_t)kb * LD + i];
     }
     __syncthreads();
     const TriBounds tb = tri_bounds(d, e, e2, N, red4);
@@ -904,37 +927,37 @@ __global__ __launch_bounds__(64) void k_stein_vec(SteinArgs s)
                 int r = N - 1;
                 for (int i = N - 2; i >= 0; --i) {
                     const double u = e[i] * rcp_nr(dm);
-                    Vt[(size_t)(i + 1) * N + j] = u;
+                    Vt[(size_t)(i + 1) * LD + j] = u;
                     dm = d[i] - lam - u * e[i];
                     if (fabs(dm) < pivmin) dm = -pivmin;
                     const double g = fabs(AT(wa, i) + dm - (d[i] - lam));
                     if (g < gbest) { gbest = g; r = i; }
                 }
                 double z = 1.0, ss = 1.0;
-                Vt[(size_t)r * N + j] = 1.0;
+                Vt[(size_t)r * LD + j] = 1.0;
                 for (int i = r; i < N - 1; ++i) {
-                    z = -Vt[(size_t)(i + 1) * N + j] * z;
-                    Vt[(size_t)(i + 1) * N + j] = z;
+                    z = -Vt[(size_t)(i + 1) * LD + j] * z;
+                    Vt[(size_t)(i + 1) * LD + j] = z;
                     ss += z * z;
                 }
                 z = 1.0;
                 for (int i = r - 1; i >= 0; --i) {
                     z = -(e[i] * rcp_nr(AT(wa, i))) * z;
-                    Vt[(size_t)i * N + j] = z;
+                    Vt[(size_t)i * LD + j] = z;
                     ss += z * z;
                 }
                 if (j > j0) {
                     // orthogonalise against the group's earlier vectors
                     for (int jp = j0; jp < j; ++jp) {
                         double dot = 0.0;
-                        for (int i = 0; i < N; ++i) dot += Vt[(size_t)i * N + j] * Vt[(size_t)i * N + jp];
-                        for (int i = 0; i < N; ++i) Vt[(size_t)i * N + j] -= dot * Vt[(size_t)i * N + jp];
+                        for (int i = 0; i < N; ++i) dot += Vt[(size_t)i * LD + j] * Vt[(size_t)i * LD + jp];
+                        for (int i = 0; i < N; ++i) Vt[(size_t)i * LD + j] -= dot * Vt[(size_t)i * LD + jp];
                     }
                     ss = 0.0;
-                    for (int i = 0; i < N; ++i) ss += Vt[(size_t)i * N + j] * Vt[(size_t)i * N + j];
+                    for (int i = 0; i < N; ++i) ss += Vt[(size_t)i * LD + j] * Vt[(size_t)i * LD + j];
                 }
                 const double scl = 1.0 / sqrt(ss);
-                for (int i = 0; i < N; ++i) Vt[(size_t)i * N + j] *= scl;
+                for (int i = 0; i < N; ++i) Vt[(size_t)i * LD + j] *= scl;
                 xjm = lam;
                 continue;
             }
@@ -996,36 +1019,36 @@ __global__ __launch_bounds__(64) void k_stein_vec(SteinArgs s)
             tolp *= eps;
             if (tolp == 0.0) tolp = eps;
             // start vector in the output column of Vt
-            for (int i = 0; i < N; ++i) Vt[(size_t)i * N + j] = start_entry(i, j);
+            for (int i = 0; i < N; ++i) Vt[(size_t)i * LD + j] = start_entry(i, j);
             int its = 0, nrmchk = 0;
             bool ok = false;
             while (its < 5) {
                 ++its;
                 double asum = 0.0;
-                for (int i = 0; i < N; ++i) asum += fabs(Vt[(size_t)i * N + j]);
+                for (int i = 0; i < N; ++i) asum += fabs(Vt[(size_t)i * LD + j]);
                 const double scl = N * onenrm * fmax(eps, fabs(AT(wa, N - 1))) / fmax(asum, DBL_MIN);
                 // forward: apply P and L
                 double yprev = Vt[j] * scl;
                 Vt[j] = yprev;
                 for (int k = 1; k < N; ++k) {
-                    double yk = Vt[(size_t)k * N + j] * scl;
+                    double yk = Vt[(size_t)k * LD + j] * scl;
                     if (AT(wy, k - 1) == 0.0) {
                         yk = yk - AT(wc, k - 1) * yprev;
                     } else {
                         const double t = yprev;
-                        Vt[(size_t)(k - 1) * N + j] = yk;
+                        Vt[(size_t)(k - 1) * LD + j] = yk;
                         yk = t - AT(wc, k - 1) * yk;
                     }
-                    Vt[(size_t)k * N + j] = yk;
+                    Vt[(size_t)k * LD + j] = yk;
                     yprev = yk;
                 }
                 // back substitution with perturbed tiny pivots (dlagts, job -1)
                 for (int k = N - 1; k >= 0; --k) {
-                    double temp = Vt[(size_t)k * N + j];
+                    double temp = Vt[(size_t)k * LD + j];
                     if (k <= N - 3)
-                        temp = temp - AT(wb, k) * Vt[(size_t)(k + 1) * N + j] - AT(wd, k) * Vt[(size_t)(k + 2) * N + j];
+                        temp = temp - AT(wb, k) * Vt[(size_t)(k + 1) * LD + j] - AT(wd, k) * Vt[(size_t)(k + 2) * LD + j];
                     else if (k == N - 2)
-                        temp = temp - AT(wb, k) * Vt[(size_t)(k + 1) * N + j];
+                        temp = temp - AT(wb, k) * Vt[(size_t)(k + 1) * LD + j];
                     double ak = AT(wa, k);
                     double pert = copysign(tolp, ak);
                     for (int g = 0; g < 64; ++g) {
@@ -1040,16 +1063,16 @@ __global__ __launch_bounds__(64) void k_stein_vec(SteinArgs s)
                         }
                         break;
                     }
-                    Vt[(size_t)k * N + j] = temp / ak;
+                    Vt[(size_t)k * LD + j] = temp / ak;
                 }
                 // reorthogonalise against earlier members of the group
                 for (int jp = j0; jp < j; ++jp) {
                     double dot = 0.0;
-                    for (int i = 0; i < N; ++i) dot += Vt[(size_t)i * N + j] * Vt[(size_t)i * N + jp];
-                    for (int i = 0; i < N; ++i) Vt[(size_t)i * N + j] -= dot * Vt[(size_t)i * N + jp];
+                    for (int i = 0; i < N; ++i) dot += Vt[(size_t)i * LD + j] * Vt[(size_t)i * LD + jp];
+                    for (int i = 0; i < N; ++i) Vt[(size_t)i * LD + j] -= dot * Vt[(size_t)i * LD + jp];
                 }
                 double amax = 0.0;
-                for (int i = 0; i < N; ++i) amax = fmax(amax, fabs(Vt[(size_t)i * N + j]));
+                for (int i = 0; i < N; ++i) amax = fmax(amax, fabs(Vt[(size_t)i * LD + j]));
                 if (amax < dtpcrt) continue;
                 ++nrmchk;
                 if (nrmchk < 3) continue;
@@ -1060,18 +1083,28 @@ __global__ __launch_bounds__(64) void k_stein_vec(SteinArgs s)
             double ss = 0.0, amax = 0.0;
             int imax = 0;
             for (int i = 0; i < N; ++i) {
-                const double v = Vt[(size_t)i * N + j];
+                const double v = Vt[(size_t)i * LD + j];
                 ss += v * v;
                 if (fabs(v) > amax) { amax = fabs(v); imax = i; }
             }
             double scl = 1.0 / sqrt(ss);
-            if (Vt[(size_t)imax * N + j] < 0.0) scl = -scl;
-            for (int i = 0; i < N; ++i) Vt[(size_t)i * N + j] *= scl;
+            if (Vt[(size_t)imax * LD + j] < 0.0) scl = -scl;
+            for (int i = 0; i < N; ++i) Vt[(size_t)i * LD + j] *= scl;
             xjm = xj;
         }
     }
 #undef AT
     if (fails) atomicAdd(s.info + kb, fails);
+    if (N < LD && blockIdx.x == 0) {       // padding: zero rows, unit columns
+        for (int t = tid; t < (LD - N) * LD; t += blockDim.x) {
+            const int i = N + t / LD, j = t % LD;
+            Vt[(size_t)i * LD + j] = (i == j) ? 1.0 : 0.0;
+        }
+        for (int t = tid; t < N * (LD - N); t += blockDim.x) {
+            const int i = t / (LD - N), j = N + t % (LD - N);
+            Vt[(size_t)i * LD + j] = 0.0;
+        }
+    }
 }
 
 
@@ -1127,8 +1160,9 @@ __global__ void k_real_to_c32_t(int N, const double *Vt, float2 *Y, size_t strid
 // order (F[:, j] = (I + E)[:, perm_j]), so that Y F is the refined, sorted
 // eigenvector matrix.
 constexpr int kOaMax = 1536;
-__global__ __launch_bounds__(1024) void k_oa_stats(int N, const double2 *T, const double2 *G, size_t stride,
-                                                   double *lam_out, int *perm, double *delta_out, double *w)
+__global__ __launch_bounds__(1024) void k_oa_stats(int N, const int *nact, const double2 *T, const double2 *G,
+                                                   size_t stride, double *lam_out, int *perm, double *delta_out,
+                                                   double *w)
 {
     __shared__ double lam[kOaMax], lsort[kOaMax];
     __shared__ int idx[kOaMax];
@@ -1136,11 +1170,12 @@ __global__ __launch_bounds__(1024) void k_oa_stats(int N, const double2 *T, cons
     const int kb = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const double2 *Tk = T + (size_t)kb * stride;
     const double2 *Gk = G + (size_t)kb * stride;
+    const int Na = nact ? nact[kb] : N;               // padding (decoupled) after Na
     double lmax = 0.0;
     for (int i = tid; i < N; i += blockDim.x) {
         const double l = Tk[(size_t)i * N + i].x / Gk[(size_t)i * N + i].x;   // G_ii = 1 - R_ii
         lam[i] = l; lsort[i] = l; idx[i] = i;
-        lmax = fmax(lmax, fabs(l));
+        if (i < Na) lmax = fmax(lmax, fabs(l));
     }
     __syncthreads();
     double ft = 0.0, fr = 0.0;
@@ -1175,8 +1210,8 @@ __global__ __launch_bounds__(1024) void k_oa_stats(int N, const double2 *T, cons
         double a = 0.0, b = 0.0, c = 0.0;
         for (int q = 0; q < (int)(blockDim.x >> 6); ++q) { a += red[0][q]; b += red[1][q]; c = fmax(c, red[2][q]); }
         delta_out[kb] = 2.0 * (sqrt(a) + c * sqrt(b));
-        // stable insertion sort (the input is nearly sorted)
-        for (int i = 1; i < N; ++i) {
+        // stable insertion sort of the active part (the input is nearly sorted)
+        for (int i = 1; i < Na; ++i) {
             const double x = lsort[i];
             const int xi = idx[i];
             int k = i - 1;
@@ -1300,10 +1335,11 @@ __global__ void k_perturb(PerturbArgs p)
 {
     const int kb = blockIdx.x * blockDim.x + threadIdx.x;
     if (kb >= p.nb) return;
-    const int N = p.n;
-    const double *w = p.w + (size_t)kb * N;
-    const double2 *px = p.px + (size_t)kb * N, *py = p.py + (size_t)kb * N;
-    double *desk = p.desk + (size_t)kb * N;
+    const int N = p.nact ? p.nact[kb] : p.n;          // active order (padding after)
+    const int LD = p.n;
+    const double *w = p.w + (size_t)kb * LD;
+    const double2 *px = p.px + (size_t)kb * LD, *py = p.py + (size_t)kb * LD;
+    double *desk = p.desk + (size_t)kb * LD;
     const double2 *Z = p.z + (size_t)kb * p.stride;
     const double2 *X = p.x + (size_t)kb * p.stride;
     const double2 *Y = p.y + (size_t)kb * p.stride;
@@ -1328,8 +1364,8 @@ __global__ void k_perturb(PerturbArgs p)
                     } else {
                         double sr = 0, si = 0;
                         for (int i = 0; i < N; ++i) {
-                            const double2 z = Z[(size_t)ja * N + i];
-                            const double2 q = zsub(X[(size_t)jb * N + i], zscale(es, Y[(size_t)jb * N + i]));
+                            const double2 z = Z[(size_t)ja * LD + i];
+                            const double2 q = zsub(X[(size_t)jb * LD + i], zscale(es, Y[(size_t)jb * LD + i]));
                             const double2 t = zcmul(z, q);
                             sr += t.x; si += t.y;
                         }
@@ -1349,9 +1385,9 @@ __global__ void k_perturb(PerturbArgs p)
         }
         start += num;
     }
-    for (int j = ncut; j < N; ++j) desk[j] = 0.0;
+    for (int j = ncut; j < LD; ++j) desk[j] = 0.0;
     p.ncut[p.k0 + kb] = ncut;
-    if (err) atomicOr(p.err, 2);
+    p.err[p.k0 + kb] = err ? 2 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1386,7 +1422,7 @@ __device__ double block_sum_tree(double v, double *sbuf)
     return r;
 }
 
-__global__ __launch_bounds__(1024) void k_fermi_sums(FermiArgs f)
+__device__ void fermi_sums_body(const FermiArgs &f)
 {
     __shared__ double s1[1024], s2[1024];
     __shared__ double mu_s;
@@ -1521,18 +1557,28 @@ __global__ __launch_bounds__(1024) void k_fermi_sums(FermiArgs f)
     }
 }
 
+__global__ __launch_bounds__(1024) void k_fermi_sums(FermiArgs f) { fermi_sums_body(f); }
+// one workgroup per structure of a plan
+__global__ __launch_bounds__(1024) void k_fermi_sums_group(const FermiArgs *f) { fermi_sums_body(f[blockIdx.x]); }
+
+__global__ void k_copy_int(const int *src, int *dst, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-void launch_phases(int npairs, const double *kpts, const double *dvec, double2 *ph, int nb,
-                   hipStream_t st)
+void launch_phases(const StructDev *sd, const int *item_s, const double *kpts, int maxpairs,
+                   double2 *ph, int nb, hipStream_t st)
 {
-    const int gx = std::max(1, std::min((npairs + 255) / 256, 64));
-    hipLaunchKernelGGL(k_phases, dim3(gx, nb), dim3(256), 0, st, npairs, kpts, dvec, ph);
+    const int gx = std::max(1, std::min((maxpairs + 255) / 256, 64));
+    hipLaunchKernelGGL(k_phases, dim3(gx, nb), dim3(256), 0, st, sd, item_s, kpts, maxpairs, ph);
 }
-void launch_build_hs(const BuildArgs &a, int natoms, int nb, hipStream_t st)
+void launch_build_hs(const BuildArgs &a, int maxnat, int nb, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_build_hs, dim3(natoms, nb), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_build_hs, dim3(maxnat + 1, nb), dim3(256), 0, st, a);
 }
 void launch_potrf_panel(int n, double2 *s, size_t stride, int j0, int *info, int nb,
                         hipStream_t st)
@@ -1652,10 +1698,10 @@ void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int
     hipLaunchKernelGGL(k_real_to_c32_t, dim3(blocks, nb), dim3(256), 0, st, n, vt, y, stride);
 }
 int oa_max_n() { return kOaMax; }
-void launch_oa_update(int n, const double2 *t, const double2 *g, double2 *f, size_t stride, double *lam,
-                      int *perm, double *delta, double *w, int nb, hipStream_t st)
+void launch_oa_update(int n, const int *nact, const double2 *t, const double2 *g, double2 *f, size_t stride,
+                      double *lam, int *perm, double *delta, double *w, int nb, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_oa_stats, dim3(nb), dim3(1024), 0, st, n, t, g, stride, lam, perm, delta, w);
+    hipLaunchKernelGGL(k_oa_stats, dim3(nb), dim3(1024), 0, st, n, nact, t, g, stride, lam, perm, delta, w);
     hipLaunchKernelGGL(k_oa_fmat, dim3((unsigned)(((size_t)n * n + 255) / 256), nb), dim3(256), 0, st, n, t,
                        g, f, stride, lam, perm, delta);
 }
@@ -1667,6 +1713,14 @@ void launch_coldot(int N, const double2 *Z, const double2 *X, const double2 *Y, 
 void launch_perturb(const PerturbArgs &p, hipStream_t st)
 {
     hipLaunchKernelGGL(k_perturb, dim3((p.nb + 63) / 64), dim3(64), 0, st, p);
+}
+void launch_fermi_sums_group(const FermiArgs *f_dev, int nstruct, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fermi_sums_group, dim3(nstruct), dim3(1024), 0, st, f_dev);
+}
+void launch_copy_int(const int *src, int *dst, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_copy_int, dim3((n + 255) / 256), dim3(256), 0, st, src, dst, n);
 }
 void launch_fermi_sums(const FermiArgs &f, hipStream_t st)
 {

@@ -24,12 +24,15 @@ constexpr int kMaxN = 1280;    // LDS-resident vectors of k_hetrd / k_stein
         if (e_ != hipSuccess) return TBK_ERR_HIP;    \
     } while (0)
 
+thread_local size_t g_dev_bytes = 0;     // device bytes allocated by this thread
+
 template <class T>
 int dalloc(T **p, size_t n)
 {
     *p = nullptr;
     if (n == 0) return TBK_OK;
     if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) { *p = nullptr; return TBK_ERR_NOMEM; }
+    g_dev_bytes += n * sizeof(T);
     return TBK_OK;
 }
 template <class T>
@@ -204,7 +207,7 @@ tbk::GemmArgsT<float> cgemm(int m, int n, int k, const float2 *a, int lda, size_
 // tridiagonalisation and back-transform on the FP32 MFMA units; FP64
 // bisection/eigenvectors of the tridiagonal; one FP64 Ogita-Aishima step on
 // the reduced problem C; FP64 Z = U^{-1} Y.
-void solve_batch_mixed(Workspace &ws, int nb, double *w_out, hipStream_t st, StageTimer &tm)
+void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hipStream_t st, StageTimer &tm)
 {
     const int n = ws.n;
     const size_t ss = ws.stride(), sp = (size_t)n * 96;
@@ -230,7 +233,7 @@ void solve_batch_mixed(Workspace &ws, int nb, double *w_out, hipStream_t st, Sta
     tbk::launch_hetrd_finish(h, nb, st);
     tm.end();
     tm.begin("stein");
-    tbk::SteinArgs sa{n, 1e-4, ws.d, ws.e, ws.w, ws.Vt, ws.scratch, ws.sinfo};
+    tbk::SteinArgs sa{n, 1e-4, nact, ws.d, ws.e, ws.w, ws.Vt, ws.scratch, ws.sinfo};
     tbk::launch_stein(sa, nb, st);
     tm.end();
     tm.begin("backtransform");
@@ -253,7 +256,7 @@ void solve_batch_mixed(Workspace &ws, int nb, double *w_out, hipStream_t st, Sta
     tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.Z, n, ss), 1, 0, nb, st);
     tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.Y, n, ss, ws.VT, n, ss), 1, 0, nb, st);
     // X = (I + E) with sorted columns; VT = Y X; Z = U^{-1} VT
-    tbk::launch_oa_update(n, ws.Z, ws.VT, ws.X, ss, ws.lam, ws.perm, ws.delta, w_out, nb, st);
+    tbk::launch_oa_update(n, nact, ws.Z, ws.VT, ws.X, ss, ws.lam, ws.perm, ws.delta, w_out, nb, st);
     tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.VT, n, ss), 0, 0, nb, st);
     tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.VT, n, ss, ws.Z, n, ss, 1), 0, 0, nb, st);
     tm.end();
@@ -261,7 +264,7 @@ void solve_batch_mixed(Workspace &ws, int nb, double *w_out, hipStream_t st, Sta
 
 // Stages 2-9 of the pipeline on ws.A = H, ws.S = S for nb k-points:
 // eigenvalues into w_out[nb*n], eigenvectors into ws.Z.
-void solve_batch(Workspace &ws, int nb, double *w_out, hipStream_t st, StageTimer &tm)
+void solve_batch(Workspace &ws, int nb, const int *nact, double *w_out, hipStream_t st, StageTimer &tm)
 {
     const int n = ws.n;
     const size_t ss = ws.stride();
@@ -277,7 +280,7 @@ void solve_batch(Workspace &ws, int nb, double *w_out, hipStream_t st, StageTime
     hetrd(ws, nb, st);
     tm.end();
     tm.begin("stein");
-    tbk::SteinArgs sa{n, 1e-4, ws.d, ws.e, w_out, ws.Vt, ws.scratch, ws.sinfo};
+    tbk::SteinArgs sa{n, 1e-4, nact, ws.d, ws.e, w_out, ws.Vt, ws.scratch, ws.sinfo};
     tbk::launch_stein(sa, nb, st);
     tm.end();
     tm.begin("backtransform");
@@ -301,34 +304,56 @@ void solve_batch(Workspace &ws, int nb, double *w_out, hipStream_t st, StageTime
 
 }  // namespace
 
+// Host view of one structure of a plan.
+struct StructHost {
+    int natoms = 0, nv = 0, npairs = 0, nk = 0, k0 = 0;   // k0: first item
+    double zfill = 0.0, thisvol = 0.0;
+};
+
 struct tbk_plan {
     int device = 0;
     hipStream_t stream = nullptr;
-    int natoms = 0, nv = 0, npairs = 0, nk = 0, batch = 0, precision = TBK_FP64;
-    double thisvol = 0.0, zfill = 0.0;
+    int nstruct = 0, N = 0, nk = 0, batch = 0, precision = TBK_FP64, maxnat = 0, maxpairs = 0;
+    std::vector<StructHost> sh;
     std::vector<double> kpts, weights;
-    // structure data on the device
+    std::vector<int> item_s_host;
+    // concatenated set-up arrays of all structures
     int *row_atom = nullptr, *row_orb = nullptr, *ksk = nullptr, *nbas = nullptr, *blk_ptr = nullptr;
     double *dvec = nullptr, *eh = nullptr, *es = nullptr, *edh = nullptr, *eds = nullptr;
     double *paros = nullptr, *dparos = nullptr, *kp = nullptr, *wts = nullptr;
-    // results
-    double *esk = nullptr, *desk = nullptr, *out = nullptr;
+    tbk::StructDev *sdev = nullptr;        // [nstruct]
+    tbk::FermiArgs *fargs = nullptr;       // [nstruct]
+    int *item_s = nullptr, *nact = nullptr; // [nk]
+    // results, per item with stride N
+    double *esk = nullptr, *desk = nullptr, *out = nullptr;   // out [nstruct][8]
     double2 *ph = nullptr;          // Bloch phases of the current batch
-    int *ncut = nullptr, *err = nullptr;
-    std::vector<int> info_host;
+    int *ncut = nullptr, *err = nullptr, *info_all = nullptr;
+    std::vector<int> info_host, err_host, status;
+    std::vector<double> out_host;
     Workspace ws;
     StageTimer timer;
     int last_mode = 0;
+    size_t bytes = 0;
+    // hipGraph of the launch sequence (replayed while mode, tkb, eigcut are unchanged)
+    bool use_graph = false, recording = false;
+    hipGraphExec_t gexec = nullptr;
+    int g_mode = 0;
+    double g_tkb = 0.0, g_eigcut = 0.0;
+    int f_mode = 0;                 // (mode, kT) of the device Fermi arguments
+    double f_tkb = 0.0;
 };
 
 static void plan_free(tbk_plan *p)
 {
     if (!p) return;
     (void)hipSetDevice(p->device);
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     dfree(p->row_atom); dfree(p->row_orb); dfree(p->ksk); dfree(p->nbas); dfree(p->blk_ptr);
     dfree(p->dvec); dfree(p->eh); dfree(p->es); dfree(p->edh); dfree(p->eds);
     dfree(p->paros); dfree(p->dparos); dfree(p->kp); dfree(p->wts);
-    dfree(p->esk); dfree(p->desk); dfree(p->out); dfree(p->ncut); dfree(p->err); dfree(p->ph);
+    dfree(p->sdev); dfree(p->fargs); dfree(p->item_s); dfree(p->nact);
+    dfree(p->esk); dfree(p->desk); dfree(p->out); dfree(p->ncut); dfree(p->err); dfree(p->info_all);
+    dfree(p->ph);
     p->ws.release();
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
@@ -344,19 +369,18 @@ static int upload(T **dst, const std::vector<T> &src)
     return TBK_OK;
 }
 
-extern "C" int tbk_plan_create(int device, const tbk_structure_desc *st, int nk,
-                               const double *kpts, const double *weights, int batch,
-                               int precision, tbk_plan **out)
+// Host set-up of one structure (bit-exact search2 / setpar / setparv and the
+// SK blocks), appended to the concatenated arrays.
+struct SetupArrays {
+    std::vector<int> row_atom, row_orb, ksk, nbas, blk_ptr;
+    std::vector<double> dvec, eh, es, edh, eds, paros, dparos;
+};
+struct SetupOffsets { size_t rows, atoms, blk, pairs; };
+
+static int setup_structure(const tbk_structure_desc *st, SetupArrays &A, StructHost &h, SetupOffsets &off)
 {
-    if (!st || !out || nk <= 0 || !kpts || !weights || st->natoms <= 0) return TBK_ERR_ARG;
-    if (precision != TBK_FP64 && precision != TBK_MIXED) return TBK_ERR_ARG;
-    *out = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return TBK_ERR_NODEVICE;
-    if (device < 0 || device >= ndev) return TBK_ERR_ARG;
-    if (hipSetDevice(device) != hipSuccess) return TBK_ERR_HIP;
+    if (!st || st->natoms <= 0) return TBK_ERR_ARG;
     const int nat = st->natoms, kinds = st->kinds;
-    // ---- host set-up (bit-exact) ----
     tbk::PairList pl;
     const int maxpairs = std::max(4096, 4000 * nat);
     int rc = tbk::search2(nat, st->lat, st->posn, st->kkind, kinds, st->nd, st->dnn, st->screenl,
@@ -379,105 +403,180 @@ extern "C" int tbk_plan_create(int device, const tbk_structure_desc *st, int nk,
         nv += nbas[a];
     }
     if (nv < 2 || nv > kMaxN) return TBK_ERR_ARG;
-    std::vector<int> row_atom(nv), row_orb(nv);
+    off.rows = A.row_atom.size(); off.atoms = A.ksk.size(); off.blk = A.blk_ptr.size();
+    off.pairs = A.dvec.size() / 3;
     for (int a = 0; a < nat; ++a)
-        for (int o = 0; o < nbas[a]; ++o) { row_atom[ksk[a] + o] = a; row_orb[ksk[a] + o] = o; }
-    std::vector<double> eh(81 * (size_t)np), es(81 * (size_t)np), edh(81 * (size_t)np),
-        eds(81 * (size_t)np), dvec(3 * (size_t)np);
-    std::vector<int> blk_ptr((size_t)nat * nat + 1, 0);
+        for (int o = 0; o < nbas[a]; ++o) { A.row_atom.push_back(a); A.row_orb.push_back(o); }
+    A.ksk.insert(A.ksk.end(), ksk.begin(), ksk.end());
+    A.nbas.insert(A.nbas.end(), nbas.begin(), nbas.end());
+    A.paros.insert(A.paros.end(), paros.begin(), paros.end());
+    A.dparos.insert(A.dparos.end(), dparos.begin(), dparos.end());
+    std::vector<int> blk((size_t)nat * nat + 1, 0);
+    const size_t e0 = A.eh.size();
+    A.eh.resize(e0 + 81 * (size_t)np); A.es.resize(e0 + 81 * (size_t)np);
+    A.edh.resize(e0 + 81 * (size_t)np); A.eds.resize(e0 + 81 * (size_t)np);
     for (int p = 0; p < np; ++p) {
         const tbk::Pair &q = pl.pairs[p];
         double v[14];
         const bool unlike = q.kind[0] != q.kind[1];
+        const size_t o = e0 + 81 * (size_t)p;
         tbk::arrange_integrals(&hop[14 * p], q.kind[0], q.kind[1], v);
-        tbk::sk_block(q.cosines[0], q.cosines[1], q.cosines[2], v, unlike, &eh[81 * p]);
+        tbk::sk_block(q.cosines[0], q.cosines[1], q.cosines[2], v, unlike, &A.eh[o]);
         tbk::arrange_integrals(&over[14 * p], q.kind[0], q.kind[1], v);
-        tbk::sk_block(q.cosines[0], q.cosines[1], q.cosines[2], v, unlike, &es[81 * p]);
+        tbk::sk_block(q.cosines[0], q.cosines[1], q.cosines[2], v, unlike, &A.es[o]);
         tbk::arrange_integrals(&dhop[14 * p], q.kind[0], q.kind[1], v);
-        tbk::sk_block(q.cosines[0], q.cosines[1], q.cosines[2], v, unlike, &edh[81 * p]);
+        tbk::sk_block(q.cosines[0], q.cosines[1], q.cosines[2], v, unlike, &A.edh[o]);
         tbk::arrange_integrals(&dover[14 * p], q.kind[0], q.kind[1], v);
-        tbk::sk_block(q.cosines[0], q.cosines[1], q.cosines[2], v, unlike, &eds[81 * p]);
+        tbk::sk_block(q.cosines[0], q.cosines[1], q.cosines[2], v, unlike, &A.eds[o]);
         const int i = q.atom[0], j = q.atom[1];
         for (int c = 0; c < 3; ++c)    // rotate.f:144-146 operand order
-            dvec[3 * p + c] = q.dlv[c] - st->posq[3 * j + c] + st->posq[3 * i + c];
-        blk_ptr[(size_t)i * nat + j + 1]++;
+            A.dvec.push_back(q.dlv[c] - st->posq[3 * j + c] + st->posq[3 * i + c]);
+        blk[(size_t)i * nat + j + 1]++;
     }
-    for (size_t b = 1; b < blk_ptr.size(); ++b) blk_ptr[b] += blk_ptr[b - 1];
+    for (size_t b = 1; b < blk.size(); ++b) blk[b] += blk[b - 1];
+    A.blk_ptr.insert(A.blk_ptr.end(), blk.begin(), blk.end());
+    h.natoms = nat; h.nv = nv; h.npairs = np;
+    h.zfill = st->zfill; h.thisvol = st->thisvol;
+    return TBK_OK;
+}
+
+extern "C" int tbk_plan_create_group(int device, int nstruct, const tbk_structure_desc *st, const int *nk,
+                                     const double *kpts, const double *weights, int batch, int precision,
+                                     tbk_plan **out)
+{
+    if (!st || !out || !nk || nstruct <= 0 || !kpts || !weights) return TBK_ERR_ARG;
+    if (precision != TBK_FP64 && precision != TBK_MIXED) return TBK_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return TBK_ERR_NODEVICE;
+    if (device < 0 || device >= ndev) return TBK_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return TBK_ERR_HIP;
+    // ---- host set-up (bit-exact) of every structure ----
+    SetupArrays A;
+    std::vector<StructHost> sh(nstruct);
+    std::vector<SetupOffsets> off(nstruct);
+    int nktot = 0, N = 0, maxnat = 0, maxpairs = 1;
+    for (int i = 0; i < nstruct; ++i) {
+        if (nk[i] <= 0) return TBK_ERR_ARG;
+        int rc = setup_structure(st + i, A, sh[i], off[i]);
+        if (rc) return rc;
+        sh[i].nk = nk[i]; sh[i].k0 = nktot;
+        nktot += nk[i];
+        N = std::max(N, sh[i].nv);
+        maxnat = std::max(maxnat, sh[i].natoms);
+        maxpairs = std::max(maxpairs, sh[i].npairs);
+    }
+    if (precision == TBK_MIXED && N > tbk::oa_max_n()) return TBK_ERR_ARG;
     // ---- device state ----
     tbk_plan *p = new tbk_plan();
+    g_dev_bytes = 0;
     p->device = device;
-    p->natoms = nat; p->nv = nv; p->npairs = np; p->nk = nk; p->precision = precision;
-    p->thisvol = st->thisvol; p->zfill = st->zfill;
-    p->kpts.assign(kpts, kpts + 3 * (size_t)nk);
-    p->weights.assign(weights, weights + nk);
-    p->batch = (batch <= 0 || batch > nk) ? nk : batch;
+    p->nstruct = nstruct; p->N = N; p->nk = nktot; p->precision = precision;
+    p->maxnat = maxnat; p->maxpairs = maxpairs;
+    p->sh = sh;
+    p->kpts.assign(kpts, kpts + 3 * (size_t)nktot);
+    p->weights.assign(weights, weights + nktot);
+    p->batch = (batch <= 0 || batch > nktot) ? nktot : batch;
     if (hipStreamCreate(&p->stream) != hipSuccess) { delete p; return TBK_ERR_HIP; }
-    rc = TBK_OK;
-    rc |= upload(&p->row_atom, row_atom); rc |= upload(&p->row_orb, row_orb);
-    rc |= upload(&p->ksk, ksk); rc |= upload(&p->nbas, nbas); rc |= upload(&p->blk_ptr, blk_ptr);
-    rc |= upload(&p->dvec, dvec); rc |= upload(&p->eh, eh); rc |= upload(&p->es, es);
-    rc |= upload(&p->edh, edh); rc |= upload(&p->eds, eds);
-    rc |= upload(&p->paros, paros); rc |= upload(&p->dparos, dparos);
+    int rc = TBK_OK;
+    rc |= upload(&p->row_atom, A.row_atom); rc |= upload(&p->row_orb, A.row_orb);
+    rc |= upload(&p->ksk, A.ksk); rc |= upload(&p->nbas, A.nbas); rc |= upload(&p->blk_ptr, A.blk_ptr);
+    rc |= upload(&p->dvec, A.dvec); rc |= upload(&p->eh, A.eh); rc |= upload(&p->es, A.es);
+    rc |= upload(&p->edh, A.edh); rc |= upload(&p->eds, A.eds);
+    rc |= upload(&p->paros, A.paros); rc |= upload(&p->dparos, A.dparos);
     rc |= upload(&p->kp, p->kpts); rc |= upload(&p->wts, p->weights);
-    rc |= dalloc(&p->esk, (size_t)nk * nv); rc |= dalloc(&p->desk, (size_t)nk * nv);
-    rc |= dalloc(&p->ncut, (size_t)nk); rc |= dalloc(&p->err, 1); rc |= dalloc(&p->out, 8);
-    if (!rc) rc = dalloc(&p->ph, (size_t)p->batch * std::max(np, 1));
-    if (!rc) rc = p->ws.alloc(nv, p->batch);
-    if (!rc && precision == TBK_MIXED && nv > tbk::oa_max_n()) rc = TBK_ERR_ARG;
+    std::vector<int> item_s(nktot), nact(nktot);
+    for (int i = 0; i < nstruct; ++i)
+        for (int k = 0; k < sh[i].nk; ++k) { item_s[sh[i].k0 + k] = i; nact[sh[i].k0 + k] = sh[i].nv; }
+    p->item_s_host = item_s;
+    rc |= upload(&p->item_s, item_s); rc |= upload(&p->nact, nact);
+    rc |= dalloc(&p->esk, (size_t)nktot * N); rc |= dalloc(&p->desk, (size_t)nktot * N);
+    rc |= dalloc(&p->ncut, (size_t)nktot); rc |= dalloc(&p->err, (size_t)nktot);
+    rc |= dalloc(&p->info_all, (size_t)nktot); rc |= dalloc(&p->out, 8 * (size_t)nstruct);
+    if (!rc) {
+        std::vector<tbk::StructDev> sd(nstruct);
+        for (int i = 0; i < nstruct; ++i) {
+            tbk::StructDev &d = sd[i];
+            d.natoms = sh[i].natoms; d.nv = sh[i].nv; d.npairs = sh[i].npairs;
+            d.row_atom = p->row_atom + off[i].rows; d.row_orb = p->row_orb + off[i].rows;
+            d.ksk = p->ksk + off[i].atoms; d.nbas = p->nbas + off[i].atoms;
+            d.blk_ptr = p->blk_ptr + off[i].blk;
+            d.dvec = p->dvec + 3 * off[i].pairs;
+            d.eh = p->eh + 81 * off[i].pairs; d.es = p->es + 81 * off[i].pairs;
+            d.edh = p->edh + 81 * off[i].pairs; d.eds = p->eds + 81 * off[i].pairs;
+            d.paros = p->paros + 4 * off[i].atoms; d.dparos = p->dparos + 4 * off[i].atoms;
+        }
+        rc |= upload(&p->sdev, sd);
+        std::vector<tbk::FermiArgs> fa(nstruct);
+        rc |= upload(&p->fargs, fa);        // filled per run (mode, kT)
+    }
+    if (!rc) rc = dalloc(&p->ph, (size_t)p->batch * maxpairs);
+    if (!rc) rc = p->ws.alloc(N, p->batch);
     if (!rc && precision == TBK_MIXED) rc = p->ws.alloc_mixed();
     if (rc) { plan_free(p); return rc == TBK_ERR_HIP ? TBK_ERR_HIP : TBK_ERR_NOMEM; }
+    p->bytes = g_dev_bytes;
     p->timer.st = p->stream;
-    p->info_host.assign(nk, 0);
+    p->info_host.assign(nktot, 0);
+    p->err_host.assign(nktot, 0);
+    p->status.assign(nstruct, TBK_OK);
+    p->out_host.assign(8 * (size_t)nstruct, 0.0);
     *out = p;
     return TBK_OK;
 }
 
+extern "C" int tbk_plan_create(int device, const tbk_structure_desc *st, int nk,
+                               const double *kpts, const double *weights, int batch,
+                               int precision, tbk_plan **out)
+{
+    if (!st || nk <= 0) return TBK_ERR_ARG;
+    return tbk_plan_create_group(device, 1, st, &nk, kpts, weights, batch, precision, out);
+}
+
+// Padding diagonal of H: above every level of a tight-binding structure in
+// Ry, and exactly decoupled from the active block.
+constexpr double kPadLevel = 1.0e4;
+
 static tbk::BuildArgs build_args(tbk_plan *p, int k0, bool deriv)
 {
     tbk::BuildArgs a;
-    a.n = p->nv; a.natoms = p->natoms; a.onsite_s = deriv ? 0 : 1; a.npairs = p->npairs;
-    a.ph = p->ph;
+    a.n = p->N; a.deriv = deriv ? 1 : 0; a.maxpairs = p->maxpairs;
     a.stride = p->ws.stride();
-    a.row_atom = p->row_atom; a.row_orb = p->row_orb; a.ksk = p->ksk; a.nbas = p->nbas;
-    a.blk_ptr = p->blk_ptr; a.dvec = p->dvec;
-    a.eh = deriv ? p->edh : p->eh; a.es = deriv ? p->eds : p->es;
-    a.onsite = deriv ? p->dparos : p->paros;
+    a.pad = kPadLevel;
+    a.sd = p->sdev;
+    a.item_s = p->item_s + k0;
+    a.ph = p->ph;
     a.h = p->ws.A; a.s = p->ws.S;
     return a;
 }
 
-extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
+// Everything one run enqueues on the plan's stream (captured into a
+// hipGraph when graphs are on).
+static void enqueue_run(tbk_plan *p, int mode, double eigcut)
 {
-    if (!p || mode < 1 || mode > 4) return TBK_ERR_ARG;
-    if (p->zfill <= 0.0) return TBK_ERR_ZFILL;
-    HIP_TRY(hipSetDevice(p->device));
     const bool lpress = mode == 2 || mode == 4;
-    const int n = p->nv;
+    const int n = p->N;
     const size_t ss = p->ws.stride();
     hipStream_t st = p->stream;
-    HIP_TRY(hipMemsetAsync(p->err, 0, sizeof(int), st));
-    HIP_TRY(hipMemsetAsync(p->desk, 0, sizeof(double) * (size_t)p->nk * n, st));
-    p->last_mode = mode;
     for (int k0 = 0; k0 < p->nk; k0 += p->batch) {
         const int nb = std::min(p->batch, p->nk - k0);
         p->timer.begin("build_hs");
-        tbk::launch_phases(p->npairs, p->kp + 3 * (size_t)k0, p->dvec, p->ph, nb, st);
-        tbk::launch_build_hs(build_args(p, k0, false), p->natoms, nb, st);
+        tbk::launch_phases(p->sdev, p->item_s + k0, p->kp + 3 * (size_t)k0, p->maxpairs, p->ph, nb, st);
+        tbk::launch_build_hs(build_args(p, k0, false), p->maxnat, nb, st);
         p->timer.end();
+        const int *nact = p->nact + k0;
         if (p->precision == TBK_MIXED)
-            solve_batch_mixed(p->ws, nb, p->esk + (size_t)k0 * n, st, p->timer);
+            solve_batch_mixed(p->ws, nb, nact, p->esk + (size_t)k0 * n, st, p->timer);
         else
-            solve_batch(p->ws, nb, p->esk + (size_t)k0 * n, st, p->timer);
-        HIP_TRY(hipMemcpyAsync(p->info_host.data() + k0, p->ws.info, sizeof(int) * nb,
-                               hipMemcpyDeviceToHost, st));
+            solve_batch(p->ws, nb, nact, p->esk + (size_t)k0 * n, st, p->timer);
+        tbk::launch_copy_int(p->ws.info, p->info_all + k0, nb, st);
         if (lpress) {
             p->timer.begin("pressure");
-            tbk::launch_build_hs(build_args(p, k0, true), p->natoms, nb, st);
+            tbk::launch_build_hs(build_args(p, k0, true), p->maxnat, nb, st);
             tbk::launch_zgemm(gemm(n, n, n, p->ws.A, n, ss, p->ws.Z, n, ss, p->ws.X, n, ss), 0, 0, nb, st);
             tbk::launch_zgemm(gemm(n, n, n, p->ws.S, n, ss, p->ws.Z, n, ss, p->ws.Y, n, ss), 0, 0, nb, st);
             tbk::launch_coldot(n, p->ws.Z, p->ws.X, p->ws.Y, ss, p->ws.px, p->ws.py, nb, st);
             tbk::PerturbArgs pa;
-            pa.n = n; pa.nb = nb; pa.k0 = k0; pa.stride = ss; pa.eigcut = eigcut;
+            pa.n = n; pa.nb = nb; pa.k0 = k0; pa.stride = ss; pa.eigcut = eigcut; pa.nact = nact;
             pa.w = p->esk + (size_t)k0 * n; pa.px = p->ws.px; pa.py = p->ws.py;
             pa.z = p->ws.Z; pa.x = p->ws.X; pa.y = p->ws.Y;
             pa.desk = p->desk + (size_t)k0 * n; pa.ncut = p->ncut; pa.err = p->err;
@@ -486,13 +585,55 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
         }
     }
     p->timer.begin("fermi_sums");
-    tbk::FermiArgs f;
-    f.nk = p->nk; f.nv = n; f.ld = n; f.mode = mode;
-    f.el = 2.0; f.tkb = tkb; f.zfill = p->zfill;
-    f.esk = p->esk; f.desk = p->desk; f.w = p->wts; f.ncut = p->ncut; f.out = p->out;
-    tbk::launch_fermi_sums(f, st);
+    tbk::launch_fermi_sums_group(p->fargs, p->nstruct, st);
     p->timer.end();
-    if (hipGetLastError() != hipSuccess) return TBK_ERR_HIP;
+}
+
+extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
+{
+    if (!p || mode < 1 || mode > 4) return TBK_ERR_ARG;
+    for (const StructHost &h : p->sh)
+        if (h.zfill <= 0.0) return TBK_ERR_ZFILL;
+    HIP_TRY(hipSetDevice(p->device));
+    p->last_mode = mode;
+    const bool params_changed = !(p->g_mode == mode && p->g_tkb == tkb && p->g_eigcut == eigcut);
+    if (p->f_mode != mode || p->f_tkb != tkb) {
+        // per-structure Fermi arguments of this (mode, kT)
+        std::vector<tbk::FermiArgs> fa(p->nstruct);
+        for (int i = 0; i < p->nstruct; ++i) {
+            const StructHost &h = p->sh[i];
+            tbk::FermiArgs &f = fa[i];
+            f.nk = h.nk; f.nv = h.nv; f.ld = p->N; f.mode = mode;
+            f.el = 2.0; f.tkb = tkb; f.zfill = h.zfill;
+            f.esk = p->esk + (size_t)h.k0 * p->N; f.desk = p->desk + (size_t)h.k0 * p->N;
+            f.w = p->wts + h.k0; f.ncut = p->ncut + h.k0; f.out = p->out + 8 * (size_t)i;
+        }
+        HIP_TRY(hipMemcpyAsync(p->fargs, fa.data(), fa.size() * sizeof(tbk::FermiArgs),
+                               hipMemcpyHostToDevice, p->stream));
+        HIP_TRY(hipStreamSynchronize(p->stream));
+        p->f_mode = mode; p->f_tkb = tkb;
+    }
+    const bool graph = p->use_graph && !p->timer.on && !p->recording;
+    if (!graph) {
+        enqueue_run(p, mode, eigcut);
+        if (hipGetLastError() != hipSuccess) return TBK_ERR_HIP;
+        p->g_mode = mode; p->g_tkb = tkb; p->g_eigcut = eigcut;
+        if (p->gexec && params_changed) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+        return TBK_OK;
+    }
+    if (!p->gexec || params_changed) {
+        if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_run(p, mode, eigcut);
+        const hipError_t ce = hipStreamEndCapture(p->stream, &g);
+        if (ce != hipSuccess || !g) return TBK_ERR_HIP;
+        const hipError_t ie = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) { p->gexec = nullptr; return TBK_ERR_HIP; }
+        p->g_mode = mode; p->g_tkb = tkb; p->g_eigcut = eigcut;
+    }
+    HIP_TRY(hipGraphLaunch(p->gexec, p->stream));
     return TBK_OK;
 }
 
@@ -500,18 +641,28 @@ extern "C" int tbk_plan_wait(tbk_plan *p, double *efermi, double *earg1, double 
 {
     if (!p) return TBK_ERR_ARG;
     HIP_TRY(hipSetDevice(p->device));
-    double o[8];
-    int err = 0;
-    HIP_TRY(hipMemcpyAsync(o, p->out, sizeof(o), hipMemcpyDeviceToHost, p->stream));
-    HIP_TRY(hipMemcpyAsync(&err, p->err, sizeof(int), hipMemcpyDeviceToHost, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
+    HIP_TRY(hipMemcpy(p->out_host.data(), p->out, p->out_host.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(p->info_host.data(), p->info_all, p->nk * sizeof(int), hipMemcpyDeviceToHost));
+    const bool lpress = p->last_mode == 2 || p->last_mode == 4;
+    if (lpress)
+        HIP_TRY(hipMemcpy(p->err_host.data(), p->err, p->nk * sizeof(int), hipMemcpyDeviceToHost));
     p->timer.collect();
-    for (int k = 0; k < p->nk; ++k)
-        if (p->info_host[k] != 0) return TBK_ERR_OVERLAP;
-    if (err & 2) return TBK_ERR_DEGEN;
-    if (efermi) *efermi = o[0];
-    if (earg1) *earg1 = o[1];
-    if (earg2) *earg2 = o[2];
+    int first = TBK_OK;
+    for (int i = 0; i < p->nstruct; ++i) {
+        const StructHost &h = p->sh[i];
+        int st = TBK_OK;
+        for (int k = h.k0; k < h.k0 + h.nk && st == TBK_OK; ++k) {
+            if (p->info_host[k] != 0) st = TBK_ERR_OVERLAP;
+            else if (lpress && (p->err_host[k] & 2)) st = TBK_ERR_DEGEN;
+        }
+        p->status[i] = st;
+        if (st && !first) first = st;
+    }
+    if (first) return first;
+    if (efermi) *efermi = p->out_host[0];
+    if (earg1) *earg1 = p->out_host[1];
+    if (earg2) *earg2 = p->out_host[2];
     return TBK_OK;
 }
 
@@ -523,6 +674,16 @@ extern "C" int tbk_plan_run(tbk_plan *p, int mode, double tkb, double eigcut, do
     return tbk_plan_wait(p, efermi, earg1, earg2);
 }
 
+extern "C" int tbk_plan_result(const tbk_plan *p, int s, double *efermi, double *earg1, double *earg2)
+{
+    if (!p || s < 0 || s >= p->nstruct) return TBK_ERR_ARG;
+    if (p->status[s]) return p->status[s];
+    if (efermi) *efermi = p->out_host[8 * (size_t)s];
+    if (earg1) *earg1 = p->out_host[8 * (size_t)s + 1];
+    if (earg2) *earg2 = p->out_host[8 * (size_t)s + 2];
+    return TBK_OK;
+}
+
 // GEMM autotuner: one mode-4 pass records the plan's GEMM shapes, then each
 // shape not yet in the table is timed with every tile (one warm-up and
 // `reps` launches on the plan's stream) and the fastest tile is kept for the
@@ -532,10 +693,12 @@ extern "C" int tbk_plan_autotune(tbk_plan *p, int reps, int *ntuned)
 {
     if (!p || reps < 1) return TBK_ERR_ARG;
     HIP_TRY(hipSetDevice(p->device));
+    p->recording = true;
     tbk::gemm_record(true);
     int rc = tbk_plan_launch(p, 4, 0.005, 1e30);
-    if (!rc) rc = tbk_plan_wait(p, nullptr, nullptr, nullptr);
+    if (!rc) rc = hipStreamSynchronize(p->stream) == hipSuccess ? TBK_OK : TBK_ERR_HIP;
     tbk::gemm_record(false);
+    p->recording = false;
     if (rc) return rc;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
@@ -561,6 +724,7 @@ extern "C" int tbk_plan_autotune(tbk_plan *p, int reps, int *ntuned)
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }   // tiles changed
     if (ntuned) *ntuned = count;
     if (!rc && hipGetLastError() != hipSuccess) rc = TBK_ERR_HIP;
     return rc;
@@ -573,13 +737,43 @@ extern "C" int tbk_gemm_tile(int precision, int opa, int opb, int m, int n, int 
     return tbk::gemm_tile(precision == TBK_MIXED ? 1 : 0, opa, opb, m, n, k, tri, nb);
 }
 
-extern "C" int tbk_plan_eigenvalues(tbk_plan *p, double *esk, double *desk)
+extern "C" int tbk_plan_set_graph(tbk_plan *p, int on)
 {
     if (!p) return TBK_ERR_ARG;
+    p->use_graph = on != 0;
+    if (!p->use_graph && p->gexec) {
+        (void)hipSetDevice(p->device);
+        (void)hipGraphExecDestroy(p->gexec);
+        p->gexec = nullptr;
+    }
+    return TBK_OK;
+}
+
+extern "C" int tbk_plan_structure_eigenvalues(tbk_plan *p, int s, double *esk, double *desk)
+{
+    if (!p || s < 0 || s >= p->nstruct) return TBK_ERR_ARG;
     HIP_TRY(hipSetDevice(p->device));
-    const size_t n = (size_t)p->nk * p->nv;
-    if (esk) HIP_TRY(hipMemcpy(esk, p->esk, n * sizeof(double), hipMemcpyDeviceToHost));
-    if (desk) HIP_TRY(hipMemcpy(desk, p->desk, n * sizeof(double), hipMemcpyDeviceToHost));
+    const StructHost &h = p->sh[s];
+    const size_t N = p->N;
+    // rows of N (padded order) -> rows of nv
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    if (esk) HIP_TRY(hipMemcpy2D(esk, h.nv * sizeof(double), p->esk + (size_t)h.k0 * N, N * sizeof(double),
+                                 h.nv * sizeof(double), h.nk, hipMemcpyDeviceToHost));
+    if (desk) HIP_TRY(hipMemcpy2D(desk, h.nv * sizeof(double), p->desk + (size_t)h.k0 * N, N * sizeof(double),
+                                  h.nv * sizeof(double), h.nk, hipMemcpyDeviceToHost));
+    return TBK_OK;
+}
+
+extern "C" int tbk_plan_eigenvalues(tbk_plan *p, double *esk, double *desk)
+{
+    return tbk_plan_structure_eigenvalues(p, 0, esk, desk);
+}
+
+// Leading nv x nv block (column stride nv) of an N x N item matrix.
+static int copy_leading(double2 *dst, const double2 *src, int N, int nv)
+{
+    HIP_TRY(hipMemcpy2D(dst, nv * sizeof(double2), src, N * sizeof(double2), nv * sizeof(double2), nv,
+                        hipMemcpyDeviceToHost));
     return TBK_OK;
 }
 
@@ -590,27 +784,29 @@ extern "C" int tbk_plan_eigenvectors(tbk_plan *p, int k, double *z)
     const int last0 = ((p->nk - 1) / p->batch) * p->batch;
     if (k < last0) return TBK_ERR_ARG;
     HIP_TRY(hipSetDevice(p->device));
+    HIP_TRY(hipStreamSynchronize(p->stream));
     const size_t ss = p->ws.stride();
-    HIP_TRY(hipMemcpy(z, p->ws.Z + (size_t)(k - last0) * ss, ss * sizeof(double2),
-                      hipMemcpyDeviceToHost));
-    return TBK_OK;
+    return copy_leading((double2 *)z, p->ws.Z + (size_t)(k - last0) * ss, p->N, p->sh[p->item_s_host[k]].nv);
 }
 
 extern "C" int tbk_plan_hs(tbk_plan *p, int k, int deriv, double *h, double *s)
 {
     if (!p || k < 0 || k >= p->nk) return TBK_ERR_ARG;
     HIP_TRY(hipSetDevice(p->device));
-    tbk::launch_phases(p->npairs, p->kp + 3 * (size_t)k, p->dvec, p->ph, 1, p->stream);
-    tbk::launch_build_hs(build_args(p, k, deriv != 0), p->natoms, 1, p->stream);
+    tbk::launch_phases(p->sdev, p->item_s + k, p->kp + 3 * (size_t)k, p->maxpairs, p->ph, 1, p->stream);
+    tbk::launch_build_hs(build_args(p, k, deriv != 0), p->maxnat, 1, p->stream);
     HIP_TRY(hipStreamSynchronize(p->stream));
-    const size_t ss = p->ws.stride();
-    if (h) HIP_TRY(hipMemcpy(h, p->ws.A, ss * sizeof(double2), hipMemcpyDeviceToHost));
-    if (s) HIP_TRY(hipMemcpy(s, p->ws.S, ss * sizeof(double2), hipMemcpyDeviceToHost));
+    const int nv = p->sh[p->item_s_host[k]].nv;
+    if (h && copy_leading((double2 *)h, p->ws.A, p->N, nv)) return TBK_ERR_HIP;
+    if (s && copy_leading((double2 *)s, p->ws.S, p->N, nv)) return TBK_ERR_HIP;
     return TBK_OK;
 }
 
-extern "C" int tbk_plan_nv(const tbk_plan *p) { return p ? p->nv : -1; }
-extern "C" int tbk_plan_npairs(const tbk_plan *p) { return p ? p->npairs : -1; }
+extern "C" int tbk_plan_nv(const tbk_plan *p) { return p ? p->sh[0].nv : -1; }
+extern "C" int tbk_plan_npairs(const tbk_plan *p) { return p ? p->sh[0].npairs : -1; }
+extern "C" int tbk_plan_order(const tbk_plan *p) { return p ? p->N : -1; }
+extern "C" int tbk_plan_nstruct(const tbk_plan *p) { return p ? p->nstruct : -1; }
+extern "C" long long tbk_plan_device_bytes(const tbk_plan *p) { return p ? (long long)p->bytes : -1; }
 
 extern "C" int tbk_plan_destroy(tbk_plan *p)
 {
@@ -654,7 +850,7 @@ extern "C" int tbk_zhegv(int device, int n, const double *h, const double *s, do
         hipMemcpyAsync(ws.S, s, ss * sizeof(double2), hipMemcpyHostToDevice, st) != hipSuccess) {
         rc = TBK_ERR_HIP;
     } else {
-        solve_batch(ws, 1, ws.w, st, tm);
+        solve_batch(ws, 1, nullptr, ws.w, st, tm);
         if (hipMemcpyAsync(w, ws.w, n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipMemcpyAsync(&info, ws.info, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
             (z && hipMemcpyAsync(z, ws.Z, ss * sizeof(double2), hipMemcpyDeviceToHost, st) != hipSuccess) ||
@@ -755,7 +951,7 @@ extern "C" int tbk_debug_stages(int device, int n, const double *h, const double
     get(refl, ws.A, ss * sizeof(double2));
     get(d, ws.d, n * sizeof(double));
     get(e, ws.e, n * sizeof(double));
-    tbk::SteinArgs sa{n, 1e-4, ws.d, ws.e, ws.w, ws.Vt, ws.scratch, ws.sinfo};
+    tbk::SteinArgs sa{n, 1e-4, nullptr, ws.d, ws.e, ws.w, ws.Vt, ws.scratch, ws.sinfo};
     tbk::launch_stein(sa, 1, st);
     (void)hipDeviceSynchronize();
     get(w, ws.w, n * sizeof(double));
@@ -770,7 +966,7 @@ extern "C" int tbk_debug_stages(int device, int n, const double *h, const double
         return TBK_ERR_HIP;
     }
     StageTimer tm;
-    solve_batch(ws, 1, ws.w, st, tm);
+    solve_batch(ws, 1, nullptr, ws.w, st, tm);
     (void)hipDeviceSynchronize();
     get(z, ws.Z, ss * sizeof(double2));
     ws.release();

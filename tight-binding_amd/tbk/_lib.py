@@ -24,6 +24,11 @@ EXPORTS = [
     "tbk_plan_launch", "tbk_plan_wait", "tbk_plan_eigenvalues", "tbk_plan_eigenvectors",
     "tbk_plan_hs", "tbk_plan_nv", "tbk_plan_npairs", "tbk_plan_destroy",
     "tbk_plan_stage_times", "tbk_plan_set_timing", "tbk_plan_autotune", "tbk_gemm_tile",
+    "tbk_plan_create_group", "tbk_plan_result", "tbk_plan_structure_eigenvalues",
+    "tbk_plan_set_graph", "tbk_plan_order", "tbk_plan_nstruct", "tbk_plan_device_bytes",
+    "tbk_multi_create", "tbk_multi_launch", "tbk_multi_wait", "tbk_multi_run", "tbk_multi_result",
+    "tbk_multi_eigenvalues", "tbk_multi_ngroups", "tbk_multi_group_order", "tbk_multi_set_graph",
+    "tbk_multi_autotune", "tbk_multi_device_bytes", "tbk_multi_destroy",
     "tbk_zhegv", "tbk_fermi",
     "tbk_error_string", "tbk_debug_stages", "tbk_debug_zgemm",
 ]
@@ -68,6 +73,29 @@ def lib():
                                  d, d, d, d]
         L.tbk_setparv.argtypes = [C.c_int, i, C.c_int, d, C.c_int, C.c_int, C.c_double, C.c_int,
                                   d, d, d, i, i, d, d, d, d]
+        L.tbk_plan_create_group.argtypes = [C.c_int, C.c_int, C.POINTER(StructureDesc), i, d, d,
+                                             C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.tbk_plan_result.argtypes = [vp, C.c_int, dp, dp, dp]
+        L.tbk_plan_structure_eigenvalues.argtypes = [vp, C.c_int, vp, vp]
+        L.tbk_plan_set_graph.argtypes = [vp, C.c_int]
+        L.tbk_plan_order.argtypes = [vp]
+        L.tbk_plan_nstruct.argtypes = [vp]
+        L.tbk_plan_device_bytes.argtypes = [vp]
+        L.tbk_plan_device_bytes.restype = C.c_longlong
+        L.tbk_multi_create.argtypes = [C.c_int, C.c_int, C.POINTER(StructureDesc), i, d, d, C.c_int,
+                                        C.c_int, C.c_longlong, C.POINTER(C.c_void_p)]
+        L.tbk_multi_launch.argtypes = [vp, C.c_int, C.c_double, C.c_double]
+        L.tbk_multi_wait.argtypes = [vp]
+        L.tbk_multi_run.argtypes = [vp, C.c_int, C.c_double, C.c_double]
+        L.tbk_multi_result.argtypes = [vp, C.c_int, dp, dp, dp]
+        L.tbk_multi_eigenvalues.argtypes = [vp, C.c_int, vp, vp]
+        L.tbk_multi_ngroups.argtypes = [vp]
+        L.tbk_multi_group_order.argtypes = [vp, C.c_int]
+        L.tbk_multi_set_graph.argtypes = [vp, C.c_int]
+        L.tbk_multi_autotune.argtypes = [vp, C.c_int]
+        L.tbk_multi_device_bytes.argtypes = [vp]
+        L.tbk_multi_device_bytes.restype = C.c_longlong
+        L.tbk_multi_destroy.argtypes = [vp]
         L.tbk_plan_autotune.argtypes = [vp, C.c_int, ip]
         L.tbk_gemm_tile.argtypes = [C.c_int] * 8
         L.tbk_plan_create.argtypes = [C.c_int, C.POINTER(StructureDesc), C.c_int, d, d, C.c_int,

@@ -68,34 +68,43 @@ class HostSetup:
                             self.dparos, self.dhop, self.dover), "tbk_setparv")
 
 
+def _structure_desc(st: Structure, keep: list) -> "_lib.StructureDesc":
+    """Fill the C descriptor of a structure; the arrays it points to are
+    appended to `keep` (they must outlive the create call)."""
+    p = st.params
+    k = dict(
+        lat=np.ascontiguousarray(st.lat.ravel(), float),
+        posn=np.ascontiguousarray(st.posn.ravel(), float),
+        posq=np.ascontiguousarray(st.posq.ravel(), float),
+        kkind=np.ascontiguousarray(st.kkind, np.int32),
+        kbas=np.array(p.kbas, np.int32), valence=np.array(p.valence, float),
+        dnn=np.array(p.dnn, float), screenl=np.array(p.screenl, float),
+        param=np.ascontiguousarray(p.param, float))
+    keep.append(k)
+    desc = _lib.StructureDesc()
+    desc.natoms = st.natoms
+    desc.lat, desc.posn, desc.posq = _dp(k["lat"]), _dp(k["posn"]), _dp(k["posq"])
+    desc.kkind = _ip(k["kkind"])
+    desc.nd = (C.c_int * 3)(*[int(x) for x in st.nd])
+    desc.kinds = p.kinds
+    desc.kbas, desc.valence = _ip(k["kbas"]), _dp(k["valence"])
+    desc.dnn, desc.screenl = _dp(k["dnn"]), _dp(k["screenl"])
+    desc.param, desc.nparam = _dp(k["param"]), len(k["param"])
+    desc.nltype, desc.jsover = p.nltype, int(p.jsover)
+    desc.thisvol, desc.zfill = st.volume, st.zfill
+    return desc
+
+
 class Plan:
     """A structure, its k-mesh and device-resident H/S/Z state."""
 
     def __init__(self, st: Structure, kpts, weights, batch: int = 0,
-                 precision: int = TBK_FP64, device: int = 0, timing: bool = False):
+                 precision: int = TBK_FP64, device: int = 0, timing: bool = False,
+                 graph: bool = False):
         L = lib()
-        p = st.params
         self.st = st
-        self._keep = dict(
-            lat=np.ascontiguousarray(st.lat.ravel(), float),
-            posn=np.ascontiguousarray(st.posn.ravel(), float),
-            posq=np.ascontiguousarray(st.posq.ravel(), float),
-            kkind=np.ascontiguousarray(st.kkind, np.int32),
-            kbas=np.array(p.kbas, np.int32), valence=np.array(p.valence, float),
-            dnn=np.array(p.dnn, float), screenl=np.array(p.screenl, float),
-            param=np.ascontiguousarray(p.param, float))
-        k = self._keep
-        desc = _lib.StructureDesc()
-        desc.natoms = st.natoms
-        desc.lat, desc.posn, desc.posq = _dp(k["lat"]), _dp(k["posn"]), _dp(k["posq"])
-        desc.kkind = _ip(k["kkind"])
-        desc.nd = (C.c_int * 3)(*[int(x) for x in st.nd])
-        desc.kinds = p.kinds
-        desc.kbas, desc.valence = _ip(k["kbas"]), _dp(k["valence"])
-        desc.dnn, desc.screenl = _dp(k["dnn"]), _dp(k["screenl"])
-        desc.param, desc.nparam = _dp(k["param"]), len(k["param"])
-        desc.nltype, desc.jsover = p.nltype, int(p.jsover)
-        desc.thisvol, desc.zfill = st.volume, st.zfill
+        self._keep = []
+        desc = _structure_desc(st, self._keep)
         self.kpts = np.ascontiguousarray(np.asarray(kpts, float).reshape(-1, 3))
         self.weights = np.ascontiguousarray(np.asarray(weights, float))
         self.nk = len(self.weights)
@@ -107,6 +116,14 @@ class Plan:
         self.npairs = L.tbk_plan_npairs(h)
         if timing:
             L.tbk_plan_set_timing(h, 1)
+        if graph:
+            L.tbk_plan_set_graph(h, 1)
+
+    def set_graph(self, on: bool = True):
+        check(lib().tbk_plan_set_graph(self.h, int(on)), "tbk_plan_set_graph")
+
+    def device_bytes(self) -> int:
+        return int(lib().tbk_plan_device_bytes(self.h))
 
     def run(self, mode: int = 4, tkb: float = 0.005, eigcut: float = 10.0) -> Dict[str, float]:
         ef, e1, e2 = C.c_double(), C.c_double(), C.c_double()
@@ -157,6 +174,132 @@ class Plan:
     def close(self):
         if getattr(self, "h", None):
             lib().tbk_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GroupPlan(Plan):
+    """Several structures of uneven orbital count solved in the same batched
+    launches (padded to the largest order).  run() returns one result dict
+    per structure; a structure whose solve failed gets {"error": code}."""
+
+    def __init__(self, structures, kpts_list, weights_list, batch: int = 0,
+                 precision: int = TBK_FP64, device: int = 0, graph: bool = False):
+        L = lib()
+        self.structures = list(structures)
+        self._keep = []
+        ns = len(self.structures)
+        descs = (_lib.StructureDesc * ns)()
+        for i, st in enumerate(self.structures):
+            descs[i] = _structure_desc(st, self._keep)
+        ks = [np.ascontiguousarray(np.asarray(k, float).reshape(-1, 3)) for k in kpts_list]
+        ws = [np.ascontiguousarray(np.asarray(w, float)) for w in weights_list]
+        self.nks = np.array([len(w) for w in ws], np.int32)
+        self.kpts = np.ascontiguousarray(np.concatenate(ks))
+        self.weights = np.ascontiguousarray(np.concatenate(ws))
+        self.nk = len(self.weights)
+        h = C.c_void_p()
+        check(L.tbk_plan_create_group(device, ns, descs, self.nks, self.kpts.ravel(), self.weights,
+                                      int(batch), int(precision), C.byref(h)), "tbk_plan_create_group")
+        self.h = h
+        self.nstruct = ns
+        self.order = L.tbk_plan_order(h)
+        self.nvs = [st.nv for st in self.structures]
+        if graph:
+            L.tbk_plan_set_graph(h, 1)
+
+    def run(self, mode: int = 4, tkb: float = 0.005, eigcut: float = 10.0):
+        self.launch(mode, tkb, eigcut)
+        return self.wait()
+
+    def wait(self):
+        L = lib()
+        ef, e1, e2 = C.c_double(), C.c_double(), C.c_double()
+        L.tbk_plan_wait(self.h, C.byref(ef), C.byref(e1), C.byref(e2))   # per-structure below
+        out = []
+        for s in range(self.nstruct):
+            rc = L.tbk_plan_result(self.h, s, C.byref(ef), C.byref(e1), C.byref(e2))
+            out.append({"error": rc} if rc else dict(efermi=ef.value, earg1=e1.value, earg2=e2.value))
+        return out
+
+    def eigenvalues(self, s: int = 0):
+        nk, nv = int(self.nks[s]), self.nvs[s]
+        e = np.zeros((nk, nv)); d = np.zeros((nk, nv))
+        check(lib().tbk_plan_structure_eigenvalues(self.h, int(s), e.ctypes.data, d.ctypes.data))
+        return e, d
+
+
+class MultiPlan:
+    """Many structures of uneven orbital count, grouped by padded order
+    (tbk_multi_*): each group is one batched plan, groups run concurrently."""
+
+    def __init__(self, structures, kpts_list, weights_list, precision: int = TBK_FP64,
+                 granularity: int = 32, max_batch_bytes: int = 16 << 30, device: int = 0,
+                 graph: bool = False):
+        L = lib()
+        self.structures = list(structures)
+        self._keep = []
+        ns = len(self.structures)
+        descs = (_lib.StructureDesc * ns)()
+        for i, st in enumerate(self.structures):
+            descs[i] = _structure_desc(st, self._keep)
+        ks = [np.ascontiguousarray(np.asarray(k, float).reshape(-1, 3)) for k in kpts_list]
+        ws = [np.ascontiguousarray(np.asarray(w, float)) for w in weights_list]
+        self.nks = np.array([len(w) for w in ws], np.int32)
+        self.kpts = np.ascontiguousarray(np.concatenate(ks))
+        self.weights = np.ascontiguousarray(np.concatenate(ws))
+        h = C.c_void_p()
+        check(L.tbk_multi_create(device, ns, descs, self.nks, self.kpts.ravel(), self.weights,
+                                 int(precision), int(granularity), int(max_batch_bytes), C.byref(h)),
+              "tbk_multi_create")
+        self.h = h
+        self.nstruct = ns
+        self.ngroups = L.tbk_multi_ngroups(h)
+        if graph:
+            L.tbk_multi_set_graph(h, 1)
+
+    def autotune(self, reps: int = 2):
+        check(lib().tbk_multi_autotune(self.h, int(reps)), "tbk_multi_autotune")
+
+    def set_graph(self, on: bool = True):
+        check(lib().tbk_multi_set_graph(self.h, int(on)), "tbk_multi_set_graph")
+
+    def launch(self, mode: int = 4, tkb: float = 0.005, eigcut: float = 10.0):
+        check(lib().tbk_multi_launch(self.h, mode, tkb, eigcut), "tbk_multi_launch")
+
+    def wait(self):
+        L = lib()
+        rc = L.tbk_multi_wait(self.h)
+        if rc in (_lib.TBK_ERR_HIP, _lib.TBK_ERR_ARG):
+            check(rc, "tbk_multi_wait")
+        out = []
+        ef, e1, e2 = C.c_double(), C.c_double(), C.c_double()
+        for s in range(self.nstruct):
+            r = L.tbk_multi_result(self.h, s, C.byref(ef), C.byref(e1), C.byref(e2))
+            out.append({"error": r} if r else dict(efermi=ef.value, earg1=e1.value, earg2=e2.value))
+        return out
+
+    def run(self, mode: int = 4, tkb: float = 0.005, eigcut: float = 10.0):
+        self.launch(mode, tkb, eigcut)
+        return self.wait()
+
+    def eigenvalues(self, s: int):
+        nk, nv = int(self.nks[s]), self.structures[s].nv
+        e = np.zeros((nk, nv)); d = np.zeros((nk, nv))
+        check(lib().tbk_multi_eigenvalues(self.h, int(s), e.ctypes.data, d.ctypes.data))
+        return e, d
+
+    def device_bytes(self) -> int:
+        return int(lib().tbk_multi_device_bytes(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tbk_multi_destroy(self.h)
             self.h = None
 
     def __del__(self):

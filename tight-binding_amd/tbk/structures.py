@@ -279,3 +279,57 @@ def config_c3(seed: int = 20261203) -> Structure:
     """108-atom fcc 3x3x3 supercell, sigma = 0.03 A, <=1% strain, N = 972."""
     return synthetic_structure("fcc", (3, 3, 3), 3.61 * ANGSTROM, synthetic_params(seed),
                                sigma_bohr=0.03 * ANGSTROM, strain=0.01, seed=seed + 2)
+
+
+def primitive_cell(kind: str, a: float, c_over_a: float = math.sqrt(8.0 / 3.0)):
+    """Lattice vectors and fractional basis of the primitive cell (1 atom for
+    fcc and bcc, 2 for hcp)."""
+    if kind == "fcc":
+        return fcc_primitive(a), np.zeros((1, 3))
+    if kind == "bcc":
+        return bcc_primitive(a), np.zeros((1, 3))
+    return conventional_cell("hcp", a, c_over_a)
+
+
+def uneven_structure(kind: str, reps, a_bohr: float, params: NRLParams, sigma_bohr: float,
+                     seed: int) -> Structure:
+    """Displaced supercell of a primitive cell."""
+    rng = np.random.default_rng(seed)
+    lat, frac = primitive_cell(kind, a_bohr)
+    lat, pos = supercell(lat, frac, reps)
+    if sigma_bohr:
+        pos = pos + sigma_bohr * rng.standard_normal(pos.shape)
+    posq = pos @ np.linalg.inv(lat)
+    nd = search_widths(lat, params.dnn[0])
+    return Structure(lat, pos, np.ones(len(pos), np.int32), params, nd, posq, 0.0,
+                     f"{kind}{tuple(reps)}")
+
+
+# nearest-neighbour distance of fcc at a = 3.61 A, kept for bcc and hcp
+_NN = 3.61 / math.sqrt(2.0)
+_A_OF = {"fcc": _NN * math.sqrt(2.0), "bcc": _NN * 2.0 / math.sqrt(3.0), "hcp": _NN}
+
+
+def config_c4(seed: int = 20261203, count: int = 256, max_atoms: int = 64, max_q: int = 8,
+              nk_budget: float = 9 * 8 ** 3):
+    """Uneven batch: `count` seeded structures, each fcc, bcc or hcp
+    (uniform), 1..max_atoms atoms (log-uniform; supercells of the primitive
+    cell with n_i <= 4), lattice constant +-3% around the same
+    nearest-neighbour distance, sigma = 0.04 A displacements, Monkhorst-Pack
+    q^3 grids (2 <= q <= max_q) with N q^3 close to nk_budget.  Returns
+    [(structure, q)]."""
+    rng = np.random.default_rng(seed)
+    params = synthetic_params(seed)
+    triples = [(a, b, c) for a in range(1, 5) for b in range(1, 5) for c in range(1, 5)]
+    out = []
+    for i in range(count):
+        kind = ("fcc", "bcc", "hcp")[int(rng.integers(3))]
+        target = math.exp(rng.uniform(0.0, math.log(max_atoms)))
+        per = 2 if kind == "hcp" else 1
+        best = min(triples, key=lambda t: (abs(math.log(per * t[0] * t[1] * t[2] / target)),
+                                           max(t) - min(t)))
+        a = _A_OF[kind] * (1.0 + rng.uniform(-0.03, 0.03)) * ANGSTROM
+        st = uneven_structure(kind, best, a, params, 0.04 * ANGSTROM, seed + 100 + i)
+        q = int(min(max_q, max(2, round((nk_budget / st.nv) ** (1.0 / 3.0)))))
+        out.append((st, q))
+    return out

@@ -91,6 +91,7 @@ int tbk_setparv(int natoms, const int *kkind, int kinds, const double *param,
  * --------------------------------------------------------------------- */
 typedef struct tbk_plan tbk_plan;
 typedef struct tbk_multi tbk_multi;
+typedef struct tbk_server tbk_server;
 
 /* Structure description passed to tbk_plan_create. */
 typedef struct tbk_structure_desc {
@@ -144,6 +145,10 @@ int tbk_plan_run(tbk_plan *plan, int mode, double tkb, double eigcut,
 int tbk_plan_launch(tbk_plan *plan, int mode, double tkb, double eigcut);
 int tbk_plan_wait(tbk_plan *plan, double *efermi, double *earg1,
                   double *earg2);
+
+/* TBK_OK when a launched run has finished (the plan's stream is idle), 1
+ * while it still runs. */
+int tbk_plan_query(const tbk_plan *plan);
 
 /* Results of structure s after tbk_plan_wait: E_F, earg1, earg2 as in
  * tbk_plan_run; returns the structure's own status (TBK_ERR_OVERLAP,
@@ -229,6 +234,34 @@ int tbk_multi_set_graph(tbk_multi *m, int on);
 int tbk_multi_autotune(tbk_multi *m, int reps);
 long long tbk_multi_device_bytes(const tbk_multi *m);
 int tbk_multi_destroy(tbk_multi *m);
+
+/* ---------------------------------------------------------------------
+ * Request loop with continuous batching (no reference counterpart).  A
+ * worker thread admits queued requests while their device footprint fits
+ * in budget_bytes and fewer than max_inflight groups run; requests admitted
+ * together are grouped by padded order (granularity) and (mode, tkb,
+ * eigcut) into group plans on their own streams, overlapping with the
+ * groups already on the device.  Results are each structure's own (the
+ * padding is decoupled).
+ * --------------------------------------------------------------------- */
+int tbk_server_create(int device, int precision, long long budget_bytes,
+                      int max_inflight, int granularity, tbk_server **out);
+/* Queue one structure (copied) with its k-mesh; *id names the request. */
+int tbk_server_submit(tbk_server *s, const tbk_structure_desc *st, int nk,
+                      const double *kpts, const double *weights, int mode,
+                      double tkb, double eigcut, long long *id);
+/* Wait up to timeout_ms (< 0: no limit) for request id: returns TBK_OK with
+ * its status and results, or 1 if it is still queued or running.  Latency
+ * is submit -> result, queue time submit -> admission, in ms. */
+int tbk_server_result(tbk_server *s, long long id, int timeout_ms,
+                      int *status, double *efermi, double *earg1,
+                      double *earg2, double *latency_ms, double *queue_ms);
+int tbk_server_release(tbk_server *s, long long id);
+/* completed, k-points done, groups launched, largest group, pending,
+ * running groups, in-flight bytes, last error. */
+int tbk_server_stats(tbk_server *s, long long *stats8);
+/* Finishes every queued request, then stops the worker. */
+int tbk_server_destroy(tbk_server *s);
 
 /* ---------------------------------------------------------------------
  * Single-call drop-ins on host arrays.

@@ -1,0 +1,60 @@
+"""The request loop: requests that arrive while others run are admitted and
+batched with them, and every result is the same alone and in the mix."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from tbk import _lib
+from tbk import structures as S
+from tbk.api import Plan, Server
+
+pytestmark = pytest.mark.gpu
+
+
+def test_server_same_alone_and_in_mix():
+    c4 = S.config_c4(count=64, max_atoms=16, max_q=4)
+    reqs = [(st, S.monkhorst_pack((q, q, q))) for st, q in c4[:16]]
+    srv = Server(precision=_lib.TBK_MIXED, budget_bytes=8 << 30, max_inflight=4)
+    ids = []
+    for i, (st, k) in enumerate(reqs):
+        ids.append(srv.submit(st, k, np.full(len(k), 1.0 / len(k))))
+        if i == 7:
+            time.sleep(0.05)          # the second half arrives while the first runs
+    res = [srv.result(r) for r in ids]
+    stats = srv.stats()
+    srv.close()
+    print(f"stats {stats}; latencies ms {[round(r['latency_ms'], 1) for r in res]}", flush=True)
+    assert stats["completed"] == len(reqs)
+    assert stats["max_group"] >= 2               # requests were batched together
+    for (st, k), r in zip(reqs, res):
+        assert r["status"] == 0
+        w = np.full(len(k), 1.0 / len(k))
+        p = Plan(st, k, w, precision=_lib.TBK_MIXED)
+        a = p.run(4)
+        p.close()
+        d = max(abs(r[x] - a[x]) for x in ("efermi", "earg1", "earg2"))
+        assert d < 1e-10, (st.nv, d)
+        ref = oracle.Setup(st).bande(k, w, mode=4, nthreads=16)
+        assert abs(r["efermi"] - ref["efermi"]) < 1e-6
+        assert abs(r["earg1"] - ref["earg1"]) / st.natoms < 1e-6
+
+
+def test_server_failing_request_does_not_affect_others():
+    p_bad = S.synthetic_params(1, overlap_scale=3.0)
+    bad = S.synthetic_structure("fcc", (1, 1, 1), 3.3 * S.ANGSTROM, p_bad)
+    good = S.config_c1()
+    k = S.monkhorst_pack((2, 2, 2))
+    w = np.full(len(k), 1.0 / len(k))
+    srv = Server(precision=_lib.TBK_FP64, budget_bytes=4 << 30)
+    ib = srv.submit(bad, k, w)
+    ig = srv.submit(good, k, w)
+    rb, rg = srv.result(ib), srv.result(ig)
+    srv.close()
+    assert rb["status"] == _lib.TBK_ERR_OVERLAP
+    assert rg["status"] == 0
+    p = Plan(good, k, w)
+    a = p.run(4)
+    p.close()
+    assert abs(rg["efermi"] - a["efermi"]) < 1e-12

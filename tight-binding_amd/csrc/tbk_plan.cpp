@@ -25,20 +25,35 @@ constexpr int kMaxN = 1280;    // LDS-resident vectors of k_hetrd / k_stein
     } while (0)
 
 thread_local size_t g_dev_bytes = 0;     // device bytes allocated by this thread
+// Plans allocate stream-ordered (hipMallocAsync / hipFreeAsync on their own
+// stream), so creating and destroying a plan never synchronises the device
+// while other plans run; the one-shot entry points use plain hipMalloc.
+thread_local hipStream_t g_alloc_stream = nullptr;
+
+struct AllocStream {
+    hipStream_t prev;
+    explicit AllocStream(hipStream_t s) : prev(g_alloc_stream) { g_alloc_stream = s; }
+    ~AllocStream() { g_alloc_stream = prev; }
+};
 
 template <class T>
 int dalloc(T **p, size_t n)
 {
     *p = nullptr;
     if (n == 0) return TBK_OK;
-    if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) { *p = nullptr; return TBK_ERR_NOMEM; }
+    const hipError_t e = g_alloc_stream ? hipMallocAsync((void **)p, n * sizeof(T), g_alloc_stream)
+                                        : hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) { *p = nullptr; return TBK_ERR_NOMEM; }
     g_dev_bytes += n * sizeof(T);
     return TBK_OK;
 }
 template <class T>
 void dfree(T *&p)
 {
-    if (p) (void)hipFree((void *)p);
+    if (p) {
+        if (g_alloc_stream) (void)hipFreeAsync((void *)p, g_alloc_stream);
+        else (void)hipFree((void *)p);
+    }
     p = nullptr;
 }
 
@@ -347,6 +362,7 @@ static void plan_free(tbk_plan *p)
 {
     if (!p) return;
     (void)hipSetDevice(p->device);
+    AllocStream as(p->stream);
     if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     dfree(p->row_atom); dfree(p->row_orb); dfree(p->ksk); dfree(p->nbas); dfree(p->blk_ptr);
     dfree(p->dvec); dfree(p->eh); dfree(p->es); dfree(p->edh); dfree(p->eds);
@@ -355,16 +371,22 @@ static void plan_free(tbk_plan *p)
     dfree(p->esk); dfree(p->desk); dfree(p->out); dfree(p->ncut); dfree(p->err); dfree(p->info_all);
     dfree(p->ph);
     p->ws.release();
-    if (p->stream) (void)hipStreamDestroy(p->stream);
+    if (p->stream) {
+        (void)hipStreamSynchronize(p->stream);
+        (void)hipStreamDestroy(p->stream);
+    }
     delete p;
 }
 
+// Allocate and copy on the allocation stream (the caller synchronises it
+// before the host vectors go away).
 template <class T>
 static int upload(T **dst, const std::vector<T> &src)
 {
     int rc = dalloc(dst, std::max<size_t>(src.size(), 1));
     if (rc) return rc;
-    if (!src.empty() && hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+    if (!src.empty() && hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice,
+                                       g_alloc_stream) != hipSuccess)
         return TBK_ERR_HIP;
     return TBK_OK;
 }
@@ -477,7 +499,9 @@ extern "C" int tbk_plan_create_group(int device, int nstruct, const tbk_structur
     p->kpts.assign(kpts, kpts + 3 * (size_t)nktot);
     p->weights.assign(weights, weights + nktot);
     p->batch = (batch <= 0 || batch > nktot) ? nktot : batch;
-    if (hipStreamCreate(&p->stream) != hipSuccess) { delete p; return TBK_ERR_HIP; }
+    // non-blocking: plans neither wait for nor hold up work on the null stream
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) { delete p; return TBK_ERR_HIP; }
+    AllocStream as(p->stream);
     int rc = TBK_OK;
     rc |= upload(&p->row_atom, A.row_atom); rc |= upload(&p->row_orb, A.row_orb);
     rc |= upload(&p->ksk, A.ksk); rc |= upload(&p->nbas, A.nbas); rc |= upload(&p->blk_ptr, A.blk_ptr);
@@ -509,10 +533,12 @@ extern "C" int tbk_plan_create_group(int device, int nstruct, const tbk_structur
         rc |= upload(&p->sdev, sd);
         std::vector<tbk::FermiArgs> fa(nstruct);
         rc |= upload(&p->fargs, fa);        // filled per run (mode, kT)
+        if (hipStreamSynchronize(p->stream) != hipSuccess) rc |= TBK_ERR_HIP;
     }
     if (!rc) rc = dalloc(&p->ph, (size_t)p->batch * maxpairs);
     if (!rc) rc = p->ws.alloc(N, p->batch);
     if (!rc && precision == TBK_MIXED) rc = p->ws.alloc_mixed();
+    if (!rc && hipStreamSynchronize(p->stream) != hipSuccess) rc = TBK_ERR_HIP;   // uploads done
     if (rc) { plan_free(p); return rc == TBK_ERR_HIP ? TBK_ERR_HIP : TBK_ERR_NOMEM; }
     p->bytes = g_dev_bytes;
     p->timer.st = p->stream;
@@ -641,12 +667,15 @@ extern "C" int tbk_plan_wait(tbk_plan *p, double *efermi, double *earg1, double 
 {
     if (!p) return TBK_ERR_ARG;
     HIP_TRY(hipSetDevice(p->device));
-    HIP_TRY(hipStreamSynchronize(p->stream));
-    HIP_TRY(hipMemcpy(p->out_host.data(), p->out, p->out_host.size() * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(p->info_host.data(), p->info_all, p->nk * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpyAsync(p->out_host.data(), p->out, p->out_host.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipMemcpyAsync(p->info_host.data(), p->info_all, p->nk * sizeof(int), hipMemcpyDeviceToHost,
+                           p->stream));
     const bool lpress = p->last_mode == 2 || p->last_mode == 4;
     if (lpress)
-        HIP_TRY(hipMemcpy(p->err_host.data(), p->err, p->nk * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpyAsync(p->err_host.data(), p->err, p->nk * sizeof(int), hipMemcpyDeviceToHost,
+                               p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
     p->timer.collect();
     int first = TBK_OK;
     for (int i = 0; i < p->nstruct; ++i) {
@@ -672,6 +701,17 @@ extern "C" int tbk_plan_run(tbk_plan *p, int mode, double tkb, double eigcut, do
     int rc = tbk_plan_launch(p, mode, tkb, eigcut);
     if (rc) return rc;
     return tbk_plan_wait(p, efermi, earg1, earg2);
+}
+
+// TBK_OK when the plan's stream is idle (a launched run has finished), 1
+// while it still runs.
+extern "C" int tbk_plan_query(const tbk_plan *p)
+{
+    if (!p) return TBK_ERR_ARG;
+    const hipError_t e = hipStreamQuery(p->stream);
+    if (e == hipSuccess) return TBK_OK;
+    if (e == hipErrorNotReady) return 1;
+    return TBK_ERR_HIP;
 }
 
 extern "C" int tbk_plan_result(const tbk_plan *p, int s, double *efermi, double *earg1, double *earg2)
@@ -756,11 +796,12 @@ extern "C" int tbk_plan_structure_eigenvalues(tbk_plan *p, int s, double *esk, d
     const StructHost &h = p->sh[s];
     const size_t N = p->N;
     // rows of N (padded order) -> rows of nv
+    if (esk) HIP_TRY(hipMemcpy2DAsync(esk, h.nv * sizeof(double), p->esk + (size_t)h.k0 * N, N * sizeof(double),
+                                      h.nv * sizeof(double), h.nk, hipMemcpyDeviceToHost, p->stream));
+    if (desk) HIP_TRY(hipMemcpy2DAsync(desk, h.nv * sizeof(double), p->desk + (size_t)h.k0 * N,
+                                       N * sizeof(double), h.nv * sizeof(double), h.nk, hipMemcpyDeviceToHost,
+                                       p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
-    if (esk) HIP_TRY(hipMemcpy2D(esk, h.nv * sizeof(double), p->esk + (size_t)h.k0 * N, N * sizeof(double),
-                                 h.nv * sizeof(double), h.nk, hipMemcpyDeviceToHost));
-    if (desk) HIP_TRY(hipMemcpy2D(desk, h.nv * sizeof(double), p->desk + (size_t)h.k0 * N, N * sizeof(double),
-                                  h.nv * sizeof(double), h.nk, hipMemcpyDeviceToHost));
     return TBK_OK;
 }
 
@@ -770,10 +811,11 @@ extern "C" int tbk_plan_eigenvalues(tbk_plan *p, double *esk, double *desk)
 }
 
 // Leading nv x nv block (column stride nv) of an N x N item matrix.
-static int copy_leading(double2 *dst, const double2 *src, int N, int nv)
+static int copy_leading(double2 *dst, const double2 *src, int N, int nv, hipStream_t st)
 {
-    HIP_TRY(hipMemcpy2D(dst, nv * sizeof(double2), src, N * sizeof(double2), nv * sizeof(double2), nv,
-                        hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy2DAsync(dst, nv * sizeof(double2), src, N * sizeof(double2), nv * sizeof(double2), nv,
+                             hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
     return TBK_OK;
 }
 
@@ -786,7 +828,8 @@ extern "C" int tbk_plan_eigenvectors(tbk_plan *p, int k, double *z)
     HIP_TRY(hipSetDevice(p->device));
     HIP_TRY(hipStreamSynchronize(p->stream));
     const size_t ss = p->ws.stride();
-    return copy_leading((double2 *)z, p->ws.Z + (size_t)(k - last0) * ss, p->N, p->sh[p->item_s_host[k]].nv);
+    return copy_leading((double2 *)z, p->ws.Z + (size_t)(k - last0) * ss, p->N, p->sh[p->item_s_host[k]].nv,
+                        p->stream);
 }
 
 extern "C" int tbk_plan_hs(tbk_plan *p, int k, int deriv, double *h, double *s)
@@ -797,8 +840,8 @@ extern "C" int tbk_plan_hs(tbk_plan *p, int k, int deriv, double *h, double *s)
     tbk::launch_build_hs(build_args(p, k, deriv != 0), p->maxnat, 1, p->stream);
     HIP_TRY(hipStreamSynchronize(p->stream));
     const int nv = p->sh[p->item_s_host[k]].nv;
-    if (h && copy_leading((double2 *)h, p->ws.A, p->N, nv)) return TBK_ERR_HIP;
-    if (s && copy_leading((double2 *)s, p->ws.S, p->N, nv)) return TBK_ERR_HIP;
+    if (h && copy_leading((double2 *)h, p->ws.A, p->N, nv, p->stream)) return TBK_ERR_HIP;
+    if (s && copy_leading((double2 *)s, p->ws.S, p->N, nv, p->stream)) return TBK_ERR_HIP;
     return TBK_OK;
 }
 

@@ -28,7 +28,9 @@ EXPORTS = [
     "tbk_plan_set_graph", "tbk_plan_order", "tbk_plan_nstruct", "tbk_plan_device_bytes",
     "tbk_multi_create", "tbk_multi_launch", "tbk_multi_wait", "tbk_multi_run", "tbk_multi_result",
     "tbk_multi_eigenvalues", "tbk_multi_ngroups", "tbk_multi_group_order", "tbk_multi_set_graph",
-    "tbk_multi_autotune", "tbk_multi_device_bytes", "tbk_multi_destroy",
+    "tbk_multi_autotune", "tbk_multi_device_bytes", "tbk_multi_destroy", "tbk_plan_query",
+    "tbk_server_create", "tbk_server_submit", "tbk_server_result", "tbk_server_release",
+    "tbk_server_stats", "tbk_server_destroy",
     "tbk_zhegv", "tbk_fermi",
     "tbk_error_string", "tbk_debug_stages", "tbk_debug_zgemm",
 ]
@@ -96,6 +98,15 @@ def lib():
         L.tbk_multi_device_bytes.argtypes = [vp]
         L.tbk_multi_device_bytes.restype = C.c_longlong
         L.tbk_multi_destroy.argtypes = [vp]
+        L.tbk_plan_query.argtypes = [vp]
+        L.tbk_server_create.argtypes = [C.c_int, C.c_int, C.c_longlong, C.c_int, C.c_int,
+                                         C.POINTER(C.c_void_p)]
+        L.tbk_server_submit.argtypes = [vp, C.POINTER(StructureDesc), C.c_int, d, d, C.c_int,
+                                         C.c_double, C.c_double, C.POINTER(C.c_longlong)]
+        L.tbk_server_result.argtypes = [vp, C.c_longlong, C.c_int, ip, dp, dp, dp, dp, dp]
+        L.tbk_server_release.argtypes = [vp, C.c_longlong]
+        L.tbk_server_stats.argtypes = [vp, C.POINTER(C.c_longlong)]
+        L.tbk_server_destroy.argtypes = [vp]
         L.tbk_plan_autotune.argtypes = [vp, C.c_int, ip]
         L.tbk_gemm_tile.argtypes = [C.c_int] * 8
         L.tbk_plan_create.argtypes = [C.c_int, C.POINTER(StructureDesc), C.c_int, d, d, C.c_int,

@@ -309,6 +309,60 @@ class MultiPlan:
             pass
 
 
+class Server:
+    """Request loop with continuous batching (tbk_server_*): submit() queues
+    a structure and returns at once; result() waits for it."""
+
+    def __init__(self, precision: int = TBK_MIXED, budget_bytes: int = 64 << 30,
+                 max_inflight: int = 8, granularity: int = 32, device: int = 0):
+        h = C.c_void_p()
+        check(lib().tbk_server_create(device, int(precision), int(budget_bytes), int(max_inflight),
+                                      int(granularity), C.byref(h)), "tbk_server_create")
+        self.h = h
+
+    def submit(self, st: Structure, kpts, weights, mode: int = 4, tkb: float = 0.005,
+               eigcut: float = 10.0) -> int:
+        keep = []
+        desc = _structure_desc(st, keep)
+        k = np.ascontiguousarray(np.asarray(kpts, float).reshape(-1, 3))
+        w = np.ascontiguousarray(np.asarray(weights, float))
+        rid = C.c_longlong()
+        check(lib().tbk_server_submit(self.h, C.byref(desc), len(w), k.ravel(), w, int(mode), tkb,
+                                      eigcut, C.byref(rid)), "tbk_server_submit")
+        return rid.value
+
+    def result(self, rid: int, timeout_ms: int = -1):
+        st = C.c_int(); ef, e1, e2, lat, q = (C.c_double() for _ in range(5))
+        rc = lib().tbk_server_result(self.h, int(rid), int(timeout_ms), C.byref(st), C.byref(ef),
+                                     C.byref(e1), C.byref(e2), C.byref(lat), C.byref(q))
+        if rc == 1:
+            return None
+        check(rc, "tbk_server_result")
+        return dict(status=st.value, efermi=ef.value, earg1=e1.value, earg2=e2.value,
+                    latency_ms=lat.value, queue_ms=q.value)
+
+    def release(self, rid: int):
+        check(lib().tbk_server_release(self.h, int(rid)), "tbk_server_release")
+
+    def stats(self):
+        a = (C.c_longlong * 8)()
+        check(lib().tbk_server_stats(self.h, a), "tbk_server_stats")
+        keys = ("completed", "kpoints", "groups", "max_group", "pending", "running", "inflight_bytes",
+                "last_error")
+        return dict(zip(keys, [int(x) for x in a]))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tbk_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def bande(st: Structure, kpts, weights, mode: int = 4, tkb: float = 0.005,
           eigcut: float = 10.0, batch: int = 0, precision: int = TBK_FP64,
           device: int = 0) -> Dict[str, float]:

@@ -8,11 +8,19 @@ band-energy / pressure sums (mode 4).  Inputs are synthetic (seeded NRL
 parameters, seeded displacements) and already resident on the GPU when the
 timed region starts.
 
-Prints one JSON line (see the driver contract), with
-  roofline      the dominant kernel's achieved rate vs the MI355X peak,
-                timed with HIP events on the plan's stream;
+Both precisions are measured (FP64, and mixed: FP32 tridiagonalisation
+and back-transform with FP64 Ogita-Aishima refinement); the headline value
+is --precision (default mixed).  Prints one JSON line with
+  roofline      k_hetrd_panel, the dominant kernel: algorithmic bytes per
+                launch over its average duration (HIP events on the plan's
+                stream around each launch) against the HBM peak;
+  serving       Poisson request streams (10, 100, 500 req/s) through the
+                continuous-batching request loop: p50/p99 latency;
   cpu_baseline  the CPU restatement (oracle) timed on this host's cores on a
                 bounded sample of the same workload.
+--profile-pass PREC runs one batched step of PREC and a 1 GiB device copy
+(the calibration kernel for counter-based traffic) and exits; it is the
+program profiled with rocprofv3 --pmc.
 """
 import argparse
 import json
@@ -36,19 +44,21 @@ FP32_PEAK_TFLOPS = 157.3     # MI355X dense FP32 matrix
 HBM_PEAK_GBS = 8000.0
 
 
-def stage_model(n: int):
-    """Algorithmic work per k-point of each stage (canonical LAPACK counts,
-    complex arithmetic; bytes for the HBM-bound build)."""
-    return {
-        "build_hs": ("hbm", 2 * 16 * n * n),                 # write H and S
-        "potrf_trtri": ("mfma", (4.0 / 3 + 4.0 / 3) * n ** 3),
-        "reduce_gemm": ("mfma", 8.0 * n ** 3),                # two triangular GEMMs
-        "hetrd": ("mfma", 16.0 / 3 * n ** 3),
-        "stein": ("mfma", 0.0),                               # O(n^2): not a contraction
-        "backtransform": ("mfma", 8.0 * n ** 3 + 4.0 * n ** 3),
-        "pressure": ("mfma", 16.0 * n ** 3),
-        "fermi_sums": ("hbm", 0.0),
-    }
+def hetrd_panel_bytes(n: int, csize: int) -> float:
+    """Algorithmic HBM bytes per k-point of all Householder panel launches
+    (k_hetrd_panel) at order n with complex elements of csize bytes: per
+    column j the lower triangle of the trailing matrix (the Hermitian
+    product), the panel's V and W rows twice (dots and the fused update),
+    the column itself, the stores of v (A, V, V2) and w, and per panel the
+    V T^H block."""
+    tot = 0.0
+    for j in range(n - 1):
+        m = n - j - 1
+        i = j % 32
+        tot += m * (m + 1) / 2 + 4 * i * m + (m + 1) + 4 * m
+    panels = (n - 2) // 32 + 1
+    tot += panels * 2 * 32 * n
+    return tot * csize
 
 
 def cpu_baseline(st, kpts, weights, seconds=12.0):
@@ -71,48 +81,178 @@ def cpu_baseline(st, kpts, weights, seconds=12.0):
                       f"full per-k path incl. pressure, OpenMP over k, {dt:.1f} s"}
 
 
+def run_batched(st, kpts, w, prec, steps, warmup, autotune=True):
+    """All k-points in one batch; returns timing, stage times, results."""
+    plan = Plan(st, kpts, w, batch=0, precision=prec)
+    tune = {"shapes": 0, "seconds": 0.0}
+    if autotune:                  # GEMM tile autotuner (outside the timed region)
+        t0 = time.perf_counter()
+        tune["shapes"] = plan.autotune(3)
+        tune["seconds"] = time.perf_counter() - t0
+    for _ in range(warmup):
+        plan.run(4, 0.005, 10.0)
+    plan.set_timing(True)         # stage events over the timed steps only
+    t0 = time.perf_counter()
+    res = None
+    for _ in range(steps):
+        res = plan.run(4, 0.005, 10.0)
+    t1 = time.perf_counter()
+    stages = {k: v / steps for k, v in plan.stage_times().items()}
+    plan.close()
+    ms = (t1 - t0) * 1e3 / steps
+    return {"ms_per_step": ms, "ksolves_per_s": len(kpts) / (ms / 1e3), "stages_ms_per_step": stages,
+            "result": res, "autotune": tune}
+
+
+def serve_stream(rates, nreq, precision, seed=20261203):
+    """Poisson streams of E_tot + pressure requests: structures as in the
+    uneven batch up to 128 atoms (N <= 1152), Monkhorst-Pack grids up to
+    6^3.  Returns p50/p99 latency and throughput per rate."""
+    from tbk.api import Server
+    pool = S.config_c4(seed=seed, count=256, max_atoms=128, max_q=6, nk_budget=9 * 6 ** 3)
+    rng = np.random.default_rng(seed)
+    out = []
+    for rate in rates:
+        srv = Server(precision=precision, budget_bytes=96 << 30, max_inflight=8)
+        picks = rng.integers(len(pool), size=nreq)
+        gaps = rng.exponential(1.0 / rate, size=nreq)
+        reqs = []
+        for i in picks:
+            st, q = pool[int(i)]
+            k = S.monkhorst_pack((q, q, q))
+            reqs.append((st, k, np.full(len(k), 1.0 / len(k))))
+        ids = []
+        t_start = time.perf_counter()
+        t_next = t_start
+        for (st, k, w), g in zip(reqs, gaps):
+            t_next += g
+            while time.perf_counter() < t_next:
+                time.sleep(min(0.001, max(0.0, t_next - time.perf_counter())))
+            ids.append(srv.submit(st, k, w))
+        res = [srv.result(r) for r in ids]
+        t_end = time.perf_counter()
+        stats = srv.stats()
+        srv.close()
+        lat = np.array([r["latency_ms"] for r in res])
+        bad = sum(1 for r in res if r["status"] != 0)
+        nk = sum(len(k) for _, k, _ in reqs)
+        out.append({"rate_req_per_s": rate, "requests": nreq, "failed": bad,
+                    "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                    "mean_queue_ms": float(np.mean([r["queue_ms"] for r in res])),
+                    "req_per_s": nreq / (t_end - t_start), "ksolves_per_s": nk / (t_end - t_start),
+                    "groups": stats["groups"], "largest_group": stats["max_group"]})
+        print(f"# serve {rate:5.0f} req/s: p50 {out[-1]['p50_ms']:.1f} ms p99 {out[-1]['p99_ms']:.1f} ms "
+              f"({stats['groups']} groups, largest {stats['max_group']})", file=sys.stderr, flush=True)
+    return out
+
+
+def profile_pass(precision):
+    """One warm-up and one batched C2 step, then a 1 GiB device-to-device
+    copy (read 1 GiB, write 1 GiB) to calibrate the traffic counters."""
+    st = S.config_c2()
+    kpts = S.monkhorst_pack((8, 8, 8))
+    w = np.full(len(kpts), 1.0 / len(kpts))
+    plan = Plan(st, kpts, w, batch=0, precision=TBK_MIXED if precision == "mixed" else TBK_FP64)
+    plan.run(4, 0.005, 10.0)
+    r = plan.run(4, 0.005, 10.0)
+    plan.close()
+    import torch
+    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile_pass": precision, "result": r, "copy_bytes": 1 << 30}))
+
+
+def bench_c3(args):
+    """108-atom strained supercell, N = 972, MP 4x4x4 = 64 k, both precisions."""
+    st = S.config_c3()
+    kpts = S.monkhorst_pack((4, 4, 4))
+    w = np.full(len(kpts), 1.0 / len(kpts))
+    out = {"workload": "C3: 108-atom fcc 3x3x3, sigma=0.03 A, <=1% strain, N=972, MP 4^3 = 64 k, mode 4",
+           "unit": "k-point solves/s"}
+    for name, prec in (("fp64", TBK_FP64), ("mixed", TBK_MIXED)):
+        r = run_batched(st, kpts, w, prec, max(1, args.steps // 2), args.warmup, not args.no_autotune)
+        out[name] = {"ksolves_per_s": r["ksolves_per_s"], "ms_per_step": r["ms_per_step"],
+                     "stages_ms_per_step": r["stages_ms_per_step"], "result": r["result"]}
+        print(f"# C3 {name}: {r['ms_per_step']:.1f} ms/step", file=sys.stderr, flush=True)
+    print(json.dumps(out))
+
+
+def bench_c4(args):
+    """256 uneven structures (N = 9..576), all solved per step, both precisions."""
+    from tbk.api import MultiPlan
+    c4 = S.config_c4()
+    sts = [st for st, _ in c4]
+    ks = [S.monkhorst_pack((q, q, q)) for _, q in c4]
+    ws = [np.full(len(k), 1.0 / len(k)) for k in ks]
+    nk = sum(len(k) for k in ks)
+    work = sum(len(k) * st.nv ** 3 for st, k in zip(sts, ks))
+    out = {"workload": "C4: 256 seeded fcc/bcc/hcp structures, 1-64 atoms (N=9-576), MP 2^3-8^3, mode 4",
+           "structures": len(sts), "kpoints": nk, "unit": "k-point solves/s"}
+    for name, prec in (("fp64", TBK_FP64), ("mixed", TBK_MIXED)):
+        m = MultiPlan(sts, ks, ws, precision=prec, graph=True)
+        if not args.no_autotune:
+            m.autotune(2)
+        for _ in range(max(1, args.warmup)):
+            m.run(4)
+        t0 = time.perf_counter()
+        steps = max(1, args.steps // 2)
+        for _ in range(steps):
+            res = m.run(4)
+        dt = (time.perf_counter() - t0) / steps
+        bad = sum(1 for r in res if "error" in r)
+        out[name] = {"ksolves_per_s": nk / dt, "ms_per_step": dt * 1e3, "groups": m.ngroups,
+                     "gib": m.device_bytes() / 2 ** 30, "failed": bad,
+                     "n3k_per_s": work / dt}
+        m.close()
+        print(f"# C4 {name}: {dt * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
+                    help="precision of the headline value (both are measured)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--profile-pass", choices=["fp64", "mixed"], default=None)
+    ap.add_argument("--workload", choices=["c2", "c3", "c4"], default="c2",
+                    help="c2: the headline line; c3 (N=972, 64 k) and c4 (256 uneven "
+                         "structures) print their own lines")
     args = ap.parse_args()
+    if args.profile_pass:
+        profile_pass(args.profile_pass)
+        return
+    if args.workload == "c3":
+        bench_c3(args)
+        return
+    if args.workload == "c4":
+        bench_c4(args)
+        return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
 
     st = S.config_c2()
     kpts = S.monkhorst_pack((8, 8, 8))
     w = np.full(len(kpts), 1.0 / len(kpts))
-    prec = TBK_MIXED if args.precision == "mixed" else TBK_FP64
-    print(f"# workload C2: natoms={st.natoms} N={st.nv} nk={len(kpts)}", file=sys.stderr, flush=True)
+    n = st.nv
+    print(f"# workload C2: natoms={st.natoms} N={n} nk={len(kpts)}", file=sys.stderr, flush=True)
 
-    # ---- headline: all k-points batched -------------------------------------
-    plan = Plan(st, kpts, w, batch=0, precision=prec)
-    tune = {"shapes": 0, "seconds": 0.0}
-    if not args.no_autotune:      # GEMM tile autotuner (outside the timed region)
-        t0 = time.perf_counter()
-        tune["shapes"] = plan.autotune(3)
-        tune["seconds"] = time.perf_counter() - t0
-        print(f"# autotuned {tune['shapes']} GEMM shapes in {tune['seconds']:.1f} s", file=sys.stderr,
-              flush=True)
-    for _ in range(args.warmup):
-        plan.run(4, 0.005, 10.0)
-    plan.set_timing(True)       # stage events over the timed steps only
-    t0 = time.perf_counter()
-    results = None
-    for _ in range(args.steps):
-        results = plan.run(4, 0.005, 10.0)
-    t1 = time.perf_counter()
-    stages = plan.stage_times()
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    value = len(kpts) / (ms_step / 1e3)
-    print(f"# batched: {ms_step:.2f} ms/step, {value:.1f} k-solves/s", file=sys.stderr, flush=True)
-    plan.close()
+    # ---- batched, both precisions -------------------------------------------
+    runs = {}
+    for name, prec in (("fp64", TBK_FP64), ("mixed", TBK_MIXED)):
+        runs[name] = run_batched(st, kpts, w, prec, args.steps, args.warmup, not args.no_autotune)
+        print(f"# batched {name}: {runs[name]['ms_per_step']:.2f} ms/step, "
+              f"{runs[name]['ksolves_per_s']:.1f} k-solves/s", file=sys.stderr, flush=True)
+    head = runs[args.precision]
 
     # ---- plain FP64: one k-point per launch sequence (sample) ----------------
     ns = min(args.plain_sample, len(kpts))
@@ -124,50 +264,50 @@ def main():
     pplain.close()
     print(f"# plain fp64: {plain_rate:.1f} k-solves/s", file=sys.stderr, flush=True)
 
-    # ---- roofline of the dominant kernel --------------------------------------
-    # stage times are summed over the timed steps (one launch of each
-    # single-kernel stage per step for the batched plan)
-    n = st.nv
-    model = stage_model(n)
-    per_step = {k: v / args.steps for k, v in stages.items()}
-    dom = max(per_step, key=lambda k: per_step[k])
-    bound, work_per_k = model.get(dom, ("mfma", 0.0))
-    t_launch = per_step[dom] / 1e3
-    if bound == "hbm":
-        achieved = work_per_k * len(kpts) / t_launch / 1e9
-        peak, unit = HBM_PEAK_GBS, "GB/s"
-    else:
-        achieved = work_per_k * len(kpts) / t_launch / 1e12
-        peak = FP32_PEAK_TFLOPS if (prec == TBK_MIXED and dom in ("hetrd", "backtransform")) else FP64_PEAK_TFLOPS
-        unit = "TFLOP/s"
-    roofline = {"kernel": dom, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-                "frac": achieved / peak, "traffic": None,
-                "launch_ms": per_step[dom]}
+    # ---- roofline of the dominant kernel: k_hetrd_panel (HBM-bound
+    # Hermitian products), HIP events around each of its launches on the
+    # plan's stream -----------------------------------------------------------
+    csize = 16 if args.precision == "fp64" else 8
+    panel_ms = head["stages_ms_per_step"].get("hetrd_panel", float("nan"))
+    launches = (n - 2) // 32 + 1
+    bytes_step = hetrd_panel_bytes(n, csize) * len(kpts)
+    achieved = bytes_step / (panel_ms / 1e3) / 1e9
+    roofline = {"kernel": "k_hetrd_panel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "launches_per_step": launches, "avg_launch_ms": panel_ms / launches,
+                "algorithmic_bytes_per_launch": bytes_step / launches}
+
+    serving = None
+    if not args.no_serve:
+        serving = serve_stream([10, 100, 500], args.serve_requests, TBK_MIXED)
 
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["ksolves_per_s"],
         "unit": "k-point solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == TBK_FP64 else "f32+f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic (seeded NRL parameters and displacements)",
         "config": {"workload": "C2: 32-atom fcc 2x2x2 supercell, a=3.61 A, sigma=0.05 A, "
                                "sp3d5 N=288 complex128, MP 8x8x8 = 512 k, mode 4 (E_tot + P), "
                                "kT=0.005 Ry, all k batched",
                    "natoms": st.natoms, "N": n, "nk": len(kpts),
                    "precision": args.precision, "parallelism": "single GPU"},
+        "batched_fp64_ksolves_per_s": runs["fp64"]["ksolves_per_s"],
+        "batched_mixed_ksolves_per_s": runs["mixed"]["ksolves_per_s"],
         "plain_fp64_ksolves_per_s": plain_rate,
-        "speedup_vs_plain": value / plain_rate,
-        "stages_ms_per_step": per_step,
-        "autotune": tune,
-        "result": results,
+        "speedup_vs_plain": head["ksolves_per_s"] / plain_rate,
+        "stages_ms_per_step": {k: r["stages_ms_per_step"] for k, r in runs.items()},
+        "autotune": {k: r["autotune"] for k, r in runs.items()},
+        "result": {k: r["result"] for k, r in runs.items()},
         "roofline": roofline,
+        "serving": serving,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(st, kpts, w)

@@ -854,25 +854,23 @@ __global__ __launch_bounds__(256) void k_stebz(SteinArgs s)
     s.w[(size_t)kb * LD + j] = 0.5 * (a + b);
 }
 
-// Eigenvectors of the tridiagonal, one thread per group of close
-// eigenvalues (consecutive gaps below ortol_rel * onenrm), 64 groups per
-// workgroup.  Isolated and well-separated eigenvalues: twisted
-// factorisation of T - lambda I (L D+ L^T from the top, U D- U^T from the
-// bottom, twist r at min |gamma_r|, z_r = 1 and the two bidiagonal
-// recurrences), orthogonalised within the group.  Near-exact degeneracies
-// (gap <= 1e-10 onenrm): inverse iteration from a distinct start vector
-// with reorthogonalisation (dstein).
+// Eigenvectors of the tridiagonal, one thread per eigenvalue (64 per
+// workgroup).  Eigenvalues separated from their predecessor by more than
+// 1e-10 * onenrm: twisted factorisation of T - lambda I (L D+ L^T from the
+// top, U D- U^T from the bottom, twist r at min |gamma_r|, z_r = 1 and the
+// two bidiagonal recurrences).  Near-exact degeneracies: inverse iteration
+// from a distinct start vector with the shift perturbed away from the
+// previous member (dstein).  Vectors of close eigenvalues are
+// orthogonalised afterwards by k_stein_orth.
 __global__ __launch_bounds__(64) void k_stein_vec(SteinArgs s)
 {
     __shared__ double d[1280], e[1280], e2[1280], w[1280];
-    __shared__ int cstart[1281];
-    __shared__ int ncl;
     __shared__ double red4[16];
     const int LD = s.n;                               // stride; order of the padded batch
     const int kb = blockIdx.y;
     const int N = s.nact ? s.nact[kb] : LD;           // active order
     const int tid = threadIdx.x;
-    const int nthr = gridDim.x * blockDim.x;                  // threads per k
+    const int nthr = gridDim.x * blockDim.x;          // threads per k
     const int gt = blockIdx.x * blockDim.x + tid;
     const double *dg = s.d + (size_t)kb * LD;
     const double *eg = s.e + (size_t)kb * LD;
@@ -888,85 +886,66 @@ __global__ __launch_bounds__(64) void k_stein_vec(SteinArgs s)
     const TriBounds tb = tri_bounds(d, e, e2, N, red4);
     const double onenrm = tb.onenrm, pivmin = tb.pivmin;
     const double eps = DBL_EPSILON * 0.5;
-    const double ortol = s.ortol_rel * onenrm;
-    if (tid == 0) {
-        int nc = 0;
-        for (int j = 0; j < N; ++j)
-            if (j == 0 || fabs(w[j] - w[j - 1]) > ortol) cstart[nc++] = j;
-        cstart[nc] = N;
-        ncl = nc;
-    }
-    __syncthreads();
     double *wa = scr + 0 * (size_t)N * nthr, *wb = scr + 1 * (size_t)N * nthr;
     double *wc = scr + 2 * (size_t)N * nthr, *wd = scr + 3 * (size_t)N * nthr;
     double *wy = scr + 4 * (size_t)N * nthr;
 #define AT(arr, i) arr[(size_t)(i) * nthr + gt]
     const double dtpcrt = sqrt(0.1 / N);
     int fails = 0;
-    const int cl = gt;
-    if (cl < ncl) {
-        const int j0 = cstart[cl], j1 = cstart[cl + 1];
-        double xjm = 0.0;
-        for (int j = j0; j < j1; ++j) {
-            if (j == j0 || w[j] - w[j - 1] > 1e-10 * onenrm) {
-                // D+ (top-down) into scratch; U_i = e_i / D-_{i+1} (bottom-up)
-                // into slot i+1 of the output column; z overwrites both.
-                const double lam = w[j];
-                double dp = d[0] - lam;
+    const int j = gt;
+    const double dtol = 1e-10 * onenrm;
+    if (j < N) {
+        if (j == 0 || w[j] - w[j - 1] > dtol) {
+            // D+ (top-down) into scratch; U_i = e_i / D-_{i+1} (bottom-up)
+            // into slot i+1 of the output column; z overwrites both.
+            const double lam = w[j];
+            double dp = d[0] - lam;
+            if (fabs(dp) < pivmin) dp = -pivmin;
+            AT(wa, 0) = dp;
+            for (int i = 0; i < N - 1; ++i) {
+                const double l = e[i] * rcp_nr(dp);
+                dp = d[i + 1] - lam - l * e[i];
                 if (fabs(dp) < pivmin) dp = -pivmin;
-                AT(wa, 0) = dp;
-                for (int i = 0; i < N - 1; ++i) {
-                    const double l = e[i] * rcp_nr(dp);
-                    dp = d[i + 1] - lam - l * e[i];
-                    if (fabs(dp) < pivmin) dp = -pivmin;
-                    AT(wa, i + 1) = dp;
-                }
-                double dm = d[N - 1] - lam;
-                if (fabs(dm) < pivmin) dm = -pivmin;
-                double gbest = fabs(AT(wa, N - 1) + dm - (d[N - 1] - lam));
-                int r = N - 1;
-                for (int i = N - 2; i >= 0; --i) {
-                    const double u = e[i] * rcp_nr(dm);
-                    Vt[(size_t)(i + 1) * LD + j] = u;
-                    dm = d[i] - lam - u * e[i];
-                    if (fabs(dm) < pivmin) dm = -pivmin;
-                    const double g = fabs(AT(wa, i) + dm - (d[i] - lam));
-                    if (g < gbest) { gbest = g; r = i; }
-                }
-                double z = 1.0, ss = 1.0;
-                Vt[(size_t)r * LD + j] = 1.0;
-                for (int i = r; i < N - 1; ++i) {
-                    z = -Vt[(size_t)(i + 1) * LD + j] * z;
-                    Vt[(size_t)(i + 1) * LD + j] = z;
-                    ss += z * z;
-                }
-                z = 1.0;
-                for (int i = r - 1; i >= 0; --i) {
-                    z = -(e[i] * rcp_nr(AT(wa, i))) * z;
-                    Vt[(size_t)i * LD + j] = z;
-                    ss += z * z;
-                }
-                if (j > j0) {
-                    // orthogonalise against the group's earlier vectors
-                    for (int jp = j0; jp < j; ++jp) {
-                        double dot = 0.0;
-                        for (int i = 0; i < N; ++i) dot += Vt[(size_t)i * LD + j] * Vt[(size_t)i * LD + jp];
-                        for (int i = 0; i < N; ++i) Vt[(size_t)i * LD + j] -= dot * Vt[(size_t)i * LD + jp];
-                    }
-                    ss = 0.0;
-                    for (int i = 0; i < N; ++i) ss += Vt[(size_t)i * LD + j] * Vt[(size_t)i * LD + j];
-                }
-                const double scl = 1.0 / sqrt(ss);
-                for (int i = 0; i < N; ++i) Vt[(size_t)i * LD + j] *= scl;
-                xjm = lam;
-                continue;
+                AT(wa, i + 1) = dp;
             }
-            // near-exact degeneracy: inverse iteration from a distinct start
-            // vector with the group's reorthogonalisation (dstein)
-            double xj = w[j];
-            if (j > j0) {
+            double dm = d[N - 1] - lam;
+            if (fabs(dm) < pivmin) dm = -pivmin;
+            double gbest = fabs(AT(wa, N - 1) + dm - (d[N - 1] - lam));
+            int r = N - 1;
+            for (int i = N - 2; i >= 0; --i) {
+                const double u = e[i] * rcp_nr(dm);
+                Vt[(size_t)(i + 1) * LD + j] = u;
+                dm = d[i] - lam - u * e[i];
+                if (fabs(dm) < pivmin) dm = -pivmin;
+                const double g = fabs(AT(wa, i) + dm - (d[i] - lam));
+                if (g < gbest) { gbest = g; r = i; }
+            }
+            double z = 1.0, ss = 1.0;
+            Vt[(size_t)r * LD + j] = 1.0;
+            for (int i = r; i < N - 1; ++i) {
+                z = -Vt[(size_t)(i + 1) * LD + j] * z;
+                Vt[(size_t)(i + 1) * LD + j] = z;
+                ss += z * z;
+            }
+            z = 1.0;
+            for (int i = r - 1; i >= 0; --i) {
+                z = -(e[i] * rcp_nr(AT(wa, i))) * z;
+                Vt[(size_t)i * LD + j] = z;
+                ss += z * z;
+            }
+            const double scl = 1.0 / sqrt(ss);
+            for (int i = 0; i < N; ++i) Vt[(size_t)i * LD + j] *= scl;
+        } else {
+            // shift of member j of the degenerate run, perturbed away from
+            // the previous member's shift (dstein's pertol)
+            int j0 = j;
+            while (j0 > 0 && w[j0] - w[j0 - 1] <= dtol) --j0;
+            double xjm = w[j0], xj = w[j0];
+            for (int jj = j0 + 1; jj <= j; ++jj) {
+                xj = w[jj];
                 const double pertol = 10.0 * fabs(eps * xj);
                 if (xj - xjm < pertol) xj = xjm + pertol;
+                xjm = xj;
             }
             // LU of T - xj I with partial pivoting (dlagtf)
             for (int i = 0; i < N; ++i) {
@@ -1065,12 +1044,6 @@ __global__ __launch_bounds__(64) void k_stein_vec(SteinArgs s)
                     }
                     Vt[(size_t)k * LD + j] = temp / ak;
                 }
-                // reorthogonalise against earlier members of the group
-                for (int jp = j0; jp < j; ++jp) {
-                    double dot = 0.0;
-                    for (int i = 0; i < N; ++i) dot += Vt[(size_t)i * LD + j] * Vt[(size_t)i * LD + jp];
-                    for (int i = 0; i < N; ++i) Vt[(size_t)i * LD + j] -= dot * Vt[(size_t)i * LD + jp];
-                }
                 double amax = 0.0;
                 for (int i = 0; i < N; ++i) amax = fmax(amax, fabs(Vt[(size_t)i * LD + j]));
                 if (amax < dtpcrt) continue;
@@ -1097,16 +1070,75 @@ __global__ __launch_bounds__(64) void k_stein_vec(SteinArgs s)
     if (fails) atomicAdd(s.info + kb, fails);
     if (N < LD && blockIdx.x == 0) {       // padding: zero rows, unit columns
         for (int t = tid; t < (LD - N) * LD; t += blockDim.x) {
-            const int i = N + t / LD, j = t % LD;
-            Vt[(size_t)i * LD + j] = (i == j) ? 1.0 : 0.0;
+            const int i = N + t / LD, jj = t % LD;
+            Vt[(size_t)i * LD + jj] = (i == jj) ? 1.0 : 0.0;
         }
         for (int t = tid; t < N * (LD - N); t += blockDim.x) {
-            const int i = t / (LD - N), j = N + t % (LD - N);
-            Vt[(size_t)i * LD + j] = 0.0;
+            const int i = t / (LD - N), jj = N + t % (LD - N);
+            Vt[(size_t)i * LD + jj] = 0.0;
         }
     }
 }
 
+// Orthogonalisation of the vectors of each group of close eigenvalues
+// (consecutive gaps <= ortol_rel * onenrm, as dstein groups them), one
+// workgroup per k: classical Gram-Schmidt applied twice per vector, in
+// ascending order, against the group's earlier (already orthonormal)
+// vectors, with the dot products of one vector formed together.
+__global__ __launch_bounds__(256) void k_stein_orth(SteinArgs s)
+{
+    __shared__ double d[1280], e[1280], e2[1280], w[1280];
+    __shared__ double dots[1280];
+    __shared__ double red4[16 * 4];
+    __shared__ double redn[4];
+    const int LD = s.n, kb = blockIdx.x, tid = threadIdx.x;
+    const int N = s.nact ? s.nact[kb] : LD;
+    const double *dg = s.d + (size_t)kb * LD;
+    const double *eg = s.e + (size_t)kb * LD;
+    double *Vt = s.v + (size_t)kb * LD * LD;
+    for (int i = tid; i < N; i += blockDim.x) {
+        d[i] = dg[i];
+        e[i] = (i < N - 1) ? eg[i] : 0.0;
+        e2[i] = e[i] * e[i];
+        w[i] = s.w[(size_t)kb * LD + i];
+    }
+    __syncthreads();
+    const TriBounds tb = tri_bounds(d, e, e2, N, red4);
+    const double ortol = s.ortol_rel * tb.onenrm;
+    int j0 = 0;
+    while (j0 < N) {
+        int j1 = j0 + 1;
+        while (j1 < N && w[j1] - w[j1 - 1] <= ortol) ++j1;
+        for (int j = j0 + 1; j < j1; ++j) {
+            const int g = j - j0;
+            for (int pass = 0; pass < 2; ++pass) {
+                for (int l = tid; l < g; l += blockDim.x) {
+                    double acc = 0.0;
+                    for (int i = 0; i < N; ++i) acc += Vt[(size_t)i * LD + j0 + l] * Vt[(size_t)i * LD + j];
+                    dots[l] = acc;
+                }
+                __syncthreads();
+                for (int i = tid; i < N; i += blockDim.x) {
+                    const double *row = Vt + (size_t)i * LD + j0;
+                    double acc = 0.0;
+                    for (int l = 0; l < g; ++l) acc += row[l] * dots[l];
+                    Vt[(size_t)i * LD + j] -= acc;
+                }
+                __syncthreads();
+            }
+            double ss = 0.0;
+            for (int i = tid; i < N; i += blockDim.x) ss += Vt[(size_t)i * LD + j] * Vt[(size_t)i * LD + j];
+            for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+            if ((tid & 63) == 0) redn[tid >> 6] = ss;
+            __syncthreads();
+            ss = redn[0] + redn[1] + redn[2] + redn[3];
+            const double scl = 1.0 / sqrt(ss);
+            for (int i = tid; i < N; i += blockDim.x) Vt[(size_t)i * LD + j] *= scl;
+            __syncthreads();
+        }
+        j0 = j1;
+    }
+}
 
 // Complex copy of the real row-major eigenvectors: Y[:, j] = v_j.
 __global__ void k_real_to_complex_t(int N, const double *Vt, double2 *Y, size_t stride)
@@ -1676,6 +1708,7 @@ void launch_stein(const SteinArgs &s, int nb, hipStream_t st)
 {
     hipLaunchKernelGGL(k_stebz, dim3((s.n + 255) / 256, nb), dim3(256), 0, st, s);
     hipLaunchKernelGGL(k_stein_vec, dim3((s.n + 63) / 64, nb), dim3(64), 0, st, s);
+    hipLaunchKernelGGL(k_stein_orth, dim3(nb), dim3(256), 0, st, s);
 }
 void launch_real_to_complex_t(int N, const double *Vt, double2 *Y, size_t stride, int nb,
                               hipStream_t st)

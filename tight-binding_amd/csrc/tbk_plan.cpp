@@ -17,6 +17,12 @@ namespace {
 
 constexpr int kNB = 32;        // panel width of the factorizations
 constexpr int kMaxN = 1280;    // LDS-resident vectors of k_hetrd / k_stein
+// Eigenvalues closer than kOrtolRel * ||T||_1 have their vectors
+// orthogonalised together.  A twisted-factorisation vector of an eigenvalue
+// with gap g is accurate to an angle of about eps ||T|| / g, so vectors
+// outside a group are orthogonal to about 1e-10 (dstein's 1e-3 serves
+// inverse iteration, whose vectors depend on the separation much more).
+constexpr double kOrtolRel = 1e-6;
 
 #define HIP_TRY(x)                                   \
     do {                                             \
@@ -166,21 +172,27 @@ struct StageTimer {
 
 // Blocked tridiagonalisation: per panel one k_hetrd_panel launch and one
 // GEMM for the trailing update A[s:, s:] -= [V W] [W V]^H.
-void hetrd(Workspace &ws, int nb, hipStream_t st)
+void hetrd(Workspace &ws, int nb, hipStream_t st, StageTimer &tm)
 {
     const int n = ws.n;
     const size_t ss = ws.stride(), sp = (size_t)n * 96;
     tbk::HetrdArgs h{n, ss, ws.A, ws.Vp, ws.Wp, ws.Tf, ws.VT, ws.d, ws.e, ws.tstride()};
     for (int i0 = 0; i0 < n - 1; i0 += kNB) {
         const int pw = std::min(kNB, n - 1 - i0);
+        tm.begin("hetrd_panel");
         tbk::launch_hetrd_panel(h, i0, nb, st);
+        tm.end();
         const int s = i0 + pw, m = n - s;
+        tm.begin("hetrd_update");
         if (m > 0)
             tbk::launch_zgemm(gemm(m, m, 2 * kNB, ws.Vp + s, n, sp, ws.Vp + (size_t)kNB * n + s, n, sp,
                                    ws.A + (size_t)s * n + s, n, ss, 4, double2{-1.0, 0.0},
                                    double2{1.0, 0.0}), 0, 1, nb, st);
+        tm.end();
     }
+    tm.begin("hetrd_update");
     tbk::launch_hetrd_finish(h, nb, st);
+    tm.end();
 }
 
 // S = U^H U and U^{-1} in place (lower triangle cleared): panel kernels for
@@ -233,22 +245,28 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
     tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.S, n, ss, ws.Y, n, ss, 3), 0, 0, nb, st);
     tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 2), 1, 0, nb, st);
     tm.end();
-    tm.begin("hetrd");
+    tm.begin("hetrd_update");
     tbk::launch_c64_to_c32(ss * nb, ws.A, ws.A32, st);
+    tm.end();
     tbk::HetrdArgsT<float> h{n, ss, ws.A32, ws.P32, nullptr, ws.T32, ws.VT32, ws.d, ws.e, ws.tstride()};
     for (int i0 = 0; i0 < n - 1; i0 += kNB) {
         const int pw = std::min(kNB, n - 1 - i0);
+        tm.begin("hetrd_panel");
         tbk::launch_hetrd_panel(h, i0, nb, st);
+        tm.end();
         const int s = i0 + pw, m = n - s;
+        tm.begin("hetrd_update");
         if (m > 0)
             tbk::launch_cgemm(cgemm(m, m, 2 * kNB, ws.P32 + s, n, sp, ws.P32 + (size_t)kNB * n + s, n, sp,
                                     ws.A32 + (size_t)s * n + s, n, ss, 4, float2{-1.f, 0.f},
                                     float2{1.f, 0.f}), 0, 1, nb, st);
+        tm.end();
     }
+    tm.begin("hetrd_update");
     tbk::launch_hetrd_finish(h, nb, st);
     tm.end();
     tm.begin("stein");
-    tbk::SteinArgs sa{n, 1e-4, nact, ws.d, ws.e, ws.w, ws.Vt, ws.scratch, ws.sinfo};
+    tbk::SteinArgs sa{n, kOrtolRel, nact, ws.d, ws.e, ws.w, ws.Vt, ws.scratch, ws.sinfo};
     tbk::launch_stein(sa, nb, st);
     tm.end();
     tm.begin("backtransform");
@@ -291,11 +309,9 @@ void solve_batch(Workspace &ws, int nb, const int *nact, double *w_out, hipStrea
     tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.S, n, ss, ws.Y, n, ss, 3), 0, 0, nb, st);
     tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 2), 1, 0, nb, st);
     tm.end();
-    tm.begin("hetrd");
-    hetrd(ws, nb, st);
-    tm.end();
+    hetrd(ws, nb, st, tm);
     tm.begin("stein");
-    tbk::SteinArgs sa{n, 1e-4, nact, ws.d, ws.e, w_out, ws.Vt, ws.scratch, ws.sinfo};
+    tbk::SteinArgs sa{n, kOrtolRel, nact, ws.d, ws.e, w_out, ws.Vt, ws.scratch, ws.sinfo};
     tbk::launch_stein(sa, nb, st);
     tm.end();
     tm.begin("backtransform");
@@ -989,12 +1005,15 @@ extern "C" int tbk_debug_stages(int device, int n, const double *h, const double
     tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 2), 1, 0, 1, st);
     (void)hipDeviceSynchronize();
     get(c, ws.A, ss * sizeof(double2));
-    hetrd(ws, 1, st);
+    {
+        StageTimer off;
+        hetrd(ws, 1, st, off);
+    }
     (void)hipDeviceSynchronize();
     get(refl, ws.A, ss * sizeof(double2));
     get(d, ws.d, n * sizeof(double));
     get(e, ws.e, n * sizeof(double));
-    tbk::SteinArgs sa{n, 1e-4, nullptr, ws.d, ws.e, ws.w, ws.Vt, ws.scratch, ws.sinfo};
+    tbk::SteinArgs sa{n, kOrtolRel, nullptr, ws.d, ws.e, ws.w, ws.Vt, ws.scratch, ws.sinfo};
     tbk::launch_stein(sa, 1, st);
     (void)hipDeviceSynchronize();
     get(w, ws.w, n * sizeof(double));

@@ -1,0 +1,62 @@
+"""The Fortran-ABI drop-in: the reference's own program (static.f and its
+sources, built by oracle/Makefile from the reference tree) links against
+bande_/fermi_ from lib/libtbk_fortran.a in place of bande.o and fermi.o.
+The link and the layout checks need no GPU; the run is in
+test_gpu_dropin.py."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+REF = "/root/reference/NRLtb/ex1"
+ORACLE = os.path.join(ROOT, "oracle")
+LIBDIR = os.path.join(ROOT, "tight-binding_amd", "lib")
+
+needs_ref = pytest.mark.skipif(not os.path.isdir(REF) or not shutil.which("make"),
+                               reason="reference tree not present")
+
+
+def nm_defined(path):
+    out = subprocess.run(["nm", "-g", "--defined-only", path], capture_output=True, text=True,
+                         check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if line.strip()}
+
+
+def test_adapter_exports_fortran_symbols():
+    syms = nm_defined(os.path.join(LIBDIR, "libtbk_fortran.a"))
+    assert {"bande_", "fermi_"} <= syms
+    undefined = subprocess.run(["nm", "-u", os.path.join(LIBDIR, "libtbk_fortran.a")],
+                               capture_output=True, text=True, check=True).stdout
+    # the adapter reads the program's COMMON blocks and calls the C ABI
+    for sym in ("codes_", "latt1_", "fazon_", "parinfo_", "cutcom_", "disply_", "lia_", "cd1_",
+                "params_", "parcom_", "tbk_plan_create", "tbk_plan_run", "tbk_fermi"):
+        assert sym in undefined, sym
+
+
+@needs_ref
+def test_reference_program_links_against_dropin():
+    r = subprocess.run(["make", "-C", ORACLE, "dropin"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    exe = os.path.join(ORACLE, "_ref", "static_tbk")
+    syms = nm_defined(exe)
+    assert "bande_" in syms and "fermi_" in syms and "codes_" in syms
+    needed = subprocess.run(["readelf", "-d", exe], capture_output=True, text=True).stdout
+    assert "libtbk.so" in needed
+
+
+@needs_ref
+def test_reference_program_reproduces_published_ex1(tmp_path):
+    """The reference program itself (not the drop-in) on ex1 reproduces the
+    published SKENG: the build recipe is faithful."""
+    subprocess.run(["make", "-C", ORACLE, "dropin"], capture_output=True, check=True)
+    work = tmp_path / "ex1"
+    shutil.copytree(os.path.join(ORACLE, "_ref", "ex1"), work)
+    r = subprocess.run("ulimit -s unlimited; " + os.path.join(ORACLE, "_ref", "static_ref"),
+                       shell=True, cwd=work, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    ours = [line.split() for line in open(work / "SKENG")]
+    pub = [line.split() for line in open(os.path.join(REF, "SKENG"))]
+    assert ours == pub

@@ -1,0 +1,38 @@
+"""The reference's own program run with the device drop-in: oracle/_ref/
+static_tbk (its Fortran objects linked against bande_/fermi_ of
+lib/libtbk_fortran.a) on the ex1 inputs writes the published SKENG."""
+import json
+import os
+import shutil
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(ROOT, "oracle", "_ref", "static_tbk")
+INPUTS = os.path.join(ROOT, "oracle", "_ref", "ex1")
+GOLD = os.path.join(ROOT, "tests", "golden", "ex1.json")
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="drop-in program not built (oracle/Makefile dropin)")
+def test_reference_program_with_device_bande(tmp_path):
+    work = tmp_path / "ex1"
+    shutil.copytree(INPUTS, work)
+    print("case: static_tbk on ex1 inputs", flush=True)
+    r = subprocess.run("ulimit -s unlimited; " + EXE, shell=True, cwd=work, capture_output=True,
+                       text=True, timeout=300, env=dict(os.environ, TBK_DEVICE="0"))
+    assert r.returncode == 0, (r.stdout[-1000:], r.stderr[-2000:])
+    rows = [line.split() for line in open(work / "SKENG") if line.strip()]
+    gold = json.load(open(GOLD))["structures"]
+    assert len(rows) == len(gold)
+    worst = 0.0
+    for row, g in zip(rows, gold):
+        vol, ef, et, pt = (float(x) for x in row[2:6])
+        assert row[0] == g["lattice"]
+        for got, ref in ((vol, g["volume"]), (ef, g["efermi"]), (et, g["et"]), (pt, g["pt"])):
+            worst = max(worst, abs(got - ref))
+    print(f"  worst |SKENG - published| = {worst:.1e} (9 printed decimals)", flush=True)
+    assert worst <= 1e-9 * 1.5

@@ -293,6 +293,10 @@ const char *tbk_error_string(int code);
 int tbk_debug_stages(int device, int n, const double *h, const double *s,
                      double *uinv, double *c, double *refl, double *d,
                      double *e, double *w, double *vt, double *z);
+/* One launch of a plain copy kernel over `bytes` bytes (a multiple of 16):
+ * the calibration point for hardware traffic counters. */
+int tbk_debug_copy(int device, long long bytes);
+
 /* C = op(A) op(B) of column-major complex matrices (complex128, or
  * complex64 when precision is TBK_MIXED) with the library's batched GEMM;
  * tile -1 = the tuned / modelled choice, else 0..3 (see tbk_gemm_tile). */

@@ -138,5 +138,6 @@ void launch_perturb(const PerturbArgs &p, hipStream_t st);
 void launch_fermi_sums(const FermiArgs &f, hipStream_t st);
 void launch_fermi_sums_group(const FermiArgs *f_dev, int nstruct, hipStream_t st);   // one WG each
 void launch_copy_int(const int *src, int *dst, int n, hipStream_t st);
+void launch_copy16(const double2 *src, double2 *dst, size_t n, hipStream_t st);
 
 }  // namespace tbk

@@ -1593,6 +1593,14 @@ __global__ __launch_bounds__(1024) void k_fermi_sums(FermiArgs f) { fermi_sums_b
 // one workgroup per structure of a plan
 __global__ __launch_bounds__(1024) void k_fermi_sums_group(const FermiArgs *f) { fermi_sums_body(f[blockIdx.x]); }
 
+// Plain streaming copy (16-byte elements), the calibration kernel for
+// counter-based traffic measurements.
+__global__ __launch_bounds__(256) void k_copy16(const double2 *src, double2 *dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_copy_int(const int *src, int *dst, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1750,6 +1758,10 @@ void launch_perturb(const PerturbArgs &p, hipStream_t st)
 void launch_fermi_sums_group(const FermiArgs *f_dev, int nstruct, hipStream_t st)
 {
     hipLaunchKernelGGL(k_fermi_sums_group, dim3(nstruct), dim3(1024), 0, st, f_dev);
+}
+void launch_copy16(const double2 *src, double2 *dst, size_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, st, src, dst, n);
 }
 void launch_copy_int(const int *src, int *dst, int n, hipStream_t st)
 {

@@ -1072,3 +1072,24 @@ extern "C" int tbk_debug_zgemm(int device, int precision, int tile, int m, int n
         return debug_gemm_t<float2, tbk::GemmArgsT<float>>(tile, m, n, k, opa, opb, tri, a, lda, b, ldb, c, ldc);
     return debug_gemm_t<double2, tbk::GemmArgsT<double>>(tile, m, n, k, opa, opb, tri, a, lda, b, ldb, c, ldc);
 }
+
+// One launch of a plain device-to-device copy kernel over `bytes` bytes
+// (reads bytes, writes bytes): the reference point for traffic counters.
+extern "C" int tbk_debug_copy(int device, long long bytes)
+{
+    if (bytes <= 0 || bytes % 16) return TBK_ERR_ARG;
+    HIP_TRY(hipSetDevice(device));
+    double2 *a = nullptr, *b = nullptr;
+    const size_t n = (size_t)bytes / 16;
+    int rc = dalloc(&a, n) | dalloc(&b, n);
+    if (!rc) {
+        (void)hipMemset(a, 0, n * sizeof(double2));
+        (void)hipDeviceSynchronize();
+        tbk::launch_copy16(a, b, n, nullptr);
+        if (hipDeviceSynchronize() != hipSuccess) rc = TBK_ERR_HIP;
+    } else {
+        rc = TBK_ERR_NOMEM;
+    }
+    dfree(a); dfree(b);
+    return rc;
+}

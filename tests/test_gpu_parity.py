@@ -224,3 +224,23 @@ def test_mixed_precision_error_vs_oracle(cfg):
     assert de < MIX_EIG
     assert dP < MIX_P_KBAR
     assert orth < MIX_ORTH and res < 1e-9
+
+
+@pytest.mark.parametrize("prec", [_lib.TBK_FP64, _lib.TBK_MIXED])
+def test_tile_and_panel_tridiagonalisation_agree(prec):
+    """Batches below 160 items of order >= 128 tridiagonalise over 64x64
+    tiles (several workgroups per item); larger batches with one workgroup
+    per item.  Both give the same results."""
+    st = S.config_c2()
+    k = S.monkhorst_pack((8, 8, 8))[:192]
+    w = np.full(len(k), 1.0 / len(k))
+    out = {}
+    for b in (192, 16):
+        p = Plan(st, k, w, batch=b, precision=prec)
+        out[b] = (p.run(4), p.eigenvalues()[0])
+        p.close()
+    de = float(np.abs(out[192][1] - out[16][1]).max())
+    dr = max(abs(out[192][0][x] - out[16][0][x]) for x in ("efermi", "earg1", "earg2"))
+    print(f"tile vs panel (precision {prec}): max|de|={de:.2e} max|dresult|={dr:.2e}", flush=True)
+    tol = 1e-11 if prec == _lib.TBK_FP64 else 1e-9
+    assert de < tol and dr < tol

@@ -63,6 +63,16 @@ struct HetrdArgsT {
 };
 typedef HetrdArgsT<double> HetrdArgs;
 
+// Scratch of the multi-workgroup tridiagonalisation (per item): v, the
+// column / w vector, the next column's delayed update (carry), tile
+// partial sums ydot, yax [tmax][n], the panel dots (2*32) and tau.
+template <class R>
+struct WideArgsT {
+    typedef typename cplx_t<R>::type C;
+    int tmax;                      // ceil((n - 1) / 64)
+    C *v, *col, *carry, *ydot, *yax, *udot, *scal;
+};
+
 struct SteinArgs {
     int n;
     double ortol_rel;              // group threshold relative to ||T||_1
@@ -120,6 +130,13 @@ void launch_gemm_tile(const GemmShape &s, int tile, hipStream_t st);   // a reco
 void launch_hetrd_panel(const HetrdArgs &h, int i0, int nb, hipStream_t st);
 void launch_hetrd_finish(const HetrdArgs &h, int nb, hipStream_t st);
 void launch_hetrd_panel(const HetrdArgsT<float> &h, int i0, int nb, hipStream_t st);
+// multi-workgroup form: one column (three launches), and V T^H after a panel
+void launch_hetrd_wide_column(const HetrdArgs &h, const WideArgsT<double> &w, int i0, int i, int nb,
+                              hipStream_t st);
+void launch_hetrd_wide_column(const HetrdArgsT<float> &h, const WideArgsT<float> &w, int i0, int i, int nb,
+                              hipStream_t st);
+void launch_hetrd_wide_vt(const HetrdArgs &h, int i0, int nb, hipStream_t st);
+void launch_hetrd_wide_vt(const HetrdArgsT<float> &h, int i0, int nb, hipStream_t st);
 void launch_hetrd_finish(const HetrdArgsT<float> &h, int nb, hipStream_t st);
 int stein_threads(int n);        // eigenvector threads per k-point
 void launch_stein(const SteinArgs &s, int nb, hipStream_t st);

@@ -736,6 +736,236 @@ __global__ __launch_bounds__(HT, (MAXQ <= 5 ? 4 : 2)) void k_hetrd_panel(HetrdAr
     }
 }
 
+// ---------------------------------------------------------------------------
+// Householder tridiagonalisation, multi-workgroup form for few items of
+// large order: per column three launches -- column update and reflector
+// (one workgroup per item), the Hermitian product over 64x64 tiles of the
+// lower triangle (many workgroups per item, partial sums per tile row and
+// tile column), then w, the WY factor and the delayed-update panel (one
+// workgroup per item).  Same algorithm and outputs as k_hetrd_panel; the
+// partial sums are combined in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr int WT = 1024;           // threads of the per-item kernels
+
+template <class R>
+__device__ __forceinline__ R block_sum_r(R v, R *red)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    R s = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += red[q];
+    __syncthreads();
+    return s;
+}
+
+template <class R>
+__global__ __launch_bounds__(WT) void k_hw_prep(HetrdArgsT<R> h, WideArgsT<R> w, int i0, int i)
+{
+    typedef typename cplx<R>::type C;
+    __shared__ R red[WT / 64];
+    __shared__ C scal[2];
+    const int N = h.n, kb = blockIdx.x, tid = threadIdx.x;
+    const int j = i0 + i;
+    C *A = h.a + (size_t)kb * h.stride;
+    C *P = h.vp + (size_t)kb * N * 96;
+    C *V = P, *W = P + (size_t)NB * N, *V2 = P + (size_t)2 * NB * N;
+    C *Tp = h.t + (size_t)kb * h.tstride + (size_t)(i0 / NB) * NB * NB;
+    C *vv = w.v + (size_t)kb * N;
+    C *col = w.col + (size_t)kb * N;
+    if (i == 0) {
+        for (int t = tid; t < N * 96; t += WT) P[t] = czero<C>();
+        for (int t = tid; t < NB * NB; t += WT) Tp[t] = czero<C>();
+        __syncthreads();
+    }
+    // delayed update of column j: carry = V[r,:] conj(W[j,:]) + W[r,:] conj(V[j,:]),
+    // accumulated by k_hw_fin of the previous column
+    const C *carry = w.carry + (size_t)kb * N;
+    R xs = 0;
+    for (int r = j + tid; r < N; r += WT) {
+        C a = A[(size_t)j * N + r];
+        if (i > 0) a = zsub(a, carry[r]);
+        if (r == j) a.y = 0;
+        col[r] = a;
+        if (r >= j + 2) xs += a.x * a.x + a.y * a.y;
+    }
+    xs = block_sum_r(xs, red);
+    if (tid == 0) {
+        const C al = col[j + 1];
+        C tau = czero<C>();
+        R beta = al.x;
+        if (!(xs == R(0) && al.y == R(0))) {
+            beta = -copysign(sqrt(al.x * al.x + al.y * al.y + xs), al.x);
+            tau = cmake<C, R>((beta - al.x) / beta, -al.y / beta);
+        }
+        scal[0] = tau; scal[1] = cmake<C, R>(beta, R(0));
+        w.scal[2 * (size_t)kb] = tau;
+        h.d[(size_t)kb * N + j] = (double)col[j].x;
+        h.e[(size_t)kb * N + j] = (double)beta;
+    }
+    __syncthreads();
+    const C tau = scal[0];
+    const R beta = scal[1].x;
+    const C al = col[j + 1];
+    const C dv = cmake<C, R>(al.x - beta, al.y);
+    const R den = dv.x * dv.x + dv.y * dv.y;
+    const bool refl = tau.x != R(0) || tau.y != R(0);
+    const C sc = refl ? cmake<C, R>(dv.x / den, -dv.y / den) : cmake<C, R>(R(1), R(0));
+    for (int r = tid; r < N; r += WT) {
+        C v;
+        if (r <= j) v = czero<C>();
+        else if (r == j + 1) v = cmake<C, R>(R(1), R(0));
+        else v = zmul(col[r], sc);
+        A[(size_t)j * N + r] = v;
+        V[(size_t)i * N + r] = v;
+        V2[(size_t)i * N + r] = v;
+        vv[r] = v;
+    }
+}
+
+// Tile (tr, tc), tr >= tc, of the lower triangle of the trailing matrix
+// (rows/columns >= t0): column dots conj(A[r, c]) v_r over the tile's rows
+// r >= c into ydot[tr][c]; row sums A[r, c] v_c over the tile's columns
+// c < r into yax[tc][r].  The extra workgroup blockIdx.x == ntile forms the
+// panel dots W[:, l]^H v and V[:, l]^H v.
+template <class R>
+__global__ __launch_bounds__(256) void k_hw_hemv(HetrdArgsT<R> h, WideArgsT<R> w, int i0, int i, int ntile)
+{
+    typedef typename cplx<R>::type C;
+    __shared__ C acc_s[4][64];
+    const int N = h.n, kb = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int j = i0 + i, t0 = j + 1, m = N - t0;
+    const int T = (m + 63) >> 6;
+    const C *A = h.a + (size_t)kb * h.stride;
+    const C *vv = w.v + (size_t)kb * N;
+    const size_t tstr = (size_t)w.tmax * N;
+    if ((int)blockIdx.x == ntile) {
+        if (i == 0) return;
+        const C *P = h.vp + (size_t)kb * N * 96;
+        const C *V = P, *W = P + (size_t)NB * N;
+        for (int task = wv; task < 2 * i; task += 4) {
+            const C *colp = task < i ? W + (size_t)task * N : V + (size_t)(task - i) * N;
+            C acc = czero<C>();
+            for (int c = t0 + lane; c < N; c += 64) acc = zadd(acc, zcmul(colp[c], vv[c]));
+            for (int off = 32; off > 0; off >>= 1) {
+                acc.x += __shfl_xor(acc.x, off, 64);
+                acc.y += __shfl_xor(acc.y, off, 64);
+            }
+            if (lane == 0) w.udot[(size_t)kb * 2 * NB + (task < i ? task : NB + task - i)] = acc;
+        }
+        return;
+    }
+    const int tile = blockIdx.x;
+    if (tile >= T * (T + 1) / 2) return;
+    int tr = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
+    while ((tr + 1) * (tr + 2) / 2 <= tile) ++tr;
+    while (tr * (tr + 1) / 2 > tile) --tr;
+    const int tc = tile - tr * (tr + 1) / 2;
+    const int r = t0 + 64 * tr + lane;
+    const C vr = r < N ? vv[r] : czero<C>();
+    C acc = czero<C>();
+    C *ydot = w.ydot + (size_t)kb * tstr + (size_t)tr * N;
+    for (int u = 0; u < 16; ++u) {
+        const int c = t0 + 64 * tc + 16 * wv + u;
+        if (c >= N) break;
+        const C a = r < N ? A[(size_t)c * N + r] : czero<C>();
+        C pd = r >= c ? zcmul(a, vr) : czero<C>();
+        if (c < r) acc = zadd(acc, zmul(a, vv[c]));
+        for (int off = 32; off > 0; off >>= 1) {
+            pd.x += __shfl_xor(pd.x, off, 64);
+            pd.y += __shfl_xor(pd.y, off, 64);
+        }
+        if (lane == 0) ydot[c] = pd;
+    }
+    acc_s[wv][lane] = acc;
+    __syncthreads();
+    if (wv == 0 && r < N)
+        w.yax[(size_t)kb * tstr + (size_t)tc * N + r] =
+            zadd(zadd(zadd(acc_s[0][lane], acc_s[1][lane]), acc_s[2][lane]), acc_s[3][lane]);
+}
+
+template <class R>
+__global__ __launch_bounds__(WT) void k_hw_fin(HetrdArgsT<R> h, WideArgsT<R> w, int i0, int i)
+{
+    typedef typename cplx<R>::type C;
+    __shared__ C udot[2 * NB];
+    __shared__ C uwn[NB], uvn[NB];     // conj(W[j+1, l]), conj(V[j+1, l])
+    __shared__ R red[2 * (WT / 64)];
+    const int N = h.n, kb = blockIdx.x, tid = threadIdx.x;
+    const int j = i0 + i, t0 = j + 1, m = N - t0;
+    const int T = (m + 63) >> 6;
+    C *P = h.vp + (size_t)kb * N * 96;
+    C *V = P, *W = P + (size_t)NB * N;
+    C *Tp = h.t + (size_t)kb * h.tstride + (size_t)(i0 / NB) * NB * NB;
+    const C *vv = w.v + (size_t)kb * N;
+    C *col = w.col + (size_t)kb * N;
+    const size_t tstr = (size_t)w.tmax * N;
+    const C *ydot = w.ydot + (size_t)kb * tstr, *yax = w.yax + (size_t)kb * tstr;
+    if (tid < 2 * NB) udot[tid] = w.udot[(size_t)kb * 2 * NB + tid];
+    if (tid < i && t0 < N) { uwn[tid] = zconj(W[(size_t)tid * N + t0]); uvn[tid] = zconj(V[(size_t)tid * N + t0]); }
+    const C tau = w.scal[2 * (size_t)kb];
+    __syncthreads();
+    // w = tau (y - V u1 - W u2); the same pass over V[r, 0:i], W[r, 0:i]
+    // accumulates the next column's delayed update into carry
+    C *carry = w.carry + (size_t)kb * N;
+    R pr = 0, pi = 0;
+    for (int r = t0 + tid; r < N; r += WT) {
+        const int tr0 = (r - t0) >> 6;
+        C y = czero<C>(), nx = czero<C>();
+        for (int tr = tr0; tr < T; ++tr) y = zadd(y, ydot[(size_t)tr * N + r]);
+        for (int tc = 0; tc <= tr0; ++tc) y = zadd(y, yax[(size_t)tc * N + r]);
+#pragma unroll 4
+        for (int l = 0; l < i; ++l) {
+            const C vl = V[(size_t)l * N + r], wl = W[(size_t)l * N + r];
+            y = zsub(y, zmul(vl, udot[l]));
+            y = zsub(y, zmul(wl, udot[NB + l]));
+            nx = zadd(nx, zmul(vl, uwn[l]));
+            nx = zadd(nx, zmul(wl, uvn[l]));
+        }
+        carry[r] = nx;
+        const C wr = zmul(tau, y);
+        col[r] = wr;
+        const C q = zcmul(wr, vv[r]);
+        pr += q.x; pi += q.y;
+    }
+    const R sr = block_sum_r(pr, red), si = block_sum_r(pi, red + WT / 64);
+    const C alpha = zscale(R(-0.5), zmul(tau, cmake<C, R>(sr, si)));
+    const C wnext = t0 < N ? zadd(col[t0], alpha) : czero<C>();     // W[j+1, i] (v_{j+1} = 1)
+    for (int r = t0 + tid; r < N; r += WT) {
+        const C wr = zadd(col[r], zmul(alpha, vv[r]));
+        W[(size_t)i * N + r] = wr;
+        carry[r] = zadd(carry[r], zadd(zmul(vv[r], zconj(wnext)), wr));
+    }
+    // WY factor: T[0:i, i] = -tau T[0:i, 0:i] (V^H v), T[i, i] = tau
+    if (tid < i) {
+        C acc = czero<C>();
+        for (int l = tid; l < i; ++l) acc = zadd(acc, zmul(Tp[(size_t)l * NB + tid], udot[NB + l]));
+        Tp[(size_t)i * NB + tid] = zmul(cmake<C, R>(-tau.x, -tau.y), acc);
+    }
+    if (tid == 0) Tp[(size_t)i * NB + i] = tau;
+}
+
+// V T^H of the panel (rows > i0), for the back-transform.
+template <class R>
+__global__ __launch_bounds__(256) void k_hw_vt(HetrdArgsT<R> h, int i0)
+{
+    typedef typename cplx<R>::type C;
+    const int N = h.n, kb = blockIdx.y;
+    const int pw = min(NB, N - 1 - i0);
+    const C *V = h.vp + (size_t)kb * N * 96;
+    const C *Tp = h.t + (size_t)kb * h.tstride + (size_t)(i0 / NB) * NB * NB;
+    C *VT = h.vt + (size_t)kb * h.stride + (size_t)i0 * N;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < N * NB; t += gridDim.x * blockDim.x) {
+        const int r = t % N, c = t / N;
+        C acc = czero<C>();
+        if (c < pw && r > i0)
+            for (int l = c; l < pw; ++l) acc = zadd(acc, zmul(V[(size_t)l * N + r], zconj(Tp[(size_t)l * NB + c])));
+        if (i0 + c < N) VT[(size_t)c * N + r] = acc;
+    }
+}
+
 // After the last panel's trailing update: d[n-1] and the empty last column.
 template <class R>
 __global__ void k_hetrd_finish(HetrdArgsT<R> h)
@@ -1707,6 +1937,28 @@ static void launch_hetrd_finish_t(const HetrdArgsT<R> &h, int nb, hipStream_t st
 {
     hipLaunchKernelGGL((k_hetrd_finish<R>), dim3(nb), dim3(256), 0, st, h);
 }
+template <class R>
+static void launch_hetrd_wide_column_t(const HetrdArgsT<R> &h, const WideArgsT<R> &w, int i0, int i, int nb,
+                                       hipStream_t st)
+{
+    const int m0 = h.n - i0 - 1;                      // trailing order at the panel's first column
+    const int T0 = (m0 + 63) / 64, ntile = T0 * (T0 + 1) / 2;
+    hipLaunchKernelGGL((k_hw_prep<R>), dim3(nb), dim3(WT), 0, st, h, w, i0, i);
+    hipLaunchKernelGGL((k_hw_hemv<R>), dim3(ntile + 1, nb), dim3(256), 0, st, h, w, i0, i, ntile);
+    hipLaunchKernelGGL((k_hw_fin<R>), dim3(nb), dim3(WT), 0, st, h, w, i0, i);
+}
+template <class R>
+static void launch_hetrd_wide_vt_t(const HetrdArgsT<R> &h, int i0, int nb, hipStream_t st)
+{
+    const int blocks = std::min((h.n * NB + 255) / 256, 64);
+    hipLaunchKernelGGL((k_hw_vt<R>), dim3(blocks, nb), dim3(256), 0, st, h, i0);
+}
+void launch_hetrd_wide_column(const HetrdArgs &h, const WideArgsT<double> &w, int i0, int i, int nb,
+                              hipStream_t st) { launch_hetrd_wide_column_t(h, w, i0, i, nb, st); }
+void launch_hetrd_wide_column(const HetrdArgsT<float> &h, const WideArgsT<float> &w, int i0, int i, int nb,
+                              hipStream_t st) { launch_hetrd_wide_column_t(h, w, i0, i, nb, st); }
+void launch_hetrd_wide_vt(const HetrdArgs &h, int i0, int nb, hipStream_t st) { launch_hetrd_wide_vt_t(h, i0, nb, st); }
+void launch_hetrd_wide_vt(const HetrdArgsT<float> &h, int i0, int nb, hipStream_t st) { launch_hetrd_wide_vt_t(h, i0, nb, st); }
 void launch_hetrd_panel(const HetrdArgs &h, int i0, int nb, hipStream_t st) { launch_hetrd_panel_t(h, i0, nb, st); }
 void launch_hetrd_finish(const HetrdArgs &h, int nb, hipStream_t st) { launch_hetrd_finish_t(h, nb, st); }
 void launch_hetrd_panel(const HetrdArgsT<float> &h, int i0, int nb, hipStream_t st) { launch_hetrd_panel_t(h, i0, nb, st); }

@@ -23,6 +23,9 @@ constexpr int kMaxN = 1280;    // LDS-resident vectors of k_hetrd / k_stein
 // outside a group are orthogonal to about 1e-10 (dstein's 1e-3 serves
 // inverse iteration, whose vectors depend on the separation much more).
 constexpr double kOrtolRel = 1e-6;
+// Tridiagonalisation over 64x64 tiles (several workgroups per item) for
+// batches below kWideMaxBatch items of order >= kWideMinN.
+constexpr int kWideMinN = 128, kWideMaxBatch = 160;
 
 #define HIP_TRY(x)                                   \
     do {                                             \
@@ -76,6 +79,13 @@ struct Workspace {
            *X32 = nullptr;
     double *lam = nullptr, *delta = nullptr;
     int *perm = nullptr;
+    // multi-workgroup tridiagonalisation (few items of large order)
+    bool wide = false;
+    double2 *wv = nullptr, *wcol = nullptr, *wcarry = nullptr, *wydot = nullptr, *wyax = nullptr,
+            *wudot = nullptr, *wscal = nullptr;
+    float2 *wv32 = nullptr, *wcol32 = nullptr, *wcarry32 = nullptr, *wydot32 = nullptr, *wyax32 = nullptr,
+           *wudot32 = nullptr, *wscal32 = nullptr;
+    int tmax() const { return (n - 1 + 63) / 64; }
     size_t stride() const { return (size_t)n * n; }
     // WY factors: one kNB x kNB block per panel, ceil(n / kNB) panels
     size_t tstride() const { return (size_t)((n + kNB - 1) / kNB) * kNB * kNB; }
@@ -104,8 +114,33 @@ struct Workspace {
         rc |= dalloc(&lam, (size_t)n * b); rc |= dalloc(&perm, (size_t)n * b); rc |= dalloc(&delta, (size_t)b);
         return rc ? TBK_ERR_NOMEM : TBK_OK;
     }
+    int alloc_wide(bool fp32)
+    {
+        wide = true;
+        const size_t bn = (size_t)n * b, bt = (size_t)tmax() * n * b;
+        int rc = TBK_OK;
+        if (!fp32) {
+            rc |= dalloc(&wv, bn); rc |= dalloc(&wcol, bn); rc |= dalloc(&wcarry, bn);
+            rc |= dalloc(&wydot, bt); rc |= dalloc(&wyax, bt);
+            rc |= dalloc(&wudot, (size_t)64 * b); rc |= dalloc(&wscal, (size_t)2 * b);
+        } else {
+            rc |= dalloc(&wv32, bn); rc |= dalloc(&wcol32, bn); rc |= dalloc(&wcarry32, bn); rc |= dalloc(&wydot32, bt);
+            rc |= dalloc(&wyax32, bt); rc |= dalloc(&wudot32, (size_t)64 * b); rc |= dalloc(&wscal32, (size_t)2 * b);
+        }
+        return rc ? TBK_ERR_NOMEM : TBK_OK;
+    }
+    tbk::WideArgsT<double> wide64() const
+    {
+        return tbk::WideArgsT<double>{tmax(), wv, wcol, wcarry, wydot, wyax, wudot, wscal};
+    }
+    tbk::WideArgsT<float> wide32() const
+    {
+        return tbk::WideArgsT<float>{tmax(), wv32, wcol32, wcarry32, wydot32, wyax32, wudot32, wscal32};
+    }
     void release()
     {
+        dfree(wv); dfree(wcol); dfree(wcarry); dfree(wydot); dfree(wyax); dfree(wudot); dfree(wscal);
+        dfree(wv32); dfree(wcol32); dfree(wcarry32); dfree(wydot32); dfree(wyax32); dfree(wudot32); dfree(wscal32);
         dfree(A); dfree(S); dfree(Y); dfree(Z); dfree(X); dfree(VT); dfree(Vp); dfree(Wp); dfree(Tf);
         dfree(px); dfree(py); dfree(d); dfree(e); dfree(w); dfree(Vt); dfree(scratch);
         dfree(info); dfree(sinfo);
@@ -180,7 +215,12 @@ void hetrd(Workspace &ws, int nb, hipStream_t st, StageTimer &tm)
     for (int i0 = 0; i0 < n - 1; i0 += kNB) {
         const int pw = std::min(kNB, n - 1 - i0);
         tm.begin("hetrd_panel");
-        tbk::launch_hetrd_panel(h, i0, nb, st);
+        if (ws.wide) {
+            for (int i = 0; i < pw; ++i) tbk::launch_hetrd_wide_column(h, ws.wide64(), i0, i, nb, st);
+            tbk::launch_hetrd_wide_vt(h, i0, nb, st);
+        } else {
+            tbk::launch_hetrd_panel(h, i0, nb, st);
+        }
         tm.end();
         const int s = i0 + pw, m = n - s;
         tm.begin("hetrd_update");
@@ -252,7 +292,12 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
     for (int i0 = 0; i0 < n - 1; i0 += kNB) {
         const int pw = std::min(kNB, n - 1 - i0);
         tm.begin("hetrd_panel");
-        tbk::launch_hetrd_panel(h, i0, nb, st);
+        if (ws.wide) {
+            for (int i = 0; i < pw; ++i) tbk::launch_hetrd_wide_column(h, ws.wide32(), i0, i, nb, st);
+            tbk::launch_hetrd_wide_vt(h, i0, nb, st);
+        } else {
+            tbk::launch_hetrd_panel(h, i0, nb, st);
+        }
         tm.end();
         const int s = i0 + pw, m = n - s;
         tm.begin("hetrd_update");
@@ -554,6 +599,9 @@ extern "C" int tbk_plan_create_group(int device, int nstruct, const tbk_structur
     if (!rc) rc = dalloc(&p->ph, (size_t)p->batch * maxpairs);
     if (!rc) rc = p->ws.alloc(N, p->batch);
     if (!rc && precision == TBK_MIXED) rc = p->ws.alloc_mixed();
+    // Few items of large order: one workgroup per item would leave most CUs
+    // idle in the tridiagonalisation, so it runs over tiles instead.
+    if (!rc && N >= kWideMinN && p->batch < kWideMaxBatch) rc = p->ws.alloc_wide(precision == TBK_MIXED);
     if (!rc && hipStreamSynchronize(p->stream) != hipSuccess) rc = TBK_ERR_HIP;   // uploads done
     if (rc) { plan_free(p); return rc == TBK_ERR_HIP ? TBK_ERR_HIP : TBK_ERR_NOMEM; }
     p->bytes = g_dev_bytes;

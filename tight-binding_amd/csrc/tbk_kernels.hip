@@ -762,7 +762,7 @@ __device__ __forceinline__ R block_sum_r(R v, R *red)
 }
 
 template <class R>
-__global__ __launch_bounds__(WT) void k_hw_prep(HetrdArgsT<R> h, WideArgsT<R> w, int i0, int i)
+__device__ void hw_prep_body(const HetrdArgsT<R> &h, const WideArgsT<R> &w, int i0, int i)
 {
     typedef typename cplx<R>::type C;
     __shared__ R red[WT / 64];
@@ -887,7 +887,7 @@ __global__ __launch_bounds__(256) void k_hw_hemv(HetrdArgsT<R> h, WideArgsT<R> w
 }
 
 template <class R>
-__global__ __launch_bounds__(WT) void k_hw_fin(HetrdArgsT<R> h, WideArgsT<R> w, int i0, int i)
+__device__ void hw_fin_body(const HetrdArgsT<R> &h, const WideArgsT<R> &w, int i0, int i)
 {
     typedef typename cplx<R>::type C;
     __shared__ C udot[2 * NB];
@@ -945,6 +945,23 @@ __global__ __launch_bounds__(WT) void k_hw_fin(HetrdArgsT<R> h, WideArgsT<R> w, 
         Tp[(size_t)i * NB + tid] = zmul(cmake<C, R>(-tau.x, -tau.y), acc);
     }
     if (tid == 0) Tp[(size_t)i * NB + i] = tau;
+}
+
+template <class R>
+__global__ __launch_bounds__(WT) void k_hw_prep(HetrdArgsT<R> h, WideArgsT<R> w, int i0, int i)
+{
+    hw_prep_body(h, w, i0, i);
+}
+// w, WY column and carry of column i, then the reflector of column i + 1
+// (same workgroup, one launch)
+template <class R>
+__global__ __launch_bounds__(WT) void k_hw_finprep(HetrdArgsT<R> h, WideArgsT<R> w, int i0, int i, int next)
+{
+    hw_fin_body(h, w, i0, i);
+    if (next) {
+        __syncthreads();
+        hw_prep_body(h, w, i0, i + 1);
+    }
 }
 
 // V T^H of the panel (rows > i0), for the back-transform.
@@ -1416,7 +1433,8 @@ __global__ void k_real_to_c32_t(int N, const double *Vt, float2 *Y, size_t strid
 //   delta    = 2 (||T - diag(lambda)||_F + max|lambda| ||R||_F),
 //   E_ij     = (T_ij + lambda_j R_ij) / (lambda_j - lambda_i)  if |lambda_i - lambda_j| > delta,
 //              R_ij / 2                                        otherwise,
-// and the refined vectors are Y (I + E).
+// and the refined vectors are Y (I + E).  T and G are Hermitian; only their
+// lower triangles are read.
 // k_oa_stats (one workgroup per k) forms lambda, delta and the ascending
 // order perm of lambda; k_oa_fmat writes F = I + E with its columns in that
 // order (F[:, j] = (I + E)[:, perm_j]), so that Y F is the refined, sorted
@@ -1451,14 +1469,15 @@ __global__ __launch_bounds__(1024) void k_oa_stats(int N, const int *nact, const
             gv[u] = tt < nn ? Gk[tt] : make_double2(1.0, 0.0);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 4; ++u) {                  // lower triangle (T, G Hermitian)
             const size_t tt = t + (size_t)u * blockDim.x;
             const int r = (int)(tt % N), c = (int)(tt / N);
+            if (tt >= nn || r < c) continue;
             double2 rv = make_double2(-gv[u].x, -gv[u].y);
-            if (tt < nn && r == c) { tv[u].x -= lam[r]; rv.x += 1.0; }
-            if (tt >= nn) rv.x += 1.0;
-            ft += tv[u].x * tv[u].x + tv[u].y * tv[u].y;
-            fr += rv.x * rv.x + rv.y * rv.y;
+            double wgt = 2.0;
+            if (r == c) { tv[u].x -= lam[r]; rv.x += 1.0; wgt = 1.0; }
+            ft += wgt * (tv[u].x * tv[u].x + tv[u].y * tv[u].y);
+            fr += wgt * (rv.x * rv.x + rv.y * rv.y);
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -1499,14 +1518,18 @@ __global__ __launch_bounds__(256) void k_oa_fmat(int N, const double2 *T, const 
     const int r = (int)(t % N), j = (int)(t / N);
     const int c = perm[(size_t)kb * N + j];
     const double *lam = lamv + (size_t)kb * N;
-    const size_t src = (size_t)kb * stride + (size_t)c * N + r;
-    const double2 gv = G[src];
+    // T and G are Hermitian and hold their lower triangles only
+    const bool low = r >= c;
+    const size_t src = (size_t)kb * stride + (low ? (size_t)c * N + r : (size_t)r * N + c);
+    double2 gv = G[src];
+    if (!low) gv.y = -gv.y;
     double2 rv = make_double2(-gv.x, -gv.y);
     if (r == c) rv.x += 1.0;
     double2 ev;
     const double gap = lam[c] - lam[r];
     if (fabs(gap) > delta[kb]) {
-        const double2 tv = T[src];
+        double2 tv = T[src];
+        if (!low) tv.y = -tv.y;
         ev = make_double2((tv.x + lam[c] * rv.x) / gap, (tv.y + lam[c] * rv.y) / gap);
     } else {
         ev = make_double2(0.5 * rv.x, 0.5 * rv.y);
@@ -1943,9 +1966,10 @@ static void launch_hetrd_wide_column_t(const HetrdArgsT<R> &h, const WideArgsT<R
 {
     const int m0 = h.n - i0 - 1;                      // trailing order at the panel's first column
     const int T0 = (m0 + 63) / 64, ntile = T0 * (T0 + 1) / 2;
-    hipLaunchKernelGGL((k_hw_prep<R>), dim3(nb), dim3(WT), 0, st, h, w, i0, i);
+    const int pw = std::min(NB, h.n - 1 - i0);
+    if (i == 0) hipLaunchKernelGGL((k_hw_prep<R>), dim3(nb), dim3(WT), 0, st, h, w, i0, i);
     hipLaunchKernelGGL((k_hw_hemv<R>), dim3(ntile + 1, nb), dim3(256), 0, st, h, w, i0, i, ntile);
-    hipLaunchKernelGGL((k_hw_fin<R>), dim3(nb), dim3(WT), 0, st, h, w, i0, i);
+    hipLaunchKernelGGL((k_hw_finprep<R>), dim3(nb), dim3(WT), 0, st, h, w, i0, i, i + 1 < pw ? 1 : 0);
 }
 template <class R>
 static void launch_hetrd_wide_vt_t(const HetrdArgsT<R> &h, int i0, int nb, hipStream_t st)

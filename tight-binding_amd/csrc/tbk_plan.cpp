@@ -329,10 +329,10 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
     tbk::launch_c32_to_c64(ss * nb, ws.Y32, ws.Y, st);
     tm.end();
     tm.begin("refine");
-    // X = C Y, Z = Y^H X (= Y^H C Y), VT = Y^H Y
+    // X = C Y, Z = Y^H X (= Y^H C Y), VT = Y^H Y (Hermitian: lower triangles)
     tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.Y, n, ss, ws.X, n, ss), 0, 0, nb, st);
-    tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.Z, n, ss), 1, 0, nb, st);
-    tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.Y, n, ss, ws.VT, n, ss), 1, 0, nb, st);
+    tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.Z, n, ss, 4), 1, 0, nb, st);   // lower
+    tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.Y, n, ss, ws.VT, n, ss, 4), 1, 0, nb, st);  // lower
     // X = (I + E) with sorted columns; VT = Y X; Z = U^{-1} VT
     tbk::launch_oa_update(n, nact, ws.Z, ws.VT, ws.X, ss, ws.lam, ws.perm, ws.delta, w_out, nb, st);
     tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.VT, n, ss), 0, 0, nb, st);

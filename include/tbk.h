@@ -297,6 +297,13 @@ int tbk_debug_stages(int device, int n, const double *h, const double *s,
  * the calibration point for hardware traffic counters. */
 int tbk_debug_copy(int device, long long bytes);
 
+/* Average time (ms) of reps launches of the batched GEMM over nb items of
+ * shape m x n x k (zero operands): the microbenchmark of the tile
+ * autotuner.  tile -1: the tuned / modelled choice. */
+int tbk_debug_gemm_time(int device, int precision, int tile, int m, int n,
+                        int k, int opa, int opb, int tri, int nb, int reps,
+                        double *ms);
+
 /* C = op(A) op(B) of column-major complex matrices (complex128, or
  * complex64 when precision is TBK_MIXED) with the library's batched GEMM;
  * tile -1 = the tuned / modelled choice, else 0..3 (see tbk_gemm_tile). */

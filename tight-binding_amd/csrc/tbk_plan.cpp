@@ -1141,3 +1141,48 @@ extern "C" int tbk_debug_copy(int device, long long bytes)
     dfree(a); dfree(b);
     return rc;
 }
+
+// Average time of `reps` launches of the batched GEMM (nb items of m x n x k,
+// operands as tbk_debug_zgemm; contents zero) with tile `tile` (-1: tuned or
+// modelled choice): the microbenchmark behind the tile autotuner.
+extern "C" int tbk_debug_gemm_time(int device, int precision, int tile, int m, int n, int k, int opa, int opb,
+                                   int tri, int nb, int reps, double *ms)
+{
+    if (m <= 0 || n <= 0 || k <= 0 || nb <= 0 || reps <= 0 || !ms || tile >= tbk::kGemmTiles) return TBK_ERR_ARG;
+    HIP_TRY(hipSetDevice(device));
+    const size_t lda = opa ? k : m, ldb = opb ? n : k;
+    const size_t sa = lda * (opa ? m : k), sb = ldb * (opb ? k : n), sc = (size_t)m * n;
+    const size_t esz = precision == TBK_MIXED ? sizeof(float2) : sizeof(double2);
+    char *a = nullptr, *b = nullptr, *c = nullptr;
+    int rc = dalloc(&a, sa * nb * esz) | dalloc(&b, sb * nb * esz) | dalloc(&c, sc * nb * esz);
+    if (rc) { dfree(a); dfree(b); dfree(c); return TBK_ERR_NOMEM; }
+    (void)hipMemset(a, 0, sa * nb * esz);
+    (void)hipMemset(b, 0, sb * nb * esz);
+    tbk::GemmShape sh{precision == TBK_MIXED ? 1 : 0, opa, opb, m, n, k, tri, nb, {}, {}};
+    if (precision == TBK_MIXED) {
+        sh.g32.m = m; sh.g32.n = n; sh.g32.k = k; sh.g32.tri = tri;
+        sh.g32.alpha = float2{1.f, 0.f}; sh.g32.beta = float2{0.f, 0.f};
+        sh.g32.a = (const float2 *)a; sh.g32.lda = (int)lda; sh.g32.sa = sa;
+        sh.g32.b = (const float2 *)b; sh.g32.ldb = (int)ldb; sh.g32.sb = sb;
+        sh.g32.c = (float2 *)c; sh.g32.ldc = m; sh.g32.sc = sc;
+    } else {
+        sh.g64 = gemm(m, n, k, (const double2 *)a, (int)lda, sa, (const double2 *)b, (int)ldb, sb, (double2 *)c, m,
+                      sc, tri);
+    }
+    const int t = tile >= 0 ? tile : tbk::gemm_tile(sh.prec, opa, opb, m, n, k, tri, nb);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    tbk::launch_gemm_tile(sh, t, nullptr);
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r) tbk::launch_gemm_tile(sh, t, nullptr);
+    (void)hipEventRecord(e1, nullptr);
+    float f = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess) rc = TBK_ERR_HIP;
+    (void)hipEventElapsedTime(&f, e0, e1);
+    *ms = f / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dfree(a); dfree(b); dfree(c);
+    return rc;
+}

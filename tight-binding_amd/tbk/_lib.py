@@ -33,6 +33,7 @@ EXPORTS = [
     "tbk_server_stats", "tbk_server_destroy",
     "tbk_zhegv", "tbk_fermi",
     "tbk_error_string", "tbk_debug_stages", "tbk_debug_zgemm", "tbk_debug_copy",
+    "tbk_debug_gemm_time",
 ]
 
 
@@ -108,6 +109,7 @@ def lib():
         L.tbk_server_stats.argtypes = [vp, C.POINTER(C.c_longlong)]
         L.tbk_server_destroy.argtypes = [vp]
         L.tbk_debug_copy.argtypes = [C.c_int, C.c_longlong]
+        L.tbk_debug_gemm_time.argtypes = [C.c_int] * 11 + [dp]
         L.tbk_plan_autotune.argtypes = [vp, C.c_int, ip]
         L.tbk_gemm_tile.argtypes = [C.c_int] * 8
         L.tbk_plan_create.argtypes = [C.c_int, C.POINTER(StructureDesc), C.c_int, d, d, C.c_int,

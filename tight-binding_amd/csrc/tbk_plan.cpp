@@ -270,6 +270,64 @@ tbk::GemmArgsT<float> cgemm(int m, int n, int k, const float2 *a, int lda, size_
     return g;
 }
 
+template <class R>
+tbk::GemmArgsT<R> gemm_t(int m, int n, int k, const typename tbk::cplx_t<R>::type *a, int lda, size_t sa,
+                         const typename tbk::cplx_t<R>::type *b, int ldb, size_t sb, typename tbk::cplx_t<R>::type *c,
+                         int ldc, size_t sc, R alpha_re = 1, R beta_re = 0)
+{
+    tbk::GemmArgsT<R> g;
+    g.m = m; g.n = n; g.k = k; g.tri = 0;
+    g.alpha.x = alpha_re; g.alpha.y = 0; g.beta.x = beta_re; g.beta.y = 0;
+    g.a = a; g.lda = lda; g.sa = sa; g.b = b; g.ldb = ldb; g.sb = sb; g.c = c; g.ldc = ldc; g.sc = sc;
+    return g;
+}
+inline void launch_gemm_any(const tbk::GemmArgsT<double> &g, int opa, int opb, int nb, hipStream_t st)
+{
+    tbk::launch_zgemm(g, opa, opb, nb, st);
+}
+inline void launch_gemm_any(const tbk::GemmArgsT<float> &g, int opa, int opb, int nb, hipStream_t st)
+{
+    tbk::launch_cgemm(g, opa, opb, nb, st);
+}
+
+// Panels of the Householder reduction applied to Y (Y <- Q Y, Q = Q_0 Q_1
+// ... with Q_p = I - V_p T_p V_p^H), kBT panels at a time.  Within a group
+// the panels' V T^H blocks are first merged from the last panel down,
+//   VT_p <- VT_p - VT_rest (V_rest^H VT_p),
+// which turns [VT_p | VT_rest] into V_c T_c^H of the group's combined WY
+// factor; each group is then two GEMMs of inner dimension up to 32 kBT.
+// A holds the reflectors (column i0.. of panel i0, zero above the
+// subdiagonal), VT the V T^H blocks at the same columns; M is scratch of
+// 96 x 32 per item (stride sm).
+constexpr int kBT = 3;
+template <class R>
+void apply_q(int n, int nb, typename tbk::cplx_t<R>::type *A, typename tbk::cplx_t<R>::type *VT,
+             typename tbk::cplx_t<R>::type *Y, typename tbk::cplx_t<R>::type *X, typename tbk::cplx_t<R>::type *M,
+             size_t sm, hipStream_t st)
+{
+    const size_t ss = (size_t)n * n;
+    const int P = (n - 2) / kNB + 1;                  // panels
+    for (int g0 = ((P - 1) / kBT) * kBT; g0 >= 0; g0 -= kBT) {
+        const int g1 = std::min(g0 + kBT, P);          // panels g0 .. g1-1
+        const int c0 = g0 * kNB;
+        const int wlast = std::min(kNB, n - 1 - (g1 - 1) * kNB);
+        const int W = (g1 - 1 - g0) * kNB + wlast;     // columns of the group
+        for (int p = g1 - 2; p >= g0; --p) {            // merge V T^H blocks
+            const int cp = p * kNB, cr = cp + kNB, wr = c0 + W - cr;
+            const int rr0 = cr + 1, k = n - rr0;         // rows where V_rest can be non-zero
+            launch_gemm_any(gemm_t<R>(wr, kNB, k, A + (size_t)cr * n + rr0, n, ss, VT + (size_t)cp * n + rr0, n,
+                                      ss, M, wr, sm), 1, 0, nb, st);
+            launch_gemm_any(gemm_t<R>(k, kNB, wr, VT + (size_t)cr * n + rr0, n, ss, M, wr, sm,
+                                      VT + (size_t)cp * n + rr0, n, ss, R(-1), R(1)), 0, 0, nb, st);
+        }
+        const int r0 = c0 + 1, m = n - r0;
+        // X[0:W, :] = (V_c T_c^H)^H Y[r0:, :];  Y[r0:, :] -= V_c X
+        launch_gemm_any(gemm_t<R>(W, n, m, VT + (size_t)c0 * n + r0, n, ss, Y + r0, n, ss, X, n, ss), 1, 0, nb, st);
+        launch_gemm_any(gemm_t<R>(m, n, W, A + (size_t)c0 * n + r0, n, ss, X, n, ss, Y + r0, n, ss, R(-1), R(1)),
+                        0, 0, nb, st);
+    }
+}
+
 // Mixed-precision solve: FP64 Cholesky, inverse and reduction; FP32 panel
 // tridiagonalisation and back-transform on the FP32 MFMA units; FP64
 // bisection/eigenvectors of the tridiagonal; one FP64 Ogita-Aishima step on
@@ -316,16 +374,7 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
     tm.end();
     tm.begin("backtransform");
     tbk::launch_real_to_c32_t(n, ws.Vt, ws.Y32, ss, nb, st);
-    for (int i0 = ((n - 2) / kNB) * kNB; i0 >= 0; i0 -= kNB) {
-        const int pw = std::min(kNB, n - 1 - i0);
-        if (pw <= 0) continue;
-        const int r0 = i0 + 1, m = n - r0;
-        const float2 *Vsub = ws.A32 + (size_t)i0 * n + r0;
-        const float2 *VTsub = ws.VT32 + (size_t)i0 * n + r0;
-        tbk::launch_cgemm(cgemm(pw, n, m, VTsub, n, ss, ws.Y32 + r0, n, ss, ws.X32, n, ss), 1, 0, nb, st);
-        tbk::launch_cgemm(cgemm(m, n, pw, Vsub, n, ss, ws.X32, n, ss, ws.Y32 + r0, n, ss, 0,
-                                float2{-1.f, 0.f}, float2{1.f, 0.f}), 0, 0, nb, st);
-    }
+    apply_q<float>(n, nb, ws.A32, ws.VT32, ws.Y32, ws.X32, ws.P32, sp, st);
     tbk::launch_c32_to_c64(ss * nb, ws.Y32, ws.Y, st);
     tm.end();
     tm.begin("refine");
@@ -361,18 +410,7 @@ void solve_batch(Workspace &ws, int nb, const int *nact, double *w_out, hipStrea
     tm.end();
     tm.begin("backtransform");
     tbk::launch_real_to_complex_t(n, ws.Vt, ws.Y, ss, nb, st);
-    for (int i0 = ((n - 2) / kNB) * kNB; i0 >= 0; i0 -= kNB) {
-        const int pw = std::min(kNB, n - 1 - i0);
-        if (pw <= 0) continue;
-        const int r0 = i0 + 1, m = n - r0;
-        const double2 *Vsub = ws.A + (size_t)i0 * n + r0;
-        const double2 *VTsub = ws.VT + (size_t)i0 * n + r0;
-        // X[0:pw, :] = (V_p T_p^H)^H Y[r0:, :]
-        tbk::launch_zgemm(gemm(pw, n, m, VTsub, n, ss, ws.Y + r0, n, ss, ws.X, n, ss), 1, 0, nb, st);
-        // Y[r0:, :] -= V_p X[0:pw, :]
-        tbk::launch_zgemm(gemm(m, n, pw, Vsub, n, ss, ws.X, n, ss, ws.Y + r0, n, ss, 0,
-                               double2{-1.0, 0.0}, double2{1.0, 0.0}), 0, 0, nb, st);
-    }
+    apply_q<double>(n, nb, ws.A, ws.VT, ws.Y, ws.X, ws.Vp, (size_t)n * 96, st);
     // Z = Uinv Y
     tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.Z, n, ss, 1), 0, 0, nb, st);
     tm.end();

@@ -264,15 +264,23 @@ __global__ __launch_bounds__(256) void k_potrf_panel(int N, double2 *S, size_t s
             A[(size_t)(j0 + c) * N + r] = zzero();
         }
     }
-    for (int c = j0 + w + tid; c < N; c += blockDim.x) {
-        double2 x[NB];
-        for (int i = 0; i < w; ++i) x[i] = A[(size_t)c * N + j0 + i];
+    // U12 = U11^{-H} S12: one half-wave per column, lane i holding row i;
+    // forward substitution with the finished x_i broadcast by shuffle
+    const int lane = tid & 63, li = lane & 31, hw = tid >> 5, nhw = blockDim.x >> 5;
+    const int m = N - j0 - w;
+    for (int it = 0; it * nhw < m; ++it) {
+        const int c = j0 + w + it * nhw + hw;
+        const bool ok = c < N && li < w;
+        double2 b = ok ? A[(size_t)c * N + j0 + li] : zzero();
         for (int i = 0; i < w; ++i) {
-            double2 v = x[i];
-            for (int l = 0; l < i; ++l) v = zsub(v, zcmul(D[l][i], x[l]));
-            x[i] = zscale(1.0 / D[i][i].x, v);
+            const double2 xl = zscale(1.0 / D[i][i].x, b);
+            double2 xi;
+            xi.x = __shfl(xl.x, (lane & 32) + i, 64);
+            xi.y = __shfl(xl.y, (lane & 32) + i, 64);
+            if (li == i) b = xi;
+            else if (li > i) b = zsub(b, zcmul(D[i][li], xi));
         }
-        for (int i = 0; i < w; ++i) A[(size_t)c * N + j0 + i] = x[i];
+        if (ok) A[(size_t)c * N + j0 + li] = b;
     }
 }
 
@@ -293,14 +301,24 @@ __global__ __launch_bounds__(256) void k_trtri_panel(int N, double2 *S, size_t s
         D[r][c] = (r <= c) ? A[(size_t)(j0 + c) * N + j0 + r] : zzero();
     }
     __syncthreads();
-    for (int i = tid; i < j0; i += blockDim.x) {
-        double2 x[NB];
-        for (int c = 0; c < w; ++c) {
-            double2 v = zscale(-1.0, X[(size_t)c * N + i]);
-            for (int l = 0; l < c; ++l) v = zsub(v, zmul(x[l], D[l][c]));
-            x[c] = zscale(1.0 / D[c][c].x, v);
+    // Uinv[i, J] = -X[i, :] U_JJ^{-1}: one half-wave per row i, lane c
+    // holding column c; substitution over c with the finished x_c broadcast
+    {
+        const int lane = tid & 63, lc = lane & 31, hw = tid >> 5, nhw = blockDim.x >> 5;
+        for (int it = 0; it * nhw < j0; ++it) {
+            const int i = it * nhw + hw;
+            const bool ok = i < j0 && lc < w;
+            double2 v = ok ? zscale(-1.0, X[(size_t)lc * N + i]) : zzero();
+            for (int c = 0; c < w; ++c) {
+                const double2 xl = zscale(1.0 / D[c][c].x, v);
+                double2 xc;
+                xc.x = __shfl(xl.x, (lane & 32) + c, 64);
+                xc.y = __shfl(xl.y, (lane & 32) + c, 64);
+                if (lc == c) v = xc;
+                else if (lc > c) v = zsub(v, zmul(xc, D[c][lc]));
+            }
+            if (ok) A[(size_t)(j0 + lc) * N + i] = v;
         }
-        for (int c = 0; c < w; ++c) A[(size_t)(j0 + c) * N + i] = x[c];
     }
     __syncthreads();
     lds_trtri(D, w);
@@ -1111,7 +1129,10 @@ __global__ __launch_bounds__(256) void k_stebz(SteinArgs s)
 // orthogonalised afterwards by k_stein_orth.
 __global__ __launch_bounds__(64) void k_stein_vec(SteinArgs s)
 {
-    __shared__ double d[1280], e[1280], e2[1280], w[1280];
+    // d, e, e2, w of the active order in dynamic LDS (4 s.n doubles): small
+    // enough that many 64-thread workgroups share a CU
+    extern __shared__ double stein_smem[];
+    double *d = stein_smem, *e = d + s.n, *e2 = e + s.n, *w = e2 + s.n;
     __shared__ double red4[16];
     const int LD = s.n;                               // stride; order of the padded batch
     const int kb = blockIdx.y;
@@ -1991,7 +2012,7 @@ int stein_threads(int n) { return (n + 63) / 64 * 64; }
 void launch_stein(const SteinArgs &s, int nb, hipStream_t st)
 {
     hipLaunchKernelGGL(k_stebz, dim3((s.n + 255) / 256, nb), dim3(256), 0, st, s);
-    hipLaunchKernelGGL(k_stein_vec, dim3((s.n + 63) / 64, nb), dim3(64), 0, st, s);
+    hipLaunchKernelGGL(k_stein_vec, dim3((s.n + 63) / 64, nb), dim3(64), 4 * (size_t)s.n * sizeof(double), st, s);
     hipLaunchKernelGGL(k_stein_orth, dim3(nb), dim3(256), 0, st, s);
 }
 void launch_real_to_complex_t(int N, const double *Vt, double2 *Y, size_t stride, int nb,

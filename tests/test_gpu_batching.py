@@ -123,3 +123,54 @@ def test_multi_uneven_subset_vs_oracle(c4, prec):
         assert worst["ef"] < 1e-6 and worst["e"] < 1e-6 and worst["eig"] < 1e-8
         assert worst["p"] < 1e-3
     m.close()
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3])
+def test_group_other_modes_match_alone(c4, mode):
+    """Modes 1-3 (E(0)/P(0) forms, energy-only) through a group plan."""
+    sel = pick(c4, [18, 36, 54])
+    ks = [S.monkhorst_pack((3, 3, 3)) for _ in sel]
+    ws = [np.full(len(k), 1.0 / len(k)) for k in ks]
+    g = GroupPlan([st for st, _ in sel], ks, ws)
+    res = g.run(mode)
+    g.close()
+    for i, (st, _) in enumerate(sel):
+        ref = oracle.Setup(st).bande(ks[i], ws[i], mode=mode, nthreads=16)
+        assert abs(res[i]["efermi"] - ref["efermi"]) < EF_TOL
+        assert abs(res[i]["earg1"] - ref["earg1"]) / st.natoms < E_TOL
+        if mode == 2:
+            assert abs(res[i]["earg2"] - ref["earg2"]) * S.RYBOHR3_KBAR < P_TOL_KBAR
+        else:
+            assert abs(res[i]["earg2"] - ref["earg2"]) / st.natoms < E_TOL
+
+
+def test_graph_replay_mixed_precision():
+    st = S.config_c2()
+    k = S.monkhorst_pack((8, 8, 8))[:64]
+    w = np.full(len(k), 1.0 / len(k))
+    p = Plan(st, k, w, precision=_lib.TBK_MIXED)
+    ref = p.run(4)
+    e_ref, _ = p.eigenvalues()
+    p.set_graph(True)
+    for _ in range(3):
+        assert p.run(4) == ref
+        assert np.array_equal(p.eigenvalues()[0], e_ref)
+    p.close()
+
+
+def test_multi_with_large_orders_uses_tiles(c4):
+    """Groups of order >= 128 with few items take the tile tridiagonalisation;
+    results against the oracle on a 2^3 mesh."""
+    sel = [(st, q) for st, q in c4 if 140 <= st.nv <= 220][:4]
+    ks = [S.monkhorst_pack((2, 2, 2)) for _ in sel]
+    ws = [np.full(len(k), 1.0 / len(k)) for k in ks]
+    m = MultiPlan([st for st, _ in sel], ks, ws)
+    res = m.run(4)
+    for i, (st, _) in enumerate(sel):
+        ref = oracle.Setup(st).bande(ks[i], ws[i], mode=4, nthreads=16, want_eig=True)
+        e, _ = m.eigenvalues(i)
+        print(f"  N={st.nv}: max|de|={float(np.abs(e - ref['eig']).max()):.1e}", flush=True)
+        assert float(np.abs(e - ref["eig"]).max()) < EIG_TOL
+        assert abs(res[i]["efermi"] - ref["efermi"]) < EF_TOL
+        assert abs(res[i]["earg2"] - ref["earg2"]) * S.RYBOHR3_KBAR < P_TOL_KBAR
+    m.close()

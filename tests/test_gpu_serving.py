@@ -58,3 +58,24 @@ def test_server_failing_request_does_not_affect_others():
     a = p.run(4)
     p.close()
     assert abs(rg["efermi"] - a["efermi"]) < 1e-12
+
+
+def test_server_fp64_matches_oracle_and_reuses_nothing_stale():
+    """FP64 requests of repeated and different structures, submitted in two
+    waves through one server: every result equals the oracle's."""
+    c4 = S.config_c4(count=32, max_atoms=8, max_q=3)
+    reqs = [(st, S.monkhorst_pack((2, 2, 2))) for st, _ in c4[:6]] * 2
+    srv = Server(precision=_lib.TBK_FP64, budget_bytes=2 << 30, max_inflight=2)
+    ids = [srv.submit(st, k, np.full(len(k), 1.0 / len(k))) for st, k in reqs[:6]]
+    res = [srv.result(r) for r in ids]
+    ids2 = [srv.submit(st, k, np.full(len(k), 1.0 / len(k))) for st, k in reqs[6:]]
+    res += [srv.result(r) for r in ids2]
+    srv.close()
+    for (st, k), r in zip(reqs, res):
+        assert r["status"] == 0
+        ref = oracle.Setup(st).bande(k, np.full(len(k), 1.0 / len(k)), mode=4, nthreads=16)
+        assert abs(r["efermi"] - ref["efermi"]) < 1e-9
+        assert abs(r["earg1"] - ref["earg1"]) / st.natoms < 1e-9
+        assert abs(r["earg2"] - ref["earg2"]) * S.RYBOHR3_KBAR < 1e-6
+    for a, b in zip(res[:6], res[6:]):
+        assert a["efermi"] == b["efermi"] and a["earg2"] == b["earg2"]

@@ -164,12 +164,14 @@ def serve_stream(rates, nreq, precision, seed=20261203):
 
 
 def profile_pass(precision):
-    """One warm-up and one batched C2 step, then a 1 GiB elementwise copy
-    kernel (read 1 GiB, write 1 GiB) to calibrate the traffic counters."""
+    """The GEMM autotuning pass, one warm-up and one batched C2 step, then a
+    1 GiB copy kernel (read 1 GiB, write 1 GiB) to calibrate the traffic
+    counters."""
     st = S.config_c2()
     kpts = S.monkhorst_pack((8, 8, 8))
     w = np.full(len(kpts), 1.0 / len(kpts))
     plan = Plan(st, kpts, w, batch=0, precision=TBK_MIXED if precision == "mixed" else TBK_FP64)
+    plan.autotune(3)                  # the same GEMM tiles as the timed bench
     plan.run(4, 0.005, 10.0)
     r = plan.run(4, 0.005, 10.0)
     plan.close()

@@ -106,73 +106,100 @@ __global__ __launch_bounds__(256) void k_phases(const StructDev *sd, const int *
     }
 }
 
-// One workgroup per (column atom ja, item); one thread per element of the
-// nc(ja) columns.  Elements of blocks (ia <= ja) are the Bloch sum over
+// One workgroup per (upper block (ia, ja), ia <= ja; chunk of kBuildItems
+// items).  Each pass takes three items, one thread per element (o, b) of the
+// 9x9 block with o, the row orbital, fastest, so that groups of lanes write
+// nine consecutive rows of one column.  The element is the Bloch sum over
 // the pairs (ia, ja) of the reference's pair list, in list order, of
-// e(a,b) * phase (rotate.f:260-326); elements of blocks ia > ja (and below
-// the diagonal of ia = ja) are the conjugates of their upper partners.  The
-// accumulation order and the absence of FMA contraction reproduce the
-// reference's rounding except for the ulp of sin/cos.  The workgroup with
-// ja = natoms writes the padding columns nv..n-1.
-__global__ __launch_bounds__(256) void k_build_hs(BuildArgs a)
+// e(o, b) * phase (rotate.f:260-326); the lower block (ja, ia) -- and the
+// lower half of a diagonal block -- is its conjugate transpose, passed
+// through LDS and written with the column orbital fastest.  The accumulation
+// order and the absence of FMA contraction reproduce the reference's
+// rounding except for the ulp of sin/cos.  The SK blocks of one atom pair
+// are shared by every item of the chunk and stay in L1/L2.  The last
+// kBuildPadWgs workgroup columns write the padding rows and columns nv..n-1.
+constexpr int kBuildItems = 24, kBuildPadWgs = 8;
+
+__global__ __launch_bounds__(256) void k_build_hs(BuildArgs a, int nb)
 {
 #pragma clang fp contract(off)
-    const int ja = blockIdx.x;
-    const int kb = blockIdx.y;
-    const StructDev &SD = a.sd[a.item_s[kb]];
-    const int N = a.n, nat = SD.natoms, nv = SD.nv;
-    if (ja > nat) return;
-    double2 *H = a.h + (size_t)kb * a.stride;
-    double2 *S = a.s + (size_t)kb * a.stride;
-    if (ja == nat) {
+    __shared__ double2 hb[3][81], sb[3][81];
+    const int tid = threadIdx.x, N = a.n;
+    const int it0 = blockIdx.y * kBuildItems, it1 = min(nb, it0 + kBuildItems);
+    const int nup = (int)gridDim.x - kBuildPadWgs;
+    if ((int)blockIdx.x >= nup) {
         const double hp = a.deriv ? 0.0 : a.pad, sp = a.deriv ? 0.0 : 1.0;
-        for (int t = threadIdx.x; t < N * (N - nv); t += blockDim.x) {
-            const int r = t % N, c = nv + t / N;
-            H[(size_t)c * N + r] = make_double2(r == c ? hp : 0.0, 0.0);
-            S[(size_t)c * N + r] = make_double2(r == c ? sp : 0.0, 0.0);
+        const int t0 = tid + 256 * ((int)blockIdx.x - nup), ts = 256 * kBuildPadWgs;
+        for (int kb = it0; kb < it1; ++kb) {
+            const int nv = a.sd[a.item_s[kb]].nv;
+            if (nv >= N) continue;
+            double2 *H = a.h + (size_t)kb * a.stride, *S = a.s + (size_t)kb * a.stride;
+            for (int t = t0; t < N * (N - nv); t += ts) {              // columns nv..N-1
+                const int r = t % N, c = nv + t / N;
+                H[(size_t)c * N + r] = make_double2(r == c ? hp : 0.0, 0.0);
+                S[(size_t)c * N + r] = make_double2(r == c ? sp : 0.0, 0.0);
+            }
+            const int np = N - nv;
+            for (int t = t0; t < np * nv; t += ts) {                   // rows nv..N-1 of columns < nv
+                const int r = nv + t % np, c = t / np;
+                H[(size_t)c * N + r] = make_double2(0.0, 0.0);
+                S[(size_t)c * N + r] = make_double2(0.0, 0.0);
+            }
         }
         return;
     }
-    const int c0 = SD.ksk[ja], nc = SD.nbas[ja];
-    const double2 *ph = a.ph + (size_t)kb * a.maxpairs;
-    const double *eh = a.deriv ? SD.edh : SD.eh, *es = a.deriv ? SD.eds : SD.es;
-    const double *onsite = a.deriv ? SD.dparos : SD.paros;
-    for (int t = threadIdx.x; t < N * nc; t += blockDim.x) {
-        const int r = t % N, b = t / N;
-        if (r >= nv) {
-            H[(size_t)(c0 + b) * N + r] = make_double2(0.0, 0.0);
-            S[(size_t)(c0 + b) * N + r] = make_double2(0.0, 0.0);
-            continue;
+    const int blk = blockIdx.x;
+    int ja = (int)((sqrt(8.0 * blk + 1.0) - 1.0) * 0.5);
+    while ((ja + 1) * (ja + 2) / 2 <= blk) ++ja;
+    while (ja * (ja + 1) / 2 > blk) --ja;
+    const int ia = blk - ja * (ja + 1) / 2;
+    const int il = tid / 81, e = tid - 81 * il;          // il < 3 for the 243 working threads
+    const int o = e % 9, b = e / 9;
+    for (int k0 = it0; k0 < it1; k0 += 3) {
+        const int kb = k0 + il;
+        const bool act = il < 3 && kb < it1;
+        const StructDev *SD = act ? &a.sd[a.item_s[kb]] : nullptr;
+        const bool inb = act && ja < SD->natoms;
+        const int r0 = inb ? SD->ksk[ia] : 0, c0 = inb ? SD->ksk[ja] : 0;
+        const int ni = inb ? SD->nbas[ia] : 0, nj = inb ? SD->nbas[ja] : 0;
+        double2 hv = make_double2(0.0, 0.0), sv = hv;
+        if (inb && o < ni && b < nj && (ia < ja || b >= o)) {
+            const int p0 = SD->blk_ptr[ia * SD->natoms + ja], p1 = SD->blk_ptr[ia * SD->natoms + ja + 1];
+            const double2 *ph = a.ph + (size_t)kb * a.maxpairs;
+            const double *eh = a.deriv ? SD->edh : SD->eh, *es = a.deriv ? SD->eds : SD->es;
+            const int idx = 9 * o + b;
+            double hr = 0.0, hi = 0.0, sr = 0.0, si = 0.0;
+            for (int p = p0; p < p1; ++p) {
+                const double2 z = ph[p];
+                const double e1 = eh[81 * (size_t)p + idx], e2 = es[81 * (size_t)p + idx];
+                hr = hr + e1 * z.x; hi = hi + e1 * z.y;
+                sr = sr + e2 * z.x; si = si + e2 * z.y;
+            }
+            if (ia == ja && b == o) {        // diagonal: real, plus on-site terms
+                const double *onsite = a.deriv ? SD->dparos : SD->paros;
+                const int cls = o == 0 ? 0 : (o <= 3 ? 1 : (o <= 6 ? 2 : 3));
+                hv = make_double2(hr + onsite[4 * ia + cls], 0.0);
+                sv = make_double2(sr + (a.deriv ? 0.0 : 1.0), 0.0);
+            } else {
+                hv = make_double2(hr, hi); sv = make_double2(sr, si);
+            }
+            double2 *H = a.h + (size_t)kb * a.stride, *S = a.s + (size_t)kb * a.stride;
+            H[(size_t)(c0 + b) * N + r0 + o] = hv;
+            S[(size_t)(c0 + b) * N + r0 + o] = sv;
         }
-        const int ia = SD.row_atom[r], o = SD.row_orb[r];
-        const bool upper = ia < ja || (ia == ja && b >= o);
-        int p0, p1, idx;
-        if (upper) {   // e(o, b) over pairs (ia, ja)
-            p0 = SD.blk_ptr[ia * nat + ja]; p1 = SD.blk_ptr[ia * nat + ja + 1];
-            idx = 9 * o + b;
-        } else {       // conj of e(b, o) over pairs (ja, ia)
-            p0 = SD.blk_ptr[ja * nat + ia]; p1 = SD.blk_ptr[ja * nat + ia + 1];
-            idx = 9 * b + o;
+        if (il < 3) { hb[il][e] = hv; sb[il][e] = sv; }
+        __syncthreads();
+        // mirror: element (row c0 + bb, column r0 + oo) = conj(element (oo, bb))
+        {
+            const int bb = e % 9, oo = e / 9;
+            if (inb && oo < ni && bb < nj && (ia < ja || bb > oo)) {
+                const double2 h2 = hb[il][9 * bb + oo], s2 = sb[il][9 * bb + oo];
+                double2 *H = a.h + (size_t)kb * a.stride, *S = a.s + (size_t)kb * a.stride;
+                H[(size_t)(r0 + oo) * N + c0 + bb] = make_double2(h2.x, -h2.y);
+                S[(size_t)(r0 + oo) * N + c0 + bb] = make_double2(s2.x, -s2.y);
+            }
         }
-        double hr = 0.0, hi = 0.0, sr = 0.0, si = 0.0;
-        for (int p = p0; p < p1; ++p) {
-            const double2 z = ph[p];
-            const double e1 = eh[81 * (size_t)p + idx], e2 = es[81 * (size_t)p + idx];
-            hr = hr + e1 * z.x; hi = hi + e1 * z.y;
-            sr = sr + e2 * z.x; si = si + e2 * z.y;
-        }
-        double2 hv, sv;
-        if (ia == ja && b == o) {            // diagonal: real, plus on-site terms
-            const int cls = o == 0 ? 0 : (o <= 3 ? 1 : (o <= 6 ? 2 : 3));
-            hv = make_double2(hr + onsite[4 * ia + cls], 0.0);
-            sv = make_double2(sr + (a.deriv ? 0.0 : 1.0), 0.0);
-        } else if (upper) {
-            hv = make_double2(hr, hi); sv = make_double2(sr, si);
-        } else {
-            hv = make_double2(hr, -hi); sv = make_double2(sr, -si);
-        }
-        H[(size_t)(c0 + b) * N + r] = hv;
-        S[(size_t)(c0 + b) * N + r] = sv;
+        __syncthreads();
     }
 }
 
@@ -180,6 +207,7 @@ __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a)
 // Cholesky S = U^H U and the triangular inverse, one workgroup per k
 // ---------------------------------------------------------------------------
 constexpr int NB = 32;
+constexpr int kSubst = 8;          // columns (rows) per half-wave in the panel substitutions
 
 // In-place upper Cholesky of a w x w block held in LDS (row stride 33).
 __device__ bool lds_potrf(double2 (*D)[NB + 1], int w)
@@ -264,23 +292,37 @@ __global__ __launch_bounds__(256) void k_potrf_panel(int N, double2 *S, size_t s
             A[(size_t)(j0 + c) * N + r] = zzero();
         }
     }
-    // U12 = U11^{-H} S12: one half-wave per column, lane i holding row i;
-    // forward substitution with the finished x_i broadcast by shuffle
+    // U12 = U11^{-H} S12: one half-wave per group of kSubst columns, lane i
+    // holding row i; forward substitution with the finished x_i broadcast by
+    // shuffle.  The kSubst independent columns put their loads in flight
+    // together and interleave their shuffle chains.
+    __shared__ double rdiag[NB];
+    if (tid < w) rdiag[tid] = 1.0 / D[tid][tid].x;
+    __syncthreads();
     const int lane = tid & 63, li = lane & 31, hw = tid >> 5, nhw = blockDim.x >> 5;
     const int m = N - j0 - w;
-    for (int it = 0; it * nhw < m; ++it) {
-        const int c = j0 + w + it * nhw + hw;
-        const bool ok = c < N && li < w;
-        double2 b = ok ? A[(size_t)c * N + j0 + li] : zzero();
+    for (int it = 0; it * nhw * kSubst < m; ++it) {          // uniform trip count
+        const int cb = j0 + w + (it * nhw + hw) * kSubst;
+        double2 b[kSubst];
+#pragma unroll
+        for (int q = 0; q < kSubst; ++q)
+            b[q] = (cb + q < N && li < w) ? A[(size_t)(cb + q) * N + j0 + li] : zzero();
         for (int i = 0; i < w; ++i) {
-            const double2 xl = zscale(1.0 / D[i][i].x, b);
-            double2 xi;
-            xi.x = __shfl(xl.x, (lane & 32) + i, 64);
-            xi.y = __shfl(xl.y, (lane & 32) + i, 64);
-            if (li == i) b = xi;
-            else if (li > i) b = zsub(b, zcmul(D[i][li], xi));
+            const double ri = rdiag[i];
+            const double2 dil = D[i][li];
+#pragma unroll
+            for (int q = 0; q < kSubst; ++q) {
+                const double2 xl = zscale(ri, b[q]);
+                double2 xi;
+                xi.x = __shfl(xl.x, (lane & 32) + i, 64);
+                xi.y = __shfl(xl.y, (lane & 32) + i, 64);
+                if (li == i) b[q] = xi;
+                else if (li > i) b[q] = zsub(b[q], zcmul(dil, xi));
+            }
         }
-        if (ok) A[(size_t)c * N + j0 + li] = b;
+#pragma unroll
+        for (int q = 0; q < kSubst; ++q)
+            if (cb + q < N && li < w) A[(size_t)(cb + q) * N + j0 + li] = b[q];
     }
 }
 
@@ -301,23 +343,36 @@ __global__ __launch_bounds__(256) void k_trtri_panel(int N, double2 *S, size_t s
         D[r][c] = (r <= c) ? A[(size_t)(j0 + c) * N + j0 + r] : zzero();
     }
     __syncthreads();
-    // Uinv[i, J] = -X[i, :] U_JJ^{-1}: one half-wave per row i, lane c
-    // holding column c; substitution over c with the finished x_c broadcast
+    // Uinv[i, J] = -X[i, :] U_JJ^{-1}: one half-wave per group of kSubst
+    // rows, lane c holding column c; substitution over c with the finished
+    // x_c broadcast, the kSubst rows interleaved
     {
+        __shared__ double rdiag[NB];
+        if (tid < w) rdiag[tid] = 1.0 / D[tid][tid].x;
+        __syncthreads();
         const int lane = tid & 63, lc = lane & 31, hw = tid >> 5, nhw = blockDim.x >> 5;
-        for (int it = 0; it * nhw < j0; ++it) {
-            const int i = it * nhw + hw;
-            const bool ok = i < j0 && lc < w;
-            double2 v = ok ? zscale(-1.0, X[(size_t)lc * N + i]) : zzero();
+        for (int it = 0; it * nhw * kSubst < j0; ++it) {     // uniform trip count
+            const int ib = (it * nhw + hw) * kSubst;
+            double2 v[kSubst];
+#pragma unroll
+            for (int q = 0; q < kSubst; ++q)
+                v[q] = (ib + q < j0 && lc < w) ? zscale(-1.0, X[(size_t)lc * N + ib + q]) : zzero();
             for (int c = 0; c < w; ++c) {
-                const double2 xl = zscale(1.0 / D[c][c].x, v);
-                double2 xc;
-                xc.x = __shfl(xl.x, (lane & 32) + c, 64);
-                xc.y = __shfl(xl.y, (lane & 32) + c, 64);
-                if (lc == c) v = xc;
-                else if (lc > c) v = zsub(v, zmul(xc, D[c][lc]));
+                const double rc = rdiag[c];
+                const double2 dcl = D[c][lc];
+#pragma unroll
+                for (int q = 0; q < kSubst; ++q) {
+                    const double2 xl = zscale(rc, v[q]);
+                    double2 xc;
+                    xc.x = __shfl(xl.x, (lane & 32) + c, 64);
+                    xc.y = __shfl(xl.y, (lane & 32) + c, 64);
+                    if (lc == c) v[q] = xc;
+                    else if (lc > c) v[q] = zsub(v[q], zmul(xc, dcl));
+                }
             }
-            if (ok) A[(size_t)(j0 + lc) * N + i] = v;
+#pragma unroll
+            for (int q = 0; q < kSubst; ++q)
+                if (ib + q < j0 && lc < w) A[(size_t)(j0 + lc) * N + ib + q] = v[q];
         }
     }
     __syncthreads();
@@ -335,7 +390,10 @@ __global__ __launch_bounds__(256) void k_trtri_panel(int N, double2 *S, size_t s
 // threads (4 waves in 2 x 2, each (BM/2) x (BN/2) = MI x NJ blocks of the
 // 16x16x4 MFMA), K staged through LDS 16 at a time with the next slab
 // prefetched into registers while the matrix cores work on the current one.
-// A complex product is four real MFMAs (re*re - im*im, re*im + im*re).
+// A complex product is three real MFMAs (Gauss): with P1 = sum ar*br,
+// P2 = sum ai*bi and P3 = sum (ar+ai)(br+bi), re = P1 - P2 and
+// im = P3 - P1 - P2.  The sums ar+ai, br+bi are formed from the LDS operands
+// just before the MFMAs, so every tile shape rounds identically.
 // tri: 1 = op(A) upper triangular (skip k < row tile), 2 = op(A) lower (skip
 // k beyond the row tile), 3 = op(B) upper (skip k beyond the column tile),
 // 4 = only tiles touching the lower triangle of C are computed.
@@ -355,6 +413,17 @@ __device__ __forceinline__ v4f mfma16(float a, float b, v4f c)
 __device__ __forceinline__ int mfma_row(double, int lane, int q) { return (lane >> 4) + 4 * q; }
 __device__ __forceinline__ int mfma_row(float, int lane, int q) { return 4 * (lane >> 4) + q; }
 
+// Workgroups reach the 8 XCDs round-robin by linear id (XCD = id % 8), and
+// each XCD has its own L2.  The output tiles of one item read the same A and
+// B, so linear ids are renumbered so that every XCD receives a contiguous
+// range of logical ids -- whole items -- and an item's operands are fetched
+// into one L2 instead of eight.  A bijection for any grid size.
+__device__ __forceinline__ unsigned xcd_logical(unsigned lin, unsigned total)
+{
+    const unsigned x = lin & 7u, q = total >> 3, r = total & 7u;
+    return x * q + min(x, r) + (lin >> 3);
+}
+
 template <class R, int OPA, int OPB, int BM, int BN>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
 {
@@ -364,7 +433,9 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
     constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
     __shared__ R As[2][BK][BM + 1];  // [re/im][k][m]
     __shared__ R Bs[2][BK][BN + 1];  // [re/im][k][n]
-    const int bm = blockIdx.x * BM, bn = blockIdx.y * BN, kb = blockIdx.z;
+    const unsigned gxy = gridDim.x * gridDim.y;
+    const unsigned lg = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y + gxy * blockIdx.z, gxy * gridDim.z);
+    const int bm = (int)(lg % gridDim.x) * BM, bn = (int)((lg / gridDim.x) % gridDim.y) * BN, kb = (int)(lg / gxy);
     if (g.tri == 4 && bm + BM <= bn) return;
     const C *A = g.a + (size_t)kb * g.sa;
     const C *B = g.b + (size_t)kb * g.sb;
@@ -376,11 +447,13 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
     if (g.tri == 2) kend = min(g.k, bm + BM);
     if (g.tri == 3) kend = min(g.k, bn + BN);
     kbeg = (kbeg / BK) * BK;
-    v4 accr[MI][NJ], acci[MI][NJ];
+    v4 acc1[MI][NJ], acc2[MI][NJ], acc3[MI][NJ];
 #pragma unroll
     for (int i = 0; i < MI; ++i)
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) { accr[i][j] = v4{0, 0, 0, 0}; acci[i][j] = v4{0, 0, 0, 0}; }
+        for (int j = 0; j < NJ; ++j) {
+            acc1[i][j] = v4{0, 0, 0, 0}; acc2[i][j] = v4{0, 0, 0, 0}; acc3[i][j] = v4{0, 0, 0, 0};
+        }
     C ra[LA], rb[LB];
     auto load = [&](int k0) {
 #pragma unroll
@@ -432,25 +505,26 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 4) {
             const int kk = ks + (lane >> 4);
-            R ar[MI], ai[MI], br[NJ], bi[NJ];
+            R ar[MI], ai[MI], as[MI], br[NJ], bi[NJ], bs[NJ];
 #pragma unroll
             for (int i = 0; i < MI; ++i) {
                 ar[i] = As[0][kk][wm + 16 * i + (lane & 15)];
                 ai[i] = As[1][kk][wm + 16 * i + (lane & 15)];
+                as[i] = ar[i] + ai[i];
             }
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
                 br[j] = Bs[0][kk][wn + 16 * j + (lane & 15)];
                 bi[j] = Bs[1][kk][wn + 16 * j + (lane & 15)];
+                bs[j] = br[j] + bi[j];
             }
 #pragma unroll
             for (int i = 0; i < MI; ++i)
 #pragma unroll
                 for (int j = 0; j < NJ; ++j) {
-                    accr[i][j] = mfma16(ar[i], br[j], accr[i][j]);
-                    accr[i][j] = mfma16(-ai[i], bi[j], accr[i][j]);
-                    acci[i][j] = mfma16(ar[i], bi[j], acci[i][j]);
-                    acci[i][j] = mfma16(ai[i], br[j], acci[i][j]);
+                    acc1[i][j] = mfma16(ar[i], br[j], acc1[i][j]);
+                    acc2[i][j] = mfma16(ai[i], bi[j], acc2[i][j]);
+                    acc3[i][j] = mfma16(as[i], bs[j], acc3[i][j]);
                 }
         }
     }
@@ -464,7 +538,8 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
                 const int r = bm + wm + 16 * i + mfma_row(R(0), lane, q);
                 const int c = bn + wn + 16 * j + (lane & 15);
                 if (r < g.m && c < g.n) {
-                    const C v = cmake<C, R>(accr[i][j][q], acci[i][j][q]);
+                    const R p1 = acc1[i][j][q], p2 = acc2[i][j][q];
+                    const C v = cmake<C, R>(p1 - p2, acc3[i][j][q] - p1 - p2);
                     C out = zmul(g.alpha, v);
                     C *p = Cm + (size_t)c * g.ldc + r;
                     if (usebeta) out = zadd(out, zmul(g.beta, *p));
@@ -1892,7 +1967,8 @@ void launch_phases(const StructDev *sd, const int *item_s, const double *kpts, i
 }
 void launch_build_hs(const BuildArgs &a, int maxnat, int nb, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_build_hs, dim3(maxnat + 1, nb), dim3(256), 0, st, a);
+    const int nblk = maxnat * (maxnat + 1) / 2 + kBuildPadWgs;     // upper blocks + padding
+    hipLaunchKernelGGL(k_build_hs, dim3(nblk, (nb + kBuildItems - 1) / kBuildItems), dim3(256), 0, st, a, nb);
 }
 void launch_potrf_panel(int n, double2 *s, size_t stride, int j0, int *info, int nb,
                         hipStream_t st)

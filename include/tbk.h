@@ -306,7 +306,11 @@ int tbk_debug_gemm_time(int device, int precision, int tile, int m, int n,
 
 /* C = op(A) op(B) of column-major complex matrices (complex128, or
  * complex64 when precision is TBK_MIXED) with the library's batched GEMM;
- * tile -1 = the tuned / modelled choice, else 0..3 (see tbk_gemm_tile). */
+ * tile -1 = the tuned / modelled choice, else 0..3 (see tbk_gemm_tile).
+ * tri: 0 full; 1 op(A) upper triangular; 2 op(A) lower triangular; 3 op(B)
+ * upper triangular; 4 only the tiles touching the lower triangle of C;
+ * 5 (m == n) the lower triangle of the product and its conjugate transpose
+ * above the diagonal. */
 int tbk_debug_zgemm(int device, int precision, int tile, int m, int n, int k,
                     int opa, int opb, int tri, const void *a, int lda,
                     const void *b, int ldb, void *c, int ldc);

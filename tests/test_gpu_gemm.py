@@ -66,3 +66,12 @@ def test_gemm_triangular_variants(tile):
     ref = X @ Lw
     low = np.tril(np.ones((n, n), bool))
     assert np.abs((c - ref)[low]).max() < 1e-11
+    # tri 5: lower triangle of Y^H X mirrored into the upper one (exactly
+    # Hermitian result, the upper triangle the conjugate of the lower)
+    for prec, tol in ((0, 1e-11), (1, 2e-3)):
+        c = run(prec, tile, n, n, n, 1, 0, 5, X, Lw)
+        ref = X.conj().T @ Lw
+        herm = np.tril(ref) + np.tril(ref, -1).conj().T
+        assert np.abs(c - herm).max() < tol, (prec, np.abs(c - herm).max())
+        off = ~np.eye(n, dtype=bool)
+        assert np.array_equal(c[off], c.conj().T[off])

@@ -47,7 +47,9 @@ struct GemmArgsT {
     const C *b; int ldb; size_t sb;
     C *c; int ldc; size_t sc;
     int tri;                       // 0 full; 1 A upper triangular; 2 A lower triangular;
-                                   // 3 B upper triangular; 4 lower triangle of C only
+                                   // 3 B upper triangular; 4 lower triangle of C only;
+                                   // 5 lower triangle of C mirrored into the upper
+                                   //   (Hermitian C; alpha real, beta 0)
 };
 typedef GemmArgsT<double> GemmArgs;
 

@@ -396,7 +396,10 @@ __global__ __launch_bounds__(256) void k_trtri_panel(int N, double2 *S, size_t s
 // just before the MFMAs, so every tile shape rounds identically.
 // tri: 1 = op(A) upper triangular (skip k < row tile), 2 = op(A) lower (skip
 // k beyond the row tile), 3 = op(B) upper (skip k beyond the column tile),
-// 4 = only tiles touching the lower triangle of C are computed.
+// 4 = only tiles touching the lower triangle of C are computed; 5 = as 4,
+// and C receives the lower triangle of the product and its conjugate
+// transpose above the diagonal (a Hermitian C from its lower half; alpha
+// real, beta 0).
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -436,7 +439,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
     const unsigned gxy = gridDim.x * gridDim.y;
     const unsigned lg = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y + gxy * blockIdx.z, gxy * gridDim.z);
     const int bm = (int)(lg % gridDim.x) * BM, bn = (int)((lg / gridDim.x) % gridDim.y) * BN, kb = (int)(lg / gxy);
-    if (g.tri == 4 && bm + BM <= bn) return;
+    if ((g.tri == 4 || g.tri == 5) && bm + BM <= bn) return;
     const C *A = g.a + (size_t)kb * g.sa;
     const C *B = g.b + (size_t)kb * g.sb;
     C *Cm = g.c + (size_t)kb * g.sc;
@@ -502,7 +505,9 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
         }
         __syncthreads();
         if (k0 + BK < kend) load(k0 + BK);
-#pragma unroll
+        // not unrolled: the operands of one k step at a time keep the
+        // register count low enough for several workgroups per CU
+#pragma unroll 1
         for (int ks = 0; ks < BK; ks += 4) {
             const int kk = ks + (lane >> 4);
             R ar[MI], ai[MI], as[MI], br[NJ], bi[NJ], bs[NJ];
@@ -542,6 +547,16 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
                     const C v = cmake<C, R>(p1 - p2, acc3[i][j][q] - p1 - p2);
                     C out = zmul(g.alpha, v);
                     C *p = Cm + (size_t)c * g.ldc + r;
+                    if (g.tri == 5) {
+                        // Hermitian from the lower half: element (r, c), r >= c,
+                        // and its conjugate at (c, r); 16 lanes of one row write
+                        // 16 consecutive rows of column r
+                        if (r >= c) {
+                            *p = out;
+                            if (r > c && r < g.n) Cm[(size_t)r * g.ldc + c] = cmake<C, R>(out.x, -out.y);
+                        }
+                        continue;
+                    }
                     if (usebeta) out = zadd(out, zmul(g.beta, *p));
                     *p = out;
                 }
@@ -665,16 +680,75 @@ __global__ __launch_bounds__(HT, (MAXQ <= 5 ? 4 : 2)) void k_hetrd_panel(HetrdAr
             }
         }
         __syncthreads();
-        // (3) y = A_t v from the lower triangle (columns r >= t0), and the 2i
-        //     panel dots W[:,l]^H v, V[:,l]^H v over rows >= t0, one wave task each
+        // (3) the 2i panel dots W[:,l]^H v, V[:,l]^H v over rows >= t0 (one
+        //     wave task each), then y = A_t v from the lower triangle (columns
+        //     r >= t0); udot[l] = W[:,l]^H v, udot[NB + l] = V[:,l]^H v
+        {
+            // kTg tasks per wave pass, every row chunk of a task loaded at once
+            constexpr int kTg = 1, QT = MAXQ <= 5 ? MAXQ : 5;
+            const int nqt = (N + 63) >> 6, qlo = t0 >> 6;
+            const int wvu = __builtin_amdgcn_readfirstlane(wv);
+            for (int tk = wvu; tk < 2 * i; tk += HW * kTg) {
+                const C *colp[kTg];
+                C acc[kTg];
+#pragma unroll
+                for (int g = 0; g < kTg; ++g) {
+                    const int task = min(tk + HW * g, 2 * i - 1);
+                    colp[g] = task < i ? W + (size_t)task * N : V + (size_t)(task - i) * N;
+                    acc[g] = czero<C>();
+                }
+                for (int q0 = qlo; q0 < nqt; q0 += QT) {
+                    C a[kTg][QT];
+#pragma unroll
+                    for (int u = 0; u < QT; ++u) {
+                        const int c = ((q0 + u) << 6) + lane;
+#pragma unroll
+                        for (int g = 0; g < kTg; ++g)
+                            a[g][u] = (tk + HW * g < 2 * i && c >= t0 && c < N) ? colp[g][c] : czero<C>();
+                    }
+#pragma unroll
+                    for (int u = 0; u < QT; ++u) {
+                        const int c = ((q0 + u) << 6) + lane;
+                        const C vc = (c >= t0 && c < N) ? vloc[c] : czero<C>();
+#pragma unroll
+                        for (int g = 0; g < kTg; ++g) acc[g] = zadd(acc[g], zcmul(a[g][u], vc));
+                    }
+                }
+                for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+                    for (int g = 0; g < kTg; ++g) {
+                        acc[g].x += __shfl_xor(acc[g].x, off, 64);
+                        acc[g].y += __shfl_xor(acc[g].y, off, 64);
+                    }
+                if (lane == 0)
+#pragma unroll
+                    for (int g = 0; g < kTg; ++g) {
+                        const int task = tk + HW * g;
+                        if (task < 2 * i) udot[task < i ? task : NB + task - i] = acc[g];
+                    }
+            }
+        }
+        if (tid < i && t0 < N) {
+            uwn[tid] = zconj(W[(size_t)tid * N + t0]);
+            uvn[tid] = zconj(V[(size_t)tid * N + t0]);
+        }
         C ypart[MAXQ];
 #pragma unroll
         for (int q = 0; q < MAXQ; ++q) ypart[q] = czero<C>();
         const int nq = (N + 63) >> 6;
         // columns r = t0 + tb + HW g (g < HG) per wave pass: HG x QC loads in
-        // flight per lane and HG independent wave reductions
-        constexpr int HG = sizeof(R) == 8 ? 2 : 4, QC = sizeof(R) == 8 ? 3 : 4;
-        for (int tb = wv; tb < m; tb += HW * HG) {
+        // flight per lane and HG independent wave reductions.  Up to order
+        // 640 a column's chunks are all loaded at once.  Row chunks of 64
+        // that lie wholly above the pass's first column hold no element of
+        // the lower triangle and are skipped by a uniform branch (loads
+        // first, then the arithmetic, so the loads of a pass stay in flight
+        // together): the product's cost follows the triangle, not the square.
+        constexpr int HG = sizeof(R) == 8 ? 2 : 4;
+        constexpr int QC = sizeof(R) == 8 ? 3 : (MAXQ <= 5 ? MAXQ : (MAXQ <= 10 ? 5 : 4));
+        // the wave index is uniform: columns and their base addresses live in
+        // scalar registers, each load needs only the lane's row offset
+        const int wvu = __builtin_amdgcn_readfirstlane(wv);
+        for (int tb = wvu; tb < m; tb += HW * HG) {
             C acc[HG], vr[HG];
             int rg[HG];
 #pragma unroll
@@ -683,21 +757,26 @@ __global__ __launch_bounds__(HT, (MAXQ <= 5 ? 4 : 2)) void k_hetrd_panel(HetrdAr
                 acc[g] = czero<C>();
                 vr[g] = rg[g] < N ? vloc[rg[g]] : czero<C>();
             }
+            const int qlo = rg[0] >> 6;                // first chunk holding rows >= rg[0]
 #pragma unroll
             for (int q0 = 0; q0 < MAXQ; q0 += QC) {
-                if (q0 < nq) {
+                if (q0 < nq && q0 + QC > qlo) {
                     C a[HG][QC];
 #pragma unroll
-                    for (int g = 0; g < HG; ++g)
+                    for (int u = 0; u < QC; ++u) {
+                        if (q0 + u >= qlo && q0 + u < nq) {
 #pragma unroll
-                        for (int u = 0; u < QC; ++u) {
-                            const int c = ((q0 + u) << 6) + lane;
-                            a[g][u] = (c >= rg[g] && c < N) ? A[(size_t)rg[g] * N + c] : czero<C>();
+                            for (int g = 0; g < HG; ++g) {
+                                const C *colg = A + (size_t)rg[g] * N;
+                                const int c = ((q0 + u) << 6) + lane;
+                                a[g][u] = (c >= rg[g] && c < N) ? colg[c] : czero<C>();
+                            }
                         }
+                    }
 #pragma unroll
                     for (int u = 0; u < QC; ++u) {
                         const int c = ((q0 + u) << 6) + lane;
-                        if (q0 + u < MAXQ) {
+                        if (q0 + u < MAXQ && q0 + u >= qlo && q0 + u < nq) {
                             const C vc = c < N ? vloc[c] : czero<C>();
 #pragma unroll
                             for (int g = 0; g < HG; ++g) {
@@ -718,28 +797,6 @@ __global__ __launch_bounds__(HT, (MAXQ <= 5 ? 4 : 2)) void k_hetrd_panel(HetrdAr
 #pragma unroll
                 for (int g = 0; g < HG; ++g)
                     if (rg[g] < N) ybuf[rg[g]] = acc[g];
-        }
-        // panel dots W[:,l]^H v (udot[l]) and V[:,l]^H v (udot[NB + l])
-        for (int task = wv; task < 2 * i; task += HW) {
-            const C *col = task < i ? W + (size_t)task * N : V + (size_t)(task - i) * N;
-            C acc = czero<C>(), acc2 = czero<C>();
-            int c = t0 + lane;
-            for (; c + 64 < N; c += 128) {
-                const C a0 = col[c], a1 = col[c + 64];
-                acc = zadd(acc, zcmul(a0, vloc[c]));
-                acc2 = zadd(acc2, zcmul(a1, vloc[c + 64]));
-            }
-            if (c < N) acc = zadd(acc, zcmul(col[c], vloc[c]));
-            acc = zadd(acc, acc2);
-            for (int off = 32; off > 0; off >>= 1) {
-                acc.x += __shfl_xor(acc.x, off, 64);
-                acc.y += __shfl_xor(acc.y, off, 64);
-            }
-            if (lane == 0) udot[task < i ? task : NB + task - i] = acc;
-        }
-        if (tid < i && t0 < N) {
-            uwn[tid] = zconj(W[(size_t)tid * N + t0]);
-            uvn[tid] = zconj(V[(size_t)tid * N + t0]);
         }
         // partials: slot[w] = part_w + part_{w+4} for w < 4
         if (wv >= 4) {
@@ -773,13 +830,24 @@ __global__ __launch_bounds__(HT, (MAXQ <= 5 ? 4 : 2)) void k_hetrd_panel(HetrdAr
             C y = zadd(zadd(zadd(zadd(ybuf[r], slot[r]), slot[(size_t)N + r]), slot[(size_t)2 * N + r]),
                        slot[(size_t)3 * N + r]);
             C nx = czero<C>();
-#pragma unroll 2
-            for (int l = 0; l < i; ++l) {
-                const C vl = V[(size_t)l * N + r], wl = W[(size_t)l * N + r];
-                y = zsub(y, zmul(vl, udot[l]));
-                y = zsub(y, zmul(wl, udot[NB + l]));
-                nx = zadd(nx, zmul(vl, uwn[l]));
-                nx = zadd(nx, zmul(wl, uvn[l]));
+            // the row's V and W entries are loaded kWst columns at a time
+            // before they are used, so their loads are in flight together
+            constexpr int kWst = 2;
+            for (int l0 = 0; l0 < i; l0 += kWst) {
+                C vv[kWst], ww[kWst];
+#pragma unroll
+                for (int u = 0; u < kWst; ++u)
+                    if (l0 + u < i) { vv[u] = V[(size_t)(l0 + u) * N + r]; ww[u] = W[(size_t)(l0 + u) * N + r]; }
+#pragma unroll
+                for (int u = 0; u < kWst; ++u) {
+                    const int l = l0 + u;
+                    if (l < i) {
+                        y = zsub(y, zmul(vv[u], udot[l]));
+                        y = zsub(y, zmul(ww[u], udot[NB + l]));
+                        nx = zadd(nx, zmul(vv[u], uwn[l]));
+                        nx = zadd(nx, zmul(ww[u], uvn[l]));
+                    }
+                }
             }
             carry[q] = nx;
             const C w = zmul(tau, y);
@@ -1529,8 +1597,8 @@ __global__ void k_real_to_c32_t(int N, const double *Vt, float2 *Y, size_t strid
 //   delta    = 2 (||T - diag(lambda)||_F + max|lambda| ||R||_F),
 //   E_ij     = (T_ij + lambda_j R_ij) / (lambda_j - lambda_i)  if |lambda_i - lambda_j| > delta,
 //              R_ij / 2                                        otherwise,
-// and the refined vectors are Y (I + E).  T and G are Hermitian; only their
-// lower triangles are read.
+// and the refined vectors are Y (I + E).  T and G are Hermitian (both
+// triangles stored, the upper one the conjugate of the lower).
 // k_oa_stats (one workgroup per k) forms lambda, delta and the ascending
 // order perm of lambda; k_oa_fmat writes F = I + E with its columns in that
 // order (F[:, j] = (I + E)[:, perm_j]), so that Y F is the refined, sorted
@@ -1614,18 +1682,15 @@ __global__ __launch_bounds__(256) void k_oa_fmat(int N, const double2 *T, const 
     const int r = (int)(t % N), j = (int)(t / N);
     const int c = perm[(size_t)kb * N + j];
     const double *lam = lamv + (size_t)kb * N;
-    // T and G are Hermitian and hold their lower triangles only
-    const bool low = r >= c;
-    const size_t src = (size_t)kb * stride + (low ? (size_t)c * N + r : (size_t)r * N + c);
-    double2 gv = G[src];
-    if (!low) gv.y = -gv.y;
+    // T and G are Hermitian, both triangles stored: column c is read down
+    const size_t src = (size_t)kb * stride + (size_t)c * N + r;
+    const double2 gv = G[src];
     double2 rv = make_double2(-gv.x, -gv.y);
     if (r == c) rv.x += 1.0;
     double2 ev;
     const double gap = lam[c] - lam[r];
     if (fabs(gap) > delta[kb]) {
-        double2 tv = T[src];
-        if (!low) tv.y = -tv.y;
+        const double2 tv = T[src];
         ev = make_double2((tv.x + lam[c] * rv.x) / gap, (tv.y + lam[c] * rv.y) / gap);
     } else {
         ev = make_double2(0.5 * rv.x, 0.5 * rv.y);

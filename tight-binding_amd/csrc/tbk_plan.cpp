@@ -378,10 +378,11 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
     tbk::launch_c32_to_c64(ss * nb, ws.Y32, ws.Y, st);
     tm.end();
     tm.begin("refine");
-    // X = C Y, Z = Y^H X (= Y^H C Y), VT = Y^H Y (Hermitian: lower triangles)
+    // X = C Y, Z = Y^H X (= Y^H C Y), VT = Y^H Y (Hermitian: lower triangles
+    // computed, mirrored into the upper ones)
     tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.Y, n, ss, ws.X, n, ss), 0, 0, nb, st);
-    tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.Z, n, ss, 4), 1, 0, nb, st);   // lower
-    tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.Y, n, ss, ws.VT, n, ss, 4), 1, 0, nb, st);  // lower
+    tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.Z, n, ss, 5), 1, 0, nb, st);
+    tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.Y, n, ss, ws.VT, n, ss, 5), 1, 0, nb, st);
     // X = (I + E) with sorted columns; VT = Y X; Z = U^{-1} VT
     tbk::launch_oa_update(n, nact, ws.Z, ws.VT, ws.X, ss, ws.lam, ws.perm, ws.delta, w_out, nb, st);
     tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.VT, n, ss), 0, 0, nb, st);
@@ -1153,6 +1154,7 @@ extern "C" int tbk_debug_zgemm(int device, int precision, int tile, int m, int n
                                int tri, const void *a, int lda, const void *b, int ldb, void *c, int ldc)
 {
     if (tile >= tbk::kGemmTiles || opa < 0 || opa > 1 || opb < 0 || opb > 1) return TBK_ERR_ARG;
+    if (tri < 0 || tri > 5 || (tri == 5 && m != n) || m <= 0 || n <= 0 || k <= 0) return TBK_ERR_ARG;
     HIP_TRY(hipSetDevice(device));
     if (precision == TBK_MIXED)
         return debug_gemm_t<float2, tbk::GemmArgsT<float>>(tile, m, n, k, opa, opb, tri, a, lda, b, ldb, c, ldc);

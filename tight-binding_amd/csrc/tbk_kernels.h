@@ -82,6 +82,7 @@ struct SteinArgs {
     const double *d, *e;
     double *w, *v, *scratch;       // w [nb][n]; v [nb][n*n]; scratch [nb][5*n*stein_threads(n)]
     int *info;
+    double abstol_rel = 0.0;       // > 0: bisection stops at abstol_rel * ||T||_1 (else dstebz's)
 };
 
 struct PerturbArgs {
@@ -124,8 +125,6 @@ void launch_phases(const StructDev *sd, const int *item_s, const double *kpts, i
 void launch_build_hs(const BuildArgs &a, int maxnat, int nb, hipStream_t st);
 void launch_potrf_panel(int n, double2 *s, size_t stride, int j0, int *info, int nb,
                         hipStream_t st);
-void launch_trtri_panel(int n, double2 *s, size_t stride, const double2 *wk, int j0,
-                        const int *info, int nb, hipStream_t st);
 void launch_zgemm(const GemmArgs &g, int opa, int opb, int nb, hipStream_t st);
 void launch_cgemm(const GemmArgsT<float> &g, int opa, int opb, int nb, hipStream_t st);
 void launch_gemm_tile(const GemmShape &s, int tile, hipStream_t st);   // a recorded shape, given tile

@@ -6,8 +6,9 @@
 // Kernels, in pipeline order:
 //   k_build_hs      Bloch sums of precomputed Slater-Koster blocks -> H, S
 //                   (setham.f + rotate.f; sethamv.f for dH/dV, dS/dV)
-//   k_potrf_panel,  blocked Cholesky S = U^H U (zpptrf, lapack.f:4466) and
-//   k_trtri_panel   U^{-1} by panels; O(n^3) updates run as GEMMs
+//   k_potrf_panel   blocked Cholesky S = U^H U (zpptrf, lapack.f:4466) and
+//                   U^{-1}: diagonal blocks factored and inverted in LDS,
+//                   panel rows, inverse blocks and updates as GEMMs
 //   k_zgemm         batched complex GEMM on the FP64 MFMA units
 //   k_hetrd_panel   blocked Householder tridiagonalisation (zhptrd,
 //                   lapack.f:3218) by panels; WY factors for the back-transform
@@ -207,7 +208,6 @@ __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a, int nb)
 // Cholesky S = U^H U and the triangular inverse, one workgroup per k
 // ---------------------------------------------------------------------------
 constexpr int NB = 32;
-constexpr int kSubst = 8;          // columns (rows) per half-wave in the panel substitutions
 
 // In-place upper Cholesky of a w x w block held in LDS (row stride 33).
 __device__ bool lds_potrf(double2 (*D)[NB + 1], int w)
@@ -259,10 +259,12 @@ __device__ void lds_trtri(double2 (*D)[NB + 1], int w)
 }
 
 // Cholesky, right-looking by panels of NB columns: this kernel factors the
-// diagonal block (LDS) and forms the panel row U12 = U11^{-H} S12; the
-// caller applies the trailing update S22 -= U12^H U12 as a GEMM.  The strict
-// lower triangle of the panel's columns is cleared (the inverse stage and
-// the GEMMs read whole tiles).  info[k] = first failing column + 1.
+// diagonal block S11 = U11^H U11 in LDS, inverts U11 there and stores
+// U11^{-1} in place of the block.  The factor itself is not needed again:
+// the caller forms the panel row U12 = U11^{-H} S12 and, later, the column
+// blocks of U^{-1} as GEMMs with U11^{-1}, and the trailing update
+// S22 -= U12^H U12.  The strict lower triangle of the panel's columns is
+// cleared (the GEMMs read whole tiles).  info[k] = first failing column + 1.
 __global__ __launch_bounds__(256) void k_potrf_panel(int N, double2 *S, size_t stride, int j0,
                                                      int *info)
 {
@@ -284,102 +286,14 @@ __global__ __launch_bounds__(256) void k_potrf_panel(int N, double2 *S, size_t s
         if (tid == 0) info[kb] = j0 + 1;
         return;
     }
-    for (int t = tid; t < w * N; t += blockDim.x) {   // write U11, clear below
+    lds_trtri(D, w);
+    for (int t = tid; t < w * N; t += blockDim.x) {   // write U11^{-1}, clear below
         const int r = t % N, c = t / N;
         if (r >= j0 && r - j0 <= c) {
             if (r - j0 < w) A[(size_t)(j0 + c) * N + r] = D[r - j0][c];
         } else if (r > j0 + c) {
             A[(size_t)(j0 + c) * N + r] = zzero();
         }
-    }
-    // U12 = U11^{-H} S12: one half-wave per group of kSubst columns, lane i
-    // holding row i; forward substitution with the finished x_i broadcast by
-    // shuffle.  The kSubst independent columns put their loads in flight
-    // together and interleave their shuffle chains.
-    __shared__ double rdiag[NB];
-    if (tid < w) rdiag[tid] = 1.0 / D[tid][tid].x;
-    __syncthreads();
-    const int lane = tid & 63, li = lane & 31, hw = tid >> 5, nhw = blockDim.x >> 5;
-    const int m = N - j0 - w;
-    for (int it = 0; it * nhw * kSubst < m; ++it) {          // uniform trip count
-        const int cb = j0 + w + (it * nhw + hw) * kSubst;
-        double2 b[kSubst];
-#pragma unroll
-        for (int q = 0; q < kSubst; ++q)
-            b[q] = (cb + q < N && li < w) ? A[(size_t)(cb + q) * N + j0 + li] : zzero();
-        for (int i = 0; i < w; ++i) {
-            const double ri = rdiag[i];
-            const double2 dil = D[i][li];
-#pragma unroll
-            for (int q = 0; q < kSubst; ++q) {
-                const double2 xl = zscale(ri, b[q]);
-                double2 xi;
-                xi.x = __shfl(xl.x, (lane & 32) + i, 64);
-                xi.y = __shfl(xl.y, (lane & 32) + i, 64);
-                if (li == i) b[q] = xi;
-                else if (li > i) b[q] = zsub(b[q], zcmul(dil, xi));
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < kSubst; ++q)
-            if (cb + q < N && li < w) A[(size_t)(cb + q) * N + j0 + li] = b[q];
-    }
-}
-
-// Triangular inverse, left to right by block columns J: the caller forms
-// X = Uinv[0:j0, 0:j0] U[0:j0, J] (GEMM) into Wk; this kernel sets
-// Uinv[0:j0, J] = -X U_JJ^{-1} and inverts the diagonal block in LDS.
-__global__ __launch_bounds__(256) void k_trtri_panel(int N, double2 *S, size_t stride,
-                                                     const double2 *Wk, int j0, const int *info)
-{
-    __shared__ double2 D[NB][NB + 1];
-    const int kb = blockIdx.x, tid = threadIdx.x;
-    if (info[kb] != 0) return;
-    double2 *A = S + (size_t)kb * stride;
-    const double2 *X = Wk + (size_t)kb * stride;
-    const int w = min(NB, N - j0);
-    for (int t = tid; t < w * w; t += blockDim.x) {
-        const int r = t % w, c = t / w;
-        D[r][c] = (r <= c) ? A[(size_t)(j0 + c) * N + j0 + r] : zzero();
-    }
-    __syncthreads();
-    // Uinv[i, J] = -X[i, :] U_JJ^{-1}: one half-wave per group of kSubst
-    // rows, lane c holding column c; substitution over c with the finished
-    // x_c broadcast, the kSubst rows interleaved
-    {
-        __shared__ double rdiag[NB];
-        if (tid < w) rdiag[tid] = 1.0 / D[tid][tid].x;
-        __syncthreads();
-        const int lane = tid & 63, lc = lane & 31, hw = tid >> 5, nhw = blockDim.x >> 5;
-        for (int it = 0; it * nhw * kSubst < j0; ++it) {     // uniform trip count
-            const int ib = (it * nhw + hw) * kSubst;
-            double2 v[kSubst];
-#pragma unroll
-            for (int q = 0; q < kSubst; ++q)
-                v[q] = (ib + q < j0 && lc < w) ? zscale(-1.0, X[(size_t)lc * N + ib + q]) : zzero();
-            for (int c = 0; c < w; ++c) {
-                const double rc = rdiag[c];
-                const double2 dcl = D[c][lc];
-#pragma unroll
-                for (int q = 0; q < kSubst; ++q) {
-                    const double2 xl = zscale(rc, v[q]);
-                    double2 xc;
-                    xc.x = __shfl(xl.x, (lane & 32) + c, 64);
-                    xc.y = __shfl(xl.y, (lane & 32) + c, 64);
-                    if (lc == c) v[q] = xc;
-                    else if (lc > c) v[q] = zsub(v[q], zmul(xc, dcl));
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < kSubst; ++q)
-                if (ib + q < j0 && lc < w) A[(size_t)(j0 + lc) * N + ib + q] = v[q];
-        }
-    }
-    __syncthreads();
-    lds_trtri(D, w);
-    for (int t = tid; t < w * w; t += blockDim.x) {
-        const int r = t % w, c = t / w;
-        A[(size_t)(j0 + c) * N + j0 + r] = (r <= c) ? D[r][c] : zzero();
     }
 }
 
@@ -1231,7 +1145,8 @@ __device__ TriBounds tri_bounds(const double *d, const double *e, const double *
 }
 
 // Eigenvalues of the tridiagonal by bisection on Sturm counts, one thread
-// per eigenvalue (dstebz with the absolute tolerance 2 eps max(|a|,|b|)).
+// per eigenvalue (dstebz with the absolute tolerance 2 eps max(|a|,|b|), or
+// the coarser abstol_rel ||T||_1 when the caller refines the eigenpairs).
 __global__ __launch_bounds__(256) void k_stebz(SteinArgs s)
 {
     __shared__ double d[1280], e[1280], e2[1280];
@@ -1253,9 +1168,10 @@ __global__ __launch_bounds__(256) void k_stebz(SteinArgs s)
     if (j >= LD) return;
     if (j >= N) { s.w[(size_t)kb * LD + j] = dg[j]; return; }    // decoupled padding
     double a = tb.gl, b = tb.gu;
+    const double atol = s.abstol_rel * tb.onenrm;
     for (int it = 0; it < 200; ++it) {
         const double mid = 0.5 * (a + b);
-        const double tol = 2.0 * eps * fmax(fabs(a), fabs(b)) + tb.pivmin;
+        const double tol = fmax(2.0 * eps * fmax(fabs(a), fabs(b)) + tb.pivmin, atol);
         if (b - a <= tol || mid <= a || mid >= b) break;
         if (sturm_count(d, e2, N, mid, tb.pivmin) >= j + 1) b = mid; else a = mid;
     }
@@ -2039,11 +1955,6 @@ void launch_potrf_panel(int n, double2 *s, size_t stride, int j0, int *info, int
                         hipStream_t st)
 {
     hipLaunchKernelGGL(k_potrf_panel, dim3(nb), dim3(256), 0, st, n, s, stride, j0, info);
-}
-void launch_trtri_panel(int n, double2 *s, size_t stride, const double2 *wk, int j0,
-                        const int *info, int nb, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_trtri_panel, dim3(nb), dim3(256), 0, st, n, s, stride, wk, j0, info);
 }
 constexpr int kTileBM[kGemmTiles] = {64, 96, 128, 32};
 constexpr int kTileBN[kGemmTiles] = {64, 96, 64, 32};

@@ -243,19 +243,25 @@ void potrf_trtri(Workspace &ws, int nb, hipStream_t st)
     const size_t ss = ws.stride();
     for (int j0 = 0; j0 < n; j0 += kNB) {
         const int w = std::min(kNB, n - j0), s0 = j0 + w, m = n - s0;
-        tbk::launch_potrf_panel(n, ws.S, ss, j0, ws.info, nb, st);
-        if (m > 0) {   // S22 -= U12^H U12
-            const double2 *u12 = ws.S + (size_t)s0 * n + j0;
+        tbk::launch_potrf_panel(n, ws.S, ss, j0, ws.info, nb, st);   // S[J, J] <- U_JJ^{-1}
+        if (m > 0) {
+            const double2 *dinv = ws.S + (size_t)j0 * n + j0;
+            double2 *u12 = ws.S + (size_t)s0 * n + j0;
+            // U12 = U_JJ^{-H} S12 in place: every output tile covers all w
+            // rows and reads only its own columns of S12 before it writes
+            tbk::launch_zgemm(gemm(w, m, w, dinv, n, ss, u12, n, ss, u12, n, ss), 1, 0, nb, st);
+            // S22 -= U12^H U12
             tbk::launch_zgemm(gemm(m, m, w, u12, n, ss, u12, n, ss, ws.S + (size_t)s0 * n + s0, n, ss,
                                    0, double2{-1.0, 0.0}, double2{1.0, 0.0}), 1, 0, nb, st);
         }
     }
-    for (int j0 = 0; j0 < n; j0 += kNB) {
+    // U^{-1} by block columns J: Y = Uinv[0:j0, 0:j0] U[0:j0, J], then
+    // Uinv[0:j0, J] = -Y U_JJ^{-1} (the diagonal block already holds U_JJ^{-1})
+    for (int j0 = kNB; j0 < n; j0 += kNB) {
         const int w = std::min(kNB, n - j0);
-        if (j0 > 0)    // Y = Uinv[0:j0, 0:j0] U[0:j0, J]
-            tbk::launch_zgemm(gemm(j0, w, j0, ws.S, n, ss, ws.S + (size_t)j0 * n, n, ss, ws.Y, n, ss, 1),
-                              0, 0, nb, st);
-        tbk::launch_trtri_panel(n, ws.S, ss, ws.Y, j0, ws.info, nb, st);
+        tbk::launch_zgemm(gemm(j0, w, j0, ws.S, n, ss, ws.S + (size_t)j0 * n, n, ss, ws.Y, n, ss, 1), 0, 0, nb, st);
+        tbk::launch_zgemm(gemm(j0, w, w, ws.Y, n, ss, ws.S + (size_t)j0 * n + j0, n, ss, ws.S + (size_t)j0 * n, n,
+                               ss, 0, double2{-1.0, 0.0}), 0, 0, nb, st);
     }
 }
 
@@ -332,6 +338,11 @@ void apply_q(int n, int nb, typename tbk::cplx_t<R>::type *A, typename tbk::cplx
 // tridiagonalisation and back-transform on the FP32 MFMA units; FP64
 // bisection/eigenvectors of the tridiagonal; one FP64 Ogita-Aishima step on
 // the reduced problem C; FP64 Z = U^{-1} Y.
+// Bisection tolerance of the mixed path, relative to ||T||_1: the
+// tridiagonal of the FP32 reduction carries FP32 errors, and its eigenvalues
+// only seed the vectors that the FP64 refinement corrects.
+constexpr double kMixedBisectTol = 1e-11;
+
 void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hipStream_t st, StageTimer &tm)
 {
     const int n = ws.n;
@@ -370,6 +381,7 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
     tm.end();
     tm.begin("stein");
     tbk::SteinArgs sa{n, kOrtolRel, nact, ws.d, ws.e, ws.w, ws.Vt, ws.scratch, ws.sinfo};
+    sa.abstol_rel = kMixedBisectTol;
     tbk::launch_stein(sa, nb, st);
     tm.end();
     tm.begin("backtransform");

@@ -75,3 +75,8 @@ def test_gemm_triangular_variants(tile):
         assert np.abs(c - herm).max() < tol, (prec, np.abs(c - herm).max())
         off = ~np.eye(n, dtype=bool)
         assert np.array_equal(c[off], c.conj().T[off])
+    # tri 6: op(A) lower (A = U^H) with the Hermitian mirror of tri 5
+    c = run(0, tile, n, n, n, 1, 0, 6, U, X)
+    ref = U.conj().T @ X
+    herm = np.tril(ref) + np.tril(ref, -1).conj().T
+    assert np.abs(c - herm).max() < 1e-11

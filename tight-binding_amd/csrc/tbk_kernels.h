@@ -49,7 +49,8 @@ struct GemmArgsT {
     int tri;                       // 0 full; 1 A upper triangular; 2 A lower triangular;
                                    // 3 B upper triangular; 4 lower triangle of C only;
                                    // 5 lower triangle of C mirrored into the upper
-                                   //   (Hermitian C; alpha real, beta 0)
+                                   //   (Hermitian C; alpha real, beta 0); 6 = 5 with
+                                   //   A lower triangular as in 2
 };
 typedef GemmArgsT<double> GemmArgs;
 

@@ -108,23 +108,22 @@ __global__ __launch_bounds__(256) void k_phases(const StructDev *sd, const int *
 }
 
 // One workgroup per (upper block (ia, ja), ia <= ja; chunk of kBuildItems
-// items).  Each pass takes three items, one thread per element (o, b) of the
-// 9x9 block with o, the row orbital, fastest, so that groups of lanes write
-// nine consecutive rows of one column.  The element is the Bloch sum over
-// the pairs (ia, ja) of the reference's pair list, in list order, of
-// e(o, b) * phase (rotate.f:260-326); the lower block (ja, ia) -- and the
-// lower half of a diagonal block -- is its conjugate transpose, passed
-// through LDS and written with the column orbital fastest.  The accumulation
-// order and the absence of FMA contraction reproduce the reference's
-// rounding except for the ulp of sin/cos.  The SK blocks of one atom pair
-// are shared by every item of the chunk and stay in L1/L2.  The last
-// kBuildPadWgs workgroup columns write the padding rows and columns nv..n-1.
+// items): three groups of 81 threads, one thread per element (o, b) of the
+// 9x9 block with o, the row orbital, fastest, each group taking every third
+// item of the chunk.  The element is the Bloch sum over the pairs (ia, ja)
+// of the reference's pair list, in list order, of e(o, b) * phase
+// (rotate.f:260-326); its conjugate goes to the partner element of the lower
+// block (ja, ia) (or of the lower half of a diagonal block).  The
+// accumulation order and the absence of FMA contraction reproduce the
+// reference's rounding except for the ulp of sin/cos.  When a group's items
+// share one structure, each pair's SK entries are loaded once for all of
+// them.  The last kBuildPadWgs workgroup columns write the padding rows and
+// columns nv..n-1.
 constexpr int kBuildItems = 24, kBuildPadWgs = 8;
 
 __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a, int nb)
 {
 #pragma clang fp contract(off)
-    __shared__ double2 hb[3][81], sb[3][81];
     const int tid = threadIdx.x, N = a.n;
     const int it0 = blockIdx.y * kBuildItems, it1 = min(nb, it0 + kBuildItems);
     const int nup = (int)gridDim.x - kBuildPadWgs;
@@ -156,51 +155,81 @@ __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a, int nb)
     const int ia = blk - ja * (ja + 1) / 2;
     const int il = tid / 81, e = tid - 81 * il;          // il < 3 for the 243 working threads
     const int o = e % 9, b = e / 9;
-    for (int k0 = it0; k0 < it1; k0 += 3) {
-        const int kb = k0 + il;
-        const bool act = il < 3 && kb < it1;
-        const StructDev *SD = act ? &a.sd[a.item_s[kb]] : nullptr;
-        const bool inb = act && ja < SD->natoms;
-        const int r0 = inb ? SD->ksk[ia] : 0, c0 = inb ? SD->ksk[ja] : 0;
-        const int ni = inb ? SD->nbas[ia] : 0, nj = inb ? SD->nbas[ja] : 0;
-        double2 hv = make_double2(0.0, 0.0), sv = hv;
-        if (inb && o < ni && b < nj && (ia < ja || b >= o)) {
-            const int p0 = SD->blk_ptr[ia * SD->natoms + ja], p1 = SD->blk_ptr[ia * SD->natoms + ja + 1];
-            const double2 *ph = a.ph + (size_t)kb * a.maxpairs;
-            const double *eh = a.deriv ? SD->edh : SD->eh, *es = a.deriv ? SD->eds : SD->es;
-            const int idx = 9 * o + b;
-            double hr = 0.0, hi = 0.0, sr = 0.0, si = 0.0;
-            for (int p = p0; p < p1; ++p) {
-                const double2 z = ph[p];
-                const double e1 = eh[81 * (size_t)p + idx], e2 = es[81 * (size_t)p + idx];
-                hr = hr + e1 * z.x; hi = hi + e1 * z.y;
-                sr = sr + e2 * z.x; si = si + e2 * z.y;
-            }
-            if (ia == ja && b == o) {        // diagonal: real, plus on-site terms
-                const double *onsite = a.deriv ? SD->dparos : SD->paros;
-                const int cls = o == 0 ? 0 : (o <= 3 ? 1 : (o <= 6 ? 2 : 3));
-                hv = make_double2(hr + onsite[4 * ia + cls], 0.0);
-                sv = make_double2(sr + (a.deriv ? 0.0 : 1.0), 0.0);
-            } else {
-                hv = make_double2(hr, hi); sv = make_double2(sr, si);
-            }
-            double2 *H = a.h + (size_t)kb * a.stride, *S = a.s + (size_t)kb * a.stride;
-            H[(size_t)(c0 + b) * N + r0 + o] = hv;
-            S[(size_t)(c0 + b) * N + r0 + o] = sv;
+    if (il >= 3) return;                                  // no barriers below
+    // items it0 + il + 3u of this thread group; when they share one structure
+    // each pair's SK entries are loaded once for all of them
+    constexpr int U = kBuildItems / 3;
+    const int kf = it0 + il;
+    if (kf >= it1) return;
+    const int s0 = a.item_s[kf];
+    bool same = true;
+#pragma unroll
+    for (int u = 1; u < U; ++u) {
+        const int kb = kf + 3 * u;
+        if (kb < it1 && a.item_s[kb] != s0) same = false;
+    }
+    auto store = [&](int kb, const StructDev &SD, double hr, double hi, double sr, double si) {
+        const int r0 = SD.ksk[ia], c0 = SD.ksk[ja];
+        double2 hv, sv;
+        if (ia == ja && b == o) {            // diagonal: real, plus on-site terms
+            const double *onsite = a.deriv ? SD.dparos : SD.paros;
+            const int cls = o == 0 ? 0 : (o <= 3 ? 1 : (o <= 6 ? 2 : 3));
+            hv = make_double2(hr + onsite[4 * ia + cls], 0.0);
+            sv = make_double2(sr + (a.deriv ? 0.0 : 1.0), 0.0);
+        } else {
+            hv = make_double2(hr, hi); sv = make_double2(sr, si);
         }
-        if (il < 3) { hb[il][e] = hv; sb[il][e] = sv; }
-        __syncthreads();
-        // mirror: element (row c0 + bb, column r0 + oo) = conj(element (oo, bb))
-        {
-            const int bb = e % 9, oo = e / 9;
-            if (inb && oo < ni && bb < nj && (ia < ja || bb > oo)) {
-                const double2 h2 = hb[il][9 * bb + oo], s2 = sb[il][9 * bb + oo];
-                double2 *H = a.h + (size_t)kb * a.stride, *S = a.s + (size_t)kb * a.stride;
-                H[(size_t)(r0 + oo) * N + c0 + bb] = make_double2(h2.x, -h2.y);
-                S[(size_t)(r0 + oo) * N + c0 + bb] = make_double2(s2.x, -s2.y);
+        double2 *H = a.h + (size_t)kb * a.stride, *S = a.s + (size_t)kb * a.stride;
+        H[(size_t)(c0 + b) * N + r0 + o] = hv;
+        S[(size_t)(c0 + b) * N + r0 + o] = sv;
+        if (ia < ja || b > o) {              // conjugate partner below the diagonal
+            H[(size_t)(r0 + o) * N + c0 + b] = make_double2(hv.x, -hv.y);
+            S[(size_t)(r0 + o) * N + c0 + b] = make_double2(sv.x, -sv.y);
+        }
+    };
+    if (same) {
+        const StructDev &SD = a.sd[s0];
+        if (ja >= SD.natoms || o >= SD.nbas[ia] || b >= SD.nbas[ja] || (ia == ja && b < o)) return;
+        const int p0 = SD.blk_ptr[ia * SD.natoms + ja], p1 = SD.blk_ptr[ia * SD.natoms + ja + 1];
+        const double *eh = a.deriv ? SD.edh : SD.eh, *es = a.deriv ? SD.eds : SD.es;
+        const int idx = 9 * o + b;
+        double acc[U][4];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc[u][0] = acc[u][1] = acc[u][2] = acc[u][3] = 0.0;
+        for (int p = p0; p < p1; ++p) {
+            const double e1 = eh[81 * (size_t)p + idx], e2 = es[81 * (size_t)p + idx];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int kb = kf + 3 * u;
+                if (kb < it1) {
+                    const double2 z = a.ph[(size_t)kb * a.maxpairs + p];
+                    acc[u][0] = acc[u][0] + e1 * z.x; acc[u][1] = acc[u][1] + e1 * z.y;
+                    acc[u][2] = acc[u][2] + e2 * z.x; acc[u][3] = acc[u][3] + e2 * z.y;
+                }
             }
         }
-        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kb = kf + 3 * u;
+            if (kb < it1) store(kb, SD, acc[u][0], acc[u][1], acc[u][2], acc[u][3]);
+        }
+        return;
+    }
+    for (int kb = kf; kb < it1; kb += 3) {   // items of several structures: one at a time
+        const StructDev &SD = a.sd[a.item_s[kb]];
+        if (ja >= SD.natoms || o >= SD.nbas[ia] || b >= SD.nbas[ja] || (ia == ja && b < o)) continue;
+        const int p0 = SD.blk_ptr[ia * SD.natoms + ja], p1 = SD.blk_ptr[ia * SD.natoms + ja + 1];
+        const double *eh = a.deriv ? SD.edh : SD.eh, *es = a.deriv ? SD.eds : SD.es;
+        const int idx = 9 * o + b;
+        const double2 *ph = a.ph + (size_t)kb * a.maxpairs;
+        double hr = 0.0, hi = 0.0, sr = 0.0, si = 0.0;
+        for (int p = p0; p < p1; ++p) {
+            const double2 z = ph[p];
+            const double e1 = eh[81 * (size_t)p + idx], e2 = es[81 * (size_t)p + idx];
+            hr = hr + e1 * z.x; hi = hi + e1 * z.y;
+            sr = sr + e2 * z.x; si = si + e2 * z.y;
+        }
+        store(kb, SD, hr, hi, sr, si);
     }
 }
 
@@ -313,7 +342,7 @@ __global__ __launch_bounds__(256) void k_potrf_panel(int N, double2 *S, size_t s
 // 4 = only tiles touching the lower triangle of C are computed; 5 = as 4,
 // and C receives the lower triangle of the product and its conjugate
 // transpose above the diagonal (a Hermitian C from its lower half; alpha
-// real, beta 0).
+// real, beta 0); 6 = 5 with op(A) lower triangular as in 2.
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -353,7 +382,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
     const unsigned gxy = gridDim.x * gridDim.y;
     const unsigned lg = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y + gxy * blockIdx.z, gxy * gridDim.z);
     const int bm = (int)(lg % gridDim.x) * BM, bn = (int)((lg / gridDim.x) % gridDim.y) * BN, kb = (int)(lg / gxy);
-    if ((g.tri == 4 || g.tri == 5) && bm + BM <= bn) return;
+    if ((g.tri == 4 || g.tri == 5 || g.tri == 6) && bm + BM <= bn) return;
     const C *A = g.a + (size_t)kb * g.sa;
     const C *B = g.b + (size_t)kb * g.sb;
     C *Cm = g.c + (size_t)kb * g.sc;
@@ -361,7 +390,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
     const int wm = (wv & 1) * (BM / 2), wn = (wv >> 1) * (BN / 2);
     int kbeg = 0, kend = g.k;
     if (g.tri == 1) kbeg = bm;
-    if (g.tri == 2) kend = min(g.k, bm + BM);
+    if (g.tri == 2 || g.tri == 6) kend = min(g.k, bm + BM);
     if (g.tri == 3) kend = min(g.k, bn + BN);
     kbeg = (kbeg / BK) * BK;
     v4 acc1[MI][NJ], acc2[MI][NJ], acc3[MI][NJ];
@@ -461,7 +490,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
                     const C v = cmake<C, R>(p1 - p2, acc3[i][j][q] - p1 - p2);
                     C out = zmul(g.alpha, v);
                     C *p = Cm + (size_t)c * g.ldc + r;
-                    if (g.tri == 5) {
+                    if (g.tri == 5 || g.tri == 6) {
                         // Hermitian from the lower half: element (r, c), r >= c,
                         // and its conjugate at (c, r); 16 lanes of one row write
                         // 16 consecutive rows of column r

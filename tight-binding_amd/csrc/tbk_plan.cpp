@@ -352,7 +352,7 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
     tm.end();
     tm.begin("reduce_gemm");
     tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.S, n, ss, ws.Y, n, ss, 3), 0, 0, nb, st);
-    tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 2), 1, 0, nb, st);
+    tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 6), 1, 0, nb, st);
     tm.end();
     tm.begin("hetrd_update");
     tbk::launch_c64_to_c32(ss * nb, ws.A, ws.A32, st);
@@ -414,7 +414,7 @@ void solve_batch(Workspace &ws, int nb, const int *nact, double *w_out, hipStrea
     tm.begin("reduce_gemm");
     // Y = H Uinv (Uinv upper) ; A = Uinv^H Y
     tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.S, n, ss, ws.Y, n, ss, 3), 0, 0, nb, st);
-    tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 2), 1, 0, nb, st);
+    tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 6), 1, 0, nb, st);
     tm.end();
     hetrd(ws, nb, st, tm);
     tm.begin("stein");
@@ -1101,7 +1101,7 @@ extern "C" int tbk_debug_stages(int device, int n, const double *h, const double
     (void)hipDeviceSynchronize();
     get(uinv, ws.S, ss * sizeof(double2));
     tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.S, n, ss, ws.Y, n, ss, 3), 0, 0, 1, st);
-    tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 2), 1, 0, 1, st);
+    tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 6), 1, 0, 1, st);
     (void)hipDeviceSynchronize();
     get(c, ws.A, ss * sizeof(double2));
     {
@@ -1166,7 +1166,7 @@ extern "C" int tbk_debug_zgemm(int device, int precision, int tile, int m, int n
                                int tri, const void *a, int lda, const void *b, int ldb, void *c, int ldc)
 {
     if (tile >= tbk::kGemmTiles || opa < 0 || opa > 1 || opb < 0 || opb > 1) return TBK_ERR_ARG;
-    if (tri < 0 || tri > 5 || (tri == 5 && m != n) || m <= 0 || n <= 0 || k <= 0) return TBK_ERR_ARG;
+    if (tri < 0 || tri > 6 || (tri >= 5 && m != n) || m <= 0 || n <= 0 || k <= 0) return TBK_ERR_ARG;
     HIP_TRY(hipSetDevice(device));
     if (precision == TBK_MIXED)
         return debug_gemm_t<float2, tbk::GemmArgsT<float>>(tile, m, n, k, opa, opb, tri, a, lda, b, ldb, c, ldc);

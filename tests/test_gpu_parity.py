@@ -25,9 +25,11 @@ def full(o, packed, n):
     return o.unpack(n, packed)
 
 
-def compare_bande(st, k, w, mode=4, tkb=0.005, eigcut=10.0, batch=0):
-    print(f"case: natoms={st.natoms} N={st.nv} nk={len(k)} mode={mode} batch={batch}", flush=True)
+def compare_bande(st, k, w, mode=4, tkb=0.005, eigcut=10.0, batch=0, window=True):
+    print(f"case: natoms={st.natoms} N={st.nv} nk={len(k)} mode={mode} batch={batch} window={window}",
+          flush=True)
     plan = Plan(st, k, w, batch=batch)
+    plan.set_window(window)
     r = plan.run(mode, tkb, eigcut)
     e, d = plan.eigenvalues()
     plan.close()
@@ -43,7 +45,10 @@ def compare_bande(st, k, w, mode=4, tkb=0.005, eigcut=10.0, batch=0):
     assert dE < E_TOL
     if mode in (2, 4):
         dP = abs(r["earg2"] - ref["earg2"]) * S.RYBOHR3_KBAR
-        dd = float(np.abs(d - ref["desk"]).max())
+        # with the occupation window (one batch) derivatives are formed only
+        # below E_F + 20 kT, the states the sums use; zero above
+        used = ref["eig"] < r["efermi"] + 20 * tkb if (window and batch == 0) else np.ones(d.shape, bool)
+        dd = float(np.abs(d - ref["desk"])[used].max())
         print(f"  dP={dP:.2e} kbar max|d desk|={dd:.2e}", flush=True)
         assert dP < P_TOL_KBAR
         assert dd < 1e-8
@@ -121,6 +126,7 @@ def test_c1_full_grid_all_modes():
     w = np.full(len(k), 1.0 / len(k))
     for mode in (4, 1, 2, 3):
         compare_bande(st, k, w, mode=mode)
+    compare_bande(st, k, w, mode=4, window=False)      # every derivative against the oracle
 
 
 def test_c1_odd_grid_degenerate_gamma():
@@ -176,9 +182,16 @@ def test_batch_invariance():
         plan = Plan(st, k, w, batch=b)
         outs.append((plan.run(4), plan.eigenvalues()))
         plan.close()
+    # one batch uses the occupation window: derivatives agree where formed
+    used = outs[0][1][0] < outs[0][0]["efermi"] + 20 * 0.005
     for r, (e, d) in outs[1:]:
         assert r == outs[0][0]
-        assert np.array_equal(e, outs[0][1][0]) and np.array_equal(d, outs[0][1][1])
+        assert np.array_equal(e, outs[0][1][0]) and np.array_equal(d[used], outs[0][1][1][used])
+    plan = Plan(st, k, w, batch=0)
+    plan.set_window(False)
+    assert plan.run(4) == outs[0][0]
+    assert np.array_equal(plan.eigenvalues()[1], outs[1][1][1])
+    plan.close()
 
 
 def test_indefinite_overlap_structure_is_an_error():
@@ -206,6 +219,7 @@ def test_mixed_precision_error_vs_oracle(cfg):
     w = np.full(len(k), 1.0 / len(k))
     print(f"mixed case {cfg}: N={st.nv} nk={len(k)}", flush=True)
     plan = Plan(st, k, w, precision=_lib.TBK_MIXED)
+    plan.set_window(False)            # every eigenvector, for the orthogonality check
     r = plan.run(4, 0.005, 10.0)
     e, d = plan.eigenvalues()
     z0 = plan.eigenvectors(0)

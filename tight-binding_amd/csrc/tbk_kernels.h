@@ -51,6 +51,8 @@ struct GemmArgsT {
                                    // 5 lower triangle of C mirrored into the upper
                                    //   (Hermitian C; alpha real, beta 0); 6 = 5 with
                                    //   A lower triangular as in 2
+    const int *ncol = nullptr;     // [nb] per item: column tiles at or beyond ncol[k]
+                                   // are skipped (C left as it was); null: all
 };
 typedef GemmArgsT<double> GemmArgs;
 
@@ -84,6 +86,7 @@ struct SteinArgs {
     double *w, *v, *scratch;       // w [nb][n]; v [nb][n*n]; scratch [nb][5*n*stein_threads(n)]
     int *info;
     double abstol_rel = 0.0;       // > 0: bisection stops at abstol_rel * ||T||_1 (else dstebz's)
+    const int *cut = nullptr;      // [nb] vectors j >= cut[k] are set to zero, not formed; null: all
 };
 
 struct PerturbArgs {
@@ -95,6 +98,7 @@ struct PerturbArgs {
     const double2 *px, *py, *z, *x, *y;
     double *desk;
     int *ncut, *err;               // ncut [k0+nb], err [k0+nb] (2: > 9 degenerate)
+    const int *cut = nullptr;      // [nb] derivatives only below cut[k] (0 above); null: all
 };
 
 struct FermiArgs {
@@ -103,6 +107,7 @@ struct FermiArgs {
     const double *esk, *desk, *w;
     const int *ncut;
     double *out;                   // efermi, earg1, earg2, esum, entropy, iters
+    int *cut = nullptr;            // [nk] occupation window per k (k_fermi_window)
 };
 
 // GEMM tiles and their choice (tbk_tune.cpp)
@@ -141,7 +146,9 @@ void launch_hetrd_wide_vt(const HetrdArgs &h, int i0, int nb, hipStream_t st);
 void launch_hetrd_wide_vt(const HetrdArgsT<float> &h, int i0, int nb, hipStream_t st);
 void launch_hetrd_finish(const HetrdArgsT<float> &h, int nb, hipStream_t st);
 int stein_threads(int n);        // eigenvector threads per k-point
-void launch_stein(const SteinArgs &s, int nb, hipStream_t st);
+void launch_stein(const SteinArgs &s, int nb, hipStream_t st);          // eigenvalues and vectors
+void launch_stebz(const SteinArgs &s, int nb, hipStream_t st);          // eigenvalues only
+void launch_stein_vectors(const SteinArgs &s, int nb, hipStream_t st);  // vectors of known eigenvalues
 void launch_real_to_complex_t(int n, const double *vt, double2 *y, size_t stride, int nb,
                               hipStream_t st);
 void launch_c64_to_c32(size_t count, const double2 *x, float2 *y, hipStream_t st);
@@ -152,10 +159,14 @@ int oa_max_n();                  // largest N of launch_oa_update
 void launch_oa_update(int n, const int *nact, const double2 *t, const double2 *g, double2 *f, size_t stride,
                       double *lam, int *perm, double *delta, double *w, int nb, hipStream_t st);
 void launch_coldot(int n, const double2 *z, const double2 *x, const double2 *y, size_t stride,
-                   double2 *px, double2 *py, int nb, hipStream_t st);
+                   double2 *px, double2 *py, int nb, hipStream_t st, const int *cut = nullptr);
 void launch_perturb(const PerturbArgs &p, hipStream_t st);
 void launch_fermi_sums(const FermiArgs &f, hipStream_t st);
 void launch_fermi_sums_group(const FermiArgs *f_dev, int nstruct, hipStream_t st);   // one WG each
+// E_F of every structure and the cut of each k: the first state at or above
+// E_F + 20 kT, moved up to the next spectral gap (no state above it enters
+// the Fermi sums; no group of close eigenvalues straddles it).
+void launch_fermi_window_group(const FermiArgs *f_dev, int nstruct, hipStream_t st);
 void launch_copy_int(const int *src, int *dst, int n, hipStream_t st);
 void launch_copy16(const double2 *src, double2 *dst, size_t n, hipStream_t st);
 

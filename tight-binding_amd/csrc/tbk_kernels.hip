@@ -383,6 +383,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
     const unsigned lg = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y + gxy * blockIdx.z, gxy * gridDim.z);
     const int bm = (int)(lg % gridDim.x) * BM, bn = (int)((lg / gridDim.x) % gridDim.y) * BN, kb = (int)(lg / gxy);
     if ((g.tri == 4 || g.tri == 5 || g.tri == 6) && bm + BM <= bn) return;
+    if (g.ncol && bn >= g.ncol[kb]) return;             // columns outside the item's window
     const C *A = g.a + (size_t)kb * g.sa;
     const C *B = g.b + (size_t)kb * g.sb;
     C *Cm = g.c + (size_t)kb * g.sc;
@@ -1250,7 +1251,9 @@ __global__ __launch_bounds__(64) void k_stein_vec(SteinArgs s)
     int fails = 0;
     const int j = gt;
     const double dtol = 1e-10 * onenrm;
-    if (j < N) {
+    if (s.cut && j < N && j >= s.cut[kb]) {               // outside the window: zero vector
+        for (int i = 0; i < N; ++i) Vt[(size_t)i * LD + j] = 0.0;
+    } else if (j < N) {
         if (j == 0 || w[j] - w[j - 1] > dtol) {
             // D+ (top-down) into scratch; U_i = e_i / D-_{i+1} (bottom-up)
             // into slot i+1 of the output column; z overwrites both.
@@ -1461,10 +1464,11 @@ __global__ __launch_bounds__(256) void k_stein_orth(SteinArgs s)
     __syncthreads();
     const TriBounds tb = tri_bounds(d, e, e2, N, red4);
     const double ortol = s.ortol_rel * tb.onenrm;
+    const int Nv = s.cut ? min(N, s.cut[kb]) : N;         // vectors formed
     int j0 = 0;
-    while (j0 < N) {
+    while (j0 < Nv) {
         int j1 = j0 + 1;
-        while (j1 < N && w[j1] - w[j1 - 1] <= ortol) ++j1;
+        while (j1 < Nv && w[j1] - w[j1 - 1] <= ortol) ++j1;
         for (int j = j0 + 1; j < j1; ++j) {
             const int g = j - j0;
             for (int pass = 0; pass < 2; ++pass) {
@@ -1650,12 +1654,12 @@ __global__ __launch_bounds__(256) void k_oa_fmat(int N, const double2 *T, const 
 // Column dots: px[j] = z_j^H X[:, j], py[j] = z_j^H Y[:, j].
 __global__ __launch_bounds__(256) void k_coldot(int N, const double2 *Z, const double2 *X,
                                                 const double2 *Y, size_t stride, double2 *px,
-                                                double2 *py)
+                                                double2 *py, const int *cut)
 {
     const int kb = blockIdx.y;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int j = blockIdx.x * 4 + wv;
-    if (j >= N) return;
+    if (j >= N || (cut && j >= cut[kb])) return;
     const double2 *z = Z + (size_t)kb * stride + (size_t)j * N;
     const double2 *x = X + (size_t)kb * stride + (size_t)j * N;
     const double2 *y = Y + (size_t)kb * stride + (size_t)j * N;
@@ -1734,6 +1738,7 @@ __global__ void k_perturb(PerturbArgs p)
     const double2 *Z = p.z + (size_t)kb * p.stride;
     const double2 *X = p.x + (size_t)kb * p.stride;
     const double2 *Y = p.y + (size_t)kb * p.stride;
+    const int wcut = p.cut ? p.cut[kb] : N;           // derivatives formed below wcut
     int start = 0, err = 0, ncut = N;
     while (start < N) {
         if (w[start] > p.eigcut) { ncut = start; break; }
@@ -1742,7 +1747,11 @@ __global__ void k_perturb(PerturbArgs p)
             if (fabs(w[j] - w[start]) > 1e-8) { num = j - start; break; }
         if (num > 9) { err = 1; ncut = start; break; }
         const double es = w[start];
-        if (num == 1) {
+        if (start >= wcut) {
+            // above the occupation window: never summed; a group cannot
+            // straddle wcut (it lies at a gap wider than the grouping tolerance)
+            for (int j = start; j < start + num; ++j) desk[j] = 0.0;
+        } else if (num == 1) {
             desk[start] = px[start].x - es * py[start].x;
         } else {
             double2 hd[81];
@@ -1813,9 +1822,11 @@ __device__ double block_sum_tree(double v, double *sbuf)
     return r;
 }
 
-__device__ void fermi_sums_body(const FermiArgs &f)
+// E_F of one structure (fermi.f:197-300): bracket, then Newton steps
+// safeguarded by bisection on the electron count; every thread of the
+// workgroup returns the same value.
+__device__ double fermi_level(const FermiArgs &f, double *s1, double *s2, int &iters_out)
 {
-    __shared__ double s1[1024], s2[1024];
     __shared__ double mu_s;
     const int tid = threadIdx.x;
     const int nk = f.nk, nv = f.nv, ld = f.ld;
@@ -1887,7 +1898,19 @@ __device__ void fermi_sums_body(const FermiArgs &f)
     }
     if (tid == 0) mu_s = mu;
     __syncthreads();
-    const double efermi = mu_s;
+    iters_out = iters;
+    return mu_s;
+}
+
+__device__ void fermi_sums_body(const FermiArgs &f)
+{
+    __shared__ double s1[1024], s2[1024];
+    const int tid = threadIdx.x;
+    const int nk = f.nk, nv = f.nv, ld = f.ld;
+    const double el = f.el, tkb = f.tkb;
+    const double nearz = 1e-5, twenty = 20.0;
+    int iters = 0;
+    const double efermi = fermi_level(f, s1, s2, iters);
     // sums (bande.f:1170-1321)
     const double tkbi = (tkb < nearz) ? 1.0 / nearz : 1.0 / tkb;
     const double etol = twenty * tkb, efp = efermi + etol, efm = efermi - etol;
@@ -1951,6 +1974,30 @@ __device__ void fermi_sums_body(const FermiArgs &f)
 __global__ __launch_bounds__(1024) void k_fermi_sums(FermiArgs f) { fermi_sums_body(f); }
 // one workgroup per structure of a plan
 __global__ __launch_bounds__(1024) void k_fermi_sums_group(const FermiArgs *f) { fermi_sums_body(f[blockIdx.x]); }
+
+// Occupation window of every k of one structure: the sums above skip every
+// state at or above E_F + 20 kT, so eigenvectors and derivatives are needed
+// only below the first such state.  The cut is moved up to the next gap
+// wider than 3e-6 max|e| (and 1e-8): no group of degenerate states
+// (perturb's 1e-8) or of close eigenvalues (the eigenvector
+// orthogonalisation's 1e-6 ||T||_1, ||T||_1 <= 3 max|e|) straddles it, so
+// every quantity below the cut is formed exactly as without the window.
+__global__ __launch_bounds__(1024) void k_fermi_window_group(const FermiArgs *fa)
+{
+    __shared__ double s1[1024], s2[1024];
+    const FermiArgs &f = fa[blockIdx.x];
+    int iters = 0;
+    const double ef = fermi_level(f, s1, s2, iters);
+    const double efp = ef + 20.0 * f.tkb;
+    for (int k = threadIdx.x; k < f.nk; k += blockDim.x) {
+        const double *ev = f.esk + (size_t)k * f.ld;
+        const double gap = fmax(1e-8, 3e-6 * fmax(fabs(ev[0]), fabs(ev[f.nv - 1])));
+        int c = 0;
+        while (c < f.nv && ev[c] < efp) ++c;
+        while (c > 0 && c < f.nv && !(ev[c] - ev[c - 1] > gap)) ++c;
+        f.cut[k] = c;
+    }
+}
 
 // Plain streaming copy (16-byte elements), the calibration kernel for
 // counter-based traffic measurements.
@@ -2090,11 +2137,19 @@ void launch_hetrd_finish(const HetrdArgs &h, int nb, hipStream_t st) { launch_he
 void launch_hetrd_panel(const HetrdArgsT<float> &h, int i0, int nb, hipStream_t st) { launch_hetrd_panel_t(h, i0, nb, st); }
 void launch_hetrd_finish(const HetrdArgsT<float> &h, int nb, hipStream_t st) { launch_hetrd_finish_t(h, nb, st); }
 int stein_threads(int n) { return (n + 63) / 64 * 64; }
-void launch_stein(const SteinArgs &s, int nb, hipStream_t st)
+void launch_stebz(const SteinArgs &s, int nb, hipStream_t st)
 {
     hipLaunchKernelGGL(k_stebz, dim3((s.n + 255) / 256, nb), dim3(256), 0, st, s);
+}
+void launch_stein_vectors(const SteinArgs &s, int nb, hipStream_t st)
+{
     hipLaunchKernelGGL(k_stein_vec, dim3((s.n + 63) / 64, nb), dim3(64), 4 * (size_t)s.n * sizeof(double), st, s);
     hipLaunchKernelGGL(k_stein_orth, dim3(nb), dim3(256), 0, st, s);
+}
+void launch_stein(const SteinArgs &s, int nb, hipStream_t st)
+{
+    launch_stebz(s, nb, st);
+    launch_stein_vectors(s, nb, st);
 }
 void launch_real_to_complex_t(int N, const double *Vt, double2 *Y, size_t stride, int nb,
                               hipStream_t st)
@@ -2125,13 +2180,17 @@ void launch_oa_update(int n, const int *nact, const double2 *t, const double2 *g
                        g, f, stride, lam, perm, delta);
 }
 void launch_coldot(int N, const double2 *Z, const double2 *X, const double2 *Y, size_t stride,
-                   double2 *px, double2 *py, int nb, hipStream_t st)
+                   double2 *px, double2 *py, int nb, hipStream_t st, const int *cut)
 {
-    hipLaunchKernelGGL(k_coldot, dim3((N + 3) / 4, nb), dim3(256), 0, st, N, Z, X, Y, stride, px, py);
+    hipLaunchKernelGGL(k_coldot, dim3((N + 3) / 4, nb), dim3(256), 0, st, N, Z, X, Y, stride, px, py, cut);
 }
 void launch_perturb(const PerturbArgs &p, hipStream_t st)
 {
     hipLaunchKernelGGL(k_perturb, dim3((p.nb + 63) / 64), dim3(64), 0, st, p);
+}
+void launch_fermi_window_group(const FermiArgs *f_dev, int nstruct, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fermi_window_group, dim3(nstruct), dim3(1024), 0, st, f_dev);
 }
 void launch_fermi_sums_group(const FermiArgs *f_dev, int nstruct, hipStream_t st)
 {

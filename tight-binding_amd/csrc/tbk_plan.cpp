@@ -159,6 +159,22 @@ tbk::GemmArgs gemm(int m, int n, int k, const double2 *a, int lda, size_t sa, co
     return g;
 }
 
+template <class G>
+G with_ncol(G g, const int *ncol)
+{
+    g.ncol = ncol;
+    return g;
+}
+
+// Occupation window of a run (all null/0 when off): the per-structure Fermi
+// arguments from which k_fermi_window sets cut[k], the number of columns
+// (eigenvectors) each item needs.
+struct Window {
+    const tbk::FermiArgs *fargs;
+    int nstruct;
+    int *cut;
+};
+
 struct StageTimer {
     bool on = false;
     hipStream_t st = nullptr;
@@ -309,7 +325,7 @@ constexpr int kBT = 3;
 template <class R>
 void apply_q(int n, int nb, typename tbk::cplx_t<R>::type *A, typename tbk::cplx_t<R>::type *VT,
              typename tbk::cplx_t<R>::type *Y, typename tbk::cplx_t<R>::type *X, typename tbk::cplx_t<R>::type *M,
-             size_t sm, hipStream_t st)
+             size_t sm, hipStream_t st, const int *ncol = nullptr)
 {
     const size_t ss = (size_t)n * n;
     const int P = (n - 2) / kNB + 1;                  // panels
@@ -328,8 +344,10 @@ void apply_q(int n, int nb, typename tbk::cplx_t<R>::type *A, typename tbk::cplx
         }
         const int r0 = c0 + 1, m = n - r0;
         // X[0:W, :] = (V_c T_c^H)^H Y[r0:, :];  Y[r0:, :] -= V_c X
-        launch_gemm_any(gemm_t<R>(W, n, m, VT + (size_t)c0 * n + r0, n, ss, Y + r0, n, ss, X, n, ss), 1, 0, nb, st);
-        launch_gemm_any(gemm_t<R>(m, n, W, A + (size_t)c0 * n + r0, n, ss, X, n, ss, Y + r0, n, ss, R(-1), R(1)),
+        launch_gemm_any(with_ncol(gemm_t<R>(W, n, m, VT + (size_t)c0 * n + r0, n, ss, Y + r0, n, ss, X, n, ss), ncol),
+                        1, 0, nb, st);
+        launch_gemm_any(with_ncol(gemm_t<R>(m, n, W, A + (size_t)c0 * n + r0, n, ss, X, n, ss, Y + r0, n, ss, R(-1),
+                                            R(1)), ncol),
                         0, 0, nb, st);
     }
 }
@@ -343,7 +361,8 @@ void apply_q(int n, int nb, typename tbk::cplx_t<R>::type *A, typename tbk::cplx
 // only seed the vectors that the FP64 refinement corrects.
 constexpr double kMixedBisectTol = 1e-11;
 
-void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hipStream_t st, StageTimer &tm)
+void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hipStream_t st, StageTimer &tm,
+                       const Window &wnd)
 {
     const int n = ws.n;
     const size_t ss = ws.stride(), sp = (size_t)n * 96;
@@ -397,14 +416,18 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
     tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.Y, n, ss, ws.VT, n, ss, 5), 1, 0, nb, st);
     // X = (I + E) with sorted columns; VT = Y X; Z = U^{-1} VT
     tbk::launch_oa_update(n, nact, ws.Z, ws.VT, ws.X, ss, ws.lam, ws.perm, ws.delta, w_out, nb, st);
-    tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.VT, n, ss), 0, 0, nb, st);
-    tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.VT, n, ss, ws.Z, n, ss, 1), 0, 0, nb, st);
+    // the eigenvalues are final: occupation window, then only the columns
+    // it keeps of Y F and U^{-1} (Y F)
+    if (wnd.cut) tbk::launch_fermi_window_group(wnd.fargs, wnd.nstruct, st);
+    tbk::launch_zgemm(with_ncol(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.VT, n, ss), wnd.cut), 0, 0, nb, st);
+    tbk::launch_zgemm(with_ncol(gemm(n, n, n, ws.S, n, ss, ws.VT, n, ss, ws.Z, n, ss, 1), wnd.cut), 0, 0, nb, st);
     tm.end();
 }
 
 // Stages 2-9 of the pipeline on ws.A = H, ws.S = S for nb k-points:
 // eigenvalues into w_out[nb*n], eigenvectors into ws.Z.
-void solve_batch(Workspace &ws, int nb, const int *nact, double *w_out, hipStream_t st, StageTimer &tm)
+void solve_batch(Workspace &ws, int nb, const int *nact, double *w_out, hipStream_t st, StageTimer &tm,
+                 const Window &wnd)
 {
     const int n = ws.n;
     const size_t ss = ws.stride();
@@ -419,13 +442,17 @@ void solve_batch(Workspace &ws, int nb, const int *nact, double *w_out, hipStrea
     hetrd(ws, nb, st, tm);
     tm.begin("stein");
     tbk::SteinArgs sa{n, kOrtolRel, nact, ws.d, ws.e, w_out, ws.Vt, ws.scratch, ws.sinfo};
-    tbk::launch_stein(sa, nb, st);
+    tbk::launch_stebz(sa, nb, st);
+    // the eigenvalues are final: occupation window, then only the vectors it keeps
+    if (wnd.cut) tbk::launch_fermi_window_group(wnd.fargs, wnd.nstruct, st);
+    sa.cut = wnd.cut;
+    tbk::launch_stein_vectors(sa, nb, st);
     tm.end();
     tm.begin("backtransform");
     tbk::launch_real_to_complex_t(n, ws.Vt, ws.Y, ss, nb, st);
-    apply_q<double>(n, nb, ws.A, ws.VT, ws.Y, ws.X, ws.Vp, (size_t)n * 96, st);
+    apply_q<double>(n, nb, ws.A, ws.VT, ws.Y, ws.X, ws.Vp, (size_t)n * 96, st, wnd.cut);
     // Z = Uinv Y
-    tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.Z, n, ss, 1), 0, 0, nb, st);
+    tbk::launch_zgemm(with_ncol(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.Z, n, ss, 1), wnd.cut), 0, 0, nb, st);
     tm.end();
 }
 
@@ -455,6 +482,8 @@ struct tbk_plan {
     double *esk = nullptr, *desk = nullptr, *out = nullptr;   // out [nstruct][8]
     double2 *ph = nullptr;          // Bloch phases of the current batch
     int *ncut = nullptr, *err = nullptr, *info_all = nullptr;
+    int *cut = nullptr;             // [nk] occupation window of the last windowed run
+    bool window = true, win_last = false;
     std::vector<int> info_host, err_host, status;
     std::vector<double> out_host;
     Workspace ws;
@@ -466,6 +495,7 @@ struct tbk_plan {
     hipGraphExec_t gexec = nullptr;
     int g_mode = 0;
     double g_tkb = 0.0, g_eigcut = 0.0;
+    bool g_window = true;
     int f_mode = 0;                 // (mode, kT) of the device Fermi arguments
     double f_tkb = 0.0;
 };
@@ -481,6 +511,7 @@ static void plan_free(tbk_plan *p)
     dfree(p->paros); dfree(p->dparos); dfree(p->kp); dfree(p->wts);
     dfree(p->sdev); dfree(p->fargs); dfree(p->item_s); dfree(p->nact);
     dfree(p->esk); dfree(p->desk); dfree(p->out); dfree(p->ncut); dfree(p->err); dfree(p->info_all);
+    dfree(p->cut);
     dfree(p->ph);
     p->ws.release();
     if (p->stream) {
@@ -628,6 +659,7 @@ extern "C" int tbk_plan_create_group(int device, int nstruct, const tbk_structur
     rc |= upload(&p->item_s, item_s); rc |= upload(&p->nact, nact);
     rc |= dalloc(&p->esk, (size_t)nktot * N); rc |= dalloc(&p->desk, (size_t)nktot * N);
     rc |= dalloc(&p->ncut, (size_t)nktot); rc |= dalloc(&p->err, (size_t)nktot);
+    rc |= dalloc(&p->cut, (size_t)nktot);
     rc |= dalloc(&p->info_all, (size_t)nktot); rc |= dalloc(&p->out, 8 * (size_t)nstruct);
     if (!rc) {
         std::vector<tbk::StructDev> sd(nstruct);
@@ -698,6 +730,12 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
     const int n = p->N;
     const size_t ss = p->ws.stride();
     hipStream_t st = p->stream;
+    // Occupation window: with every k in one batch, E_F is known once the
+    // eigenvalues are, and eigenvectors and derivatives are formed only for
+    // the states the Fermi sums use (below E_F + 20 kT).
+    const bool win = p->window && lpress && p->batch >= p->nk;
+    const Window wnd{win ? p->fargs : nullptr, p->nstruct, win ? p->cut : nullptr};
+    p->win_last = win;
     for (int k0 = 0; k0 < p->nk; k0 += p->batch) {
         const int nb = std::min(p->batch, p->nk - k0);
         p->timer.begin("build_hs");
@@ -706,17 +744,20 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
         p->timer.end();
         const int *nact = p->nact + k0;
         if (p->precision == TBK_MIXED)
-            solve_batch_mixed(p->ws, nb, nact, p->esk + (size_t)k0 * n, st, p->timer);
+            solve_batch_mixed(p->ws, nb, nact, p->esk + (size_t)k0 * n, st, p->timer, wnd);
         else
-            solve_batch(p->ws, nb, nact, p->esk + (size_t)k0 * n, st, p->timer);
+            solve_batch(p->ws, nb, nact, p->esk + (size_t)k0 * n, st, p->timer, wnd);
         tbk::launch_copy_int(p->ws.info, p->info_all + k0, nb, st);
         if (lpress) {
             p->timer.begin("pressure");
             tbk::launch_build_hs(build_args(p, k0, true), p->maxnat, nb, st);
-            tbk::launch_zgemm(gemm(n, n, n, p->ws.A, n, ss, p->ws.Z, n, ss, p->ws.X, n, ss), 0, 0, nb, st);
-            tbk::launch_zgemm(gemm(n, n, n, p->ws.S, n, ss, p->ws.Z, n, ss, p->ws.Y, n, ss), 0, 0, nb, st);
-            tbk::launch_coldot(n, p->ws.Z, p->ws.X, p->ws.Y, ss, p->ws.px, p->ws.py, nb, st);
+            tbk::launch_zgemm(with_ncol(gemm(n, n, n, p->ws.A, n, ss, p->ws.Z, n, ss, p->ws.X, n, ss), wnd.cut), 0, 0,
+                              nb, st);
+            tbk::launch_zgemm(with_ncol(gemm(n, n, n, p->ws.S, n, ss, p->ws.Z, n, ss, p->ws.Y, n, ss), wnd.cut), 0, 0,
+                              nb, st);
+            tbk::launch_coldot(n, p->ws.Z, p->ws.X, p->ws.Y, ss, p->ws.px, p->ws.py, nb, st, wnd.cut);
             tbk::PerturbArgs pa;
+            pa.cut = wnd.cut;
             pa.n = n; pa.nb = nb; pa.k0 = k0; pa.stride = ss; pa.eigcut = eigcut; pa.nact = nact;
             pa.w = p->esk + (size_t)k0 * n; pa.px = p->ws.px; pa.py = p->ws.py;
             pa.z = p->ws.Z; pa.x = p->ws.X; pa.y = p->ws.Y;
@@ -737,7 +778,8 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
         if (h.zfill <= 0.0) return TBK_ERR_ZFILL;
     HIP_TRY(hipSetDevice(p->device));
     p->last_mode = mode;
-    const bool params_changed = !(p->g_mode == mode && p->g_tkb == tkb && p->g_eigcut == eigcut);
+    const bool params_changed =
+        !(p->g_mode == mode && p->g_tkb == tkb && p->g_eigcut == eigcut && p->g_window == p->window);
     if (p->f_mode != mode || p->f_tkb != tkb) {
         // per-structure Fermi arguments of this (mode, kT)
         std::vector<tbk::FermiArgs> fa(p->nstruct);
@@ -748,6 +790,7 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
             f.el = 2.0; f.tkb = tkb; f.zfill = h.zfill;
             f.esk = p->esk + (size_t)h.k0 * p->N; f.desk = p->desk + (size_t)h.k0 * p->N;
             f.w = p->wts + h.k0; f.ncut = p->ncut + h.k0; f.out = p->out + 8 * (size_t)i;
+            f.cut = p->cut + h.k0;
         }
         HIP_TRY(hipMemcpyAsync(p->fargs, fa.data(), fa.size() * sizeof(tbk::FermiArgs),
                                hipMemcpyHostToDevice, p->stream));
@@ -758,7 +801,7 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
     if (!graph) {
         enqueue_run(p, mode, eigcut);
         if (hipGetLastError() != hipSuccess) return TBK_ERR_HIP;
-        p->g_mode = mode; p->g_tkb = tkb; p->g_eigcut = eigcut;
+        p->g_mode = mode; p->g_tkb = tkb; p->g_eigcut = eigcut; p->g_window = p->window;
         if (p->gexec && params_changed) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
         return TBK_OK;
     }
@@ -772,7 +815,7 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
         const hipError_t ie = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (ie != hipSuccess) { p->gexec = nullptr; return TBK_ERR_HIP; }
-        p->g_mode = mode; p->g_tkb = tkb; p->g_eigcut = eigcut;
+        p->g_mode = mode; p->g_tkb = tkb; p->g_eigcut = eigcut; p->g_window = p->window;
     }
     HIP_TRY(hipGraphLaunch(p->gexec, p->stream));
     return TBK_OK;
@@ -943,8 +986,21 @@ extern "C" int tbk_plan_eigenvectors(tbk_plan *p, int k, double *z)
     HIP_TRY(hipSetDevice(p->device));
     HIP_TRY(hipStreamSynchronize(p->stream));
     const size_t ss = p->ws.stride();
-    return copy_leading((double2 *)z, p->ws.Z + (size_t)(k - last0) * ss, p->N, p->sh[p->item_s_host[k]].nv,
-                        p->stream);
+    const int nv = p->sh[p->item_s_host[k]].nv;
+    int rc = copy_leading((double2 *)z, p->ws.Z + (size_t)(k - last0) * ss, p->N, nv, p->stream);
+    if (rc == TBK_OK && p->win_last) {       // columns beyond the occupation window were not formed
+        int c = nv;
+        if (hipMemcpy(&c, p->cut + k, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return TBK_ERR_HIP;
+        for (size_t t = (size_t)std::max(0, std::min(c, nv)) * nv * 2; t < (size_t)nv * nv * 2; ++t) z[t] = 0.0;
+    }
+    return rc;
+}
+
+extern "C" int tbk_plan_set_window(tbk_plan *p, int on)
+{
+    if (!p) return TBK_ERR_ARG;
+    p->window = on != 0;
+    return TBK_OK;
 }
 
 extern "C" int tbk_plan_hs(tbk_plan *p, int k, int deriv, double *h, double *s)
@@ -1008,7 +1064,7 @@ extern "C" int tbk_zhegv(int device, int n, const double *h, const double *s, do
         hipMemcpyAsync(ws.S, s, ss * sizeof(double2), hipMemcpyHostToDevice, st) != hipSuccess) {
         rc = TBK_ERR_HIP;
     } else {
-        solve_batch(ws, 1, nullptr, ws.w, st, tm);
+        solve_batch(ws, 1, nullptr, ws.w, st, tm, Window{nullptr, 0, nullptr});
         if (hipMemcpyAsync(w, ws.w, n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipMemcpyAsync(&info, ws.info, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
             (z && hipMemcpyAsync(z, ws.Z, ss * sizeof(double2), hipMemcpyDeviceToHost, st) != hipSuccess) ||
@@ -1127,7 +1183,7 @@ extern "C" int tbk_debug_stages(int device, int n, const double *h, const double
         return TBK_ERR_HIP;
     }
     StageTimer tm;
-    solve_batch(ws, 1, nullptr, ws.w, st, tm);
+    solve_batch(ws, 1, nullptr, ws.w, st, tm, Window{nullptr, 0, nullptr});
     (void)hipDeviceSynchronize();
     get(z, ws.Z, ss * sizeof(double2));
     ws.release();

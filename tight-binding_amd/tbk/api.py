@@ -122,6 +122,11 @@ class Plan:
     def set_graph(self, on: bool = True):
         check(lib().tbk_plan_set_graph(self.h, int(on)), "tbk_plan_set_graph")
 
+    def set_window(self, on: bool = True):
+        """Occupation window (default on): pressure runs with every k in one
+        batch form eigenvectors and derivatives only below E_F + 20 kT."""
+        check(lib().tbk_plan_set_window(self.h, int(on)), "tbk_plan_set_window")
+
     def device_bytes(self) -> int:
         return int(lib().tbk_plan_device_bytes(self.h))
 

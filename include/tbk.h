@@ -170,16 +170,17 @@ int tbk_plan_structure_eigenvalues(tbk_plan *plan, int s, double *esk,
  * last run when the plan holds it (last batch), complex128 column-major
  * nv x nv of the item's structure, Z^H S Z = I.  After a run that used the
  * occupation window (tbk_plan_set_window) the columns of states at or above
- * the window are zero. */
+ * the window are zero (all columns after an energy-only run). */
 int tbk_plan_eigenvectors(tbk_plan *plan, int k, double *z);
 
 /* Occupation window (no reference counterpart; on by default).  The Fermi
  * sums skip every state at or above E_F + 20 kT (bande.f:1170-1321); when a
  * pressure run (modes 2, 4) holds all k-points in one batch, E_F is found as
  * soon as the eigenvalues are final and eigenvectors and derivatives are
- * formed only below that level (moved up to the next spectral gap).  E_F,
- * energies, pressures and eigenvalues are the same as without it; on = 0
- * forms every eigenvector. */
+ * formed only below that level (moved up to the next spectral gap), and
+ * energy-only runs (modes 1, 3) form no eigenvector.  E_F, energies,
+ * pressures and eigenvalues are the same as without it; on = 0 forms every
+ * eigenvector. */
 int tbk_plan_set_window(tbk_plan *plan, int on);
 
 /* H(k), S(k) of k-point k as built on the device (full Hermitian,

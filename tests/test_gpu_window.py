@@ -75,3 +75,18 @@ def test_window_group_plan_and_batches(prec):
     pb.close()
     tol = 1e-12 if prec == _lib.TBK_FP64 else 1e-10
     assert max(abs(rb[q] - r1[1][q]) for q in rb) < tol
+
+
+@pytest.mark.parametrize("prec", [_lib.TBK_FP64, _lib.TBK_MIXED])
+@pytest.mark.parametrize("mode", [1, 3])
+def test_energy_modes_form_no_vectors(prec, mode):
+    st = S.config_c1()
+    k = S.monkhorst_pack((4, 4, 4))
+    w = np.full(len(k), 1.0 / len(k))
+    (p0, r0), (p1, r1) = run_both(lambda: Plan(st, k, w, precision=prec), mode)
+    assert r0 == r1
+    assert np.array_equal(p0.eigenvalues()[0], p1.eigenvalues()[0])
+    assert not np.any(p1.eigenvectors(len(k) - 1))
+    z0 = p0.eigenvectors(len(k) - 1)
+    assert np.abs(z0).max() > 0
+    p0.close(); p1.close()

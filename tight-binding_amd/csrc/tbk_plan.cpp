@@ -173,6 +173,7 @@ struct Window {
     const tbk::FermiArgs *fargs;
     int nstruct;
     int *cut;
+    bool vectors = true;           // false: no eigenvector is formed (energy-only modes)
 };
 
 struct StageTimer {
@@ -419,8 +420,10 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
     // the eigenvalues are final: occupation window, then only the columns
     // it keeps of Y F and U^{-1} (Y F)
     if (wnd.cut) tbk::launch_fermi_window_group(wnd.fargs, wnd.nstruct, st);
-    tbk::launch_zgemm(with_ncol(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.VT, n, ss), wnd.cut), 0, 0, nb, st);
-    tbk::launch_zgemm(with_ncol(gemm(n, n, n, ws.S, n, ss, ws.VT, n, ss, ws.Z, n, ss, 1), wnd.cut), 0, 0, nb, st);
+    if (wnd.vectors) {
+        tbk::launch_zgemm(with_ncol(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.VT, n, ss), wnd.cut), 0, 0, nb, st);
+        tbk::launch_zgemm(with_ncol(gemm(n, n, n, ws.S, n, ss, ws.VT, n, ss, ws.Z, n, ss, 1), wnd.cut), 0, 0, nb, st);
+    }
     tm.end();
 }
 
@@ -445,6 +448,7 @@ void solve_batch(Workspace &ws, int nb, const int *nact, double *w_out, hipStrea
     tbk::launch_stebz(sa, nb, st);
     // the eigenvalues are final: occupation window, then only the vectors it keeps
     if (wnd.cut) tbk::launch_fermi_window_group(wnd.fargs, wnd.nstruct, st);
+    if (!wnd.vectors) { tm.end(); return; }
     sa.cut = wnd.cut;
     tbk::launch_stein_vectors(sa, nb, st);
     tm.end();
@@ -483,7 +487,7 @@ struct tbk_plan {
     double2 *ph = nullptr;          // Bloch phases of the current batch
     int *ncut = nullptr, *err = nullptr, *info_all = nullptr;
     int *cut = nullptr;             // [nk] occupation window of the last windowed run
-    bool window = true, win_last = false;
+    bool window = true, win_last = false, vec_last = true;
     std::vector<int> info_host, err_host, status;
     std::vector<double> out_host;
     Workspace ws;
@@ -733,9 +737,12 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
     // Occupation window: with every k in one batch, E_F is known once the
     // eigenvalues are, and eigenvectors and derivatives are formed only for
     // the states the Fermi sums use (below E_F + 20 kT).
+    // Energy-only modes (1, 3) need no eigenvectors at all.
     const bool win = p->window && lpress && p->batch >= p->nk;
-    const Window wnd{win ? p->fargs : nullptr, p->nstruct, win ? p->cut : nullptr};
+    const bool vecs = lpress || !p->window;
+    const Window wnd{win ? p->fargs : nullptr, p->nstruct, win ? p->cut : nullptr, vecs};
     p->win_last = win;
+    p->vec_last = vecs;
     for (int k0 = 0; k0 < p->nk; k0 += p->batch) {
         const int nb = std::min(p->batch, p->nk - k0);
         p->timer.begin("build_hs");
@@ -987,6 +994,10 @@ extern "C" int tbk_plan_eigenvectors(tbk_plan *p, int k, double *z)
     HIP_TRY(hipStreamSynchronize(p->stream));
     const size_t ss = p->ws.stride();
     const int nv = p->sh[p->item_s_host[k]].nv;
+    if (!p->vec_last) {                      // an energy-only run formed no eigenvector
+        for (size_t t = 0; t < (size_t)nv * nv * 2; ++t) z[t] = 0.0;
+        return TBK_OK;
+    }
     int rc = copy_leading((double2 *)z, p->ws.Z + (size_t)(k - last0) * ss, p->N, nv, p->stream);
     if (rc == TBK_OK && p->win_last) {       // columns beyond the occupation window were not formed
         int c = nv;

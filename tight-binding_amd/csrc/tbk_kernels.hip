@@ -535,8 +535,13 @@ __device__ __forceinline__ double2 wave_sum2(double2 v)
     return v;
 }
 
-// dynamic LDS of k_hetrd_panel: vloc[N], ybuf[N], 4 partial-y slots [4][N]
-template <class R> constexpr size_t hetrd_lds_bytes(int n) { return (size_t)6 * n * 2 * sizeof(R); }
+// dynamic LDS of k_hetrd_panel: vloc[N], ybuf[N], partial-y slots [HS][N]
+// (one per wave up to order 320, else combined in pairs into four)
+template <int MAXQ> constexpr int hetrd_slots() { return MAXQ <= 5 ? 8 : 4; }
+template <class R, int MAXQ> constexpr size_t hetrd_lds_bytes(int n)
+{
+    return (size_t)(2 + hetrd_slots<MAXQ>()) * n * 2 * sizeof(R);
+}
 
 template <class R, int MAXQ>
 __global__ __launch_bounds__(HT, (MAXQ <= 5 ? 4 : 2)) void k_hetrd_panel(HetrdArgsT<R> h, int i0)
@@ -545,12 +550,15 @@ __global__ __launch_bounds__(HT, (MAXQ <= 5 ? 4 : 2)) void k_hetrd_panel(HetrdAr
     extern __shared__ double4 hetrd_smem[];
     C *vloc = reinterpret_cast<C *>(hetrd_smem);     // reflector of the current column
     C *ybuf = vloc + h.n;                              // dot part of A_t v
-    C *slot = ybuf + h.n;                              // [4][N] partial sums of A_t v
+    C *slot = ybuf + h.n;                              // [HS][N] partial sums of A_t v
+    constexpr int HS = hetrd_slots<MAXQ>();
     __shared__ C udot[2 * NB];         // W^H v (0..31) and V^H v (32..63)
     __shared__ C uwn[NB], uvn[NB];     // conj(W[j+1, l]), conj(V[j+1, l])
     __shared__ C Ts[NB][NB + 1];       // WY factor T of the panel, Ts[col][row]
-    __shared__ R red[2 * HW];
-    __shared__ C scal[2];              // tau, (beta, 0)
+    // separate reduction scratch for the norm (1) and for w^H v (4), so that
+    // neither phase has to wait for the other's readers
+    __shared__ R red1[HW], red4[2 * HW];
+    __shared__ C scal[3];              // tau, (beta, 0), scale of the reflector
     const int N = h.n;
     const int kb = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -585,12 +593,13 @@ __global__ __launch_bounds__(HT, (MAXQ <= 5 ? 4 : 2)) void k_hetrd_panel(HetrdAr
             if (r >= j + 2) xs += a.x * a.x + a.y * a.y;
         }
         for (int off = 32; off > 0; off >>= 1) xs += __shfl_xor(xs, off, 64);
-        if (lane == 0) red[wv] = xs;
+        if (lane == 0) red1[wv] = xs;
         __syncthreads();
-        // (2) reflector (zlarfg) for x = a[j+1:]
+        // (2) reflector (zlarfg) for x = a[j+1:], with the scale 1/(alpha - beta)
+        //     of its tail, so that no other thread needs vloc[j + 1]
         if (tid == 0) {
             R xn = 0;
-            for (int q = 0; q < HW; ++q) xn += red[q];
+            for (int q = 0; q < HW; ++q) xn += red1[q];
             const C al = vloc[j + 1];
             C tau = czero<C>();
             R beta = al.x;
@@ -598,21 +607,19 @@ __global__ __launch_bounds__(HT, (MAXQ <= 5 ? 4 : 2)) void k_hetrd_panel(HetrdAr
                 beta = -copysign(sqrt(al.x * al.x + al.y * al.y + xn), al.x);
                 tau = cmake<C, R>((beta - al.x) / beta, -al.y / beta);
             }
+            const C dv = cmake<C, R>(al.x - beta, al.y);
+            const R den = dv.x * dv.x + dv.y * dv.y;
+            const bool refl = tau.x != R(0) || tau.y != R(0);
             scal[0] = tau; scal[1] = cmake<C, R>(beta, R(0));
+            scal[2] = refl ? cmake<C, R>(dv.x / den, -dv.y / den) : cmake<C, R>(R(1), R(0));
             d[j] = (double)vloc[j].x;
             e[j] = (double)beta;
         }
         __syncthreads();
         const C tau = scal[0];
-        const R beta = scal[1].x;
         {
-            const C al = vloc[j + 1];
-            const C dv = cmake<C, R>(al.x - beta, al.y);
-            const R den = dv.x * dv.x + dv.y * dv.y;
-            const bool refl = tau.x != R(0) || tau.y != R(0);
-            const C sc = refl ? cmake<C, R>(dv.x / den, -dv.y / den) : cmake<C, R>(R(1), R(0));
-            __syncthreads();
-            for (int r = tid; r < N; r += HT) {
+            const C sc = scal[2];
+            for (int r = tid; r < N; r += HT) {     // rows owned as in (1)
                 C v;
                 if (r <= j) v = czero<C>();
                 else if (r == j + 1) v = cmake<C, R>(R(1), R(0));
@@ -742,26 +749,29 @@ __global__ __launch_bounds__(HT, (MAXQ <= 5 ? 4 : 2)) void k_hetrd_panel(HetrdAr
                 for (int g = 0; g < HG; ++g)
                     if (rg[g] < N) ybuf[rg[g]] = acc[g];
         }
-        // partials: slot[w] = part_w + part_{w+4} for w < 4
-        if (wv >= 4) {
+        // partials: one slot per wave, or slot[w] = part_w + part_{w+4}
+        if (HS == HW || wv >= 4) {
+            const int sw = HS == HW ? wv : wv - 4;
 #pragma unroll
             for (int q = 0; q < MAXQ; ++q) {
                 const int c = (q << 6) + lane;
-                if (q < nq && c >= t0 && c < N) slot[(size_t)(wv - 4) * N + c] = ypart[q];
+                if (q < nq && c >= t0 && c < N) slot[(size_t)sw * N + c] = ypart[q];
             }
         }
         __syncthreads();
-        if (wv < 4) {
+        if (HS != HW) {
+            if (wv < 4) {
 #pragma unroll
-            for (int q = 0; q < MAXQ; ++q) {
-                const int c = (q << 6) + lane;
-                if (q < nq && c >= t0 && c < N) {
-                    const size_t o = (size_t)wv * N + c;
-                    slot[o] = zadd(ypart[q], slot[o]);
+                for (int q = 0; q < MAXQ; ++q) {
+                    const int c = (q << 6) + lane;
+                    if (q < nq && c >= t0 && c < N) {
+                        const size_t o = (size_t)wv * N + c;
+                        slot[o] = zadd(ypart[q], slot[o]);
+                    }
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
         // (4) w = tau (y - V u1 - W u2), u1 = W^H v, u2 = V^H v; then the
         //     correction w += alpha v with alpha = -tau/2 (w^H v).  The same
         //     pass over V[r, 0:i], W[r, 0:i] accumulates the next column's
@@ -771,8 +781,9 @@ __global__ __launch_bounds__(HT, (MAXQ <= 5 ? 4 : 2)) void k_hetrd_panel(HetrdAr
         for (int q = 0; q < RPT; ++q) {
             const int r = tid + HT * q;
             if (r < t0 || r >= N) continue;
-            C y = zadd(zadd(zadd(zadd(ybuf[r], slot[r]), slot[(size_t)N + r]), slot[(size_t)2 * N + r]),
-                       slot[(size_t)3 * N + r]);
+            C y = ybuf[r];
+#pragma unroll
+            for (int sw = 0; sw < HS; ++sw) y = zadd(y, slot[(size_t)sw * N + r]);
             C nx = czero<C>();
             // the row's V and W entries are loaded kWst columns at a time
             // before they are used, so their loads are in flight together
@@ -803,11 +814,10 @@ __global__ __launch_bounds__(HT, (MAXQ <= 5 ? 4 : 2)) void k_hetrd_panel(HetrdAr
             pr += __shfl_xor(pr, off, 64);
             pi += __shfl_xor(pi, off, 64);
         }
-        __syncthreads();
-        if (lane == 0) { red[wv] = pr; red[HW + wv] = pi; }
+        if (lane == 0) { red4[wv] = pr; red4[HW + wv] = pi; }
         __syncthreads();
         R sr = 0, si = 0;
-        for (int q = 0; q < HW; ++q) { sr += red[q]; si += red[HW + q]; }
+        for (int q = 0; q < HW; ++q) { sr += red4[q]; si += red4[HW + q]; }
         const C alpha = zscale(R(-0.5), zmul(tau, cmake<C, R>(sr, si)));
         // W[j+1, i] (v_{j+1} = 1) for the l = i term of the next column's update
         const C wnext = t0 < N ? zadd(ybuf[t0], alpha) : czero<C>();
@@ -826,8 +836,10 @@ __global__ __launch_bounds__(HT, (MAXQ <= 5 ? 4 : 2)) void k_hetrd_panel(HetrdAr
             Ts[i][tid] = zmul(cmake<C, R>(-tau.x, -tau.y), acc);
         }
         if (tid == 0) Ts[i][i] = tau;
-        __syncthreads();
+        // no barrier here: the next column's first reads of anything written
+        // above (Ts, W, udot) follow its own barriers
     }
+    __syncthreads();
     for (int t = tid; t < NB * NB; t += HT) Tp[t] = Ts[t / NB][t % NB];
     // V T^H of the panel (rows > i0), for the back-transform
     // Q_p Y = Y - V ((V T^H)^H Y)
@@ -2086,7 +2098,7 @@ void launch_cgemm(const GemmArgsT<float> &g, int opa, int opb, int nb, hipStream
 template <class R, int MAXQ>
 static void launch_hetrd_panel_q(const HetrdArgsT<R> &h, int i0, int nb, hipStream_t st)
 {
-    const size_t lds = hetrd_lds_bytes<R>(h.n);
+    const size_t lds = hetrd_lds_bytes<R, MAXQ>(h.n);
     if (lds > 65536) {
         static bool set = false;     // per instantiation; the attribute is per function
         if (!set) {

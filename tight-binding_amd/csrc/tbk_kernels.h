@@ -162,7 +162,8 @@ void launch_coldot(int n, const double2 *z, const double2 *x, const double2 *y, 
                    double2 *px, double2 *py, int nb, hipStream_t st, const int *cut = nullptr);
 void launch_perturb(const PerturbArgs &p, hipStream_t st);
 void launch_fermi_sums(const FermiArgs &f, hipStream_t st);
-void launch_fermi_sums_group(const FermiArgs *f_dev, int nstruct, hipStream_t st);   // one WG each
+// one workgroup per structure; ef_known: E_F already set by launch_fermi_window_group
+void launch_fermi_sums_group(const FermiArgs *f_dev, int nstruct, hipStream_t st, int ef_known = 0);
 // E_F of every structure and the cut of each k: the first state at or above
 // E_F + 20 kT, moved up to the next spectral gap (no state above it enters
 // the Fermi sums; no group of close eigenvalues straddles it).

@@ -526,6 +526,33 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
 constexpr int HT = 512;            // threads of the panel kernel
 constexpr int HW = HT / 64;        // waves
 
+// Sum over the 64 lanes of a wave of K values per lane (K a power of two,
+// K <= 32), transposed: at each halving step a lane keeps half of its values
+// and adds the partner's copies of them, so K - 1 + 6 - log2 K exchanges
+// replace 6 K.  Lane l returns the total of value (l >> (6 - log2 K)).
+template <int K> struct ilog2c { static constexpr int v = K <= 1 ? 0 : 1 + ilog2c<K / 2>::v; };
+template <> struct ilog2c<1> { static constexpr int v = 0; };
+template <class R, int K>
+__device__ __forceinline__ R wave_sum_transposed(R (&v)[K], int lane)
+{
+    constexpr int LK = ilog2c<K>::v;
+#pragma unroll
+    for (int s = 0; s < LK; ++s) {
+        const int off = 32 >> s;
+        const bool up = (lane & off) != 0;
+        const int cnt = K >> (s + 1);
+#pragma unroll
+        for (int i = 0; i < cnt; ++i) {
+            const R keep = up ? v[cnt + i] : v[i];
+            const R send = up ? v[i] : v[cnt + i];
+            v[i] = keep + __shfl_xor(send, off, 64);
+        }
+    }
+#pragma unroll
+    for (int off = 32 >> LK; off > 0; off >>= 1) v[0] += __shfl_xor(v[0], off, 64);
+    return v[0];
+}
+
 __device__ __forceinline__ double2 wave_sum2(double2 v)
 {
     for (int off = 32; off > 0; off >>= 1) {
@@ -738,16 +765,31 @@ __global__ __launch_bounds__(HT, (MAXQ <= 5 ? 4 : 2)) void k_hetrd_panel(HetrdAr
                     }
                 }
             }
-            for (int off = 32; off > 0; off >>= 1)
+            if constexpr (sizeof(R) == 8) {
+                // FP64: the 2 HG sums (re, im of each column) reduced together;
+                // lane l holds value (l >> sh): column value / 2, component value % 2
+                R vals[2 * HG];
 #pragma unroll
-                for (int g = 0; g < HG; ++g) {
-                    acc[g].x += __shfl_xor(acc[g].x, off, 64);
-                    acc[g].y += __shfl_xor(acc[g].y, off, 64);
-                }
-            if (lane == 0)
+                for (int g = 0; g < HG; ++g) { vals[2 * g] = acc[g].x; vals[2 * g + 1] = acc[g].y; }
+                const R tot = wave_sum_transposed<R, 2 * HG>(vals, lane);
+                constexpr int sh = 6 - ilog2c<2 * HG>::v;
+                const int val = lane >> sh, rgl = t0 + tb + HW * (val >> 1);
+                if ((lane & ((1 << sh) - 1)) == 0 && rgl < N)
+                    reinterpret_cast<R *>(ybuf)[2 * (size_t)rgl + (val & 1)] = tot;
+            } else {
+                // FP32 (more columns per pass): plain butterflies, which need
+                // fewer registers here
+                for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
-                for (int g = 0; g < HG; ++g)
-                    if (rg[g] < N) ybuf[rg[g]] = acc[g];
+                    for (int g = 0; g < HG; ++g) {
+                        acc[g].x += __shfl_xor(acc[g].x, off, 64);
+                        acc[g].y += __shfl_xor(acc[g].y, off, 64);
+                    }
+                if (lane == 0)
+#pragma unroll
+                    for (int g = 0; g < HG; ++g)
+                        if (rg[g] < N) ybuf[rg[g]] = acc[g];
+            }
         }
         // partials: one slot per wave, or slot[w] = part_w + part_{w+4}
         if (HS == HW || wv >= 4) {
@@ -1914,15 +1956,17 @@ __device__ double fermi_level(const FermiArgs &f, double *s1, double *s2, int &i
     return mu_s;
 }
 
-__device__ void fermi_sums_body(const FermiArgs &f)
+// ef_known: k_fermi_window_group has already found E_F (out[0], out[5])
+// from the same eigenvalues in this run.
+__device__ void fermi_sums_body(const FermiArgs &f, int ef_known)
 {
     __shared__ double s1[1024], s2[1024];
     const int tid = threadIdx.x;
     const int nk = f.nk, nv = f.nv, ld = f.ld;
     const double el = f.el, tkb = f.tkb;
     const double nearz = 1e-5, twenty = 20.0;
-    int iters = 0;
-    const double efermi = fermi_level(f, s1, s2, iters);
+    int iters = ef_known ? (int)f.out[5] : 0;
+    const double efermi = ef_known ? f.out[0] : fermi_level(f, s1, s2, iters);
     // sums (bande.f:1170-1321)
     const double tkbi = (tkb < nearz) ? 1.0 / nearz : 1.0 / tkb;
     const double etol = twenty * tkb, efp = efermi + etol, efm = efermi - etol;
@@ -1983,9 +2027,12 @@ __device__ void fermi_sums_body(const FermiArgs &f)
     }
 }
 
-__global__ __launch_bounds__(1024) void k_fermi_sums(FermiArgs f) { fermi_sums_body(f); }
+__global__ __launch_bounds__(1024) void k_fermi_sums(FermiArgs f) { fermi_sums_body(f, 0); }
 // one workgroup per structure of a plan
-__global__ __launch_bounds__(1024) void k_fermi_sums_group(const FermiArgs *f) { fermi_sums_body(f[blockIdx.x]); }
+__global__ __launch_bounds__(1024) void k_fermi_sums_group(const FermiArgs *f, int ef_known)
+{
+    fermi_sums_body(f[blockIdx.x], ef_known);
+}
 
 // Occupation window of every k of one structure: the sums above skip every
 // state at or above E_F + 20 kT, so eigenvectors and derivatives are needed
@@ -2000,6 +2047,7 @@ __global__ __launch_bounds__(1024) void k_fermi_window_group(const FermiArgs *fa
     const FermiArgs &f = fa[blockIdx.x];
     int iters = 0;
     const double ef = fermi_level(f, s1, s2, iters);
+    if (threadIdx.x == 0) { f.out[0] = ef; f.out[5] = (double)iters; }   // reused by the sums
     const double efp = ef + 20.0 * f.tkb;
     for (int k = threadIdx.x; k < f.nk; k += blockDim.x) {
         const double *ev = f.esk + (size_t)k * f.ld;
@@ -2204,9 +2252,9 @@ void launch_fermi_window_group(const FermiArgs *f_dev, int nstruct, hipStream_t 
 {
     hipLaunchKernelGGL(k_fermi_window_group, dim3(nstruct), dim3(1024), 0, st, f_dev);
 }
-void launch_fermi_sums_group(const FermiArgs *f_dev, int nstruct, hipStream_t st)
+void launch_fermi_sums_group(const FermiArgs *f_dev, int nstruct, hipStream_t st, int ef_known)
 {
-    hipLaunchKernelGGL(k_fermi_sums_group, dim3(nstruct), dim3(1024), 0, st, f_dev);
+    hipLaunchKernelGGL(k_fermi_sums_group, dim3(nstruct), dim3(1024), 0, st, f_dev, ef_known);
 }
 void launch_copy16(const double2 *src, double2 *dst, size_t n, hipStream_t st)
 {

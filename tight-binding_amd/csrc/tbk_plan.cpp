@@ -774,7 +774,7 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
         }
     }
     p->timer.begin("fermi_sums");
-    tbk::launch_fermi_sums_group(p->fargs, p->nstruct, st);
+    tbk::launch_fermi_sums_group(p->fargs, p->nstruct, st, win ? 1 : 0);
     p->timer.end();
 }
 

@@ -23,6 +23,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "tbk_kernels.h"
 
@@ -1554,16 +1555,26 @@ __global__ __launch_bounds__(256) void k_stein_orth(SteinArgs s)
     }
 }
 
-// Complex copy of the real row-major eigenvectors: Y[:, j] = v_j.
-__global__ void k_real_to_complex_t(int N, const double *Vt, double2 *Y, size_t stride)
+// Complex copy of the real row-major eigenvectors (Vt[i*N + j] = v_j(i)):
+// Y[:, j] = v_j, a transpose through 32 x 32 LDS tiles so that both the
+// reads of Vt rows and the writes of Y columns are coalesced.
+template <class C>
+__global__ __launch_bounds__(256) void k_vt_to_columns(int N, const double *Vt, C *Y, size_t stride)
 {
-    const int kb = blockIdx.y;
+    typedef typename std::conditional<sizeof(C) == 16, double, float>::type R;
+    __shared__ double tile[32][33];
+    const int kb = blockIdx.z, i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
     const double *V = Vt + (size_t)kb * N * N;
-    double2 *Yk = Y + (size_t)kb * stride;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < (size_t)N * N;
-         t += (size_t)gridDim.x * blockDim.x) {
-        const int r = (int)(t % N), c = (int)(t / N);
-        Yk[t] = make_double2(V[(size_t)r * N + c], 0.0);
+    C *Yk = Y + (size_t)kb * stride;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r, j = j0 + tx;
+        tile[r][tx] = (i < N && j < N) ? V[(size_t)i * N + j] : 0.0;
+    }
+    __syncthreads();
+    for (int c = ty; c < 32; c += 8) {
+        const int j = j0 + c, i = i0 + tx;
+        if (i < N && j < N) Yk[(size_t)j * N + i] = cmake<C, R>((R)tile[tx][c], R(0));
     }
 }
 
@@ -1579,18 +1590,6 @@ __global__ void k_c32_to_c64(size_t count, const float2 *x, double2 *y)
 {
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (size_t)gridDim.x * blockDim.x)
         y[t] = make_double2((double)x[t].x, (double)x[t].y);
-}
-// Y32[:, j] = v_j of the row-major real tridiagonal eigenvectors.
-__global__ void k_real_to_c32_t(int N, const double *Vt, float2 *Y, size_t stride)
-{
-    const int kb = blockIdx.y;
-    const double *V = Vt + (size_t)kb * N * N;
-    float2 *Yk = Y + (size_t)kb * stride;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < (size_t)N * N;
-         t += (size_t)gridDim.x * blockDim.x) {
-        const int r = (int)(t % N), c = (int)(t / N);
-        Yk[t] = make_float2((float)V[(size_t)r * N + c], 0.f);
-    }
 }
 
 // One refinement step for the Hermitian eigenproblem of C with approximate
@@ -2214,8 +2213,8 @@ void launch_stein(const SteinArgs &s, int nb, hipStream_t st)
 void launch_real_to_complex_t(int N, const double *Vt, double2 *Y, size_t stride, int nb,
                               hipStream_t st)
 {
-    const int blocks = (int)std::min<size_t>(((size_t)N * N + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_real_to_complex_t, dim3(blocks, nb), dim3(256), 0, st, N, Vt, Y, stride);
+    const int t = (N + 31) / 32;
+    hipLaunchKernelGGL((k_vt_to_columns<double2>), dim3(t, t, nb), dim3(256), 0, st, N, Vt, Y, stride);
 }
 static int grid_for(size_t count) { return (int)std::min<size_t>((count + 255) / 256, 8192); }
 void launch_c64_to_c32(size_t count, const double2 *x, float2 *y, hipStream_t st)
@@ -2228,8 +2227,8 @@ void launch_c32_to_c64(size_t count, const float2 *x, double2 *y, hipStream_t st
 }
 void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int nb, hipStream_t st)
 {
-    const int blocks = (int)std::min<size_t>(((size_t)n * n + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_real_to_c32_t, dim3(blocks, nb), dim3(256), 0, st, n, vt, y, stride);
+    const int t = (n + 31) / 32;
+    hipLaunchKernelGGL((k_vt_to_columns<float2>), dim3(t, t, nb), dim3(256), 0, st, n, vt, y, stride);
 }
 int oa_max_n() { return kOaMax; }
 void launch_oa_update(int n, const int *nact, const double2 *t, const double2 *g, double2 *f, size_t stride,

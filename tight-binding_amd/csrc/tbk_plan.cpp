@@ -361,6 +361,10 @@ void apply_q(int n, int nb, typename tbk::cplx_t<R>::type *A, typename tbk::cplx
 // tridiagonal of the FP32 reduction carries FP32 errors, and its eigenvalues
 // only seed the vectors that the FP64 refinement corrects.
 constexpr double kMixedBisectTol = 1e-11;
+// FP64 path: dstebz's default absolute tolerance eps ||T||_1 as a floor
+// (the reduction to T leaves eigenvalue errors of that order anyway), so
+// that eigenvalues near zero stop with the rest.
+constexpr double kFp64BisectTol = 2.220446049250313e-16;
 
 void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hipStream_t st, StageTimer &tm,
                        const Window &wnd)
@@ -445,6 +449,7 @@ void solve_batch(Workspace &ws, int nb, const int *nact, double *w_out, hipStrea
     hetrd(ws, nb, st, tm);
     tm.begin("stein");
     tbk::SteinArgs sa{n, kOrtolRel, nact, ws.d, ws.e, w_out, ws.Vt, ws.scratch, ws.sinfo};
+    sa.abstol_rel = kFp64BisectTol;
     tbk::launch_stebz(sa, nb, st);
     // the eigenvalues are final: occupation window, then only the vectors it keeps
     if (wnd.cut) tbk::launch_fermi_window_group(wnd.fargs, wnd.nstruct, st);

@@ -1326,6 +1326,9 @@ __global__ __launch_bounds__(64) void k_stein_vec(SteinArgs s)
             if (fabs(dm) < pivmin) dm = -pivmin;
             double gbest = fabs(AT(wa, N - 1) + dm - (d[N - 1] - lam));
             int r = N - 1;
+            // unrolled so that the scratch loads of several steps, which do
+            // not depend on the recurrence, are in flight together
+#pragma unroll 8
             for (int i = N - 2; i >= 0; --i) {
                 const double u = e[i] * rcp_nr(dm);
                 Vt[(size_t)(i + 1) * LD + j] = u;
@@ -1336,18 +1339,21 @@ __global__ __launch_bounds__(64) void k_stein_vec(SteinArgs s)
             }
             double z = 1.0, ss = 1.0;
             Vt[(size_t)r * LD + j] = 1.0;
+#pragma unroll 8
             for (int i = r; i < N - 1; ++i) {
                 z = -Vt[(size_t)(i + 1) * LD + j] * z;
                 Vt[(size_t)(i + 1) * LD + j] = z;
                 ss += z * z;
             }
             z = 1.0;
+#pragma unroll 8
             for (int i = r - 1; i >= 0; --i) {
                 z = -(e[i] * rcp_nr(AT(wa, i))) * z;
                 Vt[(size_t)i * LD + j] = z;
                 ss += z * z;
             }
             const double scl = 1.0 / sqrt(ss);
+#pragma unroll 8
             for (int i = 0; i < N; ++i) Vt[(size_t)i * LD + j] *= scl;
         } else {
             // shift of member j of the degenerate run, perturbed away from

@@ -323,7 +323,8 @@ int tbk_debug_gemm_time(int device, int precision, int tile, int m, int n,
  * tri: 0 full; 1 op(A) upper triangular; 2 op(A) lower triangular; 3 op(B)
  * upper triangular; 4 only the tiles touching the lower triangle of C;
  * 5 (m == n) the lower triangle of the product and its conjugate transpose
- * above the diagonal; 6 = 5 with op(A) lower triangular. */
+ * above the diagonal; 6 = 5 with op(A) lower triangular; 7 only the tiles
+ * touching the upper triangle of C. */
 int tbk_debug_zgemm(int device, int precision, int tile, int m, int n, int k,
                     int opa, int opb, int tri, const void *a, int lda,
                     const void *b, int ldb, void *c, int ldc);

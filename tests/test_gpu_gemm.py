@@ -75,6 +75,11 @@ def test_gemm_triangular_variants(tile):
         assert np.abs(c - herm).max() < tol, (prec, np.abs(c - herm).max())
         off = ~np.eye(n, dtype=bool)
         assert np.array_equal(c[off], c.conj().T[off])
+    # tri 7: upper triangle of C only
+    c = run(0, tile, n, n, n, 0, 0, 7, X, Lw)
+    ref = X @ Lw
+    up = np.triu(np.ones((n, n), bool))
+    assert np.abs((c - ref)[up]).max() < 1e-11
     # tri 6: op(A) lower (A = U^H) with the Hermitian mirror of tri 5
     c = run(0, tile, n, n, n, 1, 0, 6, U, X)
     ref = U.conj().T @ X

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include <cstddef>
 #include <vector>
@@ -50,9 +51,14 @@ struct GemmArgsT {
                                    // 3 B upper triangular; 4 lower triangle of C only;
                                    // 5 lower triangle of C mirrored into the upper
                                    //   (Hermitian C; alpha real, beta 0); 6 = 5 with
-                                   //   A lower triangular as in 2
+                                   //   A lower triangular as in 2; 7 upper triangle of C only
     const int *ncol = nullptr;     // [nb] per item: column tiles at or beyond ncol[k]
                                    // are skipped (C left as it was); null: all
+    // optional copy of every element written to C, in the other precision
+    typedef typename cplx_t<typename std::conditional<sizeof(R) == 8, float, double>::type>::type CX;
+    CX *cx = nullptr;
+    int ldcx = 0;
+    size_t scx = 0;
 };
 typedef GemmArgsT<double> GemmArgs;
 

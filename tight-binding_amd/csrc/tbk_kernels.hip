@@ -343,7 +343,8 @@ __global__ __launch_bounds__(256) void k_potrf_panel(int N, double2 *S, size_t s
 // 4 = only tiles touching the lower triangle of C are computed; 5 = as 4,
 // and C receives the lower triangle of the product and its conjugate
 // transpose above the diagonal (a Hermitian C from its lower half; alpha
-// real, beta 0); 6 = 5 with op(A) lower triangular as in 2.
+// real, beta 0); 6 = 5 with op(A) lower triangular as in 2; 7 = only tiles
+// touching the upper triangle of C are computed.
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -384,6 +385,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
     const unsigned lg = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y + gxy * blockIdx.z, gxy * gridDim.z);
     const int bm = (int)(lg % gridDim.x) * BM, bn = (int)((lg / gridDim.x) % gridDim.y) * BN, kb = (int)(lg / gxy);
     if ((g.tri == 4 || g.tri == 5 || g.tri == 6) && bm + BM <= bn) return;
+    if (g.tri == 7 && bn + BN <= bm) return;            // tile wholly below the diagonal
     if (g.ncol && bn >= g.ncol[kb]) return;             // columns outside the item's window
     const C *A = g.a + (size_t)kb * g.sa;
     const C *B = g.b + (size_t)kb * g.sb;
@@ -492,18 +494,26 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
                     const C v = cmake<C, R>(p1 - p2, acc3[i][j][q] - p1 - p2);
                     C out = zmul(g.alpha, v);
                     C *p = Cm + (size_t)c * g.ldc + r;
+                    typedef typename GemmArgsT<R>::CX CX;
+                    typedef typename std::conditional<sizeof(R) == 8, float, double>::type RX;
+                    CX *Xm = g.cx ? g.cx + (size_t)kb * g.scx : nullptr;
                     if (g.tri == 5 || g.tri == 6) {
                         // Hermitian from the lower half: element (r, c), r >= c,
                         // and its conjugate at (c, r); 16 lanes of one row write
                         // 16 consecutive rows of column r
                         if (r >= c) {
                             *p = out;
-                            if (r > c && r < g.n) Cm[(size_t)r * g.ldc + c] = cmake<C, R>(out.x, -out.y);
+                            if (Xm) Xm[(size_t)c * g.ldcx + r] = cmake<CX, RX>((RX)out.x, (RX)out.y);
+                            if (r > c && r < g.n) {
+                                Cm[(size_t)r * g.ldc + c] = cmake<C, R>(out.x, -out.y);
+                                if (Xm) Xm[(size_t)r * g.ldcx + c] = cmake<CX, RX>((RX)out.x, (RX)-out.y);
+                            }
                         }
                         continue;
                     }
                     if (usebeta) out = zadd(out, zmul(g.beta, *p));
                     *p = out;
+                    if (Xm) Xm[(size_t)c * g.ldcx + r] = cmake<CX, RX>((RX)out.x, (RX)out.y);
                 }
             }
 }

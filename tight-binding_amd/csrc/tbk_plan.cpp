@@ -267,9 +267,10 @@ void potrf_trtri(Workspace &ws, int nb, hipStream_t st)
             // U12 = U_JJ^{-H} S12 in place: every output tile covers all w
             // rows and reads only its own columns of S12 before it writes
             tbk::launch_zgemm(gemm(w, m, w, dinv, n, ss, u12, n, ss, u12, n, ss), 1, 0, nb, st);
-            // S22 -= U12^H U12
+            // S22 -= U12^H U12 on the tiles touching the upper triangle (the
+            // factorisation reads only that; the panel kernel clears below)
             tbk::launch_zgemm(gemm(m, m, w, u12, n, ss, u12, n, ss, ws.S + (size_t)s0 * n + s0, n, ss,
-                                   0, double2{-1.0, 0.0}, double2{1.0, 0.0}), 1, 0, nb, st);
+                                   7, double2{-1.0, 0.0}, double2{1.0, 0.0}), 1, 0, nb, st);
         }
     }
     // U^{-1} by block columns J: Y = Uinv[0:j0, 0:j0] U[0:j0, J], then
@@ -376,10 +377,11 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
     tm.end();
     tm.begin("reduce_gemm");
     tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.S, n, ss, ws.Y, n, ss, 3), 0, 0, nb, st);
-    tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 6), 1, 0, nb, st);
-    tm.end();
-    tm.begin("hetrd_update");
-    tbk::launch_c64_to_c32(ss * nb, ws.A, ws.A32, st);
+    {   // C in FP64 (for the refinement) and, from the same epilogue, in FP32
+        tbk::GemmArgs g = gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 6);
+        g.cx = ws.A32; g.ldcx = n; g.scx = ss;
+        tbk::launch_zgemm(g, 1, 0, nb, st);
+    }
     tm.end();
     tbk::HetrdArgsT<float> h{n, ss, ws.A32, ws.P32, nullptr, ws.T32, ws.VT32, ws.d, ws.e, ws.tstride()};
     for (int i0 = 0; i0 < n - 1; i0 += kNB) {
@@ -1238,7 +1240,7 @@ extern "C" int tbk_debug_zgemm(int device, int precision, int tile, int m, int n
                                int tri, const void *a, int lda, const void *b, int ldb, void *c, int ldc)
 {
     if (tile >= tbk::kGemmTiles || opa < 0 || opa > 1 || opb < 0 || opb > 1) return TBK_ERR_ARG;
-    if (tri < 0 || tri > 6 || (tri >= 5 && m != n) || m <= 0 || n <= 0 || k <= 0) return TBK_ERR_ARG;
+    if (tri < 0 || tri > 7 || (tri >= 5 && tri <= 6 && m != n) || m <= 0 || n <= 0 || k <= 0) return TBK_ERR_ARG;
     HIP_TRY(hipSetDevice(device));
     if (precision == TBK_MIXED)
         return debug_gemm_t<float2, tbk::GemmArgsT<float>>(tile, m, n, k, opa, opb, tri, a, lda, b, ldb, c, ldc);

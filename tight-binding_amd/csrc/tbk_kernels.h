@@ -159,7 +159,9 @@ void launch_real_to_complex_t(int n, const double *vt, double2 *y, size_t stride
                               hipStream_t st);
 void launch_c64_to_c32(size_t count, const double2 *x, float2 *y, hipStream_t st);
 void launch_c32_to_c64(size_t count, const float2 *x, double2 *y, hipStream_t st);
-void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int nb, hipStream_t st);
+// y64_row0 (may be null): also writes row 0 of Y in FP64 there
+void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int nb, hipStream_t st,
+                          double2 *y64_row0 = nullptr);
 int oa_max_n();                  // largest N of launch_oa_update
 // Ogita-Aishima step: lam [nb][n], perm [nb][n], delta [nb] are scratch
 void launch_oa_update(int n, const int *nact, const double2 *t, const double2 *g, double2 *f, size_t stride,

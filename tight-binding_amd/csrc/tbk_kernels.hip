@@ -1575,7 +1575,8 @@ __global__ __launch_bounds__(256) void k_stein_orth(SteinArgs s)
 // Y[:, j] = v_j, a transpose through 32 x 32 LDS tiles so that both the
 // reads of Vt rows and the writes of Y columns are coalesced.
 template <class C>
-__global__ __launch_bounds__(256) void k_vt_to_columns(int N, const double *Vt, C *Y, size_t stride)
+__global__ __launch_bounds__(256) void k_vt_to_columns(int N, const double *Vt, C *Y, size_t stride,
+                                                       double2 *Y64row0)
 {
     typedef typename std::conditional<sizeof(C) == 16, double, float>::type R;
     __shared__ double tile[32][33];
@@ -1590,7 +1591,11 @@ __global__ __launch_bounds__(256) void k_vt_to_columns(int N, const double *Vt, 
     __syncthreads();
     for (int c = ty; c < 32; c += 8) {
         const int j = j0 + c, i = i0 + tx;
-        if (i < N && j < N) Yk[(size_t)j * N + i] = cmake<C, R>((R)tile[tx][c], R(0));
+        if (i < N && j < N) {
+            const R v = (R)tile[tx][c];
+            Yk[(size_t)j * N + i] = cmake<C, R>(v, R(0));
+            if (i == 0 && Y64row0) Y64row0[(size_t)kb * stride + (size_t)j * N] = make_double2((double)v, 0.0);
+        }
     }
 }
 
@@ -2230,7 +2235,8 @@ void launch_real_to_complex_t(int N, const double *Vt, double2 *Y, size_t stride
                               hipStream_t st)
 {
     const int t = (N + 31) / 32;
-    hipLaunchKernelGGL((k_vt_to_columns<double2>), dim3(t, t, nb), dim3(256), 0, st, N, Vt, Y, stride);
+    hipLaunchKernelGGL((k_vt_to_columns<double2>), dim3(t, t, nb), dim3(256), 0, st, N, Vt, Y, stride,
+                       (double2 *)nullptr);
 }
 static int grid_for(size_t count) { return (int)std::min<size_t>((count + 255) / 256, 8192); }
 void launch_c64_to_c32(size_t count, const double2 *x, float2 *y, hipStream_t st)
@@ -2241,10 +2247,11 @@ void launch_c32_to_c64(size_t count, const float2 *x, double2 *y, hipStream_t st
 {
     hipLaunchKernelGGL(k_c32_to_c64, dim3(grid_for(count)), dim3(256), 0, st, count, x, y);
 }
-void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int nb, hipStream_t st)
+void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int nb, hipStream_t st,
+                          double2 *y64_row0)
 {
     const int t = (n + 31) / 32;
-    hipLaunchKernelGGL((k_vt_to_columns<float2>), dim3(t, t, nb), dim3(256), 0, st, n, vt, y, stride);
+    hipLaunchKernelGGL((k_vt_to_columns<float2>), dim3(t, t, nb), dim3(256), 0, st, n, vt, y, stride, y64_row0);
 }
 int oa_max_n() { return kOaMax; }
 void launch_oa_update(int n, const int *nact, const double2 *t, const double2 *g, double2 *f, size_t stride,

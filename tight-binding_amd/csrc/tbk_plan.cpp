@@ -327,7 +327,8 @@ constexpr int kBT = 3;
 template <class R>
 void apply_q(int n, int nb, typename tbk::cplx_t<R>::type *A, typename tbk::cplx_t<R>::type *VT,
              typename tbk::cplx_t<R>::type *Y, typename tbk::cplx_t<R>::type *X, typename tbk::cplx_t<R>::type *M,
-             size_t sm, hipStream_t st, const int *ncol = nullptr)
+             size_t sm, hipStream_t st, const int *ncol = nullptr,
+             typename tbk::GemmArgsT<R>::CX *ycx = nullptr)
 {
     const size_t ss = (size_t)n * n;
     const int P = (n - 2) / kNB + 1;                  // panels
@@ -348,9 +349,12 @@ void apply_q(int n, int nb, typename tbk::cplx_t<R>::type *A, typename tbk::cplx
         // X[0:W, :] = (V_c T_c^H)^H Y[r0:, :];  Y[r0:, :] -= V_c X
         launch_gemm_any(with_ncol(gemm_t<R>(W, n, m, VT + (size_t)c0 * n + r0, n, ss, Y + r0, n, ss, X, n, ss), ncol),
                         1, 0, nb, st);
-        launch_gemm_any(with_ncol(gemm_t<R>(m, n, W, A + (size_t)c0 * n + r0, n, ss, X, n, ss, Y + r0, n, ss, R(-1),
-                                            R(1)), ncol),
-                        0, 0, nb, st);
+        tbk::GemmArgsT<R> g = with_ncol(gemm_t<R>(m, n, W, A + (size_t)c0 * n + r0, n, ss, X, n, ss, Y + r0, n, ss,
+                                                  R(-1), R(1)), ncol);
+        if (g0 == 0 && ycx) {          // the last group: Y rows 1.. also in the other precision
+            g.cx = ycx + r0; g.ldcx = n; g.scx = ss;
+        }
+        launch_gemm_any(g, 0, 0, nb, st);
     }
 }
 
@@ -411,9 +415,11 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
     tbk::launch_stein(sa, nb, st);
     tm.end();
     tm.begin("backtransform");
-    tbk::launch_real_to_c32_t(n, ws.Vt, ws.Y32, ss, nb, st);
-    apply_q<float>(n, nb, ws.A32, ws.VT32, ws.Y32, ws.X32, ws.P32, sp, st);
-    tbk::launch_c32_to_c64(ss * nb, ws.Y32, ws.Y, st);
+    // Y in FP32, back-transformed; its FP64 copy for the refinement comes
+    // from the conversion (row 0, which no reflector touches) and from the
+    // last back-transform GEMM's epilogue (rows 1..)
+    tbk::launch_real_to_c32_t(n, ws.Vt, ws.Y32, ss, nb, st, ws.Y);
+    apply_q<float>(n, nb, ws.A32, ws.VT32, ws.Y32, ws.X32, ws.P32, sp, st, nullptr, ws.Y);
     tm.end();
     tm.begin("refine");
     // X = C Y, Z = Y^H X (= Y^H C Y), VT = Y^H Y (Hermitian: lower triangles

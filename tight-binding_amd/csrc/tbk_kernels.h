@@ -157,8 +157,6 @@ void launch_stebz(const SteinArgs &s, int nb, hipStream_t st);          // eigen
 void launch_stein_vectors(const SteinArgs &s, int nb, hipStream_t st);  // vectors of known eigenvalues
 void launch_real_to_complex_t(int n, const double *vt, double2 *y, size_t stride, int nb,
                               hipStream_t st);
-void launch_c64_to_c32(size_t count, const double2 *x, float2 *y, hipStream_t st);
-void launch_c32_to_c64(size_t count, const float2 *x, double2 *y, hipStream_t st);
 // y64_row0 (may be null): also writes row 0 of Y in FP64 there
 void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int nb, hipStream_t st,
                           double2 *y64_row0 = nullptr);

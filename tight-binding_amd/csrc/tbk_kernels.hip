@@ -9,12 +9,15 @@
 //   k_potrf_panel   blocked Cholesky S = U^H U (zpptrf, lapack.f:4466) and
 //                   U^{-1}: diagonal blocks factored and inverted in LDS,
 //                   panel rows, inverse blocks and updates as GEMMs
-//   k_zgemm         batched complex GEMM on the FP64 MFMA units
+//   k_gemm          batched complex GEMM on the FP64 / FP32 MFMA units
 //   k_hetrd_panel   blocked Householder tridiagonalisation (zhptrd,
 //                   lapack.f:3218) by panels; WY factors for the back-transform
+//                   (k_hw_*: the same for few items of large order)
 //   k_stebz, k_stein_vec   bisection eigenvalues; eigenvectors by twisted
 //                   factorisation (isolated) or inverse iteration (groups)
+//   k_oa_stats, k_oa_fmat  Ogita-Aishima refinement (mixed precision)
 //   k_coldot, k_perturb   first-order derivatives (perturb.f:87-221)
+//   k_fermi_window_group  E_F and the occupation window of each k
 //   k_fermi_sums    Fermi level + band-energy / pressure sums
 //                   (fermi.f:163-433, bande.f:1170-1470)
 #include <hip/hip_runtime.h>
@@ -1600,18 +1603,8 @@ __global__ __launch_bounds__(256) void k_vt_to_columns(int N, const double *Vt, 
 }
 
 // ---------------------------------------------------------------------------
-// Mixed precision: conversions and the Ogita-Aishima refinement step
+// Mixed precision: the Ogita-Aishima refinement step
 // ---------------------------------------------------------------------------
-__global__ void k_c64_to_c32(size_t count, const double2 *x, float2 *y)
-{
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (size_t)gridDim.x * blockDim.x)
-        y[t] = make_float2((float)x[t].x, (float)x[t].y);
-}
-__global__ void k_c32_to_c64(size_t count, const float2 *x, double2 *y)
-{
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (size_t)gridDim.x * blockDim.x)
-        y[t] = make_double2((double)x[t].x, (double)x[t].y);
-}
 
 // One refinement step for the Hermitian eigenproblem of C with approximate
 // orthonormal eigenvectors Y (Ogita & Aishima, RefSyEv, extended to complex
@@ -2237,15 +2230,6 @@ void launch_real_to_complex_t(int N, const double *Vt, double2 *Y, size_t stride
     const int t = (N + 31) / 32;
     hipLaunchKernelGGL((k_vt_to_columns<double2>), dim3(t, t, nb), dim3(256), 0, st, N, Vt, Y, stride,
                        (double2 *)nullptr);
-}
-static int grid_for(size_t count) { return (int)std::min<size_t>((count + 255) / 256, 8192); }
-void launch_c64_to_c32(size_t count, const double2 *x, float2 *y, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_c64_to_c32, dim3(grid_for(count)), dim3(256), 0, st, count, x, y);
-}
-void launch_c32_to_c64(size_t count, const float2 *x, double2 *y, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_c32_to_c64, dim3(grid_for(count)), dim3(256), 0, st, count, x, y);
 }
 void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int nb, hipStream_t st,
                           double2 *y64_row0)

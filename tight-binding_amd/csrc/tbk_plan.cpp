@@ -8,6 +8,8 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <thread>
+#include <atomic>
 
 #include "tbk.h"
 #include "tbk_internal.h"
@@ -622,6 +624,32 @@ static int setup_structure(const tbk_structure_desc *st, SetupArrays &A, StructH
     return TBK_OK;
 }
 
+// Appends one structure's set-up arrays (built from empty) to the plan's.
+static void append_setup(SetupArrays &A, const SetupArrays &L, SetupOffsets &off)
+{
+    off.rows = A.row_atom.size(); off.atoms = A.ksk.size(); off.blk = A.blk_ptr.size();
+    off.pairs = A.dvec.size() / 3;
+    auto cat = [](auto &dst, const auto &src) { dst.insert(dst.end(), src.begin(), src.end()); };
+    cat(A.row_atom, L.row_atom); cat(A.row_orb, L.row_orb); cat(A.ksk, L.ksk); cat(A.nbas, L.nbas);
+    cat(A.blk_ptr, L.blk_ptr); cat(A.dvec, L.dvec); cat(A.eh, L.eh); cat(A.es, L.es);
+    cat(A.edh, L.edh); cat(A.eds, L.eds); cat(A.paros, L.paros); cat(A.dparos, L.dparos);
+}
+
+// f(0) .. f(n-1) on up to 16 host threads (each index once; no shared state
+// is written by f other than its own outputs).
+template <class F>
+static void parallel_for(int n, F f)
+{
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = (int)std::min<unsigned>({(unsigned)n, hw, 16u});
+    if (nt <= 1) { for (int i = 0; i < n; ++i) f(i); return; }
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&]() { for (int i; (i = next.fetch_add(1)) < n;) f(i); });
+    for (std::thread &x : th) x.join();
+}
+
 extern "C" int tbk_plan_create_group(int device, int nstruct, const tbk_structure_desc *st, const int *nk,
                                      const double *kpts, const double *weights, int batch, int precision,
                                      tbk_plan **out)
@@ -633,15 +661,25 @@ extern "C" int tbk_plan_create_group(int device, int nstruct, const tbk_structur
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return TBK_ERR_NODEVICE;
     if (device < 0 || device >= ndev) return TBK_ERR_ARG;
     if (hipSetDevice(device) != hipSuccess) return TBK_ERR_HIP;
-    // ---- host set-up (bit-exact) of every structure ----
+    // ---- host set-up (bit-exact) of every structure: the structures on
+    //      host threads, then their arrays concatenated in order ----
     SetupArrays A;
     std::vector<StructHost> sh(nstruct);
     std::vector<SetupOffsets> off(nstruct);
+    for (int i = 0; i < nstruct; ++i)
+        if (nk[i] <= 0) return TBK_ERR_ARG;
+    {
+        std::vector<SetupArrays> loc(nstruct);
+        std::vector<int> rcs(nstruct, TBK_OK);
+        std::vector<SetupOffsets> unused(nstruct);
+        parallel_for(nstruct, [&](int i) { rcs[i] = setup_structure(st + i, loc[i], sh[i], unused[i]); });
+        for (int i = 0; i < nstruct; ++i) {
+            if (rcs[i]) return rcs[i];
+            append_setup(A, loc[i], off[i]);
+        }
+    }
     int nktot = 0, N = 0, maxnat = 0, maxpairs = 1;
     for (int i = 0; i < nstruct; ++i) {
-        if (nk[i] <= 0) return TBK_ERR_ARG;
-        int rc = setup_structure(st + i, A, sh[i], off[i]);
-        if (rc) return rc;
         sh[i].nk = nk[i]; sh[i].k0 = nktot;
         nktot += nk[i];
         N = std::max(N, sh[i].nv);

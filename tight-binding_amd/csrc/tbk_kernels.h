@@ -160,10 +160,13 @@ void launch_real_to_complex_t(int n, const double *vt, double2 *y, size_t stride
 // y64_row0 (may be null): also writes row 0 of Y in FP64 there
 void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int nb, hipStream_t st,
                           double2 *y64_row0 = nullptr);
-int oa_max_n();                  // largest N of launch_oa_update
+int oa_max_n();                  // largest N of launch_oa_stats
 // Ogita-Aishima step: lam [nb][n], perm [nb][n], delta [nb] are scratch
-void launch_oa_update(int n, const int *nact, const double2 *t, const double2 *g, double2 *f, size_t stride,
-                      double *lam, int *perm, double *delta, double *w, int nb, hipStream_t st);
+void launch_oa_stats(int n, const int *nact, const double2 *t, const double2 *g, size_t stride, double *lam,
+                     int *perm, double *delta, double *w, int nb, hipStream_t st);
+// F = I + E with sorted columns; columns j >= cut[k] are not formed (cut may be null)
+void launch_oa_fmat(int n, const double2 *t, const double2 *g, double2 *f, size_t stride, const double *lam,
+                    const int *perm, const double *delta, int nb, hipStream_t st, const int *cut = nullptr);
 void launch_coldot(int n, const double2 *z, const double2 *x, const double2 *y, size_t stride,
                    double2 *px, double2 *py, int nb, hipStream_t st, const int *cut = nullptr);
 void launch_perturb(const PerturbArgs &p, hipStream_t st);

@@ -1639,25 +1639,27 @@ __global__ __launch_bounds__(1024) void k_oa_stats(int N, const int *nact, const
     }
     __syncthreads();
     double ft = 0.0, fr = 0.0;
-    const size_t nn = (size_t)N * N;
-    for (size_t t = tid; t < nn; t += 4 * blockDim.x) {
-        double2 tv[4], gv[4];
+    // lower triangles only (T, G Hermitian): wave wv takes columns wv, wv + 16,
+    // ..., its lanes the rows r >= c, two row chunks in flight
+    for (int c = wv; c < N; c += (int)(blockDim.x >> 6)) {
+        for (int r0 = c; r0 < N; r0 += 128) {
+            double2 tv[2], gv[2];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const size_t tt = t + (size_t)u * blockDim.x;
-            tv[u] = tt < nn ? Tk[tt] : make_double2(0.0, 0.0);
-            gv[u] = tt < nn ? Gk[tt] : make_double2(1.0, 0.0);
-        }
+            for (int u = 0; u < 2; ++u) {
+                const int r = r0 + 64 * u + lane;
+                tv[u] = r < N ? Tk[(size_t)c * N + r] : make_double2(0.0, 0.0);
+                gv[u] = r < N ? Gk[(size_t)c * N + r] : make_double2(1.0, 0.0);
+            }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {                  // lower triangle (T, G Hermitian)
-            const size_t tt = t + (size_t)u * blockDim.x;
-            const int r = (int)(tt % N), c = (int)(tt / N);
-            if (tt >= nn || r < c) continue;
-            double2 rv = make_double2(-gv[u].x, -gv[u].y);
-            double wgt = 2.0;
-            if (r == c) { tv[u].x -= lam[r]; rv.x += 1.0; wgt = 1.0; }
-            ft += wgt * (tv[u].x * tv[u].x + tv[u].y * tv[u].y);
-            fr += wgt * (rv.x * rv.x + rv.y * rv.y);
+            for (int u = 0; u < 2; ++u) {
+                const int r = r0 + 64 * u + lane;
+                if (r >= N) continue;
+                double2 rv = make_double2(-gv[u].x, -gv[u].y);
+                double wgt = 2.0;
+                if (r == c) { tv[u].x -= lam[r]; rv.x += 1.0; wgt = 1.0; }
+                ft += wgt * (tv[u].x * tv[u].x + tv[u].y * tv[u].y);
+                fr += wgt * (rv.x * rv.x + rv.y * rv.y);
+            }
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -1690,12 +1692,13 @@ __global__ __launch_bounds__(1024) void k_oa_stats(int N, const int *nact, const
 
 __global__ __launch_bounds__(256) void k_oa_fmat(int N, const double2 *T, const double2 *G, double2 *F,
                                                  size_t stride, const double *lamv, const int *perm,
-                                                 const double *delta)
+                                                 const double *delta, const int *cut)
 {
     const int kb = blockIdx.y;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)N * N) return;
     const int r = (int)(t % N), j = (int)(t / N);
+    if (cut && j >= cut[kb]) return;                  // column outside the occupation window
     const int c = perm[(size_t)kb * N + j];
     const double *lam = lamv + (size_t)kb * N;
     // T and G are Hermitian, both triangles stored: column c is read down
@@ -2238,12 +2241,16 @@ void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int
     hipLaunchKernelGGL((k_vt_to_columns<float2>), dim3(t, t, nb), dim3(256), 0, st, n, vt, y, stride, y64_row0);
 }
 int oa_max_n() { return kOaMax; }
-void launch_oa_update(int n, const int *nact, const double2 *t, const double2 *g, double2 *f, size_t stride,
-                      double *lam, int *perm, double *delta, double *w, int nb, hipStream_t st)
+void launch_oa_stats(int n, const int *nact, const double2 *t, const double2 *g, size_t stride, double *lam,
+                     int *perm, double *delta, double *w, int nb, hipStream_t st)
 {
     hipLaunchKernelGGL(k_oa_stats, dim3(nb), dim3(1024), 0, st, n, nact, t, g, stride, lam, perm, delta, w);
+}
+void launch_oa_fmat(int n, const double2 *t, const double2 *g, double2 *f, size_t stride, const double *lam,
+                    const int *perm, const double *delta, int nb, hipStream_t st, const int *cut)
+{
     hipLaunchKernelGGL(k_oa_fmat, dim3((unsigned)(((size_t)n * n + 255) / 256), nb), dim3(256), 0, st, n, t,
-                       g, f, stride, lam, perm, delta);
+                       g, f, stride, lam, perm, delta, cut);
 }
 void launch_coldot(int N, const double2 *Z, const double2 *X, const double2 *Y, size_t stride,
                    double2 *px, double2 *py, int nb, hipStream_t st, const int *cut)

@@ -430,11 +430,12 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
     tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.Z, n, ss, 5), 1, 0, nb, st);
     tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.Y, n, ss, ws.VT, n, ss, 5), 1, 0, nb, st);
     // X = (I + E) with sorted columns; VT = Y X; Z = U^{-1} VT
-    tbk::launch_oa_update(n, nact, ws.Z, ws.VT, ws.X, ss, ws.lam, ws.perm, ws.delta, w_out, nb, st);
+    tbk::launch_oa_stats(n, nact, ws.Z, ws.VT, ss, ws.lam, ws.perm, ws.delta, w_out, nb, st);
     // the eigenvalues are final: occupation window, then only the columns
-    // it keeps of Y F and U^{-1} (Y F)
+    // it keeps of F, Y F and U^{-1} (Y F)
     if (wnd.cut) tbk::launch_fermi_window_group(wnd.fargs, wnd.nstruct, st);
     if (wnd.vectors) {
+        tbk::launch_oa_fmat(n, ws.Z, ws.VT, ws.X, ss, ws.lam, ws.perm, ws.delta, nb, st, wnd.cut);
         tbk::launch_zgemm(with_ncol(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.VT, n, ss), wnd.cut), 0, 0, nb, st);
         tbk::launch_zgemm(with_ncol(gemm(n, n, n, ws.S, n, ss, ws.VT, n, ss, ws.Z, n, ss, 1), wnd.cut), 0, 0, nb, st);
     }

@@ -87,6 +87,66 @@ int tbk_setparv(int natoms, const int *kkind, int kinds, const double *param,
                 double *dparos, double *dhop, double *dover);
 
 /* ---------------------------------------------------------------------
+ * k-point meshes (host).  Bit-exact restatements of the reference's
+ * kptin/rdspgp/kptgen, whose k-points and weights the plans take.
+ * Arrays are row-major: lat/avec/bvec[3*v + c] = component c of vector v
+ * (bvec with the factor 2 pi), bij[3*i + j] = b_i . b_j, operations
+ * rot[9*k + 3*i + j] = R_k(i, j), k-points q[3*n + c] in reciprocal-lattice
+ * coordinates (the units bande and tbk_plan_create take).
+ * --------------------------------------------------------------------- */
+
+/* Reciprocal lattice, metric and cell volume.  Replaces the NEWSYM block of
+ * kptin (kptin.f:94-155, AVEC, BVEC, BIJ, WSVOL of COMMON /LATTYP/). */
+int tbk_reciprocal_lattice(const double *lat, double *avec, double *bvec,
+                           double *bij, double *wsvol);
+
+/* Space-group operations from rotations as read (direct-lattice
+ * coordinates if ilat, else Cartesian) and non-primitive translations tnp
+ * [3*nop] (direct-lattice coordinates), with an origin shift (may be NULL).
+ * Outputs rspc (Cartesian), rspct (reciprocal-lattice coordinates), tnpspc
+ * [3*nop].  Replaces the transforms of rdspgp (rdspgp.f:106-289; lattic 4
+ * snaps Cartesian entries to +-sqrt(3)/2). */
+int tbk_space_group(int ilat, int lattic, const double *avec,
+                    const double *bvec, int nop, const double *rot,
+                    const double *tnp, const double *shift, double *rspc,
+                    double *rspct, double *tnpspc);
+
+/* The same from a space-group file in rdspgp's format (A10,I5,5X,3F10.5
+ * header; 18F4.0 or 3D20.10 rotations; 9F8.5 translations).  Replaces
+ * rdspgp(iprt,ilat,io), rdspgp.f:1 (TBK_ERR_ARG for an unreadable file or
+ * nop > maxop, where the reference stops).  name: 11 chars or NULL. */
+int tbk_space_group_read(const char *path, int ilat, int lattic,
+                         const double *avec, const double *bvec, int maxop,
+                         int *nop, double *rspc, double *rspct,
+                         double *tnpspc, char *name);
+
+/* Irreducible k-points and weights of the kdiv mesh (lspec: special-point
+ * mesh), folded by the nop operations rspct and time reversal.  Replaces
+ * kptgen(nw), kptgen.f:1 (lattic 2 = bcc through the simple-cubic cell,
+ * 4 = hexagonal, 7-9 = body-centred tetragonal).  maxfull bounds the full
+ * mesh (the reference's nkbzd = 2 nkd; <= 0: unbounded), maxk the
+ * irreducible points; TBK_ERR_ARG where the reference stops. */
+int tbk_kptgen(int lattic, const double *avec, const double *bvec,
+               const double *bij, double wsvol, int nop, const double *rspct,
+               const int *kdiv, int lspec, int maxfull, int maxk, int *nk,
+               double *q, double *w);
+
+/* kptin's weights of a generated mesh (zero-weight information points
+ * appended by the caller): renormalisation, the kpst..kpend range and the
+ * final scaling, in place (kptin.f:312-351).  *nk_out points remain. */
+int tbk_kmesh_weights(int nk, double *q, double *w, int kpst, int kpend,
+                      int *nk_out);
+
+/* kptin's explicit list: raw[4*n] = k (units of 1/sfk), weight; q = k/sfk,
+ * w normalised to sum 1 (kptin.f:214, 266-273, 348-351). */
+int tbk_klist(int nk, const double *raw, double sfk, double *q, double *w);
+
+/* kptin's line mode (nkv = 999): npath segments of kd[p] points from
+ * ends[6p .. 6p+2] to ends[6p+3 .. 6p+5] (kptin.f:223-251). */
+int tbk_kpath(int npath, const int *kd, const double *ends, int maxk,
+              int *nk, double *q, double *w);
+
+/* ---------------------------------------------------------------------
  * Device plans: one structure, its k-mesh, device-resident H/S/Z state.
  * --------------------------------------------------------------------- */
 typedef struct tbk_plan tbk_plan;

@@ -94,8 +94,41 @@ def ref():
         L.ref_fermi.argtypes = [ip, dp, _d, _d, dp, dp]
         L.ref_fermi.restype = C.c_double
         L.ref_bande.argtypes = [ip, ip, _d, _d, dp, dp, dp, dp, dp, dp, dp, ip]
+        L.ref_kptin.argtypes = [_d, C.c_char_p, ip, C.c_char_p, ip, ip, ip, _d, _d, _d, _d, _d,
+                                dp, ip, ip, _d, _d, _d, ip]
         _ref = L
     return _ref
+
+
+def ref_kptin(lat, lines, workdir, maxk: int = 50000) -> dict:
+    """The reference's kptin (with its rdspgp and kptgen) on the NEWSYM block
+    ``lines`` of a SKIN (written to workdir/kin), for the lattice rows
+    ``lat``: the k-points and weights bande receives, /LATTYP/ and /SPACE/."""
+    L = ref()
+    fin = os.path.join(workdir, "kin")
+    with open(fin, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    fscr = os.path.join(workdir, "kin.15")
+    a = np.ascontiguousarray(np.asarray(lat, float))       # rows = vectors: lat(c, v) in Fortran
+    q, w = np.zeros(3 * maxk), np.zeros(maxk)
+    av, bv, bij = np.zeros(9), np.zeros(9), np.zeros(9)
+    rsp, rspt, tnp = np.zeros(9 * 48), np.zeros(9 * 48), np.zeros(3 * 48)
+    nk, lattic, nop, ierr = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    vol = C.c_double()
+    fb, sb = os.fsencode(fin), os.fsencode(fscr)
+    L.ref_kptin(a.ravel(), fb, C.byref(C.c_int(len(fb))), sb, C.byref(C.c_int(len(sb))),
+                C.byref(C.c_int(maxk)), C.byref(nk), q, w, av, bv, bij, C.byref(vol),
+                C.byref(lattic), C.byref(nop), rsp, rspt, tnp, C.byref(ierr))
+    if ierr.value:
+        raise RuntimeError(f"ref_kptin: ierr {ierr.value}")
+    n, m = nk.value, nop.value
+    # Fortran column-major (i, j) -> rows i, columns j
+    f3 = lambda x: x.reshape(3, 3).T.copy()
+    return dict(q=q[:3 * n].reshape(n, 3).copy(), w=w[:n].copy(), avec=f3(av).T.copy(),
+                bvec=f3(bv).T.copy(), bij=f3(bij), wsvol=vol.value, lattic=lattic.value,
+                rspc=rsp[:9 * m].reshape(m, 3, 3).transpose(0, 2, 1).copy(),
+                rspct=rspt[:9 * m].reshape(m, 3, 3).transpose(0, 2, 1).copy(),
+                tnpspc=tnp[:3 * m].reshape(m, 3).copy())
 
 
 class Setup:

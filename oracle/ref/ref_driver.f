@@ -348,3 +348,80 @@ c
      $     smag,.false.,.false.,.false.,0,label,iberr)
       return
       end
+c
+c     ref_kptin: the reference's k-point input (kptin, which calls rdspgp
+c     and kptgen) on a unit-10 input file fin (the NEWSYM block of a
+c     SKIN), with plv from lat.  Unit 15 goes to the file fscr.  Returns
+c     the k-points and weights bande receives (/cd1/, /kmesh1/), the
+c     lattice (/lattyp/) and the operations (/space/).  File names are
+c     passed as byte arrays of the given lengths.
+      subroutine ref_kptin(lat,fin,nfin,fscr,nfscr,maxk,nk,qk,wk,
+     $     avo,bvo,bijo,volo,lato,nopo,rspo,rspto,tnpo,ierr)
+     $     bind(C,name='ref_kptin')
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      include 'P3'
+      integer nfin,nfscr,maxk,nk,lato,nopo,ierr
+      integer*1 fin(nfin),fscr(nfscr)
+      real*8 lat(3,3),qk(3,maxk),wk(maxk)
+      real*8 avo(3,3),bvo(3,3),bijo(3,3),volo
+      real*8 rspo(3,3,nopd),rspto(3,3,nopd),tnpo(3,nopd)
+      character*255 fn
+      common /latt1/plv(3,3)
+      logical jsover
+      common /lia/ npts,jsover
+      common /cd1/ x(nkd),y(nkd),z(nkd)
+      common /kmesh1/ qkpt(3,nkd,nwdd),wghtk(nkd,nwdd),nkv(nwdd)
+      COMMON /LATTYP/ AVEC(3,3),BVEC(3,3),BIJ(3,3),WSVOL,LATTIC
+      COMMON /SPACE / RSPC(3,3,NOPD),RSPCT(3,3,NOPD),TNPSPC(3,NOPD),NOP
+      ierr = 0
+      do i = 1,3
+         do j = 1,3
+            plv(j,i) = lat(i,j)
+         end do
+      end do
+      fn = ' '
+      do i = 1,min(nfscr,255)
+         fn(i:i) = char(fscr(i))
+      end do
+      open(15,file=fn,status='unknown')
+      fn = ' '
+      do i = 1,min(nfin,255)
+         fn(i:i) = char(fin(i))
+      end do
+      open(10,file=fn,status='old')
+      call kptin
+      close(10)
+      close(15)
+      nk = npts
+      if(nk.gt.maxk) then
+         ierr = 10
+         return
+      end if
+      do k = 1,nk
+         qk(1,k) = x(k)
+         qk(2,k) = y(k)
+         qk(3,k) = z(k)
+         wk(k) = wghtk(k,1)
+      end do
+      do i = 1,3
+         do j = 1,3
+            avo(i,j) = avec(i,j)
+            bvo(i,j) = bvec(i,j)
+            bijo(i,j) = bij(i,j)
+         end do
+      end do
+      volo = wsvol
+      lato = lattic
+      nopo = nop
+      do k = 1,nop
+         do j = 1,3
+            tnpo(j,k) = tnpspc(j,k)
+            do i = 1,3
+               rspo(i,j,k) = rspc(i,j,k)
+               rspto(i,j,k) = rspct(i,j,k)
+            end do
+         end do
+      end do
+      return
+      end

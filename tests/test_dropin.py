@@ -27,12 +27,13 @@ def nm_defined(path):
 
 def test_adapter_exports_fortran_symbols():
     syms = nm_defined(os.path.join(LIBDIR, "libtbk_fortran.a"))
-    assert {"bande_", "fermi_"} <= syms
+    assert {"bande_", "fermi_", "kptgen_"} <= syms
     undefined = subprocess.run(["nm", "-u", os.path.join(LIBDIR, "libtbk_fortran.a")],
                                capture_output=True, text=True, check=True).stdout
     # the adapter reads the program's COMMON blocks and calls the C ABI
     for sym in ("codes_", "latt1_", "fazon_", "parinfo_", "cutcom_", "disply_", "lia_", "cd1_",
-                "params_", "parcom_", "tbk_plan_create", "tbk_plan_run", "tbk_fermi"):
+                "params_", "parcom_", "tbk_plan_create", "tbk_plan_run", "tbk_fermi",
+                "lattyp_", "space_", "kmesh1_", "kmesh2_", "tbk_kptgen"):
         assert sym in undefined, sym
 
 

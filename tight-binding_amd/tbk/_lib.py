@@ -20,7 +20,9 @@ TBK_FP64, TBK_MIXED = 0, 1
 
 # Every symbol include/tbk.h declares (checked by the CPU test suite).
 EXPORTS = [
-    "tbk_search2", "tbk_setpar", "tbk_setparv", "tbk_plan_create", "tbk_plan_run",
+    "tbk_search2", "tbk_setpar", "tbk_setparv", "tbk_reciprocal_lattice", "tbk_space_group",
+    "tbk_space_group_read", "tbk_kptgen", "tbk_kmesh_weights", "tbk_klist", "tbk_kpath",
+    "tbk_plan_create", "tbk_plan_run",
     "tbk_plan_launch", "tbk_plan_wait", "tbk_plan_eigenvalues", "tbk_plan_eigenvectors",
     "tbk_plan_hs", "tbk_plan_nv", "tbk_plan_npairs", "tbk_plan_destroy",
     "tbk_plan_stage_times", "tbk_plan_set_timing", "tbk_plan_autotune", "tbk_gemm_tile",
@@ -76,6 +78,15 @@ def lib():
                                  d, d, d, d]
         L.tbk_setparv.argtypes = [C.c_int, i, C.c_int, d, C.c_int, C.c_int, C.c_double, C.c_int,
                                   d, d, d, i, i, d, d, d, d]
+        L.tbk_reciprocal_lattice.argtypes = [d, d, d, d, dp]
+        L.tbk_space_group.argtypes = [C.c_int, C.c_int, d, d, C.c_int, d, d, C.c_void_p, d, d, d]
+        L.tbk_space_group_read.argtypes = [C.c_char_p, C.c_int, C.c_int, d, d, C.c_int, ip, d, d,
+                                           d, C.c_char_p]
+        L.tbk_kptgen.argtypes = [C.c_int, d, d, d, C.c_double, C.c_int, d, i, C.c_int, C.c_int,
+                                 C.c_int, ip, d, d]
+        L.tbk_kmesh_weights.argtypes = [C.c_int, d, d, C.c_int, C.c_int, ip]
+        L.tbk_klist.argtypes = [C.c_int, d, C.c_double, d, d]
+        L.tbk_kpath.argtypes = [C.c_int, i, d, C.c_int, ip, d, d]
         L.tbk_plan_create_group.argtypes = [C.c_int, C.c_int, C.POINTER(StructureDesc), i, d, d,
                                              C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.tbk_plan_result.argtypes = [vp, C.c_int, dp, dp, dp]

@@ -245,50 +245,61 @@ constexpr int NB = 32;
 // In-place upper Cholesky of a w x w block held in LDS (row stride 33).
 __device__ bool lds_potrf(double2 (*D)[NB + 1], int w)
 {
+    // Right-looking on the unscaled rows: step j subtracts
+    // conj(D[j][r]) D[j][c] / p_j from the upper triangle of the trailing
+    // block (rows > j only, so row j is read and never written within the
+    // step: one barrier per column); the pivots p_j are kept and every row
+    // is scaled by 1 / sqrt(p_j) at the end.
+    __shared__ double piv[NB];
     __shared__ int bad;
     if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
     for (int j = 0; j < w; ++j) {
+        const double p = D[j][j].x;
+        const bool ok = p > 0.0;
+        const double pinv = ok ? 1.0 / p : 1.0;
         if (threadIdx.x == 0) {
-            const double d = D[j][j].x;
-            if (!(d > 0.0)) { bad = 1; D[j][j] = make_double2(1.0, 0.0); }
-            else D[j][j] = make_double2(sqrt(d), 0.0);
+            piv[j] = ok ? p : 1.0;
+            if (!ok) bad = 1;
         }
-        __syncthreads();
-        const double inv = 1.0 / D[j][j].x;
-        for (int c = j + 1 + threadIdx.x; c < w; c += blockDim.x) D[j][c] = zscale(inv, D[j][c]);
-        __syncthreads();
         const int m = w - j - 1;
         for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
             const int r = j + 1 + t % m, c = j + 1 + t / m;
-            if (r <= c) D[r][c] = zsub(D[r][c], zcmul(D[j][r], D[j][c]));
+            if (r <= c) D[r][c] = zsub(D[r][c], zscale(pinv, zcmul(D[j][r], D[j][c])));
         }
         __syncthreads();
     }
-    const int b = bad;
+    for (int t = threadIdx.x; t < w * w; t += blockDim.x) {
+        const int r = t % w, c = t / w;
+        if (r < c) D[r][c] = zscale(1.0 / sqrt(piv[r]), D[r][c]);
+        else if (r == c) D[r][r] = make_double2(sqrt(piv[r]), 0.0);
+    }
     __syncthreads();
-    return b == 0;
+    return bad == 0;
 }
 
-// In-place inverse of a w x w upper triangular block in LDS (ztrti2).
+// In-place inverse of a w x w upper triangular block in LDS: rows of
+// X = U^{-1} from the last up, X[i][j] = -(sum_{k=i+1..j} U[i][k] X[k][j]) /
+// U[i][i], X[i][i] = 1 / U[i][i]; row i of X goes to the scratch block E
+// (rows below i of E are final, row i of D still holds U), one barrier per
+// row; E is copied back at the end.
 __device__ void lds_trtri(double2 (*D)[NB + 1], int w)
 {
-    for (int j = 0; j < w; ++j) {
-        // column j above the diagonal: x = -inv(D[j][j]) * Dinv[0:j,0:j] * D[0:j, j]
-        __shared__ double2 col[NB];
-        if (threadIdx.x < j) col[threadIdx.x] = D[threadIdx.x][j];
-        __syncthreads();
-        const double djj = 1.0 / D[j][j].x;
-        if (threadIdx.x < j) {
-            const int i = threadIdx.x;
+    __shared__ double2 E[NB][NB + 1];
+    for (int i = w - 1; i >= 0; --i) {
+        const int j = i + (int)threadIdx.x;
+        if (j < w) {
+            const double dinv = 1.0 / D[i][i].x;
             double2 acc = zzero();
-            for (int l = i; l < j; ++l) acc = zadd(acc, zmul(D[i][l], col[l]));
-            D[i][j] = zscale(-djj, acc);
+            for (int k = i + 1; k <= j; ++k) acc = zadd(acc, zmul(D[i][k], E[k][j]));
+            E[i][j] = j == i ? make_double2(dinv, 0.0) : zscale(-dinv, acc);
         }
         __syncthreads();
-        if (threadIdx.x == 0) D[j][j] = make_double2(djj, 0.0);
-        __syncthreads();
     }
+    for (int t = threadIdx.x; t < w * w; t += blockDim.x) {
+        const int r = t % w, c = t / w;
+        if (r <= c) D[r][c] = E[r][c];
+    }
+    __syncthreads();
 }
 
 // Cholesky, right-looking by panels of NB columns: this kernel factors the

@@ -292,6 +292,14 @@ def main():
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.precision),
                 "launches_per_step": launches, "avg_launch_ms": panel_ms / launches,
                 "algorithmic_bytes_per_launch": bytes_step / launches}
+    # ---- the H/S build (k_phases + k_build_hs, HBM-bound): H and S written
+    # once, 2 * 16 * N^2 bytes per k, over the build stage's event time
+    build_ms = head["stages_ms_per_step"].get("build_hs", float("nan"))
+    build_bytes = 2 * 16 * n * n * len(kpts)
+    build_gbs = build_bytes / (build_ms / 1e3) / 1e9
+    roofline_build = {"kernel": "k_build_hs", "bound": "hbm", "achieved": build_gbs, "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": build_gbs / HBM_PEAK_GBS, "algorithmic_bytes": build_bytes,
+                      "stage_ms": build_ms}
 
     serving = None
     if not args.no_serve:
@@ -325,6 +333,7 @@ def main():
         "plan_create_ms": {k: r["plan_create_ms"] for k, r in runs.items()},
         "result": {k: r["result"] for k, r in runs.items()},
         "roofline": roofline,
+        "roofline_build": roofline_build,
         "serving": serving,
     }
     if not args.no_cpu:

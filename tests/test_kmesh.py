@@ -14,7 +14,9 @@ Pinned by
   share of the half-open mesh cell in its orbit under the operations and
   time reversal;
 * a_i . b_j = 2 pi delta_ij, the volume, integer operations closed under
-  products.
+  products;
+* sums over the irreducible mesh equal those over the full mesh (E_F,
+  E(T), P(T) of the ex1 Pd structure through the oracle).
 """
 import json
 import os
@@ -29,6 +31,7 @@ import kmesh_cases as KC
 import oracle
 from conftest import ROOT
 from tbk import _lib, kmesh
+from tbk import structures as S
 
 GOLD = os.path.join(ROOT, "tests", "golden", "kmesh.json")
 EX1_JSON = os.path.join(ROOT, "tests", "golden", "ex1.json")
@@ -230,3 +233,29 @@ def test_reference_program_with_dropin_kptgen(tmp_path):
         outs[exe] = ((work / "SKOUT").read_text(), (work / "SKENG").read_text())
     assert "K-POINT GENERATOR IS INVOKED" in outs["static_kpt"][0]
     assert outs["static_kpt"] == outs["static_ref"]
+
+
+@pytest.mark.parametrize("lattice", ["fcc", "bcc"])
+def test_irreducible_mesh_reproduces_full_mesh_sums(lattice):
+    """Folding is exact: the ex1 Pd structure (O_h) summed over the
+    irreducible special points with kptgen's weights gives the E_F, E(T) and
+    P(T) of the full 6x6x6 Monkhorst-Pack mesh with equal weights (oracle,
+    CPU).  A wrong weight or a missing star member changes the sums."""
+    gold = json.load(open(EX1_JSON))
+    p = S.NRLParams(1, [9], [gold["valence"]], [gold["rcut"]], [gold["screenl"]],
+                    np.array(gold["param"]), 0, True)
+    a = 6.9
+    lat = S.fcc_primitive(a) if lattice == "fcc" else S.bcc_primitive(a)
+    st = S.Structure(lat, np.zeros((1, 3)), [1], p, gold["nd"])
+    L = kmesh.reciprocal_lattice(lat, 1)                 # generic code: the primitive cell's mesh
+    g = kmesh.space_group(L, KC.oh_cartesian(), np.zeros((48, 3)), ilat=False)
+    q, w = kmesh.kptgen(L, g, (6, 6, 6), lspec=True)
+    full = S.monkhorst_pack((6, 6, 6))
+    assert len(q) < len(full) // 4
+    setup = oracle.Setup(st)
+    r_ibz = setup.bande(q, w, mode=4, tkb=gold["tkb"], eigcut=10.0, nthreads=4)
+    r_full = setup.bande(full, np.full(len(full), 1.0 / len(full)), mode=4, tkb=gold["tkb"], eigcut=10.0,
+                         nthreads=4)
+    assert abs(r_ibz["efermi"] - r_full["efermi"]) < 1e-11
+    assert abs(r_ibz["earg1"] - r_full["earg1"]) < 1e-11
+    assert abs(r_ibz["earg2"] - r_full["earg2"]) * S.RYBOHR3_KBAR < 1e-6

@@ -258,3 +258,38 @@ def test_tile_and_panel_tridiagonalisation_agree(prec):
     print(f"tile vs panel (precision {prec}): max|de|={de:.2e} max|dresult|={dr:.2e}", flush=True)
     tol = 1e-11 if prec == _lib.TBK_FP64 else 1e-9
     assert de < tol and dr < tol
+
+
+@pytest.fixture(scope="module")
+def c2_full_ref():
+    st = S.config_c2()
+    k = S.monkhorst_pack((8, 8, 8))
+    w = np.full(len(k), 1.0 / len(k))
+    ref = oracle.Setup(st).bande(k, w, mode=4, nthreads=16, want_eig=True)
+    assert ref["err"] == 0
+    return st, k, w, ref
+
+
+@pytest.mark.parametrize("prec", [_lib.TBK_FP64, _lib.TBK_MIXED])
+def test_c2_full_bench_workload_vs_oracle(c2_full_ref, prec):
+    """The bench's own workload -- all 512 k of C2 in one batch, occupation
+    window on -- against the oracle: the plain bounds in FP64, the mixed
+    bounds in mixed precision."""
+    st, k, w, ref = c2_full_ref
+    print(f"case: C2 full, 512 k, precision {prec}", flush=True)
+    plan = Plan(st, k, w, precision=prec)
+    r = plan.run(4, 0.005, 10.0)
+    e, d = plan.eigenvalues()
+    plan.close()
+    de = float(np.abs(e - ref["eig"]).max())
+    def_ = abs(r["efermi"] - ref["efermi"])
+    dE = abs(r["earg1"] - ref["earg1"]) / st.natoms
+    dP = abs(r["earg2"] - ref["earg2"]) * S.RYBOHR3_KBAR
+    used = ref["eig"] < r["efermi"] + 20 * 0.005
+    dd = float(np.abs(d - ref["desk"])[used].max())
+    print(f"  max|de|={de:.2e} dEF={def_:.2e} dE/atom={dE:.2e} dP={dP:.2e} kbar max|d desk|={dd:.2e}",
+          flush=True)
+    if prec == _lib.TBK_FP64:
+        assert de < EIG_TOL and def_ < EF_TOL and dE < E_TOL and dP < P_TOL_KBAR and dd < 1e-8
+    else:
+        assert def_ < MIX_EF and dE < MIX_E and de < MIX_EIG and dP < MIX_P_KBAR

@@ -14,10 +14,13 @@ is --precision (default mixed).  Prints one JSON line with
   roofline      k_hetrd_panel, the dominant kernel: algorithmic bytes per
                 launch over its average duration (HIP events on the plan's
                 stream around each launch) against the HBM peak;
+  roofline_build  the H/S build: 2 * 16 * N^2 bytes per k written over the
+                build stage's event time, against the HBM peak;
   serving       Poisson request streams (10, 100, 500 req/s) through the
                 continuous-batching request loop: p50/p99 latency;
-  cpu_baseline  the CPU restatement (oracle) timed on this host's cores on a
-                bounded sample of the same workload.
+  cpu_baseline  the CPU restatement (oracle) timed on this host's cores (at
+                most 16) on a bounded sample of the same workload, and on
+                one core; the CPU model.
 --profile-pass PREC runs one batched step of PREC and a 1 GiB device copy
 (the calibration kernel for counter-based traffic) and exits; it is the
 program profiled with rocprofv3 --pmc.
@@ -75,12 +78,27 @@ def hetrd_panel_bytes(n: int, csize: int) -> float:
     return tot * csize
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(st, kpts, weights, seconds=12.0):
-    """The oracle on this host's cores over a bounded sample of k-points."""
+    """The oracle on this host's cores (at most 16, the host share of one
+    GPU) over a bounded sample of k-points, and on one core over a smaller
+    sample."""
     import oracle
-    threads = min(os.cpu_count() or 1, 64)
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    threads = max(1, min(avail, 16))
     setup = oracle.Setup(st)
-    nk = max(threads, 8)
     t0 = time.perf_counter()
     setup.bande(kpts[:threads], weights[:threads], mode=4, nthreads=threads)
     dt = time.perf_counter() - t0
@@ -90,9 +108,15 @@ def cpu_baseline(st, kpts, weights, seconds=12.0):
     t0 = time.perf_counter()
     setup.bande(kpts[:nk], weights[:nk], mode=4, nthreads=threads)
     dt = time.perf_counter() - t0
+    n1 = 8
+    t0 = time.perf_counter()
+    setup.bande(kpts[:n1], weights[:n1], mode=4, nthreads=1)
+    dt1 = time.perf_counter() - t0
     return {"value": nk / dt, "unit": "k-point solves/s", "cores": threads, "kind": "port",
             "sample": f"{nk} of the 512 k-points of the same structure (N={st.nv}), "
-                      f"full per-k path incl. pressure, OpenMP over k, {dt:.1f} s"}
+                      f"full per-k path incl. pressure, OpenMP over k, {dt:.1f} s",
+            "value_1core": n1 / dt1, "sample_1core": f"{n1} k-points on one thread, {dt1:.1f} s",
+            "host_cpus_visible": os.cpu_count(), "cpu_model": cpu_model()}
 
 
 def run_batched(st, kpts, w, prec, steps, warmup, autotune=True):

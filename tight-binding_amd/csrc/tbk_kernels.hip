@@ -119,9 +119,10 @@ __global__ __launch_bounds__(256) void k_phases(const StructDev *sd, const int *
 // (rotate.f:260-326); its conjugate goes to the partner element of the lower
 // block (ja, ia) (or of the lower half of a diagonal block).  The
 // accumulation order and the absence of FMA contraction reproduce the
-// reference's rounding except for the ulp of sin/cos.  When a group's items
-// share one structure, each pair's SK entries are loaded once for all of
-// them.  The last kBuildPadWgs workgroup columns write the padding rows and
+// reference's rounding except for the ulp of sin/cos.  When the chunk's
+// items share one structure, the block's SK entries and the items' phases
+// are staged through LDS 16 pairs at a time and each is loaded once for all
+// items.  The last kBuildPadWgs workgroup columns write the padding rows and
 // columns nv..n-1.
 constexpr int kBuildItems = 24, kBuildPadWgs = 8;
 
@@ -159,6 +160,93 @@ __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a, int nb)
     const int ia = blk - ja * (ja + 1) / 2;
     const int il = tid / 81, e = tid - 81 * il;          // il < 3 for the 243 working threads
     const int o = e % 9, b = e / 9;
+    // chunk of one structure: the block's SK entries and the items' phases
+    // are staged through LDS a few pairs at a time, shared by the three
+    // groups (every thread takes part in the loads and barriers)
+    bool chunk_same = true;
+    for (int kb = it0 + 1; kb < it1; ++kb)
+        if (a.item_s[kb] != a.item_s[it0]) chunk_same = false;
+    if (chunk_same) {
+        constexpr int PC = 16;                           // pairs per stage
+        __shared__ double seh[PC * 81], ses[PC * 81];
+        __shared__ double2 sph[kBuildItems][PC];
+        const StructDev &SD = a.sd[a.item_s[it0]];
+        if (ja >= SD.natoms) return;                     // uniform over the workgroup
+        const int kf = it0 + il;
+        const bool active = il < 3 && kf < it1 && o < SD.nbas[ia] && b < SD.nbas[ja] && !(ia == ja && b < o);
+        const int p0 = SD.blk_ptr[ia * SD.natoms + ja], p1 = SD.blk_ptr[ia * SD.natoms + ja + 1];
+        const double *eh = a.deriv ? SD.edh : SD.eh, *es = a.deriv ? SD.eds : SD.es;
+        const int idx = 9 * o + b, nit = it1 - it0;
+        constexpr int U = kBuildItems / 3;
+        double acc[U][4];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc[u][0] = acc[u][1] = acc[u][2] = acc[u][3] = 0.0;
+        for (int pc = p0; pc < p1; pc += PC) {
+            const int np = min(PC, p1 - pc);
+            for (int t = tid; t < 81 * np; t += 256) {
+                seh[t] = eh[81 * (size_t)pc + t];
+                ses[t] = es[81 * (size_t)pc + t];
+            }
+            for (int t = tid; t < nit * np; t += 256) {
+                const int u = t / np, q = t - u * np;
+                sph[u][q] = a.ph[(size_t)(it0 + u) * a.maxpairs + pc + q];
+            }
+            __syncthreads();
+            if (active) {
+                for (int q = 0; q < np; ++q) {
+                    const double e1 = seh[81 * q + idx], e2 = ses[81 * q + idx];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int kb = kf + 3 * u;
+                        if (kb < it1) {
+                            const double2 z = sph[kb - it0][q];
+                            acc[u][0] = acc[u][0] + e1 * z.x; acc[u][1] = acc[u][1] + e1 * z.y;
+                            acc[u][2] = acc[u][2] + e2 * z.x; acc[u][3] = acc[u][3] + e2 * z.y;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // stores, one item at a time: the upper element from its own
+        // thread (row orbital fastest: 9-row column segments), the
+        // conjugate partner through an LDS transpose of the block so that
+        // those stores are 9-row column segments too
+        __shared__ double2 tr_h[3][81], tr_s[3][81];
+        const int r0 = SD.ksk[ia], c0 = SD.ksk[ja];
+        const int beta = e % 9, omega = e / 9;           // transposed role of this thread
+        const bool tr_ok = il < 3 && omega < SD.nbas[ia] && beta < SD.nbas[ja] && (ia < ja || beta > omega);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kb = kf + 3 * u;
+            const bool live = active && kb < it1;
+            double2 hv = make_double2(0.0, 0.0), sv = make_double2(0.0, 0.0);
+            if (live) {
+                if (ia == ja && b == o) {
+                    const double *onsite = a.deriv ? SD.dparos : SD.paros;
+                    const int cls = o == 0 ? 0 : (o <= 3 ? 1 : (o <= 6 ? 2 : 3));
+                    hv = make_double2(acc[u][0] + onsite[4 * ia + cls], 0.0);
+                    sv = make_double2(acc[u][2] + (a.deriv ? 0.0 : 1.0), 0.0);
+                } else {
+                    hv = make_double2(acc[u][0], acc[u][1]); sv = make_double2(acc[u][2], acc[u][3]);
+                }
+                double2 *H = a.h + (size_t)kb * a.stride, *S = a.s + (size_t)kb * a.stride;
+                H[(size_t)(c0 + b) * N + r0 + o] = hv;
+                S[(size_t)(c0 + b) * N + r0 + o] = sv;
+            }
+            if (il < 3) { tr_h[il][e] = hv; tr_s[il][e] = sv; }
+            __syncthreads();
+            if (tr_ok && kb < it1) {
+                // partner of upper element (o, b) = (omega, beta): row c0 + beta, column r0 + omega
+                const double2 ph_ = tr_h[il][omega + 9 * beta], ps_ = tr_s[il][omega + 9 * beta];
+                double2 *H = a.h + (size_t)kb * a.stride, *S = a.s + (size_t)kb * a.stride;
+                H[(size_t)(r0 + omega) * N + c0 + beta] = make_double2(ph_.x, -ph_.y);
+                S[(size_t)(r0 + omega) * N + c0 + beta] = make_double2(ps_.x, -ps_.y);
+            }
+            __syncthreads();
+        }
+        return;
+    }
     if (il >= 3) return;                                  // no barriers below
     // items it0 + il + 3u of this thread group; when they share one structure
     // each pair's SK entries are loaded once for all of them

@@ -41,7 +41,7 @@ extern "C" {
 #define TBK_ERR_NOMEM     11  /* device allocation failed                  */
 #define TBK_ERR_NODEVICE  12  /* no usable GPU                             */
 
-/* Precision of a plan (BASELINE north star "plain" / "mixed"). */
+/* Precision of a plan: the plain all-FP64 method or the mixed method. */
 #define TBK_FP64          0   /* all-FP64 plain method                     */
 #define TBK_MIXED         1   /* FP32 tridiagonalisation + back-transform,
                                  FP64 Cholesky, Fermi sums and refinement  */

@@ -1,6 +1,6 @@
 """Parity of the HIP path (lib/libtbk.so on the GPU) with the oracle.
 
-Tolerances: eigenvalues 1e-10 Ry, E_F and E_tot
+Tolerances of the plain FP64 method: eigenvalues 1e-10 Ry, E_F and E_tot
 1e-9 Ry (per atom), pressure 1e-6 kbar; H/S elements 1e-14 (the build is
 the reference's arithmetic except for the ulp of sin/cos).
 """

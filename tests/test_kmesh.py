@@ -259,3 +259,60 @@ def test_irreducible_mesh_reproduces_full_mesh_sums(lattice):
     assert abs(r_ibz["efermi"] - r_full["efermi"]) < 1e-11
     assert abs(r_ibz["earg1"] - r_full["earg1"]) < 1e-11
     assert abs(r_ibz["earg2"] - r_full["earg2"]) * S.RYBOHR3_KBAR < 1e-6
+
+
+def test_space_group_reader_fortran_fields(tmp_path):
+    """rdspgp's fixed fields as a Fortran runtime reads them: implied
+    decimals in fields without a point (F8.5, F4.0), D exponents, blank
+    fields as zero, records shorter than the format padded with blanks,
+    text past the last field ignored."""
+    L = kmesh.reciprocal_lattice(KC.sc(), 3)
+    ops = KC.oh_cartesian()[:2]                          # identity and one more operation
+    lines = ["Test      " + "    2" + "     " + "   0.00000" + "      0.25" + "          " + " trailing text"]
+    for m in ops:
+        for row in m:
+            lines.append("".join(f"{v:20.10E}".replace("E", "D") for v in row))
+    # translations: op 1 blank (zero), op 2 (0.5, 0.25, -0.125) with implied decimals
+    lines.append(" " * 24 + "   50000" + "   25000" + " -12500")
+    path = tmp_path / "g.grp"
+    path.write_text("\n".join(lines) + "\n")
+    g = kmesh.read_space_group(str(path), L, ilat=False)
+    assert g.nop == 2 and g.name == "Test"
+    assert np.array_equal(g.rspc, ops)
+    # the origin shift (0, 0.25, 0) moves the translations: t' = R s + t - s
+    s = np.array([0.0, 0.25, 0.0])
+    t = np.array([[0.0, 0.0, 0.0], [0.5, 0.25, -0.125]])
+    for k in range(2):
+        rdir = np.rint(np.linalg.solve(KC.sc().T, ops[k] @ KC.sc().T))
+        assert np.allclose(g.tnpspc[k], rdir @ s + t[k] - s, atol=1e-15)
+    # lattice coordinates in 18F4.0 records, a field without a point
+    lines = ["Test2         2     " + "   0.00000" * 3]
+    vals = [int(v) for v in np.concatenate([m.ravel() for m in ops])]
+    rec = "".join(f"{v:4d}" for v in vals)
+    lines.append(rec)
+    lines.append("")                                   # blank translation record: zeros
+    path.write_text("\n".join(lines) + "\n")
+    g2 = kmesh.read_space_group(str(path), L, ilat=True)
+    assert np.array_equal(g2.rspc, ops) and np.array_equal(g2.tnpspc, np.zeros((2, 3)))
+
+
+@needs_ref
+def test_space_group_reader_fields_vs_reference(tmp_path):
+    """The same odd-format file read by the reference's rdspgp (through its
+    kptin) and by tbk_space_group_read: identical operations and
+    translations."""
+    L = kmesh.reciprocal_lattice(KC.sc(), 3)
+    ops = KC.oh_cartesian()
+    lines = ["Oh-odd    " + "   48" + "     " + "   0.00000" + "      0.25" + "          "]
+    for m in ops:
+        for row in m:
+            lines.append("".join(f"{v:20.10E}".replace("E", "D") for v in row))
+    tv = [f"{int(x):8d}" for x in np.arange(48 * 3) * 1000 % 100000]     # implied decimals
+    for i in range(0, len(tv), 9):
+        lines.append("".join(tv[i:i + 9]))
+    path = tmp_path / "odd.grp"
+    path.write_text("\n".join(lines) + "\n")
+    g = kmesh.read_space_group(str(path), L, ilat=False)
+    r = oracle.ref_kptin(KC.sc(), KC.newsym_block(str(path), 3, False, (2, 2, 2), False), str(tmp_path))
+    for key, mine in (("rspc", g.rspc), ("rspct", g.rspct), ("tnpspc", g.tnpspc)):
+        assert np.array_equal(mine, r[key]), key

@@ -36,9 +36,15 @@ def test_error_strings():
         assert L.tbk_error_string(code)
 
 
-@pytest.mark.parametrize("cfg", ["c1", "c2", "ex1", "hcp"])
+@pytest.mark.parametrize("cfg", ["c1", "c2", "ex1", "hcp", "large"])
 def test_host_setup_bit_exact(cfg):
-    if cfg == "c1":
+    """Neighbour list and parameters bit-identical to the oracle's serial
+    restatement; "large" (128 atoms, 16,712 pairs) runs the set-up's
+    per-atom and per-pair-chunk work on several host threads."""
+    if cfg == "large":
+        pool = S.config_c4(seed=20261203, count=256, max_atoms=128, max_q=6, nk_budget=9 * 6 ** 3)
+        st = max((p[0] for p in pool), key=lambda s: s.nv)
+    elif cfg == "c1":
         st = S.config_c1()
     elif cfg == "c2":
         st = S.config_c2()

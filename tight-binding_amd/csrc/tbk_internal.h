@@ -2,9 +2,32 @@
 // driver and the C ABI.
 #pragma once
 
+#include <algorithm>
+#include <atomic>
+#include <thread>
 #include <vector>
 
 namespace tbk {
+
+// f(0) .. f(n-1) on up to 16 host threads (each index once; f writes only
+// its own outputs).  A call made from inside another parallel_for runs
+// serially, so nested set-up work does not multiply the thread count.
+inline thread_local bool t_in_parallel = false;
+template <class F>
+void parallel_for(int n, F f)
+{
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = t_in_parallel ? 1 : (int)std::min<unsigned>({(unsigned)std::max(n, 0), hw, 16u});
+    if (nt <= 1) { for (int i = 0; i < n; ++i) f(i); return; }
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&]() {
+            t_in_parallel = true;
+            for (int i; (i = next.fetch_add(1)) < n;) f(i);
+        });
+    for (std::thread &x : th) x.join();
+}
 
 // One entry of the neighbour list (COMMON /pairs/, /neigh/, search2.f:72-81).
 struct Pair {

@@ -597,27 +597,33 @@ static int setup_structure(const tbk_structure_desc *st, SetupArrays &A, StructH
     A.paros.insert(A.paros.end(), paros.begin(), paros.end());
     A.dparos.insert(A.dparos.end(), dparos.begin(), dparos.end());
     std::vector<int> blk((size_t)nat * nat + 1, 0);
-    const size_t e0 = A.eh.size();
+    const size_t e0 = A.eh.size(), d0 = A.dvec.size();
     A.eh.resize(e0 + 81 * (size_t)np); A.es.resize(e0 + 81 * (size_t)np);
     A.edh.resize(e0 + 81 * (size_t)np); A.eds.resize(e0 + 81 * (size_t)np);
-    for (int p = 0; p < np; ++p) {
-        const tbk::Pair &q = pl.pairs[p];
-        double v[14];
-        const bool unlike = q.kind[0] != q.kind[1];
-        const size_t o = e0 + 81 * (size_t)p;
-        tbk::arrange_integrals(&hop[14 * p], q.kind[0], q.kind[1], v);
-        tbk::sk_block(q.cosines[0], q.cosines[1], q.cosines[2], v, unlike, &A.eh[o]);
-        tbk::arrange_integrals(&over[14 * p], q.kind[0], q.kind[1], v);
-        tbk::sk_block(q.cosines[0], q.cosines[1], q.cosines[2], v, unlike, &A.es[o]);
-        tbk::arrange_integrals(&dhop[14 * p], q.kind[0], q.kind[1], v);
-        tbk::sk_block(q.cosines[0], q.cosines[1], q.cosines[2], v, unlike, &A.edh[o]);
-        tbk::arrange_integrals(&dover[14 * p], q.kind[0], q.kind[1], v);
-        tbk::sk_block(q.cosines[0], q.cosines[1], q.cosines[2], v, unlike, &A.eds[o]);
-        const int i = q.atom[0], j = q.atom[1];
-        for (int c = 0; c < 3; ++c)    // rotate.f:144-146 operand order
-            A.dvec.push_back(q.dlv[c] - st->posq[3 * j + c] + st->posq[3 * i + c]);
-        blk[(size_t)i * nat + j + 1]++;
-    }
+    A.dvec.resize(d0 + 3 * (size_t)np);
+    // SK blocks per pair (independent: chunks of pairs on host threads)
+    constexpr int kChunk = 256;
+    tbk::parallel_for((np + kChunk - 1) / kChunk, [&](int ch) {
+        const int pe = std::min(np, (ch + 1) * kChunk);
+        for (int p = ch * kChunk; p < pe; ++p) {
+            const tbk::Pair &q = pl.pairs[p];
+            double v[14];
+            const bool unlike = q.kind[0] != q.kind[1];
+            const size_t o = e0 + 81 * (size_t)p;
+            tbk::arrange_integrals(&hop[14 * p], q.kind[0], q.kind[1], v);
+            tbk::sk_block(q.cosines[0], q.cosines[1], q.cosines[2], v, unlike, &A.eh[o]);
+            tbk::arrange_integrals(&over[14 * p], q.kind[0], q.kind[1], v);
+            tbk::sk_block(q.cosines[0], q.cosines[1], q.cosines[2], v, unlike, &A.es[o]);
+            tbk::arrange_integrals(&dhop[14 * p], q.kind[0], q.kind[1], v);
+            tbk::sk_block(q.cosines[0], q.cosines[1], q.cosines[2], v, unlike, &A.edh[o]);
+            tbk::arrange_integrals(&dover[14 * p], q.kind[0], q.kind[1], v);
+            tbk::sk_block(q.cosines[0], q.cosines[1], q.cosines[2], v, unlike, &A.eds[o]);
+            const int i = q.atom[0], j = q.atom[1];
+            for (int c = 0; c < 3; ++c)    // rotate.f:144-146 operand order
+                A.dvec[d0 + 3 * (size_t)p + c] = q.dlv[c] - st->posq[3 * j + c] + st->posq[3 * i + c];
+        }
+    });
+    for (int p = 0; p < np; ++p) blk[(size_t)pl.pairs[p].atom[0] * nat + pl.pairs[p].atom[1] + 1]++;
     for (size_t b = 1; b < blk.size(); ++b) blk[b] += blk[b - 1];
     A.blk_ptr.insert(A.blk_ptr.end(), blk.begin(), blk.end());
     h.natoms = nat; h.nv = nv; h.npairs = np;
@@ -634,21 +640,6 @@ static void append_setup(SetupArrays &A, const SetupArrays &L, SetupOffsets &off
     cat(A.row_atom, L.row_atom); cat(A.row_orb, L.row_orb); cat(A.ksk, L.ksk); cat(A.nbas, L.nbas);
     cat(A.blk_ptr, L.blk_ptr); cat(A.dvec, L.dvec); cat(A.eh, L.eh); cat(A.es, L.es);
     cat(A.edh, L.edh); cat(A.eds, L.eds); cat(A.paros, L.paros); cat(A.dparos, L.dparos);
-}
-
-// f(0) .. f(n-1) on up to 16 host threads (each index once; no shared state
-// is written by f other than its own outputs).
-template <class F>
-static void parallel_for(int n, F f)
-{
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const int nt = (int)std::min<unsigned>({(unsigned)n, hw, 16u});
-    if (nt <= 1) { for (int i = 0; i < n; ++i) f(i); return; }
-    std::atomic<int> next{0};
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t)
-        th.emplace_back([&]() { for (int i; (i = next.fetch_add(1)) < n;) f(i); });
-    for (std::thread &x : th) x.join();
 }
 
 extern "C" int tbk_plan_create_group(int device, int nstruct, const tbk_structure_desc *st, const int *nk,
@@ -673,7 +664,7 @@ extern "C" int tbk_plan_create_group(int device, int nstruct, const tbk_structur
         std::vector<SetupArrays> loc(nstruct);
         std::vector<int> rcs(nstruct, TBK_OK);
         std::vector<SetupOffsets> unused(nstruct);
-        parallel_for(nstruct, [&](int i) { rcs[i] = setup_structure(st + i, loc[i], sh[i], unused[i]); });
+        tbk::parallel_for(nstruct, [&](int i) { rcs[i] = setup_structure(st + i, loc[i], sh[i], unused[i]); });
         for (int i = 0; i < nstruct; ++i) {
             if (rcs[i]) return rcs[i];
             append_setup(A, loc[i], off[i]);

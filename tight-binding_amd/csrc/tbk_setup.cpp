@@ -48,10 +48,17 @@ int search2(int natoms, const double *lat, const double *posn, const int *kkind,
 {
     out.clear();
     const double eps = 1e-3;
-    int kneigh = 0;   // COMMON /relat/ kneigh of a fresh process
-    double dismin2 = 1e10;
     const int kd0 = 2 * nd[0] + 1, kd1 = 2 * nd[1] + 1, kd2 = 2 * nd[2] + 1;
-    for (int ia = 0; ia < natoms; ++ia) {
+    // Pass 1, one atom ia per task (host threads): every candidate within
+    // the cutoff, in the reference's order, with the pair values and its
+    // effect on the kneigh flag of COMMON /relat/ (search2.f:168-175):
+    // 1 sets it, 0 clears it, -1 (distance exactly at the cutoff) keeps it.
+    struct Cand { Pair p; signed char eff; };
+    std::vector<std::vector<Cand>> per(natoms);
+    std::vector<double> dmin(natoms, 1e10);
+    parallel_for(natoms, [&](int ia) {
+        std::vector<Cand> &list = per[ia];
+        double dm = 1e10;
         for (int ja = ia; ja < natoms; ++ja) {
             const int ki = kkind[ia], kj = kkind[ja];
             int itype;
@@ -81,27 +88,40 @@ int search2(int natoms, const double *lat, const double *posn, const int *kkind,
                 }
                 const double dd = dsq - eps;
                 if (dd > rcsq) continue;
-                if (dd < rcsq) kneigh = 1;
-                if (dd < eps) kneigh = 0;
-                if (dd > eps && dsq < dismin2) dismin2 = dsq;
-                if (!kneigh) continue;
-                if (out.n >= maxpairs) return TBK_ERR_PAIRS;
-                const double r = std::sqrt(dsq);
-                const double rinv = 1.0 / r;
-                const double ex = std::exp((r - rscreen) * ellinv);
-                const double f = 1.0 / (ex + 1.0);
-                Pair p;
-                for (int c = 0; c < 3; ++c) {
-                    p.cosines[c] = t[c] * rinv;
-                    p.dlv[c] = (double)m[c];
+                Cand cd{};
+                cd.eff = dd < eps ? 0 : (dd < rcsq ? 1 : -1);
+                if (dd > eps && dsq < dm) dm = dsq;
+                if (cd.eff != 0) {
+                    const double r = std::sqrt(dsq);
+                    const double rinv = 1.0 / r;
+                    const double ex = std::exp((r - rscreen) * ellinv);
+                    const double f = 1.0 / (ex + 1.0);
+                    for (int c = 0; c < 3; ++c) {
+                        cd.p.cosines[c] = t[c] * rinv;
+                        cd.p.dlv[c] = (double)m[c];
+                    }
+                    cd.p.dist = r;
+                    cd.p.screen = f;
+                    cd.p.dscrn = -ex * f * ellinv;
+                    cd.p.kind[0] = ki; cd.p.kind[1] = kj;
+                    cd.p.atom[0] = ia; cd.p.atom[1] = ja;
                 }
-                p.dist = r;
-                p.screen = f;
-                p.dscrn = -ex * f * ellinv;
-                p.kind[0] = ki; p.kind[1] = kj;
-                p.atom[0] = ia; p.atom[1] = ja;
-                out.push(p);
+                list.push_back(cd);
             }
+        }
+        dmin[ia] = dm;
+    });
+    // Pass 2, in the reference's order: the flag (that of a fresh process
+    // at the start) decides which candidates become pairs.
+    int kneigh = 0;
+    double dismin2 = 1e10;
+    for (int ia = 0; ia < natoms; ++ia) {
+        dismin2 = std::min(dismin2, dmin[ia]);
+        for (const Cand &cd : per[ia]) {
+            if (cd.eff >= 0) kneigh = cd.eff;
+            if (!kneigh) continue;
+            if (out.n >= maxpairs) return TBK_ERR_PAIRS;
+            out.push(cd.p);
         }
     }
     out.dismin = std::sqrt(dismin2);
@@ -118,21 +138,30 @@ int setpar(int natoms, const int *kkind, int kinds, const double *param, int nlt
     const bool harrison = nltype == 2 || nltype == 3;
     const double twtrd = 2.0 / 3.0;
     for (int i = 0; i < natoms * kinds; ++i) den[i] = 0.0;
+    // densities, both directions, summed in pair order (setpar.f:508-536)
     for (int ip = 0; ip < pl.n; ++ip) {
         const Pair &pr = pl.pairs[ip];
         const int ia = pr.atom[0], ja = pr.atom[1];
         const int ki = pr.kind[0], kj = pr.kind[1];
         const double r = pr.dist, f = pr.screen;
-        {   // densities, both directions (setpar.f:508-536)
-            const int i0 = onsite_base(kj) + 1;
-            const double b = P(i0) * P(i0);
-            den[ia * kinds + kj - 1] = den[ia * kinds + kj - 1] + std::exp(-b * r) * f;
-            if (ja != ia) {
-                const int i1 = onsite_base(ki) + 1;
-                const double b1 = P(i1) * P(i1);
-                den[ja * kinds + ki - 1] = den[ja * kinds + ki - 1] + std::exp(-b1 * r) * f;
-            }
+        const int i0 = onsite_base(kj) + 1;
+        const double b = P(i0) * P(i0);
+        den[ia * kinds + kj - 1] = den[ia * kinds + kj - 1] + std::exp(-b * r) * f;
+        if (ja != ia) {
+            const int i1 = onsite_base(ki) + 1;
+            const double b1 = P(i1) * P(i1);
+            den[ja * kinds + ki - 1] = den[ja * kinds + ki - 1] + std::exp(-b1 * r) * f;
         }
+    }
+    // bond and overlap integrals, each pair on its own (chunks of pairs on
+    // host threads)
+    constexpr int kChunk = 512;
+    parallel_for((pl.n + kChunk - 1) / kChunk, [&](int ch) {
+    const int ipe = std::min(pl.n, (ch + 1) * kChunk);
+    for (int ip = ch * kChunk; ip < ipe; ++ip) {
+        const Pair &pr = pl.pairs[ip];
+        const int ki = pr.kind[0], kj = pr.kind[1];
+        const double r = pr.dist, f = pr.screen;
         const int lo = ki < kj ? ki : kj, hi = ki < kj ? kj : ki;
         const int npk = lo == hi ? kLike : kUnlike;
         double *h = hop + 14 * ip;
@@ -171,6 +200,7 @@ int setpar(int natoms, const int *kkind, int kinds, const double *param, int nlt
             }
         }
     }
+    });
     for (int ia = 0; ia < natoms; ++ia) {      // setpar.f:795-867
         const int k1 = kkind[ia];
         int b = onsite_base(k1) + 1;
@@ -200,24 +230,33 @@ int setparv(int natoms, const int *kkind, int kinds, const double *param, int nl
     const bool harrison = nltype == 2 || nltype == 3;
     const double third = 1.0 / 3.0, twtrd = 2.0 / 3.0, threeh = 3.0 / 2.0;
     std::vector<double> vden((size_t)natoms * kinds, 0.0);
+    // density derivatives, summed in pair order (setparv.f:464-544)
     for (int ip = 0; ip < pl.n; ++ip) {
         const Pair &pr = pl.pairs[ip];
         const int ia = pr.atom[0], ja = pr.atom[1];
         const int ki = pr.kind[0], kj = pr.kind[1];
         const double r = pr.dist, f = pr.screen, df = pr.dscrn;
         const double vdist = third * r / thisvol;          // setparv.f:459
-        {
-            const int i0 = onsite_base(kj) + 1;
-            const double b = P(i0) * P(i0);
-            const double dent = std::exp(-b * r) * f;
-            vden[ia * kinds + kj - 1] = vden[ia * kinds + kj - 1] + (df - b) * dent * vdist;
-            if (ja != ia) {
-                const int i1 = onsite_base(ki) + 1;
-                const double b1 = P(i1) * P(i1);
-                const double dent1 = std::exp(-b1 * r) * f;
-                vden[ja * kinds + ki - 1] = vden[ja * kinds + ki - 1] + (df - b1) * dent1 * vdist;
-            }
+        const int i0 = onsite_base(kj) + 1;
+        const double b = P(i0) * P(i0);
+        const double dent = std::exp(-b * r) * f;
+        vden[ia * kinds + kj - 1] = vden[ia * kinds + kj - 1] + (df - b) * dent * vdist;
+        if (ja != ia) {
+            const int i1 = onsite_base(ki) + 1;
+            const double b1 = P(i1) * P(i1);
+            const double dent1 = std::exp(-b1 * r) * f;
+            vden[ja * kinds + ki - 1] = vden[ja * kinds + ki - 1] + (df - b1) * dent1 * vdist;
         }
+    }
+    // derivatives of the bond and overlap integrals, each pair on its own
+    constexpr int kChunk = 512;
+    parallel_for((pl.n + kChunk - 1) / kChunk, [&](int ch) {
+    const int ipe = std::min(pl.n, (ch + 1) * kChunk);
+    for (int ip = ch * kChunk; ip < ipe; ++ip) {
+        const Pair &pr = pl.pairs[ip];
+        const int ki = pr.kind[0], kj = pr.kind[1];
+        const double r = pr.dist, f = pr.screen, df = pr.dscrn;
+        const double vdist = third * r / thisvol;          // setparv.f:459
         const int lo = ki < kj ? ki : kj, hi = ki < kj ? kj : ki;
         const int npk = lo == hi ? kLike : kUnlike;
         const bool usehar = harrison && lo == hi;
@@ -272,6 +311,7 @@ int setparv(int natoms, const int *kkind, int kinds, const double *param, int nl
             }
         }
     }
+    });
     for (int ia = 0; ia < natoms; ++ia) {
         const int k1 = kkind[ia];
         int b = onsite_base(k1) + 1;

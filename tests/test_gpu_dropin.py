@@ -1,6 +1,7 @@
 """The reference's own program run with the device drop-in: oracle/_ref/
 static_tbk (its Fortran objects linked against bande_/fermi_ of
-lib/libtbk_fortran.a) on the ex1 inputs writes the published SKENG."""
+lib/libtbk_fortran.a) on the ex1 inputs writes the published SKENG, and on
+ex2 (hcp Ti, mesh line aligned) the SKENG of the unmodified program."""
 import json
 import os
 import shutil
@@ -8,6 +9,7 @@ import subprocess
 
 import pytest
 
+import ex2_case
 from conftest import ROOT
 
 pytestmark = pytest.mark.gpu
@@ -15,6 +17,8 @@ pytestmark = pytest.mark.gpu
 EXE = os.path.join(ROOT, "oracle", "_ref", "static_tbk")
 INPUTS = os.path.join(ROOT, "oracle", "_ref", "ex1")
 GOLD = os.path.join(ROOT, "tests", "golden", "ex1.json")
+INPUTS2 = os.path.join(ROOT, "oracle", "_ref", "ex2")
+GOLD2 = os.path.join(ROOT, "tests", "golden", "ex2.json")
 
 
 @pytest.mark.skipif(not os.path.exists(EXE), reason="drop-in program not built (oracle/Makefile dropin)")
@@ -35,4 +39,29 @@ def test_reference_program_with_device_bande(tmp_path):
         for got, ref in ((vol, g["volume"]), (ef, g["efermi"]), (et, g["et"]), (pt, g["pt"])):
             worst = max(worst, abs(got - ref))
     print(f"  worst |SKENG - published| = {worst:.1e} (9 printed decimals)", flush=True)
+    assert worst <= 1e-9 * 1.5
+
+
+@pytest.mark.skipif(not os.path.exists(EXE) or not os.path.isdir(INPUTS2),
+                    reason="drop-in program not built (oracle/Makefile dropin)")
+def test_reference_program_with_device_bande_ex2_hcp(tmp_path):
+    """Two atoms of nine orbitals (N = 18), 133 irreducible k of a 12x12x12
+    mesh, 13 c/a values, mode 3 (E_F and E(T), no pressure): the SKENG rows
+    agree with the reference program's to their 9 printed decimals."""
+    work = tmp_path / "ex2"
+    shutil.copytree(INPUTS2, work)
+    (work / "SKIN").write_text(ex2_case.skin_text((work / "SKIN").read_text()))
+    print("case: static_tbk on ex2 inputs (hcp Ti, mesh line aligned)", flush=True)
+    r = subprocess.run("ulimit -s unlimited; " + EXE, shell=True, cwd=work, capture_output=True,
+                       text=True, timeout=300, env=dict(os.environ, TBK_DEVICE="0"))
+    assert r.returncode == 0, (r.stdout[-1000:], r.stderr[-2000:])
+    rows = ex2_case.read_skeng(work / "SKENG")
+    gold = json.load(open(GOLD2))["structures"]
+    assert len(rows) == len(gold)
+    worst = 0.0
+    for (label, vol, ef, et), g in zip(rows, gold):
+        assert label == g["label"] and vol == g["volume"]
+        for got, ref in ((ef, g["efermi"]), (et, g["et"])):
+            worst = max(worst, abs(got - ref))
+    print(f"  worst |SKENG - reference program| = {worst:.1e} (9 printed decimals)", flush=True)
     assert worst <= 1e-9 * 1.5

@@ -27,6 +27,7 @@ import subprocess
 import numpy as np
 import pytest
 
+import ex2_case
 import kmesh_cases as KC
 import oracle
 from conftest import ROOT
@@ -35,6 +36,7 @@ from tbk import structures as S
 
 GOLD = os.path.join(ROOT, "tests", "golden", "kmesh.json")
 EX1_JSON = os.path.join(ROOT, "tests", "golden", "ex1.json")
+EX2_JSON = os.path.join(ROOT, "tests", "golden", "ex2.json")
 REF = "/root/reference/NRLtb/ex1"
 ORACLE = os.path.join(ROOT, "oracle")
 
@@ -233,6 +235,53 @@ def test_reference_program_with_dropin_kptgen(tmp_path):
         outs[exe] = ((work / "SKOUT").read_text(), (work / "SKENG").read_text())
     assert "K-POINT GENERATOR IS INVOKED" in outs["static_kpt"][0]
     assert outs["static_kpt"] == outs["static_ref"]
+
+
+def _run_ex2(tmp_path, exe, aligned):
+    work = tmp_path / f"{exe}_{int(aligned)}"
+    shutil.copytree(os.path.join(ORACLE, "_ref", "ex2"), work)
+    (work / "SKIN").write_text(ex2_case.skin_text((work / "SKIN").read_text(), aligned))
+    p = subprocess.run("ulimit -s unlimited; " + os.path.join(ORACLE, "_ref", exe), shell=True, cwd=work,
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr[-2000:]
+    skeng = (work / "SKENG").read_text() if (work / "SKENG").exists() else ""
+    return p.stdout + p.stderr, (work / "SKOUT").read_text(), skeng
+
+
+@needs_ref
+def test_dropin_kptgen_stops_where_the_reference_does(tmp_path):
+    """ex2 as published asks kptgen for 12000^3 divisions (tests/ex2_case.py):
+    the reference stops at its full-mesh bound (kptgen.f:141-145) before any
+    band energy; the drop-in kptgen_ prints the same lines and stops the
+    program the same way, with the same SKOUT and no SKENG row (not rows of
+    zeros from an empty mesh)."""
+    subprocess.run(["make", "-C", ORACLE, "dropin"], capture_output=True, check=True)
+    out_ref, skout_ref, skeng_ref = _run_ex2(tmp_path, "static_ref", aligned=False)
+    out_kpt, skout_kpt, skeng_kpt = _run_ex2(tmp_path, "static_kpt", aligned=False)
+    assert "KDIV1,KDIV2,KDIV3=120001200012000" in skout_ref
+    for out in (out_ref, out_kpt):
+        assert "KPTGEN: NKBZD DIM. EXCEED., NK=  100001" in out
+        assert "KPTGEN: NKBZD DIMENSION EXCEEDED" in out
+    assert skout_kpt == skout_ref
+    assert skeng_ref.strip() == "" and skeng_kpt.strip() == ""
+
+
+@needs_ref
+def test_dropin_kptgen_on_ex2_hcp(tmp_path):
+    """ex2 with its mesh line in 5I5 fields: hcp with the operations given
+    in direct-lattice coordinates (ILAT=T), a 12x12x12 uniform mesh, 13 c/a
+    values; the program with the drop-in kptgen_ writes the SKOUT and SKENG
+    of the unmodified one, and both match tests/golden/ex2.json."""
+    subprocess.run(["make", "-C", ORACLE, "dropin"], capture_output=True, check=True)
+    _, skout_ref, skeng_ref = _run_ex2(tmp_path, "static_ref", aligned=True)
+    _, skout_kpt, skeng_kpt = _run_ex2(tmp_path, "static_kpt", aligned=True)
+    assert (skout_kpt, skeng_kpt) == (skout_ref, skeng_ref)
+    gold = json.load(open(EX2_JSON))
+    assert [int(x) for x in re.findall(r"NUMBER OF K-POINTS=\s+(\d+)", skout_ref)] == gold["nk"]
+    ef = [float(x) for x in re.findall(r"EFermi =\s+([0-9.Ee+-]+)", skout_ref)]
+    et = [float(x) for x in re.findall(r"E\(T\) =\s+([0-9.Ee+-]+)", skout_ref)]
+    assert ef == [g["efermi"] for g in gold["structures"]]
+    assert et == [g["et"] for g in gold["structures"]]
 
 
 @pytest.mark.parametrize("lattice", ["fcc", "bcc"])

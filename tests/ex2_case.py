@@ -26,6 +26,15 @@ def skin_text(published: str, aligned: bool = True) -> str:
     return out
 
 
+def write_skin(work, aligned: bool = True):
+    """Rewrite work/SKIN in place (as a new file: the copied one may be
+    read-only)."""
+    path = work / "SKIN"
+    text = skin_text(path.read_text(), aligned)
+    path.unlink()
+    path.write_text(text)
+
+
 def read_skeng(path):
     """SKENG rows: (label, volume, E_F, E(T)) in mode 3 (static.f's writer)."""
     rows = []

@@ -11,6 +11,7 @@ Run in a tree where the reference is mounted:
 """
 import json
 import os
+import pathlib
 import re
 import shutil
 import subprocess
@@ -28,8 +29,7 @@ def main():
     with tempfile.TemporaryDirectory() as tmp:
         for f in ex2_case.INPUTS:
             shutil.copy(os.path.join(ref, "ex2", f), tmp)
-        skin = open(os.path.join(tmp, "SKIN")).read()
-        open(os.path.join(tmp, "SKIN"), "w").write(ex2_case.skin_text(skin))
+        ex2_case.write_skin(pathlib.Path(tmp))
         subprocess.run("ulimit -s unlimited; " + os.path.join(ref, "static_ref"), shell=True, cwd=tmp,
                        check=True, capture_output=True)
         skout = open(os.path.join(tmp, "SKOUT")).read()

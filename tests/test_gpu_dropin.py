@@ -50,7 +50,7 @@ def test_reference_program_with_device_bande_ex2_hcp(tmp_path):
     agree with the reference program's to their 9 printed decimals."""
     work = tmp_path / "ex2"
     shutil.copytree(INPUTS2, work)
-    (work / "SKIN").write_text(ex2_case.skin_text((work / "SKIN").read_text()))
+    ex2_case.write_skin(work)
     print("case: static_tbk on ex2 inputs (hcp Ti, mesh line aligned)", flush=True)
     r = subprocess.run("ulimit -s unlimited; " + EXE, shell=True, cwd=work, capture_output=True,
                        text=True, timeout=300, env=dict(os.environ, TBK_DEVICE="0"))

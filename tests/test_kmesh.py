@@ -228,6 +228,7 @@ def test_reference_program_with_dropin_kptgen(tmp_path):
         gen = re.sub(r"-1313[^\n]*\n(fcc|bcc)\.08[^\n]*\n", "    0\nLSPEC=T LINFOK=F\n    6    6    6    0    0\n",
                      skin)
         assert gen.count("LSPEC=T") == 2
+        (work / "SKIN").unlink()                        # the copy may be read-only
         (work / "SKIN").write_text(gen)
         p = subprocess.run("ulimit -s unlimited; " + os.path.join(ORACLE, "_ref", exe), shell=True, cwd=work,
                            capture_output=True, text=True, timeout=300)
@@ -240,7 +241,7 @@ def test_reference_program_with_dropin_kptgen(tmp_path):
 def _run_ex2(tmp_path, exe, aligned):
     work = tmp_path / f"{exe}_{int(aligned)}"
     shutil.copytree(os.path.join(ORACLE, "_ref", "ex2"), work)
-    (work / "SKIN").write_text(ex2_case.skin_text((work / "SKIN").read_text(), aligned))
+    ex2_case.write_skin(work, aligned)
     p = subprocess.run("ulimit -s unlimited; " + os.path.join(ORACLE, "_ref", exe), shell=True, cwd=work,
                        capture_output=True, text=True, timeout=300)
     assert p.returncode == 0, p.stderr[-2000:]

@@ -86,6 +86,27 @@ int tbk_setparv(int natoms, const int *kkind, int kinds, const double *param,
                 const int *jkind, const int *jatm, const double *den,
                 double *dparos, double *dhop, double *dover);
 
+/* Spin channel jspin (1 or 2) of a collinear spin-polarised set.  Replaces
+ * setpar(jspin,param,jsover,den,flipos,.false.,...), setpar.f:1, and
+ * setparv(jspin,...), setparv.f:1, with jspins = 2: param2 [2*nparam] holds
+ * param(:,1) then param(:,2); bond and overlap integrals come from channel
+ * jspin, the densities and on-site terms of an atom with flipos[a] != 0
+ * from the other channel (setpar.f:500-536, 795-867); flipos may be NULL.
+ * Outputs as tbk_setpar / tbk_setparv. */
+int tbk_setpar_spin(int natoms, const int *kkind, int kinds,
+                    const double *param2, int nparam, int jspin,
+                    const int *flipos, int nltype, int jsover, int npairs,
+                    const double *dist, const double *screen, const int *jkind,
+                    const int *jatm, double *den, double *paros, double *hop,
+                    double *over);
+int tbk_setparv_spin(int natoms, const int *kkind, int kinds,
+                     const double *param2, int nparam, int jspin,
+                     const int *flipos, int nltype, int jsover, double thisvol,
+                     int npairs, const double *dist, const double *screen,
+                     const double *dscrn, const int *jkind, const int *jatm,
+                     const double *den, double *dparos, double *dhop,
+                     double *dover);
+
 /* ---------------------------------------------------------------------
  * k-point meshes (host).  Bit-exact restatements of the reference's
  * kptin/rdspgp/kptgen, whose k-points and weights the plans take.
@@ -194,9 +215,31 @@ int tbk_plan_create_group(int device, int nstruct,
                           const double *kpts, const double *weights,
                           int batch, int precision, tbk_plan **out);
 
+/* A collinear spin-polarised structure: bande with jspins = 2 and lnoncol
+ * false (bande.f:816-889, 1069-1081, 1342-1402).  st->param is the spin-up
+ * channel (param(:,1), input.f:327-359), param_dn [st->nparam] the
+ * spin-down channel (param(:,2)); flipos [natoms] (may be NULL: none)
+ * marks atoms whose densities and on-site terms come from the other
+ * channel (setpar.f:500-536, 795-867; input1.f:189-195).  The plan holds two
+ * structures, 0 = spin up and 1 = spin down, each with the nk k-points and
+ * weights given, solved in the same launches; tbk_plan_run returns one E_F
+ * over both channels with one electron per state and the sums over both,
+ * tbk_plan_result gives the same for s = 0 and 1, and
+ * tbk_plan_structure_eigenvalues the channels' eigenvalues. */
+int tbk_plan_create_spin(int device, const tbk_structure_desc *st,
+                         const double *param_dn, const int *flipos, int nk,
+                         const double *kpts, const double *weights, int batch,
+                         int precision, tbk_plan **out);
+
+/* Magnetic moment of the last run of a spin-polarised plan, spin-up minus
+ * spin-down electrons per cell (the smag of bande.f:1388-1402); 0 for other
+ * plans. */
+int tbk_plan_magnetization(const tbk_plan *plan, double *smag);
+
 /* Run the whole bande k-loop + Fermi + sums on the device.  Replaces
  * bande(mode,...,efermi,earg1,earg2,...,iberr), bande.f:1, for the
- * collinear non-magnetic case.  Blocks until results are on the host. */
+ * collinear case (spin-polarised through tbk_plan_create_spin).  Blocks
+ * until results are on the host. */
 int tbk_plan_run(tbk_plan *plan, int mode, double tkb, double eigcut,
                  double *efermi, double *earg1, double *earg2);
 

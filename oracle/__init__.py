@@ -94,6 +94,8 @@ def ref():
         L.ref_fermi.argtypes = [ip, dp, _d, _d, dp, dp]
         L.ref_fermi.restype = C.c_double
         L.ref_bande.argtypes = [ip, ip, _d, _d, dp, dp, dp, dp, dp, dp, dp, ip]
+        L.ref_setpar_spin.argtypes = [ip, dp, _d, ip, _i, _d, _d, _d, _d, _d, _d, _d, ip]
+        L.ref_bande_spin.argtypes = [ip, ip, _d, _d, dp, dp, dp, dp, _d, ip, _i, dp, dp, dp, dp, ip]
         L.ref_kptin.argtypes = [_d, C.c_char_p, ip, C.c_char_p, ip, ip, ip, _d, _d, _d, _d, _d,
                                 dp, ip, ip, _d, _d, _d, ip]
         _ref = L
@@ -262,6 +264,38 @@ class RefSetup:
         L.ref_setpar(C.byref(C.c_double(st.volume)), self.paros, self.hop, self.over,
                      self.dparos, self.dhop, self.dover, self.den, C.byref(ierr))
         self.nv = st.nv
+
+    def setpar_spin(self, jspin, param_dn, flipos=None):
+        """The reference's setpar/setparv for spin channel jspin of a
+        spin-polarised set (param(:,2) = param_dn; atoms flipped where
+        flipos is non-zero); replaces this object's per-structure arrays,
+        which the later setham calls use."""
+        st = self.st
+        ierr = C.c_int(0)
+        pdn = np.ascontiguousarray(np.asarray(param_dn, float))
+        fl = np.zeros(st.natoms, np.int32) if flipos is None else np.ascontiguousarray(flipos, np.int32)
+        ref().ref_setpar_spin(C.byref(C.c_int(int(jspin))), C.byref(C.c_double(st.volume)), pdn,
+                              C.byref(C.c_int(len(pdn))), fl, self.paros, self.hop, self.over,
+                              self.dparos, self.dhop, self.dover, self.den, C.byref(ierr))
+        if ierr.value:
+            raise RuntimeError(f"ref_setpar_spin ierr={ierr.value}")
+
+    def bande_spin(self, kpts, weights, param_dn, flipos=None, mode=4, tkb=0.005, eigcut=10.0):
+        """The reference's bande with jspins = 2 (param(:,1) = the
+        structure's parameters, param(:,2) = param_dn)."""
+        st = self.st
+        ef, e1, e2, sm = C.c_double(0), C.c_double(0), C.c_double(0), C.c_double(0)
+        ib = C.c_int(0)
+        kp = np.ascontiguousarray(np.asarray(kpts, float).ravel())
+        pdn = np.ascontiguousarray(np.asarray(param_dn, float))
+        fl = np.zeros(st.natoms, np.int32) if flipos is None else np.ascontiguousarray(flipos, np.int32)
+        ref().ref_bande_spin(C.byref(C.c_int(mode)), C.byref(C.c_int(len(kpts))), kp,
+                             np.ascontiguousarray(np.asarray(weights, float)),
+                             C.byref(C.c_double(st.volume)), C.byref(C.c_double(st.zfill)),
+                             C.byref(C.c_double(tkb)), C.byref(C.c_double(eigcut)), pdn,
+                             C.byref(C.c_int(len(pdn))), fl, C.byref(ef), C.byref(e1), C.byref(e2),
+                             C.byref(sm), C.byref(ib))
+        return dict(err=ib.value, efermi=ef.value, earg1=e1.value, earg2=e2.value, smag=sm.value)
 
     def hs(self, ak, deriv=False):
         np_ = self.nv * (self.nv + 1) // 2

@@ -888,13 +888,15 @@ double orc_fermi(int ldn, int nk, int nv, double el, const double *weight,
     int nfill = (int)(zfill / el + nearz);
     int linst = fabs(zfill / el - (double)nfill) < tol;
 #define EIG(i, k) eigen[(size_t)(k) * ldn + (i) - 1]
+/* levels above nv: rows no k-point wrote, zero in the reference's static
+   eigenvalue storage (full bands, or one electron per state) */
+#define EIGZ(i, k) ((i) <= nv ? EIG(i, k) : 0.0)
     if (nfill < 1) nfill = 1;          /* guards for reading eigen(nfill+1) */
-    if (nfill > nv - 1) nfill = nv > 1 ? nv - 1 : 1;
     double muhigh = EIG(nv, 0), mulow = EIG(1, 0);
-    double stlow = EIG(nfill, 0), sthigh = EIG(nfill + 1, 0);
+    double stlow = EIGZ(nfill, 0), sthigh = EIGZ(nfill + 1, 0);
     for (int k = 0; k < nk; ++k) {
-        if (EIG(nfill, k) > stlow) stlow = EIG(nfill, k);
-        if (EIG(nfill + 1, k) < sthigh) sthigh = EIG(nfill + 1, k);
+        if (EIGZ(nfill, k) > stlow) stlow = EIGZ(nfill, k);
+        if (EIGZ(nfill + 1, k) < sthigh) sthigh = EIGZ(nfill + 1, k);
         for (int i = 1; i <= nv; ++i) {
             if (EIG(i, k) > muhigh) muhigh = EIG(i, k);
             if (EIG(i, k) < mulow) mulow = EIG(i, k);
@@ -935,6 +937,7 @@ double orc_fermi(int ldn, int nk, int nv, double el, const double *weight,
         if (muhigh - mulow > tol) { mu = mu1; continue; }
         return mu;
     }
+#undef EIGZ
 #undef EIG
 }
 

@@ -134,6 +134,55 @@ c
       integer ierr
       real*8 dpar(4,*),hpo(14,*),opo(14,*),dparo(4,*),dhpo(14,*)
       real*8 dopo(14,*),deno(mkind,*)
+      logical flipos(matom)
+      do i = 1,matom
+         flipos(i) = .false.
+      end do
+      call ref_setpar_js(1,flipos,thisvol,dpar,hpo,opo,dparo,dhpo,
+     $     dopo,deno,ierr)
+      return
+      end
+c
+c     Spin channel jspin of a spin-polarised set: param(:,2) from
+c     pdn(npar), atom ia flipped where iflip(ia) is non-zero.
+c
+      subroutine ref_setpar_spin(jspin,thisvol,pdn,npar,iflip,dpar,
+     $     hpo,opo,dparo,dhpo,dopo,deno,ierr)
+     $     bind(C,name='ref_setpar_spin')
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      integer jspin,npar,ierr,iflip(*)
+      real*8 pdn(npar)
+      real*8 dpar(4,*),hpo(14,*),opo(14,*),dparo(4,*),dhpo(14,*)
+      real*8 dopo(14,*),deno(mkind,*)
+      common /codes/posn(matom,3),kkind(matom),natoms
+      common /params/ param(npard,mspin)
+      logical flipos(matom)
+      do i = 1,npard
+         param(i,2) = 0d0
+      end do
+      do i = 1,npar
+         param(i,2) = pdn(i)
+      end do
+      do i = 1,matom
+         flipos(i) = .false.
+      end do
+      do i = 1,natoms
+         flipos(i) = iflip(i).ne.0
+      end do
+      call ref_setpar_js(jspin,flipos,thisvol,dpar,hpo,opo,dparo,dhpo,
+     $     dopo,deno,ierr)
+      return
+      end
+c
+      subroutine ref_setpar_js(jspin,flipos,thisvol,dpar,hpo,opo,dparo,
+     $     dhpo,dopo,deno,ierr)
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      integer jspin,ierr
+      logical flipos(matom)
+      real*8 dpar(4,*),hpo(14,*),opo(14,*),dparo(4,*),dhpo(14,*)
+      real*8 dopo(14,*),deno(mkind,*)
       common /codes/posn(matom,3),kkind(matom),natoms
       common /pairs/tt_list(3,mpair),
      $     dlv_list(3,mpair),dist_list(mpair),
@@ -148,10 +197,6 @@ c
      $     over(mpkind,mpair),dparos(4,matom),dhop(mpkind,mpair),
      $     dover(mpkind,mpair),den(mkind,matom,mspin)
       real*8 spinos(4,matom),dspnos(4,matom)
-      logical flipos(matom)
-      do i = 1,matom
-         flipos(i) = .false.
-      end do
       do ip = 1,npairs
          do j = 1,mpkind
             hop(j,ip) = 0d0
@@ -160,10 +205,10 @@ c
             dover(j,ip) = 0d0
          end do
       end do
-      call setpar(1,param,jsover,den,flipos,.false.,
+      call setpar(jspin,param,jsover,den,flipos,.false.,
      $     paros,spinos,hop,over,.false.,ierr)
       if(ierr.ne.0) return
-      call setparv(1,param,thisvol,jsover,den,flipos,.false.,
+      call setparv(jspin,param,thisvol,jsover,den,flipos,.false.,
      $     dparos,dspnos,dhop,dover,.false.,ierr)
       do ia = 1,natoms
          do k = 1,4
@@ -344,6 +389,60 @@ c
       end do
       label = 'ref_bande'
       call bande(mode,1,flipos,.false.,spinat,.false.,dspth,dspphi,
+     $     .false.,thisvol,zfill,tkb,eigcut,wt,efermi,earg1,earg2,
+     $     smag,.false.,.false.,.false.,0,label,iberr)
+      return
+      end
+c
+c     The same with jspins = 2 (collinear): param(:,2) from pdn(npar),
+c     atom ia flipped where iflip(ia) is non-zero; smag returned.
+c
+      subroutine ref_bande_spin(mode,nk,kp,w,thisvol,zfill,tkb,eigcut,
+     $     pdn,npar,iflip,efermi,earg1,earg2,smag,iberr)
+     $     bind(C,name='ref_bande_spin')
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      include 'P3'
+      integer mode,nk,npar,iberr,iflip(*)
+      real*8 kp(3,*),w(*),thisvol,zfill,tkb,eigcut,pdn(npar)
+      real*8 efermi,earg1,earg2,smag
+      logical master,multiprc
+      common /MPIcomm/ MPISIZE,MPIRANK,master,multiprc
+      logical jsover
+      common /lia/ npts,jsover
+      common /cd1/ x(nkd),y(nkd),z(nkd)
+      common /refeig/ esk(mh,nkdproc),wt(nkd)
+      common /codes/posn(matom,3),kkind(matom),natoms
+      common /params/ param(npard,mspin)
+      logical flipos(matom)
+      real*8 spinat(3,matom),dspth(3,matom),dspphi(3,matom)
+      character*20 label
+      iberr = 0
+      if(nk.gt.nkd.or.mspin.lt.2) then
+         iberr = 10
+         return
+      end if
+      do i = 1,npard
+         param(i,2) = 0d0
+      end do
+      do i = 1,npar
+         param(i,2) = pdn(i)
+      end do
+      do i = 1,matom
+         flipos(i) = .false.
+      end do
+      do i = 1,natoms
+         flipos(i) = iflip(i).ne.0
+      end do
+      npts = nk
+      do k = 1,nk
+         x(k) = kp(1,k)
+         y(k) = kp(2,k)
+         z(k) = kp(3,k)
+         wt(k) = w(k)
+      end do
+      label = 'ref_bande_spin'
+      call bande(mode,2,flipos,.false.,spinat,.false.,dspth,dspphi,
      $     .false.,thisvol,zfill,tkb,eigcut,wt,efermi,earg1,earg2,
      $     smag,.false.,.false.,.false.,0,label,iberr)
       return

@@ -1,0 +1,168 @@
+"""Collinear spin polarisation on the device (tbk_plan_create_spin, and
+bande_ with jspins = 2 in the reference program) against the reference's
+own bande with jspins = 2 (tests/golden/spin.json) and against properties
+the physics fixes:
+* equal channels give the unpolarised result and no moment;
+* exchanging the channels leaves E_F and the energies and reverses the
+  moment;
+* with channels that differ only in their decay and on-site values,
+  flipping every atom equals exchanging the channels.
+"""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import spin_case
+from conftest import ROOT
+from tbk._lib import TBK_MIXED
+from tbk import structures as S
+from tbk.api import Plan, SpinPlan
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(ROOT, "tests", "golden", "spin.json")
+EXE = os.path.join(ROOT, "oracle", "_ref", "static_tbk")
+EX1 = os.path.join(ROOT, "oracle", "_ref", "ex1")
+
+
+def load():
+    return json.load(open(GOLD))
+
+
+def structure(c, params=None):
+    p = params or S.pd_like_params()
+    return S.Structure(np.array(c["lat"]), np.array(c["posn"]), np.ones(len(c["posn"]), np.int32), p,
+                       nd=(4, 4, 4))
+
+
+def with_param(p, values):
+    q = S.NRLParams(p.kinds, p.kbas, p.valence, p.dnn, p.screenl, np.array(values, float), p.nltype,
+                    p.jsover, p.title)
+    return q
+
+
+@pytest.mark.parametrize("name", ["bcc2_ferro", "bcc2_flip", "fcc4_flip"])
+def test_spin_plan_vs_reference(name):
+    g = load()
+    c = {x["name"]: x for x in g["cases"]}[name]
+    st = structure(c)
+    print(f"case: spin plan {name} FP64, {len(c['weights'])} k, modes 1-4", flush=True)
+    plan = SpinPlan(st, c["param_dn"], c["kpts"], c["weights"], flipos=c["flipos"])
+    worst = 0.0
+    for mode, want in c["modes"].items():
+        r = plan.run(int(mode), g["tkb"], g["eigcut"])
+        for key in ("efermi", "earg1", "earg2", "smag"):
+            d = abs(r[key] - want[key])
+            worst = max(worst, d)
+            assert d <= 1e-9, (mode, key, r[key], want[key])
+    print(f"  worst |device - reference| = {worst:.1e}", flush=True)
+    plan.close()
+
+
+def test_spin_plan_mixed_precision():
+    g = load()
+    c = {x["name"]: x for x in g["cases"]}["fcc4_flip"]
+    st = structure(c)
+    print("case: spin plan fcc4_flip mixed, mode 4", flush=True)
+    plan = SpinPlan(st, c["param_dn"], c["kpts"], c["weights"], flipos=c["flipos"], precision=TBK_MIXED)
+    r = plan.run(4, g["tkb"], g["eigcut"])
+    want = c["modes"]["4"]
+    for key in ("efermi", "earg1", "earg2", "smag"):
+        assert abs(r[key] - want[key]) <= 1e-6, (key, r[key], want[key])
+    plan.close()
+
+
+def test_equal_channels_are_unpolarised():
+    c = load()["cases"][0]
+    st = structure(c)
+    print("case: spin plan with equal channels vs unpolarised plan", flush=True)
+    a = SpinPlan(st, st.params.param, c["kpts"], c["weights"])
+    b = Plan(st, c["kpts"], c["weights"])
+    for mode in (1, 2, 3, 4):
+        ra, rb = a.run(mode), b.run(mode)
+        assert abs(ra["smag"]) <= 1e-12
+        for key in ("efermi", "earg1", "earg2"):
+            assert abs(ra[key] - rb[key]) <= 1e-10, (mode, key)
+    eu, _ = a.eigenvalues(0)
+    ed, _ = a.eigenvalues(1)
+    e1, _ = b.eigenvalues()
+    assert np.array_equal(eu, ed) and np.abs(eu - e1).max() <= 1e-12
+    a.close(); b.close()
+
+
+def test_exchanged_channels_reverse_the_moment():
+    c = {x["name"]: x for x in load()["cases"]}["bcc2_flip"]
+    p = S.pd_like_params()
+    up, dn = np.array(p.param), np.array(c["param_dn"])
+    print("case: spin plan with exchanged channels (bcc2_flip)", flush=True)
+    a = SpinPlan(structure(c, with_param(p, up)), dn, c["kpts"], c["weights"], flipos=c["flipos"])
+    b = SpinPlan(structure(c, with_param(p, dn)), up, c["kpts"], c["weights"], flipos=c["flipos"])
+    ra, rb = a.run(4), b.run(4)
+    assert abs(ra["smag"]) > 0.05
+    assert abs(ra["smag"] + rb["smag"]) <= 1e-10
+    for key in ("efermi", "earg1", "earg2"):
+        assert abs(ra[key] - rb[key]) <= 1e-10, key
+    a.close(); b.close()
+
+
+def test_flipping_every_atom_equals_exchanging_channels():
+    c = {x["name"]: x for x in load()["cases"]}["fcc4_flip"]
+    p = S.pd_like_params()
+    up = np.array(p.param)
+    dn = spin_case.spin_down_values(up, seed=9, n_changed=17)      # decay and on-site only
+    nat = len(c["posn"])
+    print("case: all atoms flipped vs channels exchanged (fcc4)", flush=True)
+    a = SpinPlan(structure(c, with_param(p, up)), dn, c["kpts"], c["weights"], flipos=[1] * nat)
+    b = SpinPlan(structure(c, with_param(p, dn)), up, c["kpts"], c["weights"])
+    n = SpinPlan(structure(c, with_param(p, up)), dn, c["kpts"], c["weights"])
+    ra, rb, rn = a.run(4), b.run(4), n.run(4)
+    assert abs(rn["smag"]) > 0.01
+    for key in ("efermi", "earg1", "earg2", "smag"):
+        assert abs(ra[key] - rb[key]) <= 1e-10, key
+    assert abs(ra["smag"] + rn["smag"]) <= 1e-10
+    a.close(); b.close(); n.close()
+
+
+def test_spin_plan_batches_and_graph_bitwise():
+    g = load()
+    c = {x["name"]: x for x in g["cases"]}["bcc2_flip"]
+    st = structure(c)
+    print("case: spin plan batch 16 + graph vs one batch (bcc2_flip)", flush=True)
+    a = SpinPlan(st, c["param_dn"], c["kpts"], c["weights"], flipos=c["flipos"])
+    b = SpinPlan(st, c["param_dn"], c["kpts"], c["weights"], flipos=c["flipos"], batch=16, graph=True)
+    for mode in (4, 3, 4):
+        ra, rb = a.run(mode), b.run(mode)
+        assert ra == rb, mode
+    a.close(); b.close()
+
+
+@pytest.mark.skipif(not os.path.exists(EXE) or not os.path.isdir(EX1),
+                    reason="drop-in program not built (oracle/Makefile dropin)")
+def test_reference_program_with_device_bande_magnetic(tmp_path):
+    """ex1 with pd_par made a two-channel file: the reference program's
+    jspins = 2 path through the device bande_; SKENG rows (with the moment
+    column) agree with the unmodified program's to their printed digits."""
+    g = load()["ex1_magnetic"]
+    work = tmp_path / "ex1"
+    shutil.copytree(EX1, work)
+    src = (work / "pd_par").read_text()
+    (work / "pd_par").unlink()
+    (work / "pd_par").write_text(spin_case.nm_text(src, np.array(g["param_dn"])))
+    print("case: static_tbk on ex1 with a two-channel pd_par", flush=True)
+    r = subprocess.run("ulimit -s unlimited; " + EXE, shell=True, cwd=work, capture_output=True, text=True,
+                       timeout=300, env=dict(os.environ, TBK_DEVICE="0"))
+    assert r.returncode == 0, (r.stdout[-1000:], r.stderr[-2000:])
+    rows = spin_case.read_skeng(work / "SKENG")
+    assert len(rows) == len(g["skeng"])
+    worst = worst_m = 0.0
+    for row, want in zip(rows, g["skeng"]):
+        assert row[0] == want[0] and row[1] == want[1]
+        worst = max(worst, max(abs(a - b) for a, b in zip(row[2:5], want[2:5])))
+        worst_m = max(worst_m, abs(row[5] - want[5]))
+    print(f"  worst |SKENG - reference program| = {worst:.1e} (9 decimals), moment {worst_m:.1e} (6)",
+          flush=True)
+    assert worst <= 1.5e-9 and worst_m <= 1.5e-6

@@ -51,12 +51,15 @@ struct PairList {
 int search2(int natoms, const double *lat, const double *posn, const int *kkind,
             int kinds, const int *nd, const double *dnn, const double *screenl,
             int maxpairs, PairList &out);
+// param: the spin channel being set up; atoms with flip[a] != 0 take their
+// density decay and on-site terms from param_flip (the other channel).
 int setpar(int natoms, const int *kkind, int kinds, const double *param, int nltype,
            int jsover, const PairList &pl, double *den, double *paros, double *hop,
-           double *over);
+           double *over, const double *param_flip = nullptr, const int *flip = nullptr);
 int setparv(int natoms, const int *kkind, int kinds, const double *param, int nltype,
             int jsover, double thisvol, const PairList &pl, const double *den,
-            double *dparos, double *dhop, double *dover);
+            double *dparos, double *dhop, double *dover, const double *param_flip = nullptr,
+            const int *flip = nullptr);
 void sk_block(double l, double m, double n, const double *p, bool unlike, double *e);
 void arrange_integrals(const double *stored, int k1, int k2, double *out);
 

@@ -112,8 +112,11 @@ struct FermiArgs {
     double el, tkb, zfill;
     const double *esk, *desk, *w;
     const int *ncut;
-    double *out;                   // efermi, earg1, earg2, esum, entropy, iters
+    double *out;                   // efermi, earg1, earg2, esum, entropy, iters, smag
     int *cut = nullptr;            // [nk] occupation window per k (k_fermi_window)
+    // > 0: items [0, khalf) are spin up and [khalf, nk) spin down of one
+    // structure (collinear, el = 1); out[6] = up - down electrons
+    int khalf = 0;
 };
 
 // GEMM tiles and their choice (tbk_tune.cpp)

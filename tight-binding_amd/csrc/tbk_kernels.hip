@@ -2005,13 +2005,16 @@ __device__ double fermi_level(const FermiArgs &f, double *s1, double *s2, int &i
     int nfill = (int)(zfill / el + nearz);
     const bool linst = fabs(zfill / el - (double)nfill) < tol;
     if (nfill < 1) nfill = 1;
-    if (nfill > nv - 1) nfill = nv > 1 ? nv - 1 : 1;
-    // bracket (fermi.f:197-227)
+    // bracket (fermi.f:197-227).  Levels nfill and nfill + 1 lie above the
+    // nv computed ones when the bands are full, as with one electron per
+    // state (jspins = 2, el = 1: nfill = zfill); the reference then reads
+    // rows of its eigenvalue array that no k-point wrote, zero in its
+    // static storage, and so does this search.
     double stlow = -DBL_MAX, sthigh = DBL_MAX, mulow = DBL_MAX, muhigh = -DBL_MAX;
     for (int k = tid; k < nk; k += blockDim.x) {
         const double *ev = f.esk + (size_t)k * ld;
-        stlow = fmax(stlow, ev[nfill - 1]);
-        sthigh = fmin(sthigh, ev[nfill]);
+        stlow = fmax(stlow, nfill - 1 < nv ? ev[nfill - 1] : 0.0);
+        sthigh = fmin(sthigh, nfill < nv ? ev[nfill] : 0.0);
         mulow = fmin(mulow, ev[0]);
         muhigh = fmax(muhigh, ev[nv - 1]);
     }
@@ -2089,10 +2092,20 @@ __device__ void fermi_sums_body(const FermiArgs &f, int ef_known)
     const bool lentro = f.mode < 3;
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     // esum desum1 desum2 desum3 dfbsum dferm entropy dent1 dent2
+    double occ = 0.0;               // spin up minus spin down (bande.f:1342-1402)
     for (int k = tid; k < nk; k += blockDim.x) {
         const double *ev = f.esk + (size_t)k * ld;
         const double *dv = f.desk + (size_t)k * ld;
         const double wt = el * f.w[k];
+        if (f.khalf > 0) {
+            double c = 0.0;
+            for (int i = 0; i < nv; ++i) {
+                const double x = ev[i];
+                if (x < efm) c = c + f.w[k];
+                else if (x < efp) c = c + f.w[k] * (1.0 / (1.0 + exp(tkbi * (x - efermi))));
+            }
+            occ = k < f.khalf ? occ + c : occ - c;
+        }
         const int ni = lpress ? f.ncut[k] : nv;
         for (int i = 0; i < ni; ++i) {
             const double x = ev[i];
@@ -2125,6 +2138,7 @@ __device__ void fermi_sums_body(const FermiArgs &f, int ef_known)
     }
     double tot[9];
     for (int q = 0; q < 9; ++q) tot[q] = block_sum_tree(acc[q], s1);
+    const double smag = f.khalf > 0 ? block_sum_tree(occ, s1) : 0.0;
     if (tid == 0) {
         const double esum = tot[0];
         double e1, e2 = 0.0;
@@ -2139,6 +2153,7 @@ __device__ void fermi_sums_body(const FermiArgs &f, int ef_known)
         f.out[0] = efermi; f.out[1] = e1; f.out[2] = e2;
         f.out[3] = esum; f.out[4] = tot[6];
         f.out[5] = (double)iters;
+        f.out[6] = smag;
     }
 }
 

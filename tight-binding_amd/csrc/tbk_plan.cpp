@@ -496,7 +496,11 @@ struct tbk_plan {
     double *dvec = nullptr, *eh = nullptr, *es = nullptr, *edh = nullptr, *eds = nullptr;
     double *paros = nullptr, *dparos = nullptr, *kp = nullptr, *wts = nullptr;
     tbk::StructDev *sdev = nullptr;        // [nstruct]
-    tbk::FermiArgs *fargs = nullptr;       // [nstruct]
+    tbk::FermiArgs *fargs = nullptr;       // [nfermi]
+    // spin: a collinear spin-polarised structure, structure 0 the spin-up
+    // and structure 1 the spin-down channel, one Fermi level over both
+    bool spin = false;
+    int nfermi = 0;                        // Fermi groups: nstruct, or 1 with spin
     int *item_s = nullptr, *nact = nullptr; // [nk]
     // results, per item with stride N
     double *esk = nullptr, *desk = nullptr, *out = nullptr;   // out [nstruct][8]
@@ -562,7 +566,12 @@ struct SetupArrays {
 };
 struct SetupOffsets { size_t rows, atoms, blk, pairs; };
 
-static int setup_structure(const tbk_structure_desc *st, SetupArrays &A, StructHost &h, SetupOffsets &off)
+// Spin channel selection of a structure's set-up: atoms with flip[a] != 0
+// take their densities and on-site terms from param_flip.
+struct SpinSel { const double *param_flip = nullptr; const int *flip = nullptr; };
+
+static int setup_structure(const tbk_structure_desc *st, SetupArrays &A, StructHost &h, SetupOffsets &off,
+                           const SpinSel &sel = SpinSel())
 {
     if (!st || st->natoms <= 0) return TBK_ERR_ARG;
     const int nat = st->natoms, kinds = st->kinds;
@@ -576,9 +585,9 @@ static int setup_structure(const tbk_structure_desc *st, SetupArrays &A, StructH
     std::vector<double> hop(14 * (size_t)np), over(14 * (size_t)np), dhop(14 * (size_t)np),
         dover(14 * (size_t)np);
     tbk::setpar(nat, st->kkind, kinds, st->param, st->nltype, st->jsover, pl, den.data(),
-                paros.data(), hop.data(), over.data());
+                paros.data(), hop.data(), over.data(), sel.param_flip, sel.flip);
     tbk::setparv(nat, st->kkind, kinds, st->param, st->nltype, st->jsover, st->thisvol, pl,
-                 den.data(), dparos.data(), dhop.data(), dover.data());
+                 den.data(), dparos.data(), dhop.data(), dover.data(), sel.param_flip, sel.flip);
     std::vector<int> ksk(nat), nbas(nat);
     int nv = 0;
     for (int a = 0; a < nat; ++a) {
@@ -642,9 +651,8 @@ static void append_setup(SetupArrays &A, const SetupArrays &L, SetupOffsets &off
     cat(A.edh, L.edh); cat(A.eds, L.eds); cat(A.paros, L.paros); cat(A.dparos, L.dparos);
 }
 
-extern "C" int tbk_plan_create_group(int device, int nstruct, const tbk_structure_desc *st, const int *nk,
-                                     const double *kpts, const double *weights, int batch, int precision,
-                                     tbk_plan **out)
+static int plan_create(int device, int nstruct, const tbk_structure_desc *st, const int *nk, const double *kpts,
+                       const double *weights, int batch, int precision, const SpinSel *sel, tbk_plan **out)
 {
     if (!st || !out || !nk || nstruct <= 0 || !kpts || !weights) return TBK_ERR_ARG;
     if (precision != TBK_FP64 && precision != TBK_MIXED) return TBK_ERR_ARG;
@@ -664,7 +672,9 @@ extern "C" int tbk_plan_create_group(int device, int nstruct, const tbk_structur
         std::vector<SetupArrays> loc(nstruct);
         std::vector<int> rcs(nstruct, TBK_OK);
         std::vector<SetupOffsets> unused(nstruct);
-        tbk::parallel_for(nstruct, [&](int i) { rcs[i] = setup_structure(st + i, loc[i], sh[i], unused[i]); });
+        tbk::parallel_for(nstruct, [&](int i) {
+            rcs[i] = setup_structure(st + i, loc[i], sh[i], unused[i], sel ? sel[i] : SpinSel());
+        });
         for (int i = 0; i < nstruct; ++i) {
             if (rcs[i]) return rcs[i];
             append_setup(A, loc[i], off[i]);
@@ -684,6 +694,7 @@ extern "C" int tbk_plan_create_group(int device, int nstruct, const tbk_structur
     g_dev_bytes = 0;
     p->device = device;
     p->nstruct = nstruct; p->N = N; p->nk = nktot; p->precision = precision;
+    p->nfermi = nstruct;
     p->maxnat = maxnat; p->maxpairs = maxpairs;
     p->sh = sh;
     p->kpts.assign(kpts, kpts + 3 * (size_t)nktot);
@@ -744,6 +755,48 @@ extern "C" int tbk_plan_create_group(int device, int nstruct, const tbk_structur
     return TBK_OK;
 }
 
+extern "C" int tbk_plan_create_group(int device, int nstruct, const tbk_structure_desc *st, const int *nk,
+                                     const double *kpts, const double *weights, int batch, int precision,
+                                     tbk_plan **out)
+{
+    return plan_create(device, nstruct, st, nk, kpts, weights, batch, precision, nullptr, out);
+}
+
+// bande.f:816-889 with jspins = 2: setpar/setparv once per spin channel
+// (bond integrals from that channel, densities and on-site terms of flipped
+// atoms from the other), both channels' k-points solved in the same
+// launches, one Fermi level with one electron per state (bande.f:1069-1081)
+// and the sums over both channels.
+extern "C" int tbk_plan_create_spin(int device, const tbk_structure_desc *st, const double *param_dn,
+                                    const int *flipos, int nk, const double *kpts, const double *weights,
+                                    int batch, int precision, tbk_plan **out)
+{
+    if (!st || !param_dn || !st->param || nk <= 0 || !kpts || !weights || !out) return TBK_ERR_ARG;
+    tbk_structure_desc d[2] = {*st, *st};
+    d[1].param = param_dn;
+    SpinSel sel[2];
+    sel[0].param_flip = param_dn; sel[0].flip = flipos;
+    sel[1].param_flip = st->param; sel[1].flip = flipos;
+    const int nks[2] = {nk, nk};
+    std::vector<double> k2(6 * (size_t)nk), w2(2 * (size_t)nk);
+    for (int c = 0; c < 2; ++c) {
+        std::copy(kpts, kpts + 3 * (size_t)nk, k2.begin() + 3 * (size_t)nk * c);
+        std::copy(weights, weights + nk, w2.begin() + (size_t)nk * c);
+    }
+    const int rc = plan_create(device, 2, d, nks, k2.data(), w2.data(), batch, precision, sel, out);
+    if (rc) return rc;
+    (*out)->spin = true;
+    (*out)->nfermi = 1;
+    return TBK_OK;
+}
+
+extern "C" int tbk_plan_magnetization(const tbk_plan *p, double *smag)
+{
+    if (!p || !smag) return TBK_ERR_ARG;
+    *smag = p->spin ? p->out_host[6] : 0.0;
+    return TBK_OK;
+}
+
 extern "C" int tbk_plan_create(int device, const tbk_structure_desc *st, int nk,
                                const double *kpts, const double *weights, int batch,
                                int precision, tbk_plan **out)
@@ -783,7 +836,7 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
     // Energy-only modes (1, 3) need no eigenvectors at all.
     const bool win = p->window && lpress && p->batch >= p->nk;
     const bool vecs = lpress || !p->window;
-    const Window wnd{win ? p->fargs : nullptr, p->nstruct, win ? p->cut : nullptr, vecs};
+    const Window wnd{win ? p->fargs : nullptr, p->nfermi, win ? p->cut : nullptr, vecs};
     p->win_last = win;
     p->vec_last = vecs;
     for (int k0 = 0; k0 < p->nk; k0 += p->batch) {
@@ -817,7 +870,7 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
         }
     }
     p->timer.begin("fermi_sums");
-    tbk::launch_fermi_sums_group(p->fargs, p->nstruct, st, win ? 1 : 0);
+    tbk::launch_fermi_sums_group(p->fargs, p->nfermi, st, win ? 1 : 0);
     p->timer.end();
 }
 
@@ -832,8 +885,8 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
         !(p->g_mode == mode && p->g_tkb == tkb && p->g_eigcut == eigcut && p->g_window == p->window);
     if (p->f_mode != mode || p->f_tkb != tkb) {
         // per-structure Fermi arguments of this (mode, kT)
-        std::vector<tbk::FermiArgs> fa(p->nstruct);
-        for (int i = 0; i < p->nstruct; ++i) {
+        std::vector<tbk::FermiArgs> fa(p->nfermi);
+        for (int i = 0; i < p->nfermi; ++i) {
             const StructHost &h = p->sh[i];
             tbk::FermiArgs &f = fa[i];
             f.nk = h.nk; f.nv = h.nv; f.ld = p->N; f.mode = mode;
@@ -841,6 +894,11 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
             f.esk = p->esk + (size_t)h.k0 * p->N; f.desk = p->desk + (size_t)h.k0 * p->N;
             f.w = p->wts + h.k0; f.ncut = p->ncut + h.k0; f.out = p->out + 8 * (size_t)i;
             f.cut = p->cut + h.k0;
+            if (p->spin) {          // both channels' items follow each other
+                f.nk = p->sh[0].nk + p->sh[1].nk;
+                f.el = 1.0;
+                f.khalf = p->sh[0].nk;
+            }
         }
         HIP_TRY(hipMemcpyAsync(p->fargs, fa.data(), fa.size() * sizeof(tbk::FermiArgs),
                                hipMemcpyHostToDevice, p->stream));
@@ -926,6 +984,10 @@ extern "C" int tbk_plan_result(const tbk_plan *p, int s, double *efermi, double 
 {
     if (!p || s < 0 || s >= p->nstruct) return TBK_ERR_ARG;
     if (p->status[s]) return p->status[s];
+    if (p->spin) {                  // one result for both channels
+        if (p->status[1 - s]) return p->status[1 - s];
+        s = 0;
+    }
     if (efermi) *efermi = p->out_host[8 * (size_t)s];
     if (earg1) *earg1 = p->out_host[8 * (size_t)s + 1];
     if (earg2) *earg2 = p->out_host[8 * (size_t)s + 2];

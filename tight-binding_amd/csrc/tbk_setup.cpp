@@ -128,12 +128,16 @@ int search2(int natoms, const double *lat, const double *posn, const int *kkind,
     return TBK_OK;
 }
 
-// setpar.f:398-884 (collinear, jspin = 1, no spin flips).
+// setpar.f:398-884, collinear.  param: the channel of jspin (bond and
+// overlap integrals); an atom with flip[a] != 0 takes its density decay and
+// its on-site terms from param_flip, the other channel (setpar.f:500-536,
+// 795-867: jsfrm = 3 - jspin).  flip may be null (no flips).
 int setpar(int natoms, const int *kkind, int kinds, const double *param, int nltype,
            int jsover, const PairList &pl, double *den, double *paros, double *hop,
-           double *over)
+           double *over, const double *param_flip, const int *flip)
 {
     const ParamView P{param};
+    auto site = [&](int a) { return ParamView{flip && flip[a] ? param_flip : param}; };
     const bool realov = nltype == 1 || nltype == 3;
     const bool harrison = nltype == 2 || nltype == 3;
     const double twtrd = 2.0 / 3.0;
@@ -145,11 +149,13 @@ int setpar(int natoms, const int *kkind, int kinds, const double *param, int nlt
         const int ki = pr.kind[0], kj = pr.kind[1];
         const double r = pr.dist, f = pr.screen;
         const int i0 = onsite_base(kj) + 1;
-        const double b = P(i0) * P(i0);
+        const ParamView Pi = site(ia);
+        const double b = Pi(i0) * Pi(i0);
         den[ia * kinds + kj - 1] = den[ia * kinds + kj - 1] + std::exp(-b * r) * f;
         if (ja != ia) {
             const int i1 = onsite_base(ki) + 1;
-            const double b1 = P(i1) * P(i1);
+            const ParamView Pj = site(ja);
+            const double b1 = Pj(i1) * Pj(i1);
             den[ja * kinds + ki - 1] = den[ja * kinds + ki - 1] + std::exp(-b1 * r) * f;
         }
     }
@@ -202,11 +208,12 @@ int setpar(int natoms, const int *kkind, int kinds, const double *param, int nlt
     }
     });
     for (int ia = 0; ia < natoms; ++ia) {      // setpar.f:795-867
+        const ParamView Ps = site(ia);
         const int k1 = kkind[ia];
         int b = onsite_base(k1) + 1;
         double v = std::pow(den[ia * kinds + k1 - 1], twtrd);
         for (int c = 0; c < 4; ++c, b += 4)
-            paros[4 * ia + c] = P(b + 1) + v * (P(b + 2) + v * (P(b + 3) + v * P(b + 4)));
+            paros[4 * ia + c] = Ps(b + 1) + v * (Ps(b + 2) + v * (Ps(b + 3) + v * Ps(b + 4)));
         for (int k2 = 1; k2 <= kinds; ++k2) {
             if (k2 == k1 || den[ia * kinds + k2 - 1] == 0.0) continue;
             v = std::pow(den[ia * kinds + k2 - 1], twtrd);
@@ -214,18 +221,20 @@ int setpar(int natoms, const int *kkind, int kinds, const double *param, int nlt
             b = kinds * (1 + kOnsite * 4) + 6 * kOnsite * ((hi - 1) * (hi - 2) / 2 + lo - 1);
             if (k2 < hi) b += 3 * kOnsite;
             for (int c = 0; c < 4; ++c, b += 3)
-                paros[4 * ia + c] = paros[4 * ia + c] + v * (P(b + 1) + v * (P(b + 2) + v * P(b + 3)));
+                paros[4 * ia + c] = paros[4 * ia + c] + v * (Ps(b + 1) + v * (Ps(b + 2) + v * Ps(b + 3)));
         }
     }
     return TBK_OK;
 }
 
-// setparv.f:426-886.
+// setparv.f:426-886, collinear; param, param_flip and flip as in setpar
+// (setparv.f:507-539, 800-884).
 int setparv(int natoms, const int *kkind, int kinds, const double *param, int nltype,
             int jsover, double thisvol, const PairList &pl, const double *den,
-            double *dparos, double *dhop, double *dover)
+            double *dparos, double *dhop, double *dover, const double *param_flip, const int *flip)
 {
     const ParamView P{param};
+    auto site = [&](int a) { return ParamView{flip && flip[a] ? param_flip : param}; };
     const bool realov = nltype == 1 || nltype == 3;
     const bool harrison = nltype == 2 || nltype == 3;
     const double third = 1.0 / 3.0, twtrd = 2.0 / 3.0, threeh = 3.0 / 2.0;
@@ -238,12 +247,14 @@ int setparv(int natoms, const int *kkind, int kinds, const double *param, int nl
         const double r = pr.dist, f = pr.screen, df = pr.dscrn;
         const double vdist = third * r / thisvol;          // setparv.f:459
         const int i0 = onsite_base(kj) + 1;
-        const double b = P(i0) * P(i0);
+        const ParamView Pi = site(ia);
+        const double b = Pi(i0) * Pi(i0);
         const double dent = std::exp(-b * r) * f;
         vden[ia * kinds + kj - 1] = vden[ia * kinds + kj - 1] + (df - b) * dent * vdist;
         if (ja != ia) {
             const int i1 = onsite_base(ki) + 1;
-            const double b1 = P(i1) * P(i1);
+            const ParamView Pj = site(ja);
+            const double b1 = Pj(i1) * Pj(i1);
             const double dent1 = std::exp(-b1 * r) * f;
             vden[ja * kinds + ki - 1] = vden[ja * kinds + ki - 1] + (df - b1) * dent1 * vdist;
         }
@@ -313,6 +324,7 @@ int setparv(int natoms, const int *kkind, int kinds, const double *param, int nl
     }
     });
     for (int ia = 0; ia < natoms; ++ia) {
+        const ParamView Ps = site(ia);
         const int k1 = kkind[ia];
         int b = onsite_base(k1) + 1;
         double v13 = std::pow(den[ia * kinds + k1 - 1], third);
@@ -320,7 +332,7 @@ int setparv(int natoms, const int *kkind, int kinds, const double *param, int nl
         double v2t = 2.0 * v23, v3t = threeh * v23;
         double drho = twtrd * vden[ia * kinds + k1 - 1] / v13;
         for (int c = 0; c < 4; ++c, b += 4)
-            dparos[4 * ia + c] = drho * (P(b + 2) + v2t * (P(b + 3) + v3t * P(b + 4)));
+            dparos[4 * ia + c] = drho * (Ps(b + 2) + v2t * (Ps(b + 3) + v3t * Ps(b + 4)));
         for (int k2 = 1; k2 <= kinds; ++k2) {
             if (k2 == k1 || den[ia * kinds + k2 - 1] == 0.0) continue;
             v13 = std::pow(den[ia * kinds + k2 - 1], third);
@@ -331,7 +343,7 @@ int setparv(int natoms, const int *kkind, int kinds, const double *param, int nl
             b = kinds * (1 + kOnsite * 4) + 6 * kOnsite * ((hi - 1) * (hi - 2) / 2 + lo - 1);
             if (k2 < hi) b += 3 * kOnsite;
             for (int c = 0; c < 4; ++c, b += 3)
-                dparos[4 * ia + c] = dparos[4 * ia + c] + drho * (P(b + 1) + v2t * (P(b + 2) + v3t * P(b + 3)));
+                dparos[4 * ia + c] = dparos[4 * ia + c] + drho * (Ps(b + 1) + v2t * (Ps(b + 2) + v3t * Ps(b + 3)));
         }
     }
     return TBK_OK;
@@ -496,6 +508,31 @@ extern "C" int tbk_setpar(int natoms, const int *kkind, int kinds, const double 
     if (natoms <= 0 || npairs < 0 || !param) return TBK_ERR_ARG;
     tbk::PairList pl = pairs_from_arrays(npairs, dist, screen, nullptr, jkind, jatm);
     return tbk::setpar(natoms, kkind, kinds, param, nltype, jsover, pl, den, paros, hop, over);
+}
+
+extern "C" int tbk_setpar_spin(int natoms, const int *kkind, int kinds, const double *param2, int nparam,
+                               int jspin, const int *flipos, int nltype, int jsover, int npairs,
+                               const double *dist, const double *screen, const int *jkind, const int *jatm,
+                               double *den, double *paros, double *hop, double *over)
+{
+    if (natoms <= 0 || npairs < 0 || !param2 || nparam < 1 || (jspin != 1 && jspin != 2)) return TBK_ERR_ARG;
+    tbk::PairList pl = pairs_from_arrays(npairs, dist, screen, nullptr, jkind, jatm);
+    const double *own = param2 + (size_t)(jspin - 1) * nparam, *other = param2 + (size_t)(2 - jspin) * nparam;
+    return tbk::setpar(natoms, kkind, kinds, own, nltype, jsover, pl, den, paros, hop, over, other, flipos);
+}
+
+extern "C" int tbk_setparv_spin(int natoms, const int *kkind, int kinds, const double *param2, int nparam,
+                                int jspin, const int *flipos, int nltype, int jsover, double thisvol,
+                                int npairs, const double *dist, const double *screen, const double *dscrn,
+                                const int *jkind, const int *jatm, const double *den, double *dparos,
+                                double *dhop, double *dover)
+{
+    if (natoms <= 0 || npairs < 0 || !param2 || nparam < 1 || (jspin != 1 && jspin != 2) || thisvol <= 0)
+        return TBK_ERR_ARG;
+    tbk::PairList pl = pairs_from_arrays(npairs, dist, screen, dscrn, jkind, jatm);
+    const double *own = param2 + (size_t)(jspin - 1) * nparam, *other = param2 + (size_t)(2 - jspin) * nparam;
+    return tbk::setparv(natoms, kkind, kinds, own, nltype, jsover, thisvol, pl, den, dparos, dhop, dover,
+                        other, flipos);
 }
 
 extern "C" int tbk_setparv(int natoms, const int *kkind, int kinds, const double *param,

@@ -29,9 +29,12 @@ def _ip(a):
 
 class HostSetup:
     """Neighbour list and SK parameters from the library's host set-up
-    (search2.f, setpar.f, setparv.f restated bit-exactly)."""
+    (search2.f, setpar.f, setparv.f restated bit-exactly).  With param_dn,
+    spin channel jspin of a spin-polarised set (st.params the spin-up
+    channel; flipos the flipped atoms)."""
 
-    def __init__(self, st: Structure, maxpairs: Optional[int] = None):
+    def __init__(self, st: Structure, maxpairs: Optional[int] = None, param_dn=None, jspin: int = 1,
+                 flipos=None):
         L = lib()
         p = st.params
         nat = st.natoms
@@ -58,14 +61,30 @@ class HostSetup:
         self.den = np.zeros(nat * p.kinds); self.paros = np.zeros(4 * nat)
         self.hop = np.zeros(14 * max(n, 1)); self.over = np.zeros(14 * max(n, 1))
         param = np.ascontiguousarray(p.param, float)
-        check(L.tbk_setpar(nat, st.kkind, p.kinds, param, p.nltype, int(p.jsover), n, self.dist,
-                           self.screen, self.jkind, self.jatm, self.den, self.paros, self.hop,
-                           self.over), "tbk_setpar")
         self.dparos = np.zeros(4 * nat)
         self.dhop = np.zeros(14 * max(n, 1)); self.dover = np.zeros(14 * max(n, 1))
-        check(L.tbk_setparv(nat, st.kkind, p.kinds, param, p.nltype, int(p.jsover), st.volume, n,
-                            self.dist, self.screen, self.dscrn, self.jkind, self.jatm, self.den,
-                            self.dparos, self.dhop, self.dover), "tbk_setparv")
+        if param_dn is None:
+            check(L.tbk_setpar(nat, st.kkind, p.kinds, param, p.nltype, int(p.jsover), n, self.dist,
+                               self.screen, self.jkind, self.jatm, self.den, self.paros, self.hop,
+                               self.over), "tbk_setpar")
+            check(L.tbk_setparv(nat, st.kkind, p.kinds, param, p.nltype, int(p.jsover), st.volume, n,
+                                self.dist, self.screen, self.dscrn, self.jkind, self.jatm, self.den,
+                                self.dparos, self.dhop, self.dover), "tbk_setparv")
+        else:
+            p2 = np.ascontiguousarray(np.concatenate([param, np.asarray(param_dn, float)]))
+            if len(p2) != 2 * len(param):
+                raise ValueError("param_dn must have as many values as the spin-up channel")
+            fl = None
+            if flipos is not None:
+                self._flip = np.ascontiguousarray(np.asarray(flipos, np.int32))
+                fl = self._flip.ctypes.data
+            check(L.tbk_setpar_spin(nat, st.kkind, p.kinds, p2, len(param), int(jspin), fl, p.nltype,
+                                    int(p.jsover), n, self.dist, self.screen, self.jkind, self.jatm,
+                                    self.den, self.paros, self.hop, self.over), "tbk_setpar_spin")
+            check(L.tbk_setparv_spin(nat, st.kkind, p.kinds, p2, len(param), int(jspin), fl, p.nltype,
+                                     int(p.jsover), st.volume, n, self.dist, self.screen, self.dscrn,
+                                     self.jkind, self.jatm, self.den, self.dparos, self.dhop, self.dover),
+                  "tbk_setparv_spin")
 
 
 def _structure_desc(st: Structure, keep: list) -> "_lib.StructureDesc":
@@ -236,6 +255,66 @@ class GroupPlan(Plan):
         nk, nv = int(self.nks[s]), self.nvs[s]
         e = np.zeros((nk, nv)); d = np.zeros((nk, nv))
         check(lib().tbk_plan_structure_eigenvalues(self.h, int(s), e.ctypes.data, d.ctypes.data))
+        return e, d
+
+
+class SpinPlan(Plan):
+    """A collinear spin-polarised structure (bande with jspins = 2,
+    bande.f:816-889, 1069-1081): st.params holds the spin-up channel,
+    param_dn the spin-down parameters in the same order (an "NM"/"OM"
+    parameter file's second block, input.f:327-359), flipos the atoms whose
+    densities and on-site terms come from the other channel
+    (setpar.f:500-536).  One E_F over both channels with one electron per
+    state; run() and wait() add "smag", spin-up minus spin-down electrons
+    (bande.f:1388-1402)."""
+
+    def __init__(self, st: Structure, param_dn, kpts, weights, flipos=None, batch: int = 0,
+                 precision: int = TBK_FP64, device: int = 0, graph: bool = False):
+        L = lib()
+        self.st = st
+        self._keep = []
+        desc = _structure_desc(st, self._keep)
+        self.param_dn = np.ascontiguousarray(np.asarray(param_dn, float))
+        if len(self.param_dn) != len(st.params.param):
+            raise ValueError("param_dn must have as many values as the spin-up channel")
+        self.flipos = None
+        if flipos is not None:
+            self.flipos = np.ascontiguousarray(np.asarray(flipos, np.int32))
+            if len(self.flipos) != st.natoms:
+                raise ValueError("flipos needs one entry per atom")
+        self.kpts = np.ascontiguousarray(np.asarray(kpts, float).reshape(-1, 3))
+        self.weights = np.ascontiguousarray(np.asarray(weights, float))
+        self.nk = len(self.weights)
+        h = C.c_void_p()
+        fl = None if self.flipos is None else self.flipos.ctypes.data
+        check(L.tbk_plan_create_spin(device, C.byref(desc), self.param_dn, fl, self.nk, self.kpts.ravel(),
+                                     self.weights, int(batch), int(precision), C.byref(h)),
+              "tbk_plan_create_spin")
+        self.h = h
+        self.nv = L.tbk_plan_nv(h)
+        self.npairs = L.tbk_plan_npairs(h)
+        if graph:
+            L.tbk_plan_set_graph(h, 1)
+
+    def magnetization(self) -> float:
+        m = C.c_double()
+        check(lib().tbk_plan_magnetization(self.h, C.byref(m)), "tbk_plan_magnetization")
+        return m.value
+
+    def run(self, mode: int = 4, tkb: float = 0.005, eigcut: float = 10.0) -> Dict[str, float]:
+        r = super().run(mode, tkb, eigcut)
+        r["smag"] = self.magnetization()
+        return r
+
+    def wait(self) -> Dict[str, float]:
+        r = super().wait()
+        r["smag"] = self.magnetization()
+        return r
+
+    def eigenvalues(self, spin: int = 0):
+        """Eigenvalues and derivatives of channel spin (0 up, 1 down)."""
+        e = np.zeros((self.nk, self.nv)); d = np.zeros((self.nk, self.nv))
+        check(lib().tbk_plan_structure_eigenvalues(self.h, int(spin), e.ctypes.data, d.ctypes.data))
         return e, d
 
 

@@ -107,6 +107,21 @@ int tbk_setparv_spin(int natoms, const int *kkind, int kinds,
                      const double *den, double *dparos, double *dhop,
                      double *dover);
 
+/* Non-collinear set-up (lnoncol): setpar(1,param,jsover,den,flipos,.true.,
+ * paros,spinos,hop,over,...) and setparv(1,...,.true.,dparos,dspnos,dhop,
+ * dover,...), setpar.f:1, setparv.f:1, for a two-channel param2
+ * [2*nparam].  den2 [2*natoms*kinds] (spin up, then down), paros/spinos and
+ * dparos/dspnos [4*natoms] (average and half difference of the channels'
+ * on-site terms), hop/over/dhop/dover [14*npairs] from channel 1. */
+int tbk_setpar_noncol(int natoms, const int *kkind, int kinds,
+                      const double *param2, int nparam, int nltype, int jsover,
+                      double thisvol, int npairs, const double *dist,
+                      const double *screen, const double *dscrn,
+                      const int *jkind, const int *jatm, double *den2,
+                      double *paros, double *spinos, double *dparos,
+                      double *dspnos, double *hop, double *over, double *dhop,
+                      double *dover);
+
 /* ---------------------------------------------------------------------
  * k-point meshes (host).  Bit-exact restatements of the reference's
  * kptin/rdspgp/kptgen, whose k-points and weights the plans take.
@@ -230,6 +245,20 @@ int tbk_plan_create_spin(int device, const tbk_structure_desc *st,
                          const double *param_dn, const int *flipos, int nk,
                          const double *kpts, const double *weights, int batch,
                          int precision, tbk_plan **out);
+
+/* A non-collinear structure: bande with jspins = 2 and lnoncol true, no
+ * torques or spin directions out (bande.f:713-720, 859-905, 945-1036).
+ * st->param is channel 1 (its bond and overlap integrals are used),
+ * param_dn [st->nparam] channel 2; the on-site terms are the channels'
+ * average and the spin terms half their difference (setpar.f:466-485,
+ * 744-795), turned along spinat [3*natoms], each atom's unit spin direction
+ * (spinham.f:110-150).  The items are the 2 nv spinor matrices of
+ * addspins.f:68-96 (spin up rows first), one electron per state; results as
+ * tbk_plan_run, tbk_plan_nv gives 2 nv. */
+int tbk_plan_create_noncol(int device, const tbk_structure_desc *st,
+                           const double *param_dn, const double *spinat,
+                           int nk, const double *kpts, const double *weights,
+                           int batch, int precision, tbk_plan **out);
 
 /* Magnetic moment of the last run of a spin-polarised plan, spin-up minus
  * spin-down electrons per cell (the smag of bande.f:1388-1402); 0 for other

@@ -95,6 +95,8 @@ def ref():
         L.ref_fermi.restype = C.c_double
         L.ref_bande.argtypes = [ip, ip, _d, _d, dp, dp, dp, dp, dp, dp, dp, ip]
         L.ref_setpar_spin.argtypes = [ip, dp, _d, ip, _i, _d, _d, _d, _d, _d, _d, _d, ip]
+        L.ref_setpar_nc.argtypes = [dp, _d, ip, _d, _d, _d, _d, _d, _d, _d, _d, _d, ip]
+        L.ref_bande_nc.argtypes = [ip, ip, _d, _d, dp, dp, dp, dp, _d, ip, _d, dp, dp, dp, ip]
         L.ref_bande_spin.argtypes = [ip, ip, _d, _d, dp, dp, dp, dp, _d, ip, _i, dp, dp, dp, dp, ip]
         L.ref_kptin.argtypes = [_d, C.c_char_p, ip, C.c_char_p, ip, ip, ip, _d, _d, _d, _d, _d,
                                 dp, ip, ip, _d, _d, _d, ip]
@@ -279,6 +281,40 @@ class RefSetup:
                               self.dparos, self.dhop, self.dover, self.den, C.byref(ierr))
         if ierr.value:
             raise RuntimeError(f"ref_setpar_spin ierr={ierr.value}")
+
+    def setpar_noncol(self, param_dn):
+        """The reference's setpar/setparv with lnoncol = .true.: paros,
+        spinos, dparos, dspnos, den [2][natoms][mkind] and channel 1's
+        integrals."""
+        st = self.st
+        n, nat = self.npairs, st.natoms
+        out = dict(paros=np.zeros(4 * nat), spinos=np.zeros(4 * nat), dparos=np.zeros(4 * nat),
+                   dspnos=np.zeros(4 * nat), hop=np.zeros(14 * max(n, 1)), over=np.zeros(14 * max(n, 1)),
+                   dhop=np.zeros(14 * max(n, 1)), dover=np.zeros(14 * max(n, 1)), den=np.zeros(2 * 20 * 2))
+        ierr = C.c_int(0)
+        pdn = np.ascontiguousarray(np.asarray(param_dn, float))
+        ref().ref_setpar_nc(C.byref(C.c_double(st.volume)), pdn, C.byref(C.c_int(len(pdn))), out["paros"],
+                            out["spinos"], out["dparos"], out["dspnos"], out["hop"], out["over"], out["dhop"],
+                            out["dover"], out["den"], C.byref(ierr))
+        if ierr.value:
+            raise RuntimeError(f"ref_setpar_nc ierr={ierr.value}")
+        return out
+
+    def bande_noncol(self, kpts, weights, param_dn, spinat, mode=4, tkb=0.005, eigcut=10.0):
+        """The reference's bande with lnoncol (jspins = 2): param(:,2) =
+        param_dn, spinat [natoms, 3] unit spin directions."""
+        st = self.st
+        ef, e1, e2 = C.c_double(0), C.c_double(0), C.c_double(0)
+        ib = C.c_int(0)
+        kp = np.ascontiguousarray(np.asarray(kpts, float).ravel())
+        pdn = np.ascontiguousarray(np.asarray(param_dn, float))
+        sp = np.ascontiguousarray(np.asarray(spinat, float).ravel())
+        ref().ref_bande_nc(C.byref(C.c_int(mode)), C.byref(C.c_int(len(kpts))), kp,
+                           np.ascontiguousarray(np.asarray(weights, float)),
+                           C.byref(C.c_double(st.volume)), C.byref(C.c_double(st.zfill)),
+                           C.byref(C.c_double(tkb)), C.byref(C.c_double(eigcut)), pdn,
+                           C.byref(C.c_int(len(pdn))), sp, C.byref(ef), C.byref(e1), C.byref(e2), C.byref(ib))
+        return dict(err=ib.value, efermi=ef.value, earg1=e1.value, earg2=e2.value)
 
     def bande_spin(self, kpts, weights, param_dn, flipos=None, mode=4, tkb=0.005, eigcut=10.0):
         """The reference's bande with jspins = 2 (param(:,1) = the

@@ -175,6 +175,82 @@ c
       return
       end
 c
+c     Non-collinear set-up (lnoncol): param(:,2) from pdn(npar); returns
+c     paros, spinos, dparos, dspnos, both spins' densities and channel 1's
+c     bond and overlap integrals.
+c
+      subroutine ref_setpar_nc(thisvol,pdn,npar,dpar,spo,dparo,dspo,
+     $     hpo,opo,dhpo,dopo,deno,ierr) bind(C,name='ref_setpar_nc')
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      integer npar,ierr
+      real*8 pdn(npar),dpar(4,*),spo(4,*),dparo(4,*),dspo(4,*)
+      real*8 hpo(14,*),opo(14,*),dhpo(14,*),dopo(14,*)
+      real*8 deno(mkind,matom,2)
+      common /codes/posn(matom,3),kkind(matom),natoms
+      common /pairs/tt_list(3,mpair),
+     $     dlv_list(3,mpair),dist_list(mpair),
+     $     screen_list(mpair),
+     $     jkind_list(mpair,2),
+     $     jatm_list(mpair,2),
+     $     npairs
+      logical jsover
+      common /lia/ npts,jsover
+      common /params/ param(npard,mspin)
+      common /refstate/ paros(4,matom),hop(mpkind,mpair),
+     $     over(mpkind,mpair),dparos(4,matom),dhop(mpkind,mpair),
+     $     dover(mpkind,mpair),den(mkind,matom,mspin)
+      real*8 spinos(4,matom),dspnos(4,matom)
+      logical flipos(matom)
+      do i = 1,npard
+         param(i,2) = 0d0
+      end do
+      do i = 1,npar
+         param(i,2) = pdn(i)
+      end do
+      do i = 1,matom
+         flipos(i) = .false.
+         do k = 1,4
+            spinos(k,i) = 0d0
+            dspnos(k,i) = 0d0
+         end do
+      end do
+      do ip = 1,npairs
+         do j = 1,mpkind
+            hop(j,ip) = 0d0
+            over(j,ip) = 0d0
+            dhop(j,ip) = 0d0
+            dover(j,ip) = 0d0
+         end do
+      end do
+      call setpar(1,param,jsover,den,flipos,.true.,
+     $     paros,spinos,hop,over,.false.,ierr)
+      if(ierr.ne.0) return
+      call setparv(1,param,thisvol,jsover,den,flipos,.true.,
+     $     dparos,dspnos,dhop,dover,.false.,ierr)
+      do ia = 1,natoms
+         do k = 1,4
+            dpar(k,ia) = paros(k,ia)
+            spo(k,ia) = spinos(k,ia)
+            dparo(k,ia) = dparos(k,ia)
+            dspo(k,ia) = dspnos(k,ia)
+         end do
+         do k = 1,mkind
+            deno(k,ia,1) = den(k,ia,1)
+            deno(k,ia,2) = den(k,ia,2)
+         end do
+      end do
+      do ip = 1,npairs
+         do j = 1,14
+            hpo(j,ip) = hop(j,ip)
+            opo(j,ip) = over(j,ip)
+            dhpo(j,ip) = dhop(j,ip)
+            dopo(j,ip) = dover(j,ip)
+         end do
+      end do
+      return
+      end
+c
       subroutine ref_setpar_js(jspin,flipos,thisvol,dpar,hpo,opo,dparo,
      $     dhpo,dopo,deno,ierr)
       implicit real*8 (a-h,o-z)
@@ -443,6 +519,65 @@ c
       end do
       label = 'ref_bande_spin'
       call bande(mode,2,flipos,.false.,spinat,.false.,dspth,dspphi,
+     $     .false.,thisvol,zfill,tkb,eigcut,wt,efermi,earg1,earg2,
+     $     smag,.false.,.false.,.false.,0,label,iberr)
+      return
+      end
+c
+c     The same with lnoncol (non-collinear, jspins = 2): param(:,2) from
+c     pdn(npar), spinat(3,natoms) the atoms' unit spin directions.
+c
+      subroutine ref_bande_nc(mode,nk,kp,w,thisvol,zfill,tkb,eigcut,
+     $     pdn,npar,spn,efermi,earg1,earg2,iberr)
+     $     bind(C,name='ref_bande_nc')
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      include 'P3'
+      integer mode,nk,npar,iberr
+      real*8 kp(3,*),w(*),thisvol,zfill,tkb,eigcut,pdn(npar),spn(3,*)
+      real*8 efermi,earg1,earg2
+      logical master,multiprc
+      common /MPIcomm/ MPISIZE,MPIRANK,master,multiprc
+      logical jsover
+      common /lia/ npts,jsover
+      common /cd1/ x(nkd),y(nkd),z(nkd)
+      common /refeig/ esk(mh,nkdproc),wt(nkd)
+      common /codes/posn(matom,3),kkind(matom),natoms
+      common /params/ param(npard,mspin)
+      logical flipos(matom)
+      real*8 spinat(3,matom),dspth(3,matom),dspphi(3,matom)
+      character*20 label
+      iberr = 0
+      if(nk.gt.nkd.or.mspin.lt.2.or.mcol.lt.2) then
+         iberr = 10
+         return
+      end if
+      do i = 1,npard
+         param(i,2) = 0d0
+      end do
+      do i = 1,npar
+         param(i,2) = pdn(i)
+      end do
+      do i = 1,matom
+         flipos(i) = .false.
+         do j = 1,3
+            spinat(j,i) = 0d0
+         end do
+      end do
+      do i = 1,natoms
+         do j = 1,3
+            spinat(j,i) = spn(j,i)
+         end do
+      end do
+      npts = nk
+      do k = 1,nk
+         x(k) = kp(1,k)
+         y(k) = kp(2,k)
+         z(k) = kp(3,k)
+         wt(k) = w(k)
+      end do
+      label = 'ref_bande_nc'
+      call bande(mode,2,flipos,.true.,spinat,.false.,dspth,dspphi,
      $     .false.,thisvol,zfill,tkb,eigcut,wt,efermi,earg1,earg2,
      $     smag,.false.,.false.,.false.,0,label,iberr)
       return

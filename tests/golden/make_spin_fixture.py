@@ -1,6 +1,7 @@
 """Writes tests/golden/spin.json: the reference's own bande with jspins = 2
-(collinear spin polarisation) on small structures, for the device path's
-parity tests where the reference is not present.
+(collinear spin polarisation, and non-collinear with lnoncol) on small
+structures, for the device path's parity tests where the reference is not
+present.
 
 Data only: inputs (structure, the spin-down parameter channel, flipped
 atoms, k-points) and the reference's E_F, earg1, earg2 and smag per mode,
@@ -74,6 +75,24 @@ def ex1_magnetic():
     return dict(param_dn=pdn.tolist(), skeng=[list(r) for r in rows])
 
 
+def noncol_cases():
+    """Non-collinear structures (lnoncol): spin directions per atom."""
+    out = []
+    for c in cases():
+        st = c.pop("st")
+        nat = len(c["posn"])
+        if nat == 2:
+            t = 0.7
+            spn = [[0.0, 0.0, 1.0], [np.sin(t), 0.0, np.cos(t)]]
+        else:
+            r3 = 1.0 / np.sqrt(3.0)
+            spn = [[0.0, 0.0, 1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [r3, r3, r3]][:nat]
+        c["spinat"] = spn
+        c.pop("flipos")
+        out.append((c, st))
+    return out[0::2]
+
+
 def main():
     res = []
     for c in cases():
@@ -87,11 +106,22 @@ def main():
             c["modes"][str(mode)] = r
         res.append(c)
         print(c["name"], c["modes"]["4"])
+    nc = []
+    for c, st in noncol_cases():
+        R = oracle.RefSetup(st)
+        c["modes"] = {}
+        for mode in (1, 2, 3, 4):
+            r = R.bande_noncol(np.array(c["kpts"]), np.array(c["weights"]), np.array(c["param_dn"]),
+                               np.array(c["spinat"]), mode=mode, tkb=0.005, eigcut=10.0)
+            assert r.pop("err") == 0
+            c["modes"][str(mode)] = r
+        nc.append(c)
+        print("noncol", c["name"], c["modes"]["4"])
     ex1 = ex1_magnetic()
     print("ex1 NM:", ex1["skeng"][0])
     out = dict(source="reference bande (jspins = 2) through oracle/_ref/libnrlref.so; Pd spin-up channel; "
                       "ex1 with a two-channel pd_par through oracle/_ref/static_ref",
-               tkb=0.005, eigcut=10.0, nd=[4, 4, 4], cases=res, ex1_magnetic=ex1)
+               tkb=0.005, eigcut=10.0, nd=[4, 4, 4], cases=res, ex1_magnetic=ex1, noncol=nc)
     path = os.path.join(HERE, "spin.json")
     json.dump(out, open(path, "w"), indent=1)
     print(f"wrote {path}")

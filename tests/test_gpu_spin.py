@@ -20,7 +20,7 @@ import spin_case
 from conftest import ROOT
 from tbk._lib import TBK_MIXED
 from tbk import structures as S
-from tbk.api import Plan, SpinPlan
+from tbk.api import NonCollinearPlan, Plan, SpinPlan
 
 pytestmark = pytest.mark.gpu
 
@@ -166,3 +166,56 @@ def test_reference_program_with_device_bande_magnetic(tmp_path):
     print(f"  worst |SKENG - reference program| = {worst:.1e} (9 decimals), moment {worst_m:.1e} (6)",
           flush=True)
     assert worst <= 1.5e-9 and worst_m <= 1.5e-6
+
+
+@pytest.mark.parametrize("idx", [0, 1])
+def test_noncollinear_plan_vs_reference(idx):
+    """Spinor items of order 2 nv (addspins.f) against the reference's
+    lnoncol bande, modes 1-4, FP64; mode 4 mixed within 1e-6."""
+    g = load()
+    c = g["noncol"][idx]
+    st = structure(c)
+    print(f"case: non-collinear plan {c['name']} ({len(c['posn'])} atoms), modes 1-4", flush=True)
+    plan = NonCollinearPlan(st, c["param_dn"], c["spinat"], c["kpts"], c["weights"])
+    assert plan.nv == 2 * st.nv
+    worst = 0.0
+    for mode, want in c["modes"].items():
+        r = plan.run(int(mode), g["tkb"], g["eigcut"])
+        for key in ("efermi", "earg1", "earg2"):
+            d = abs(r[key] - want[key])
+            worst = max(worst, d)
+            assert d <= 1e-9, (mode, key, r[key], want[key])
+    print(f"  worst |device - reference| = {worst:.1e}", flush=True)
+    plan.close()
+    m = NonCollinearPlan(st, c["param_dn"], c["spinat"], c["kpts"], c["weights"], precision=TBK_MIXED)
+    r = m.run(4, g["tkb"], g["eigcut"])
+    for key in ("efermi", "earg1", "earg2"):
+        assert abs(r[key] - c["modes"]["4"][key]) <= 1e-6, key
+    m.close()
+
+
+def test_noncollinear_rotation_and_collinear_limit():
+    """A common rotation of every spin leaves the spectrum; spins along z
+    with channels differing only in decay and on-site values reproduce the
+    collinear spin-polarised plan (already pinned against the reference)."""
+    c = load()["cases"][0]
+    st = structure(c)
+    pdn = spin_case.spin_down_values(st.params.param, seed=9, n_changed=17)
+    print("case: non-collinear z / rotated spins vs collinear plan (bcc2)", flush=True)
+    z = NonCollinearPlan(st, pdn, [[0, 0, 1.0]] * 2, c["kpts"], c["weights"])
+    t, f = 0.9, 0.4
+    d = [np.sin(t) * np.cos(f), np.sin(t) * np.sin(f), np.cos(t)]
+    rot = NonCollinearPlan(st, pdn, [d] * 2, c["kpts"], c["weights"])
+    col = SpinPlan(st, pdn, c["kpts"], c["weights"])
+    rz, rr, rc = z.run(4), rot.run(4), col.run(4)
+    for key in ("efermi", "earg1", "earg2"):
+        assert abs(rz[key] - rr[key]) <= 1e-11, key
+        assert abs(rz[key] - rc[key]) <= 1e-10, key
+    ez, _ = z.eigenvalues()
+    er, _ = rot.eigenvalues()
+    assert np.abs(ez - er).max() <= 1e-11
+    eu, _ = col.eigenvalues(0)
+    ed, _ = col.eigenvalues(1)
+    both = np.sort(np.concatenate([eu, ed], axis=1), axis=1)
+    assert np.abs(ez - both).max() <= 1e-11
+    z.close(); rot.close(); col.close()

@@ -146,3 +146,66 @@ def test_reference_identical_channels_equal_unpolarised():
         assert r2["smag"] == 0.0
         for key in ("efermi", "earg1", "earg2"):
             assert abs(r1[key] - r2[key]) < 1e-9, (mode, key)
+
+
+def _host_noncol(st, pdn):
+    L = _lib.lib()
+    H = HostSetup(st)
+    n, nat, p = H.npairs, st.natoms, st.params
+    o = {k: np.zeros(4 * nat) for k in ("paros", "spinos", "dparos", "dspnos")}
+    for k in ("hop", "over", "dhop", "dover"):
+        o[k] = np.zeros(14 * max(n, 1))
+    o["den2"] = np.zeros(2 * nat * p.kinds)
+    p2 = np.ascontiguousarray(np.concatenate([p.param, pdn]))
+    assert L.tbk_setpar_noncol(nat, st.kkind, p.kinds, p2, len(p.param), p.nltype, int(p.jsover), st.volume,
+                               n, H.dist, H.screen, H.dscrn, H.jkind, H.jatm, o["den2"], o["paros"],
+                               o["spinos"], o["dparos"], o["dspnos"], o["hop"], o["over"], o["dhop"],
+                               o["dover"]) == 0
+    return o, n
+
+
+@needs_ref
+@pytest.mark.parametrize("which", ["bcc2", "fcc4", "two_kinds"])
+def test_noncollinear_setup_bit_exact(which):
+    """setpar/setparv with lnoncol (both spins' densities, average and
+    half-difference on-site terms, and for other kinds the reference's own
+    replace / full-difference / spin-up-derivative forms) bit-identical."""
+    cases = {c["name"].split("_")[0]: c for c in json.load(open(GOLD))["cases"]}
+    if which == "two_kinds":
+        st, _ = two_kind_structure()
+        rng = np.random.default_rng(2)
+        pdn = st.params.param * (1 + 0.02 * rng.standard_normal(len(st.params.param)))
+    else:
+        st = structure(cases[which])
+        pdn = np.array(cases[which]["param_dn"])
+    r = oracle.RefSetup(st).setpar_noncol(pdn)
+    h, n = _host_noncol(st, pdn)
+    for k in ("paros", "spinos", "dparos", "dspnos"):
+        assert np.array_equal(h[k], r[k]), k
+    for k in ("hop", "over", "dhop", "dover"):
+        assert np.array_equal(h[k][:14 * n], r[k][:14 * n]), k
+    assert np.abs(h["spinos"]).max() > 0.0
+
+
+@needs_ref
+def test_reference_noncollinear_invariants():
+    """The reference's lnoncol bande: a common rotation of all spins changes
+    nothing, and spins along z with channels that differ only in decay and
+    on-site values give its collinear jspins = 2 result."""
+    import spin_case
+    g = json.load(open(GOLD))
+    c = g["cases"][0]
+    st = structure(c)
+    pdn = spin_case.spin_down_values(st.params.param, seed=9, n_changed=17)
+    R = oracle.RefSetup(st)
+    k, w = np.array(c["kpts"]), np.array(c["weights"])
+    rz = R.bande_noncol(k, w, pdn, [[0, 0, 1.0]] * 2)
+    rx = R.bande_noncol(k, w, pdn, [[1.0, 0, 0]] * 2)
+    rc = R.bande_spin(k, w, pdn)
+    for key in ("efermi", "earg1", "earg2"):
+        assert abs(rz[key] - rx[key]) < 1e-12 and abs(rz[key] - rc[key]) < 1e-9, key
+    for nc in g["noncol"]:
+        R = oracle.RefSetup(structure(nc))
+        r = R.bande_noncol(np.array(nc["kpts"]), np.array(nc["weights"]), np.array(nc["param_dn"]),
+                           np.array(nc["spinat"]), mode=4)
+        assert r.pop("err") == 0 and r == nc["modes"]["4"], nc["name"]

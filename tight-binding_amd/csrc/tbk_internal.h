@@ -60,6 +60,14 @@ int setparv(int natoms, const int *kkind, int kinds, const double *param, int nl
             int jsover, double thisvol, const PairList &pl, const double *den,
             double *dparos, double *dhop, double *dover, const double *param_flip = nullptr,
             const int *flip = nullptr);
+// Non-collinear (lnoncol) on-site terms: den2 [2][natoms*kinds], paros and
+// spinos [4*natoms]; their volume derivatives; the per-orbital spinor terms.
+void setpar_noncol(int natoms, const int *kkind, int kinds, const double *param_up, const double *param_dn,
+                   const PairList &pl, double *den2, double *paros, double *spinos);
+void setparv_noncol(int natoms, const int *kkind, int kinds, const double *param_up, const double *param_dn,
+                    double thisvol, const PairList &pl, const double *den2, double *dparos, double *dspnos);
+void spinor_terms(int natoms, const int *nbas, const double *spinos, const double *spinat, double *d,
+                  double *mre, double *mim);
 void sk_block(double l, double m, double n, const double *p, bool unlike, double *e);
 void arrange_integrals(const double *stored, int k1, int k2, double *out);
 

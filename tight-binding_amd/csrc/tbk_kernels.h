@@ -20,6 +20,10 @@ struct StructDev {
     const double *eh, *es;           // [81*npairs] SK blocks e(a,b) of H, S
     const double *edh, *eds;         // [81*npairs] volume derivatives
     const double *paros, *dparos;    // [4*natoms] on-site terms and derivatives
+    // non-collinear structures: [6*nv] per orbital the spin terms d, Re m,
+    // Im m and their volume derivatives; the items have order 2 nv.
+    // nullptr otherwise.
+    const double *spin = nullptr;
 };
 
 // Build of H(k), S(k) (or dH, dS) for a batch of items (structure, k-point)
@@ -135,6 +139,9 @@ void gemm_record(bool on);                  // collect the shapes of launches
 void gemm_note(const GemmShape &s);
 std::vector<GemmShape> gemm_recorded();
 
+// Non-collinear items: the spatial H, S (or dH, dS) just built in the
+// leading nv block become the 2 nv spinor matrices of addspins.f:68-96.
+void launch_spinor_expand(const BuildArgs &a, int maxnv, int nb, hipStream_t st);
 void launch_phases(const StructDev *sd, const int *item_s, const double *kpts, int maxpairs,
                    double2 *ph, int nb, hipStream_t st);
 void launch_build_hs(const BuildArgs &a, int maxnat, int nb, hipStream_t st);

@@ -2189,6 +2189,38 @@ __global__ __launch_bounds__(1024) void k_fermi_window_group(const FermiArgs *fa
     }
 }
 
+// addspins.f:68-96 on dense matrices: with H, S the spatial matrices in the
+// leading nv x nv block of an item of order N = 2 nv (the rest decoupled
+// padding from k_build_hs), the spin-down block becomes H - diag(d), S; the
+// spin-up diagonal H + diag(d); the mixing entries H(i, nv+i) = m_i and its
+// conjugate; the other off-diagonal entries stay zero.  One thread per
+// element of the spatial block; the derivative build uses the derivative
+// terms.
+__global__ __launch_bounds__(256) void k_spinor_expand(BuildArgs a, int nb)
+{
+    const int kb = blockIdx.y;
+    if (kb >= nb) return;
+    const StructDev &sd = a.sd[a.item_s[kb]];
+    if (!sd.spin) return;
+    const int nv = sd.nv, N = a.n;
+    if (2 * nv > N) return;
+    const double *sp = sd.spin + (a.deriv ? 3 * nv : 0);
+    double2 *H = a.h + (size_t)kb * a.stride, *S = a.s + (size_t)kb * a.stride;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nv * nv; t += gridDim.x * blockDim.x) {
+        const int r = t % nv, c = t / nv;
+        const double2 h = H[(size_t)c * N + r];
+        double2 hd = h;
+        if (r == c) hd.x = h.x - sp[r];
+        H[(size_t)(nv + c) * N + nv + r] = hd;
+        S[(size_t)(nv + c) * N + nv + r] = S[(size_t)c * N + r];
+        if (r == c) {
+            H[(size_t)c * N + r] = make_double2(h.x + sp[r], h.y);
+            H[(size_t)(nv + r) * N + r] = make_double2(sp[nv + r], sp[2 * nv + r]);
+            H[(size_t)r * N + nv + r] = make_double2(sp[nv + r], -sp[2 * nv + r]);
+        }
+    }
+}
+
 // Plain streaming copy (16-byte elements), the calibration kernel for
 // counter-based traffic measurements.
 __global__ __launch_bounds__(256) void k_copy16(const double2 *src, double2 *dst, size_t n)
@@ -2211,6 +2243,11 @@ void launch_phases(const StructDev *sd, const int *item_s, const double *kpts, i
 {
     const int gx = std::max(1, std::min((maxpairs + 255) / 256, 64));
     hipLaunchKernelGGL(k_phases, dim3(gx, nb), dim3(256), 0, st, sd, item_s, kpts, maxpairs, ph);
+}
+void launch_spinor_expand(const BuildArgs &a, int maxnv, int nb, hipStream_t st)
+{
+    const int gx = std::max(1, std::min((maxnv * maxnv + 255) / 256, 64));
+    hipLaunchKernelGGL(k_spinor_expand, dim3(gx, nb), dim3(256), 0, st, a, nb);
 }
 void launch_build_hs(const BuildArgs &a, int maxnat, int nb, hipStream_t st)
 {

@@ -480,7 +480,8 @@ void solve_batch(Workspace &ws, int nb, const int *nact, double *w_out, hipStrea
 
 // Host view of one structure of a plan.
 struct StructHost {
-    int natoms = 0, nv = 0, npairs = 0, nk = 0, k0 = 0;   // k0: first item
+    // nv: states per k (2 norb for a non-collinear structure), norb: orbitals
+    int natoms = 0, nv = 0, npairs = 0, nk = 0, k0 = 0, norb = 0;   // k0: first item
     double zfill = 0.0, thisvol = 0.0;
 };
 
@@ -501,6 +502,11 @@ struct tbk_plan {
     // and structure 1 the spin-down channel, one Fermi level over both
     bool spin = false;
     int nfermi = 0;                        // Fermi groups: nstruct, or 1 with spin
+    // noncol: non-collinear structures (2 norb spinor items, one electron
+    // per state); spin_terms their per-orbital terms on the device
+    bool noncol = false;
+    int maxnorb = 0;
+    double *spin_terms = nullptr;
     int *item_s = nullptr, *nact = nullptr; // [nk]
     // results, per item with stride N
     double *esk = nullptr, *desk = nullptr, *out = nullptr;   // out [nstruct][8]
@@ -533,7 +539,7 @@ static void plan_free(tbk_plan *p)
     dfree(p->row_atom); dfree(p->row_orb); dfree(p->ksk); dfree(p->nbas); dfree(p->blk_ptr);
     dfree(p->dvec); dfree(p->eh); dfree(p->es); dfree(p->edh); dfree(p->eds);
     dfree(p->paros); dfree(p->dparos); dfree(p->kp); dfree(p->wts);
-    dfree(p->sdev); dfree(p->fargs); dfree(p->item_s); dfree(p->nact);
+    dfree(p->sdev); dfree(p->fargs); dfree(p->item_s); dfree(p->nact); dfree(p->spin_terms);
     dfree(p->esk); dfree(p->desk); dfree(p->out); dfree(p->ncut); dfree(p->err); dfree(p->info_all);
     dfree(p->cut);
     dfree(p->ph);
@@ -562,13 +568,18 @@ static int upload(T **dst, const std::vector<T> &src)
 // SK blocks), appended to the concatenated arrays.
 struct SetupArrays {
     std::vector<int> row_atom, row_orb, ksk, nbas, blk_ptr;
-    std::vector<double> dvec, eh, es, edh, eds, paros, dparos;
+    std::vector<double> dvec, eh, es, edh, eds, paros, dparos, spin;
 };
-struct SetupOffsets { size_t rows, atoms, blk, pairs; };
+struct SetupOffsets { size_t rows, atoms, blk, pairs, spin; bool noncol; };
 
 // Spin channel selection of a structure's set-up: atoms with flip[a] != 0
 // take their densities and on-site terms from param_flip.
-struct SpinSel { const double *param_flip = nullptr; const int *flip = nullptr; };
+struct SpinSel {
+    const double *param_flip = nullptr; const int *flip = nullptr;
+    // non-collinear (lnoncol): the spin-down channel and the atoms' spin
+    // directions [3*natoms]
+    const double *nc_param_dn = nullptr, *nc_spinat = nullptr;
+};
 
 static int setup_structure(const tbk_structure_desc *st, SetupArrays &A, StructHost &h, SetupOffsets &off,
                            const SpinSel &sel = SpinSel())
@@ -596,7 +607,24 @@ static int setup_structure(const tbk_structure_desc *st, SetupArrays &A, StructH
         if (nbas[a] < 1 || nbas[a] > 9) return TBK_ERR_ARG;
         nv += nbas[a];
     }
-    if (nv < 2 || nv > kMaxN) return TBK_ERR_ARG;
+    const bool noncol = sel.nc_param_dn != nullptr;
+    if (nv < 2 || (noncol ? 2 * nv : nv) > kMaxN) return TBK_ERR_ARG;
+    if (noncol) {
+        // bande.f:859-905 with lnoncol: integrals from channel 1, on-site
+        // average and spin terms from both (setpar.f:466-485, 744-795),
+        // then the per-orbital spinor terms (spinham.f:110-150)
+        std::vector<double> den2(2 * (size_t)nat * kinds), spinos(4 * nat), dspnos(4 * nat);
+        tbk::setpar_noncol(nat, st->kkind, kinds, st->param, sel.nc_param_dn, pl, den2.data(), paros.data(),
+                           spinos.data());
+        tbk::setparv_noncol(nat, st->kkind, kinds, st->param, sel.nc_param_dn, st->thisvol, pl, den2.data(),
+                            dparos.data(), dspnos.data());
+        std::vector<double> t(6 * (size_t)nv);
+        tbk::spinor_terms(nat, nbas.data(), spinos.data(), sel.nc_spinat, &t[0], &t[nv], &t[2 * nv]);
+        tbk::spinor_terms(nat, nbas.data(), dspnos.data(), sel.nc_spinat, &t[3 * nv], &t[4 * nv], &t[5 * nv]);
+        off.spin = A.spin.size();
+        A.spin.insert(A.spin.end(), t.begin(), t.end());
+    }
+    off.noncol = noncol;
     off.rows = A.row_atom.size(); off.atoms = A.ksk.size(); off.blk = A.blk_ptr.size();
     off.pairs = A.dvec.size() / 3;
     for (int a = 0; a < nat; ++a)
@@ -635,7 +663,7 @@ static int setup_structure(const tbk_structure_desc *st, SetupArrays &A, StructH
     for (int p = 0; p < np; ++p) blk[(size_t)pl.pairs[p].atom[0] * nat + pl.pairs[p].atom[1] + 1]++;
     for (size_t b = 1; b < blk.size(); ++b) blk[b] += blk[b - 1];
     A.blk_ptr.insert(A.blk_ptr.end(), blk.begin(), blk.end());
-    h.natoms = nat; h.nv = nv; h.npairs = np;
+    h.natoms = nat; h.norb = nv; h.nv = noncol ? 2 * nv : nv; h.npairs = np;
     h.zfill = st->zfill; h.thisvol = st->thisvol;
     return TBK_OK;
 }
@@ -645,7 +673,9 @@ static void append_setup(SetupArrays &A, const SetupArrays &L, SetupOffsets &off
 {
     off.rows = A.row_atom.size(); off.atoms = A.ksk.size(); off.blk = A.blk_ptr.size();
     off.pairs = A.dvec.size() / 3;
+    off.spin = A.spin.size();
     auto cat = [](auto &dst, const auto &src) { dst.insert(dst.end(), src.begin(), src.end()); };
+    cat(A.spin, L.spin);
     cat(A.row_atom, L.row_atom); cat(A.row_orb, L.row_orb); cat(A.ksk, L.ksk); cat(A.nbas, L.nbas);
     cat(A.blk_ptr, L.blk_ptr); cat(A.dvec, L.dvec); cat(A.eh, L.eh); cat(A.es, L.es);
     cat(A.edh, L.edh); cat(A.eds, L.eds); cat(A.paros, L.paros); cat(A.dparos, L.dparos);
@@ -678,6 +708,7 @@ static int plan_create(int device, int nstruct, const tbk_structure_desc *st, co
         for (int i = 0; i < nstruct; ++i) {
             if (rcs[i]) return rcs[i];
             append_setup(A, loc[i], off[i]);
+            off[i].noncol = unused[i].noncol;
         }
     }
     int nktot = 0, N = 0, maxnat = 0, maxpairs = 1;
@@ -688,6 +719,12 @@ static int plan_create(int device, int nstruct, const tbk_structure_desc *st, co
         maxnat = std::max(maxnat, sh[i].natoms);
         maxpairs = std::max(maxpairs, sh[i].npairs);
     }
+    bool any_noncol = false;
+    int maxnorb = 0;
+    for (int i = 0; i < nstruct; ++i) {
+        any_noncol = any_noncol || off[i].noncol;
+        maxnorb = std::max(maxnorb, sh[i].norb);
+    }
     if (precision == TBK_MIXED && N > tbk::oa_max_n()) return TBK_ERR_ARG;
     // ---- device state ----
     tbk_plan *p = new tbk_plan();
@@ -695,6 +732,7 @@ static int plan_create(int device, int nstruct, const tbk_structure_desc *st, co
     p->device = device;
     p->nstruct = nstruct; p->N = N; p->nk = nktot; p->precision = precision;
     p->nfermi = nstruct;
+    p->noncol = any_noncol; p->maxnorb = maxnorb;
     p->maxnat = maxnat; p->maxpairs = maxpairs;
     p->sh = sh;
     p->kpts.assign(kpts, kpts + 3 * (size_t)nktot);
@@ -709,6 +747,7 @@ static int plan_create(int device, int nstruct, const tbk_structure_desc *st, co
     rc |= upload(&p->dvec, A.dvec); rc |= upload(&p->eh, A.eh); rc |= upload(&p->es, A.es);
     rc |= upload(&p->edh, A.edh); rc |= upload(&p->eds, A.eds);
     rc |= upload(&p->paros, A.paros); rc |= upload(&p->dparos, A.dparos);
+    rc |= upload(&p->spin_terms, A.spin);
     rc |= upload(&p->kp, p->kpts); rc |= upload(&p->wts, p->weights);
     std::vector<int> item_s(nktot), nact(nktot);
     for (int i = 0; i < nstruct; ++i)
@@ -723,7 +762,8 @@ static int plan_create(int device, int nstruct, const tbk_structure_desc *st, co
         std::vector<tbk::StructDev> sd(nstruct);
         for (int i = 0; i < nstruct; ++i) {
             tbk::StructDev &d = sd[i];
-            d.natoms = sh[i].natoms; d.nv = sh[i].nv; d.npairs = sh[i].npairs;
+            d.natoms = sh[i].natoms; d.nv = sh[i].norb; d.npairs = sh[i].npairs;
+            d.spin = off[i].noncol ? p->spin_terms + off[i].spin : nullptr;
             d.row_atom = p->row_atom + off[i].rows; d.row_orb = p->row_orb + off[i].rows;
             d.ksk = p->ksk + off[i].atoms; d.nbas = p->nbas + off[i].atoms;
             d.blk_ptr = p->blk_ptr + off[i].blk;
@@ -790,6 +830,21 @@ extern "C" int tbk_plan_create_spin(int device, const tbk_structure_desc *st, co
     return TBK_OK;
 }
 
+// bande.f:713-720, 859-905, 945-1036 with lnoncol (jspins = 2): the
+// spatial H, S from channel 1's integrals and the channels' average on-site
+// terms, doubled into the spinor matrices of order 2 nv with each atom's
+// spin direction (spinham.f, addspins.f); one electron per state.
+extern "C" int tbk_plan_create_noncol(int device, const tbk_structure_desc *st, const double *param_dn,
+                                      const double *spinat, int nk, const double *kpts, const double *weights,
+                                      int batch, int precision, tbk_plan **out)
+{
+    if (!st || !param_dn || !spinat || !st->param || nk <= 0 || !kpts || !weights || !out) return TBK_ERR_ARG;
+    SpinSel sel;
+    sel.nc_param_dn = param_dn;
+    sel.nc_spinat = spinat;
+    return plan_create(device, 1, st, &nk, kpts, weights, batch, precision, &sel, out);
+}
+
 extern "C" int tbk_plan_magnetization(const tbk_plan *p, double *smag)
 {
     if (!p || !smag) return TBK_ERR_ARG;
@@ -844,6 +899,7 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
         p->timer.begin("build_hs");
         tbk::launch_phases(p->sdev, p->item_s + k0, p->kp + 3 * (size_t)k0, p->maxpairs, p->ph, nb, st);
         tbk::launch_build_hs(build_args(p, k0, false), p->maxnat, nb, st);
+        if (p->noncol) tbk::launch_spinor_expand(build_args(p, k0, false), p->maxnorb, nb, st);
         p->timer.end();
         const int *nact = p->nact + k0;
         if (p->precision == TBK_MIXED)
@@ -854,6 +910,7 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
         if (lpress) {
             p->timer.begin("pressure");
             tbk::launch_build_hs(build_args(p, k0, true), p->maxnat, nb, st);
+            if (p->noncol) tbk::launch_spinor_expand(build_args(p, k0, true), p->maxnorb, nb, st);
             tbk::launch_zgemm(with_ncol(gemm(n, n, n, p->ws.A, n, ss, p->ws.Z, n, ss, p->ws.X, n, ss), wnd.cut), 0, 0,
                               nb, st);
             tbk::launch_zgemm(with_ncol(gemm(n, n, n, p->ws.S, n, ss, p->ws.Z, n, ss, p->ws.Y, n, ss), wnd.cut), 0, 0,
@@ -894,6 +951,7 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
             f.esk = p->esk + (size_t)h.k0 * p->N; f.desk = p->desk + (size_t)h.k0 * p->N;
             f.w = p->wts + h.k0; f.ncut = p->ncut + h.k0; f.out = p->out + 8 * (size_t)i;
             f.cut = p->cut + h.k0;
+            if (p->noncol) f.el = 1.0;          // one electron per spinor state (bande.f:1069)
             if (p->spin) {          // both channels' items follow each other
                 f.nk = p->sh[0].nk + p->sh[1].nk;
                 f.el = 1.0;
@@ -1125,6 +1183,7 @@ extern "C" int tbk_plan_hs(tbk_plan *p, int k, int deriv, double *h, double *s)
     HIP_TRY(hipSetDevice(p->device));
     tbk::launch_phases(p->sdev, p->item_s + k, p->kp + 3 * (size_t)k, p->maxpairs, p->ph, 1, p->stream);
     tbk::launch_build_hs(build_args(p, k, deriv != 0), p->maxnat, 1, p->stream);
+    if (p->noncol) tbk::launch_spinor_expand(build_args(p, k, deriv != 0), p->maxnorb, 1, p->stream);
     HIP_TRY(hipStreamSynchronize(p->stream));
     const int nv = p->sh[p->item_s_host[k]].nv;
     if (h && copy_leading((double2 *)h, p->ws.A, p->N, nv, p->stream)) return TBK_ERR_HIP;

@@ -349,6 +349,144 @@ int setparv(int natoms, const int *kkind, int kinds, const double *param, int nl
     return TBK_OK;
 }
 
+// setpar.f:466-485, 744-795 (lnoncol): densities of both spins from each
+// channel's decay, on-site terms as the average (paros) and half difference
+// (spinos) of the two channels.  The terms from other kinds replace, not add
+// to, the own-kind values and take four coefficients per orbital type with
+// the full difference, as setpar.f:772-794 does.  den2[s*natoms*kinds + ...].
+void setpar_noncol(int natoms, const int *kkind, int kinds, const double *param_up, const double *param_dn,
+                   const PairList &pl, double *den2, double *paros, double *spinos)
+{
+    const ParamView U{param_up}, D{param_dn};
+    const double twtrd = 2.0 / 3.0, half = 0.5;
+    const size_t na = (size_t)natoms * kinds;
+    for (size_t i = 0; i < 2 * na; ++i) den2[i] = 0.0;
+    double *du = den2, *dd = den2 + na;
+    for (int ip = 0; ip < pl.n; ++ip) {
+        const Pair &pr = pl.pairs[ip];
+        const int ia = pr.atom[0], ja = pr.atom[1];
+        const int ki = pr.kind[0], kj = pr.kind[1];
+        const double r = pr.dist, f = pr.screen;
+        const int i0 = onsite_base(kj) + 1;
+        const double bup = U(i0) * U(i0), bdn = D(i0) * D(i0);
+        du[ia * kinds + kj - 1] = du[ia * kinds + kj - 1] + std::exp(-bup * r) * f;
+        dd[ia * kinds + kj - 1] = dd[ia * kinds + kj - 1] + std::exp(-bdn * r) * f;
+        if (ja != ia) {
+            const int i1 = onsite_base(ki) + 1;
+            const double bu1 = U(i1) * U(i1), bd1 = D(i1) * D(i1);
+            du[ja * kinds + ki - 1] = du[ja * kinds + ki - 1] + std::exp(-bu1 * r) * f;
+            dd[ja * kinds + ki - 1] = dd[ja * kinds + ki - 1] + std::exp(-bd1 * r) * f;
+        }
+    }
+    for (int ia = 0; ia < natoms; ++ia) {
+        const int k1 = kkind[ia];
+        int b = onsite_base(k1) + 1;
+        double vu = std::pow(du[ia * kinds + k1 - 1], twtrd), vd = std::pow(dd[ia * kinds + k1 - 1], twtrd);
+        for (int c = 0; c < 4; ++c, b += 4) {
+            const double pup = U(b + 1) + vu * (U(b + 2) + vu * (U(b + 3) + vu * U(b + 4)));
+            const double pdn = D(b + 1) + vd * (D(b + 2) + vd * (D(b + 3) + vd * D(b + 4)));
+            paros[4 * ia + c] = half * (pup + pdn);
+            spinos[4 * ia + c] = half * (pup - pdn);
+        }
+        for (int k2 = 1; k2 <= kinds; ++k2) {
+            if (k2 == k1 || !(std::max(du[ia * kinds + k2 - 1], dd[ia * kinds + k2 - 1]) > 0.0)) continue;
+            vu = std::pow(du[ia * kinds + k2 - 1], twtrd);
+            vd = std::pow(dd[ia * kinds + k2 - 1], twtrd);
+            const int hi = k1 > k2 ? k1 : k2, lo = k1 < k2 ? k1 : k2;
+            b = kinds * (1 + kOnsite * 4) + 6 * kOnsite * ((hi - 1) * (hi - 2) / 2 + lo - 1);
+            if (k2 < hi) b += 3 * kOnsite;
+            for (int c = 0; c < 4; ++c, b += 4) {
+                const double pup = U(b + 1) + vu * (U(b + 2) + vu * (U(b + 3) + vu * U(b + 4)));
+                const double pdn = D(b + 1) + vd * (D(b + 2) + vd * (D(b + 3) + vd * D(b + 4)));
+                paros[4 * ia + c] = half * (pup + pdn);
+                spinos[4 * ia + c] = pup - pdn;
+            }
+        }
+    }
+}
+
+// setparv.f:436-490, 725-798 (lnoncol): volume derivatives of both spins'
+// densities and of the average / half-difference on-site terms.  For other
+// kinds the reference takes the spin-down density derivative from the
+// spin-up one and steps three coefficients per orbital type
+// (setparv.f:760-797); both are kept.
+void setparv_noncol(int natoms, const int *kkind, int kinds, const double *param_up, const double *param_dn,
+                    double thisvol, const PairList &pl, const double *den2, double *dparos, double *dspnos)
+{
+    const ParamView U{param_up}, D{param_dn};
+    const double third = 1.0 / 3.0, twtrd = 2.0 / 3.0, threeh = 3.0 / 2.0, half = 0.5;
+    const size_t na = (size_t)natoms * kinds;
+    std::vector<double> vden(2 * na, 0.0);
+    double *vu = vden.data(), *vd = vden.data() + na;
+    const double *du = den2, *dd = den2 + na;
+    for (int ip = 0; ip < pl.n; ++ip) {
+        const Pair &pr = pl.pairs[ip];
+        const int ia = pr.atom[0], ja = pr.atom[1];
+        const int ki = pr.kind[0], kj = pr.kind[1];
+        const double r = pr.dist, f = pr.screen, df = pr.dscrn;
+        const double vdist = third * r / thisvol;
+        const int i0 = onsite_base(kj) + 1;
+        double bup = U(i0) * U(i0), bdn = D(i0) * D(i0);
+        vu[ia * kinds + kj - 1] = vu[ia * kinds + kj - 1] + vdist * (df - bup) * std::exp(-bup * r) * f;
+        vd[ia * kinds + kj - 1] = vd[ia * kinds + kj - 1] + vdist * (df - bdn) * std::exp(-bdn * r) * f;
+        if (ja != ia) {
+            const int i1 = onsite_base(ki) + 1;
+            bup = U(i1) * U(i1); bdn = D(i1) * D(i1);
+            vu[ja * kinds + ki - 1] = vu[ja * kinds + ki - 1] + vdist * (df - bup) * std::exp(-bup * r) * f;
+            vd[ja * kinds + ki - 1] = vd[ja * kinds + ki - 1] + vdist * (df - bdn) * std::exp(-bdn * r) * f;
+        }
+    }
+    for (int ia = 0; ia < natoms; ++ia) {
+        const int k1 = kkind[ia];
+        int b = onsite_base(k1) + 1;
+        double u13 = std::pow(du[ia * kinds + k1 - 1], third), d13 = std::pow(dd[ia * kinds + k1 - 1], third);
+        double u23 = u13 * u13, d23 = d13 * d13;
+        double u2t = 2.0 * u23, u3t = threeh * u23, d2t = 2.0 * d23, d3t = threeh * d23;
+        double dru = twtrd * vu[ia * kinds + k1 - 1] / u13, drd = twtrd * vd[ia * kinds + k1 - 1] / d13;
+        for (int c = 0; c < 4; ++c, b += 4) {
+            const double dpup = dru * (U(b + 2) + u2t * (U(b + 3) + u3t * U(b + 4)));
+            const double dpdn = drd * (D(b + 2) + d2t * (D(b + 3) + d3t * D(b + 4)));
+            dparos[4 * ia + c] = half * (dpup + dpdn);
+            dspnos[4 * ia + c] = half * (dpup - dpdn);
+        }
+        for (int k2 = 1; k2 <= kinds; ++k2) {
+            if (k2 == k1 || du[ia * kinds + k2 - 1] == 0.0) continue;
+            u13 = std::pow(du[ia * kinds + k2 - 1], third); u23 = u13 * u13;
+            u2t = 2.0 * u23; u3t = threeh * u23;
+            dru = twtrd * vu[ia * kinds + k2 - 1] / u13;
+            d13 = std::pow(dd[ia * kinds + k2 - 1], third); d23 = d13 * d13;
+            d2t = 2.0 * d23; d3t = threeh * d23;
+            drd = twtrd * vu[ia * kinds + k2 - 1] / d13;
+            const int hi = k1 > k2 ? k1 : k2, lo = k1 < k2 ? k1 : k2;
+            b = kinds * (1 + kOnsite * 4) + 6 * kOnsite * ((hi - 1) * (hi - 2) / 2 + lo - 1);
+            if (k2 < hi) b += 3 * kOnsite;
+            for (int c = 0; c < 4; ++c, b += 3) {
+                const double dpup = dru * (U(b + 2) + u2t * (U(b + 3) + u3t * U(b + 4)));
+                const double dpdn = drd * (D(b + 2) + d2t * (D(b + 3) + d3t * D(b + 4)));
+                dparos[4 * ia + c] = half * (dpup + dpdn);
+                dspnos[4 * ia + c] = half * (dpup - dpdn);
+            }
+        }
+    }
+}
+
+// spinham.f:110-150: per orbital (rows in atom order), the diagonal spin
+// term t(q, atom) * s_z and the mixing term t(q, atom) * (s_x + i s_y) of
+// an atom of spin direction s, q the orbital type (s; p; t2g; eg).
+void spinor_terms(int natoms, const int *nbas, const double *spinos, const double *spinat, double *d,
+                  double *mre, double *mim)
+{
+    int i = 0;
+    for (int a = 0; a < natoms; ++a)
+        for (int k = 1; k <= nbas[a]; ++k, ++i) {
+            const int q = k == 1 ? 0 : (k < 5 ? 1 : (k < 8 ? 2 : 3));
+            const double t = spinos[4 * a + q];
+            d[i] = t * spinat[3 * a + 2];
+            mre[i] = t * spinat[3 * a];
+            mim[i] = t * spinat[3 * a + 1];
+        }
+}
+
 // rotate.f:112-258: two-centre table e(i,j) for direction cosines (l,m,n)
 // and integrals p(1..14) (p(11..14) used only for unlike kinds).
 void sk_block(double l, double m, double n, const double *pp, bool unlike, double *e /*9x9 row-major*/)
@@ -533,6 +671,25 @@ extern "C" int tbk_setparv_spin(int natoms, const int *kkind, int kinds, const d
     const double *own = param2 + (size_t)(jspin - 1) * nparam, *other = param2 + (size_t)(2 - jspin) * nparam;
     return tbk::setparv(natoms, kkind, kinds, own, nltype, jsover, thisvol, pl, den, dparos, dhop, dover,
                         other, flipos);
+}
+
+extern "C" int tbk_setpar_noncol(int natoms, const int *kkind, int kinds, const double *param2, int nparam,
+                                 int nltype, int jsover, double thisvol, int npairs, const double *dist,
+                                 const double *screen, const double *dscrn, const int *jkind, const int *jatm,
+                                 double *den2, double *paros, double *spinos, double *dparos, double *dspnos,
+                                 double *hop, double *over, double *dhop, double *dover)
+{
+    if (natoms <= 0 || npairs < 0 || !param2 || nparam < 1 || thisvol <= 0) return TBK_ERR_ARG;
+    tbk::PairList pl = pairs_from_arrays(npairs, dist, screen, dscrn, jkind, jatm);
+    const double *up = param2, *dn = param2 + nparam;
+    // bond and overlap integrals from channel 1 (setpar/setparv with jspin = 1)
+    std::vector<double> den1((size_t)natoms * kinds), scratch(4 * (size_t)natoms);
+    int rc = tbk::setpar(natoms, kkind, kinds, up, nltype, jsover, pl, den1.data(), scratch.data(), hop, over);
+    if (!rc) tbk::setpar_noncol(natoms, kkind, kinds, up, dn, pl, den2, paros, spinos);
+    if (!rc) rc = tbk::setparv(natoms, kkind, kinds, up, nltype, jsover, thisvol, pl, den1.data(),
+                               scratch.data(), dhop, dover);
+    if (!rc) tbk::setparv_noncol(natoms, kkind, kinds, up, dn, thisvol, pl, den2, dparos, dspnos);
+    return rc;
 }
 
 extern "C" int tbk_setparv(int natoms, const int *kkind, int kinds, const double *param,

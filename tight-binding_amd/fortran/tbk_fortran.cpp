@@ -56,8 +56,7 @@ static int device_ordinal()
 }
 
 // iberr beyond the reference's 1..4: 5 = feature outside this path
-// (non-collinear magnetisation, torques, spin directions, modes other than
-// 1..4), 6 = device or library error,
+// (torques, spin directions out, modes other than 1..4), 6 = device or library error,
 // 7 = S(k) not positive definite (the reference's ioverr), 8 = more than 9
 // degenerate states (perturb.f STOP).
 static int iberr_of(int rc)
@@ -77,11 +76,11 @@ extern "C" void bande_(int *mode, int *jspins, f_logical *flipos, f_logical *lno
                        double *earg1, double *earg2, double *smag, f_logical *lp_par, f_logical *lp_parv,
                        f_logical *lp_klst, int *kptpr, char *label, int *iberr, size_t label_len)
 {
-    (void)spinat; (void)dspth; (void)dspphi; (void)lp_par; (void)lp_parv; (void)lp_klst;
+    (void)dspth; (void)dspphi; (void)lp_par; (void)lp_parv; (void)lp_klst;
     (void)kptpr; (void)label; (void)label_len;
     *iberr = 0;
     if (*lnoncol && *jspins == 1) { *iberr = 1; return; }          // bande.f:695-703
-    if (*jspins < 1 || *jspins > 2 || *lnoncol || *ltorque || *lspdir || *mode < 1 || *mode > 4) {
+    if (*jspins < 1 || *jspins > 2 || *ltorque || *lspdir || *mode < 1 || *mode > 4) {
         *iberr = 5;
         return;
     }
@@ -116,6 +115,11 @@ extern "C" void bande_(int *mode, int *jspins, f_logical *flipos, f_logical *lno
     *smag = 0.0;
     if (*jspins == 1) {
         rc = tbk_plan_create(device_ordinal(), &d, nk, kp.data(), w.data(), 0, TBK_FP64, &p);
+    } else if (*lnoncol) {
+        // spinat(3, matom): the atoms' spin directions (input1.f); smag
+        // stays 0 as in the reference's non-collinear branch
+        rc = tbk_plan_create_noncol(device_ordinal(), &d, params_.param[1], spinat, nk, kp.data(), w.data(), 0,
+                                    TBK_FP64, &p);
     } else {
         // both channels of /params/ (param(npard, mspin)); flipos from input1
         std::vector<int> flip(nat);
@@ -124,7 +128,7 @@ extern "C" void bande_(int *mode, int *jspins, f_logical *flipos, f_logical *lno
                                   0, TBK_FP64, &p);
     }
     if (!rc) rc = tbk_plan_run(p, *mode, *tkb, *eigcut, efermi, earg1, earg2);
-    if (!rc && *jspins == 2) rc = tbk_plan_magnetization(p, smag);
+    if (!rc && *jspins == 2 && !*lnoncol) rc = tbk_plan_magnetization(p, smag);
     if (p) tbk_plan_destroy(p);
     *iberr = iberr_of(rc);
     if (rc) std::fprintf(stderr, "bande (device): %s\n", tbk_error_string(rc));

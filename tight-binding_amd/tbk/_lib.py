@@ -20,7 +20,7 @@ TBK_FP64, TBK_MIXED = 0, 1
 
 # Every symbol include/tbk.h declares (checked by the CPU test suite).
 EXPORTS = [
-    "tbk_search2", "tbk_setpar", "tbk_setparv", "tbk_setpar_spin", "tbk_setparv_spin",
+    "tbk_search2", "tbk_setpar", "tbk_setparv", "tbk_setpar_spin", "tbk_setparv_spin", "tbk_setpar_noncol",
     "tbk_reciprocal_lattice", "tbk_space_group",
     "tbk_space_group_read", "tbk_kptgen", "tbk_kmesh_weights", "tbk_klist", "tbk_kpath",
     "tbk_plan_create", "tbk_plan_run",
@@ -28,7 +28,7 @@ EXPORTS = [
     "tbk_plan_hs", "tbk_plan_nv", "tbk_plan_npairs", "tbk_plan_destroy",
     "tbk_plan_stage_times", "tbk_plan_set_timing", "tbk_plan_autotune", "tbk_gemm_tile",
     "tbk_plan_create_group", "tbk_plan_result", "tbk_plan_structure_eigenvalues",
-    "tbk_plan_create_spin", "tbk_plan_magnetization",
+    "tbk_plan_create_spin", "tbk_plan_magnetization", "tbk_plan_create_noncol",
     "tbk_plan_set_graph", "tbk_plan_set_window", "tbk_plan_order", "tbk_plan_nstruct", "tbk_plan_device_bytes",
     "tbk_multi_create", "tbk_multi_launch", "tbk_multi_wait", "tbk_multi_run", "tbk_multi_result",
     "tbk_multi_eigenvalues", "tbk_multi_ngroups", "tbk_multi_group_order", "tbk_multi_set_graph",
@@ -84,6 +84,8 @@ def lib():
                                       C.c_int, d, d, i, i, d, d, d, d]
         L.tbk_setparv_spin.argtypes = [C.c_int, i, C.c_int, d, C.c_int, C.c_int, vp, C.c_int, C.c_int,
                                        C.c_double, C.c_int, d, d, d, i, i, d, d, d, d]
+        L.tbk_setpar_noncol.argtypes = [C.c_int, i, C.c_int, d, C.c_int, C.c_int, C.c_int, C.c_double,
+                                        C.c_int, d, d, d, i, i, d, d, d, d, d, d, d, d, d]
         L.tbk_reciprocal_lattice.argtypes = [d, d, d, d, dp]
         L.tbk_space_group.argtypes = [C.c_int, C.c_int, d, d, C.c_int, d, d, C.c_void_p, d, d, d]
         L.tbk_space_group_read.argtypes = [C.c_char_p, C.c_int, C.c_int, d, d, C.c_int, ip, d, d,
@@ -98,6 +100,8 @@ def lib():
         L.tbk_plan_create_spin.argtypes = [C.c_int, C.POINTER(StructureDesc), d, vp, C.c_int, d, d,
                                             C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.tbk_plan_magnetization.argtypes = [vp, dp]
+        L.tbk_plan_create_noncol.argtypes = [C.c_int, C.POINTER(StructureDesc), d, d, C.c_int, d, d,
+                                              C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.tbk_plan_result.argtypes = [vp, C.c_int, dp, dp, dp]
         L.tbk_plan_structure_eigenvalues.argtypes = [vp, C.c_int, vp, vp]
         L.tbk_plan_set_graph.argtypes = [vp, C.c_int]

@@ -318,6 +318,38 @@ class SpinPlan(Plan):
         return e, d
 
 
+class NonCollinearPlan(Plan):
+    """A non-collinear structure (bande with jspins = 2, lnoncol true):
+    st.params is channel 1, param_dn channel 2, spinat [natoms, 3] each
+    atom's unit spin direction.  Items are the 2 nv spinor matrices
+    (addspins.f), one electron per state; eigenvalues() has 2 nv columns."""
+
+    def __init__(self, st: Structure, param_dn, spinat, kpts, weights, batch: int = 0,
+                 precision: int = TBK_FP64, device: int = 0, graph: bool = False):
+        L = lib()
+        self.st = st
+        self._keep = []
+        desc = _structure_desc(st, self._keep)
+        self.param_dn = np.ascontiguousarray(np.asarray(param_dn, float))
+        if len(self.param_dn) != len(st.params.param):
+            raise ValueError("param_dn must have as many values as channel 1")
+        self.spinat = np.ascontiguousarray(np.asarray(spinat, float).reshape(-1))
+        if len(self.spinat) != 3 * st.natoms:
+            raise ValueError("spinat needs three components per atom")
+        self.kpts = np.ascontiguousarray(np.asarray(kpts, float).reshape(-1, 3))
+        self.weights = np.ascontiguousarray(np.asarray(weights, float))
+        self.nk = len(self.weights)
+        h = C.c_void_p()
+        check(L.tbk_plan_create_noncol(device, C.byref(desc), self.param_dn, self.spinat, self.nk,
+                                       self.kpts.ravel(), self.weights, int(batch), int(precision),
+                                       C.byref(h)), "tbk_plan_create_noncol")
+        self.h = h
+        self.nv = L.tbk_plan_nv(h)
+        self.npairs = L.tbk_plan_npairs(h)
+        if graph:
+            L.tbk_plan_set_graph(h, 1)
+
+
 class MultiPlan:
     """Many structures of uneven orbital count, grouped by padded order
     (tbk_multi_*): each group is one batched plan, groups run concurrently."""

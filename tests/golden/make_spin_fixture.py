@@ -72,7 +72,22 @@ def ex1_magnetic():
                        check=True, capture_output=True)
         rows = spin_case.read_skeng(os.path.join(work, "SKENG"))
     assert len(rows) == 18 and all(len(r) == 6 for r in rows)
-    return dict(param_dn=pdn.tolist(), skeng=[list(r) for r in rows])
+    out = dict(param_dn=pdn.tolist(), skeng=[list(r) for r in rows])
+    # the same with spin input on every structure (input1.f:518-545)
+    for kind in ("noncol", "flip"):
+        with tempfile.TemporaryDirectory() as tmp:
+            work = os.path.join(tmp, "ex1")
+            shutil.copytree(os.path.join(ref, "ex1"), work)
+            for f in ("pd_par", "SKIN"):
+                os.unlink(os.path.join(work, f))
+            open(os.path.join(work, "pd_par"), "w").write(spin_case.nm_text(src, pdn))
+            open(os.path.join(work, "SKIN"), "w").write(
+                spin_case.skin_spin(open(os.path.join(ref, "ex1", "SKIN")).read(), kind))
+            subprocess.run("ulimit -s unlimited; " + os.path.join(ref, "static_ref"), shell=True, cwd=work,
+                           check=True, capture_output=True)
+            out["skeng_" + kind] = spin_case.read_skeng_rows(os.path.join(work, "SKENG"))
+        assert len(out["skeng_" + kind]) == 18
+    return out
 
 
 def noncol_cases():

@@ -39,3 +39,26 @@ def read_skeng(path):
             t = line.split()
             rows.append((" ".join(t[:2]),) + tuple(float(x) for x in t[2:7]))
     return rows
+
+
+def skin_spin(text, kind):
+    """ex1's SKIN with the spin input of input1.f:518-545, 736-786 on every
+    structure: kind "noncol" turns on non-collinear magnetisation ("FN")
+    with the atom's direction theta = 30, phi = 45 degrees; kind "flip"
+    allows spin flips ("FT") and flips the atom (magdir = 2)."""
+    import re
+    flag, extra = {"noncol": ("N", "  30.0 45.0"), "flip": ("T", "  2")}[kind]
+    out, n1 = re.subn(r"(?m)^F (\s+\(Logical variable)", "F" + flag + r"\1", text)
+    out, n2 = re.subn(r"(?m)^( 1  0\.000  0\.000  0\.000  0 0 0)", r"\1" + extra, out)
+    assert n1 == n2 == 18
+    return out
+
+
+def read_skeng_rows(path):
+    """SKENG rows as label and numbers (any column count)."""
+    rows = []
+    for line in open(path):
+        if line.strip():
+            t = line.split()
+            rows.append([" ".join(t[:2])] + [float(x) for x in t[2:]])
+    return rows

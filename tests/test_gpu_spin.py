@@ -142,6 +142,40 @@ def test_spin_plan_batches_and_graph_bitwise():
 
 @pytest.mark.skipif(not os.path.exists(EXE) or not os.path.isdir(EX1),
                     reason="drop-in program not built (oracle/Makefile dropin)")
+@pytest.mark.parametrize("kind", ["noncol", "flip"])
+def test_reference_program_with_device_bande_spin_input(tmp_path, kind):
+    """ex1 with a two-channel pd_par and spin input on every structure
+    (non-collinear at theta = 30, phi = 45 degrees; or the atom flipped):
+    the reference program through the device bande_ writes the SKENG of
+    the unmodified program to its printed digits."""
+    g = load()["ex1_magnetic"]
+    work = tmp_path / "ex1"
+    shutil.copytree(EX1, work)
+    src = (work / "pd_par").read_text()
+    skin = (work / "SKIN").read_text()
+    for f in ("pd_par", "SKIN"):
+        (work / f).unlink()
+    (work / "pd_par").write_text(spin_case.nm_text(src, np.array(g["param_dn"])))
+    (work / "SKIN").write_text(spin_case.skin_spin(skin, kind))
+    print(f"case: static_tbk on ex1, two-channel pd_par, spin input {kind}", flush=True)
+    r = subprocess.run("ulimit -s unlimited; " + EXE, shell=True, cwd=work, capture_output=True, text=True,
+                       timeout=300, env=dict(os.environ, TBK_DEVICE="0"))
+    assert r.returncode == 0, (r.stdout[-1000:], r.stderr[-2000:])
+    rows = spin_case.read_skeng_rows(work / "SKENG")
+    want = g["skeng_" + kind]
+    assert len(rows) == len(want)
+    worst = 0.0
+    for row, w in zip(rows, want):
+        assert row[0] == w[0] and len(row) == len(w) and row[1] == w[1]
+        worst = max(worst, max(abs(a - b) for a, b in zip(row[2:5], w[2:5])))
+        if len(w) > 5:
+            assert abs(row[5] - w[5]) <= 1.5e-6
+    print(f"  worst |SKENG - reference program| = {worst:.1e} (9 decimals)", flush=True)
+    assert worst <= 1.5e-9
+
+
+@pytest.mark.skipif(not os.path.exists(EXE) or not os.path.isdir(EX1),
+                    reason="drop-in program not built (oracle/Makefile dropin)")
 def test_reference_program_with_device_bande_magnetic(tmp_path):
     """ex1 with pd_par made a two-channel file: the reference program's
     jspins = 2 path through the device bande_; SKENG rows (with the moment

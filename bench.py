@@ -219,6 +219,46 @@ def bench_c3(args):
     print(json.dumps(out))
 
 
+def bench_spin(args):
+    """The C2 structure (32 atoms, 512 k) with a second parameter channel
+    (the first 57 values scaled by 1 + 0.02 N(0,1), seed 7): collinear spin
+    polarisation (1024 solves of N = 288 per step) and non-collinear spins
+    along alternating directions (512 solves of N = 576), both precisions."""
+    from tbk.api import NonCollinearPlan, SpinPlan
+    st = S.config_c2()
+    kpts = S.monkhorst_pack((8, 8, 8))
+    w = np.full(len(kpts), 1.0 / len(kpts))
+    rng = np.random.default_rng(7)
+    pdn = np.array(st.params.param, float)
+    pdn[:57] = pdn[:57] * (1.0 + 0.02 * rng.standard_normal(57))
+    t = np.deg2rad(30.0)
+    spinat = [[np.sin(t), 0.0, np.cos(t)] if a % 2 else [0.0, 0.0, 1.0] for a in range(st.natoms)]
+    out = {"workload": "C2 structure with two parameter channels: jspins=2 (collinear) and lnoncol, mode 4",
+           "unit": "k-point solves/s (one solve = one matrix of the path: N=288 per channel, N=576 spinor)"}
+    steps = max(1, args.steps // 2)
+    for kind in ("collinear", "noncollinear"):
+        for name, prec in (("fp64", TBK_FP64), ("mixed", TBK_MIXED)):
+            if kind == "collinear":
+                plan = SpinPlan(st, pdn, kpts, w, precision=prec)
+                nsolve = 2 * len(kpts)
+            else:
+                plan = NonCollinearPlan(st, pdn, spinat, kpts, w, precision=prec)
+                nsolve = len(kpts)
+            if not args.no_autotune:
+                plan.autotune(3)
+            for _ in range(args.warmup):
+                plan.run(4, 0.005, 10.0)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                res = plan.run(4, 0.005, 10.0)
+            ms = (time.perf_counter() - t0) * 1e3 / steps
+            plan.close()
+            out[f"{kind}_{name}"] = {"ms_per_step": ms, "solves_per_s": nsolve / (ms / 1e3), "order": plan.nv,
+                                     "result": res}
+            print(f"# {kind} {name}: N={plan.nv}, {ms:.1f} ms/step", file=sys.stderr, flush=True)
+    print(json.dumps(out))
+
+
 def bench_c4(args):
     """256 uneven structures (N = 9..576), all solved per step, both precisions."""
     from tbk.api import MultiPlan
@@ -263,15 +303,19 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--profile-pass", choices=["fp64", "mixed"], default=None)
-    ap.add_argument("--workload", choices=["c2", "c3", "c4"], default="c2",
-                    help="c2: the headline line; c3 (N=972, 64 k) and c4 (256 uneven "
-                         "structures) print their own lines")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "spin"], default="c2",
+                    help="c2: the headline line; c3 (N=972, 64 k), c4 (256 uneven "
+                         "structures) and spin (C2 spin-polarised and non-collinear) print their "
+                         "own lines")
     args = ap.parse_args()
     if args.profile_pass:
         profile_pass(args.profile_pass)
         return
     if args.workload == "c3":
         bench_c3(args)
+        return
+    if args.workload == "spin":
+        bench_spin(args)
         return
     if args.workload == "c4":
         bench_c4(args)

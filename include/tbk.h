@@ -419,6 +419,17 @@ int tbk_server_destroy(tbk_server *s);
 int tbk_zhegv(int device, int n, const double *h, const double *s,
               double *w, double *z);
 
+/* tbk_zhegv without any allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (tight serving loops, code
+ * that forbids hipMalloc).  The same kernels and results as tbk_zhegv; the
+ * device workspace is a fixed array the library reserves on each GPU when
+ * it is loaded there (about 39 MB, held until the process ends), so
+ * n <= 512 (TBK_ERR_ARG beyond).  Runs on the null stream and returns when
+ * the results are on the host.  Calls on one GPU share the array: this
+ * process's threads are serialised per GPU. */
+int tbk_zhegv_noalloc(int device, int n, const double *h, const double *s,
+                      double *w, double *z);
+
 /* Fermi level.  Replaces fermi(ntbd,nkptd,nkloc,istart,ifin,nv,jspins,el,
  * weight,eigen,tkb,zfill), fermi.f:1 (one process, jspins = 1).
  * eigen[k*ldn+i].  Evaluated on the device. */

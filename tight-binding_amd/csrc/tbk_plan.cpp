@@ -10,6 +10,8 @@
 #include <vector>
 #include <thread>
 #include <atomic>
+#include <mutex>
+#include <type_traits>
 
 #include "tbk.h"
 #include "tbk_internal.h"
@@ -130,6 +132,28 @@ struct Workspace {
             rc |= dalloc(&wyax32, bt); rc |= dalloc(&wudot32, (size_t)64 * b); rc |= dalloc(&wscal32, (size_t)2 * b);
         }
         return rc ? TBK_ERR_NOMEM : TBK_OK;
+    }
+    // The buffers of alloc(n_, b_) (FP64 path) laid out from base instead,
+    // each on a 256-byte boundary; returns the bytes they take (base null:
+    // measure only, the pointers stay null).
+    size_t carve(unsigned char *base, int n_, int b_)
+    {
+        n = n_; b = b_;
+        size_t off = 0;
+        auto take = [&](auto *&p, size_t count) {
+            typedef std::remove_reference_t<decltype(*p)> T;
+            p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+            off += (count * sizeof(T) + 255) / 256 * 256;
+        };
+        const size_t nn = (size_t)n * n * b, np = (size_t)n * kNB * b;
+        take(A, nn); take(S, nn); take(Y, nn); take(Z, nn); take(X, nn); take(VT, nn);
+        take(Vp, (size_t)n * 96 * b); take(Wp, np); take(Tf, tstride() * b);
+        take(px, (size_t)n * b); take(py, (size_t)n * b);
+        take(d, (size_t)n * b); take(e, (size_t)n * b); take(w, (size_t)n * b);
+        take(Vt, nn);
+        take(scratch, (size_t)5 * n * tbk::stein_threads(n) * b);
+        take(info, (size_t)b); take(sinfo, (size_t)b);
+        return off;
     }
     tbk::WideArgsT<double> wide64() const
     {
@@ -1468,4 +1492,66 @@ extern "C" int tbk_debug_gemm_time(int device, int precision, int tile, int m, i
     (void)hipEventDestroy(e1);
     dfree(a); dfree(b); dfree(c);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// tbk_zhegv_noalloc: tbk_zhegv with its whole device workspace in `scratch`,
+// device memory the code object reserves when the library is loaded on a GPU
+// (one copy per GPU, held until the process ends), for callers that cannot
+// allocate.  No hipMalloc, hipFree or stream creation: host copies and the
+// solve run on the null stream.  One call at a time per GPU (a per-device
+// mutex serialises this process's threads; each call overwrites the array).
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kNoAllocMaxN = 512;
+constexpr int kNoAllocDevices = 64;
+// Workspace::carve(nullptr, n, 1) for n = kNoAllocMaxN, in bytes: six
+// complex N x N slabs, the real N x N vectors, the panel and WY buffers, the
+// per-thread tridiagonal scratch (5 N ceil64(N) doubles) and the flags.
+constexpr size_t noalloc_bytes(size_t n)
+{
+    const size_t a = 256;
+    auto up = [a](size_t x) { return (x + a - 1) / a * a; };
+    const size_t nn = n * n, tst = (n + kNB - 1) / kNB * kNB * kNB, sth = (n + 63) / 64 * 64;
+    return 6 * up(nn * 16) + up(n * 96 * 16) + up(n * kNB * 16) + up(tst * 16) + 2 * up(n * 16) +
+           3 * up(n * 8) + up(nn * 8) + up(5 * n * sth * 8) + 2 * up(sizeof(int));
+}
+std::mutex g_noalloc_mu[kNoAllocDevices];
+unsigned char *g_noalloc_ptr[kNoAllocDevices];    // device address of scratch per GPU
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[noalloc_bytes(kNoAllocMaxN)];
+
+extern "C" int tbk_zhegv_noalloc(int device, int n, const double *h, const double *s, double *w, double *z)
+{
+    if (n < 2 || n > kNoAllocMaxN || !h || !s || !w || device < 0 || device >= kNoAllocDevices)
+        return TBK_ERR_ARG;
+    Workspace ws;
+    const size_t need = ws.carve(nullptr, n, 1);
+    if (need > sizeof(scratch)) return TBK_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return TBK_ERR_NODEVICE;
+    if (device >= ndev) return TBK_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu[device]);
+    HIP_TRY(hipSetDevice(device));
+    if (!g_noalloc_ptr[device]) {
+        // first call on this GPU: the address of its own copy of the array
+        void *ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return TBK_ERR_HIP;
+        g_noalloc_ptr[device] = static_cast<unsigned char *>(ptr);
+    }
+    ws.carve(g_noalloc_ptr[device], n, 1);
+    const size_t ss = ws.stride();
+    hipStream_t st = nullptr;
+    StageTimer tm;
+    int info = 0;
+    HIP_TRY(hipMemcpy(ws.A, h, ss * sizeof(double2), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ws.S, s, ss * sizeof(double2), hipMemcpyHostToDevice));
+    solve_batch(ws, 1, nullptr, ws.w, st, tm, Window{nullptr, 0, nullptr});
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(w, ws.w, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&info, ws.info, sizeof(int), hipMemcpyDeviceToHost));
+    if (z) HIP_TRY(hipMemcpy(z, ws.Z, ss * sizeof(double2), hipMemcpyDeviceToHost));
+    return info != 0 ? TBK_ERR_OVERLAP : TBK_OK;
 }

@@ -16,6 +16,11 @@ is --precision (default mixed).  Prints one JSON line with
                 stream around each launch) against the HBM peak;
   roofline_build  the H/S build: 2 * 16 * N^2 bytes per k written over the
                 build stage's event time, against the HBM peak;
+  roofline_solve  per precision, the eigensolve's LAPACK flop count (22.67 N^3
+                per k) over the summed event time of the solve stages,
+                against the FP64 (plain) or FP32 (mixed) matrix-core peak;
+  c3, c4        the 108-atom cell (N = 972, 64 k) and the 256-structure
+                uneven batch, both precisions (--no-extra skips them);
   serving       Poisson request streams (10, 100, 500 req/s) through the
                 continuous-batching request loop: p50/p99 latency;
   cpu_baseline  the CPU restatement (oracle) timed on this host's cores (at
@@ -76,6 +81,28 @@ def hetrd_panel_bytes(n: int, csize: int) -> float:
     panels = (n - 2) // 32 + 1
     tot += panels * 2 * 32 * n
     return tot * csize
+
+
+# LAPACK's flop counts for the generalized eigensolve with vectors, per k
+# at order N: ZPOTRF 4/3, ZHEGST 4, ZHETRD 16/3, apply Q 8 and the
+# triangular back-transform 4 (in N^3 real flops).
+SOLVE_FLOP_PER_N3 = 4.0 / 3.0 + 4.0 + 16.0 / 3.0 + 8.0 + 4.0
+SOLVE_STAGES = ("potrf_trtri", "reduce_gemm", "hetrd_panel", "hetrd_update", "stein", "backtransform",
+                "refine")
+
+
+def solve_roofline(n, nk, stages, precision):
+    """The eigensolve against the matrix-core peak of its precision:
+    22.67 N^3 flops per k over the summed time of the solve stages (HIP
+    events on the plan's stream), FP64 peak for the plain method, FP32 peak
+    for the mixed one (whose tridiagonalisation and back-transform run in
+    FP32)."""
+    ms = sum(stages.get(k, 0.0) for k in SOLVE_STAGES)
+    flops = SOLVE_FLOP_PER_N3 * n ** 3 * nk
+    tf = flops / (ms / 1e3) / 1e12 if ms > 0 else float("nan")
+    peak = FP64_PEAK_TFLOPS if precision == "fp64" else FP32_PEAK_TFLOPS
+    return {"bound": "mfma", "achieved": tf, "peak": peak, "unit": "TFLOP/s", "frac": tf / peak,
+            "algorithmic_flops": flops, "solve_ms": ms, "stages": [k for k in SOLVE_STAGES if k in stages]}
 
 
 def cpu_model() -> str:
@@ -214,9 +241,10 @@ def bench_c3(args):
     for name, prec in (("fp64", TBK_FP64), ("mixed", TBK_MIXED)):
         r = run_batched(st, kpts, w, prec, max(1, args.steps // 2), args.warmup, not args.no_autotune)
         out[name] = {"ksolves_per_s": r["ksolves_per_s"], "ms_per_step": r["ms_per_step"],
-                     "stages_ms_per_step": r["stages_ms_per_step"], "result": r["result"]}
+                     "stages_ms_per_step": r["stages_ms_per_step"], "result": r["result"],
+                     "roofline_solve": solve_roofline(st.nv, len(kpts), r["stages_ms_per_step"], name)}
         print(f"# C3 {name}: {r['ms_per_step']:.1f} ms/step", file=sys.stderr, flush=True)
-    print(json.dumps(out))
+    return out
 
 
 def bench_spin(args):
@@ -284,10 +312,11 @@ def bench_c4(args):
         bad = sum(1 for r in res if "error" in r)
         out[name] = {"ksolves_per_s": nk / dt, "ms_per_step": dt * 1e3, "groups": m.ngroups,
                      "gib": m.device_bytes() / 2 ** 30, "failed": bad,
-                     "n3k_per_s": work / dt}
+                     "n3k_per_s": work / dt,
+                     "solve_tflops": SOLVE_FLOP_PER_N3 * work / dt / 1e12}
         m.close()
         print(f"# C4 {name}: {dt * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
-    print(json.dumps(out))
+    return out
 
 
 def main():
@@ -302,6 +331,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the 108-atom and uneven-batch workloads the default line carries as "
+                         "extra keys c3 and c4")
     ap.add_argument("--profile-pass", choices=["fp64", "mixed"], default=None)
     ap.add_argument("--workload", choices=["c2", "c3", "c4", "spin"], default="c2",
                     help="c2: the headline line; c3 (N=972, 64 k), c4 (256 uneven "
@@ -312,13 +344,13 @@ def main():
         profile_pass(args.profile_pass)
         return
     if args.workload == "c3":
-        bench_c3(args)
+        print(json.dumps(bench_c3(args)))
         return
     if args.workload == "spin":
         bench_spin(args)
         return
     if args.workload == "c4":
-        bench_c4(args)
+        print(json.dumps(bench_c4(args)))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -358,6 +390,8 @@ def main():
     achieved = bytes_step / (panel_ms / 1e3) / 1e9
     roofline = {"kernel": "k_hetrd_panel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.precision),
+                "traffic_source": "profiles/c2_hetrd_panel_traffic.txt (rocprofv3 --pmc FETCH_SIZE, "
+                                  "WRITE_SIZE of the same step, scaled by a 1 GiB copy); not measured in this run",
                 "launches_per_step": launches, "avg_launch_ms": panel_ms / launches,
                 "algorithmic_bytes_per_launch": bytes_step / launches}
     # ---- the H/S build (k_phases + k_build_hs, HBM-bound): H and S written
@@ -368,6 +402,18 @@ def main():
     roofline_build = {"kernel": "k_build_hs", "bound": "hbm", "achieved": build_gbs, "peak": HBM_PEAK_GBS,
                       "unit": "GB/s", "frac": build_gbs / HBM_PEAK_GBS, "algorithmic_bytes": build_bytes,
                       "stage_ms": build_ms}
+
+    # ---- the eigensolve against the matrix-core peak, both precisions -----
+    roofline_solve = {k: solve_roofline(n, len(kpts), r["stages_ms_per_step"], k) for k, r in runs.items()}
+    for k, r in roofline_solve.items():
+        print(f"# solve {k}: {r['achieved']:.1f} TFLOP/s = {r['frac']:.3f} of {r['peak']}", file=sys.stderr,
+              flush=True)
+
+    # ---- the other workloads, as extra keys ---------------------------------
+    extra = {}
+    if not args.no_extra:
+        extra["c3"] = bench_c3(args)
+        extra["c4"] = bench_c4(args)
 
     serving = None
     if not args.no_serve:
@@ -402,8 +448,10 @@ def main():
         "result": {k: r["result"] for k, r in runs.items()},
         "roofline": roofline,
         "roofline_build": roofline_build,
+        "roofline_solve": roofline_solve,
         "serving": serving,
     }
+    out.update(extra)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(st, kpts, w)
     print(json.dumps(out))

@@ -23,6 +23,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "libnrl_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libnrlref.so")
+# the same routines built with matom = 128, nkd = 1024 (make -C oracle refbig)
+REF_SO_BIG = os.path.join(HERE, "_ref", "big", "libnrlref_big.so")
 
 _d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _i = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -71,17 +73,29 @@ def lib():
     return _lib
 
 
-def ref_available() -> bool:
-    return os.path.exists(REF_SO)
+_ref_path = REF_SO
+
+
+def ref_available(big: bool = False) -> bool:
+    return os.path.exists(REF_SO_BIG if big else REF_SO)
+
+
+def use_big_reference():
+    """Make ref() load the build with matom = 128 (structures of up to 128
+    atoms, up to 1024 k-points per call).  Only before the first ref()."""
+    global _ref_path
+    if _ref is not None and _ref_path != REF_SO_BIG:
+        raise RuntimeError("the ex1-dimensioned reference is already loaded in this process")
+    _ref_path = REF_SO_BIG
 
 
 def ref():
     """The reference's own routines (only where it was built)."""
     global _ref
     if _ref is None:
-        if not ref_available():
-            raise RuntimeError("reference build not present (make -C oracle ref)")
-        L = C.CDLL(REF_SO)
+        if not os.path.exists(_ref_path):
+            raise RuntimeError(f"reference build not present: {_ref_path} (make -C oracle ref / refbig)")
+        L = C.CDLL(_ref_path)
         ip = C.POINTER(C.c_int)
         dp = C.POINTER(C.c_double)
         L.ref_setup.argtypes = [ip, _d, _d, _d, _i, _i, ip, _i, _d, _d, _d, _d, ip, ip,

@@ -1,0 +1,187 @@
+"""The benchmark workloads at full size against the reference's own results
+(tests/golden/ref_large.npz, from the reference's sources compiled with
+matom = 128 by tests/golden/make_ref_large_fixture.py):
+
+* the 32-atom cell (N = 288), all 512 k of its 8x8x8 mesh in one batch:
+  every eigenvalue of every k, E_F, E(T), P(T);
+* the 108-atom strained cell (N = 972), all 64 k of its 4x4x4 mesh;
+* the 256-structure uneven batch (N = 9..576, 25,969 k) in one multi plan:
+  E_F, E(T), P(T) of every structure in FP64, and the mixed-precision
+  results against the FP64 ones;
+* a Poisson stream of 120 requests of up to 128 atoms (N <= 1152) through
+  the request loop: each result equal to the same request run alone, and
+  to an FP64 plan within the serving accuracy.
+
+Bounds: FP64 as the plain method's (eigenvalues 1e-10 Ry, E_F and E/atom
+1e-9 Ry, P 1e-6 kbar).  Mixed precision: within about ten times the largest
+error measured on these workloads (DESIGN section 5 lists them).
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from tbk import _lib
+from tbk import structures as S
+from tbk.api import MultiPlan, Plan, Server
+
+pytestmark = pytest.mark.gpu
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_large.npz"))
+SEED = int(GOLD["seed"][0])
+FP64_TOL = dict(eig=1e-10, ef=1e-9, e=1e-9, p=1e-6)
+MIX_TOL = {"c2": dict(eig=1e-9, ef=2e-12, e=1e-10, p=2e-5),
+           "c3": dict(eig=1e-9, ef=5e-12, e=2e-10, p=5e-6)}
+# mixed against FP64 over the 256 structures, and a served result against an FP64 plan
+MIX_C4 = dict(ef=5e-12, e=1e-10, p=1e-3)
+SERVE_FP64 = dict(ef=1e-6, e=1e-6)
+
+
+def workload(name):
+    if name == "c2":
+        return S.config_c2(SEED), S.monkhorst_pack((8, 8, 8))
+    return S.config_c3(SEED), S.monkhorst_pack((4, 4, 4))
+
+
+def errors(st, r, e, name):
+    ef, e1, e2 = GOLD[name + "_out"]
+    return dict(eig=float(np.abs(e - GOLD[name + "_eig"]).max()), ef=abs(r["efermi"] - ef),
+                e=abs(r["earg1"] - e1) / st.natoms, p=abs(r["earg2"] - e2) * S.RYBOHR3_KBAR)
+
+
+@pytest.mark.parametrize("prec", [_lib.TBK_FP64, _lib.TBK_MIXED])
+@pytest.mark.parametrize("name", ["c2", "c3"])
+def test_cell_full_mesh_vs_reference(name, prec):
+    st, k = workload(name)
+    w = np.full(len(k), 1.0 / len(k))
+    print(f"case: {name} full mesh N={st.nv} nk={len(k)} precision {prec}", flush=True)
+    plan = Plan(st, k, w, precision=prec)
+    try:
+        r = plan.run(4, 0.005, 10.0)
+        e, _ = plan.eigenvalues()
+    finally:
+        plan.close()
+    err = errors(st, r, e, name)
+    print("  " + " ".join(f"{x}={v:.2e}" for x, v in err.items()), flush=True)
+    tol = FP64_TOL if prec == _lib.TBK_FP64 else MIX_TOL[name]
+    for x in tol:
+        assert err[x] < tol[x], (x, err[x], tol[x])
+
+
+@pytest.fixture(scope="module")
+def c4_fp64():
+    pool = S.config_c4(SEED)
+    ks = [S.monkhorst_pack((q, q, q)) for _, q in pool]
+    ws = [np.full(len(k), 1.0 / len(k)) for k in ks]
+    print(f"case: uneven batch, {len(pool)} structures, {sum(len(k) for k in ks)} k, FP64", flush=True)
+    m = MultiPlan([st for st, _ in pool], ks, ws, precision=_lib.TBK_FP64)
+    try:
+        res = m.run(4)
+    finally:
+        m.close()
+    return pool, ks, ws, res
+
+
+def test_uneven_batch_fp64_vs_reference(c4_fp64):
+    pool, _, _, res = c4_fp64
+    ref = GOLD["c4_out"]
+    worst = dict(ef=0.0, e=0.0, p=0.0)
+    for i, (st, _) in enumerate(pool):
+        assert "error" not in res[i], (i, res[i])
+        worst["ef"] = max(worst["ef"], abs(res[i]["efermi"] - ref[i, 0]))
+        worst["e"] = max(worst["e"], abs(res[i]["earg1"] - ref[i, 1]) / st.natoms)
+        worst["p"] = max(worst["p"], abs(res[i]["earg2"] - ref[i, 2]) * S.RYBOHR3_KBAR)
+    print(f"  256 structures vs reference: {worst}", flush=True)
+    for x in worst:
+        assert worst[x] < FP64_TOL[x], (x, worst[x])
+
+
+def test_uneven_batch_mixed_vs_fp64(c4_fp64):
+    pool, ks, ws, res64 = c4_fp64
+    print("case: uneven batch, mixed precision", flush=True)
+    m = MultiPlan([st for st, _ in pool], ks, ws, precision=_lib.TBK_MIXED, graph=True)
+    try:
+        res = m.run(4)
+    finally:
+        m.close()
+    worst = dict(ef=0.0, e=0.0, p=0.0)
+    dps = []
+    for i, (st, q) in enumerate(pool):
+        assert "error" not in res[i], (i, res[i])
+        worst["ef"] = max(worst["ef"], abs(res[i]["efermi"] - res64[i]["efermi"]))
+        worst["e"] = max(worst["e"], abs(res[i]["earg1"] - res64[i]["earg1"]) / st.natoms)
+        dp = abs(res[i]["earg2"] - res64[i]["earg2"]) * S.RYBOHR3_KBAR
+        worst["p"] = max(worst["p"], dp)
+        dps.append((dp, i, st.label, st.nv, q))
+    print(f"  256 structures, mixed vs FP64: {worst}", flush=True)
+    print("  largest dP (kbar, index, structure, N, q): " + str(sorted(dps)[-6:]), flush=True)
+    for x in worst:
+        assert worst[x] < MIX_C4[x], (x, worst[x])
+
+
+@pytest.fixture(scope="module")
+def served():
+    """120 requests drawn from the serving pool (up to 128 atoms, meshes up
+    to 6^3), submitted at Poisson arrival times (100 req/s) to a mixed
+    precision server."""
+    pool = S.config_c4(seed=SEED, count=256, max_atoms=128, max_q=6, nk_budget=9 * 6 ** 3)
+    rng = np.random.default_rng(SEED + 5)
+    picks = [int(i) for i in rng.integers(len(pool), size=120)]
+    big = max(range(len(pool)), key=lambda i: pool[i][0].nv)
+    picks[7] = big                      # the largest order of the pool is in the stream
+    gaps = rng.exponential(1.0 / 100.0, size=len(picks))
+    srv = Server(precision=_lib.TBK_MIXED, budget_bytes=64 << 30, max_inflight=8)
+    ids = []
+    print(f"case: served stream, {len(picks)} requests, largest N={pool[big][0].nv}", flush=True)
+    t_next = time.perf_counter()
+    try:
+        for i, g in zip(picks, gaps):
+            st, q = pool[i]
+            k = S.monkhorst_pack((q, q, q))
+            t_next += g
+            while time.perf_counter() < t_next:
+                time.sleep(0.0005)
+            ids.append(srv.submit(st, k, np.full(len(k), 1.0 / len(k))))
+        res = [srv.result(r) for r in ids]
+        stats = srv.stats()
+    finally:
+        srv.close()
+    print(f"  stats {stats}", flush=True)
+    return pool, picks, res, stats
+
+
+def test_served_stream_same_as_alone(served):
+    pool, picks, res, stats = served
+    assert stats["completed"] == len(picks) and stats["max_group"] >= 2
+    worst = 0.0
+    for i, r in zip(picks, res):
+        assert r["status"] == 0, r
+        st, q = pool[i]
+        k = S.monkhorst_pack((q, q, q))
+        p = Plan(st, k, np.full(len(k), 1.0 / len(k)), precision=_lib.TBK_MIXED)
+        try:
+            a = p.run(4)
+        finally:
+            p.close()
+        worst = max(worst, max(abs(r[x] - a[x]) for x in ("efermi", "earg1", "earg2")))
+    print(f"  served vs alone: max |d| = {worst:.2e}", flush=True)
+    assert worst < 1e-10
+
+
+def test_served_stream_vs_fp64(served):
+    pool, picks, res, _ = served
+    worst = dict(ef=0.0, e=0.0)
+    for i, r in zip(picks, res):
+        st, q = pool[i]
+        k = S.monkhorst_pack((q, q, q))
+        p = Plan(st, k, np.full(len(k), 1.0 / len(k)), precision=_lib.TBK_FP64)
+        try:
+            a = p.run(4)
+        finally:
+            p.close()
+        worst["ef"] = max(worst["ef"], abs(r["efermi"] - a["efermi"]))
+        worst["e"] = max(worst["e"], abs(r["earg1"] - a["earg1"]) / st.natoms)
+    print(f"  served (mixed) vs FP64 plans: {worst}", flush=True)
+    for x in worst:
+        assert worst[x] < SERVE_FP64[x]

@@ -36,4 +36,5 @@ def test_c3_slice_vs_oracle(c3_ref, prec):
     if prec == _lib.TBK_FP64:
         assert de < 1e-10 and def_ < 1e-9 and dE < 1e-9 and dP < 1e-6
     else:
-        assert de < 1e-8 and def_ < 1e-6 and dE < 1e-6 and dP < 1e-3
+        # about ten times the errors measured (4.6e-11, 3.4e-13, 1.6e-11 Ry, 1.6e-7 kbar)
+        assert de < 5e-10 and def_ < 4e-12 and dE < 2e-10 and dP < 2e-6

@@ -25,6 +25,11 @@ from tbk.api import NonCollinearPlan, Plan, SpinPlan
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(ROOT, "tests", "golden", "spin.json")
+# mixed precision against the reference's results, about ten times the
+# largest error measured: E_F, energies and moment (measured 4.0e-11 Ry),
+# pressure (measured 6.5e-9 Ry/bohr^3 = 1e-3 kbar, fcc4_flip on 27 k)
+MIX_SPIN = 5e-10
+MIX_SPIN_P = 7e-8
 EXE = os.path.join(ROOT, "oracle", "_ref", "static_tbk")
 EX1 = os.path.join(ROOT, "oracle", "_ref", "ex1")
 
@@ -71,8 +76,10 @@ def test_spin_plan_mixed_precision():
     plan = SpinPlan(st, c["param_dn"], c["kpts"], c["weights"], flipos=c["flipos"], precision=TBK_MIXED)
     r = plan.run(4, g["tkb"], g["eigcut"])
     want = c["modes"]["4"]
-    for key in ("efermi", "earg1", "earg2", "smag"):
-        assert abs(r[key] - want[key]) <= 1e-6, (key, r[key], want[key])
+    err = {key: abs(r[key] - want[key]) for key in ("efermi", "earg1", "earg2", "smag")}
+    print(f"  mixed vs reference: {err}", flush=True)
+    for key in err:
+        assert err[key] <= (MIX_SPIN_P if key == "earg2" else MIX_SPIN), (key, r[key], want[key])
     plan.close()
 
 
@@ -205,7 +212,7 @@ def test_reference_program_with_device_bande_magnetic(tmp_path):
 @pytest.mark.parametrize("idx", [0, 1])
 def test_noncollinear_plan_vs_reference(idx):
     """Spinor items of order 2 nv (addspins.f) against the reference's
-    lnoncol bande, modes 1-4, FP64; mode 4 mixed within 1e-6."""
+    lnoncol bande, modes 1-4, FP64; mode 4 mixed within MIX_SPIN (MIX_SPIN_P for P)."""
     g = load()
     c = g["noncol"][idx]
     st = structure(c)
@@ -223,8 +230,10 @@ def test_noncollinear_plan_vs_reference(idx):
     plan.close()
     m = NonCollinearPlan(st, c["param_dn"], c["spinat"], c["kpts"], c["weights"], precision=TBK_MIXED)
     r = m.run(4, g["tkb"], g["eigcut"])
-    for key in ("efermi", "earg1", "earg2"):
-        assert abs(r[key] - c["modes"]["4"][key]) <= 1e-6, key
+    err = {key: abs(r[key] - c["modes"]["4"][key]) for key in ("efermi", "earg1", "earg2")}
+    print(f"  mixed vs reference: {err}", flush=True)
+    for key in err:
+        assert err[key] <= (MIX_SPIN_P if key == "earg2" else MIX_SPIN), key
     m.close()
 
 

@@ -257,3 +257,30 @@ def test_pressure_is_minus_dE_dV():
     dv = base.volume * ((1 + h) ** 3 - (1 - h) ** 3)
     p_fd = -(ep - em) / dv
     assert abs(r0["earg2"] - p_fd) < 2e-6 * max(1.0, abs(p_fd)) + 1e-7
+
+
+def test_oracle_vs_reference_at_supercell_size():
+    """The restatement against the reference's own runs at the sizes of the
+    benchmark workloads (tests/golden/ref_large.npz, the reference built with
+    matom = 128): eigenvalues of the 32-atom (N = 288) and 108-atom (N = 972)
+    cells at a few k-points, and E_F, E(T), P(T) of three structures of the
+    uneven batch."""
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_large.npz"))
+    seed = int(gold["seed"][0])
+    for name, st, k, idx in (("c2", S.config_c2(seed), S.monkhorst_pack((8, 8, 8)), (0, 137, 511)),
+                             ("c3", S.config_c3(seed), S.monkhorst_pack((4, 4, 4)), (21,))):
+        o = oracle.Setup(st)
+        for i in idx:
+            hp, sp = o.hs(k[i])
+            info, e, _ = oracle.zhegv(o.unpack(st.nv, hp), o.unpack(st.nv, sp), vectors=False)
+            assert info == 0
+            assert np.abs(e - gold[name + "_eig"][i]).max() < 1e-12, (name, i)
+    pool = S.config_c4(seed)
+    for i in (0, 1, 2):
+        st, q = pool[i]
+        k = S.monkhorst_pack((q, q, q))
+        r = oracle.Setup(st).bande(k, np.full(len(k), 1.0 / len(k)), mode=4, nthreads=8)
+        ef, e1, e2 = gold["c4_out"][i]
+        assert abs(r["efermi"] - ef) < 1e-12
+        assert abs(r["earg1"] - e1) / st.natoms < 1e-12
+        assert abs(r["earg2"] - e2) * S.RYBOHR3_KBAR < 1e-7

@@ -9,7 +9,8 @@ matom = 128 by tests/golden/make_ref_large_fixture.py):
   E_F, E(T), P(T) of every structure in FP64, and the mixed-precision
   results against the FP64 ones;
 * a Poisson stream of 120 requests of up to 128 atoms (N <= 1152) through
-  the request loop: each result equal to the same request run alone, and
+  the request loop: each result equal to the same request run alone (to
+  the mixed path's rounding, which the group a request joins changes), and
   to an FP64 plan within the serving accuracy.
 
 Bounds: FP64 as the plain method's (eigenvalues 1e-10 Ry, E_F and E/atom
@@ -36,6 +37,11 @@ MIX_TOL = {"c2": dict(eig=1e-9, ef=2e-12, e=1e-10, p=2e-5),
 # mixed against FP64 over the 256 structures, and a served result against an FP64 plan
 MIX_C4 = dict(ef=5e-12, e=1e-10, p=1e-3)
 SERVE_FP64 = dict(ef=1e-6, e=1e-6)
+# a served result against the same request alone: the group a request joins
+# (its padded order, its batch) changes the FP32 rounding of the mixed path,
+# so the two agree to the mixed path's own error (measured 2.5e-12 .. 1.4e-10
+# Ry per cell, about 1e-12 Ry per atom)
+SERVE_ALONE = dict(ef=1e-10, e=1e-10, p=1e-4)
 
 
 def workload(name):
@@ -154,7 +160,7 @@ def served():
 def test_served_stream_same_as_alone(served):
     pool, picks, res, stats = served
     assert stats["completed"] == len(picks) and stats["max_group"] >= 2
-    worst = 0.0
+    worst = dict(ef=0.0, e=0.0, p=0.0)
     for i, r in zip(picks, res):
         assert r["status"] == 0, r
         st, q = pool[i]
@@ -164,9 +170,12 @@ def test_served_stream_same_as_alone(served):
             a = p.run(4)
         finally:
             p.close()
-        worst = max(worst, max(abs(r[x] - a[x]) for x in ("efermi", "earg1", "earg2")))
-    print(f"  served vs alone: max |d| = {worst:.2e}", flush=True)
-    assert worst < 1e-10
+        worst["ef"] = max(worst["ef"], abs(r["efermi"] - a["efermi"]))
+        worst["e"] = max(worst["e"], abs(r["earg1"] - a["earg1"]) / st.natoms)
+        worst["p"] = max(worst["p"], abs(r["earg2"] - a["earg2"]) * S.RYBOHR3_KBAR)
+    print(f"  served vs alone: {worst}", flush=True)
+    for x in worst:
+        assert worst[x] < SERVE_ALONE[x], (x, worst[x])
 
 
 def test_served_stream_vs_fp64(served):

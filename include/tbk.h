@@ -40,6 +40,7 @@ extern "C" {
 #define TBK_ERR_HIP       10  /* HIP runtime error                         */
 #define TBK_ERR_NOMEM     11  /* device allocation failed                  */
 #define TBK_ERR_NODEVICE  12  /* no usable GPU                             */
+#define TBK_END           (-1) /* tbk_skin_next: no more structures        */
 
 /* Precision of a plan: the plain all-FP64 method or the mixed method. */
 #define TBK_FP64          0   /* all-FP64 plain method                     */
@@ -181,6 +182,87 @@ int tbk_klist(int nk, const double *raw, double sfk, double *q, double *w);
  * ends[6p .. 6p+2] to ends[6p+3 .. 6p+5] (kptin.f:223-251). */
 int tbk_kpath(int npath, const int *kd, const double *ends, int maxk,
               int *nk, double *q, double *w);
+
+/* ---------------------------------------------------------------------
+ * The reference program's input and output files, read and written
+ * natively (host).  Reads follow its Fortran I/O statement by statement;
+ * lattices and positions are bit-identical to those it passes to bande.
+ * --------------------------------------------------------------------- */
+#define TBK_MAXKINDS 8
+
+/* One NRL parameter file (input.f:181-359). */
+typedef struct tbk_param_set {
+    int jsover;           /* first header character: 'O' orthogonal, else 1  */
+    int jspins;           /* second: 'M' two spin channels, else 1           */
+    int nltype;           /* I5 style field: 0..3 or 90000 (others read as 0)  */
+    int kinds;
+    int nparam;           /* values the file holds per channel (the rest are
+                             zero); the program needs kinds*(29+68*kinds)    */
+    int kbas[TBK_MAXKINDS];
+    double valence[TBK_MAXKINDS];                        /* s + p + d      */
+    double dnn[TBK_MAXKINDS * (TBK_MAXKINDS + 1) / 2];    /* cutoff R_c     */
+    double screenl[TBK_MAXKINDS * (TBK_MAXKINDS + 1) / 2];
+    char title[80];
+} tbk_param_set;
+
+/* Read an NRL parameter file.  Replaces input(nparam,jspins) (input.f:139,
+ * its reads of unit 32, input.f:181-359).  param [cap]: jspins channels of
+ * kinds*(29+68*kinds) values, param(:,1) then param(:,2).  TBK_ERR_ARG for
+ * an unreadable file, kinds > TBK_MAXKINDS or cap too small. */
+int tbk_read_params(const char *path, tbk_param_set *ps, int cap,
+                    double *param);
+
+/* A SKIN control file, read structure by structure. */
+typedef struct tbk_skin tbk_skin;
+
+/* What the reference program reads of one structure (static.f:938-950,
+ * setlat.f, input1.f, kptin.f). */
+typedef struct tbk_skin_structure {
+    char title[72];       /* (a70)                                        */
+    char label[24];       /* (a20) SKENG label, 20 characters             */
+    double zcharge;       /* electrons above the kinds' valence           */
+    double zfill;         /* electrons (input1.f:735, 820-829)            */
+    double thisvol;       /* cell volume (setlat.f:997-1000)              */
+    double lat[9];        /* lat[3*v+c] = plv(v,c), bohr                  */
+    int latnum, strain;   /* setlat's lattice and strain options          */
+    int natoms, nd[3];
+    int lphonon, spinflip, lnoncol, ltorque, lspdir;
+    int nk;               /* k-points (the previous ones for NEWSYM=F)    */
+    int inpstat;          /* input1's error code, 1000 + setlat's         */
+} tbk_skin_structure;
+
+/* Open a SKIN file and read its header: the mode (static.f:706,
+ * modeparse.f), kT and eigcut (static.f:818) and the parameter file name
+ * (input.f:139), resolved against the SKIN file's directory. */
+int tbk_skin_open(const char *path, tbk_skin **out, int *mode, double *tkb,
+                  double *eigcut, char *param_path, int param_path_cap);
+
+/* The next structure: TBK_OK, TBK_END at the end of the file (where the
+ * program ends normally), or an error where the program stops.  ps gives the
+ * kinds' valence.  Arrays of capacity maxatoms / maxk: kkind, posn
+ * (Cartesian, static.f:1088-1103), posq (phase origins, lattice units),
+ * flipos and spinat [3*natoms] (may be NULL), kpts [3*maxk], weights
+ * [maxk] (kptin.f, bit-identical). */
+int tbk_skin_next(tbk_skin *s, const tbk_param_set *ps, int maxatoms,
+                  int maxk, tbk_skin_structure *st, int *kkind, double *posn,
+                  double *posq, int *flipos, double *spinat, double *kpts,
+                  double *weights);
+int tbk_skin_close(tbk_skin *s);
+
+/* One SKENG line (static.f:1238-1254): (a20,f12.6,3f15.9) in modes 1, 2 and
+ * 4, (a20,f12.6,2f15.9) in mode 3, with f10.6 of the moment appended when
+ * with_smag (collinear jspins = 2).  buf: at least 96 characters. */
+int tbk_skeng_line(const char *label, double thisvol, double efermi,
+                   double earg1, double earg2, double smag, int mode,
+                   int with_smag, char *buf, int cap);
+
+/* The reference program's loop for this path (static.f:925-1255): every
+ * structure of dir/SKIN through the device plans (modes 1-4; jspins 1 and 2,
+ * collinear and non-collinear), its line written to dir/SKENG as each one
+ * finishes.  Stops at the first structure the reference would stop on, or
+ * the device cannot run (N = 1, torques and spin directions out), with the
+ * lines of the structures before it written.  *nstruct: structures done. */
+int tbk_static_run(const char *dir, int device, int precision, int *nstruct);
 
 /* ---------------------------------------------------------------------
  * Device plans: one structure, its k-mesh, device-resident H/S/Z state.

@@ -114,6 +114,9 @@ def ref():
         L.ref_bande_spin.argtypes = [ip, ip, _d, _d, dp, dp, dp, dp, _d, ip, _i, dp, dp, dp, dp, ip]
         L.ref_kptin.argtypes = [_d, C.c_char_p, ip, C.c_char_p, ip, ip, ip, _d, _d, _d, _d, _d,
                                 dp, ip, ip, _d, _d, _d, ip]
+        L.ref_input.argtypes = [C.c_char_p, ip, ip, ip, ip, ip, ip, _i, _d, _d, _d, _d, ip]
+        L.ref_input1.argtypes = [C.c_char_p, ip, ip, _i, _d, dp, _d, dp, ip, _i, _i, _d, _d, dp, _i, ip, _d,
+                                 ip]
         _ref = L
     return _ref
 
@@ -378,3 +381,53 @@ class RefSetup:
                         C.byref(C.c_double(eigcut)), C.byref(ef), C.byref(e1),
                         C.byref(e2), C.byref(ib))
         return dict(err=ib.value, efermi=ef.value, earg1=e1.value, earg2=e2.value)
+
+
+# dimensions of the ex1 reference build (NRLtb/ex1/P1)
+REF_MATOM, REF_MKIND = 20, 2
+REF_NPARD = REF_MKIND * (29 + 68 * REF_MKIND)
+
+
+def ref_input(skin_line_file: str) -> dict:
+    """The reference's own parameter-file reader, input(nparam, jspins),
+    on a unit-10 file whose line is the parameter file's name."""
+    L = ref()
+    fn = os.fsencode(skin_line_file)
+    ci = lambda: C.c_int(0)
+    npo, jsp, jso, nlt, kin, ierr = ci(), ci(), ci(), ci(), ci(), ci()
+    kb = np.zeros(REF_MKIND, np.int32)
+    val = np.zeros(REF_MKIND)
+    mint = REF_MKIND * (REF_MKIND + 1) // 2
+    dnn, scr = np.zeros(mint), np.zeros(mint)
+    par = np.zeros(2 * REF_NPARD)
+    L.ref_input(fn, C.byref(C.c_int(len(fn))), C.byref(npo), C.byref(jsp), C.byref(jso), C.byref(nlt),
+                C.byref(kin), kb, val, dnn, scr, par, C.byref(ierr))
+    k = kin.value
+    npard = k * (29 + 68 * k)
+    # param(npard_dim, mspin) column-major: channel j at [j * REF_NPARD, ...]
+    chans = np.stack([par[j * REF_NPARD:j * REF_NPARD + npard] for j in range(jsp.value)])
+    return dict(nparam=npo.value, jspins=jsp.value, jsover=jso.value, nltype=nlt.value, kinds=k,
+                kbas=kb[:k].copy(), valence=val[:k].copy(), dnn=dnn[:k * (k + 1) // 2].copy(),
+                screenl=scr[:k * (k + 1) // 2].copy(), param=chans)
+
+
+def ref_input1(block_file: str, kbas, valence, zcharge: float = 0.0) -> dict:
+    """The reference's own structure reader (setlat + input1, then the main
+    program's conversion to Cartesian positions) on a unit-10 file holding a
+    SKIN structure block from its lattice line on."""
+    L = ref()
+    fn = os.fsencode(block_file)
+    nk = len(kbas)
+    plv, vol, nat = np.zeros(9), C.c_double(0), C.c_int(0)
+    nd = np.zeros(3, np.int32)
+    kk = np.zeros(REF_MATOM, np.int32); fl = np.zeros(REF_MATOM, np.int32)
+    posn, posq, sp = np.zeros(3 * REF_MATOM), np.zeros(3 * REF_MATOM), np.zeros(3 * REF_MATOM)
+    zf, lnc, inp = C.c_double(0), C.c_int(0), C.c_int(0)
+    L.ref_input1(fn, C.byref(C.c_int(len(fn))), C.byref(C.c_int(nk)), np.asarray(kbas, np.int32),
+                 np.asarray(valence, float), C.byref(C.c_double(zcharge)), plv, C.byref(vol), C.byref(nat), nd,
+                 kk, posn, posq, C.byref(zf), fl, C.byref(lnc), sp, C.byref(inp))
+    n = nat.value
+    return dict(inpstat=inp.value, lat=plv.reshape(3, 3).copy(), volume=vol.value, natoms=n, nd=nd.copy(),
+                kkind=kk[:n].copy(), posn=posn[:3 * n].reshape(n, 3).copy(),
+                posq=posq[:3 * n].reshape(n, 3).copy(), zfill=zf.value, flipos=fl[:n].copy(),
+                lnoncol=lnc.value, spinat=sp[:3 * n].reshape(n, 3).copy())

@@ -659,3 +659,135 @@ c     passed as byte arrays of the given lengths.
       end do
       return
       end
+c
+c     ref_input: the reference's parameter-file reader, input(nparam,
+c     jspins), on unit 10 = the file fin (one line: the parameter file's
+c     name, as SKIN holds it), unit 15 to /dev/null.  Returns what it
+c     leaves in /params/, /parinfo/, /cutcom/, /parcom/ and /lia/.
+      subroutine ref_input(fin,nfin,npo,jspo,jsovo,nlto,kindo,kbo,
+     $     valo,dnno,scro,paro,ierr) bind(C,name='ref_input')
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      integer nfin,npo,jspo,jsovo,nlto,kindo,ierr
+      integer*1 fin(nfin)
+      integer kbo(mkind)
+      real*8 valo(mkind),dnno(mint),scro(mint),paro(npard,mspin)
+      character*255 fn
+      common /parinfo/ valence(mkind),kinds,kbas(mkind)
+      common /cutcom/ dnnsq(mint),dnn(mint),
+     $     screenl(mint),scrninv(mint)
+      common /params/ param(npard,mspin)
+      logical realov,harrison
+      common /parcom/ nltype,realov,harrison
+      logical jsover
+      common /lia/ npts,jsover
+      ierr = 0
+      fn = ' '
+      do i = 1,min(nfin,255)
+         fn(i:i) = char(fin(i))
+      end do
+      open(15,file='/dev/null',status='unknown')
+      open(10,file=fn,status='old',blank='zero')
+      call input(nparam,jspins)
+      close(10)
+      close(15)
+      npo = nparam
+      jspo = jspins
+      jsovo = 0
+      if(jsover) jsovo = 1
+      nlto = nltype
+      kindo = kinds
+      do i = 1,kinds
+         kbo(i) = kbas(i)
+         valo(i) = valence(i)
+      end do
+      do i = 1,kinds*(kinds+1)/2
+         dnno(i) = dnn(i)
+         scro(i) = screenl(i)
+      end do
+      do j = 1,mspin
+         do i = 1,npard
+            paro(i,j) = param(i,j)
+         end do
+      end do
+      return
+      end
+c
+c     ref_input1: the reference's structure reader on unit 10 = the file
+c     fin (a SKIN structure block from its lattice line on), for kinds
+c     with kbin orbitals and valin electrons: setlat and input1, then the
+c     conversion of the positions to Cartesian coordinates the main
+c     program makes before search2 (static.f:1088-1103).  Returns
+c     /latt1/, the volume, the atoms (/codes/, /fazon/), the search
+c     widths, zfill, the spin flags and directions, and inpstat.
+      subroutine ref_input1(fin,nfin,nkin,kbin,valin,zch,plvo,volo,
+     $     nato,ndo,kko,posno,posqo,zfo,iflo,lnco,spo,inpo)
+     $     bind(C,name='ref_input1')
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      integer nfin,nkin,nato,inpo,lnco
+      integer*1 fin(nfin)
+      integer kbin(nkin),ndo(3),kko(matom),iflo(matom)
+      real*8 valin(nkin),zch,plvo(3,3),volo,zfo
+      real*8 posno(3,matom),posqo(3,matom),spo(3,matom)
+      character*255 fn
+      common /codes/posn(matom,3),kkind(matom),natoms
+      common /latt1/plv(3,3)
+      common /parinfo/ valence(mkind),kinds,kbas(mkind)
+      common /fazon/posq(matom,3)
+      integer nkdarry(6)
+      common/disply/ nkdarry
+      logical flipos(matom),lnoncol,ltorque,lspdir
+      real*8 spinat(3,matom),dspth(3,matom),dspphi(3,matom),temp(3)
+      kinds = nkin
+      do i = 1,nkin
+         kbas(i) = kbin(i)
+         valence(i) = valin(i)
+      end do
+      fn = ' '
+      do i = 1,min(nfin,255)
+         fn(i:i) = char(fin(i))
+      end do
+      open(15,file='/dev/null',status='unknown')
+      open(10,file=fn,status='old',blank='zero')
+      call input1(zch,zfill,thisvol,inpstat,flipos,
+     $     lnoncol,ltorque,lspdir,spinat,dspth,dspphi)
+      close(10)
+      close(15)
+      inpo = inpstat
+      if(inpstat.ne.0) return
+      do iat = 1,natoms
+         do i = 1,3
+            tempi = 0d0
+            do j = 1,3
+               tempi = tempi+plv(j,i)*posn(iat,j)
+            end do
+            temp(i) = tempi
+         end do
+         do i = 1,3
+            posn(iat,i) = temp(i)
+         end do
+      end do
+      do i = 1,3
+         do j = 1,3
+            plvo(j,i) = plv(i,j)
+         end do
+         ndo(i) = nkdarry(i)
+      end do
+      volo = thisvol
+      nato = natoms
+      zfo = zfill
+      lnco = 0
+      if(lnoncol) lnco = 1
+      do ia = 1,natoms
+         kko(ia) = kkind(ia)
+         iflo(ia) = 0
+         if(flipos(ia)) iflo(ia) = 1
+         do j = 1,3
+            posno(j,ia) = posn(ia,j)
+            posqo(j,ia) = posq(ia,j)
+            spo(j,ia) = spinat(j,ia)
+         end do
+      end do
+      return
+      end

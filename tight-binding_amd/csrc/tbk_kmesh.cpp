@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "tbk.h"
+#include "tbk_fortran_io.h"
 
 namespace {
 
@@ -139,85 +140,6 @@ void heap_index_sort(int n, const std::vector<double> &key, std::vector<int> &in
     }
 }
 
-// Fortran formatted input of fixed-width fields (the F, D, I and A edit
-// descriptors rdspgp.f:83-90 uses), blanks ignored, records padded with
-// blanks, the format re-used from its start on a new record when a list is
-// longer than it.
-class FixedReader {
-public:
-    explicit FixedReader(FILE *f) : f_(f) {}
-    bool next_record()
-    {
-        line_.clear();
-        pos_ = 0;
-        int ch;
-        bool any = false;
-        while ((ch = std::fgetc(f_)) != EOF) {
-            any = true;
-            if (ch == '\n') break;
-            if (ch != '\r') line_.push_back((char)ch);
-        }
-        return any;
-    }
-    std::string field(int w)
-    {
-        std::string s;
-        for (int i = 0; i < w; ++i, ++pos_) s.push_back(pos_ < line_.size() ? line_[pos_] : ' ');
-        return s;
-    }
-    void skip(int w) { pos_ += (size_t)w; }
-    // Fw.d / Dw.d: a field without a decimal point carries d implied decimals
-    bool real(int w, int d, double *out)
-    {
-        std::string s;
-        for (char c : field(w))
-            if (c != ' ' && c != '\t') s.push_back(c == 'd' || c == 'D' ? 'E' : c);
-        if (s.empty()) { *out = 0.0; return true; }
-        const bool has_point = s.find('.') != std::string::npos;
-        size_t e = s.find_first_of("Ee");
-        if (e == std::string::npos) {   // exponent written as a bare sign: 1.0+05
-            for (size_t i = 1; i < s.size(); ++i)
-                if ((s[i] == '+' || s[i] == '-') && std::isdigit((unsigned char)s[i - 1])) {
-                    s.insert(i, "E");
-                    e = i;
-                    break;
-                }
-        }
-        if (!has_point && d > 0) {
-            std::string mant = e == std::string::npos ? s : s.substr(0, e);
-            long ex = e == std::string::npos ? 0 : std::strtol(s.c_str() + e + 1, nullptr, 10);
-            s = mant + "E" + std::to_string(ex - d);
-        }
-        char *end = nullptr;
-        *out = std::strtod(s.c_str(), &end);
-        return end && *end == '\0';
-    }
-    bool integer(int w, int *out)
-    {
-        std::string s;
-        for (char c : field(w))
-            if (c != ' ' && c != '\t') s.push_back(c);
-        if (s.empty()) { *out = 0; return true; }
-        char *end = nullptr;
-        const long v = std::strtol(s.c_str(), &end, 10);
-        *out = (int)v;
-        return end && *end == '\0';
-    }
-    // a list of n reals under a format of per_rec fields Fw.d per record
-    bool real_list(int n, int per_rec, int w, int d, double *out)
-    {
-        for (int i = 0; i < n; ++i) {
-            if (i % per_rec == 0 && !next_record()) return false;
-            if (!real(w, d, out + i)) return false;
-        }
-        return true;
-    }
-
-private:
-    FILE *f_;
-    std::string line_;
-    size_t pos_ = 0;
-};
 
 }  // namespace
 
@@ -336,7 +258,7 @@ int tbk_space_group_read(const char *path, int ilat, int lattic, const double *a
     if (!path || !avec || !bvec || !nop || !rspc || !rspct || !tnpspc || maxop < 1) return TBK_ERR_ARG;
     FILE *f = std::fopen(path, "r");
     if (!f) return TBK_ERR_ARG;
-    FixedReader rd(f);
+    tbk::FortranReader rd(f);        // rdspgp.f:83-90 (unit 25, blanks ignored)
     int rc = TBK_OK, n = 0;
     double shift[3] = {0.0, 0.0, 0.0};
     std::vector<double> rot, tnp;

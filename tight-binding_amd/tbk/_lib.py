@@ -17,6 +17,7 @@ TBK_OK, TBK_ERR_ARG, TBK_ERR_PAIRS, TBK_ERR_OVERLAP = 0, 1, 2, 3
 TBK_ERR_CONVERGE, TBK_ERR_ZFILL, TBK_ERR_DEGEN = 4, 5, 6
 TBK_ERR_HIP, TBK_ERR_NOMEM, TBK_ERR_NODEVICE = 10, 11, 12
 TBK_FP64, TBK_MIXED = 0, 1
+TBK_END = -1
 
 # Every symbol include/tbk.h declares (checked by the CPU test suite).
 EXPORTS = [
@@ -36,6 +37,7 @@ EXPORTS = [
     "tbk_server_create", "tbk_server_submit", "tbk_server_result", "tbk_server_release",
     "tbk_server_stats", "tbk_server_destroy",
     "tbk_zhegv", "tbk_zhegv_noalloc", "tbk_fermi",
+    "tbk_read_params", "tbk_skin_open", "tbk_skin_next", "tbk_skin_close", "tbk_skeng_line", "tbk_static_run",
     "tbk_error_string", "tbk_debug_stages", "tbk_debug_zgemm", "tbk_debug_copy",
     "tbk_debug_gemm_time",
 ]
@@ -56,6 +58,29 @@ class StructureDesc(C.Structure):
         ("dnn", C.POINTER(C.c_double)), ("screenl", C.POINTER(C.c_double)),
         ("param", C.POINTER(C.c_double)), ("nparam", C.c_int), ("nltype", C.c_int),
         ("jsover", C.c_int), ("thisvol", C.c_double), ("zfill", C.c_double),
+    ]
+
+
+TBK_MAXKINDS = 8
+_MINT = TBK_MAXKINDS * (TBK_MAXKINDS + 1) // 2
+
+
+class ParamSet(C.Structure):
+    """tbk_param_set: one NRL parameter file's header (input.f:181-359)."""
+    _fields_ = [
+        ("jsover", C.c_int), ("jspins", C.c_int), ("nltype", C.c_int), ("kinds", C.c_int),
+        ("nparam", C.c_int), ("kbas", C.c_int * TBK_MAXKINDS), ("valence", C.c_double * TBK_MAXKINDS),
+        ("dnn", C.c_double * _MINT), ("screenl", C.c_double * _MINT), ("title", C.c_char * 80),
+    ]
+
+
+class SkinStructure(C.Structure):
+    """tbk_skin_structure: what the reference program reads of a structure."""
+    _fields_ = [
+        ("title", C.c_char * 72), ("label", C.c_char * 24), ("zcharge", C.c_double), ("zfill", C.c_double),
+        ("thisvol", C.c_double), ("lat", C.c_double * 9), ("latnum", C.c_int), ("strain", C.c_int),
+        ("natoms", C.c_int), ("nd", C.c_int * 3), ("lphonon", C.c_int), ("spinflip", C.c_int),
+        ("lnoncol", C.c_int), ("ltorque", C.c_int), ("lspdir", C.c_int), ("nk", C.c_int), ("inpstat", C.c_int),
     ]
 
 
@@ -156,6 +181,14 @@ def lib():
                                 C.c_double, dp]
         L.tbk_debug_stages.argtypes = [C.c_int, C.c_int] + [vp] * 10
         L.tbk_debug_zgemm.argtypes = [C.c_int] * 9 + [vp, C.c_int, vp, C.c_int, vp, C.c_int]
+        L.tbk_read_params.argtypes = [C.c_char_p, C.POINTER(ParamSet), C.c_int, d]
+        L.tbk_skin_open.argtypes = [C.c_char_p, C.POINTER(vp), ip, dp, dp, C.c_char_p, C.c_int]
+        L.tbk_skin_next.argtypes = [vp, C.POINTER(ParamSet), C.c_int, C.c_int, C.POINTER(SkinStructure), i, d, d,
+                                    i, d, d, d]
+        L.tbk_skin_close.argtypes = [vp]
+        L.tbk_skeng_line.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_int, C.c_int, C.c_char_p, C.c_int]
+        L.tbk_static_run.argtypes = [C.c_char_p, C.c_int, C.c_int, ip]
         L.tbk_error_string.argtypes = [C.c_int]
         L.tbk_error_string.restype = C.c_char_p
         _lib = L

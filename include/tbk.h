@@ -349,8 +349,11 @@ int tbk_plan_magnetization(const tbk_plan *plan, double *smag);
 
 /* Run the whole bande k-loop + Fermi + sums on the device.  Replaces
  * bande(mode,...,efermi,earg1,earg2,...,iberr), bande.f:1, for the
- * collinear case (spin-polarised through tbk_plan_create_spin).  Blocks
- * until results are on the host. */
+ * collinear case (spin-polarised through tbk_plan_create_spin).  Modes 1-4,
+ * and the energy-only modes whose sums are mode 3's and which write QLMT
+ * (6, 9, 10: with 6 and 10 the run also forms the Lowdin projections,
+ * tbk_plan_projections) or QAPW (8) files; mode 5 is refused (the
+ * reference's program aborts on it).  Blocks until results are on the host. */
 int tbk_plan_run(tbk_plan *plan, int mode, double tkb, double eigcut,
                  double *efermi, double *earg1, double *earg2);
 
@@ -386,6 +389,16 @@ int tbk_plan_structure_eigenvalues(tbk_plan *plan, int s, double *esk,
  * occupation window (tbk_plan_set_window) the columns of states at or above
  * the window are zero (all columns after an energy-only run). */
 int tbk_plan_eigenvectors(tbk_plan *plan, int k, double *z);
+
+/* Lowdin projections of item k's states (diag.f:526-758) after a run in
+ * mode 6 or 10 (QLMT): the states with e <= eigcut (*ntot of them, the
+ * reference's ntot, diag.f:635-641), their eigenvalues e[j] and weights
+ * proj[j*nv + i] = |(S^{1/2} z_j)_i|^2 on the item's nv orbitals (atoms in
+ * order, kbas orbitals each; for a non-collinear item the spin-down block
+ * follows the spin-up one).  S^{1/2} is formed on the device from S's own
+ * eigendecomposition; sum_i proj = 1.  e [nv], proj [nv*nv]. */
+int tbk_plan_projections(tbk_plan *plan, int k, double eigcut, int *ntot,
+                         double *e, double *proj);
 
 /* Occupation window (no reference counterpart; on by default).  The Fermi
  * sums skip every state at or above E_F + 20 kT (bande.f:1170-1321); when a

@@ -1,11 +1,13 @@
 """Writes tests/golden/static.json: the SKENG lines the reference program
 itself writes on the end-to-end cases of tests/skin_cases.py (ex1 as
 published and in modes 1-3, ex2 with its mesh line aligned and as
-published, a SKIN walking every lattice menu and strain option, and ex1
-with a two-channel parameter file, spin-flipped and non-collinear), for the
-test of the native program tbk_static on the GPU.
+published, a SKIN walking every lattice menu and strain option, ex1 with a
+two-channel parameter file, spin-flipped and non-collinear, and the QLMT /
+QAPW modes 6, 8, 9, 10 on ex1, ex2 and the spin inputs), for the test of the
+native program tbk_static on the GPU.
 
-Data only: the SKENG lines and the program's exit status from
+Data only: the SKENG lines, the QLMT or QAPW file text and the program's
+exit status from
 oracle/_ref/static_ref (the reference's sources compiled where they lie by
 oracle/Makefile's dropin target), run on the CPU in a scratch directory.
 Run where the reference is built:
@@ -35,7 +37,12 @@ def main():
             sk = os.path.join(tmp, "SKENG")
             lines = [l.rstrip("\n") for l in open(sk)] if os.path.exists(sk) else []
             stops = [l.strip() for l in (p.stdout + p.stderr).splitlines() if "STOP" in l or "stop" in l.lower()]
+            qname = SC.QFILE.get(name)
+            qtext = open(os.path.join(tmp, qname)).read() if qname else None
         out[name] = dict(skeng=lines, returncode=p.returncode, stop=stops[-1] if stops else "")
+        if qtext is not None:
+            out[name]["qfile"] = qname
+            out[name]["qtext"] = qtext
         print(f"{name}: {len(lines)} SKENG lines, exit {p.returncode} {out[name]['stop']}", flush=True)
     path = os.path.join(HERE, "static.json")
     json.dump(out, open(path, "w"), indent=1)

@@ -157,15 +157,35 @@ def kpoint_blocks(group_fcc, kfile):
 # native program (tbk_static) on the GPU
 # ---------------------------------------------------------------------------
 STATIC_CASES = ("ex1", "ex1_mode1", "ex1_mode2", "ex1_mode3", "ex2_aligned", "ex2_published", "lattices",
-                "spin_flip", "noncol")
+                "spin_flip", "noncol", "ex1_qlmt6", "ex1_qlmt10", "ex1_qlmt9", "ex1_qapw8", "ex2_qlmt6",
+                "ex2_qlmt10", "spin_qlmt6", "noncol_qlmt6")
+# the QLMT / QAPW file each case writes (modes 6, 9, 10 / 8, static.f:757-778)
+QFILE = {"ex1_qlmt6": "QLMT", "ex1_qlmt10": "QLMT", "ex1_qlmt9": "QLMT", "ex1_qapw8": "QAPW",
+         "ex2_qlmt6": "QLMT", "ex2_qlmt10": "QLMT", "spin_qlmt6": "QLMT", "noncol_qlmt6": "QLMT"}
 
 
-def _first_structures(text, n):
-    """ex1's SKIN cut after its n-th structure (each starts with its title
-    line 'Palladium ...')."""
+def _first_structures(text, n, title="Palladium"):
+    """A SKIN cut after its n-th structure (each starts with its title line,
+    'Palladium ...' in ex1, 'Titanium ...' in ex2)."""
     lines = text.splitlines()
-    starts = [i for i, l in enumerate(lines) if l.startswith("Palladium")]
+    starts = [i for i, l in enumerate(lines) if l.startswith(title)]
     return "\n".join(lines[:starts[n]] if n < len(starts) else lines) + "\n"
+
+
+def _mesh4(text):
+    """ex1's first k-point block with the k-point file (-1313, fcc.08)
+    replaced by a generated uniform 4x4x4 mesh (kptin.f:276-351)."""
+    lines = text.splitlines()
+    i = next(j for j, l in enumerate(lines) if l.startswith("-1313"))
+    lines[i:i + 2] = ["    0  1.000000     (generate)", "lspec=f,linfok=f", "    4    4    4    0    0"]
+    return "\n".join(lines) + "\n"
+
+
+def _mode(text, mode):
+    """The SKIN's mode line set to `mode` (static.f:706, read as (5x,a2))."""
+    lines = text.splitlines()
+    lines[0] = f"Mode={mode:<2d}" + lines[0][7:]
+    return "\n".join(lines) + "\n"
 
 
 def build_static_case(name, work, ex1_dir, ex2_dir):
@@ -181,7 +201,11 @@ def build_static_case(name, work, ex1_dir, ex2_dir):
             shutil.copy(os.path.join(ex2_dir, f), os.path.join(work, f))
         text = open(os.path.join(ex2_dir, "SKIN")).read()
         os.remove(os.path.join(work, "SKIN"))
-        open(os.path.join(work, "SKIN"), "w").write(ex2_case.skin_text(text, aligned=name == "ex2_aligned"))
+        text = ex2_case.skin_text(text, aligned=name != "ex2_published")
+        if "qlmt" in name:           # the first structure on a 4x4x4 mesh, mode 6 or 10
+            text = _mode(_first_structures(text.replace(ex2_case.MESH_ALIGNED, "    4    4    4"), 1, "Titanium"),
+                         int(name.split("qlmt")[1]))
+        open(os.path.join(work, "SKIN"), "w").write(text)
         return
     for f in ("pd_par", "fcc.08", "bcc.08", "spcgrp.fcc", "spcgrp.bcc"):
         shutil.copy(os.path.join(ex1_dir, f), os.path.join(work, f))
@@ -190,6 +214,8 @@ def build_static_case(name, work, ex1_dir, ex2_dir):
         out = skin
     elif name.startswith("ex1_mode"):
         out = _first_structures(skin, 3).replace("Mode=4", "Mode=" + name[-1], 1)
+    elif name.startswith("ex1_q"):
+        out = _mode(_mesh4(_first_structures(skin, 1)), int(name[len(name.rstrip("0123456789")):]))
     elif name == "lattices":
         write_trivial_group(os.path.join(work, "trivial.grp"))
         write_skin(os.path.join(work, "SKIN"), "pd_par", lattice_structures())
@@ -199,5 +225,9 @@ def build_static_case(name, work, ex1_dir, ex2_dir):
         pdn = spin_case.spin_down_values(spin_case.read_channel(par))
         os.remove(os.path.join(work, "pd_par"))
         open(os.path.join(work, "pd_par"), "w").write(spin_case.nm_text(par, pdn))
-        out = _first_structures(spin_case.skin_spin(skin, "flip" if name == "spin_flip" else "noncol"), 3)
+        kind = "flip" if name.startswith("spin") else "noncol"
+        if name.endswith("qlmt6"):
+            out = _mode(_mesh4(_first_structures(spin_case.skin_spin(skin, kind), 1)), 6)
+        else:
+            out = _first_structures(spin_case.skin_spin(skin, kind), 3)
     open(os.path.join(work, "SKIN"), "w").write(out)

@@ -4,7 +4,9 @@ the same input files (tests/golden/static.json from
 tests/golden/make_static_fixture.py): ex1 as published and in modes 1-3, ex2
 with its mesh line aligned, ex2 as published (where both stop in kptgen), a
 SKIN walking every lattice menu and strain option of setlat, and ex1 with a
-two-channel parameter file, spin-flipped and non-collinear.
+two-channel parameter file, spin-flipped and non-collinear; and the QLMT /
+QAPW files of modes 6, 8, 9 and 10 (Lowdin projections on the device,
+diag.f:526-758) on ex1, ex2 and the spin inputs.
 
 A line matches when its label and volume fields are the same text and every
 number agrees within one unit of its last printed digit (the FP64 device
@@ -15,6 +17,7 @@ import json
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 import skin_cases as SC
@@ -43,6 +46,78 @@ def same_line(a, b):
     return True
 
 
+def _is_level(line):
+    t = line.split()
+    return len(t) == 2 and "E+" in t[1]
+
+
+def _nums(line):
+    return [float(x.replace("D", "E")) for x in line.split()]
+
+
+def _close(u, v, text_v):
+    """u within one unit of the last printed digit of the reference value v."""
+    frac = text_v.split(".")[1] if "." in text_v else ""
+    digits = len(frac.split("E")[0].split("D")[0])
+    return abs(u - v) <= 1.01 * 10.0 ** -digits + 1e-15
+
+
+def _qapw_fields(line):
+    """The fixed-width fields of a QAPW row: three k components (f9.6) on a
+    record's first row, then levels in 9 columns from column 39."""
+    f = [] if line[:38].strip() == "" else [line[1:10], line[10:19], line[19:28]]
+    rest = line[38:]
+    f += [rest[j:j + 9] for j in range(0, len(rest), 9) if rest[j:j + 9].strip()]
+    return f
+
+
+def same_qfile(mine, ref):
+    """QLMT / QAPW text of the native program against the reference's:
+    header and k-point record lines verbatim; levels and QAPW values within
+    one unit of their last digit; Lowdin weights summed over each group of
+    degenerate levels (within such a group the individual weights depend on
+    the basis the eigensolver picks inside the subspace), within that
+    rounding times the group size."""
+    a, b = mine.splitlines(), ref.splitlines()
+    assert len(a) == len(b), (len(a), len(b))
+    i = 0
+    while i < len(b):
+        if "SPINS" in b[i] or "K-POINTS" in b[i] or b[i].endswith(" K"):
+            assert a[i] == b[i], (a[i], b[i])
+            i += 1
+            continue
+        if not _is_level(b[i]):                  # QAPW rows (1x,3f9.6,10x,4f9.5/(38x,4f9.5))
+            if a[i] != b[i]:
+                ta, tb = _qapw_fields(a[i]), _qapw_fields(b[i])
+                assert len(ta) == len(tb), (a[i], b[i])
+                for u, v in zip(ta, tb):
+                    assert _close(float(u), float(v), v.strip()), (a[i], b[i])
+            i += 1
+            continue
+        # a run of states: level line + its weight lines, until the next k record
+        states_a, states_b = [], []
+        while i < len(b) and _is_level(b[i]):
+            la, lb = _nums(a[i])[0], _nums(b[i])[0]
+            assert _close(la, lb, b[i].split()[0]), (a[i], b[i])
+            j = i + 1
+            while j < len(b) and not _is_level(b[j]) and not b[j].endswith(" K") and "SPINS" not in b[j]:
+                j += 1
+            states_a.append((la, [_nums(x) for x in a[i + 1:j]]))
+            states_b.append((lb, [_nums(x) for x in b[i + 1:j]]))
+            i = j
+        g0 = 0
+        while g0 < len(states_b):
+            g1 = g0 + 1
+            while g1 < len(states_b) and abs(states_b[g1][0] - states_b[g0][0]) < 2e-7:
+                g1 += 1
+            if states_b[g0][1]:                  # weight lines (modes 6, 10; none in mode 9)
+                sa = np.sum([np.array(w) for _, w in states_a[g0:g1]], axis=0)
+                sb = np.sum([np.array(w) for _, w in states_b[g0:g1]], axis=0)
+                assert np.abs(sa - sb).max() <= (g1 - g0) * 1.01e-9 + 1e-13, (g0, g1, sa, sb)
+            g0 = g1
+    return True
+
+
 @pytest.mark.parametrize("name", SC.STATIC_CASES)
 def test_native_program_writes_the_reference_skeng(tmp_path, name):
     if not (os.path.isdir(EX1) and os.path.isdir(EX2)):
@@ -64,3 +139,7 @@ def test_native_program_writes_the_reference_skeng(tmp_path, name):
     print(f"  {same} of {len(got)} lines identical", flush=True)
     for a, b in zip(got, want["skeng"]):
         assert same_line(a, b), (a, b)
+    if "qtext" in want:
+        mine = (tmp_path / want["qfile"]).read_text()
+        print(f"  {want['qfile']}: {len(mine.splitlines())} lines, {sum(x == y for x, y in zip(mine.splitlines(), want['qtext'].splitlines()))} identical", flush=True)
+        assert same_qfile(mine, want["qtext"])

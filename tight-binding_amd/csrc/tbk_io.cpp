@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -733,6 +734,63 @@ std::string fmt_f(double x, int w, int d)
     if ((int)s.size() > w) s.assign(w, '*');
     return s;
 }
+// Fortran Ew.d / Dw.d output: 0.d1d2...dd followed by E+xx (or D+xx)
+std::string fmt_e(double x, int w, int d, char letter = 'E')
+{
+    std::string m;
+    int ex = 0;
+    if (x == 0.0) {
+        m = "0." + std::string(d, '0');
+    } else {
+        char buf[64];
+        std::snprintf(buf, sizeof(buf), "%.*e", d - 1, std::fabs(x));   // d.ddde+xx: the same d digits
+        std::string t(buf);
+        const size_t epos = t.find('e');
+        std::string digits;
+        for (size_t i = 0; i < epos; ++i)
+            if (t[i] != '.') digits.push_back(t[i]);
+        ex = std::atoi(t.c_str() + epos + 1) + 1;
+        m = (x < 0 ? "-0." : "0.") + digits;
+    }
+    char eb[16];
+    std::snprintf(eb, sizeof(eb), "%c%c%02d", letter, ex < 0 ? '-' : '+', std::abs(ex));
+    std::string s = m + eb;
+    if ((int)s.size() > w) return std::string(w, '*');
+    return std::string(w - s.size(), ' ') + s;
+}
+std::string fmt_i(long v, int w)
+{
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%*ld", w, v);
+    std::string s(buf);
+    if ((int)s.size() > w) s.assign(w, '*');
+    return s;
+}
+
+// One k-point's record header of a QLMT file, (3f10.6,2X,f10.5,i10,d20.10,' K')
+// (bande.f:1628-1629)
+std::string qlmt_k_line(const double *ak, double wt, int ntot)
+{
+    return fmt_f(ak[0], 10, 6) + fmt_f(ak[1], 10, 6) + fmt_f(ak[2], 10, 6) + "  " + fmt_f(wt, 10, 5) +
+           fmt_i(ntot, 10) + fmt_e(wt, 20, 10, 'D') + " K";
+}
+
+// A QAPW record, (1x,3f9.6,10x,4f9.5/(38x,4f9.5)) (bande.f:1690-1691): with
+// exactly four levels the slash still ends the record and an empty one
+// follows, as the Fortran runtime writes it.
+std::string qapw_record(const double *ak, const double *e, int n)
+{
+    std::string out = " " + fmt_f(ak[0], 9, 6) + fmt_f(ak[1], 9, 6) + fmt_f(ak[2], 9, 6) + std::string(10, ' ');
+    for (int i = 0; i < n && i < 4; ++i) out += fmt_f(e[i], 9, 5);
+    out += "\n";
+    if (n == 4) out += "\n";
+    for (int i = 4; i < n; i += 4) {
+        out += std::string(38, ' ');
+        for (int j = i; j < n && j < i + 4; ++j) out += fmt_f(e[j], 9, 5);
+        out += "\n";
+    }
+    return out;
+}
 }  // namespace
 
 extern "C" int tbk_skeng_line(const char *label, double thisvol, double efermi, double earg1, double earg2,
@@ -749,6 +807,68 @@ extern "C" int tbk_skeng_line(const char *label, double thisvol, double efermi, 
     std::memcpy(buf, s.c_str(), s.size() + 1);
     return TBK_OK;
 }
+
+// ---------------------------------------------------------------------------
+// QLMT / QAPW records of one structure after its run (bande.f:1568-1703, as
+// diag.f:635-758 forms them): for each spin channel and k-point, modes 6 and
+// 10 the states up to eigcut with the Lowdin weights per atom (s, p, d sums;
+// mode 10 every orbital), mode 9 every eigenvalue, mode 8 (QAPW) every
+// eigenvalue in its 4-per-line layout.
+// ---------------------------------------------------------------------------
+namespace {
+int write_qlmt(FILE *qf, tbk_plan *p, int mode, double eigcut, const tbk_skin_structure &st, const int *kkind,
+               const tbk_param_set &ps, bool noncol, int natx, const double *kp, const double *wt)
+{
+    const int nv = tbk_plan_nv(p), nk = st.nk, nstr = tbk_plan_nstruct(p);
+    std::vector<double> esk((size_t)nk * nv), desk((size_t)nk * nv), e(nv), proj((size_t)nv * nv);
+    for (int s = 0; s < nstr; ++s) {               // spin channels (jsx, bande.f:711-721)
+        int rc = tbk_plan_structure_eigenvalues(p, s, esk.data(), desk.data());
+        if (rc) return rc;
+        for (int k = 0; k < nk; ++k) {
+            const double *ak = kp + 3 * (size_t)k;
+            const double *ek = esk.data() + (size_t)k * nv;
+            if (mode == 8) {
+                std::fputs(qapw_record(ak, ek, nv).c_str(), qf);
+                continue;
+            }
+            if (mode == 9) {
+                std::fprintf(qf, "%s\n", qlmt_k_line(ak, wt[k], nv).c_str());
+                for (int i = 0; i < nv; ++i)
+                    std::fprintf(qf, "%s%s\n", fmt_f(ek[i], 12, 7).c_str(), fmt_e(0.0, 15, 6).c_str());
+                continue;
+            }
+            int ntot = 0;
+            rc = tbk_plan_projections(p, s * nk + k, eigcut, &ntot, e.data(), proj.data());
+            if (rc) return rc;
+            std::fprintf(qf, "%s\n", qlmt_k_line(ak, wt[k], ntot).c_str());
+            for (int j = 0; j < ntot; ++j) {
+                const double en = e[j];
+                const bool wide = mode == 6 && (en <= -10.0 || en >= 100.0);
+                std::fprintf(qf, "%s%s\n", fmt_f(en, 10, wide ? 6 : 7).c_str(), fmt_e(0.0, 15, 6).c_str());
+                int i = 0;
+                for (int a = 0; a < natx; ++a) {
+                    const int atom = a < st.natoms ? a : a - st.natoms;
+                    const int nbas = ps.kbas[kkind[atom] - 1];
+                    const double *occ = proj.data() + (size_t)j * nv + i;
+                    i += nbas;
+                    std::string line;
+                    if (mode == 10) {
+                        for (int b = 0; b < nbas; ++b) line += fmt_f(occ[b], 15, 9);
+                    } else {
+                        double sv = occ[0], pv = 0.0, dv = 0.0;
+                        for (int b = 1; b < std::min(4, nbas); ++b) pv = pv + occ[b];
+                        for (int b = 4; b < nbas; ++b) dv = dv + occ[b];
+                        line = fmt_f(sv, 15, 9) + fmt_f(pv, 15, 9) + fmt_f(dv, 15, 9) + fmt_f(0.0, 15, 9);
+                    }
+                    std::fprintf(qf, "%s\n", line.c_str());
+                }
+            }
+        }
+    }
+    std::fflush(qf);
+    return TBK_OK;
+}
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // The program's loop over the structures of a SKIN file (static.f:925-1255)
@@ -769,9 +889,17 @@ extern "C" int tbk_static_run(const char *dir, int device, int precision, int *n
     rc = tbk_read_params(ppath, &ps, (int)param.size(), param.data());
     const int npard = rc ? 0 : ps.kinds * (29 + 68 * ps.kinds);
     if (!rc && ps.nparam != npard) rc = TBK_ERR_ARG;            // static.f:864-874
-    if (!rc && (mode < 1 || mode > 4)) rc = TBK_ERR_ARG;        // modes outside this path
+    // modes 1-4 and the QLMT (6, 9, 10) / QAPW (8) modes; 5 aborts in the
+    // reference's own program, 7 it does not implement
+    const bool qmode = mode == 6 || mode == 8 || mode == 9 || mode == 10;
+    if (!rc && !((mode >= 1 && mode <= 4) || qmode)) rc = TBK_ERR_ARG;
     FILE *out = rc ? nullptr : std::fopen((d + "/SKENG").c_str(), "w");
     if (!rc && !out) rc = TBK_ERR_ARG;
+    FILE *qf = nullptr;                                          // static.f:757-778
+    if (!rc && qmode) {
+        qf = std::fopen((d + (mode == 8 ? "/QAPW" : "/QLMT")).c_str(), "w");
+        if (!qf) rc = TBK_ERR_ARG;
+    }
     const int maxatoms = 4096, maxk = kNkd;
     std::vector<int> kkind(maxatoms), flipos(maxatoms);
     std::vector<double> posn(3 * maxatoms), posq(3 * maxatoms), spinat(3 * maxatoms), kp(3 * (size_t)maxk),
@@ -785,6 +913,13 @@ extern "C" int tbk_static_run(const char *dir, int device, int precision, int *n
         bool noncol = st.lnoncol != 0;
         if (noncol && ps.jspins == 1) noncol = false;              // static.f:982-996
         if (noncol && (st.ltorque || st.lspdir)) { rc = TBK_ERR_ARG; break; }   // outputs outside this path
+        const int natx = noncol ? 2 * st.natoms : st.natoms;
+        if (qf && mode != 8) {        // static.f:1127-1137, 1152-1157: the structure's header lines
+            std::fprintf(qf, "%s%s%s%s     SPINS, WIND., SC. WIND.\n", fmt_i(1, 5).c_str(), fmt_i(1, 5).c_str(),
+                         fmt_i(0, 5).c_str(), fmt_i(mode == 9 ? 0 : st.natoms, 5).c_str());
+            std::fprintf(qf, "%s K-POINTS%s%s%s\n", fmt_i(st.nk, 5).c_str(), fmt_i(st.natoms, 6).c_str(),
+                         fmt_f(st.zfill, 10, 5).c_str(), fmt_f(tkb, 10, 5).c_str());
+        }
         tbk_structure_desc sd;
         sd.natoms = st.natoms;
         sd.lat = st.lat; sd.posn = posn.data(); sd.posq = posq.data(); sd.kkind = kkind.data();
@@ -802,9 +937,11 @@ extern "C" int tbk_static_run(const char *dir, int device, int precision, int *n
         else
             rc = tbk_plan_create_spin(device, &sd, param.data() + npard, flipos.data(), st.nk, kp.data(),
                                       wt.data(), 0, precision, &p);
+        if (!rc && (mode == 6 || mode == 10)) rc = tbk_plan_set_window(p, 0);
         if (!rc) rc = tbk_plan_run(p, mode, tkb, eigcut, &ef, &e1, &e2);
         const bool with_smag = ps.jspins == 2 && !noncol;
         if (!rc && with_smag) rc = tbk_plan_magnetization(p, &smag);
+        if (!rc && qf) rc = write_qlmt(qf, p, mode, eigcut, st, kkind.data(), ps, noncol, natx, kp.data(), wt.data());
         if (p) tbk_plan_destroy(p);
         if (rc) break;
         char line[160];
@@ -815,6 +952,7 @@ extern "C" int tbk_static_run(const char *dir, int device, int precision, int *n
         ++*nstruct;
     }
     if (out) std::fclose(out);
+    if (qf) std::fclose(qf);
     tbk_skin_close(skin);
     return rc;
 }

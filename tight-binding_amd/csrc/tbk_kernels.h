@@ -188,6 +188,11 @@ void launch_fermi_sums_group(const FermiArgs *f_dev, int nstruct, hipStream_t st
 // the Fermi sums; no group of close eigenvalues straddles it).
 void launch_fermi_window_group(const FermiArgs *f_dev, int nstruct, hipStream_t st);
 void launch_copy_int(const int *src, int *dst, int n, hipStream_t st);
+// Lowdin projections: S = I on every item; T[i, j] *= sqrt(lam[k][i]);
+// proj[k][e] = |Y[k][e]|^2 over the n x n slabs of nb items
+void launch_set_identity(int n, double2 *s, size_t stride, int nb, hipStream_t st);
+void launch_scale_rows_sqrt(int n, double2 *t, size_t stride, const double *lam, int nb, hipStream_t st);
+void launch_abs2(int n, const double2 *y, size_t stride, double *proj, int nb, hipStream_t st);
 void launch_copy16(const double2 *src, double2 *dst, size_t n, hipStream_t st);
 
 }  // namespace tbk

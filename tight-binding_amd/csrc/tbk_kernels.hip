@@ -2236,6 +2236,41 @@ __global__ void k_copy_int(const int *src, int *dst, int n)
 }
 
 // ---------------------------------------------------------------------------
+// Lowdin projections (diag.f:526-758): with S = V diag(lambda) V^H, the
+// states y_j = S^{1/2} z_j = V diag(lambda^{1/2}) V^H z_j and their weights
+// |y_j(i)|^2 on the orbitals (sum_i = 1 as z^H S z = 1).
+// ---------------------------------------------------------------------------
+// S of every item of the batch set to the identity
+__global__ __launch_bounds__(256) void k_set_identity(int n, double2 *s, size_t stride)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * n) return;
+    const int i = (int)(t % n), j = (int)(t / n);
+    s[(size_t)blockIdx.y * stride + t] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+}
+
+// T[i, j] *= sqrt(lambda_i)  (lambda: the item's eigenvalues of S)
+__global__ __launch_bounds__(256) void k_scale_rows_sqrt(int n, double2 *t, size_t stride, const double *lam)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    const int kb = blockIdx.y, i = (int)(e % n);
+    const double f = sqrt(lam[(size_t)kb * n + i]);
+    double2 &v = t[(size_t)kb * stride + e];
+    v.x *= f;
+    v.y *= f;
+}
+
+// proj[k][j*n + i] = |Y[i, j]|^2 for the items k0 .. k0 + nb - 1
+__global__ __launch_bounds__(256) void k_abs2(int n, const double2 *y, size_t stride, double *proj)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    const double2 v = y[(size_t)blockIdx.y * stride + e];
+    proj[(size_t)blockIdx.y * n * n + e] = v.x * v.x + v.y * v.y;
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 void launch_phases(const StructDev *sd, const int *item_s, const double *kpts, int maxpairs,
@@ -2423,6 +2458,21 @@ void launch_fermi_sums_group(const FermiArgs *f_dev, int nstruct, hipStream_t st
 void launch_copy16(const double2 *src, double2 *dst, size_t n, hipStream_t st)
 {
     hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, st, src, dst, n);
+}
+void launch_set_identity(int n, double2 *s, size_t stride, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_set_identity, dim3((unsigned)(((size_t)n * n + 255) / 256), nb), dim3(256), 0, st, n, s,
+                       stride);
+}
+void launch_scale_rows_sqrt(int n, double2 *t, size_t stride, const double *lam, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_scale_rows_sqrt, dim3((unsigned)(((size_t)n * n + 255) / 256), nb), dim3(256), 0, st, n,
+                       t, stride, lam);
+}
+void launch_abs2(int n, const double2 *y, size_t stride, double *proj, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_abs2, dim3((unsigned)(((size_t)n * n + 255) / 256), nb), dim3(256), 0, st, n, y, stride,
+                       proj);
 }
 void launch_copy_int(const int *src, int *dst, int n, hipStream_t st)
 {

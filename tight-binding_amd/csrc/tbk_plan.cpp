@@ -541,6 +541,11 @@ struct tbk_plan {
     std::vector<int> info_host, err_host, status;
     std::vector<double> out_host;
     Workspace ws;
+    // Lowdin projections (QLMT modes 6, 10): a second workspace for the
+    // eigendecomposition of S, and |(S^{1/2} z_j)_i|^2 of every item,
+    // proj[k][j*N + i]
+    Workspace ws2;
+    double *proj = nullptr;
     StageTimer timer;
     int last_mode = 0;
     size_t bytes = 0;
@@ -567,7 +572,9 @@ static void plan_free(tbk_plan *p)
     dfree(p->esk); dfree(p->desk); dfree(p->out); dfree(p->ncut); dfree(p->err); dfree(p->info_all);
     dfree(p->cut);
     dfree(p->ph);
+    dfree(p->proj);
     p->ws.release();
+    p->ws2.release();
     if (p->stream) {
         (void)hipStreamSynchronize(p->stream);
         (void)hipStreamDestroy(p->stream);
@@ -913,8 +920,11 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
     // eigenvalues are, and eigenvectors and derivatives are formed only for
     // the states the Fermi sums use (below E_F + 20 kT).
     // Energy-only modes (1, 3) need no eigenvectors at all.
+    // QLMT modes (6, 10) need every eigenvector below eigcut and S^{1/2};
+    // 8 and 9 only eigenvalues
+    const bool qproj = mode == 6 || mode == 10;
     const bool win = p->window && lpress && p->batch >= p->nk;
-    const bool vecs = lpress || !p->window;
+    const bool vecs = lpress || qproj || (!p->window && mode <= 4);
     const Window wnd{win ? p->fargs : nullptr, p->nfermi, win ? p->cut : nullptr, vecs};
     p->win_last = win;
     p->vec_last = vecs;
@@ -931,6 +941,25 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
         else
             solve_batch(p->ws, nb, nact, p->esk + (size_t)k0 * n, st, p->timer, wnd);
         tbk::launch_copy_int(p->ws.info, p->info_all + k0, nb, st);
+        if (qproj) {
+            // S(k) again, into ws2.A (H into ws2.S, replaced by I), its
+            // eigenpairs S = V diag(lambda) V^H, then Y = V (lambda^{1/2} (V^H Z))
+            p->timer.begin("projections");
+            Workspace &w2 = p->ws2;
+            tbk::BuildArgs b = build_args(p, k0, false);
+            b.h = w2.S;
+            b.s = w2.A;
+            tbk::launch_build_hs(b, p->maxnat, nb, st);
+            if (p->noncol) tbk::launch_spinor_expand(b, p->maxnorb, nb, st);
+            tbk::launch_set_identity(n, w2.S, ss, nb, st);
+            StageTimer quiet;
+            solve_batch(w2, nb, nact, w2.w, st, quiet, Window{nullptr, 0, nullptr, true});
+            tbk::launch_zgemm(gemm(n, n, n, w2.Z, n, ss, p->ws.Z, n, ss, w2.X, n, ss), 1, 0, nb, st);
+            tbk::launch_scale_rows_sqrt(n, w2.X, ss, w2.w, nb, st);
+            tbk::launch_zgemm(gemm(n, n, n, w2.Z, n, ss, w2.X, n, ss, w2.Y, n, ss), 0, 0, nb, st);
+            tbk::launch_abs2(n, w2.Y, ss, p->proj + (size_t)k0 * n * n, nb, st);
+            p->timer.end();
+        }
         if (lpress) {
             p->timer.begin("pressure");
             tbk::launch_build_hs(build_args(p, k0, true), p->maxnat, nb, st);
@@ -955,12 +984,29 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
     p->timer.end();
 }
 
+// Modes of bande (bande.f:99-123) a plan runs: 1-4, and the energy-only
+// modes that write QLMT (6, 9, 10) or QAPW (8) files, whose sums are those
+// of mode 3.  Mode 5 (the unformatted QLMT) is refused: the reference's own
+// program aborts on it (a formatted write to the unformatted unit 45,
+// bande.f:1609).
+static bool plan_mode_ok(int mode) { return (mode >= 1 && mode <= 4) || mode == 6 || mode == 8 || mode == 9 || mode == 10; }
+
 extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
 {
-    if (!p || mode < 1 || mode > 4) return TBK_ERR_ARG;
+    if (!p || !plan_mode_ok(mode)) return TBK_ERR_ARG;
     for (const StructHost &h : p->sh)
         if (h.zfill <= 0.0) return TBK_ERR_ZFILL;
     HIP_TRY(hipSetDevice(p->device));
+    if ((mode == 6 || mode == 10) && !p->proj) {   // the projection buffers, on first use
+        AllocStream as(p->stream);
+        const size_t before = g_dev_bytes;
+        if (p->ws2.alloc(p->N, p->batch) || dalloc(&p->proj, (size_t)p->nk * p->N * p->N)) {
+            p->ws2.release();
+            dfree(p->proj);
+            return TBK_ERR_NOMEM;
+        }
+        p->bytes += g_dev_bytes - before;
+    }
     p->last_mode = mode;
     const bool params_changed =
         !(p->g_mode == mode && p->g_tkb == tkb && p->g_eigcut == eigcut && p->g_window == p->window);
@@ -970,7 +1016,7 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
         for (int i = 0; i < p->nfermi; ++i) {
             const StructHost &h = p->sh[i];
             tbk::FermiArgs &f = fa[i];
-            f.nk = h.nk; f.nv = h.nv; f.ld = p->N; f.mode = mode;
+            f.nk = h.nk; f.nv = h.nv; f.ld = p->N; f.mode = mode > 4 ? 3 : mode;
             f.el = 2.0; f.tkb = tkb; f.zfill = h.zfill;
             f.esk = p->esk + (size_t)h.k0 * p->N; f.desk = p->desk + (size_t)h.k0 * p->N;
             f.w = p->wts + h.k0; f.ncut = p->ncut + h.k0; f.out = p->out + 8 * (size_t)i;
@@ -1192,6 +1238,28 @@ extern "C" int tbk_plan_eigenvectors(tbk_plan *p, int k, double *z)
         for (size_t t = (size_t)std::max(0, std::min(c, nv)) * nv * 2; t < (size_t)nv * nv * 2; ++t) z[t] = 0.0;
     }
     return rc;
+}
+
+extern "C" int tbk_plan_projections(tbk_plan *p, int k, double eigcut, int *ntot, double *e, double *proj)
+{
+    if (!p || k < 0 || k >= p->nk || !ntot || !e || !proj) return TBK_ERR_ARG;
+    if (!p->proj || !(p->last_mode == 6 || p->last_mode == 10)) return TBK_ERR_ARG;
+    HIP_TRY(hipSetDevice(p->device));
+    const int n = p->N, nv = p->sh[p->item_s_host[k]].nv;
+    std::vector<double> ek(n);
+    HIP_TRY(hipMemcpyAsync(ek.data(), p->esk + (size_t)k * n, n * sizeof(double), hipMemcpyDeviceToHost,
+                           p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    int m = nv;                                   // diag.f:635-641: states up to eigcut
+    for (int j = 0; j < nv; ++j)
+        if (ek[j] > eigcut) { m = j; break; }
+    *ntot = m;
+    for (int j = 0; j < m; ++j) e[j] = ek[j];
+    if (m > 0)
+        HIP_TRY(hipMemcpy2DAsync(proj, nv * sizeof(double), p->proj + (size_t)k * n * n, n * sizeof(double),
+                                 nv * sizeof(double), m, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return TBK_OK;
 }
 
 extern "C" int tbk_plan_set_window(tbk_plan *p, int on)

@@ -26,7 +26,7 @@ EXPORTS = [
     "tbk_space_group_read", "tbk_kptgen", "tbk_kmesh_weights", "tbk_klist", "tbk_kpath",
     "tbk_plan_create", "tbk_plan_run",
     "tbk_plan_launch", "tbk_plan_wait", "tbk_plan_eigenvalues", "tbk_plan_eigenvectors",
-    "tbk_plan_hs", "tbk_plan_nv", "tbk_plan_npairs", "tbk_plan_destroy",
+    "tbk_plan_hs", "tbk_plan_projections", "tbk_plan_nv", "tbk_plan_npairs", "tbk_plan_destroy",
     "tbk_plan_stage_times", "tbk_plan_set_timing", "tbk_plan_autotune", "tbk_gemm_tile",
     "tbk_plan_create_group", "tbk_plan_result", "tbk_plan_structure_eigenvalues",
     "tbk_plan_create_spin", "tbk_plan_magnetization", "tbk_plan_create_noncol",
@@ -170,6 +170,7 @@ def lib():
         L.tbk_plan_eigenvalues.argtypes = [vp, vp, vp]
         L.tbk_plan_eigenvectors.argtypes = [vp, C.c_int, vp]
         L.tbk_plan_hs.argtypes = [vp, C.c_int, C.c_int, vp, vp]
+        L.tbk_plan_projections.argtypes = [vp, C.c_int, C.c_double, ip, vp, vp]
         L.tbk_plan_nv.argtypes = [vp]
         L.tbk_plan_npairs.argtypes = [vp]
         L.tbk_plan_destroy.argtypes = [vp]

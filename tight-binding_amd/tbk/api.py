@@ -173,6 +173,17 @@ class Plan:
         check(lib().tbk_plan_eigenvectors(self.h, int(k), z.ctypes.data), "tbk_plan_eigenvectors")
         return z.reshape(self.nv, self.nv).T.copy()          # [row, col]
 
+    def projections(self, k: int, eigcut: float = 10.0):
+        """Lowdin projections of item k after a run in mode 6 or 10
+        (diag.f:526-758): (levels e <= eigcut, weights [state, orbital] =
+        |(S^{1/2} z)_i|^2)."""
+        n = self.nv
+        e = np.zeros(n); w = np.zeros(n * n); nt = C.c_int()
+        check(lib().tbk_plan_projections(self.h, int(k), float(eigcut), C.byref(nt), e.ctypes.data,
+                                         w.ctypes.data), "tbk_plan_projections")
+        m = nt.value
+        return e[:m].copy(), w[:m * n].reshape(m, n).copy()
+
     def hs(self, k: int, deriv: bool = False):
         n = self.nv
         h = np.zeros(n * n, np.complex128); s = np.zeros(n * n, np.complex128)

@@ -159,29 +159,44 @@ def run_batched(st, kpts, w, prec, steps, warmup, autotune=True):
         tune["seconds"] = time.perf_counter() - t0
     for _ in range(warmup):
         plan.run(4, 0.005, 10.0)
-    plan.set_timing(True)         # stage events over the timed steps only
+    # the timed steps: the captured launch sequence replayed (the capture
+    # itself is one more untimed step)
+    plan.set_graph(True)
+    plan.run(4, 0.005, 10.0)
     t0 = time.perf_counter()
     res = None
     for _ in range(steps):
         res = plan.run(4, 0.005, 10.0)
     t1 = time.perf_counter()
+    plan.set_graph(False)
+    # stage times: the same steps launched eagerly with events between the
+    # stages (which a captured sequence does not carry)
+    plan.set_timing(True)
+    t2 = time.perf_counter()
+    for _ in range(steps):
+        res_eager = plan.run(4, 0.005, 10.0)
+    t3 = time.perf_counter()
     stages = {k: v / steps for k, v in plan.stage_times().items()}
     plan.close()
+    if res_eager != res:
+        raise RuntimeError(f"graph replay and eager launch disagree: {res} vs {res_eager}")
     ms = (t1 - t0) * 1e3 / steps
     return {"ms_per_step": ms, "ksolves_per_s": len(kpts) / (ms / 1e3), "stages_ms_per_step": stages,
+            "eager_timed_ms_per_step": (t3 - t2) * 1e3 / steps,
             "result": res, "autotune": tune, "plan_create_ms": create_ms}
 
 
-def serve_stream(rates, nreq, precision, seed=20261203):
+def serve_stream(rates, nreq, precision, seed=20261203, inflight=8):
     """Poisson streams of E_tot + pressure requests: structures as in the
     uneven batch up to 128 atoms (N <= 1152), Monkhorst-Pack grids up to
-    6^3.  Returns p50/p99 latency and throughput per rate."""
+    6^3.  Per rate: latency percentiles, served rate, the device's busy
+    fraction and the work rate (sum of nv^3 nk per second)."""
     from tbk.api import Server
     pool = S.config_c4(seed=seed, count=256, max_atoms=128, max_q=6, nk_budget=9 * 6 ** 3)
     rng = np.random.default_rng(seed)
     out = []
     for rate in rates:
-        srv = Server(precision=precision, budget_bytes=96 << 30, max_inflight=8)
+        srv = Server(precision=precision, budget_bytes=96 << 30, max_inflight=inflight)
         picks = rng.integers(len(pool), size=nreq)
         gaps = rng.exponential(1.0 / rate, size=nreq)
         reqs = []
@@ -197,20 +212,34 @@ def serve_stream(rates, nreq, precision, seed=20261203):
             while time.perf_counter() < t_next:
                 time.sleep(min(0.001, max(0.0, t_next - time.perf_counter())))
             ids.append(srv.submit(st, k, w))
+        t_sub = time.perf_counter()
         res = [srv.result(r) for r in ids]
         t_end = time.perf_counter()
         stats = srv.stats()
+        tm = srv.timing()
         srv.close()
         lat = np.array([r["latency_ms"] for r in res])
         bad = sum(1 for r in res if r["status"] != 0)
         nk = sum(len(k) for _, k, _ in reqs)
+        p99 = float(np.percentile(lat, 99))
+        span = tm["span_ms"] / 1e3
         out.append({"rate_req_per_s": rate, "requests": nreq, "failed": bad,
-                    "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                    "offered_req_per_s": nreq / (t_sub - t_start),
+                    "p50_ms": float(np.percentile(lat, 50)), "p90_ms": float(np.percentile(lat, 90)),
+                    "p99_ms": p99, "samples_above_p99": int((lat > p99).sum()),
                     "mean_queue_ms": float(np.mean([r["queue_ms"] for r in res])),
                     "req_per_s": nreq / (t_end - t_start), "ksolves_per_s": nk / (t_end - t_start),
+                    "device_busy_frac": tm["busy_ms"] / tm["span_ms"] if tm["span_ms"] > 0 else None,
+                    "work_n3k_per_s": tm["work_n3k"] / span if span > 0 else None,
+                    "work_n3k_per_busy_s": tm["work_n3k"] / (tm["busy_ms"] / 1e3) if tm["busy_ms"] > 0 else None,
+                    "host_setup_ms_per_req": tm["setup_ms"] / nreq,
+                    "launch_ms_per_group": tm["launch_ms"] / max(1, stats["groups"]),
                     "groups": stats["groups"], "largest_group": stats["max_group"]})
-        print(f"# serve {rate:5.0f} req/s: p50 {out[-1]['p50_ms']:.1f} ms p99 {out[-1]['p99_ms']:.1f} ms "
-              f"({stats['groups']} groups, largest {stats['max_group']})", file=sys.stderr, flush=True)
+        o = out[-1]
+        print(f"# serve {rate:5.0f} req/s: served {o['req_per_s']:.1f} req/s, p50 {o['p50_ms']:.1f} ms "
+              f"p99 {o['p99_ms']:.1f} ms, device busy {o['device_busy_frac']:.2f}, "
+              f"{o['work_n3k_per_s'] / 1e9:.1f} G N^3k/s ({stats['groups']} groups, largest {stats['max_group']})",
+              file=sys.stderr, flush=True)
     return out
 
 
@@ -328,17 +357,19 @@ def main():
                     help="precision of the headline value (both are measured)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-rates", default="10,20,50,100,200,300,500")
+    ap.add_argument("--serve-inflight", type=int, default=8, help="in-flight groups (streams) of the server")
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the 108-atom and uneven-batch workloads the default line carries as "
                          "extra keys c3 and c4")
     ap.add_argument("--profile-pass", choices=["fp64", "mixed"], default=None)
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "spin"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "spin", "serve"], default="c2",
                     help="c2: the headline line; c3 (N=972, 64 k), c4 (256 uneven "
-                         "structures) and spin (C2 spin-polarised and non-collinear) print their "
-                         "own lines")
+                         "structures), spin (C2 spin-polarised and non-collinear) and serve (the "
+                         "request stream sweep) print their own lines")
     args = ap.parse_args()
     if args.profile_pass:
         profile_pass(args.profile_pass)
@@ -351,6 +382,10 @@ def main():
         return
     if args.workload == "c4":
         print(json.dumps(bench_c4(args)))
+        return
+    if args.workload == "serve":
+        print(json.dumps({"serving": serve_stream([float(x) for x in args.serve_rates.split(",")],
+                                                  args.serve_requests, TBK_MIXED, inflight=args.serve_inflight)}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -417,7 +452,8 @@ def main():
 
     serving = None
     if not args.no_serve:
-        serving = serve_stream([10, 100, 500], args.serve_requests, TBK_MIXED)
+        serving = serve_stream([float(x) for x in args.serve_rates.split(",")], args.serve_requests,
+                               TBK_MIXED, inflight=args.serve_inflight)
 
     out = {
         "metric": METRIC,

@@ -476,12 +476,13 @@ long long tbk_multi_device_bytes(const tbk_multi *m);
 int tbk_multi_destroy(tbk_multi *m);
 
 /* ---------------------------------------------------------------------
- * Request loop with continuous batching (no reference counterpart).  A
- * worker thread admits queued requests while their device footprint fits
- * in budget_bytes and fewer than max_inflight groups run; requests admitted
- * together are grouped by padded order (granularity) and (mode, tkb,
- * eigcut) into group plans on their own streams, overlapping with the
- * groups already on the device.  Results are each structure's own (the
+ * Request loop with continuous batching (no reference counterpart).  Host
+ * threads build each queued request's set-up (neighbour list, SK blocks) as
+ * it arrives; a worker thread admits prepared requests while their device
+ * footprint fits in budget_bytes and one of max_inflight streams is free;
+ * requests admitted together are grouped by padded order (granularity) and
+ * (mode, tkb, eigcut) into group plans, one per stream, overlapping with
+ * the groups already on the device.  Results are each structure's own (the
  * padding is decoupled).
  * --------------------------------------------------------------------- */
 int tbk_server_create(int device, int precision, long long budget_bytes,
@@ -500,6 +501,11 @@ int tbk_server_release(tbk_server *s, long long id);
 /* completed, k-points done, groups launched, largest group, pending,
  * running groups, in-flight bytes, last error. */
 int tbk_server_stats(tbk_server *s, long long *stats8);
+/* t6: device busy ms (union over finished groups of launch -> completion on
+ * the device), work done (sum of nv^3 nk over completed requests), ms from
+ * the first submission to the last completion, host set-up ms and plan
+ * creation + launch ms (summed), set-up threads. */
+int tbk_server_timing(tbk_server *s, double *t6);
 /* Finishes every queued request, then stops the worker. */
 int tbk_server_destroy(tbk_server *s);
 

@@ -129,14 +129,15 @@ def test_uneven_batch_mixed_vs_fp64(c4_fp64):
 @pytest.fixture(scope="module")
 def served():
     """120 requests drawn from the serving pool (up to 128 atoms, meshes up
-    to 6^3), submitted at Poisson arrival times (100 req/s) to a mixed
+    to 6^3), submitted at Poisson arrival times (400 req/s, above what the
+    server keeps up with, so requests queue and are grouped) to a mixed
     precision server."""
     pool = S.config_c4(seed=SEED, count=256, max_atoms=128, max_q=6, nk_budget=9 * 6 ** 3)
     rng = np.random.default_rng(SEED + 5)
     picks = [int(i) for i in rng.integers(len(pool), size=120)]
     big = max(range(len(pool)), key=lambda i: pool[i][0].nv)
     picks[7] = big                      # the largest order of the pool is in the stream
-    gaps = rng.exponential(1.0 / 100.0, size=len(picks))
+    gaps = rng.exponential(1.0 / 400.0, size=len(picks))
     srv = Server(precision=_lib.TBK_MIXED, budget_bytes=64 << 30, max_inflight=8)
     ids = []
     print(f"case: served stream, {len(picks)} requests, largest N={pool[big][0].nv}", flush=True)

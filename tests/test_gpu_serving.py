@@ -62,7 +62,9 @@ def test_server_failing_request_does_not_affect_others():
 
 def test_server_fp64_matches_oracle_and_reuses_nothing_stale():
     """FP64 requests of repeated and different structures, submitted in two
-    waves through one server: every result equals the oracle's."""
+    waves through one server: every result equals the oracle's, and a
+    structure's result is the same bits in both waves although the groups it
+    joins differ (each group is solved at the padded order of its key)."""
     c4 = S.config_c4(count=32, max_atoms=8, max_q=3)
     reqs = [(st, S.monkhorst_pack((2, 2, 2))) for st, _ in c4[:6]] * 2
     srv = Server(precision=_lib.TBK_FP64, budget_bytes=2 << 30, max_inflight=2)
@@ -79,3 +81,32 @@ def test_server_fp64_matches_oracle_and_reuses_nothing_stale():
         assert abs(r["earg2"] - ref["earg2"]) * S.RYBOHR3_KBAR < 1e-6
     for a, b in zip(res[:6], res[6:]):
         assert a["efermi"] == b["efermi"] and a["earg2"] == b["earg2"]
+
+
+def test_server_timing_and_admission_limits():
+    """One slot and a budget no request fits: groups run one at a time
+    (each admitted only when nothing runs), results agree with a plan's (the
+    server solves at the padded order, a plan at nv: the rounding differs),
+    and the timing counters add up: work is the sum of nv^3 nk, the device
+    intervals lie inside the submission-to-completion span."""
+    c4 = S.config_c4(count=16, max_atoms=6, max_q=3)
+    reqs = [(st, S.monkhorst_pack((2, 2, 2))) for st, _ in c4[:5]]
+    srv = Server(precision=_lib.TBK_FP64, budget_bytes=1, max_inflight=1)
+    ids = [srv.submit(st, k, np.full(len(k), 1.0 / len(k))) for st, k in reqs]
+    res = [srv.result(r) for r in ids]
+    stats, tm = srv.stats(), srv.timing()
+    srv.close()
+    print(f"stats {stats} timing {tm}", flush=True)
+    assert stats["completed"] == len(reqs) and stats["groups"] == len(reqs) and stats["max_group"] == 1
+    assert tm["work_n3k"] == sum(float(st.nv) ** 3 * len(k) for st, k in reqs)
+    assert 0.0 < tm["busy_ms"] <= tm["span_ms"]
+    assert tm["setup_threads"] >= 2 and tm["setup_ms"] > 0.0
+    for (st, k), r in zip(reqs, res):
+        w = np.full(len(k), 1.0 / len(k))
+        p = Plan(st, k, w)
+        a = p.run(4)
+        p.close()
+        assert r["status"] == 0
+        assert abs(r["efermi"] - a["efermi"]) < 1e-12
+        assert abs(r["earg1"] - a["earg1"]) / st.natoms < 1e-12
+        assert abs(r["earg2"] - a["earg2"]) * S.RYBOHR3_KBAR < 1e-8

@@ -2349,14 +2349,16 @@ template <class R, int MAXQ>
 static void launch_hetrd_panel_q(const HetrdArgsT<R> &h, int i0, int nb, hipStream_t st)
 {
     const size_t lds = hetrd_lds_bytes<R, MAXQ>(h.n);
-    if (lds > 65536) {
-        static bool set = false;     // per instantiation; the attribute is per function
-        if (!set) {
+    // once per instantiation (thread-safe initialisation): the dynamic LDS
+    // limit for the largest order it serves (MAXQ column slots of 64 rows)
+    static const bool limit_set = [] {
+        const size_t most = hetrd_lds_bytes<R, MAXQ>(std::min(64 * MAXQ, 1280));
+        if (most > 65536)
             (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_hetrd_panel<R, MAXQ>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            set = true;
-        }
-    }
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)most);
+        return true;
+    }();
+    (void)limit_set;
     hipLaunchKernelGGL((k_hetrd_panel<R, MAXQ>), dim3(nb), dim3(HT), lds, st, h, i0);
 }
 template <class R>

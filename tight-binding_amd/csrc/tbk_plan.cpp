@@ -16,6 +16,7 @@
 #include "tbk.h"
 #include "tbk_internal.h"
 #include "tbk_kernels.h"
+#include "tbk_prepared.h"
 
 namespace {
 
@@ -512,6 +513,7 @@ struct StructHost {
 struct tbk_plan {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool own_stream = true;         // false: borrowed from the caller, not destroyed
     int nstruct = 0, N = 0, nk = 0, batch = 0, precision = TBK_FP64, maxnat = 0, maxpairs = 0;
     std::vector<StructHost> sh;
     std::vector<double> kpts, weights;
@@ -577,7 +579,7 @@ static void plan_free(tbk_plan *p)
     p->ws2.release();
     if (p->stream) {
         (void)hipStreamSynchronize(p->stream);
-        (void)hipStreamDestroy(p->stream);
+        if (p->own_stream) (void)hipStreamDestroy(p->stream);
     }
     delete p;
 }
@@ -712,35 +714,61 @@ static void append_setup(SetupArrays &A, const SetupArrays &L, SetupOffsets &off
     cat(A.edh, L.edh); cat(A.eds, L.eds); cat(A.paros, L.paros); cat(A.dparos, L.dparos);
 }
 
-static int plan_create(int device, int nstruct, const tbk_structure_desc *st, const int *nk, const double *kpts,
-                       const double *weights, int batch, int precision, const SpinSel *sel, tbk_plan **out)
+// Host set-up of one structure, built ahead of the plan that uploads it
+// (the request server prepares structures on host threads while the device
+// works on earlier groups).
+struct tbk_prepared {
+    SetupArrays a;
+    StructHost sh;
+    SetupOffsets off{};
+};
+
+int tbk_prepare(const tbk_structure_desc *st, tbk_prepared **out)
 {
-    if (!st || !out || !nk || nstruct <= 0 || !kpts || !weights) return TBK_ERR_ARG;
+    if (!out) return TBK_ERR_ARG;
+    *out = nullptr;
+    tbk_prepared *p = new tbk_prepared();
+    const int rc = setup_structure(st, p->a, p->sh, p->off);
+    if (rc) { delete p; return rc; }
+    *out = p;
+    return TBK_OK;
+}
+
+void tbk_prepared_free(tbk_prepared *p) { delete p; }
+
+int tbk_prepared_nv(const tbk_prepared *p) { return p ? p->sh.nv : 0; }
+
+size_t tbk_prepared_bytes(const tbk_prepared *p)
+{
+    if (!p) return 0;
+    const SetupArrays &a = p->a;
+    return 4 * (a.row_atom.size() + a.row_orb.size() + a.ksk.size() + a.nbas.size() + a.blk_ptr.size()) +
+           8 * (a.dvec.size() + a.eh.size() + a.es.size() + a.edh.size() + a.eds.size() + a.paros.size() +
+                a.dparos.size() + a.spin.size());
+}
+
+// The device part of plan creation from prepared structures: concatenated
+// set-up arrays, per-item results and the batch workspace.  stream: the
+// plan's stream, borrowed (not destroyed with the plan), or null for a
+// stream of its own.  The matrix order is the largest structure's, raised to
+// min_order where that fits (a fixed order makes each structure's results
+// independent of the others in the plan).
+static int plan_create_from(int device, int nstruct, const tbk_prepared *const *ps, const int *nk,
+                            const double *kpts, const double *weights, int batch, int precision,
+                            hipStream_t stream, int min_order, tbk_plan **out)
+{
+    if (!ps || !out || !nk || nstruct <= 0 || !kpts || !weights) return TBK_ERR_ARG;
     if (precision != TBK_FP64 && precision != TBK_MIXED) return TBK_ERR_ARG;
     *out = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return TBK_ERR_NODEVICE;
-    if (device < 0 || device >= ndev) return TBK_ERR_ARG;
     if (hipSetDevice(device) != hipSuccess) return TBK_ERR_HIP;
-    // ---- host set-up (bit-exact) of every structure: the structures on
-    //      host threads, then their arrays concatenated in order ----
     SetupArrays A;
     std::vector<StructHost> sh(nstruct);
     std::vector<SetupOffsets> off(nstruct);
-    for (int i = 0; i < nstruct; ++i)
-        if (nk[i] <= 0) return TBK_ERR_ARG;
-    {
-        std::vector<SetupArrays> loc(nstruct);
-        std::vector<int> rcs(nstruct, TBK_OK);
-        std::vector<SetupOffsets> unused(nstruct);
-        tbk::parallel_for(nstruct, [&](int i) {
-            rcs[i] = setup_structure(st + i, loc[i], sh[i], unused[i], sel ? sel[i] : SpinSel());
-        });
-        for (int i = 0; i < nstruct; ++i) {
-            if (rcs[i]) return rcs[i];
-            append_setup(A, loc[i], off[i]);
-            off[i].noncol = unused[i].noncol;
-        }
+    for (int i = 0; i < nstruct; ++i) {
+        if (nk[i] <= 0 || !ps[i]) return TBK_ERR_ARG;
+        append_setup(A, ps[i]->a, off[i]);
+        off[i].noncol = ps[i]->off.noncol;
+        sh[i] = ps[i]->sh;
     }
     int nktot = 0, N = 0, maxnat = 0, maxpairs = 1;
     for (int i = 0; i < nstruct; ++i) {
@@ -757,6 +785,7 @@ static int plan_create(int device, int nstruct, const tbk_structure_desc *st, co
         maxnorb = std::max(maxnorb, sh[i].norb);
     }
     if (precision == TBK_MIXED && N > tbk::oa_max_n()) return TBK_ERR_ARG;
+    N = std::max(N, std::min(min_order, precision == TBK_MIXED ? std::min(kMaxN, tbk::oa_max_n()) : kMaxN));
     // ---- device state ----
     tbk_plan *p = new tbk_plan();
     g_dev_bytes = 0;
@@ -770,7 +799,13 @@ static int plan_create(int device, int nstruct, const tbk_structure_desc *st, co
     p->weights.assign(weights, weights + nktot);
     p->batch = (batch <= 0 || batch > nktot) ? nktot : batch;
     // non-blocking: plans neither wait for nor hold up work on the null stream
-    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) { delete p; return TBK_ERR_HIP; }
+    if (stream) {
+        p->stream = stream;
+        p->own_stream = false;
+    } else if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete p;
+        return TBK_ERR_HIP;
+    }
     AllocStream as(p->stream);
     int rc = TBK_OK;
     rc |= upload(&p->row_atom, A.row_atom); rc |= upload(&p->row_orb, A.row_orb);
@@ -824,6 +859,39 @@ static int plan_create(int device, int nstruct, const tbk_structure_desc *st, co
     p->out_host.assign(8 * (size_t)nstruct, 0.0);
     *out = p;
     return TBK_OK;
+}
+
+int tbk_plan_create_prepared(int device, int nstruct, const tbk_prepared *const *ps, const int *nk,
+                             const double *kpts, const double *weights, int precision, hipStream_t stream,
+                             int order, tbk_plan **out)
+{
+    return plan_create_from(device, nstruct, ps, nk, kpts, weights, 0, precision, stream, order, out);
+}
+
+static int plan_create(int device, int nstruct, const tbk_structure_desc *st, const int *nk, const double *kpts,
+                       const double *weights, int batch, int precision, const SpinSel *sel, tbk_plan **out)
+{
+    if (!st || !out || !nk || nstruct <= 0 || !kpts || !weights) return TBK_ERR_ARG;
+    if (precision != TBK_FP64 && precision != TBK_MIXED) return TBK_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return TBK_ERR_NODEVICE;
+    if (device < 0 || device >= ndev) return TBK_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return TBK_ERR_HIP;
+    for (int i = 0; i < nstruct; ++i)
+        if (nk[i] <= 0) return TBK_ERR_ARG;
+    // host set-up (bit-exact) of every structure on host threads
+    std::vector<tbk_prepared> loc(nstruct);
+    std::vector<int> rcs(nstruct, TBK_OK);
+    tbk::parallel_for(nstruct, [&](int i) {
+        rcs[i] = setup_structure(st + i, loc[i].a, loc[i].sh, loc[i].off, sel ? sel[i] : SpinSel());
+    });
+    std::vector<const tbk_prepared *> ps(nstruct);
+    for (int i = 0; i < nstruct; ++i) {
+        if (rcs[i]) return rcs[i];
+        ps[i] = &loc[i];
+    }
+    return plan_create_from(device, nstruct, ps.data(), nk, kpts, weights, batch, precision, nullptr, 0, out);
 }
 
 extern "C" int tbk_plan_create_group(int device, int nstruct, const tbk_structure_desc *st, const int *nk,

@@ -35,7 +35,7 @@ EXPORTS = [
     "tbk_multi_eigenvalues", "tbk_multi_ngroups", "tbk_multi_group_order", "tbk_multi_set_graph",
     "tbk_multi_autotune", "tbk_multi_device_bytes", "tbk_multi_destroy", "tbk_plan_query",
     "tbk_server_create", "tbk_server_submit", "tbk_server_result", "tbk_server_release",
-    "tbk_server_stats", "tbk_server_destroy",
+    "tbk_server_stats", "tbk_server_timing", "tbk_server_destroy",
     "tbk_zhegv", "tbk_zhegv_noalloc", "tbk_fermi",
     "tbk_read_params", "tbk_skin_open", "tbk_skin_next", "tbk_skin_close", "tbk_skeng_line", "tbk_static_run",
     "tbk_error_string", "tbk_debug_stages", "tbk_debug_zgemm", "tbk_debug_copy",
@@ -157,6 +157,7 @@ def lib():
         L.tbk_server_result.argtypes = [vp, C.c_longlong, C.c_int, ip, dp, dp, dp, dp, dp]
         L.tbk_server_release.argtypes = [vp, C.c_longlong]
         L.tbk_server_stats.argtypes = [vp, C.POINTER(C.c_longlong)]
+        L.tbk_server_timing.argtypes = [vp, dp]
         L.tbk_server_destroy.argtypes = [vp]
         L.tbk_debug_copy.argtypes = [C.c_int, C.c_longlong]
         L.tbk_debug_gemm_time.argtypes = [C.c_int] * 11 + [dp]

@@ -478,6 +478,16 @@ class Server:
                 "last_error")
         return dict(zip(keys, [int(x) for x in a]))
 
+    def timing(self):
+        """Device busy ms (union of the groups' device intervals), work done
+        (sum of nv^3 nk), ms from the first submission to the last
+        completion, summed host set-up and plan creation + launch ms, and
+        the set-up threads."""
+        a = (C.c_double * 6)()
+        check(lib().tbk_server_timing(self.h, a), "tbk_server_timing")
+        keys = ("busy_ms", "work_n3k", "span_ms", "setup_ms", "launch_ms", "setup_threads")
+        return dict(zip(keys, [float(x) for x in a]))
+
     def close(self):
         if getattr(self, "h", None):
             lib().tbk_server_destroy(self.h)

@@ -409,6 +409,13 @@ int tbk_plan_projections(tbk_plan *plan, int k, double eigcut, int *ntot,
  * pressures and eigenvalues are the same as without it; on = 0 forms every
  * eigenvector. */
 int tbk_plan_set_window(tbk_plan *plan, int on);
+/* Tridiagonalisation of the plan's eigensolves (no reference counterpart:
+ * both replace ZHPTRD, lapack.f:3218, as diag.f:396-412 calls it):
+ * 1 one-stage blocked Householder panels; 2 two-stage (reduction to a band
+ * of width 16 by panel QR and GEMMs, then bulge chasing to tridiagonal);
+ * 0 the default (two-stage for FP64 plans from order 512).  Results agree
+ * to rounding.  Two-stage needs order >= 40 (TBK_ERR_ARG below). */
+int tbk_plan_set_tridiag(tbk_plan *plan, int stages);
 
 /* H(k), S(k) of k-point k as built on the device (full Hermitian,
  * column-major complex128), for parity checks of the build.  deriv = 1

@@ -55,7 +55,9 @@ struct GemmArgsT {
                                    // 3 B upper triangular; 4 lower triangle of C only;
                                    // 5 lower triangle of C mirrored into the upper
                                    //   (Hermitian C; alpha real, beta 0); 6 = 5 with
-                                   //   A lower triangular as in 2; 7 upper triangle of C only
+                                   //   A lower triangular as in 2; 7 upper triangle of C only;
+                                   //   8 = 5 with beta (C = alpha A B + beta C on the
+                                   //   lower triangle, mirrored)
     const int *ncol = nullptr;     // [nb] per item: column tiles at or beyond ncol[k]
                                    // are skipped (C left as it was); null: all
     // optional copy of every element written to C, in the other precision
@@ -77,6 +79,52 @@ struct HetrdArgsT {
     size_t tstride;                // ceil(n/32) panels of 32x32 WY factors
 };
 typedef HetrdArgsT<double> HetrdArgs;
+
+// Two-stage tridiagonalisation: band width TB; the band is stored by
+// columns, TBLD entries each (diagonals 0 .. 2 TB - 1, room for the fill of
+// the bulge chasing).
+constexpr int TB = 16;
+constexpr int TBLD = 2 * TB;
+constexpr int kMaxBandBlocks = 1280 / TB + 2;     // sweep blocks of the largest order
+
+template <class R>
+struct BandArgsT {                 // stage 1 (full -> band)
+    typedef typename cplx_t<R>::type C;
+    int n;
+    size_t ss;
+    C *a, *vt;                     // [nb][n*n]: matrix (both triangles); V T^H of the panels
+    C *p;                          // [nb][n*96]: columns 0..TB-1 V, TB..2TB-1 Y/Z, 2TB..3TB-1 V, 3TB..4TB-1 V T
+    size_t sp;
+    C *t;                          // [nb][tstride]: TB x TB T factor of each panel
+    size_t st;
+    C *band;                       // [nb][n*TBLD]
+    size_t sb;
+};
+template <class R>
+struct BulgeArgsT {                // stage 2 (band -> tridiagonal)
+    typedef typename cplx_t<R>::type C;
+    int n;
+    C *band;                       // [nb][n*TBLD] (overwritten)
+    size_t sb;
+    C *v2;                         // [nb][diamonds*TB*TB] reflectors; entry 0 of each is its tau
+    size_t sv;
+    double *d, *e;                 // [nb][n] real tridiagonal
+    size_t sde;
+    double2 *ph;                   // [nb][n] phases: eigenvector rows of the real tridiagonal times ph[r]
+    size_t sph;
+};
+template <class R>
+struct Q2ArgsT {                   // back-transform through the stage-2 reflectors
+    typedef typename cplx_t<R>::type C;
+    int n;
+    C *y;                          // [nb][n*n] vectors, Y <- Q2 Y
+    size_t sy;
+    C *v2;                         // as BulgeArgsT
+    size_t sv;
+    C *t2;                         // [nb][diamonds*TB*TB] T factors
+    size_t st2;
+    const int *ncol = nullptr;     // [nb] columns to transform (null: all)
+};
 
 // Scratch of the multi-workgroup tridiagonalisation (per item): v, the
 // column / w vector, the next column's delayed update (carry), tile
@@ -165,11 +213,24 @@ int stein_threads(int n);        // eigenvector threads per k-point
 void launch_stein(const SteinArgs &s, int nb, hipStream_t st);          // eigenvalues and vectors
 void launch_stebz(const SteinArgs &s, int nb, hipStream_t st);          // eigenvalues only
 void launch_stein_vectors(const SteinArgs &s, int nb, hipStream_t st);  // vectors of known eigenvalues
+// ph (may be null): row i of Y times ph[k][i] (phases of the two-stage reduction)
 void launch_real_to_complex_t(int n, const double *vt, double2 *y, size_t stride, int nb,
-                              hipStream_t st);
+                              hipStream_t st, const double2 *ph = nullptr);
 // y64_row0 (may be null): also writes row 0 of Y in FP64 there
 void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int nb, hipStream_t st,
-                          double2 *y64_row0 = nullptr);
+                          double2 *y64_row0 = nullptr, const double2 *ph = nullptr);
+// two-stage tridiagonalisation (panel c0 = panel * TB of stage 1)
+int band_diamonds(int n);        // reflector groups of stage 2 per item
+void launch_band_qr(const BandArgsT<double> &g, int panel, int nb, hipStream_t st);
+void launch_band_qr(const BandArgsT<float> &g, int panel, int nb, hipStream_t st);
+void launch_band_tail(const BandArgsT<double> &g, int c_end, int nb, hipStream_t st);
+void launch_band_tail(const BandArgsT<float> &g, int c_end, int nb, hipStream_t st);
+void launch_bulge(const BulgeArgsT<double> &g, int nb, hipStream_t st);
+void launch_bulge(const BulgeArgsT<float> &g, int nb, hipStream_t st);
+void launch_q2(const Q2ArgsT<double> &g, int nb, hipStream_t st);      // T factors, then Y <- Q2 Y
+void launch_q2(const Q2ArgsT<float> &g, int nb, hipStream_t st);
+void launch_rows_c32_to_c64(int n, int r0, int r1, const float2 *y32, double2 *y64, size_t stride, int nb,
+                            hipStream_t st);
 int oa_max_n();                  // largest N of launch_oa_stats
 // Ogita-Aishima step: lam [nb][n], perm [nb][n], delta [nb] are scratch
 void launch_oa_stats(int n, const int *nact, const double2 *t, const double2 *g, size_t stride, double *lam,

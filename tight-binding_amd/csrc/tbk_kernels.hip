@@ -486,7 +486,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
     const unsigned gxy = gridDim.x * gridDim.y;
     const unsigned lg = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y + gxy * blockIdx.z, gxy * gridDim.z);
     const int bm = (int)(lg % gridDim.x) * BM, bn = (int)((lg / gridDim.x) % gridDim.y) * BN, kb = (int)(lg / gxy);
-    if ((g.tri == 4 || g.tri == 5 || g.tri == 6) && bm + BM <= bn) return;
+    if ((g.tri == 4 || g.tri == 5 || g.tri == 6 || g.tri == 8) && bm + BM <= bn) return;
     if (g.tri == 7 && bn + BN <= bm) return;            // tile wholly below the diagonal
     if (g.ncol && bn >= g.ncol[kb]) return;             // columns outside the item's window
     const C *A = g.a + (size_t)kb * g.sa;
@@ -599,10 +599,11 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
                     typedef typename GemmArgsT<R>::CX CX;
                     typedef typename std::conditional<sizeof(R) == 8, float, double>::type RX;
                     CX *Xm = g.cx ? g.cx + (size_t)kb * g.scx : nullptr;
-                    if (g.tri == 5 || g.tri == 6) {
+                    if (g.tri == 5 || g.tri == 6 || g.tri == 8) {
                         // Hermitian from the lower half: element (r, c), r >= c,
                         // and its conjugate at (c, r); 16 lanes of one row write
-                        // 16 consecutive rows of column r
+                        // 16 consecutive rows of column r (tri 8: with beta C)
+                        if (g.tri == 8 && r >= c && usebeta) out = zadd(out, zmul(g.beta, *p));
                         if (r >= c) {
                             *p = out;
                             if (Xm) Xm[(size_t)c * g.ldcx + r] = cmake<CX, RX>((RX)out.x, (RX)out.y);
@@ -1678,7 +1679,7 @@ __global__ __launch_bounds__(256) void k_stein_orth(SteinArgs s)
 // reads of Vt rows and the writes of Y columns are coalesced.
 template <class C>
 __global__ __launch_bounds__(256) void k_vt_to_columns(int N, const double *Vt, C *Y, size_t stride,
-                                                       double2 *Y64row0)
+                                                       double2 *Y64row0, const double2 *ph)
 {
     typedef typename std::conditional<sizeof(C) == 16, double, float>::type R;
     __shared__ double tile[32][33];
@@ -1694,9 +1695,11 @@ __global__ __launch_bounds__(256) void k_vt_to_columns(int N, const double *Vt, 
     for (int c = ty; c < 32; c += 8) {
         const int j = j0 + c, i = i0 + tx;
         if (i < N && j < N) {
-            const R v = (R)tile[tx][c];
-            Yk[(size_t)j * N + i] = cmake<C, R>(v, R(0));
-            if (i == 0 && Y64row0) Y64row0[(size_t)kb * stride + (size_t)j * N] = make_double2((double)v, 0.0);
+            const double v = tile[tx][c];
+            const double2 f = ph ? ph[(size_t)kb * N + i] : make_double2(1.0, 0.0);
+            Yk[(size_t)j * N + i] = cmake<C, R>((R)(v * f.x), (R)(v * f.y));
+            if (i == 0 && Y64row0)
+                Y64row0[(size_t)kb * stride + (size_t)j * N] = make_double2((double)(R)(v * f.x), (double)(R)(v * f.y));
         }
     }
 }
@@ -2271,6 +2274,645 @@ __global__ __launch_bounds__(256) void k_abs2(int n, const double2 *y, size_t st
 }
 
 // ---------------------------------------------------------------------------
+// Two-stage tridiagonalisation
+// ---------------------------------------------------------------------------
+// Stage 1 reduces the Hermitian matrix (both triangles stored) to a lower
+// band of width TB, TB columns at a time: k_band_qr factors the panel below
+// the band, A[c0+TB:, c0:c0+TB] = Q R, and the trailing matrix takes the
+// two-sided update Q^H A22 Q as GEMMs (Y = A22 V T, Z = Y - 1/2 V T^H V^H Y,
+// A22 -= V Z^H + Z V^H).  Stage 2 (k_bulge) chases the band to tridiagonal
+// form, one column per sweep, sweeps pipelined across the waves of a
+// workgroup.  Eigenvectors go back through the stage-2 reflectors grouped
+// TB at a time (k_q2_tfactors, k_apply_q2) and then the stage-1 panels.
+
+// Sum over the 64 lanes of a wave (every lane gets the total).
+template <class R>
+__device__ __forceinline__ R wsum(R v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+template <class C>
+__device__ __forceinline__ C cshfl(C v, int src)
+{
+    v.x = __shfl(v.x, src, 64);
+    v.y = __shfl(v.y, src, 64);
+    return v;
+}
+template <class C>
+__device__ __forceinline__ C cshfl_xor(C v, int off)
+{
+    v.x = __shfl_xor(v.x, off, 64);
+    v.y = __shfl_xor(v.y, off, 64);
+    return v;
+}
+// 1 / z
+template <class C>
+__device__ __forceinline__ C crecip(C z)
+{
+    typedef decltype(z.x) R;
+    const R d = z.x * z.x + z.y * z.y;
+    return cmake<C, R>(z.x / d, -z.y / d);
+}
+
+// Householder reflector of a column x held one row per thread: returns tau
+// and beta (H^H x = beta e1 with H = I - tau v v^H, v[0] = 1, beta real) and
+// the scale 1 / (alpha - beta) of the rows below the first; tau = 0 when the
+// column is already of that form.
+template <class R, class C>
+__device__ __forceinline__ void larfg_coef(C alpha, R xn2, C &tau, R &beta, C &scale)
+{
+    if (xn2 == R(0) && alpha.y == R(0)) {
+        tau = czero<C>();
+        beta = alpha.x;
+        scale = czero<C>();
+        return;
+    }
+    beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
+    tau = cmake<C, R>((beta - alpha.x) / beta, -alpha.y / beta);
+    scale = crecip(cmake<C, R>(alpha.x - beta, alpha.y));
+}
+
+// ---- stage 1: panel QR ------------------------------------------------------
+// One workgroup of NW waves per item; the panel's m = n - c0 - TB rows are
+// held in registers, row tid + k NT in slot k (RPT slots).  Column j: the
+// norm and the TB dot products of the reflector with the other columns (the
+// later columns for the update, the earlier reflectors for T) are each one
+// block reduction.  Writes: the reflectors V (unit diagonal, explicit) into
+// the panel of A and into P columns 0..TB-1 and 2TB..3TB-1, T, and the band
+// columns c0..c0+TB-1 (V T and V T^H follow as GEMMs).
+template <class R, int NW, int RPT>
+__global__ __launch_bounds__(64 * NW) void k_band_qr(BandArgsT<R> g, int c0, int panel)
+{
+    typedef typename cplx<R>::type C;
+    constexpr int NT = 64 * NW;
+    const int n = g.n, m = n - c0 - TB, s0 = c0 + TB, kb = blockIdx.x;
+    C *A = g.a + (size_t)kb * g.ss;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    __shared__ R red[NW][2 * TB];
+    __shared__ R rnorm[NW];
+    __shared__ C alpha_sh;
+    __shared__ C Tm[TB][TB + 1];
+    for (int e = tid; e < TB * (TB + 1); e += NT) Tm[e / (TB + 1)][e % (TB + 1)] = czero<C>();
+    C p[RPT][TB];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int r = tid + k * NT;
+#pragma unroll
+        for (int c = 0; c < TB; ++c)
+            p[k][c] = r < m ? A[(size_t)(c0 + c) * n + s0 + r] : czero<C>();
+    }
+    // Columns rotate through the slots: at step j, slot q holds column
+    // (q + j) mod TB, so the current column is always slot 0, the later
+    // columns slots 1 .. TB-1-j and the finished reflectors the rest (no
+    // register array is indexed by j); after TB steps the order is restored.
+    const C one = cmake<C, R>(R(1), R(0));
+    for (int j = 0; j < TB; ++j) {
+        R nrm = 0;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int r = tid + k * NT;
+            if (r > j && r < m) nrm += p[k][0].x * p[k][0].x + p[k][0].y * p[k][0].y;
+        }
+        if (tid == j) alpha_sh = p[0][0];
+        nrm = wsum(nrm);
+        if (lane == 0) rnorm[wv] = nrm;
+        __syncthreads();
+        R xn2 = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) xn2 += rnorm[w];
+        const C alpha = alpha_sh;
+        C tau = czero<C>(), scale = czero<C>();
+        R beta = alpha.x;
+        const bool have = j < m - 1;            // rows j and j + 1 exist
+        if (have) larfg_coef<R, C>(alpha, xn2, tau, beta, scale);
+        // v (rows > j scaled in place), R(j, j) = beta
+        C vr[RPT];
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int r = tid + k * NT;
+            const C x = p[k][0];
+            C v = czero<C>();
+            if (r == j) v = one;
+            else if (r > j && r < m) v = have ? zmul(x, scale) : x;
+            vr[k] = v;
+            p[k][0] = (r == j) ? (have ? cmake<C, R>(beta, R(0)) : alpha) : ((r > j && r < m) ? v : x);
+        }
+        // partial dots by slot q (column c = (q + j) mod TB): the earlier
+        // reflectors (q >= TB - j): conj(V[r, c]) v[r]; the later columns
+        // (1 <= q < TB - j): conj(v[r]) P[r, c]; two halves of TB / 2 slots
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            R part[TB];
+#pragma unroll
+            for (int qq = 0; qq < TB / 2; ++qq) {
+                const int q = h * (TB / 2) + qq;
+                C acc = czero<C>();
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    const int r = tid + k * NT;
+                    if (r >= j && r < m) {
+                        if (q >= TB - j) acc = zadd(acc, zcmul(p[k][q], vr[k]));
+                        else if (q >= 1) acc = zadd(acc, zcmul(vr[k], p[k][q]));
+                    }
+                }
+                part[2 * qq] = acc.x;
+                part[2 * qq + 1] = acc.y;
+            }
+            const R tot = wave_sum_transposed<R, TB>(part, lane);
+            if ((lane & 3) == 0) {
+                const int v = lane >> 2, q = h * (TB / 2) + (v >> 1), c = (q + j) & (TB - 1);
+                red[wv][2 * c + (v & 1)] = tot;                 // stored by column
+            }
+        }
+        __syncthreads();
+        // update of the later columns: P[r, c] -= conj(tau) v[r] w[c]
+        const C ctau = zconj(tau);
+#pragma unroll
+        for (int q = 1; q < TB; ++q) {
+            if (q >= TB - j) continue;
+            const int c = q + j;
+            C w = czero<C>();
+#pragma unroll
+            for (int u = 0; u < NW; ++u) w = zadd(w, cmake<C, R>(red[u][2 * c], red[u][2 * c + 1]));
+            const C f = zmul(ctau, w);
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int r = tid + k * NT;
+                if (r >= j && r < m) p[k][q] = zsub(p[k][q], zmul(vr[k], f));
+            }
+        }
+        // column j of T: T[l, j] = -tau sum_{q=l}^{j-1} T[l, q] (V^H v)[q]
+        if (tid < j) {
+            C acc = czero<C>();
+            for (int q = tid; q < j; ++q) {
+                C gq = czero<C>();
+                for (int u = 0; u < NW; ++u) gq = zadd(gq, cmake<C, R>(red[u][2 * q], red[u][2 * q + 1]));
+                acc = zadd(acc, zmul(Tm[tid][q], gq));
+            }
+            Tm[tid][j] = zmul(cmake<C, R>(-tau.x, -tau.y), acc);
+        }
+        if (tid == j) Tm[j][j] = tau;
+        // rotate the slots left by one
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const C first = p[k][0];
+#pragma unroll
+            for (int q = 0; q < TB - 1; ++q) p[k][q] = p[k][q + 1];
+            p[k][TB - 1] = first;
+        }
+        __syncthreads();
+    }
+    // ---- outputs ----
+    C *P = g.p + (size_t)kb * g.sp;
+    C *Tg = g.t + (size_t)kb * g.st + (size_t)panel * TB * TB;
+    for (int e = tid; e < TB * TB; e += NT) Tg[e] = Tm[e % TB][e / TB];      // column-major
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int r = tid + k * NT;
+        if (r >= m) continue;
+        const size_t row = (size_t)s0 + r;
+#pragma unroll
+        for (int c = 0; c < TB; ++c) {
+            const C vc = r > c ? p[k][c] : (r == c ? one : czero<C>());   // V[r, c]
+            A[(size_t)(c0 + c) * n + row] = vc;
+            P[(size_t)c * n + row] = vc;
+            P[(size_t)(2 * TB + c) * n + row] = vc;
+        }
+    }
+    // band columns c0 .. c0+TB-1: the diagonal block from A, R, zeros
+    C *band = g.band + (size_t)kb * g.sb;
+    if (tid < TB) {
+#pragma unroll
+        for (int c = 0; c < TB; ++c)
+            if (c >= tid) band[(size_t)(c0 + c) * TBLD + TB + tid - c] = tid < m ? p[0][c] : czero<C>();
+    }
+    for (int e = tid; e < TB * TBLD; e += NT) {
+        const int c = e / TBLD, dd = e % TBLD;
+        if (dd <= TB - 1 - c) band[(size_t)(c0 + c) * TBLD + dd] = A[(size_t)(c0 + c) * n + c0 + c + dd];
+        else if (dd > TB) band[(size_t)(c0 + c) * TBLD + dd] = czero<C>();
+    }
+    // the panel's columns of A and vt above the reflectors are zero for the
+    // back-transform, whose merged panels read them from the first
+    // reflector row of the group (the diagonal block was read just above)
+    __syncthreads();
+    C *VT = g.vt + (size_t)kb * g.ss;
+    for (int e = tid; e < TB * s0; e += NT) {
+        const int c = e / s0, r = e % s0;
+        A[(size_t)(c0 + c) * n + r] = czero<C>();
+        VT[(size_t)(c0 + c) * n + r] = czero<C>();
+    }
+}
+
+// Band columns c_end .. n-1 (after the last panel) from the trailing block.
+template <class R>
+__global__ __launch_bounds__(256) void k_band_tail(BandArgsT<R> g, int c_end)
+{
+    typedef typename cplx<R>::type C;
+    const int n = g.n, kb = blockIdx.x;
+    const C *A = g.a + (size_t)kb * g.ss;
+    C *band = g.band + (size_t)kb * g.sb;
+    for (int e = threadIdx.x; e < (n - c_end) * TBLD; e += blockDim.x) {
+        const int c = c_end + e / TBLD, dd = e % TBLD;
+        band[(size_t)c * TBLD + dd] = (dd <= TB && c + dd < n) ? A[(size_t)c * n + c + dd] : czero<C>();
+    }
+}
+
+// ---- stage 2: bulge chasing -------------------------------------------------
+// Sweep i annihilates column i below the subdiagonal; its step s works on
+// rows r0 = i + 1 + s TB .. r0 + TB - 1: the previous step's reflector is
+// applied from the right to the block E = A[r0:, r0-TB:r0] (filling it in
+// below the band), a new reflector annihilates E's first column below its
+// first row and is applied from the left to E and two-sided to the
+// diagonal block D = A[r0:, r0:].  Step (i, s) runs at time t = 2 i + s:
+// steps at the same time touch disjoint elements, and every step that
+// must precede it ran at an earlier time.  One wave per step; a barrier
+// between times.  The band is stored by columns, band[c TBLD + (r - c)],
+// r - c < 2 TB (the fill of E reaches r - c = 2 TB - 1).
+// Reflectors of the sweeps in flight are kept in LDS slots (i mod NSLOT):
+// at most (steps of sweep 0) / 2 + 1 sweeps run at once: 10 at order 288
+// (the largest FP64 band kept in LDS), 19 at 576 (FP32), 41 at 1280.
+
+__device__ __forceinline__ int bulge_steps(int n, int i) { return (n - 2 - i) / TB + 1; }
+
+template <class R, class C, int NSLOT>
+__device__ __forceinline__ void bulge_step(C *band, int n, int i, int s, C (*slots)[TB], C *v2, const int *dbase,
+                                           int lane)
+{
+    const int a = lane & (TB - 1), cq = lane >> 4;       // row a, columns 4 cq .. 4 cq + 3
+    const int r0 = i + 1 + s * TB, L = min(TB, n - r0);
+    const C one = cmake<C, R>(R(1), R(0));
+    C tau = czero<C>(), va = czero<C>();
+    R beta = 0;
+    if (s == 0) {
+        // column i, rows r0 .. r0 + L - 1 (L >= 2)
+        const C x = (cq == 0 && a < L) ? band[(size_t)i * TBLD + 1 + a] : czero<C>();
+        const R xn2 = wsum((cq == 0 && a >= 1) ? x.x * x.x + x.y * x.y : R(0));
+        const C alpha = cshfl(x, 0);
+        C scale;
+        larfg_coef<R, C>(alpha, xn2, tau, beta, scale);
+        const C xa = cshfl(x, a);
+        va = a == 0 ? one : (a < L ? zmul(xa, scale) : czero<C>());
+        if (cq == 0 && a < L)
+            band[(size_t)i * TBLD + 1 + a] = a == 0 ? cmake<C, R>(beta, R(0)) : czero<C>();
+    } else {
+        const int p0 = r0 - TB;
+        const int sl = i % NSLOT;
+        const C taup = slots[sl][0];
+        C E[4], vp[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = 4 * cq + q;
+            E[q] = a < L ? band[(size_t)(p0 + c) * TBLD + TB + a - c] : czero<C>();
+            vp[q] = c == 0 ? one : slots[sl][c];
+        }
+        // E <- E Hp: y = E vp, E -= taup y vp^H
+        C y = czero<C>();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) y = zadd(y, zmul(E[q], vp[q]));
+        y = zadd(y, cshfl_xor(y, 16));
+        y = zadd(y, cshfl_xor(y, 32));
+        const C ty = zmul(taup, y);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) E[q] = zsub(E[q], zmul(ty, zconj(vp[q])));
+        if (L >= 2) {
+            // reflector of E's first column (lanes cq == 0, slot 0)
+            const C x = cq == 0 ? E[0] : czero<C>();
+            const R xn2 = wsum((cq == 0 && a >= 1) ? x.x * x.x + x.y * x.y : R(0));
+            const C alpha = cshfl(x, 0);
+            C scale;
+            larfg_coef<R, C>(alpha, xn2, tau, beta, scale);
+            const C xa = cshfl(x, a);
+            va = a == 0 ? one : (a < L ? zmul(xa, scale) : czero<C>());
+            // E[:, 1:] -= conj(tau) v (v^H E[:, 1:])
+            const C ct = zconj(tau);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                C w = zcmul(va, E[q]);
+                w = zadd(w, cshfl_xor(w, 1));
+                w = zadd(w, cshfl_xor(w, 2));
+                w = zadd(w, cshfl_xor(w, 4));
+                w = zadd(w, cshfl_xor(w, 8));
+                const int c = 4 * cq + q;
+                if (c == 0) E[q] = a == 0 ? cmake<C, R>(beta, R(0)) : czero<C>();
+                else E[q] = zsub(E[q], zmul(zmul(ct, va), w));
+            }
+        }
+        if (a < L) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int c = 4 * cq + q;
+                band[(size_t)(p0 + c) * TBLD + TB + a - c] = E[q];
+            }
+        }
+        if (L < 2) return;
+    }
+    // the reflector, for the next step of this sweep and the back-transform
+    const int sl = i % NSLOT;
+    const int jb = i / TB;
+    if (cq == 0) {
+        const C sv = a == 0 ? tau : va;
+        slots[sl][a] = sv;
+        v2[((size_t)(dbase[jb] + s) * TB + (i - jb * TB)) * TB + a] = sv;
+    }
+    // D <- H^H D H: p = D v, w = tau p - 1/2 |tau|^2 (v^H D v) v,
+    // D -= v w^H + w v^H
+    C D[4], vc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = 4 * cq + q;
+        C dv = czero<C>();
+        if (a < L && c < L)
+            dv = c <= a ? band[(size_t)(r0 + c) * TBLD + a - c] : zconj(band[(size_t)(r0 + a) * TBLD + c - a]);
+        D[q] = dv;
+        vc[q] = cshfl(va, c);
+    }
+    C pp = czero<C>();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) pp = zadd(pp, zmul(D[q], vc[q]));
+    pp = zadd(pp, cshfl_xor(pp, 16));
+    pp = zadd(pp, cshfl_xor(pp, 32));
+    const C wv = zmul(tau, pp);
+    C gam = zcmul(va, wv);
+    gam = zadd(gam, cshfl_xor(gam, 1));
+    gam = zadd(gam, cshfl_xor(gam, 2));
+    gam = zadd(gam, cshfl_xor(gam, 4));
+    gam = zadd(gam, cshfl_xor(gam, 8));
+    const C h = zscale(R(0.5), zmul(zconj(tau), gam));
+    const C wp = zsub(wv, zmul(h, va));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = 4 * cq + q;
+        const C wpc = cshfl(wp, c);
+        D[q] = zsub(D[q], zadd(zmul(va, zconj(wpc)), zmul(wp, zconj(vc[q]))));
+        if (a < L && c <= a) band[(size_t)(r0 + c) * TBLD + a - c] = D[q];
+    }
+}
+
+// One workgroup per item; dynamic LDS holds the band when LDSB.
+template <class R, bool LDSB>
+__global__ __launch_bounds__(1024) void k_bulge(BulgeArgsT<R> g)
+{
+    typedef typename cplx<R>::type C;
+    constexpr int NSLOT = LDSB ? (sizeof(R) == 8 ? 16 : 32) : 64;
+    extern __shared__ unsigned char bulge_dyn[];
+    __shared__ C slots[NSLOT][TB];
+    __shared__ int dbase[kMaxBandBlocks + 1];
+    const int n = g.n, kb = blockIdx.x;
+    const int nw = blockDim.x >> 6, wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    C *gband = g.band + (size_t)kb * g.sb;
+    C *band = LDSB ? reinterpret_cast<C *>(bulge_dyn) : gband;
+    const int nj = (n - 3) / TB + 1;                     // sweep blocks
+    if (threadIdx.x == 0) {
+        int b = 0;
+        for (int j = 0; j < nj; ++j) { dbase[j] = b; b += bulge_steps(n, j * TB); }
+        dbase[nj] = b;
+    }
+    if (LDSB)
+        for (int e = threadIdx.x; e < n * TBLD; e += blockDim.x) band[e] = gband[e];
+    __syncthreads();
+    C *v2 = g.v2 + (size_t)kb * g.sv;
+    for (int t = 0; t <= 2 * n - 6; ++t) {
+        const int ihi = min(t >> 1, n - 3);
+        for (int i = ihi - wv; i >= 0; i -= nw) {
+            const int s = t - 2 * i;
+            if (s >= bulge_steps(n, i)) break;
+            bulge_step<R, C, NSLOT>(band, n, i, s, slots, v2, dbase, lane);
+        }
+        __syncthreads();
+    }
+    // tridiagonal: d, |e|, and the phases that make e real
+    double *d = g.d + (size_t)kb * g.sde, *e = g.e + (size_t)kb * g.sde;
+    double2 *ph = g.ph + (size_t)kb * g.sph;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+        d[r] = (double)band[(size_t)r * TBLD].x;
+        if (r < n - 1) {
+            const C ec = band[(size_t)r * TBLD + 1];
+            e[r] = sqrt((double)ec.x * ec.x + (double)ec.y * ec.y);
+        } else {
+            e[r] = 0.0;
+        }
+    }
+    if (threadIdx.x == 0) {
+        double2 f = make_double2(1.0, 0.0);
+        ph[0] = f;
+        for (int r = 0; r < n - 1; ++r) {
+            const C ec = band[(size_t)r * TBLD + 1];
+            const double ax = sqrt((double)ec.x * ec.x + (double)ec.y * ec.y);
+            if (ax > 0.0) f = zmul(f, make_double2((double)ec.x / ax, (double)ec.y / ax));
+            ph[r + 1] = f;
+        }
+    }
+}
+
+// ---- back-transform through the stage-2 reflectors -------------------------
+// Diamond (j, s) groups the reflectors of sweeps jTB .. jTB+TB-1 at step s:
+// reflector c acts on rows jTB + 1 + sTB + c .. + TB - 1, so their WY form
+// I - V T V^H spans 2 TB - 1 rows with V[c + q, c] = v_c[q].  Reflectors
+// that do not exist (sweeps beyond n - 3, steps beyond a sweep's last, or a
+// last step of one row) are cleared to zero here; T is built by the forward
+// recurrence T[:c, c] = -tau_c T[:c, :c] (V^H V)[:c, c].  One wave per
+// diamond.
+template <class R>
+__global__ __launch_bounds__(64) void k_q2_tfactors(Q2ArgsT<R> g)
+{
+    typedef typename cplx<R>::type C;
+    __shared__ C vv[TB][TB];       // vv[c][q] = v_c[q] (q = 0: tau)
+    __shared__ C G[TB][TB + 1];
+    __shared__ C Tm[TB][TB + 1];
+    const int n = g.n, kb = blockIdx.y, dix = blockIdx.x, lane = threadIdx.x;
+    const int nj = (n - 3) / TB + 1;
+    int j = 0, b = 0;
+    while (j < nj && b + bulge_steps(n, j * TB) <= dix) { b += bulge_steps(n, j * TB); ++j; }
+    if (j >= nj) return;
+    const int s = dix - b;
+    C *blk = g.v2 + (size_t)kb * g.sv + (size_t)dix * TB * TB;
+    for (int e = lane; e < TB * TB; e += 64) {
+        const int c = e / TB, q = e % TB, i = j * TB + c;
+        const bool ok = i <= n - 3 && s < bulge_steps(n, i) && min(TB, n - (i + 1 + s * TB)) >= 2 &&
+                        q < min(TB, n - (i + 1 + s * TB));
+        C x = ok ? blk[e] : czero<C>();
+        blk[e] = x;
+        vv[c][q] = x;
+    }
+    __syncthreads();
+    // G[l][c] = (V^H V)[l][c], l < c: sum over rows of conj(v_l[r - l]) v_c[r - c]
+    for (int e = lane; e < TB * TB; e += 64) {
+        const int l = e / TB, c = e % TB;
+        if (l >= c) continue;
+        C acc = czero<C>();
+        for (int r = c; r < l + TB; ++r) {
+            const C vl = (r == l) ? cmake<C, R>(R(1), R(0)) : vv[l][r - l];
+            const C vcc = (r == c) ? cmake<C, R>(R(1), R(0)) : vv[c][r - c];
+            acc = zadd(acc, zcmul(vl, vcc));
+        }
+        G[l][c] = acc;
+    }
+    __syncthreads();
+    for (int c = 0; c < TB; ++c) {
+        const C tc = vv[c][0];
+        if (lane < c) {
+            C acc = czero<C>();
+            for (int q = lane; q < c; ++q) acc = zadd(acc, zmul(Tm[lane][q], G[q][c]));
+            Tm[lane][c] = zmul(cmake<C, R>(-tc.x, -tc.y), acc);
+        } else if (lane < TB) {
+            Tm[lane][c] = lane == c ? tc : czero<C>();
+        }
+        __syncthreads();
+    }
+    C *T = g.t2 + (size_t)kb * g.st2 + (size_t)dix * TB * TB;
+    for (int e = lane; e < TB * TB; e += 64) T[e] = Tm[e % TB][e / TB];     // column-major
+}
+
+// Y <- Q2 Y, 16 columns per wave, every wave on its own (no LDS, no
+// barriers): diamonds by block j descending, step s ascending; each is
+//   W = V^H Y_rows (16 x 16),  W <- T W,  Y_rows -= V W  (32 rows)
+// on the matrix cores, complex products as three real ones.  The K index of
+// every product is laid out like the rows of an accumulator (slot q of lane
+// group g4 is row mfma_row(q)), so each accumulator is the next product's B
+// operand in the same lane, and the 32 rows of Y a wave holds (two halves of
+// 16) are both B operand and update target.  Consecutive diamonds of a
+// block overlap by 16 rows: the lower half moves up, one new half is loaded.
+constexpr int kQ2Waves = 4;
+constexpr int kQ2Tiles = 2;               // 16-column tiles per wave (sharing the V and T operands)
+template <class R>
+__global__ __launch_bounds__(64 * kQ2Waves) void k_apply_q2(Q2ArgsT<R> g)
+{
+    typedef typename cplx<R>::type C;
+    typedef R v4 __attribute__((ext_vector_type(4)));
+    constexpr int NT = kQ2Tiles;
+    const int n = g.n, kb = blockIdx.y, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ncols = g.ncol ? min(g.ncol[kb], n) : n;
+    const int wc0 = (blockIdx.x * kQ2Waves + wv) * 16 * NT;
+    if (wc0 >= ncols) return;                            // per wave: nothing below synchronises
+    const int r16 = lane & 15;
+    bool colok[NT];
+    C *Y[NT];
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+        colok[u] = wc0 + 16 * u + r16 < ncols;
+        Y[u] = g.y + (size_t)kb * g.sy + (size_t)(wc0 + 16 * u + r16) * n;
+    }
+    const C *V2 = g.v2 + (size_t)kb * g.sv;
+    const C *T2 = g.t2 + (size_t)kb * g.st2;
+    int rm[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) rm[q] = mfma_row(R(0), lane, q);
+    const C one = cmake<C, R>(R(1), R(0));
+    // V[k][c] of a diamond block: reflector c's entry k - c (1 on the diagonal)
+    auto vel = [&](const C *blk, int k, int c) -> C {
+        const int q = k - c;
+        if (q == 0) return one;
+        if (q < 0 || q >= TB) return czero<C>();
+        return blk[c * TB + q];
+    };
+    const int nj = (n - 3) / TB + 1;
+    int dend = 0;
+    for (int j = 0; j < nj; ++j) dend += bulge_steps(n, j * TB);
+    for (int j = nj - 1; j >= 0; --j) {
+        const int ns = bulge_steps(n, j * TB), d0 = dend - ns;
+        dend = d0;
+        int rlo = j * TB + 1;
+        C y[NT][2][4];
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = rlo + 16 * h + rm[q];
+                    y[u][h][q] = (colok[u] && r < n) ? Y[u][r] : czero<C>();
+                }
+        for (int s = 0; s < ns; ++s, rlo += TB) {
+            const C *blk = V2 + (size_t)(d0 + s) * TB * TB;
+            const C *tb = T2 + (size_t)(d0 + s) * TB * TB;
+            C va[2][4], vb[2][4], tt[4];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    va[h][q] = vel(blk, 16 * h + rm[q], r16);     // V[k][c], k = 16 h + rm[q], c = r16
+                    vb[h][q] = vel(blk, 16 * h + r16, rm[q]);     // V[16 h + r16][rm[q]]
+                }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) tt[q] = tb[rm[q] * TB + r16];   // T[r16][rm[q]] (column-major)
+#pragma unroll
+            for (int u = 0; u < NT; ++u) {
+                // W = V^H Y
+                v4 a1 = {0, 0, 0, 0}, a2 = {0, 0, 0, 0}, a3 = {0, 0, 0, 0};
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const R ar = va[h][q].x, ai = -va[h][q].y;
+                        a1 = mfma16(ar, y[u][h][q].x, a1);
+                        a2 = mfma16(ai, y[u][h][q].y, a2);
+                        a3 = mfma16(ar + ai, y[u][h][q].x + y[u][h][q].y, a3);
+                    }
+                C w[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) w[q] = cmake<C, R>(a1[q] - a2[q], a3[q] - a1[q] - a2[q]);
+                // W <- T W
+                a1 = v4{0, 0, 0, 0}; a2 = v4{0, 0, 0, 0}; a3 = v4{0, 0, 0, 0};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    a1 = mfma16(tt[q].x, w[q].x, a1);
+                    a2 = mfma16(tt[q].y, w[q].y, a2);
+                    a3 = mfma16(tt[q].x + tt[q].y, w[q].x + w[q].y, a3);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) w[q] = cmake<C, R>(a1[q] - a2[q], a3[q] - a1[q] - a2[q]);
+                // Y_rows -= V W
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    a1 = v4{0, 0, 0, 0}; a2 = v4{0, 0, 0, 0}; a3 = v4{0, 0, 0, 0};
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        a1 = mfma16(vb[h][q].x, w[q].x, a1);
+                        a2 = mfma16(vb[h][q].y, w[q].y, a2);
+                        a3 = mfma16(vb[h][q].x + vb[h][q].y, w[q].x + w[q].y, a3);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        y[u][h][q] = cmake<C, R>(y[u][h][q].x - (a1[q] - a2[q]), y[u][h][q].y - (a3[q] - a1[q] - a2[q]));
+                }
+            }
+            // rows rlo .. rlo+15 are done for this block; the lower half moves up
+#pragma unroll
+            for (int u = 0; u < NT; ++u)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = rlo + rm[q];
+                    if (colok[u] && r < n) Y[u][r] = y[u][0][q];
+                    y[u][0][q] = y[u][1][q];
+                    const int rn = rlo + 32 + rm[q];
+                    y[u][1][q] = (s + 1 < ns && colok[u] && rn < n) ? Y[u][rn] : czero<C>();
+                }
+        }
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = rlo + rm[q];
+                if (colok[u] && r < n) Y[u][r] = y[u][0][q];
+            }
+    }
+}
+
+// Y64[r, j] = Y32[r, j] for rows r0 .. r1 - 1 of every column.
+__global__ __launch_bounds__(256) void k_rows_c32_to_c64(int n, int r0, int r1, const float2 *y32, double2 *y64,
+                                                         size_t stride)
+{
+    const int kb = blockIdx.y, rows = r1 - r0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < rows * n; e += gridDim.x * blockDim.x) {
+        const int r = r0 + e % rows, c = e / rows;
+        const float2 v = y32[(size_t)kb * stride + (size_t)c * n + r];
+        y64[(size_t)kb * stride + (size_t)c * n + r] = make_double2((double)v.x, (double)v.y);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 void launch_phases(const StructDev *sd, const int *item_s, const double *kpts, int maxpairs,
@@ -2416,17 +3058,88 @@ void launch_stein(const SteinArgs &s, int nb, hipStream_t st)
     launch_stein_vectors(s, nb, st);
 }
 void launch_real_to_complex_t(int N, const double *Vt, double2 *Y, size_t stride, int nb,
-                              hipStream_t st)
+                              hipStream_t st, const double2 *ph)
 {
     const int t = (N + 31) / 32;
     hipLaunchKernelGGL((k_vt_to_columns<double2>), dim3(t, t, nb), dim3(256), 0, st, N, Vt, Y, stride,
-                       (double2 *)nullptr);
+                       (double2 *)nullptr, ph);
 }
 void launch_real_to_c32_t(int n, const double *vt, float2 *y, size_t stride, int nb, hipStream_t st,
-                          double2 *y64_row0)
+                          double2 *y64_row0, const double2 *ph)
 {
     const int t = (n + 31) / 32;
-    hipLaunchKernelGGL((k_vt_to_columns<float2>), dim3(t, t, nb), dim3(256), 0, st, n, vt, y, stride, y64_row0);
+    hipLaunchKernelGGL((k_vt_to_columns<float2>), dim3(t, t, nb), dim3(256), 0, st, n, vt, y, stride, y64_row0,
+                       ph);
+}
+int band_diamonds(int n)
+{
+    int b = 0;
+    for (int i = 0; i <= n - 3; i += TB) b += (n - 2 - i) / TB + 1;
+    return b;
+}
+// The panel rows live in registers, RPT per thread: FP64 keeps RPT at 2
+// (a double2 slot row of TB columns is 64 registers) and adds waves with m.
+void launch_band_qr(const BandArgsT<double> &g, int panel, int nb, hipStream_t st)
+{
+    const int c0 = panel * TB, m = g.n - c0 - TB;
+    if (m <= 128) hipLaunchKernelGGL((k_band_qr<double, 1, 2>), dim3(nb), dim3(64), 0, st, g, c0, panel);
+    else if (m <= 256) hipLaunchKernelGGL((k_band_qr<double, 2, 2>), dim3(nb), dim3(128), 0, st, g, c0, panel);
+    else if (m <= 512) hipLaunchKernelGGL((k_band_qr<double, 4, 2>), dim3(nb), dim3(256), 0, st, g, c0, panel);
+    else if (m <= 1024) hipLaunchKernelGGL((k_band_qr<double, 8, 2>), dim3(nb), dim3(512), 0, st, g, c0, panel);
+    else hipLaunchKernelGGL((k_band_qr<double, 10, 2>), dim3(nb), dim3(640), 0, st, g, c0, panel);
+}
+void launch_band_qr(const BandArgsT<float> &g, int panel, int nb, hipStream_t st)
+{
+    const int c0 = panel * TB, m = g.n - c0 - TB;
+    if (m <= 128) hipLaunchKernelGGL((k_band_qr<float, 1, 2>), dim3(nb), dim3(64), 0, st, g, c0, panel);
+    else if (m <= 320) hipLaunchKernelGGL((k_band_qr<float, 1, 5>), dim3(nb), dim3(64), 0, st, g, c0, panel);
+    else hipLaunchKernelGGL((k_band_qr<float, 4, 5>), dim3(nb), dim3(256), 0, st, g, c0, panel);
+}
+template <class R>
+static void launch_band_tail_t(const BandArgsT<R> &g, int c_end, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_band_tail<R>), dim3(nb), dim3(256), 0, st, g, c_end);
+}
+void launch_band_tail(const BandArgsT<double> &g, int c_end, int nb, hipStream_t st) { launch_band_tail_t(g, c_end, nb, st); }
+void launch_band_tail(const BandArgsT<float> &g, int c_end, int nb, hipStream_t st) { launch_band_tail_t(g, c_end, nb, st); }
+// LDS of one workgroup (bytes) and what k_bulge's static arrays take at most
+constexpr size_t kLdsBytes = 160 * 1024, kBulgeStatic = 10 * 1024;
+template <class R>
+static void launch_bulge_t(const BulgeArgsT<R> &g, int nb, hipStream_t st)
+{
+    typedef typename cplx<R>::type C;
+    const int n = g.n;
+    const int nw = std::min(16, (n - 2) / TB / 2 + 2);        // sweeps in flight at most
+    const size_t bytes = (size_t)n * TBLD * sizeof(C);
+    if (bytes + kBulgeStatic <= kLdsBytes) {
+        static const bool limit_set = [] {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bulge<R, true>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kLdsBytes - kBulgeStatic));
+            return true;
+        }();
+        (void)limit_set;
+        hipLaunchKernelGGL((k_bulge<R, true>), dim3(nb), dim3(64 * nw), bytes, st, g);
+    } else {
+        hipLaunchKernelGGL((k_bulge<R, false>), dim3(nb), dim3(64 * nw), 0, st, g);
+    }
+}
+void launch_bulge(const BulgeArgsT<double> &g, int nb, hipStream_t st) { launch_bulge_t(g, nb, st); }
+void launch_bulge(const BulgeArgsT<float> &g, int nb, hipStream_t st) { launch_bulge_t(g, nb, st); }
+template <class R>
+static void launch_q2_t(const Q2ArgsT<R> &g, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_q2_tfactors<R>), dim3(band_diamonds(g.n), nb), dim3(64), 0, st, g);
+    constexpr int cols = 16 * kQ2Waves * kQ2Tiles;
+    hipLaunchKernelGGL((k_apply_q2<R>), dim3((g.n + cols - 1) / cols, nb), dim3(64 * kQ2Waves), 0, st, g);
+}
+void launch_q2(const Q2ArgsT<double> &g, int nb, hipStream_t st) { launch_q2_t(g, nb, st); }
+void launch_q2(const Q2ArgsT<float> &g, int nb, hipStream_t st) { launch_q2_t(g, nb, st); }
+void launch_rows_c32_to_c64(int n, int r0, int r1, const float2 *y32, double2 *y64, size_t stride, int nb,
+                            hipStream_t st)
+{
+    if (r1 <= r0) return;
+    const int gx = std::max(1, std::min(((r1 - r0) * n + 255) / 256, 64));
+    hipLaunchKernelGGL(k_rows_c32_to_c64, dim3(gx, nb), dim3(256), 0, st, n, r0, r1, y32, y64, stride);
 }
 int oa_max_n() { return kOaMax; }
 void launch_oa_stats(int n, const int *nact, const double2 *t, const double2 *g, size_t stride, double *lam,

@@ -31,6 +31,16 @@ constexpr double kOrtolRel = 1e-6;
 // Tridiagonalisation over 64x64 tiles (several workgroups per item) for
 // batches below kWideMaxBatch items of order >= kWideMinN.
 constexpr int kWideMinN = 128, kWideMaxBatch = 160;
+// Two-stage tridiagonalisation (band width TB) by default from this order
+// up, in FP64 (at smaller orders the one-stage panels are faster; in the
+// mixed method the FP32 two-stage reduction leaves the refinement about
+// ten times larger eigenvalue errors).
+constexpr int kTwoStageMinN = 512;
+using tbk::TB;
+using tbk::TBLD;
+// The two-stage reduction keeps its band and its stage-2 reflectors and T
+// factors in n x n work matrices: true where they fit (every order >= 40).
+inline bool two_stage_fits(int n) { return n >= 40 && (size_t)tbk::band_diamonds(n) * TB * TB <= (size_t)n * n; }
 
 #define HIP_TRY(x)                                   \
     do {                                             \
@@ -86,6 +96,9 @@ struct Workspace {
     int *perm = nullptr;
     // multi-workgroup tridiagonalisation (few items of large order)
     bool wide = false;
+    // two-stage tridiagonalisation (band reduction + bulge chasing) instead
+    // of the one-stage panels
+    bool two_stage = false;
     double2 *wv = nullptr, *wcol = nullptr, *wcarry = nullptr, *wydot = nullptr, *wyax = nullptr,
             *wudot = nullptr, *wscal = nullptr;
     float2 *wv32 = nullptr, *wcol32 = nullptr, *wcarry32 = nullptr, *wydot32 = nullptr, *wyax32 = nullptr,
@@ -97,6 +110,7 @@ struct Workspace {
     int alloc(int n_, int b_)
     {
         n = n_; b = b_;
+        two_stage = n >= kTwoStageMinN && two_stage_fits(n);
         const size_t nn = (size_t)n * n * b, np = (size_t)n * kNB * b;
         int rc = TBK_OK;
         rc |= dalloc(&A, nn); rc |= dalloc(&S, nn); rc |= dalloc(&Y, nn);
@@ -140,6 +154,7 @@ struct Workspace {
     size_t carve(unsigned char *base, int n_, int b_)
     {
         n = n_; b = b_;
+        two_stage = n >= kTwoStageMinN && two_stage_fits(n);
         size_t off = 0;
         auto take = [&](auto *&p, size_t count) {
             typedef std::remove_reference_t<decltype(*p)> T;
@@ -341,48 +356,92 @@ inline void launch_gemm_any(const tbk::GemmArgsT<float> &g, int opa, int opb, in
     tbk::launch_cgemm(g, opa, opb, nb, st);
 }
 
-// Panels of the Householder reduction applied to Y (Y <- Q Y, Q = Q_0 Q_1
-// ... with Q_p = I - V_p T_p V_p^H), kBT panels at a time.  Within a group
-// the panels' V T^H blocks are first merged from the last panel down,
+// Panels of a Householder reduction applied to Y (Y <- Q Y, Q = Q_0 Q_1
+// ... with Q_p = I - V_p T_p V_p^H), `group` panels at a time.  Within a
+// group the panels' V T^H blocks are first merged from the last panel down,
 //   VT_p <- VT_p - VT_rest (V_rest^H VT_p),
 // which turns [VT_p | VT_rest] into V_c T_c^H of the group's combined WY
-// factor; each group is then two GEMMs of inner dimension up to 32 kBT.
-// A holds the reflectors (column i0.. of panel i0, zero above the
-// subdiagonal), VT the V T^H blocks at the same columns; M is scratch of
-// 96 x 32 per item (stride sm).
+// factor; each group is then two GEMMs of inner dimension up to pw group.
+// A holds the reflectors (panel p covers columns p pw .. p pw + pw - 1, its
+// reflector of column c non-zero from row c + off down), VT the V T^H
+// blocks at the same place; M is scratch of 96 x 32 per item (stride sm).
+struct QPanels {
+    int pw, off, np, group;        // panel width, first row of a reflector below its column, panels, merge
+};
 constexpr int kBT = 3;
+inline QPanels one_stage_panels(int n) { return QPanels{kNB, 1, (n - 2) / kNB + 1, kBT}; }
+// stage 1 of the two-stage reduction: panels c0 = p TB while c0 + TB + 1 < n
+inline int band_panels(int n) { return n >= TB + 2 ? (n - TB - 2) / TB + 1 : 0; }
+inline QPanels band_stage_panels(int n) { return QPanels{TB, TB, band_panels(n), 6}; }
 template <class R>
 void apply_q(int n, int nb, typename tbk::cplx_t<R>::type *A, typename tbk::cplx_t<R>::type *VT,
              typename tbk::cplx_t<R>::type *Y, typename tbk::cplx_t<R>::type *X, typename tbk::cplx_t<R>::type *M,
-             size_t sm, hipStream_t st, const int *ncol = nullptr,
-             typename tbk::GemmArgsT<R>::CX *ycx = nullptr)
+             size_t sm, hipStream_t st, const int *ncol, typename tbk::GemmArgsT<R>::CX *ycx, const QPanels &qp)
 {
     const size_t ss = (size_t)n * n;
-    const int P = (n - 2) / kNB + 1;                  // panels
-    for (int g0 = ((P - 1) / kBT) * kBT; g0 >= 0; g0 -= kBT) {
-        const int g1 = std::min(g0 + kBT, P);          // panels g0 .. g1-1
-        const int c0 = g0 * kNB;
-        const int wlast = std::min(kNB, n - 1 - (g1 - 1) * kNB);
-        const int W = (g1 - 1 - g0) * kNB + wlast;     // columns of the group
+    const int P = qp.np, pw = qp.pw, off = qp.off;
+    if (P <= 0) return;
+    for (int g0 = ((P - 1) / qp.group) * qp.group; g0 >= 0; g0 -= qp.group) {
+        const int g1 = std::min(g0 + qp.group, P);     // panels g0 .. g1-1
+        const int c0 = g0 * pw;
+        const int wlast = std::min(pw, n - off - (g1 - 1) * pw);
+        const int W = (g1 - 1 - g0) * pw + wlast;      // columns of the group
         for (int p = g1 - 2; p >= g0; --p) {            // merge V T^H blocks
-            const int cp = p * kNB, cr = cp + kNB, wr = c0 + W - cr;
-            const int rr0 = cr + 1, k = n - rr0;         // rows where V_rest can be non-zero
-            launch_gemm_any(gemm_t<R>(wr, kNB, k, A + (size_t)cr * n + rr0, n, ss, VT + (size_t)cp * n + rr0, n,
+            const int cp = p * pw, cr = cp + pw, wr = c0 + W - cr;
+            const int rr0 = cr + off, k = n - rr0;       // rows where V_rest can be non-zero
+            launch_gemm_any(gemm_t<R>(wr, pw, k, A + (size_t)cr * n + rr0, n, ss, VT + (size_t)cp * n + rr0, n,
                                       ss, M, wr, sm), 1, 0, nb, st);
-            launch_gemm_any(gemm_t<R>(k, kNB, wr, VT + (size_t)cr * n + rr0, n, ss, M, wr, sm,
+            launch_gemm_any(gemm_t<R>(k, pw, wr, VT + (size_t)cr * n + rr0, n, ss, M, wr, sm,
                                       VT + (size_t)cp * n + rr0, n, ss, R(-1), R(1)), 0, 0, nb, st);
         }
-        const int r0 = c0 + 1, m = n - r0;
+        const int r0 = c0 + off, m = n - r0;
         // X[0:W, :] = (V_c T_c^H)^H Y[r0:, :];  Y[r0:, :] -= V_c X
         launch_gemm_any(with_ncol(gemm_t<R>(W, n, m, VT + (size_t)c0 * n + r0, n, ss, Y + r0, n, ss, X, n, ss), ncol),
                         1, 0, nb, st);
         tbk::GemmArgsT<R> g = with_ncol(gemm_t<R>(m, n, W, A + (size_t)c0 * n + r0, n, ss, X, n, ss, Y + r0, n, ss,
                                                   R(-1), R(1)), ncol);
-        if (g0 == 0 && ycx) {          // the last group: Y rows 1.. also in the other precision
+        if (g0 == 0 && ycx) {          // the last group: Y rows r0.. also in the other precision
             g.cx = ycx + r0; g.ldcx = n; g.scx = ss;
         }
         launch_gemm_any(g, 0, 0, nb, st);
     }
+}
+
+// Stage 1 of the two-stage tridiagonalisation on the batch: A (both
+// triangles) to a lower band of width TB, panel by panel (k_band_qr, then
+// V T, V T^H and the two-sided trailing update as GEMMs), the band into
+// `band` (item stride sb).  P: [V | Y/Z | V | V T] columns (stride sp);
+// scr: 2 TB^2 per item (stride sscr).
+template <class R>
+void band_reduce(int n, int nb, typename tbk::cplx_t<R>::type *A, typename tbk::cplx_t<R>::type *P, size_t sp,
+                 typename tbk::cplx_t<R>::type *T, size_t tstride, typename tbk::cplx_t<R>::type *VT,
+                 typename tbk::cplx_t<R>::type *band, size_t sb, typename tbk::cplx_t<R>::type *scr, size_t sscr,
+                 hipStream_t st)
+{
+    typedef typename tbk::cplx_t<R>::type C;
+    const size_t ss = (size_t)n * n;
+    tbk::BandArgsT<R> b{n, ss, A, VT, P, sp, T, tstride, band, sb};
+    const int np = band_panels(n);
+    for (int p = 0; p < np; ++p) {
+        const int c0 = p * TB, m = n - c0 - TB, s = c0 + TB;
+        tbk::launch_band_qr(b, p, nb, st);
+        C *V = P + s, *Y = P + (size_t)TB * n + s, *W0 = P + (size_t)3 * TB * n + s;
+        C *Tp = T + (size_t)p * TB * TB, *M = scr, *X = scr + TB * TB;
+        // W0 = V T and, for the back-transform, V T^H
+        launch_gemm_any(gemm_t<R>(m, TB, TB, V, n, sp, Tp, TB, tstride, W0, n, sp), 0, 0, nb, st);
+        launch_gemm_any(gemm_t<R>(m, TB, TB, V, n, sp, Tp, TB, tstride, VT + (size_t)c0 * n + s, n, ss), 0, 1, nb,
+                        st);
+        // Y = A22 V T;  Z = Y - 1/2 V (T^H (V^H Y))  (in place)
+        launch_gemm_any(gemm_t<R>(m, TB, m, A + (size_t)s * n + s, n, ss, W0, n, sp, Y, n, sp), 0, 0, nb, st);
+        launch_gemm_any(gemm_t<R>(TB, TB, m, V, n, sp, Y, n, sp, M, TB, sscr), 1, 0, nb, st);
+        launch_gemm_any(gemm_t<R>(TB, TB, TB, Tp, TB, tstride, M, TB, sscr, X, TB, sscr), 1, 0, nb, st);
+        launch_gemm_any(gemm_t<R>(m, TB, TB, V, n, sp, X, TB, sscr, Y, n, sp, R(-0.5), R(1)), 0, 0, nb, st);
+        // A22 -= V Z^H + Z V^H, both triangles
+        tbk::GemmArgsT<R> g = gemm_t<R>(m, m, 2 * TB, V, n, sp, Y, n, sp, A + (size_t)s * n + s, n, ss, R(-1), R(1));
+        g.tri = 8;
+        launch_gemm_any(g, 0, 1, nb, st);
+    }
+    tbk::launch_band_tail(b, np * TB, nb, st);
 }
 
 // Mixed-precision solve: FP64 Cholesky, inverse and reduction; FP32 panel
@@ -414,28 +473,39 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
         tbk::launch_zgemm(g, 1, 0, nb, st);
     }
     tm.end();
-    tbk::HetrdArgsT<float> h{n, ss, ws.A32, ws.P32, nullptr, ws.T32, ws.VT32, ws.d, ws.e, ws.tstride()};
-    for (int i0 = 0; i0 < n - 1; i0 += kNB) {
-        const int pw = std::min(kNB, n - 1 - i0);
-        tm.begin("hetrd_panel");
-        if (ws.wide) {
-            for (int i = 0; i < pw; ++i) tbk::launch_hetrd_wide_column(h, ws.wide32(), i0, i, nb, st);
-            tbk::launch_hetrd_wide_vt(h, i0, nb, st);
-        } else {
-            tbk::launch_hetrd_panel(h, i0, nb, st);
-        }
+    if (ws.two_stage) {
+        // band in Y32, stage-2 reflectors in X32 (scratch of stage 1 before)
+        tm.begin("hetrd_band");
+        band_reduce<float>(n, nb, ws.A32, ws.P32, sp, ws.T32, ws.tstride(), ws.VT32, ws.Y32, ss, ws.X32, ss, st);
         tm.end();
-        const int s = i0 + pw, m = n - s;
+        tm.begin("hetrd_bulge");
+        tbk::launch_bulge(tbk::BulgeArgsT<float>{n, ws.Y32, ss, ws.X32, ss, ws.d, ws.e, (size_t)n, ws.px, (size_t)n},
+                          nb, st);
+        tm.end();
+    } else {
+        tbk::HetrdArgsT<float> h{n, ss, ws.A32, ws.P32, nullptr, ws.T32, ws.VT32, ws.d, ws.e, ws.tstride()};
+        for (int i0 = 0; i0 < n - 1; i0 += kNB) {
+            const int pw = std::min(kNB, n - 1 - i0);
+            tm.begin("hetrd_panel");
+            if (ws.wide) {
+                for (int i = 0; i < pw; ++i) tbk::launch_hetrd_wide_column(h, ws.wide32(), i0, i, nb, st);
+                tbk::launch_hetrd_wide_vt(h, i0, nb, st);
+            } else {
+                tbk::launch_hetrd_panel(h, i0, nb, st);
+            }
+            tm.end();
+            const int s = i0 + pw, m = n - s;
+            tm.begin("hetrd_update");
+            if (m > 0)
+                tbk::launch_cgemm(cgemm(m, m, 2 * kNB, ws.P32 + s, n, sp, ws.P32 + (size_t)kNB * n + s, n, sp,
+                                        ws.A32 + (size_t)s * n + s, n, ss, 4, float2{-1.f, 0.f},
+                                        float2{1.f, 0.f}), 0, 1, nb, st);
+            tm.end();
+        }
         tm.begin("hetrd_update");
-        if (m > 0)
-            tbk::launch_cgemm(cgemm(m, m, 2 * kNB, ws.P32 + s, n, sp, ws.P32 + (size_t)kNB * n + s, n, sp,
-                                    ws.A32 + (size_t)s * n + s, n, ss, 4, float2{-1.f, 0.f},
-                                    float2{1.f, 0.f}), 0, 1, nb, st);
+        tbk::launch_hetrd_finish(h, nb, st);
         tm.end();
     }
-    tm.begin("hetrd_update");
-    tbk::launch_hetrd_finish(h, nb, st);
-    tm.end();
     tm.begin("stein");
     tbk::SteinArgs sa{n, kOrtolRel, nact, ws.d, ws.e, ws.w, ws.Vt, ws.scratch, ws.sinfo};
     sa.abstol_rel = kMixedBisectTol;
@@ -445,8 +515,20 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
     // Y in FP32, back-transformed; its FP64 copy for the refinement comes
     // from the conversion (row 0, which no reflector touches) and from the
     // last back-transform GEMM's epilogue (rows 1..)
-    tbk::launch_real_to_c32_t(n, ws.Vt, ws.Y32, ss, nb, st, ws.Y);
-    apply_q<float>(n, nb, ws.A32, ws.VT32, ws.Y32, ws.X32, ws.P32, sp, st, nullptr, ws.Y);
+    if (ws.two_stage) {
+        // rows of the real tridiagonal's vectors times the phases, then the
+        // stage-2 reflectors (T factors in Z's storage, unused until the
+        // refinement), FP64 copies of rows 1 .. TB-1 (no stage-1 reflector
+        // reaches them), then the stage-1 panels
+        tbk::launch_real_to_c32_t(n, ws.Vt, ws.Y32, ss, nb, st, ws.Y, ws.px);
+        tbk::launch_q2(tbk::Q2ArgsT<float>{n, ws.Y32, ss, ws.X32, ss, reinterpret_cast<float2 *>(ws.Z), ss, nullptr},
+                       nb, st);
+        tbk::launch_rows_c32_to_c64(n, 1, std::min(TB, n), ws.Y32, ws.Y, ss, nb, st);
+        apply_q<float>(n, nb, ws.A32, ws.VT32, ws.Y32, ws.X32, ws.P32, sp, st, nullptr, ws.Y, band_stage_panels(n));
+    } else {
+        tbk::launch_real_to_c32_t(n, ws.Vt, ws.Y32, ss, nb, st, ws.Y);
+        apply_q<float>(n, nb, ws.A32, ws.VT32, ws.Y32, ws.X32, ws.P32, sp, st, nullptr, ws.Y, one_stage_panels(n));
+    }
     tm.end();
     tm.begin("refine");
     // X = C Y, Z = Y^H X (= Y^H C Y), VT = Y^H Y (Hermitian: lower triangles
@@ -482,7 +564,20 @@ void solve_batch(Workspace &ws, int nb, const int *nact, double *w_out, hipStrea
     tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.S, n, ss, ws.Y, n, ss, 3), 0, 0, nb, st);
     tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.A, n, ss, 6), 1, 0, nb, st);
     tm.end();
-    hetrd(ws, nb, st, tm);
+    if (ws.two_stage) {
+        // band in Y, stage-2 reflectors in Z (both free until the back-transform)
+        const size_t sp = (size_t)n * 96;
+        tm.begin("hetrd_band");
+        band_reduce<double>(n, nb, ws.A, ws.Vp, sp, ws.Tf, ws.tstride(), ws.VT, ws.Y, ss, ws.Wp, (size_t)n * kNB,
+                            st);
+        tm.end();
+        tm.begin("hetrd_bulge");
+        tbk::launch_bulge(tbk::BulgeArgsT<double>{n, ws.Y, ss, ws.Z, ss, ws.d, ws.e, (size_t)n, ws.px, (size_t)n},
+                          nb, st);
+        tm.end();
+    } else {
+        hetrd(ws, nb, st, tm);
+    }
     tm.begin("stein");
     tbk::SteinArgs sa{n, kOrtolRel, nact, ws.d, ws.e, w_out, ws.Vt, ws.scratch, ws.sinfo};
     sa.abstol_rel = kFp64BisectTol;
@@ -494,8 +589,20 @@ void solve_batch(Workspace &ws, int nb, const int *nact, double *w_out, hipStrea
     tbk::launch_stein_vectors(sa, nb, st);
     tm.end();
     tm.begin("backtransform");
-    tbk::launch_real_to_complex_t(n, ws.Vt, ws.Y, ss, nb, st);
-    apply_q<double>(n, nb, ws.A, ws.VT, ws.Y, ws.X, ws.Vp, (size_t)n * 96, st, wnd.cut);
+    if (ws.two_stage) {
+        // phases, the stage-2 reflectors (T factors in X, scratch of the
+        // panels' back-transform afterwards), the stage-1 panels
+        tbk::launch_real_to_complex_t(n, ws.Vt, ws.Y, ss, nb, st, ws.px);
+        tbk::Q2ArgsT<double> q{n, ws.Y, ss, ws.Z, ss, ws.X, ss};
+        q.ncol = wnd.cut;
+        tbk::launch_q2(q, nb, st);
+        apply_q<double>(n, nb, ws.A, ws.VT, ws.Y, ws.X, ws.Vp, (size_t)n * 96, st, wnd.cut, nullptr,
+                        band_stage_panels(n));
+    } else {
+        tbk::launch_real_to_complex_t(n, ws.Vt, ws.Y, ss, nb, st);
+        apply_q<double>(n, nb, ws.A, ws.VT, ws.Y, ws.X, ws.Vp, (size_t)n * 96, st, wnd.cut, nullptr,
+                        one_stage_panels(n));
+    }
     // Z = Uinv Y
     tbk::launch_zgemm(with_ncol(gemm(n, n, n, ws.S, n, ss, ws.Y, n, ss, ws.Z, n, ss, 1), wnd.cut), 0, 0, nb, st);
     tm.end();
@@ -845,7 +952,10 @@ static int plan_create_from(int device, int nstruct, const tbk_prepared *const *
     }
     if (!rc) rc = dalloc(&p->ph, (size_t)p->batch * maxpairs);
     if (!rc) rc = p->ws.alloc(N, p->batch);
-    if (!rc && precision == TBK_MIXED) rc = p->ws.alloc_mixed();
+    if (!rc && precision == TBK_MIXED) {
+        rc = p->ws.alloc_mixed();
+        p->ws.two_stage = false;
+    }
     // Few items of large order: one workgroup per item would leave most CUs
     // idle in the tridiagonalisation, so it runs over tiles instead.
     if (!rc && N >= kWideMinN && p->batch < kWideMaxBatch) rc = p->ws.alloc_wide(precision == TBK_MIXED);
@@ -1073,6 +1183,7 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
             dfree(p->proj);
             return TBK_ERR_NOMEM;
         }
+        p->ws2.two_stage = p->ws.two_stage;
         p->bytes += g_dev_bytes - before;
     }
     p->last_mode = mode;
@@ -1327,6 +1438,17 @@ extern "C" int tbk_plan_projections(tbk_plan *p, int k, double eigcut, int *ntot
         HIP_TRY(hipMemcpy2DAsync(proj, nv * sizeof(double), p->proj + (size_t)k * n * n, n * sizeof(double),
                                  nv * sizeof(double), m, hipMemcpyDeviceToHost, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
+    return TBK_OK;
+}
+
+extern "C" int tbk_plan_set_tridiag(tbk_plan *p, int stages)
+{
+    if (!p || stages < 0 || stages > 2) return TBK_ERR_ARG;
+    const bool two = stages == 0 ? p->N >= kTwoStageMinN && p->precision == TBK_FP64 : stages == 2;
+    if (two && !two_stage_fits(p->N)) return TBK_ERR_ARG;
+    p->ws.two_stage = two;
+    p->ws2.two_stage = two;
+    if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
     return TBK_OK;
 }
 

@@ -141,6 +141,11 @@ class Plan:
     def set_graph(self, on: bool = True):
         check(lib().tbk_plan_set_graph(self.h, int(on)), "tbk_plan_set_graph")
 
+    def set_tridiag(self, stages: int):
+        """1: one-stage Householder panels; 2: two-stage (band + bulge
+        chasing); 0: the default (two-stage for FP64 from order 512)."""
+        check(lib().tbk_plan_set_tridiag(self.h, int(stages)), "tbk_plan_set_tridiag")
+
     def set_window(self, on: bool = True):
         """Occupation window (default on): pressure runs with every k in one
         batch form eigenvectors and derivatives only below E_F + 20 kT."""

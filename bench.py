@@ -146,12 +146,14 @@ def cpu_baseline(st, kpts, weights, seconds=12.0):
             "host_cpus_visible": os.cpu_count(), "cpu_model": cpu_model()}
 
 
-def run_batched(st, kpts, w, prec, steps, warmup, autotune=True):
+def run_batched(st, kpts, w, prec, steps, warmup, autotune=True, tridiag=0):
     """All k-points in one batch; returns timing, stage times, results and
     the plan creation time (host set-up, uploads, allocation)."""
     t0 = time.perf_counter()
     plan = Plan(st, kpts, w, batch=0, precision=prec)
     create_ms = (time.perf_counter() - t0) * 1e3
+    if tridiag:
+        plan.set_tridiag(tridiag)
     tune = {"shapes": 0, "seconds": 0.0}
     if autotune:                  # GEMM tile autotuner (outside the timed region)
         t0 = time.perf_counter()
@@ -243,7 +245,7 @@ def serve_stream(rates, nreq, precision, seed=20261203, inflight=8):
     return out
 
 
-def profile_pass(precision):
+def profile_pass(precision, tridiag=0):
     """The GEMM autotuning pass, one warm-up and one batched C2 step, then a
     1 GiB copy kernel (read 1 GiB, write 1 GiB) to calibrate the traffic
     counters."""
@@ -251,6 +253,8 @@ def profile_pass(precision):
     kpts = S.monkhorst_pack((8, 8, 8))
     w = np.full(len(kpts), 1.0 / len(kpts))
     plan = Plan(st, kpts, w, batch=0, precision=TBK_MIXED if precision == "mixed" else TBK_FP64)
+    if tridiag:
+        plan.set_tridiag(tridiag)
     plan.autotune(3)                  # the same GEMM tiles as the timed bench
     plan.run(4, 0.005, 10.0)
     r = plan.run(4, 0.005, 10.0)
@@ -366,13 +370,15 @@ def main():
                     help="skip the 108-atom and uneven-batch workloads the default line carries as "
                          "extra keys c3 and c4")
     ap.add_argument("--profile-pass", choices=["fp64", "mixed"], default=None)
+    ap.add_argument("--tridiag", type=int, choices=[0, 1, 2], default=0,
+                    help="tridiagonalisation of the C2 plans: 0 the library's choice, 1 one-stage, 2 two-stage")
     ap.add_argument("--workload", choices=["c2", "c3", "c4", "spin", "serve"], default="c2",
                     help="c2: the headline line; c3 (N=972, 64 k), c4 (256 uneven "
                          "structures), spin (C2 spin-polarised and non-collinear) and serve (the "
                          "request stream sweep) print their own lines")
     args = ap.parse_args()
     if args.profile_pass:
-        profile_pass(args.profile_pass)
+        profile_pass(args.profile_pass, args.tridiag)
         return
     if args.workload == "c3":
         print(json.dumps(bench_c3(args)))
@@ -400,7 +406,7 @@ def main():
     # ---- batched, both precisions -------------------------------------------
     runs = {}
     for name, prec in (("fp64", TBK_FP64), ("mixed", TBK_MIXED)):
-        runs[name] = run_batched(st, kpts, w, prec, args.steps, args.warmup, not args.no_autotune)
+        runs[name] = run_batched(st, kpts, w, prec, args.steps, args.warmup, not args.no_autotune, args.tridiag)
         print(f"# batched {name}: {runs[name]['ms_per_step']:.2f} ms/step, "
               f"{runs[name]['ksolves_per_s']:.1f} k-solves/s", file=sys.stderr, flush=True)
     head = runs[args.precision]

@@ -657,8 +657,13 @@ __device__ __forceinline__ R wave_sum_transposed(R (&v)[K], int lane)
         const int cnt = K >> (s + 1);
 #pragma unroll
         for (int i = 0; i < cnt; ++i) {
-            const R keep = up ? v[cnt + i] : v[i];
-            const R send = up ? v[i] : v[cnt + i];
+            // the two values as plain registers: otherwise the lane-dependent
+            // choice between them becomes an indexed read of the array
+            // (a compare/select chain over all K entries)
+            R lo = v[i], hi = v[cnt + i];
+            asm volatile("" : "+v"(lo), "+v"(hi));
+            const R keep = up ? hi : lo;
+            const R send = up ? lo : hi;
             v[i] = keep + __shfl_xor(send, off, 64);
         }
     }
@@ -2351,6 +2356,7 @@ __global__ __launch_bounds__(64 * NW) void k_band_qr(BandArgsT<R> g, int c0, int
     C *A = g.a + (size_t)kb * g.ss;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     __shared__ R red[NW][2 * TB];
+    __shared__ R red2[2 * TB];         // the partial sums combined over the waves
     __shared__ R rnorm[NW];
     __shared__ C alpha_sh;
     __shared__ C Tm[TB][TB + 1];
@@ -2373,7 +2379,8 @@ __global__ __launch_bounds__(64 * NW) void k_band_qr(BandArgsT<R> g, int c0, int
 #pragma unroll
         for (int k = 0; k < RPT; ++k) {
             const int r = tid + k * NT;
-            if (r > j && r < m) nrm += p[k][0].x * p[k][0].x + p[k][0].y * p[k][0].y;
+            const R x2 = p[k][0].x * p[k][0].x + p[k][0].y * p[k][0].y;
+            nrm += r > j ? x2 : R(0);
         }
         if (tid == j) alpha_sh = p[0][0];
         nrm = wsum(nrm);
@@ -2387,36 +2394,35 @@ __global__ __launch_bounds__(64 * NW) void k_band_qr(BandArgsT<R> g, int c0, int
         R beta = alpha.x;
         const bool have = j < m - 1;            // rows j and j + 1 exist
         if (have) larfg_coef<R, C>(alpha, xn2, tau, beta, scale);
-        // v (rows > j scaled in place), R(j, j) = beta
+        // v (rows > j scaled in place; zero above row j, and rows >= m hold
+        // zeros throughout), R(j, j) = beta
         C vr[RPT];
+        const C sc = have ? scale : one;
 #pragma unroll
         for (int k = 0; k < RPT; ++k) {
             const int r = tid + k * NT;
             const C x = p[k][0];
-            C v = czero<C>();
-            if (r == j) v = one;
-            else if (r > j && r < m) v = have ? zmul(x, scale) : x;
+            const C v = r > j ? zmul(x, sc) : (r == j ? one : czero<C>());
             vr[k] = v;
-            p[k][0] = (r == j) ? (have ? cmake<C, R>(beta, R(0)) : alpha) : ((r > j && r < m) ? v : x);
+            p[k][0] = r == j ? (have ? cmake<C, R>(beta, R(0)) : alpha) : (r > j ? v : x);
         }
         // partial dots by slot q (column c = (q + j) mod TB): the earlier
         // reflectors (q >= TB - j): conj(V[r, c]) v[r]; the later columns
-        // (1 <= q < TB - j): conj(v[r]) P[r, c]; two halves of TB / 2 slots
+        // (1 <= q < TB - j): conj(v[r]) P[r, c]; two halves of TB / 2 slots.
+        // v is zero above row j, so every row can enter every sum.
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             R part[TB];
 #pragma unroll
             for (int qq = 0; qq < TB / 2; ++qq) {
                 const int q = h * (TB / 2) + qq;
-                C acc = czero<C>();
+                C ea = czero<C>(), eb = czero<C>();
 #pragma unroll
                 for (int k = 0; k < RPT; ++k) {
-                    const int r = tid + k * NT;
-                    if (r >= j && r < m) {
-                        if (q >= TB - j) acc = zadd(acc, zcmul(p[k][q], vr[k]));
-                        else if (q >= 1) acc = zadd(acc, zcmul(vr[k], p[k][q]));
-                    }
+                    ea = zadd(ea, zcmul(p[k][q], vr[k]));
+                    eb = zadd(eb, zcmul(vr[k], p[k][q]));
                 }
+                const C acc = q >= TB - j ? ea : (q >= 1 ? eb : czero<C>());
                 part[2 * qq] = acc.x;
                 part[2 * qq + 1] = acc.y;
             }
@@ -2427,30 +2433,31 @@ __global__ __launch_bounds__(64 * NW) void k_band_qr(BandArgsT<R> g, int c0, int
             }
         }
         __syncthreads();
-        // update of the later columns: P[r, c] -= conj(tau) v[r] w[c]
+        if (NW > 1) {
+            if (tid < 2 * TB) {
+                R t = 0;
+#pragma unroll
+                for (int u = 0; u < NW; ++u) t += red[u][tid];
+                red2[tid] = t;
+            }
+            __syncthreads();
+        }
+        const R *rs = NW > 1 ? red2 : red[0];
+        // update of the later columns: P[r, c] -= conj(tau) v[r] w[c] (rows
+        // above j have v = 0; the earlier reflectors' slots take f = 0)
         const C ctau = zconj(tau);
 #pragma unroll
         for (int q = 1; q < TB; ++q) {
-            if (q >= TB - j) continue;
-            const int c = q + j;
-            C w = czero<C>();
+            const int c = (q + j) & (TB - 1);
+            const C w = cmake<C, R>(rs[2 * c], rs[2 * c + 1]);
+            const C f = q < TB - j ? zmul(ctau, w) : czero<C>();
 #pragma unroll
-            for (int u = 0; u < NW; ++u) w = zadd(w, cmake<C, R>(red[u][2 * c], red[u][2 * c + 1]));
-            const C f = zmul(ctau, w);
-#pragma unroll
-            for (int k = 0; k < RPT; ++k) {
-                const int r = tid + k * NT;
-                if (r >= j && r < m) p[k][q] = zsub(p[k][q], zmul(vr[k], f));
-            }
+            for (int k = 0; k < RPT; ++k) p[k][q] = zsub(p[k][q], zmul(vr[k], f));
         }
         // column j of T: T[l, j] = -tau sum_{q=l}^{j-1} T[l, q] (V^H v)[q]
         if (tid < j) {
             C acc = czero<C>();
-            for (int q = tid; q < j; ++q) {
-                C gq = czero<C>();
-                for (int u = 0; u < NW; ++u) gq = zadd(gq, cmake<C, R>(red[u][2 * q], red[u][2 * q + 1]));
-                acc = zadd(acc, zmul(Tm[tid][q], gq));
-            }
+            for (int q = tid; q < j; ++q) acc = zadd(acc, zmul(Tm[tid][q], cmake<C, R>(rs[2 * q], rs[2 * q + 1])));
             Tm[tid][j] = zmul(cmake<C, R>(-tau.x, -tau.y), acc);
         }
         if (tid == j) Tm[j][j] = tau;
@@ -2545,6 +2552,17 @@ __device__ __forceinline__ void bulge_step(C *band, int n, int i, int s, C (*slo
     const C one = cmake<C, R>(R(1), R(0));
     C tau = czero<C>(), va = czero<C>();
     R beta = 0;
+    // the diagonal block D = A[r0:, r0:] (Hermitian, from the lower band),
+    // loaded first: nothing below writes it before its own update
+    C D[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = 4 * cq + q;
+        C dv = czero<C>();
+        if (a < L && c < L)
+            dv = c <= a ? band[(size_t)(r0 + c) * TBLD + a - c] : zconj(band[(size_t)(r0 + a) * TBLD + c - a]);
+        D[q] = dv;
+    }
     if (s == 0) {
         // column i, rows r0 .. r0 + L - 1 (L >= 2)
         const C x = (cq == 0 && a < L) ? band[(size_t)i * TBLD + 1 + a] : czero<C>();
@@ -2618,16 +2636,9 @@ __device__ __forceinline__ void bulge_step(C *band, int n, int i, int s, C (*slo
     }
     // D <- H^H D H: p = D v, w = tau p - 1/2 |tau|^2 (v^H D v) v,
     // D -= v w^H + w v^H
-    C D[4], vc[4];
+    C vc[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int c = 4 * cq + q;
-        C dv = czero<C>();
-        if (a < L && c < L)
-            dv = c <= a ? band[(size_t)(r0 + c) * TBLD + a - c] : zconj(band[(size_t)(r0 + a) * TBLD + c - a]);
-        D[q] = dv;
-        vc[q] = cshfl(va, c);
-    }
+    for (int q = 0; q < 4; ++q) vc[q] = cshfl(va, 4 * cq + q);
     C pp = czero<C>();
 #pragma unroll
     for (int q = 0; q < 4; ++q) pp = zadd(pp, zmul(D[q], vc[q]));
@@ -2765,138 +2776,161 @@ __global__ __launch_bounds__(64) void k_q2_tfactors(Q2ArgsT<R> g)
     for (int e = lane; e < TB * TB; e += 64) T[e] = Tm[e % TB][e / TB];     // column-major
 }
 
-// Y <- Q2 Y, 16 columns per wave, every wave on its own (no LDS, no
-// barriers): diamonds by block j descending, step s ascending; each is
+// Y <- Q2 Y: diamonds by block j descending, step s ascending; each is
 //   W = V^H Y_rows (16 x 16),  W <- T W,  Y_rows -= V W  (32 rows)
-// on the matrix cores, complex products as three real ones.  The K index of
-// every product is laid out like the rows of an accumulator (slot q of lane
-// group g4 is row mfma_row(q)), so each accumulator is the next product's B
-// operand in the same lane, and the 32 rows of Y a wave holds (two halves of
-// 16) are both B operand and update target.  Consecutive diamonds of a
-// block overlap by 16 rows: the lower half moves up, one new half is loaded.
+// on the matrix cores, complex products as three real ones.  A workgroup
+// stages each diamond's V (rows padded to 32) and T in LDS, double-buffered
+// (one barrier per diamond); every wave holds kQ2Tiles tiles of 16 columns
+// of Y.  The K index of every product is laid out like the rows of an
+// accumulator (slot q of lane group g4 is row mfma_row(q)), so each
+// accumulator is the next product's B operand in the same lane, and the 32
+// rows of Y a wave holds per tile (two halves of 16) are both B operand and
+// update target.  Consecutive diamonds of a block overlap by 16 rows: the
+// lower half moves up and one new half is loaded.
 constexpr int kQ2Waves = 4;
-constexpr int kQ2Tiles = 2;               // 16-column tiles per wave (sharing the V and T operands)
+constexpr int kQ2Tiles = 2;
 template <class R>
-__global__ __launch_bounds__(64 * kQ2Waves) void k_apply_q2(Q2ArgsT<R> g)
+__global__ __launch_bounds__(64 * kQ2Waves, 2) void k_apply_q2(Q2ArgsT<R> g)
 {
     typedef typename cplx<R>::type C;
     typedef R v4 __attribute__((ext_vector_type(4)));
     constexpr int NT = kQ2Tiles;
-    const int n = g.n, kb = blockIdx.y, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __shared__ C Vs[2][32][TB + 1];
+    __shared__ C Ts[2][TB][TB + 1];
+    const int n = g.n, kb = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int ncols = g.ncol ? min(g.ncol[kb], n) : n;
-    const int wc0 = (blockIdx.x * kQ2Waves + wv) * 16 * NT;
-    if (wc0 >= ncols) return;                            // per wave: nothing below synchronises
+    const int col0 = blockIdx.x * kQ2Waves * 16 * NT;
+    if (col0 >= ncols) return;                           // uniform over the workgroup
+    const int wc0 = col0 + wv * 16 * NT;
     const int r16 = lane & 15;
     bool colok[NT];
     C *Y[NT];
 #pragma unroll
     for (int u = 0; u < NT; ++u) {
         colok[u] = wc0 + 16 * u + r16 < ncols;
-        Y[u] = g.y + (size_t)kb * g.sy + (size_t)(wc0 + 16 * u + r16) * n;
+        Y[u] = g.y + (size_t)kb * g.sy + (size_t)min(wc0 + 16 * u + r16, n - 1) * n;
     }
     const C *V2 = g.v2 + (size_t)kb * g.sv;
     const C *T2 = g.t2 + (size_t)kb * g.st2;
     int rm[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) rm[q] = mfma_row(R(0), lane, q);
-    const C one = cmake<C, R>(R(1), R(0));
-    // V[k][c] of a diamond block: reflector c's entry k - c (1 on the diagonal)
-    auto vel = [&](const C *blk, int k, int c) -> C {
-        const int q = k - c;
-        if (q == 0) return one;
-        if (q < 0 || q >= TB) return czero<C>();
-        return blk[c * TB + q];
-    };
     const int nj = (n - 3) / TB + 1;
-    int dend = 0;
-    for (int j = 0; j < nj; ++j) dend += bulge_steps(n, j * TB);
-    for (int j = nj - 1; j >= 0; --j) {
-        const int ns = bulge_steps(n, j * TB), d0 = dend - ns;
-        dend = d0;
-        int rlo = j * TB + 1;
-        C y[NT][2][4];
+    int dtot = 0;
+    for (int j = 0; j < nj; ++j) dtot += bulge_steps(n, j * TB);
+    // V[r][c] = v_c[r - c] (1 on the diagonal, 0 outside the staircase): the
+    // zeros are written once, each diamond then fills the staircase; thread
+    // e = c TB + q (256 threads for the TB^2 entries) moves entry q of
+    // reflector c and entry e of T, loaded one diamond ahead
+    static_assert(64 * kQ2Waves == TB * TB, "one V and one T entry per thread");
+    for (int e = tid; e < 2 * 32 * (TB + 1); e += 64 * kQ2Waves) (&Vs[0][0][0])[e] = czero<C>();
+    __syncthreads();
+    const int sc = tid / TB, sq = tid % TB;
+    auto put = [&](int b, C pv, C pt) {
+        Vs[b][sc + sq][sc] = sq == 0 ? cmake<C, R>(R(1), R(0)) : pv;
+        Ts[b][sq][sc] = pt;
+    };
+    // diamonds in application order: j descending, s ascending
+    int j = nj - 1, s = 0, d0 = dtot - bulge_steps(n, j * TB), ns = bulge_steps(n, j * TB);
+    put(0, V2[(size_t)d0 * TB * TB + tid], T2[(size_t)d0 * TB * TB + tid]);
+    C y[NT][2][4];
+    int it = 0;
+    while (true) {
+        const int rlo = j * TB + 1 + s * TB;
+        if (s == 0) {                                    // a new block: load its first 32 rows
 #pragma unroll
-        for (int u = 0; u < NT; ++u)
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int r = rlo + 16 * h + rm[q];
-                    y[u][h][q] = (colok[u] && r < n) ? Y[u][r] : czero<C>();
-                }
-        for (int s = 0; s < ns; ++s, rlo += TB) {
-            const C *blk = V2 + (size_t)(d0 + s) * TB * TB;
-            const C *tb = T2 + (size_t)(d0 + s) * TB * TB;
-            C va[2][4], vb[2][4], tt[4];
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    va[h][q] = vel(blk, 16 * h + rm[q], r16);     // V[k][c], k = 16 h + rm[q], c = r16
-                    vb[h][q] = vel(blk, 16 * h + r16, rm[q]);     // V[16 h + r16][rm[q]]
-                }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) tt[q] = tb[rm[q] * TB + r16];   // T[r16][rm[q]] (column-major)
-#pragma unroll
-            for (int u = 0; u < NT; ++u) {
-                // W = V^H Y
-                v4 a1 = {0, 0, 0, 0}, a2 = {0, 0, 0, 0}, a3 = {0, 0, 0, 0};
+            for (int u = 0; u < NT; ++u)
 #pragma unroll
                 for (int h = 0; h < 2; ++h)
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        const R ar = va[h][q].x, ai = -va[h][q].y;
-                        a1 = mfma16(ar, y[u][h][q].x, a1);
-                        a2 = mfma16(ai, y[u][h][q].y, a2);
-                        a3 = mfma16(ar + ai, y[u][h][q].x + y[u][h][q].y, a3);
+                        const int r = rlo + 16 * h + rm[q];
+                        y[u][h][q] = (colok[u] && r < n) ? Y[u][r] : czero<C>();
                     }
-                C w[4];
+        }
+        // the next diamond (position in the order): its V and T entries and
+        // this block's next 16 rows of Y are loaded now, used after the products
+        int jn = j, sn = s + 1, d0n = d0, nsn = ns;
+        if (sn == ns) { jn = j - 1; sn = 0; if (jn >= 0) { nsn = bulge_steps(n, jn * TB); d0n = d0 - nsn; } }
+        C pv = czero<C>(), pt = czero<C>();
+        if (jn >= 0) {
+            pv = V2[(size_t)(d0n + sn) * TB * TB + tid];
+            pt = T2[(size_t)(d0n + sn) * TB * TB + tid];
+        }
+        const bool last = s + 1 == ns;
+        C ynext[NT][4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) w[q] = cmake<C, R>(a1[q] - a2[q], a3[q] - a1[q] - a2[q]);
-                // W <- T W
+        for (int u = 0; u < NT; ++u)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rn = rlo + 32 + rm[q];
+                ynext[u][q] = (!last && colok[u] && rn < n) ? Y[u][rn] : czero<C>();
+            }
+        __syncthreads();                                 // buffer it & 1 staged; buffer (it+1) & 1 free
+        const int b = it & 1;
+#pragma unroll
+        for (int u = 0; u < NT; ++u) {
+            // W = V^H Y
+            v4 a1 = {0, 0, 0, 0}, a2 = {0, 0, 0, 0}, a3 = {0, 0, 0, 0};
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const C v = Vs[b][16 * h + rm[q]][r16];
+                    const R ar = v.x, ai = -v.y;
+                    a1 = mfma16(ar, y[u][h][q].x, a1);
+                    a2 = mfma16(ai, y[u][h][q].y, a2);
+                    a3 = mfma16(ar + ai, y[u][h][q].x + y[u][h][q].y, a3);
+                }
+            C w[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) w[q] = cmake<C, R>(a1[q] - a2[q], a3[q] - a1[q] - a2[q]);
+            // W <- T W
+            a1 = v4{0, 0, 0, 0}; a2 = v4{0, 0, 0, 0}; a3 = v4{0, 0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const C t = Ts[b][r16][rm[q]];
+                a1 = mfma16(t.x, w[q].x, a1);
+                a2 = mfma16(t.y, w[q].y, a2);
+                a3 = mfma16(t.x + t.y, w[q].x + w[q].y, a3);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) w[q] = cmake<C, R>(a1[q] - a2[q], a3[q] - a1[q] - a2[q]);
+            // Y_rows -= V W
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
                 a1 = v4{0, 0, 0, 0}; a2 = v4{0, 0, 0, 0}; a3 = v4{0, 0, 0, 0};
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    a1 = mfma16(tt[q].x, w[q].x, a1);
-                    a2 = mfma16(tt[q].y, w[q].y, a2);
-                    a3 = mfma16(tt[q].x + tt[q].y, w[q].x + w[q].y, a3);
+                    const C v = Vs[b][16 * h + r16][rm[q]];
+                    a1 = mfma16(v.x, w[q].x, a1);
+                    a2 = mfma16(v.y, w[q].y, a2);
+                    a3 = mfma16(v.x + v.y, w[q].x + w[q].y, a3);
                 }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) w[q] = cmake<C, R>(a1[q] - a2[q], a3[q] - a1[q] - a2[q]);
-                // Y_rows -= V W
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    a1 = v4{0, 0, 0, 0}; a2 = v4{0, 0, 0, 0}; a3 = v4{0, 0, 0, 0};
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        a1 = mfma16(vb[h][q].x, w[q].x, a1);
-                        a2 = mfma16(vb[h][q].y, w[q].y, a2);
-                        a3 = mfma16(vb[h][q].x + vb[h][q].y, w[q].x + w[q].y, a3);
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        y[u][h][q] = cmake<C, R>(y[u][h][q].x - (a1[q] - a2[q]), y[u][h][q].y - (a3[q] - a1[q] - a2[q]));
-                }
+                for (int q = 0; q < 4; ++q)
+                    y[u][h][q] = cmake<C, R>(y[u][h][q].x - (a1[q] - a2[q]), y[u][h][q].y - (a3[q] - a1[q] - a2[q]));
             }
-            // rows rlo .. rlo+15 are done for this block; the lower half moves up
-#pragma unroll
-            for (int u = 0; u < NT; ++u)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int r = rlo + rm[q];
-                    if (colok[u] && r < n) Y[u][r] = y[u][0][q];
-                    y[u][0][q] = y[u][1][q];
-                    const int rn = rlo + 32 + rm[q];
-                    y[u][1][q] = (s + 1 < ns && colok[u] && rn < n) ? Y[u][rn] : czero<C>();
-                }
         }
+        // rows rlo .. rlo+15 are done for this block; the lower half moves up
+        // (or, after the block's last diamond, is stored too)
 #pragma unroll
         for (int u = 0; u < NT; ++u)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int r = rlo + rm[q];
                 if (colok[u] && r < n) Y[u][r] = y[u][0][q];
+                if (last) {
+                    if (colok[u] && r + 16 < n) Y[u][r + 16] = y[u][1][q];
+                } else {
+                    y[u][0][q] = y[u][1][q];
+                    y[u][1][q] = ynext[u][q];
+                }
             }
+        if (jn >= 0) put((it + 1) & 1, pv, pt);
+        if (jn < 0) break;
+        j = jn; s = sn; d0 = d0n; ns = nsn;
+        ++it;
     }
 }
 

@@ -260,7 +260,7 @@ int tbk_skeng_line(const char *label, double thisvol, double efermi,
  * structure of dir/SKIN through the device plans (modes 1-4; jspins 1 and 2,
  * collinear and non-collinear), its line written to dir/SKENG as each one
  * finishes.  Stops at the first structure the reference would stop on, or
- * the device cannot run (N = 1, torques and spin directions out), with the
+ * the device cannot run (torques and spin directions out), with the
  * lines of the structures before it written.  *nstruct: structures done. */
 int tbk_static_run(const char *dir, int device, int precision, int *nstruct);
 

@@ -1052,13 +1052,28 @@ int orc_bande(int mode, int nv, int natoms, const int *kkind, const int *ksk,
                    tt, dlv, jkind, jatm, hop, over, paros, 1, 1, hp, sp);
         orc_unpack(nv, hp, hf);
         orc_unpack(nv, sp, sf);
-        int info = orc_zhegv(nv, hf, sf, esk + (size_t)k * nv, zf);
+        int info;
+        if (nv == 1) {
+            /* diag.f:320-349: the 1 x 1 pencil in closed form, e = Re h / Re s
+             * (s <= 0 flags ioverr); no eigenvector is formed, so perturb
+             * sees zeig = 0 and the derivative is zero (ncut counts the
+             * state when it lies below eigcut) */
+            const double h1 = hp[0], s1 = sp[0];
+            info = s1 > 0.0 ? 0 : 1;
+            esk[(size_t)k * nv] = h1 / s1;
+            if (!info && lpress) {
+                desk[(size_t)k * nv] = 0.0;
+                ncut[k] = esk[(size_t)k * nv] > eigcut ? 0 : 1;
+            }
+        } else {
+            info = orc_zhegv(nv, hf, sf, esk + (size_t)k * nv, zf);
+        }
         if (info) {
 #ifdef _OPENMP
 #pragma omp atomic write
 #endif
             err = 3;
-        } else if (lpress) {
+        } else if (lpress && nv > 1) {
             orc_setham(nv, kpts + 3 * k, natoms, kkind, ksk, kbas, posq, npairs,
                        tt, dlv, jkind, jatm, dhop, dover, dparos, 1, 0, hp, sp);
             if (orc_perturb(nv, hp, sp, zf, esk + (size_t)k * nv, eigcut,

@@ -17,6 +17,8 @@ pd_harrison    Pd-like parameters with nltype 2 (Harrison signs only)
 h_atom         s-only atoms (kbas = 1) with nltype 90000, the H-atom form of
                both integrals (setpar.f:615-628, 693-705), whose volume
                derivative takes the standard form (setparv.f:640 tests 9000)
+h_atom_single  one such atom in the cell: order N = 1, the 1 x 1 pencil
+               (zhpgv with n = 1 as diag.f:396-412 calls it)
 orthogonal     an orthogonal basis, jsover false: S = I, zhpev in place of
                zhpgv (diag.f:385-414, setham.f:399-453)
 degenerate     the perfect 32-atom fcc supercell at Gamma: groups of 12 equal
@@ -112,6 +114,8 @@ def cases(c_par: dict):
     lat, pos = _cell("fcc", 5.0)
     out.append(dict(name="h_atom", params=_h_atom_params(), lat=lat, posn=pos, kkind=[1, 1, 1, 1], q=4,
                     expect="ok"))
+    out.append(dict(name="h_atom_single", params=_h_atom_params(), lat=S.fcc_primitive(5.0).tolist(),
+                    posn=[[0.0, 0.0, 0.0]], kkind=[1], q=6, expect="ok"))
     lat, pos = _cell("fcc", 3.61 * S.ANGSTROM, (2, 2, 2))
     out.append(dict(name="degenerate", params=_params_dict(S.synthetic_params(SEED)), lat=lat, posn=pos,
                     kkind=[1] * 32, q=1, expect="degen"))

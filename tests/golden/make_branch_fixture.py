@@ -11,7 +11,8 @@ Data only, from the reference's sources compiled where they lie
 * per case with expect "ok": E_F, earg1, earg2 of the reference's bande in
   modes 1-4 on the case's Monkhorst-Pack mesh (kT = 0.005 Ry, eigcut =
   10 Ry), and at branch_cases.GENERAL_K the packed H(k) (and S(k) for a
-  non-orthogonal basis) of its setham and the eigenvalues of its diag;
+  non-orthogonal basis) of its setham and the eigenvalues of its diag (the
+  one-orbital case runs first, as in a fresh program);
 * per case with expect "degen" / "zfill": the line the reference's program
   prints when it stops (perturb.f:106, fermi.f:165), from a child process.
 
@@ -60,7 +61,11 @@ def run_stop_case(name):
 def main():
     cp = c_par_dict()
     out = dict(c_par=cp, general_k=B.GENERAL_K, tkb=TKB, eigcut=EIGCUT, cases={})
-    for case in B.cases(cp):
+    # A one-orbital cell first: the reference's diag forms no eigenvector for
+    # a 1 x 1 pencil (diag.f:320-349), so its perturb reads the eigenvector
+    # storage as the previous structure left it -- zero only in a fresh run
+    cases = sorted(B.cases(cp), key=lambda c: c["name"] != "h_atom_single")
+    for case in cases:
         name = case["name"]
         if case["expect"] != "ok":
             p = subprocess.run([sys.executable, __file__, "--stop", name], capture_output=True, text=True,

@@ -41,7 +41,7 @@ extern "C" int tbk_multi_create(int device, int nstruct, const tbk_structure_des
         if (nk[i] <= 0) return TBK_ERR_ARG;
         k0[i + 1] = k0[i] + nk[i];
         const int nv = orbital_count(st[i]);
-        if (nv < 2) return TBK_ERR_ARG;
+        if (nv < 1) return TBK_ERR_ARG;
         groups[(nv + granularity - 1) / granularity * granularity].push_back(i);
     }
     tbk_multi *m = new tbk_multi();

@@ -748,7 +748,7 @@ static int setup_structure(const tbk_structure_desc *st, SetupArrays &A, StructH
         nv += nbas[a];
     }
     const bool noncol = sel.nc_param_dn != nullptr;
-    if (nv < 2 || (noncol ? 2 * nv : nv) > kMaxN) return TBK_ERR_ARG;
+    if (nv < 1 || (noncol ? 2 * nv : nv) > kMaxN) return TBK_ERR_ARG;
     if (noncol) {
         // bande.f:859-905 with lnoncol: integrals from channel 1, on-site
         // average and spin terms from both (setpar.f:466-485, 744-795),
@@ -893,6 +893,7 @@ static int plan_create_from(int device, int nstruct, const tbk_prepared *const *
     }
     if (precision == TBK_MIXED && N > tbk::oa_max_n()) return TBK_ERR_ARG;
     N = std::max(N, std::min(min_order, precision == TBK_MIXED ? std::min(kMaxN, tbk::oa_max_n()) : kMaxN));
+    N = std::max(N, 2);           // a one-orbital cell is solved at order 2 (one padding row)
     // ---- device state ----
     tbk_plan *p = new tbk_plan();
     g_dev_bytes = 0;
@@ -1538,7 +1539,7 @@ extern "C" int tbk_zhegv(int device, int n, const double *h, const double *s, do
 extern "C" int tbk_fermi(int device, int ldn, int nk, int nv, double el, const double *weight,
                          const double *eigen, double tkb, double zfill, double *efermi)
 {
-    if (!weight || !eigen || !efermi || nk <= 0 || nv < 2 || ldn < nv) return TBK_ERR_ARG;
+    if (!weight || !eigen || !efermi || nk <= 0 || nv < 1 || ldn < nv) return TBK_ERR_ARG;
     if (zfill <= 0.0) return TBK_ERR_ZFILL;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return TBK_ERR_NODEVICE;

@@ -421,7 +421,7 @@ extern "C" int tbk_server_submit(tbk_server *s, const tbk_structure_desc *st, in
     if (st->natoms <= 0 || !st->kkind || !st->kbas) return TBK_ERR_ARG;
     std::unique_ptr<Request> r(new Request());
     r->st.reset(new OwnedStructure(*st));
-    if (r->st->nv < 2 || r->st->nv > 1280) return TBK_ERR_ARG;
+    if (r->st->nv < 1 || r->st->nv > 1280) return TBK_ERR_ARG;
     r->kpts.assign(kpts, kpts + 3 * (size_t)nk);
     r->weights.assign(weights, weights + nk);
     r->mode = mode; r->tkb = tkb; r->eigcut = eigcut;

@@ -541,6 +541,16 @@ int tbk_zhegv_noalloc(int device, int n, const double *h, const double *s,
 /* Fermi level.  Replaces fermi(ntbd,nkptd,nkloc,istart,ifin,nv,jspins,el,
  * weight,eigen,tkb,zfill), fermi.f:1 (one process, jspins = 1).
  * eigen[k*ldn+i].  Evaluated on the device. */
+/* First-order derivatives of one k's eigenvalues on the device.  Replaces
+ * perturb(mxd,n,dhmat,dsmat,z,eig,eigcut,ntot,deig,work), perturb.f:1-2:
+ * dh, ds full n x n (complex, column-major) dH/dV, dS/dV; z the S-orthonormal
+ * eigenvectors (column j = state j); eig ascending.  Groups of eigenvalues
+ * within 1e-8 are diagonalised together (perturb.f:97-221); *ntot = states
+ * at or below eigcut, deig[0..ntot).  TBK_ERR_DEGEN where a group exceeds 9
+ * (the reference stops, perturb.f:106). */
+int tbk_perturb(int device, int n, const double *dh, const double *ds,
+                const double *z, const double *eig, double eigcut, int *ntot,
+                double *deig);
 int tbk_fermi(int device, int ldn, int nk, int nv, double el,
               const double *weight, const double *eigen, double tkb,
               double zfill, double *efermi);

@@ -248,6 +248,21 @@ def zhegv(h, s, vectors=True):
     return info, w, (z.reshape(n, n).T.copy() if vectors else None)
 
 
+def perturb(dhp, dsp, z, eig, eigcut=10.0):
+    """Oracle perturb (perturb.f:87-221): dhp, dsp packed upper (LAPACK 'U'),
+    z full ([row, col]) eigenvectors, eig ascending; returns deig[:ntot]."""
+    n = len(eig)
+    zc = np.ascontiguousarray(np.asarray(z).T, np.complex128).ravel()
+    e = np.ascontiguousarray(eig, float)
+    d = np.zeros(n)
+    nt = C.c_int(0)
+    rc = lib().orc_perturb(n, np.ascontiguousarray(dhp, np.complex128), np.ascontiguousarray(dsp, np.complex128),
+                           zc, e, float(eigcut), C.byref(nt), d)
+    if rc:
+        raise RuntimeError("oracle perturb: degenerate group larger than 9")
+    return d[:nt.value]
+
+
 class RefSetup:
     """The same structure pushed through the reference's own routines."""
 

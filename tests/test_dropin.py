@@ -27,13 +27,13 @@ def nm_defined(path):
 
 def test_adapter_exports_fortran_symbols():
     syms = nm_defined(os.path.join(LIBDIR, "libtbk_fortran.a"))
-    assert {"bande_", "fermi_", "kptgen_"} <= syms
+    assert {"bande_", "fermi_", "kptgen_", "diag_", "perturb_"} <= syms
     undefined = subprocess.run(["nm", "-u", os.path.join(LIBDIR, "libtbk_fortran.a")],
                                capture_output=True, text=True, check=True).stdout
     # the adapter reads the program's COMMON blocks and calls the C ABI
     for sym in ("codes_", "latt1_", "fazon_", "parinfo_", "cutcom_", "disply_", "lia_", "cd1_",
                 "params_", "parcom_", "tbk_plan_create", "tbk_plan_run", "tbk_fermi",
-                "lattyp_", "space_", "kmesh1_", "kmesh2_", "tbk_kptgen"):
+                "lattyp_", "space_", "kmesh1_", "kmesh2_", "tbk_kptgen", "tbk_zhegv_noalloc", "tbk_perturb"):
         assert sym in undefined, sym
 
 
@@ -61,3 +61,16 @@ def test_reference_program_reproduces_published_ex1(tmp_path):
     ours = [line.split() for line in open(work / "SKENG")]
     pub = [line.split() for line in open(os.path.join(REF, "SKENG"))]
     assert ours == pub
+
+
+@needs_ref
+def test_reference_program_links_against_diag_dropin():
+    """The reference's bande and fermi with diag_ / perturb_ from the
+    archive in place of diag.o and perturb.o."""
+    r = subprocess.run(["make", "-C", ORACLE, "dropin"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    exe = os.path.join(ORACLE, "_ref", "static_diag")
+    syms = nm_defined(exe)
+    assert {"diag_", "perturb_", "bande_", "fermi_"} <= syms
+    needed = subprocess.run(["readelf", "-d", exe], capture_output=True, text=True).stdout
+    assert "libtbk.so" in needed

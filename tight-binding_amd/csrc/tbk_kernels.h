@@ -157,6 +157,10 @@ struct PerturbArgs {
     double *desk;
     int *ncut, *err;               // ncut [k0+nb], err [k0+nb] (2: > 9 degenerate)
     const int *cut = nullptr;      // [nb] derivatives only below cut[k] (0 above); null: all
+    // an item of one active orbital takes a zero derivative, as the
+    // reference's perturb computes it on the eigenvector its closed-form
+    // 1 x 1 diag never formed (diag.f:320-349)
+    bool one_orbital_zero = false;
 };
 
 struct FermiArgs {

@@ -1928,7 +1928,7 @@ __global__ void k_perturb(PerturbArgs p)
             // above the occupation window: never summed; a group cannot
             // straddle wcut (it lies at a gap wider than the grouping tolerance)
             for (int j = start; j < start + num; ++j) desk[j] = 0.0;
-        } else if (N == 1) {
+        } else if (N == 1 && p.one_orbital_zero) {
             // a one-orbital cell: diag.f:320-349 forms no eigenvector for the
             // 1 x 1 pencil, so the reference's perturb works on a zero vector
             desk[start] = 0.0;

@@ -1154,6 +1154,7 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
             pa.w = p->esk + (size_t)k0 * n; pa.px = p->ws.px; pa.py = p->ws.py;
             pa.z = p->ws.Z; pa.x = p->ws.X; pa.y = p->ws.Y;
             pa.desk = p->desk + (size_t)k0 * n; pa.ncut = p->ncut; pa.err = p->err;
+            pa.one_orbital_zero = true;
             tbk::launch_perturb(pa, st);
             p->timer.end();
         }
@@ -1533,6 +1534,51 @@ extern "C" int tbk_zhegv(int device, int n, const double *h, const double *s, do
     }
     (void)hipStreamDestroy(st);
     ws.release();
+    return rc;
+}
+
+// perturb.f:1-221 for one k on the device: X = dH Z, Y = dS Z (GEMMs), the
+// column dots, then the degenerate groups (k_coldot, k_perturb).
+extern "C" int tbk_perturb(int device, int n, const double *dh, const double *ds, const double *z,
+                           const double *eig, double eigcut, int *ntot, double *deig)
+{
+    if (n < 1 || n > kMaxN || !dh || !ds || !z || !eig || !ntot || !deig) return TBK_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return TBK_ERR_NODEVICE;
+    if (device < 0 || device >= ndev) return TBK_ERR_ARG;
+    HIP_TRY(hipSetDevice(device));
+    Workspace ws;
+    if (ws.alloc(n, 1)) { ws.release(); return TBK_ERR_NOMEM; }
+    hipStream_t st;
+    if (hipStreamCreate(&st) != hipSuccess) { ws.release(); return TBK_ERR_HIP; }
+    const size_t ss = ws.stride();
+    int rc = TBK_OK, ncut = 0, err = 0;
+    if (hipMemcpyAsync(ws.A, dh, ss * sizeof(double2), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(ws.S, ds, ss * sizeof(double2), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(ws.Z, z, ss * sizeof(double2), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(ws.w, eig, n * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) {
+        rc = TBK_ERR_HIP;
+    } else {
+        tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.Z, n, ss, ws.X, n, ss), 0, 0, 1, st);
+        tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Z, n, ss, ws.Y, n, ss), 0, 0, 1, st);
+        tbk::launch_coldot(n, ws.Z, ws.X, ws.Y, ss, ws.px, ws.py, 1, st);
+        tbk::PerturbArgs pa;
+        pa.n = n; pa.nb = 1; pa.k0 = 0; pa.stride = ss; pa.eigcut = eigcut; pa.nact = nullptr;
+        pa.w = ws.w; pa.px = ws.px; pa.py = ws.py; pa.z = ws.Z; pa.x = ws.X; pa.y = ws.Y;
+        pa.desk = ws.d; pa.ncut = ws.info; pa.err = ws.sinfo;
+        tbk::launch_perturb(pa, st);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(deig, ws.d, n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(&ncut, ws.info, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(&err, ws.sinfo, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = TBK_ERR_HIP;
+        else if (err & 2)
+            rc = TBK_ERR_DEGEN;
+    }
+    (void)hipStreamDestroy(st);
+    ws.release();
+    if (!rc) *ntot = ncut;
     return rc;
 }
 

@@ -36,7 +36,7 @@ EXPORTS = [
     "tbk_multi_autotune", "tbk_multi_device_bytes", "tbk_multi_destroy", "tbk_plan_query",
     "tbk_server_create", "tbk_server_submit", "tbk_server_result", "tbk_server_release",
     "tbk_server_stats", "tbk_server_timing", "tbk_server_destroy",
-    "tbk_zhegv", "tbk_zhegv_noalloc", "tbk_fermi",
+    "tbk_zhegv", "tbk_zhegv_noalloc", "tbk_perturb", "tbk_fermi",
     "tbk_read_params", "tbk_skin_open", "tbk_skin_next", "tbk_skin_close", "tbk_skeng_line", "tbk_static_run",
     "tbk_error_string", "tbk_debug_stages", "tbk_debug_zgemm", "tbk_debug_copy",
     "tbk_debug_gemm_time",
@@ -180,6 +180,7 @@ def lib():
         L.tbk_plan_set_timing.argtypes = [vp, C.c_int]
         L.tbk_zhegv.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp]
         L.tbk_zhegv_noalloc.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp]
+        L.tbk_perturb.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, C.c_double, ip, vp]
         L.tbk_fermi.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, d, d, C.c_double,
                                 C.c_double, dp]
         L.tbk_debug_stages.argtypes = [C.c_int, C.c_int] + [vp] * 10

@@ -528,6 +528,19 @@ def zhegv(h: np.ndarray, s: np.ndarray, vectors: bool = True, device: int = 0):
     return w, (z.reshape(n, n).T.copy() if vectors else None)
 
 
+def perturb(dh: np.ndarray, ds: np.ndarray, z: np.ndarray, eig, eigcut: float = 10.0, device: int = 0):
+    """Device first-order derivatives of one k's eigenvalues (perturb.f:1):
+    dh, ds, z full ([row, col]), eig ascending; returns deig[:ntot]."""
+    n = dh.shape[0]
+    a = [np.ascontiguousarray(m.T, np.complex128) for m in (dh, ds, z)]
+    e = np.ascontiguousarray(eig, float)
+    d = np.zeros(n)
+    nt = C.c_int()
+    check(lib().tbk_perturb(device, n, a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data, e.ctypes.data,
+                            float(eigcut), C.byref(nt), d.ctypes.data), "tbk_perturb")
+    return d[:nt.value]
+
+
 def fermi(eig: np.ndarray, weights, zfill: float, tkb: float = 0.005, el: float = 2.0,
           device: int = 0) -> float:
     """Device Fermi level (fermi.f:1) of eigenvalues eig[k, i]."""

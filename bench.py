@@ -26,9 +26,9 @@ is --precision (default mixed).  Prints one JSON line with
   cpu_baseline  the CPU restatement (oracle) timed on this host's cores (at
                 most 16) on a bounded sample of the same workload, and on
                 one core; the CPU model.
---profile-pass PREC runs one batched step of PREC and a 1 GiB device copy
-(the calibration kernel for counter-based traffic) and exits; it is the
-program profiled with rocprofv3 --pmc.
+--profile-pass PREC runs one batched step of PREC (C2, or C3 with
+--workload c3) and a 1 GiB device copy (the calibration kernel for
+counter-based traffic) and exits; it is the program profiled with rocprofv3.
 """
 import argparse
 import json
@@ -87,8 +87,8 @@ def hetrd_panel_bytes(n: int, csize: int) -> float:
 # at order N: ZPOTRF 4/3, ZHEGST 4, ZHETRD 16/3, apply Q 8 and the
 # triangular back-transform 4 (in N^3 real flops).
 SOLVE_FLOP_PER_N3 = 4.0 / 3.0 + 4.0 + 16.0 / 3.0 + 8.0 + 4.0
-SOLVE_STAGES = ("potrf_trtri", "reduce_gemm", "hetrd_panel", "hetrd_update", "stein", "backtransform",
-                "refine")
+SOLVE_STAGES = ("potrf_trtri", "reduce_gemm", "hetrd_panel", "hetrd_update", "hetrd_band", "hetrd_bulge", "stein",
+                "backtransform", "refine")
 
 
 def solve_roofline(n, nk, stages, precision):
@@ -245,12 +245,12 @@ def serve_stream(rates, nreq, precision, seed=20261203, inflight=8):
     return out
 
 
-def profile_pass(precision, tridiag=0):
-    """The GEMM autotuning pass, one warm-up and one batched C2 step, then a
-    1 GiB copy kernel (read 1 GiB, write 1 GiB) to calibrate the traffic
-    counters."""
-    st = S.config_c2()
-    kpts = S.monkhorst_pack((8, 8, 8))
+def profile_pass(precision, tridiag=0, workload="c2"):
+    """The GEMM autotuning pass, one warm-up and one batched step of C2 (or
+    C3), then a 1 GiB copy kernel (read 1 GiB, write 1 GiB) to calibrate the
+    traffic counters."""
+    st = S.config_c3() if workload == "c3" else S.config_c2()
+    kpts = S.monkhorst_pack((4, 4, 4) if workload == "c3" else (8, 8, 8))
     w = np.full(len(kpts), 1.0 / len(kpts))
     plan = Plan(st, kpts, w, batch=0, precision=TBK_MIXED if precision == "mixed" else TBK_FP64)
     if tridiag:
@@ -272,7 +272,7 @@ def bench_c3(args):
     out = {"workload": "C3: 108-atom fcc 3x3x3, sigma=0.03 A, <=1% strain, N=972, MP 4^3 = 64 k, mode 4",
            "unit": "k-point solves/s"}
     for name, prec in (("fp64", TBK_FP64), ("mixed", TBK_MIXED)):
-        r = run_batched(st, kpts, w, prec, max(1, args.steps // 2), args.warmup, not args.no_autotune)
+        r = run_batched(st, kpts, w, prec, max(1, args.steps // 2), args.warmup, not args.no_autotune, args.tridiag)
         out[name] = {"ksolves_per_s": r["ksolves_per_s"], "ms_per_step": r["ms_per_step"],
                      "stages_ms_per_step": r["stages_ms_per_step"], "result": r["result"],
                      "roofline_solve": solve_roofline(st.nv, len(kpts), r["stages_ms_per_step"], name)}
@@ -371,14 +371,14 @@ def main():
                          "extra keys c3 and c4")
     ap.add_argument("--profile-pass", choices=["fp64", "mixed"], default=None)
     ap.add_argument("--tridiag", type=int, choices=[0, 1, 2], default=0,
-                    help="tridiagonalisation of the C2 plans: 0 the library's choice, 1 one-stage, 2 two-stage")
+                    help="tridiagonalisation of the C2 and C3 plans: 0 the library's choice, 1 one-stage, 2 two-stage")
     ap.add_argument("--workload", choices=["c2", "c3", "c4", "spin", "serve"], default="c2",
                     help="c2: the headline line; c3 (N=972, 64 k), c4 (256 uneven "
                          "structures), spin (C2 spin-polarised and non-collinear) and serve (the "
                          "request stream sweep) print their own lines")
     args = ap.parse_args()
     if args.profile_pass:
-        profile_pass(args.profile_pass, args.tridiag)
+        profile_pass(args.profile_pass, args.tridiag, args.workload)
         return
     if args.workload == "c3":
         print(json.dumps(bench_c3(args)))

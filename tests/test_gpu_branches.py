@@ -28,7 +28,7 @@ GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "branche
 CASES = {c["name"]: c for c in B.cases(GOLD["c_par"])}
 OK = [n for n, c in CASES.items() if c["expect"] == "ok"]
 TOL = {_lib.TBK_FP64: dict(eig=1e-10, ef=1e-9, e=1e-9, p=1e-6),
-       _lib.TBK_MIXED: dict(eig=5e-10, ef=1e-10, e=1e-10, p=3e-5)}
+       _lib.TBK_MIXED: dict(eig=5e-14, ef=2e-13, e=2e-13, p=2e-8)}
 
 
 def packed_full(gold, key, n):

@@ -203,12 +203,13 @@ def test_indefinite_overlap_structure_is_an_error():
 
 
 # Mixed precision (FP32 tridiagonalisation and back-transform, one FP64
-# Ogita-Aishima refinement): bounds about ten times the largest error measured
-# on these cases (C1 6^3 and C2: eigenvalues 4.5e-10 Ry, E_F 3.8e-13 Ry,
-# E/atom 7.2e-12 Ry, P 2.6e-5 kbar, |Z^H S Z - I| 3.1e-8); residual
-# |HZ - SZ diag(e)| 1e-9.  One refinement step leaves |Z^H S Z - I| of order
-# |R|^2 with R the FP32 orthogonality defect (about N * 2^-24).
-MIX_EF, MIX_E, MIX_EIG, MIX_P_KBAR, MIX_ORTH = 4e-12, 1e-10, 5e-9, 3e-4, 3e-7
+# refinement step with clusters rotated exactly): bounds about ten times the
+# largest error measured on these cases (C1 6^3 and C2: eigenvalues 8.2e-13 Ry,
+# E_F 1.4e-14 Ry, E/atom 7.9e-15 Ry, P 1.0e-8 kbar, |Z^H S Z - I| 7.2e-9);
+# residual |HZ - SZ diag(e)| 1e-9.  One refinement step leaves
+# |Z^H S Z - I| of order |E|^2 (|E| <= 1e-2 between states of different
+# clusters) and of the FP32 orthogonality defect squared.
+MIX_EF, MIX_E, MIX_EIG, MIX_P_KBAR, MIX_ORTH = 2e-13, 1e-13, 1e-11, 2e-7, 1e-7
 
 
 @pytest.mark.parametrize("cfg", ["c1", "c2"])

@@ -35,10 +35,10 @@ def test_server_same_alone_and_in_mix():
         a = p.run(4)
         p.close()
         d = max(abs(r[x] - a[x]) for x in ("efermi", "earg1", "earg2"))
-        assert d < 1e-10, (st.nv, d)
+        assert d < 1e-12, (st.nv, d)
         ref = oracle.Setup(st).bande(k, w, mode=4, nthreads=16)
-        assert abs(r["efermi"] - ref["efermi"]) < 1e-6
-        assert abs(r["earg1"] - ref["earg1"]) / st.natoms < 1e-6
+        assert abs(r["efermi"] - ref["efermi"]) < 1e-10
+        assert abs(r["earg1"] - ref["earg1"]) / st.natoms < 1e-10
 
 
 def test_server_failing_request_does_not_affect_others():

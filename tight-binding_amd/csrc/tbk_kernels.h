@@ -235,13 +235,25 @@ void launch_q2(const Q2ArgsT<double> &g, int nb, hipStream_t st);      // T fact
 void launch_q2(const Q2ArgsT<float> &g, int nb, hipStream_t st);
 void launch_rows_c32_to_c64(int n, int r0, int r1, const float2 *y32, double2 *y64, size_t stride, int nb,
                             hipStream_t st);
-int oa_max_n();                  // largest N of launch_oa_stats
-// Ogita-Aishima step: lam [nb][n], perm [nb][n], delta [nb] are scratch
-void launch_oa_stats(int n, const int *nact, const double2 *t, const double2 *g, size_t stride, double *lam,
-                     int *perm, double *delta, double *w, int nb, hipStream_t st);
-// F = I + E with sorted columns; columns j >= cut[k] are not formed (cut may be null)
-void launch_oa_fmat(int n, const double2 *t, const double2 *g, double2 *f, size_t stride, const double *lam,
-                    const int *perm, const double *delta, int nb, hipStream_t st, const int *cut = nullptr);
+int oa_max_n();                  // largest N of the refinement kernels
+int oa_max_cluster();            // largest cluster the refinement rotates
+// The refinement step of the mixed method, on T = Y^H C Y and G = Y^H Y
+// (both rotated in place): eigenvalues (ascending) into w, then F = P (I + E)
+// with sorted columns by launch_oa_fmat.  The other arrays are scratch:
+// lam, nu, w, perm, perm2, cinfo, clist [nb][n], tau, ncl [nb],
+// pb [nb][n][oa_max_cluster()].
+struct OaArgs {
+    int n;
+    size_t stride;
+    const int *nact;
+    double2 *T, *G;
+    double *lam, *nu, *tau, *w;
+    int *perm, *perm2, *cinfo, *clist, *ncl;
+    double2 *pb;
+};
+void launch_oa_refine(const OaArgs &a, int nb, hipStream_t st);
+// columns j >= cut[k] are not formed (cut may be null)
+void launch_oa_fmat(const OaArgs &a, double2 *f, int nb, hipStream_t st, const int *cut = nullptr);
 void launch_coldot(int n, const double2 *z, const double2 *x, const double2 *y, size_t stride,
                    double2 *px, double2 *py, int nb, hipStream_t st, const int *cut = nullptr);
 void launch_perturb(const PerturbArgs &p, hipStream_t st);

@@ -1710,76 +1710,94 @@ __global__ __launch_bounds__(256) void k_vt_to_columns(int N, const double *Vt, 
 }
 
 // ---------------------------------------------------------------------------
-// Mixed precision: the Ogita-Aishima refinement step
+// Mixed precision: the refinement step
 // ---------------------------------------------------------------------------
 
 // One refinement step for the Hermitian eigenproblem of C with approximate
-// orthonormal eigenvectors Y (Ogita & Aishima, RefSyEv, extended to complex
-// Hermitian): on entry T = Y^H C Y and G = Y^H Y.  With R = I - G,
-//   lambda_i = T_ii / (1 - R_ii),
-//   delta    = 2 (||T - diag(lambda)||_F + max|lambda| ||R||_F),
-//   E_ij     = (T_ij + lambda_j R_ij) / (lambda_j - lambda_i)  if |lambda_i - lambda_j| > delta,
-//              R_ij / 2                                        otherwise,
-// and the refined vectors are Y (I + E).  T and G are Hermitian (both
-// triangles stored, the upper one the conjugate of the lower).
-// k_oa_stats (one workgroup per k) forms lambda, delta and the ascending
-// order perm of lambda; k_oa_fmat writes F = I + E with its columns in that
-// order (F[:, j] = (I + E)[:, perm_j]), so that Y F is the refined, sorted
-// eigenvector matrix.
+// eigenvectors Y (Ogita & Aishima's step, RefSyEv, extended to complex
+// Hermitian, with clusters rotated exactly): on entry T = Y^H C Y and
+// G = Y^H Y, both triangles stored.  With R = I - G:
+//  1. lambda_i = T_ii / G_ii; tau = kOaRho * max_{i != j} |T_ij - lambda_j G_ij|;
+//     clusters = runs of the ascending lambda whose consecutive gaps are
+//     below tau (k_oa_stats).
+//  2. Each cluster c of 2 .. kOaMaxCl states is rotated by Rayleigh-Ritz:
+//     L = I + R_cc / 2 (G_cc^{-1/2} to second order), M = L T_cc L = Q L Q^H
+//     by cyclic Jacobi, P_c = L Q; its states take the eigenvalues of M
+//     (k_oa_rr).  T and G are carried into the rotated basis in place,
+//     T <- P^H T P, G <- P^H G P, P block-diagonal (k_oa_xform).
+//  3. Between states of different clusters the first-order step
+//     E_ij = (T_ij - lambda_j G_ij) / (lambda_j - lambda_i), |E_ij| <= 1 / kOaRho;
+//     within a cluster E_ij = R_ij / 2; the diagonal
+//     E_jj = R_jj / 2 + nu_j, nu_j = sum_{i != j} Re(conj(E_ij) R_ij) - |E_ij|^2 / 2,
+//     normalises the refined vector to second order, and
+//     lambda'_j = lambda_j + sum_{i not in j's cluster} |T_ij - lambda_j G_ij|^2 / (lambda_j - lambda_i)
+//     is its Rayleigh quotient to second order (k_oa_second); lambda' is
+//     sorted (k_oa_sort).
+//  4. F = P (I + E) with its columns in that order (k_oa_fmat): Y F is the
+//     refined, sorted eigenvector matrix.
+// Ogita & Aishima's step leaves every pair closer than its global bound
+// delta = 2 (||T - diag lambda||_F + max|lambda| ||R||_F) unrotated and
+// reports the Rayleigh quotients of the unrefined vectors; at order ~1000 with
+// FP32 vectors delta is ~1e-3, so most neighbouring levels fall inside it.
 constexpr int kOaMax = 1536;
-__global__ __launch_bounds__(1024) void k_oa_stats(int N, const int *nact, const double2 *T, const double2 *G,
-                                                   size_t stride, double *lam_out, int *perm, double *delta_out,
-                                                   double *w)
+constexpr int kOaMaxCl = 16;
+constexpr double kOaRho = 100.0;
+constexpr int kOaSlots = 8;                  // workgroups per item in the cluster kernels
+
+__device__ __forceinline__ bool oa_first_order(int bi, int bc, double li, double lc, double tau)
+{
+    return bi != bc && fabs(lc - li) > tau;
+}
+
+__global__ __launch_bounds__(1024) void k_oa_stats(OaArgs a)
 {
     __shared__ double lam[kOaMax], lsort[kOaMax];
-    __shared__ int idx[kOaMax];
-    __shared__ double red[3][16];
-    const int kb = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const double2 *Tk = T + (size_t)kb * stride;
-    const double2 *Gk = G + (size_t)kb * stride;
-    const int Na = nact ? nact[kb] : N;               // padding (decoupled) after Na
-    double lmax = 0.0;
+    __shared__ int idx[kOaMax], cinf[kOaMax];
+    __shared__ double red[16];
+    __shared__ int ncl_s;
+    const int N = a.n, kb = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nw = (int)(blockDim.x >> 6);
+    const double2 *Tk = a.T + (size_t)kb * a.stride;
+    const double2 *Gk = a.G + (size_t)kb * a.stride;
+    const int Na = a.nact ? a.nact[kb] : N;          // padding (decoupled) after Na
     for (int i = tid; i < N; i += blockDim.x) {
-        const double l = Tk[(size_t)i * N + i].x / Gk[(size_t)i * N + i].x;   // G_ii = 1 - R_ii
+        const double l = Tk[(size_t)i * N + i].x / Gk[(size_t)i * N + i].x;
         lam[i] = l; lsort[i] = l; idx[i] = i;
-        if (i < Na) lmax = fmax(lmax, fabs(l));
     }
     __syncthreads();
-    double ft = 0.0, fr = 0.0;
-    // lower triangles only (T, G Hermitian): wave wv takes columns wv, wv + 16,
-    // ..., its lanes the rows r >= c, two row chunks in flight
-    for (int c = wv; c < N; c += (int)(blockDim.x >> 6)) {
-        for (int r0 = c; r0 < N; r0 += 128) {
+    // largest coupling over the active off-diagonal pairs: element (r, c),
+    // r > c, of the lower triangles gives |T_rc - lambda_c G_rc| and, by
+    // Hermitian symmetry, |T_cr - lambda_r G_cr| = |T_rc - lambda_r G_rc|
+    double cm = 0.0;
+    for (int c = wv; c < Na; c += nw) {
+        const double lc = lam[c];
+        for (int r0 = c + 1; r0 < Na; r0 += 128) {
             double2 tv[2], gv[2];
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
                 const int r = r0 + 64 * u + lane;
-                tv[u] = r < N ? Tk[(size_t)c * N + r] : make_double2(0.0, 0.0);
-                gv[u] = r < N ? Gk[(size_t)c * N + r] : make_double2(1.0, 0.0);
+                tv[u] = r < Na ? Tk[(size_t)c * N + r] : make_double2(0.0, 0.0);
+                gv[u] = r < Na ? Gk[(size_t)c * N + r] : make_double2(0.0, 0.0);
             }
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
                 const int r = r0 + 64 * u + lane;
-                if (r >= N) continue;
-                double2 rv = make_double2(-gv[u].x, -gv[u].y);
-                double wgt = 2.0;
-                if (r == c) { tv[u].x -= lam[r]; rv.x += 1.0; wgt = 1.0; }
-                ft += wgt * (tv[u].x * tv[u].x + tv[u].y * tv[u].y);
-                fr += wgt * (rv.x * rv.x + rv.y * rv.y);
+                if (r >= Na) continue;
+                const double lr = lam[r];
+                const double ax = tv[u].x - lc * gv[u].x, ay = tv[u].y - lc * gv[u].y;
+                const double bx = tv[u].x - lr * gv[u].x, by = tv[u].y - lr * gv[u].y;
+                cm = fmax(cm, fmax(ax * ax + ay * ay, bx * bx + by * by));
             }
         }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        ft += __shfl_xor(ft, off, 64);
-        fr += __shfl_xor(fr, off, 64);
-        lmax = fmax(lmax, __shfl_xor(lmax, off, 64));
-    }
-    if (lane == 0) { red[0][wv] = ft; red[1][wv] = fr; red[2][wv] = lmax; }
+    for (int off = 32; off > 0; off >>= 1) cm = fmax(cm, __shfl_xor(cm, off, 64));
+    if (lane == 0) red[wv] = cm;
     __syncthreads();
     if (tid == 0) {
-        double a = 0.0, b = 0.0, c = 0.0;
-        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) { a += red[0][q]; b += red[1][q]; c = fmax(c, red[2][q]); }
-        delta_out[kb] = 2.0 * (sqrt(a) + c * sqrt(b));
+        double c2 = 0.0;
+        for (int q = 0; q < nw; ++q) c2 = fmax(c2, red[q]);
+        const double tau = kOaRho * sqrt(c2);
+        a.tau[kb] = tau;
         // stable insertion sort of the active part (the input is nearly sorted)
         for (int i = 1; i < Na; ++i) {
             const double x = lsort[i];
@@ -1788,41 +1806,340 @@ __global__ __launch_bounds__(1024) void k_oa_stats(int N, const int *nact, const
             while (k >= 0 && lsort[k] > x) { lsort[k + 1] = lsort[k]; idx[k + 1] = idx[k]; --k; }
             lsort[k + 1] = x; idx[k + 1] = xi;
         }
+        // clusters: runs with consecutive gaps below tau; those of 2 ..
+        // kOaMaxCl states are rotated, longer runs stay unrotated
+        int ncl = 0, s0 = 0;
+        int *clist = a.clist + (size_t)kb * N;
+        while (s0 < Na) {
+            int e = s0 + 1;
+            while (e < Na && lsort[e] - lsort[e - 1] < tau) ++e;
+            const int m = e - s0;
+            const bool rot = m >= 2 && m <= kOaMaxCl;
+            if (rot) clist[ncl++] = s0;
+            for (int q = 0; q < m; ++q) cinf[idx[s0 + q]] = s0 | (rot ? (q << 16) | (m << 24) : (1 << 24));
+            s0 = e;
+        }
+        for (int i = Na; i < N; ++i) cinf[i] = i | (1 << 24);
+        ncl_s = ncl;
     }
     __syncthreads();
+    if (tid == 0) a.ncl[kb] = ncl_s;
     for (int i = tid; i < N; i += blockDim.x) {
-        w[(size_t)kb * N + i] = lsort[i];
-        perm[(size_t)kb * N + i] = idx[i];
-        lam_out[(size_t)kb * N + i] = lam[i];
+        a.perm[(size_t)kb * N + i] = idx[i];
+        a.lam[(size_t)kb * N + i] = lam[i];
+        a.cinfo[(size_t)kb * N + i] = cinf[i];
     }
 }
 
-__global__ __launch_bounds__(256) void k_oa_fmat(int N, const double2 *T, const double2 *G, double2 *F,
-                                                 size_t stride, const double *lamv, const int *perm,
-                                                 const double *delta, const int *cut)
+// Rayleigh-Ritz in each rotated cluster: one wave per cluster, the m x m
+// matrices in LDS.
+__global__ __launch_bounds__(64) void k_oa_rr(OaArgs a)
 {
-    const int kb = blockIdx.y;
+    __shared__ double2 Am[kOaMaxCl][kOaMaxCl + 1], Qm[kOaMaxCl][kOaMaxCl + 1], Lm[kOaMaxCl][kOaMaxCl + 1];
+    __shared__ double ev[kOaMaxCl];
+    __shared__ int ord[kOaMaxCl];
+    const int N = a.n, kb = blockIdx.y, lane = threadIdx.x;
+    const double2 *Tk = a.T + (size_t)kb * a.stride;
+    const double2 *Gk = a.G + (size_t)kb * a.stride;
+    const int *perm = a.perm + (size_t)kb * N;
+    const int ncl = a.ncl[kb];
+    for (int ci = blockIdx.x; ci < ncl; ci += gridDim.x) {
+        const int b = a.clist[(size_t)kb * N + ci];
+        const int m = (a.cinfo[(size_t)kb * N + perm[b]] >> 24) & 0xff;
+        // T_cc into Am, L = I + (I - G_cc) / 2 into Lm
+        for (int t = lane; t < m * m; t += 64) {
+            const int q = t % m, k = t / m;
+            const size_t e = (size_t)perm[b + k] * N + perm[b + q];      // element (c_q, c_k)
+            const double2 tv = Tk[e], gv = Gk[e];
+            Am[q][k] = tv;
+            Lm[q][k] = q == k ? make_double2(1.5 - 0.5 * gv.x, 0.0) : make_double2(-0.5 * gv.x, -0.5 * gv.y);
+        }
+        __syncthreads();
+        // M = L T L: Qm = T L, then Am = L Qm, Hermitian part
+        for (int t = lane; t < m * m; t += 64) {
+            const int q = t % m, k = t / m;
+            double2 acc = make_double2(0.0, 0.0);
+            for (int p = 0; p < m; ++p) acc = zadd(acc, zmul(Am[q][p], Lm[p][k]));
+            Qm[q][k] = acc;
+        }
+        __syncthreads();
+        double2 mv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int t = lane + 64 * u;
+            mv[u] = make_double2(0.0, 0.0);
+            if (t < m * m) {
+                const int q = t % m, k = t / m;
+                for (int p = 0; p < m; ++p) mv[u] = zadd(mv[u], zmul(Lm[q][p], Qm[p][k]));
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int t = lane + 64 * u;
+            if (t < m * m) Am[t % m][t / m] = mv[u];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int t = lane + 64 * u;
+            if (t < m * m) {
+                const int q = t % m, k = t / m;
+                const double2 x = Am[q][k], y = Am[k][q];
+                mv[u] = q == k ? make_double2(x.x, 0.0) : make_double2(0.5 * (x.x + y.x), 0.5 * (x.y - y.y));
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int t = lane + 64 * u;
+            if (t < m * m) {
+                const int q = t % m, k = t / m;
+                Am[q][k] = mv[u];
+                Qm[q][k] = make_double2(q == k ? 1.0 : 0.0, 0.0);
+            }
+        }
+        __syncthreads();
+        // cyclic Jacobi: A <- J^H A J, Q <- Q J with J_pp = J_qq = c,
+        // J_pq = s ph, J_qp = -s conj(ph), ph = A_pq / |A_pq|
+        for (int sweep = 0; sweep < 32; ++sweep) {
+            double off = 0.0, dia = 0.0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int t = lane + 64 * u;
+                if (t < m * m) {
+                    const int q = t % m, k = t / m;
+                    const double2 x = Am[q][k];
+                    if (q == k) dia += x.x * x.x;
+                    else off += x.x * x.x + x.y * x.y;
+                }
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                off += __shfl_xor(off, o, 64);
+                dia += __shfl_xor(dia, o, 64);
+            }
+            if (!(off > 1e-34 * dia)) break;
+            for (int p = 0; p < m - 1; ++p)
+                for (int q = p + 1; q < m; ++q) {
+                    const double2 apq = Am[p][q];
+                    const double aa = sqrt(apq.x * apq.x + apq.y * apq.y);
+                    if (!(aa > 1e-300)) continue;
+                    const double app = Am[p][p].x, aqq = Am[q][q].x;
+                    const double2 ph = make_double2(apq.x / aa, apq.y / aa);
+                    const double th = (aqq - app) / (2.0 * aa);
+                    const double t = th == 0.0 ? 1.0 : copysign(1.0, th) / (fabs(th) + sqrt(1.0 + th * th));
+                    const double c = 1.0 / sqrt(1.0 + t * t), sn = t * c;
+                    const double2 sp = make_double2(sn * ph.x, sn * ph.y);       // s ph
+                    const double2 spc = make_double2(sn * ph.x, -sn * ph.y);     // s conj(ph)
+                    if (lane < m) {
+                        const double2 x = Am[lane][p], y = Am[lane][q];
+                        Am[lane][p] = zsub(zscale(c, x), zmul(spc, y));
+                        Am[lane][q] = zadd(zmul(sp, x), zscale(c, y));
+                    }
+                    __syncthreads();
+                    if (lane < m) {
+                        const double2 x = Am[p][lane], y = Am[q][lane];
+                        Am[p][lane] = zsub(zscale(c, x), zmul(sp, y));
+                        Am[q][lane] = zadd(zmul(spc, x), zscale(c, y));
+                    }
+                    __syncthreads();
+                    if (lane == 0) {
+                        Am[p][q] = make_double2(0.0, 0.0);
+                        Am[q][p] = make_double2(0.0, 0.0);
+                        Am[p][p].y = 0.0;
+                        Am[q][q].y = 0.0;
+                    }
+                    if (lane < m) {
+                        const double2 x = Qm[lane][p], y = Qm[lane][q];
+                        Qm[lane][p] = zsub(zscale(c, x), zmul(spc, y));
+                        Qm[lane][q] = zadd(zmul(sp, x), zscale(c, y));
+                    }
+                    __syncthreads();
+                }
+        }
+        if (lane == 0) {
+            for (int k = 0; k < m; ++k) { ev[k] = Am[k][k].x; ord[k] = k; }
+            for (int k = 1; k < m; ++k) {            // ascending, stable
+                const double x = ev[k];
+                const int xi = ord[k];
+                int j = k - 1;
+                while (j >= 0 && ev[j] > x) { ev[j + 1] = ev[j]; ord[j + 1] = ord[j]; --j; }
+                ev[j + 1] = x; ord[j + 1] = xi;
+            }
+        }
+        __syncthreads();
+        // P = L Q (columns in ascending order) into the rows b .. b + m - 1
+        // of pb; the eigenvalues go to the members in sorted order
+        double2 *pb = a.pb + (size_t)kb * N * kOaMaxCl;
+        for (int t = lane; t < m * m; t += 64) {
+            const int q = t % m, k = t / m;
+            double2 acc = make_double2(0.0, 0.0);
+            for (int p = 0; p < m; ++p) acc = zadd(acc, zmul(Lm[q][p], Qm[p][ord[k]]));
+            pb[(size_t)(b + q) * kOaMaxCl + k] = acc;
+        }
+        if (lane < m) a.lam[(size_t)kb * N + perm[b + lane]] = ev[lane];
+        __syncthreads();
+    }
+}
+
+// T <- T P (rows = 0) or T <- P^H T (rows = 1) over each rotated cluster's
+// columns (rows), and the same for G.  Clusters own disjoint columns
+// (rows), so the clusters of an item run in parallel within one pass.
+__global__ __launch_bounds__(256) void k_oa_xform(OaArgs a, int rows)
+{
+    __shared__ double2 P[kOaMaxCl][kOaMaxCl];
+    __shared__ int mem[kOaMaxCl];
+    const int N = a.n, kb = blockIdx.y, tid = threadIdx.x;
+    const int *perm = a.perm + (size_t)kb * N;
+    const double2 *pb = a.pb + (size_t)kb * N * kOaMaxCl;
+    const int ncl = a.ncl[kb];
+    for (int ci = blockIdx.x; ci < ncl; ci += gridDim.x) {
+        const int b = a.clist[(size_t)kb * N + ci];
+        const int m = (a.cinfo[(size_t)kb * N + perm[b]] >> 24) & 0xff;
+        for (int t = tid; t < m * m; t += blockDim.x) P[t % m][t / m] = pb[(size_t)(b + t % m) * kOaMaxCl + t / m];
+        if (tid < m) mem[tid] = perm[b + tid];
+        __syncthreads();
+        for (int mat = 0; mat < 2; ++mat) {
+            double2 *M = (mat == 0 ? a.T : a.G) + (size_t)kb * a.stride;
+            for (int i = tid; i < N; i += blockDim.x) {
+                double2 v[kOaMaxCl];
+#pragma unroll
+                for (int q = 0; q < kOaMaxCl; ++q)
+                    if (q < m) v[q] = rows ? M[(size_t)i * N + mem[q]] : M[(size_t)mem[q] * N + i];
+#pragma unroll
+                for (int k = 0; k < kOaMaxCl; ++k) {
+                    if (k >= m) break;
+                    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int q = 0; q < kOaMaxCl; ++q)
+                        if (q < m) acc = zadd(acc, rows ? zmul(zconj(P[q][k]), v[q]) : zmul(v[q], P[q][k]));
+                    if (rows) M[(size_t)i * N + mem[k]] = acc;
+                    else M[(size_t)mem[k] * N + i] = acc;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Second-order terms per column c (one wave per column): lambda'_c into w
+// (unsorted), nu_c.
+__global__ __launch_bounds__(1024) void k_oa_second(OaArgs a)
+{
+    const int N = a.n, kb = blockIdx.y, lane = threadIdx.x & 63;
+    const int c = blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+    const int Na = a.nact ? a.nact[kb] : N;
+    if (c >= N) return;
+    const double *lam = a.lam + (size_t)kb * N;
+    const int *cinfo = a.cinfo + (size_t)kb * N;
+    const double lc = lam[c], tau = a.tau[kb];
+    if (c >= Na) {
+        if (lane == 0) { a.w[(size_t)kb * N + c] = lc; a.nu[(size_t)kb * N + c] = 0.0; }
+        return;
+    }
+    const double2 *Tc = a.T + (size_t)kb * a.stride + (size_t)c * N;
+    const double2 *Gc = a.G + (size_t)kb * a.stride + (size_t)c * N;
+    const int bc = cinfo[c] & 0xffff;
+    double corr = 0.0, nu = 0.0;
+    for (int i = lane; i < Na; i += 64) {
+        if (i == c) continue;
+        const double2 g = Gc[i];
+        const double rx = -g.x, ry = -g.y;                          // R_ic
+        const double li = lam[i];
+        if (oa_first_order(cinfo[i] & 0xffff, bc, li, lc, tau)) {
+            const double2 t = Tc[i];
+            const double mx = t.x - lc * g.x, my = t.y - lc * g.y;   // T_ic - lambda_c G_ic
+            const double den = lc - li;
+            const double ex = mx / den, ey = my / den;
+            corr += (mx * mx + my * my) / den;
+            nu += ex * rx + ey * ry - 0.5 * (ex * ex + ey * ey);
+        } else {
+            nu += 0.375 * (rx * rx + ry * ry);                       // E_ic = R_ic / 2
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        corr += __shfl_xor(corr, o, 64);
+        nu += __shfl_xor(nu, o, 64);
+    }
+    if (lane == 0) {
+        a.w[(size_t)kb * N + c] = lc + corr;
+        a.nu[(size_t)kb * N + c] = nu;
+    }
+}
+
+// lambda' sorted (from the order of the first sort, which it nearly keeps):
+// the final order perm2 and the eigenvalues w (ascending).
+__global__ __launch_bounds__(256) void k_oa_sort(OaArgs a)
+{
+    __shared__ double ls[kOaMax];
+    __shared__ int ix[kOaMax];
+    const int N = a.n, kb = blockIdx.x, tid = threadIdx.x;
+    const int Na = a.nact ? a.nact[kb] : N;
+    double *w = a.w + (size_t)kb * N;
+    const int *perm = a.perm + (size_t)kb * N;
+    for (int s = tid; s < N; s += blockDim.x) {
+        const int i = s < Na ? perm[s] : s;
+        ix[s] = i;
+        ls[s] = w[i];
+    }
+    __syncthreads();
+    if (tid == 0)
+        for (int i = 1; i < Na; ++i) {
+            const double x = ls[i];
+            const int xi = ix[i];
+            int k = i - 1;
+            while (k >= 0 && ls[k] > x) { ls[k + 1] = ls[k]; ix[k + 1] = ix[k]; --k; }
+            ls[k + 1] = x; ix[k + 1] = xi;
+        }
+    __syncthreads();
+    for (int s = tid; s < N; s += blockDim.x) {
+        w[s] = ls[s];
+        a.perm2[(size_t)kb * N + s] = ix[s];
+    }
+}
+
+// Element (i, c) of I + E in the rotated basis.
+__device__ __forceinline__ double2 oa_ehat(const double2 *Tc, const double2 *Gc, const double *lam,
+                                           const int *cinfo, const double *nu, int i, int c, double tau)
+{
+    const double2 g = Gc[i];
+    if (i == c) return make_double2(1.0 + 0.5 * (1.0 - g.x) + nu[c], 0.0);
+    const double lc = lam[c], li = lam[i];
+    if (oa_first_order(cinfo[i] & 0xffff, cinfo[c] & 0xffff, li, lc, tau)) {
+        const double2 t = Tc[i];
+        const double den = lc - li;
+        return make_double2((t.x - lc * g.x) / den, (t.y - lc * g.y) / den);
+    }
+    return make_double2(-0.5 * g.x, -0.5 * g.y);
+}
+
+// F = P (I + E) with columns in the final order: F[:, j] = (P (I + E))[:, perm2_j].
+__global__ __launch_bounds__(256) void k_oa_fmat(OaArgs a, double2 *F, const int *cut)
+{
+    const int N = a.n, kb = blockIdx.y;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)N * N) return;
     const int r = (int)(t % N), j = (int)(t / N);
     if (cut && j >= cut[kb]) return;                  // column outside the occupation window
-    const int c = perm[(size_t)kb * N + j];
-    const double *lam = lamv + (size_t)kb * N;
-    // T and G are Hermitian, both triangles stored: column c is read down
-    const size_t src = (size_t)kb * stride + (size_t)c * N + r;
-    const double2 gv = G[src];
-    double2 rv = make_double2(-gv.x, -gv.y);
-    if (r == c) rv.x += 1.0;
-    double2 ev;
-    const double gap = lam[c] - lam[r];
-    if (fabs(gap) > delta[kb]) {
-        const double2 tv = T[src];
-        ev = make_double2((tv.x + lam[c] * rv.x) / gap, (tv.y + lam[c] * rv.y) / gap);
+    const int c = a.perm2[(size_t)kb * N + j];
+    const double *lam = a.lam + (size_t)kb * N;
+    const int *cinfo = a.cinfo + (size_t)kb * N;
+    const double *nu = a.nu + (size_t)kb * N;
+    const double tau = a.tau[kb];
+    const double2 *Tc = a.T + (size_t)kb * a.stride + (size_t)c * N;
+    const double2 *Gc = a.G + (size_t)kb * a.stride + (size_t)c * N;
+    const int ci = cinfo[r], m = (ci >> 24) & 0xff;
+    double2 v;
+    if (m < 2) {
+        v = oa_ehat(Tc, Gc, lam, cinfo, nu, r, c, tau);
     } else {
-        ev = make_double2(0.5 * rv.x, 0.5 * rv.y);
+        const int b = ci & 0xffff, q = (ci >> 16) & 0xff;
+        const int *perm = a.perm + (size_t)kb * N;
+        const double2 *prow = a.pb + ((size_t)kb * N + b + q) * kOaMaxCl;
+        v = make_double2(0.0, 0.0);
+        for (int k = 0; k < m; ++k) v = zadd(v, zmul(prow[k], oa_ehat(Tc, Gc, lam, cinfo, nu, perm[b + k], c, tau)));
     }
-    if (r == c) ev.x += 1.0;
-    F[(size_t)kb * stride + t] = ev;
+    F[(size_t)kb * a.stride + t] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -3180,16 +3497,20 @@ void launch_rows_c32_to_c64(int n, int r0, int r1, const float2 *y32, double2 *y
     hipLaunchKernelGGL(k_rows_c32_to_c64, dim3(gx, nb), dim3(256), 0, st, n, r0, r1, y32, y64, stride);
 }
 int oa_max_n() { return kOaMax; }
-void launch_oa_stats(int n, const int *nact, const double2 *t, const double2 *g, size_t stride, double *lam,
-                     int *perm, double *delta, double *w, int nb, hipStream_t st)
+int oa_max_cluster() { return kOaMaxCl; }
+void launch_oa_refine(const OaArgs &a, int nb, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_oa_stats, dim3(nb), dim3(1024), 0, st, n, nact, t, g, stride, lam, perm, delta, w);
+    hipLaunchKernelGGL(k_oa_stats, dim3(nb), dim3(1024), 0, st, a);
+    hipLaunchKernelGGL(k_oa_rr, dim3(kOaSlots, nb), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_oa_xform, dim3(kOaSlots, nb), dim3(256), 0, st, a, 0);
+    hipLaunchKernelGGL(k_oa_xform, dim3(kOaSlots, nb), dim3(256), 0, st, a, 1);
+    hipLaunchKernelGGL(k_oa_second, dim3((a.n + 15) / 16, nb), dim3(1024), 0, st, a);
+    hipLaunchKernelGGL(k_oa_sort, dim3(nb), dim3(256), 0, st, a);
 }
-void launch_oa_fmat(int n, const double2 *t, const double2 *g, double2 *f, size_t stride, const double *lam,
-                    const int *perm, const double *delta, int nb, hipStream_t st, const int *cut)
+void launch_oa_fmat(const OaArgs &a, double2 *f, int nb, hipStream_t st, const int *cut)
 {
-    hipLaunchKernelGGL(k_oa_fmat, dim3((unsigned)(((size_t)n * n + 255) / 256), nb), dim3(256), 0, st, n, t,
-                       g, f, stride, lam, perm, delta, cut);
+    hipLaunchKernelGGL(k_oa_fmat, dim3((unsigned)(((size_t)a.n * a.n + 255) / 256), nb), dim3(256), 0, st, a, f,
+                       cut);
 }
 void launch_coldot(int N, const double2 *Z, const double2 *X, const double2 *Y, size_t stride,
                    double2 *px, double2 *py, int nb, hipStream_t st, const int *cut)

@@ -32,9 +32,7 @@ constexpr double kOrtolRel = 1e-6;
 // batches below kWideMaxBatch items of order >= kWideMinN.
 constexpr int kWideMinN = 128, kWideMaxBatch = 160;
 // Two-stage tridiagonalisation (band width TB) by default from this order
-// up, in FP64 (at smaller orders the one-stage panels are faster; in the
-// mixed method the FP32 two-stage reduction leaves the refinement about
-// ten times larger eigenvalue errors).
+// up, in both methods (at smaller orders the one-stage panels are faster).
 constexpr int kTwoStageMinN = 512;
 using tbk::TB;
 using tbk::TBLD;
@@ -92,8 +90,10 @@ struct Workspace {
     // mixed precision: FP32 copies for the reduction and back-transform
     float2 *A32 = nullptr, *P32 = nullptr, *T32 = nullptr, *VT32 = nullptr, *Y32 = nullptr,
            *X32 = nullptr;
-    double *lam = nullptr, *delta = nullptr;
-    int *perm = nullptr;
+    // the refinement step's scratch (tbk::OaArgs)
+    double *lam = nullptr, *nu = nullptr, *tau = nullptr;
+    int *perm = nullptr, *perm2 = nullptr, *cinfo = nullptr, *clist = nullptr, *ncl = nullptr;
+    double2 *pb = nullptr;
     // multi-workgroup tridiagonalisation (few items of large order)
     bool wide = false;
     // two-stage tridiagonalisation (band reduction + bulge chasing) instead
@@ -130,7 +130,10 @@ struct Workspace {
         int rc = TBK_OK;
         rc |= dalloc(&A32, nn); rc |= dalloc(&VT32, nn); rc |= dalloc(&Y32, nn); rc |= dalloc(&X32, nn);
         rc |= dalloc(&P32, (size_t)n * 96 * b); rc |= dalloc(&T32, tstride() * b);
-        rc |= dalloc(&lam, (size_t)n * b); rc |= dalloc(&perm, (size_t)n * b); rc |= dalloc(&delta, (size_t)b);
+        rc |= dalloc(&lam, (size_t)n * b); rc |= dalloc(&nu, (size_t)n * b); rc |= dalloc(&tau, (size_t)b);
+        rc |= dalloc(&perm, (size_t)n * b); rc |= dalloc(&perm2, (size_t)n * b); rc |= dalloc(&cinfo, (size_t)n * b);
+        rc |= dalloc(&clist, (size_t)n * b); rc |= dalloc(&ncl, (size_t)b);
+        rc |= dalloc(&pb, (size_t)n * tbk::oa_max_cluster() * b);
         return rc ? TBK_ERR_NOMEM : TBK_OK;
     }
     int alloc_wide(bool fp32)
@@ -186,7 +189,9 @@ struct Workspace {
         dfree(A); dfree(S); dfree(Y); dfree(Z); dfree(X); dfree(VT); dfree(Vp); dfree(Wp); dfree(Tf);
         dfree(px); dfree(py); dfree(d); dfree(e); dfree(w); dfree(Vt); dfree(scratch);
         dfree(info); dfree(sinfo);
-        dfree(A32); dfree(P32); dfree(T32); dfree(VT32); dfree(Y32); dfree(X32); dfree(lam); dfree(perm); dfree(delta);
+        dfree(A32); dfree(P32); dfree(T32); dfree(VT32); dfree(Y32); dfree(X32);
+        dfree(lam); dfree(nu); dfree(tau); dfree(perm); dfree(perm2); dfree(cinfo); dfree(clist); dfree(ncl);
+        dfree(pb);
     }
 };
 
@@ -444,10 +449,11 @@ void band_reduce(int n, int nb, typename tbk::cplx_t<R>::type *A, typename tbk::
     tbk::launch_band_tail(b, np * TB, nb, st);
 }
 
-// Mixed-precision solve: FP64 Cholesky, inverse and reduction; FP32 panel
+// Mixed-precision solve: FP64 Cholesky, inverse and reduction; FP32
 // tridiagonalisation and back-transform on the FP32 MFMA units; FP64
-// bisection/eigenvectors of the tridiagonal; one FP64 Ogita-Aishima step on
-// the reduced problem C; FP64 Z = U^{-1} Y.
+// bisection/eigenvectors of the tridiagonal; one FP64 refinement step on the
+// reduced problem C (Ogita-Aishima, with clusters rotated exactly and the
+// eigenvalues to second order); FP64 Z = U^{-1} Y.
 // Bisection tolerance of the mixed path, relative to ||T||_1: the
 // tridiagonal of the FP32 reduction carries FP32 errors, and its eigenvalues
 // only seed the vectors that the FP64 refinement corrects.
@@ -536,13 +542,16 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
     tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.Y, n, ss, ws.X, n, ss), 0, 0, nb, st);
     tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.Z, n, ss, 5), 1, 0, nb, st);
     tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.Y, n, ss, ws.VT, n, ss, 5), 1, 0, nb, st);
-    // X = (I + E) with sorted columns; VT = Y X; Z = U^{-1} VT
-    tbk::launch_oa_stats(n, nact, ws.Z, ws.VT, ss, ws.lam, ws.perm, ws.delta, w_out, nb, st);
+    // eigenvalues into w_out, then X = P (I + E) with sorted columns;
+    // VT = Y X; Z = U^{-1} VT
+    const tbk::OaArgs oa{n, ss, nact, ws.Z, ws.VT, ws.lam, ws.nu, ws.tau, w_out,
+                         ws.perm, ws.perm2, ws.cinfo, ws.clist, ws.ncl, ws.pb};
+    tbk::launch_oa_refine(oa, nb, st);
     // the eigenvalues are final: occupation window, then only the columns
     // it keeps of F, Y F and U^{-1} (Y F)
     if (wnd.cut) tbk::launch_fermi_window_group(wnd.fargs, wnd.nstruct, st);
     if (wnd.vectors) {
-        tbk::launch_oa_fmat(n, ws.Z, ws.VT, ws.X, ss, ws.lam, ws.perm, ws.delta, nb, st, wnd.cut);
+        tbk::launch_oa_fmat(oa, ws.X, nb, st, wnd.cut);
         tbk::launch_zgemm(with_ncol(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.VT, n, ss), wnd.cut), 0, 0, nb, st);
         tbk::launch_zgemm(with_ncol(gemm(n, n, n, ws.S, n, ss, ws.VT, n, ss, ws.Z, n, ss, 1), wnd.cut), 0, 0, nb, st);
     }
@@ -955,7 +964,6 @@ static int plan_create_from(int device, int nstruct, const tbk_prepared *const *
     if (!rc) rc = p->ws.alloc(N, p->batch);
     if (!rc && precision == TBK_MIXED) {
         rc = p->ws.alloc_mixed();
-        p->ws.two_stage = false;
     }
     // Few items of large order: one workgroup per item would leave most CUs
     // idle in the tridiagonalisation, so it runs over tiles instead.
@@ -1446,7 +1454,7 @@ extern "C" int tbk_plan_projections(tbk_plan *p, int k, double eigcut, int *ntot
 extern "C" int tbk_plan_set_tridiag(tbk_plan *p, int stages)
 {
     if (!p || stages < 0 || stages > 2) return TBK_ERR_ARG;
-    const bool two = stages == 0 ? p->N >= kTwoStageMinN && p->precision == TBK_FP64 : stages == 2;
+    const bool two = stages == 0 ? p->N >= kTwoStageMinN && two_stage_fits(p->N) : stages == 2;
     if (two && !two_stage_fits(p->N)) return TBK_ERR_ARG;
     p->ws.two_stage = two;
     p->ws2.two_stage = two;

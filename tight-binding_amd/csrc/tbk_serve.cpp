@@ -138,7 +138,7 @@ struct tbk_server {
 // results (mixed precision adds the FP32 copies), its set-up arrays, slack.
 static long long estimate_bytes(int N, int nk, size_t setup_bytes, int precision)
 {
-    const long long per_item = (precision == TBK_MIXED ? 180LL : 148LL) * N * N + 64LL * N + 4096;
+    const long long per_item = (precision == TBK_MIXED ? 180LL * N * N + 384LL * N : 148LL * N * N + 64LL * N) + 4096;
     return per_item * nk + (long long)setup_bytes + (16LL << 20);
 }
 

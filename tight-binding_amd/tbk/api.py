@@ -143,7 +143,7 @@ class Plan:
 
     def set_tridiag(self, stages: int):
         """1: one-stage Householder panels; 2: two-stage (band + bulge
-        chasing); 0: the default (two-stage for FP64 from order 512)."""
+        chasing); 0: the default (two-stage from order 512)."""
         check(lib().tbk_plan_set_tridiag(self.h, int(stages)), "tbk_plan_set_tridiag")
 
     def set_window(self, on: bool = True):

@@ -409,6 +409,13 @@ int tbk_plan_projections(tbk_plan *plan, int k, double eigcut, int *ntot,
  * pressures and eigenvalues are the same as without it; on = 0 forms every
  * eigenvector. */
 int tbk_plan_set_window(tbk_plan *plan, int on);
+/* Build of the pressure modes (no reference counterpart; the derivative
+ * matrices of sethamv.f): on = 0 (default) builds dH/dV, dS/dV after the
+ * eigensolve into its workspace; on = 1 builds H, S, dH/dV and dS/dV in one
+ * pass over the phases and Slater-Koster blocks, into two more matrices per
+ * k-point allocated on the first pressure run (32 N^2 bytes per item).
+ * Results are bit-identical; at order 288 the one pass is the slower. */
+int tbk_plan_set_fused_build(tbk_plan *plan, int on);
 /* Tridiagonalisation of the plan's eigensolves (no reference counterpart:
  * both replace ZHPTRD, lapack.f:3218, as diag.f:396-412 calls it):
  * 1 one-stage blocked Householder panels; 2 two-stage (reduction to a band
@@ -420,7 +427,9 @@ int tbk_plan_set_tridiag(tbk_plan *plan, int stages);
 
 /* H(k), S(k) of k-point k as built on the device (full Hermitian,
  * column-major complex128), for parity checks of the build.  deriv = 1
- * returns dH/dV, dS/dV (sethamv.f). */
+ * returns dH/dV, dS/dV (sethamv.f).  The build is the one the plan's
+ * pressure runs use (the four-matrix pass unless tbk_plan_set_fused_build
+ * turned it off). */
 int tbk_plan_hs(tbk_plan *plan, int k, int deriv, double *h, double *s);
 
 int tbk_plan_nv(const tbk_plan *plan);

@@ -37,6 +37,7 @@ struct BuildArgs {
     const int *item_s;             // [nb] structure of each item
     const double2 *ph;             // [nb][maxpairs] Bloch phases
     double2 *h, *s;
+    double2 *dh = nullptr, *ds = nullptr;   // non-null: H, S, dH/dV, dS/dV in one pass (deriv unused)
 };
 
 template <class R> struct cplx_t;

@@ -15,7 +15,8 @@
 //                   (k_hw_*: the same for few items of large order)
 //   k_stebz, k_stein_vec   bisection eigenvalues; eigenvectors by twisted
 //                   factorisation (isolated) or inverse iteration (groups)
-//   k_oa_stats, k_oa_fmat  Ogita-Aishima refinement (mixed precision)
+//   k_oa_*          the refinement step of the mixed method (Ogita-Aishima,
+//                   clusters rotated exactly)
 //   k_coldot, k_perturb   first-order derivatives (perturb.f:87-221)
 //   k_fermi_window_group  E_F and the occupation window of each k
 //   k_fermi_sums    Fermi level + band-energy / pressure sums
@@ -124,31 +125,43 @@ __global__ __launch_bounds__(256) void k_phases(const StructDev *sd, const int *
 // are staged through LDS 16 pairs at a time and each is loaded once for all
 // items.  The last kBuildPadWgs workgroup columns write the padding rows and
 // columns nv..n-1.
-constexpr int kBuildItems = 24, kBuildPadWgs = 8;
+// NM = 2 builds H, S (deriv = 0) or dH/dV, dS/dV (deriv = 1, sethamv.f);
+// NM = 4 builds all four from the same phases in one pass.
+constexpr int kBuildPadWgs = 8;
+// items per workgroup: 24 for two matrices, 12 for four (the same
+// accumulator registers)
+template <int NM> constexpr int build_items() { return NM == 4 ? 12 : 24; }
 
+template <int NM>
 __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a, int nb)
 {
 #pragma clang fp contract(off)
+    constexpr int kBuildItems = build_items<NM>();
     const int tid = threadIdx.x, N = a.n;
     const int it0 = blockIdx.y * kBuildItems, it1 = min(nb, it0 + kBuildItems);
     const int nup = (int)gridDim.x - kBuildPadWgs;
+    const bool dv = NM == 2 && a.deriv;                  // the pair of a 2-matrix build is dH, dS
+    double2 *out[NM];
+    out[0] = a.h; out[1] = a.s;
+    if (NM == 4) { out[2] = a.dh; out[3] = a.ds; }
     if ((int)blockIdx.x >= nup) {
-        const double hp = a.deriv ? 0.0 : a.pad, sp = a.deriv ? 0.0 : 1.0;
         const int t0 = tid + 256 * ((int)blockIdx.x - nup), ts = 256 * kBuildPadWgs;
         for (int kb = it0; kb < it1; ++kb) {
             const int nv = a.sd[a.item_s[kb]].nv;
             if (nv >= N) continue;
-            double2 *H = a.h + (size_t)kb * a.stride, *S = a.s + (size_t)kb * a.stride;
-            for (int t = t0; t < N * (N - nv); t += ts) {              // columns nv..N-1
-                const int r = t % N, c = nv + t / N;
-                H[(size_t)c * N + r] = make_double2(r == c ? hp : 0.0, 0.0);
-                S[(size_t)c * N + r] = make_double2(r == c ? sp : 0.0, 0.0);
-            }
-            const int np = N - nv;
-            for (int t = t0; t < np * nv; t += ts) {                   // rows nv..N-1 of columns < nv
-                const int r = nv + t % np, c = t / np;
-                H[(size_t)c * N + r] = make_double2(0.0, 0.0);
-                S[(size_t)c * N + r] = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                const double dg = (m >= 2 || dv) ? 0.0 : (m == 0 ? a.pad : 1.0);
+                double2 *M = out[m] + (size_t)kb * a.stride;
+                for (int t = t0; t < N * (N - nv); t += ts) {              // columns nv..N-1
+                    const int r = t % N, c = nv + t / N;
+                    M[(size_t)c * N + r] = make_double2(r == c ? dg : 0.0, 0.0);
+                }
+                const int np = N - nv;
+                for (int t = t0; t < np * nv; t += ts) {                   // rows nv..N-1 of columns < nv
+                    const int r = nv + t % np, c = t / np;
+                    M[(size_t)c * N + r] = make_double2(0.0, 0.0);
+                }
             }
         }
         return;
@@ -160,6 +173,16 @@ __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a, int nb)
     const int ia = blk - ja * (ja + 1) / 2;
     const int il = tid / 81, e = tid - 81 * il;          // il < 3 for the 243 working threads
     const int o = e % 9, b = e / 9;
+    // SK tables, on-site terms (matrices 0, 2) and diagonal constants (1, 3)
+    auto table = [&](const StructDev &SD, int m) -> const double * {
+        if (NM == 2) return m == 0 ? (dv ? SD.edh : SD.eh) : (dv ? SD.eds : SD.es);
+        return m == 0 ? SD.eh : (m == 1 ? SD.es : (m == 2 ? SD.edh : SD.eds));
+    };
+    auto diag_value = [&](const StructDev &SD, int m, double acc, int cls) -> double {
+        if (m == 0) return acc + (dv ? SD.dparos : SD.paros)[4 * ia + cls];
+        if (m == 2) return acc + SD.dparos[4 * ia + cls];
+        return acc + ((m == 1 && !dv) ? 1.0 : 0.0);
+    };
     // chunk of one structure: the block's SK entries and the items' phases
     // are staged through LDS a few pairs at a time, shared by the three
     // groups (every thread takes part in the loads and barriers)
@@ -167,26 +190,33 @@ __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a, int nb)
     for (int kb = it0 + 1; kb < it1; ++kb)
         if (a.item_s[kb] != a.item_s[it0]) chunk_same = false;
     if (chunk_same) {
-        constexpr int PC = 16;                           // pairs per stage
-        __shared__ double seh[PC * 81], ses[PC * 81];
+        constexpr int PC = NM == 4 ? 8 : 16;             // pairs per stage
+        // the staged SK entries, and after the sums the LDS transpose of the
+        // stores, share one buffer
+        constexpr int kSe = NM * PC * 81, kTr = NM * 3 * 81 * 2;
+        __shared__ double smem[kSe > kTr ? kSe : kTr];
+        double (*se)[PC * 81] = reinterpret_cast<double (*)[PC * 81]>(smem);
         __shared__ double2 sph[kBuildItems][PC];
         const StructDev &SD = a.sd[a.item_s[it0]];
         if (ja >= SD.natoms) return;                     // uniform over the workgroup
         const int kf = it0 + il;
         const bool active = il < 3 && kf < it1 && o < SD.nbas[ia] && b < SD.nbas[ja] && !(ia == ja && b < o);
         const int p0 = SD.blk_ptr[ia * SD.natoms + ja], p1 = SD.blk_ptr[ia * SD.natoms + ja + 1];
-        const double *eh = a.deriv ? SD.edh : SD.eh, *es = a.deriv ? SD.eds : SD.es;
+        const double *et[NM];
+#pragma unroll
+        for (int m = 0; m < NM; ++m) et[m] = table(SD, m);
         const int idx = 9 * o + b, nit = it1 - it0;
         constexpr int U = kBuildItems / 3;
-        double acc[U][4];
+        double acc[U][2 * NM];
 #pragma unroll
-        for (int u = 0; u < U; ++u) acc[u][0] = acc[u][1] = acc[u][2] = acc[u][3] = 0.0;
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int q = 0; q < 2 * NM; ++q) acc[u][q] = 0.0;
         for (int pc = p0; pc < p1; pc += PC) {
             const int np = min(PC, p1 - pc);
-            for (int t = tid; t < 81 * np; t += 256) {
-                seh[t] = eh[81 * (size_t)pc + t];
-                ses[t] = es[81 * (size_t)pc + t];
-            }
+            for (int t = tid; t < 81 * np; t += 256)
+#pragma unroll
+                for (int m = 0; m < NM; ++m) se[m][t] = et[m][81 * (size_t)pc + t];
             for (int t = tid; t < nit * np; t += 256) {
                 const int u = t / np, q = t - u * np;
                 sph[u][q] = a.ph[(size_t)(it0 + u) * a.maxpairs + pc + q];
@@ -194,14 +224,19 @@ __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a, int nb)
             __syncthreads();
             if (active) {
                 for (int q = 0; q < np; ++q) {
-                    const double e1 = seh[81 * q + idx], e2 = ses[81 * q + idx];
+                    double ev[NM];
+#pragma unroll
+                    for (int m = 0; m < NM; ++m) ev[m] = se[m][81 * q + idx];
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
                         const int kb = kf + 3 * u;
                         if (kb < it1) {
                             const double2 z = sph[kb - it0][q];
-                            acc[u][0] = acc[u][0] + e1 * z.x; acc[u][1] = acc[u][1] + e1 * z.y;
-                            acc[u][2] = acc[u][2] + e2 * z.x; acc[u][3] = acc[u][3] + e2 * z.y;
+#pragma unroll
+                            for (int m = 0; m < NM; ++m) {
+                                acc[u][2 * m] = acc[u][2 * m] + ev[m] * z.x;
+                                acc[u][2 * m + 1] = acc[u][2 * m + 1] + ev[m] * z.y;
+                            }
                         }
                     }
                 }
@@ -212,116 +247,73 @@ __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a, int nb)
         // thread (row orbital fastest: 9-row column segments), the
         // conjugate partner through an LDS transpose of the block so that
         // those stores are 9-row column segments too
-        __shared__ double2 tr_h[3][81], tr_s[3][81];
+        double2 (*tr)[3][81] = reinterpret_cast<double2 (*)[3][81]>(smem);
         const int r0 = SD.ksk[ia], c0 = SD.ksk[ja];
         const int beta = e % 9, omega = e / 9;           // transposed role of this thread
         const bool tr_ok = il < 3 && omega < SD.nbas[ia] && beta < SD.nbas[ja] && (ia < ja || beta > omega);
+        const int cls = o == 0 ? 0 : (o <= 3 ? 1 : (o <= 6 ? 2 : 3));
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int kb = kf + 3 * u;
             const bool live = active && kb < it1;
-            double2 hv = make_double2(0.0, 0.0), sv = make_double2(0.0, 0.0);
-            if (live) {
-                if (ia == ja && b == o) {
-                    const double *onsite = a.deriv ? SD.dparos : SD.paros;
-                    const int cls = o == 0 ? 0 : (o <= 3 ? 1 : (o <= 6 ? 2 : 3));
-                    hv = make_double2(acc[u][0] + onsite[4 * ia + cls], 0.0);
-                    sv = make_double2(acc[u][2] + (a.deriv ? 0.0 : 1.0), 0.0);
-                } else {
-                    hv = make_double2(acc[u][0], acc[u][1]); sv = make_double2(acc[u][2], acc[u][3]);
+            double2 v[NM];
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                v[m] = make_double2(0.0, 0.0);
+                if (live) {
+                    v[m] = (ia == ja && b == o) ? make_double2(diag_value(SD, m, acc[u][2 * m], cls), 0.0)
+                                                : make_double2(acc[u][2 * m], acc[u][2 * m + 1]);
+                    out[m][(size_t)kb * a.stride + (size_t)(c0 + b) * N + r0 + o] = v[m];
                 }
-                double2 *H = a.h + (size_t)kb * a.stride, *S = a.s + (size_t)kb * a.stride;
-                H[(size_t)(c0 + b) * N + r0 + o] = hv;
-                S[(size_t)(c0 + b) * N + r0 + o] = sv;
+                if (il < 3) tr[m][il][e] = v[m];
             }
-            if (il < 3) { tr_h[il][e] = hv; tr_s[il][e] = sv; }
             __syncthreads();
             if (tr_ok && kb < it1) {
                 // partner of upper element (o, b) = (omega, beta): row c0 + beta, column r0 + omega
-                const double2 ph_ = tr_h[il][omega + 9 * beta], ps_ = tr_s[il][omega + 9 * beta];
-                double2 *H = a.h + (size_t)kb * a.stride, *S = a.s + (size_t)kb * a.stride;
-                H[(size_t)(r0 + omega) * N + c0 + beta] = make_double2(ph_.x, -ph_.y);
-                S[(size_t)(r0 + omega) * N + c0 + beta] = make_double2(ps_.x, -ps_.y);
+#pragma unroll
+                for (int m = 0; m < NM; ++m) {
+                    const double2 pv = tr[m][il][omega + 9 * beta];
+                    out[m][(size_t)kb * a.stride + (size_t)(r0 + omega) * N + c0 + beta] = make_double2(pv.x, -pv.y);
+                }
             }
             __syncthreads();
         }
         return;
     }
     if (il >= 3) return;                                  // no barriers below
-    // items it0 + il + 3u of this thread group; when they share one structure
-    // each pair's SK entries are loaded once for all of them
-    constexpr int U = kBuildItems / 3;
-    const int kf = it0 + il;
-    if (kf >= it1) return;
-    const int s0 = a.item_s[kf];
-    bool same = true;
-#pragma unroll
-    for (int u = 1; u < U; ++u) {
-        const int kb = kf + 3 * u;
-        if (kb < it1 && a.item_s[kb] != s0) same = false;
-    }
-    auto store = [&](int kb, const StructDev &SD, double hr, double hi, double sr, double si) {
-        const int r0 = SD.ksk[ia], c0 = SD.ksk[ja];
-        double2 hv, sv;
-        if (ia == ja && b == o) {            // diagonal: real, plus on-site terms
-            const double *onsite = a.deriv ? SD.dparos : SD.paros;
-            const int cls = o == 0 ? 0 : (o <= 3 ? 1 : (o <= 6 ? 2 : 3));
-            hv = make_double2(hr + onsite[4 * ia + cls], 0.0);
-            sv = make_double2(sr + (a.deriv ? 0.0 : 1.0), 0.0);
-        } else {
-            hv = make_double2(hr, hi); sv = make_double2(sr, si);
-        }
-        double2 *H = a.h + (size_t)kb * a.stride, *S = a.s + (size_t)kb * a.stride;
-        H[(size_t)(c0 + b) * N + r0 + o] = hv;
-        S[(size_t)(c0 + b) * N + r0 + o] = sv;
-        if (ia < ja || b > o) {              // conjugate partner below the diagonal
-            H[(size_t)(r0 + o) * N + c0 + b] = make_double2(hv.x, -hv.y);
-            S[(size_t)(r0 + o) * N + c0 + b] = make_double2(sv.x, -sv.y);
-        }
-    };
-    if (same) {
-        const StructDev &SD = a.sd[s0];
-        if (ja >= SD.natoms || o >= SD.nbas[ia] || b >= SD.nbas[ja] || (ia == ja && b < o)) return;
-        const int p0 = SD.blk_ptr[ia * SD.natoms + ja], p1 = SD.blk_ptr[ia * SD.natoms + ja + 1];
-        const double *eh = a.deriv ? SD.edh : SD.eh, *es = a.deriv ? SD.eds : SD.es;
-        const int idx = 9 * o + b;
-        double acc[U][4];
-#pragma unroll
-        for (int u = 0; u < U; ++u) acc[u][0] = acc[u][1] = acc[u][2] = acc[u][3] = 0.0;
-        for (int p = p0; p < p1; ++p) {
-            const double e1 = eh[81 * (size_t)p + idx], e2 = es[81 * (size_t)p + idx];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int kb = kf + 3 * u;
-                if (kb < it1) {
-                    const double2 z = a.ph[(size_t)kb * a.maxpairs + p];
-                    acc[u][0] = acc[u][0] + e1 * z.x; acc[u][1] = acc[u][1] + e1 * z.y;
-                    acc[u][2] = acc[u][2] + e2 * z.x; acc[u][3] = acc[u][3] + e2 * z.y;
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int kb = kf + 3 * u;
-            if (kb < it1) store(kb, SD, acc[u][0], acc[u][1], acc[u][2], acc[u][3]);
-        }
-        return;
-    }
-    for (int kb = kf; kb < it1; kb += 3) {   // items of several structures: one at a time
+    // items it0 + il + 3u of this thread group, one at a time (several
+    // structures in the chunk)
+    const int cls = o == 0 ? 0 : (o <= 3 ? 1 : (o <= 6 ? 2 : 3));
+    for (int kb = it0 + il; kb < it1; kb += 3) {
         const StructDev &SD = a.sd[a.item_s[kb]];
         if (ja >= SD.natoms || o >= SD.nbas[ia] || b >= SD.nbas[ja] || (ia == ja && b < o)) continue;
         const int p0 = SD.blk_ptr[ia * SD.natoms + ja], p1 = SD.blk_ptr[ia * SD.natoms + ja + 1];
-        const double *eh = a.deriv ? SD.edh : SD.eh, *es = a.deriv ? SD.eds : SD.es;
+        const double *et[NM];
+#pragma unroll
+        for (int m = 0; m < NM; ++m) et[m] = table(SD, m);
         const int idx = 9 * o + b;
         const double2 *ph = a.ph + (size_t)kb * a.maxpairs;
-        double hr = 0.0, hi = 0.0, sr = 0.0, si = 0.0;
+        double acc[2 * NM];
+#pragma unroll
+        for (int q = 0; q < 2 * NM; ++q) acc[q] = 0.0;
         for (int p = p0; p < p1; ++p) {
             const double2 z = ph[p];
-            const double e1 = eh[81 * (size_t)p + idx], e2 = es[81 * (size_t)p + idx];
-            hr = hr + e1 * z.x; hi = hi + e1 * z.y;
-            sr = sr + e2 * z.x; si = si + e2 * z.y;
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                const double ev = et[m][81 * (size_t)p + idx];
+                acc[2 * m] = acc[2 * m] + ev * z.x;
+                acc[2 * m + 1] = acc[2 * m + 1] + ev * z.y;
+            }
         }
-        store(kb, SD, hr, hi, sr, si);
+        const int r0 = SD.ksk[ia], c0 = SD.ksk[ja];
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            const double2 v = (ia == ja && b == o) ? make_double2(diag_value(SD, m, acc[2 * m], cls), 0.0)
+                                                   : make_double2(acc[2 * m], acc[2 * m + 1]);
+            double2 *M = out[m] + (size_t)kb * a.stride;
+            M[(size_t)(c0 + b) * N + r0 + o] = v;
+            if (ia < ja || b > o) M[(size_t)(r0 + o) * N + c0 + b] = make_double2(v.x, -v.y);   // conjugate partner
+        }
     }
 }
 
@@ -3284,7 +3276,12 @@ void launch_spinor_expand(const BuildArgs &a, int maxnv, int nb, hipStream_t st)
 void launch_build_hs(const BuildArgs &a, int maxnat, int nb, hipStream_t st)
 {
     const int nblk = maxnat * (maxnat + 1) / 2 + kBuildPadWgs;     // upper blocks + padding
-    hipLaunchKernelGGL(k_build_hs, dim3(nblk, (nb + kBuildItems - 1) / kBuildItems), dim3(256), 0, st, a, nb);
+    if (a.dh)
+        hipLaunchKernelGGL(k_build_hs<4>, dim3(nblk, (nb + build_items<4>() - 1) / build_items<4>()), dim3(256), 0,
+                           st, a, nb);
+    else
+        hipLaunchKernelGGL(k_build_hs<2>, dim3(nblk, (nb + build_items<2>() - 1) / build_items<2>()), dim3(256), 0,
+                           st, a, nb);
 }
 void launch_potrf_panel(int n, double2 *s, size_t stride, int j0, int *info, int nb,
                         hipStream_t st)

@@ -85,6 +85,8 @@ struct Workspace {
     double2 *A = nullptr, *S = nullptr, *Y = nullptr, *Z = nullptr, *X = nullptr;
     double2 *Vp = nullptr, *Wp = nullptr, *Tf = nullptr, *px = nullptr, *py = nullptr;
     double2 *VT = nullptr;
+    // dH/dV, dS/dV of the pressure modes, built with H and S in one pass
+    double2 *dA = nullptr, *dS = nullptr;
     double *d = nullptr, *e = nullptr, *Vt = nullptr, *scratch = nullptr, *w = nullptr;
     int *info = nullptr, *sinfo = nullptr;
     // mixed precision: FP32 copies for the reduction and back-transform
@@ -122,6 +124,13 @@ struct Workspace {
         rc |= dalloc(&Vt, nn);
         rc |= dalloc(&scratch, (size_t)5 * n * tbk::stein_threads(n) * b);
         rc |= dalloc(&info, (size_t)b); rc |= dalloc(&sinfo, (size_t)b);
+        return rc ? TBK_ERR_NOMEM : TBK_OK;
+    }
+    int alloc_deriv()
+    {
+        const size_t nn = (size_t)n * n * b;
+        const int rc = dalloc(&dA, nn) | dalloc(&dS, nn);
+        if (rc) { dfree(dA); dfree(dS); }
         return rc ? TBK_ERR_NOMEM : TBK_OK;
     }
     int alloc_mixed()
@@ -187,6 +196,7 @@ struct Workspace {
         dfree(wv); dfree(wcol); dfree(wcarry); dfree(wydot); dfree(wyax); dfree(wudot); dfree(wscal);
         dfree(wv32); dfree(wcol32); dfree(wcarry32); dfree(wydot32); dfree(wyax32); dfree(wudot32); dfree(wscal32);
         dfree(A); dfree(S); dfree(Y); dfree(Z); dfree(X); dfree(VT); dfree(Vp); dfree(Wp); dfree(Tf);
+        dfree(dA); dfree(dS);
         dfree(px); dfree(py); dfree(d); dfree(e); dfree(w); dfree(Vt); dfree(scratch);
         dfree(info); dfree(sinfo);
         dfree(A32); dfree(P32); dfree(T32); dfree(VT32); dfree(Y32); dfree(X32);
@@ -656,6 +666,11 @@ struct tbk_plan {
     int *ncut = nullptr, *err = nullptr, *info_all = nullptr;
     int *cut = nullptr;             // [nk] occupation window of the last windowed run
     bool window = true, win_last = false, vec_last = true;
+    // pressure modes: H, S, dH/dV, dS/dV in one build pass (ws.dA, ws.dS,
+    // allocated on the first pressure run) instead of dH, dS after the
+    // solve; off by default: at order 288 the one pass takes 1.5 ms against
+    // 2 x 0.6 ms for the two
+    bool fused_build = false;
     std::vector<int> info_host, err_host, status;
     std::vector<double> out_host;
     Workspace ws;
@@ -1082,7 +1097,9 @@ extern "C" int tbk_plan_create(int device, const tbk_structure_desc *st, int nk,
 // Ry, and exactly decoupled from the active block.
 constexpr double kPadLevel = 1.0e4;
 
-static tbk::BuildArgs build_args(tbk_plan *p, int k0, bool deriv)
+// H, S (deriv false) or dH/dV, dS/dV (deriv true) into ws.A, ws.S; with
+// four, all of them, the derivatives into ws.dA, ws.dS.
+static tbk::BuildArgs build_args(tbk_plan *p, int k0, bool deriv, bool four = false)
 {
     tbk::BuildArgs a;
     a.n = p->N; a.deriv = deriv ? 1 : 0; a.maxpairs = p->maxpairs;
@@ -1092,7 +1109,24 @@ static tbk::BuildArgs build_args(tbk_plan *p, int k0, bool deriv)
     a.item_s = p->item_s + k0;
     a.ph = p->ph;
     a.h = p->ws.A; a.s = p->ws.S;
+    if (four) { a.dh = p->ws.dA; a.ds = p->ws.dS; }
     return a;
+}
+// The derivative pair of a four-matrix build, for the spinor expansion.
+static tbk::BuildArgs deriv_args(tbk_plan *p, int k0)
+{
+    tbk::BuildArgs a = build_args(p, k0, true);
+    a.h = p->ws.dA; a.s = p->ws.dS;
+    return a;
+}
+// Buffers of the four-matrix build, on the first pressure run (a plan
+// that cannot hold them builds dH, dS after the solve instead).
+static void ensure_deriv_buffers(tbk_plan *p)
+{
+    if (!p->fused_build || p->ws.dA) return;
+    AllocStream as(p->stream);
+    const size_t before = g_dev_bytes;
+    if (p->ws.alloc_deriv() == TBK_OK) p->bytes += g_dev_bytes - before;
 }
 
 // Everything one run enqueues on the plan's stream (captured into a
@@ -1115,12 +1149,19 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
     const Window wnd{win ? p->fargs : nullptr, p->nfermi, win ? p->cut : nullptr, vecs};
     p->win_last = win;
     p->vec_last = vecs;
+    // pressure modes: dH/dV, dS/dV from the same pass as H, S when the
+    // plan holds their buffers
+    const bool fused = lpress && p->fused_build && p->ws.dA;
+    const double2 *dH = fused ? p->ws.dA : p->ws.A, *dS = fused ? p->ws.dS : p->ws.S;
     for (int k0 = 0; k0 < p->nk; k0 += p->batch) {
         const int nb = std::min(p->batch, p->nk - k0);
         p->timer.begin("build_hs");
         tbk::launch_phases(p->sdev, p->item_s + k0, p->kp + 3 * (size_t)k0, p->maxpairs, p->ph, nb, st);
-        tbk::launch_build_hs(build_args(p, k0, false), p->maxnat, nb, st);
-        if (p->noncol) tbk::launch_spinor_expand(build_args(p, k0, false), p->maxnorb, nb, st);
+        tbk::launch_build_hs(build_args(p, k0, false, fused), p->maxnat, nb, st);
+        if (p->noncol) {
+            tbk::launch_spinor_expand(build_args(p, k0, false), p->maxnorb, nb, st);
+            if (fused) tbk::launch_spinor_expand(deriv_args(p, k0), p->maxnorb, nb, st);
+        }
         p->timer.end();
         const int *nact = p->nact + k0;
         if (p->precision == TBK_MIXED)
@@ -1149,11 +1190,13 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
         }
         if (lpress) {
             p->timer.begin("pressure");
-            tbk::launch_build_hs(build_args(p, k0, true), p->maxnat, nb, st);
-            if (p->noncol) tbk::launch_spinor_expand(build_args(p, k0, true), p->maxnorb, nb, st);
-            tbk::launch_zgemm(with_ncol(gemm(n, n, n, p->ws.A, n, ss, p->ws.Z, n, ss, p->ws.X, n, ss), wnd.cut), 0, 0,
+            if (!fused) {
+                tbk::launch_build_hs(build_args(p, k0, true), p->maxnat, nb, st);
+                if (p->noncol) tbk::launch_spinor_expand(build_args(p, k0, true), p->maxnorb, nb, st);
+            }
+            tbk::launch_zgemm(with_ncol(gemm(n, n, n, dH, n, ss, p->ws.Z, n, ss, p->ws.X, n, ss), wnd.cut), 0, 0,
                               nb, st);
-            tbk::launch_zgemm(with_ncol(gemm(n, n, n, p->ws.S, n, ss, p->ws.Z, n, ss, p->ws.Y, n, ss), wnd.cut), 0, 0,
+            tbk::launch_zgemm(with_ncol(gemm(n, n, n, dS, n, ss, p->ws.Z, n, ss, p->ws.Y, n, ss), wnd.cut), 0, 0,
                               nb, st);
             tbk::launch_coldot(n, p->ws.Z, p->ws.X, p->ws.Y, ss, p->ws.px, p->ws.py, nb, st, wnd.cut);
             tbk::PerturbArgs pa;
@@ -1196,6 +1239,7 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
         p->ws2.two_stage = p->ws.two_stage;
         p->bytes += g_dev_bytes - before;
     }
+    if (mode == 2 || mode == 4) ensure_deriv_buffers(p);
     p->last_mode = mode;
     const bool params_changed =
         !(p->g_mode == mode && p->g_tkb == tkb && p->g_eigcut == eigcut && p->g_window == p->window);
@@ -1462,6 +1506,14 @@ extern "C" int tbk_plan_set_tridiag(tbk_plan *p, int stages)
     return TBK_OK;
 }
 
+extern "C" int tbk_plan_set_fused_build(tbk_plan *p, int on)
+{
+    if (!p) return TBK_ERR_ARG;
+    p->fused_build = on != 0;
+    if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+    return TBK_OK;
+}
+
 extern "C" int tbk_plan_set_window(tbk_plan *p, int on)
 {
     if (!p) return TBK_ERR_ARG;
@@ -1473,13 +1525,26 @@ extern "C" int tbk_plan_hs(tbk_plan *p, int k, int deriv, double *h, double *s)
 {
     if (!p || k < 0 || k >= p->nk) return TBK_ERR_ARG;
     HIP_TRY(hipSetDevice(p->device));
+    // the build the plan's runs use: four matrices in one pass when it
+    // holds their buffers (pressure runs), else the pair asked for
+    ensure_deriv_buffers(p);
+    const bool four = p->fused_build && p->ws.dA;
     tbk::launch_phases(p->sdev, p->item_s + k, p->kp + 3 * (size_t)k, p->maxpairs, p->ph, 1, p->stream);
-    tbk::launch_build_hs(build_args(p, k, deriv != 0), p->maxnat, 1, p->stream);
-    if (p->noncol) tbk::launch_spinor_expand(build_args(p, k, deriv != 0), p->maxnorb, 1, p->stream);
+    if (four) {
+        tbk::launch_build_hs(build_args(p, k, false, true), p->maxnat, 1, p->stream);
+        if (p->noncol) {
+            tbk::launch_spinor_expand(build_args(p, k, false), p->maxnorb, 1, p->stream);
+            tbk::launch_spinor_expand(deriv_args(p, k), p->maxnorb, 1, p->stream);
+        }
+    } else {
+        tbk::launch_build_hs(build_args(p, k, deriv != 0), p->maxnat, 1, p->stream);
+        if (p->noncol) tbk::launch_spinor_expand(build_args(p, k, deriv != 0), p->maxnorb, 1, p->stream);
+    }
     HIP_TRY(hipStreamSynchronize(p->stream));
     const int nv = p->sh[p->item_s_host[k]].nv;
-    if (h && copy_leading((double2 *)h, p->ws.A, p->N, nv, p->stream)) return TBK_ERR_HIP;
-    if (s && copy_leading((double2 *)s, p->ws.S, p->N, nv, p->stream)) return TBK_ERR_HIP;
+    const double2 *hs = four && deriv ? p->ws.dA : p->ws.A, *ss = four && deriv ? p->ws.dS : p->ws.S;
+    if (h && copy_leading((double2 *)h, hs, p->N, nv, p->stream)) return TBK_ERR_HIP;
+    if (s && copy_leading((double2 *)s, ss, p->N, nv, p->stream)) return TBK_ERR_HIP;
     return TBK_OK;
 }
 

@@ -135,7 +135,8 @@ struct tbk_server {
 };
 
 // Device bytes of a request solved at padded order N: per-k workspace and
-// results (mixed precision adds the FP32 copies), its set-up arrays, slack.
+// results (mixed precision adds the FP32 copies and the refinement's
+// scratch), its set-up arrays, slack.
 static long long estimate_bytes(int N, int nk, size_t setup_bytes, int precision)
 {
     const long long per_item = (precision == TBK_MIXED ? 180LL * N * N + 384LL * N : 148LL * N * N + 64LL * N) + 4096;

@@ -30,7 +30,7 @@ EXPORTS = [
     "tbk_plan_stage_times", "tbk_plan_set_timing", "tbk_plan_autotune", "tbk_gemm_tile",
     "tbk_plan_create_group", "tbk_plan_result", "tbk_plan_structure_eigenvalues",
     "tbk_plan_create_spin", "tbk_plan_magnetization", "tbk_plan_create_noncol",
-    "tbk_plan_set_graph", "tbk_plan_set_window", "tbk_plan_set_tridiag", "tbk_plan_order", "tbk_plan_nstruct", "tbk_plan_device_bytes",
+    "tbk_plan_set_graph", "tbk_plan_set_window", "tbk_plan_set_fused_build", "tbk_plan_set_tridiag", "tbk_plan_order", "tbk_plan_nstruct", "tbk_plan_device_bytes",
     "tbk_multi_create", "tbk_multi_launch", "tbk_multi_wait", "tbk_multi_run", "tbk_multi_result",
     "tbk_multi_eigenvalues", "tbk_multi_ngroups", "tbk_multi_group_order", "tbk_multi_set_graph",
     "tbk_multi_autotune", "tbk_multi_device_bytes", "tbk_multi_destroy", "tbk_plan_query",
@@ -132,6 +132,7 @@ def lib():
         L.tbk_plan_set_graph.argtypes = [vp, C.c_int]
         L.tbk_plan_set_tridiag.argtypes = [vp, C.c_int]
         L.tbk_plan_set_window.argtypes = [vp, C.c_int]
+        L.tbk_plan_set_fused_build.argtypes = [vp, C.c_int]
         L.tbk_plan_order.argtypes = [vp]
         L.tbk_plan_nstruct.argtypes = [vp]
         L.tbk_plan_device_bytes.argtypes = [vp]

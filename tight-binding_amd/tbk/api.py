@@ -151,6 +151,11 @@ class Plan:
         batch form eigenvectors and derivatives only below E_F + 20 kT."""
         check(lib().tbk_plan_set_window(self.h, int(on)), "tbk_plan_set_window")
 
+    def set_fused_build(self, on: bool = True):
+        """Pressure modes: H, S, dH/dV, dS/dV in one build pass (on), or
+        dH/dV, dS/dV after the eigensolve with no extra memory (default)."""
+        check(lib().tbk_plan_set_fused_build(self.h, int(on)), "tbk_plan_set_fused_build")
+
     def device_bytes(self) -> int:
         return int(lib().tbk_plan_device_bytes(self.h))
 

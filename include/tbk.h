@@ -420,9 +420,12 @@ int tbk_plan_set_fused_build(tbk_plan *plan, int on);
  * both replace ZHPTRD, lapack.f:3218, as diag.f:396-412 calls it):
  * 1 one-stage blocked Householder panels; 2 two-stage (reduction to a band
  * of width 16 by panel QR and GEMMs, then bulge chasing to tridiagonal);
- * 0 the default (two-stage from order 512, in both methods).  Results
- * agree to rounding (FP64) or to the mixed method's stated errors.
- * Two-stage needs order >= 40 (TBK_ERR_ARG below). */
+ * 3 resident (unblocked Householder steps with the whole matrix held on
+ * chip, one workgroup per k-point: orders up to 288 in the mixed method,
+ * 160 in FP64); 0 the default (resident where the order allows it, else
+ * two-stage from order 512, else one-stage).  Results agree to rounding
+ * (FP64) or to the mixed method's stated errors.  Two-stage needs order
+ * >= 40, resident the orders above (TBK_ERR_ARG otherwise). */
 int tbk_plan_set_tridiag(tbk_plan *plan, int stages);
 
 /* H(k), S(k) of k-point k as built on the device (full Hermitian,

@@ -214,6 +214,12 @@ void launch_hetrd_wide_column(const HetrdArgsT<float> &h, const WideArgsT<float>
 void launch_hetrd_wide_vt(const HetrdArgs &h, int i0, int nb, hipStream_t st);
 void launch_hetrd_wide_vt(const HetrdArgsT<float> &h, int i0, int nb, hipStream_t st);
 void launch_hetrd_finish(const HetrdArgsT<float> &h, int nb, hipStream_t st);
+// resident form (the whole reduction of an item in one workgroup, matrix in
+// registers; reflectors, WY factors and V T^H as the panels leave them) for
+// orders up to hetrd_resident_max_n
+int hetrd_resident_max_n(bool fp32);
+void launch_hetrd_resident(const HetrdArgsT<float> &h, int nb, hipStream_t st);
+void launch_hetrd_resident(const HetrdArgs &h, int nb, hipStream_t st);
 int stein_threads(int n);        // eigenvector threads per k-point
 void launch_stein(const SteinArgs &s, int nb, hipStream_t st);          // eigenvalues and vectors
 void launch_stebz(const SteinArgs &s, int nb, hipStream_t st);          // eigenvalues only

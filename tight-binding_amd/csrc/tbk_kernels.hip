@@ -2652,6 +2652,347 @@ __device__ __forceinline__ void larfg_coef(C alpha, R xn2, C &tau, R &beta, C &s
     scale = crecip(cmake<C, R>(alpha.x - beta, alpha.y));
 }
 
+// ---------------------------------------------------------------------------
+// Resident Householder tridiagonalisation (zhetd2's unblocked steps), one
+// workgroup of 512 threads per item, for orders up to 16 * NTM: the lower
+// triangle of the item's matrix stays in registers for the whole reduction,
+// as 16 x 16 tiles (diagonal tiles held in full, both triangles), so a
+// column costs register arithmetic and LDS traffic only -- no pass over
+// the trailing matrix in memory, no delayed update.
+//
+// Tiles are numbered from the last tile column leftwards, t = cnt(J + 1) +
+// I - J with cnt(x) = (NT - x)(NT - x + 1) / 2 the tiles of columns >= x,
+// and wave w holds tiles w, w + 8, ...: the live tiles (columns beyond the
+// current one) are always a prefix of that order, so every wave keeps an
+// equal share to the end.  Lane l of a wave holds column c = l & 15 and rows
+// 4 g .. 4 g + 3 (g = l >> 4) of each of its tiles.
+//
+// Column j: (U) A -= v w^H + w v^H with the previous column's v, w (exact
+// zeros outside the trailing rows, so dead parts of a tile stay untouched
+// and add nothing), column j copied out; (R) the reflector, computed by
+// every wave from the same LDS copy (identical results), v into LDS and into
+// column j of A; (H) y = A v: per tile the row sums A_IJ v_J (reduced over a
+// 16-lane DPP row) and, below the diagonal, the column sums A_IJ^H v_I
+// (reduced over the four rows by lane-half swaps) into one LDS slot each,
+// and v^H A v;
+// (W) y_r from the slots of its tile row and tile column in a fixed order,
+// w = tau y - |tau|^2 (v^H A v) / 2 v.  Deterministic: no atomics.
+// The current 32-column panel's reflectors are kept in LDS: their dots with
+// each new v (16 threads per dot) give the WY factor's next column, and at
+// the panel's end T and V T^H are written.  Outputs as k_hetrd_panel's: the
+// reflectors in A, d, e, T and V T^H of every panel.
+// ---------------------------------------------------------------------------
+constexpr int RZ_THREADS = 512, RZ_WAVES = RZ_THREADS / 64;
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum over the 16 lanes of a DPP row (c = lane & 15) of 8 values per lane,
+// transposed: each halving step keeps half of the values and adds the
+// partner's copies of them (partners c ^ 8, c ^ 7, c ^ 3, c ^ 1: row_ror:8,
+// row_half_mirror, quad_perm [3,2,1,0], [1,0,3,2]).  Lane c returns the
+// total of value (c >> 1) & 7.
+template <class R>
+__device__ __forceinline__ R row16_sum8(R (&v)[8], int c)
+{
+    const bool b3 = (c & 8) != 0, b2 = (c & 4) != 0, b1 = (c & 2) != 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        R lo = v[q], hi = v[q + 4];
+        asm volatile("" : "+v"(lo), "+v"(hi));
+        v[q] = (b3 ? hi : lo) + dpp_mov<0x128>(b3 ? lo : hi);
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        R lo = v[q], hi = v[q + 2];
+        asm volatile("" : "+v"(lo), "+v"(hi));
+        v[q] = (b2 ? hi : lo) + dpp_mov<0x141>(b2 ? lo : hi);
+    }
+    {
+        R lo = v[0], hi = v[1];
+        asm volatile("" : "+v"(lo), "+v"(hi));
+        v[0] = (b1 ? hi : lo) + dpp_mov<0x1B>(b1 ? lo : hi);
+    }
+    return v[0] + dpp_mov<0xB1>(v[0]);
+}
+
+__host__ __device__ constexpr int rz_cnt(int nt, int x) { return (nt - x) * (nt - x + 1) / 2; }
+
+// v + v(lane ^ 32) and v + v(lane ^ 16) through gfx950's lane-half swaps
+// (identical in both lanes of a pair: the addition commutes)
+__device__ __forceinline__ float xor32_sum(float v)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float xor16_sum(float v)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ double xor32_sum(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+    const double a = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+    const double c = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+    return a + c;
+}
+__device__ __forceinline__ double xor16_sum(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+    const double a = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+    const double c = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+    return a + c;
+}
+// Sum over the 16 lanes of each DPP row (row_ror 8, 4, 2, 1): every lane of
+// the row ends with the same value.
+template <class R>
+__device__ __forceinline__ R row16_sum(R v)
+{
+    v += dpp_mov<0x128>(v);
+    v += dpp_mov<0x124>(v);
+    v += dpp_mov<0x122>(v);
+    v += dpp_mov<0x121>(v);
+    return v;
+}
+// Sum over the 64 lanes of a wave, the same value in every lane.
+template <class R>
+__device__ __forceinline__ R wave_sum_fast(R v)
+{
+    return xor32_sum(xor16_sum(row16_sum(v)));
+}
+
+// LDS of the resident reduction: tile dimension NTM, NTM (NTM + 1) / 2 tiles
+template <class R, int NTM>
+struct ResLds {
+    typedef typename cplx<R>::type C;
+    static constexpr int LM = NTM * (NTM + 1) / 2;
+    C colb[16 * NTM];              // column j after its last update
+    C vb[16 * NTM], wb[16 * NTM];  // v and w of the current column
+    C slotR[LM][16], slotC[LM][16];
+    C Vp[NB][16 * NTM];            // the current panel's reflectors
+    C Ts[NB][NB + 1];              // its WY factor, Ts[row][col]
+    C Gc[NB];                      // V_l^H v of the current column
+    R red[RZ_WAVES];
+};
+
+template <class R, int NTM, int MT>
+__global__ __launch_bounds__(RZ_THREADS, 1) void k_hetrd_res(HetrdArgsT<R> h)
+{
+    typedef typename cplx<R>::type C;
+    typedef ResLds<R, NTM> L_t;
+    constexpr int LM = L_t::LM;
+    static_assert(LM <= RZ_WAVES * MT, "every tile needs a register slot");
+    static_assert(16 * NTM <= RZ_THREADS && 16 * NTM <= 40 * RZ_WAVES, "one row per thread");
+    static_assert(sizeof(L_t) <= 160 * 1024, "LDS of one CU");
+    __shared__ L_t S;
+    const int N = h.n, NT = (N + 15) >> 4;
+    const int kb = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int c = lane & 15, g = lane >> 4;
+    C *A = h.a + (size_t)kb * h.stride;
+    double *dd = h.d + (size_t)kb * N, *ee = h.e + (size_t)kb * N;
+    const C one = cmake<C, R>(R(1), R(0));
+
+    // tiles of this wave: (tI, tJ), tI = -1 where the index is beyond the last
+    int tI[MT], tJ[MT];
+#pragma unroll
+    for (int k = 0; k < MT; ++k) {
+        const int t = wv + RZ_WAVES * k;
+        int J = NT - 1;
+        while (J >= 0 && t >= rz_cnt(NT, J)) --J;
+        tJ[k] = J;
+        tI[k] = J >= 0 ? J + t - rz_cnt(NT, J + 1) : -1;
+    }
+    C x[MT][4];
+#pragma unroll
+    for (int k = 0; k < MT; ++k) {
+        const int cc = 16 * tJ[k] + c;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = 16 * tI[k] + 4 * g + i;
+            C v = czero<C>();
+            if (tI[k] >= 0 && r < N && cc < N) {
+                if (r > cc) v = A[(size_t)cc * N + r];
+                else if (r < cc) v = zconj(A[(size_t)r * N + cc]);
+                else v = cmake<C, R>(A[(size_t)cc * N + r].x, R(0));
+            }
+            x[k][i] = v;
+        }
+    }
+    for (int r = tid; r < 16 * NTM; r += RZ_THREADS) {
+        S.vb[r] = czero<C>(); S.wb[r] = czero<C>(); S.colb[r] = czero<C>();
+    }
+    __syncthreads();
+
+    for (int j = 0; j < N; ++j) {
+        // (U) the previous column's rank-2 update on tile columns >= j / 16;
+        // column j into colb
+        const int Ju = j >> 4, Pu = rz_cnt(NT, Ju), jc = j & 15;
+#pragma unroll
+        for (int k = 0; k < MT; ++k) {
+            if (wv + RZ_WAVES * k < Pu) {
+                // the tile coordinates pass through an opaque copy each
+                // column, so that no per-tile lane address is kept live
+                // across the column loop (they are recomputed at each use)
+                int I = tI[k], J = tJ[k];
+                asm volatile("" : "+s"(I), "+s"(J));
+                if (j > 0) {
+                    const C vJc = zconj(S.vb[16 * J + c]), wJc = zconj(S.wb[16 * J + c]);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int r = 16 * I + 4 * g + i;
+                        x[k][i] = zsub(x[k][i], zadd(zmul(S.vb[r], wJc), zmul(S.wb[r], vJc)));
+                    }
+                }
+                if (J == Ju && c == jc) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) S.colb[16 * I + 4 * g + i] = x[k][i];
+                }
+            }
+        }
+        __syncthreads();
+        if (j == N - 1) {                   // the last diagonal element
+            if (tid == 0) { dd[j] = (double)S.colb[j].x; ee[j] = 0.0; }
+            for (int r = tid; r < N; r += RZ_THREADS) A[(size_t)j * N + r] = czero<C>();
+            break;
+        }
+        const int pcol = j % NB, i0 = j - pcol;
+        // (R) reflector of x = colb[j+1 ..]: H^H x = beta e1, H = I - tau v v^H
+        R xs = 0;
+        for (int r = j + 2 + lane; r < N; r += 64) {
+            const C a = S.colb[r];
+            xs += a.x * a.x + a.y * a.y;
+        }
+        xs = wave_sum_fast(xs);
+        C tau, scale;
+        R beta;
+        larfg_coef<R, C>(S.colb[j + 1], xs, tau, beta, scale);
+        if (tid == 0) { dd[j] = (double)S.colb[j].x; ee[j] = (double)beta; }
+        {
+            const int r = 40 * wv + lane;
+            if (lane < 40 && r < 16 * NT) {
+                C v = czero<C>();
+                if (r == j + 1) v = one;
+                else if (r > j + 1 && r < N) v = zmul(S.colb[r], scale);
+                S.vb[r] = v;
+                S.Vp[pcol][r] = v;
+                if (r < N) A[(size_t)j * N + r] = v;
+            }
+        }
+        __syncthreads();
+        // (H) y = A v over the tile columns >= (j + 1) / 16; first the panel
+        // dots V_l^H v (l < pcol), 16 threads each over rows > j
+        {
+            const int l = tid >> 4, part = tid & 15;
+            if (l < pcol) {
+                C acc = czero<C>();
+                for (int r = j + 1 + part; r < N; r += 16) acc = zadd(acc, zcmul(S.Vp[l][r], S.vb[r]));
+                acc.x = row16_sum(acc.x);
+                acc.y = row16_sum(acc.y);
+                if (part == 0) S.Gc[l] = acc;
+            }
+        }
+        const int Jh = (j + 1) >> 4, Ph = rz_cnt(NT, Jh);
+        R vav = 0;
+#pragma unroll
+        for (int k = 0; k < MT; ++k) {
+            const int t = wv + RZ_WAVES * k;
+            if (t < Ph) {
+                int I = tI[k], J = tJ[k];
+                asm volatile("" : "+s"(I), "+s"(J));
+                const C vJ = S.vb[16 * J + c];
+                R vals[8];
+                R vv = 0;
+                C q = czero<C>();
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const C vI = S.vb[16 * I + 4 * g + i];
+                    const C pr = zmul(x[k][i], vJ);
+                    vals[2 * i] = pr.x;
+                    vals[2 * i + 1] = pr.y;
+                    vv += vI.x * pr.x + vI.y * pr.y;         // Re(conj(v_I) A v_J)
+                    q = zadd(q, zcmul(x[k][i], vI));          // conj(A) v_I
+                }
+                vav += I == J ? vv : R(2) * vv;
+                const R rs = row16_sum8<R>(vals, c);
+                if ((c & 1) == 0) reinterpret_cast<R *>(&S.slotR[t][4 * g + (c >> 2)])[(c >> 1) & 1] = rs;
+                if (I != J) {
+                    q.x = xor32_sum(xor16_sum(q.x));
+                    q.y = xor32_sum(xor16_sum(q.y));
+                    if (g == 0) S.slotC[t][c] = q;
+                }
+            }
+        }
+        vav = wave_sum_fast(vav);
+        if (lane == 0) S.red[wv] = vav;
+        __syncthreads();
+        // (W) w = tau y - |tau|^2 (v^H A v) / 2 v; the WY column
+        // T[0:pcol, pcol] = -tau T[0:pcol, 0:pcol] G, T[pcol, pcol] = tau
+        {
+            R s = 0;
+#pragma unroll
+            for (int q = 0; q < RZ_WAVES; ++q) s += S.red[q];
+            const R hw = R(-0.5) * (tau.x * tau.x + tau.y * tau.y) * s;
+            const int r = tid;
+            if (r < 16 * NT) {
+                C wr = czero<C>();
+                if (r > j && r < N) {
+                    const int I = r >> 4, a = r & 15;
+                    C y = czero<C>();
+                    for (int J = Jh; J <= I; ++J) y = zadd(y, S.slotR[rz_cnt(NT, J + 1) + I - J][a]);
+                    const int base = rz_cnt(NT, I + 1) - I;
+                    for (int I2 = I + 1; I2 < NT; ++I2) y = zadd(y, S.slotC[base + I2][a]);
+                    wr = zadd(zmul(tau, y), zscale(hw, S.vb[r]));
+                }
+                S.wb[r] = wr;
+            }
+            if (tid < pcol) {
+                C acc = czero<C>();
+                for (int q = tid; q < pcol; ++q) acc = zadd(acc, zmul(S.Ts[tid][q], S.Gc[q]));
+                S.Ts[tid][pcol] = zmul(cmake<C, R>(-tau.x, -tau.y), acc);
+            } else if (tid == pcol) {
+                S.Ts[pcol][pcol] = tau;
+            }
+        }
+        __syncthreads();
+        if (pcol == NB - 1 || j == N - 2) {
+            // the panel is complete: T and V T^H (rows > i0) of its reflectors
+            const int pw = pcol + 1;
+            C *Tp = h.t + (size_t)kb * h.tstride + (size_t)(i0 / NB) * NB * NB;
+            for (int q = tid; q < NB * NB; q += RZ_THREADS) {
+                const int row = q % NB, col = q / NB;
+                Tp[q] = (row <= col && col < pw) ? S.Ts[row][col] : czero<C>();
+            }
+            C *VT = h.vt + (size_t)kb * h.stride + (size_t)i0 * N;
+            for (int q = tid; q < N * NB; q += RZ_THREADS) {
+                const int r = q % N, cc = q / N;
+                if (i0 + cc >= N) continue;
+                C acc = czero<C>();
+                if (cc < pw && r > i0)
+                    for (int l = cc; l < pw; ++l) acc = zadd(acc, zmul(S.Vp[l][r], zconj(S.Ts[cc][l])));
+                VT[(size_t)cc * N + r] = acc;
+            }
+            // (the next column rewrites Vp and Ts only after its first barrier)
+        }
+    }
+}
+
 // ---- stage 1: panel QR ------------------------------------------------------
 // One workgroup of NW waves per item; the panel's m = n - c0 - TB rows are
 // held in registers, row tid + k NT in slot k (RPT slots).  Column j: the
@@ -3390,6 +3731,30 @@ void launch_hetrd_wide_column(const HetrdArgsT<float> &h, const WideArgsT<float>
                               hipStream_t st) { launch_hetrd_wide_column_t(h, w, i0, i, nb, st); }
 void launch_hetrd_wide_vt(const HetrdArgs &h, int i0, int nb, hipStream_t st) { launch_hetrd_wide_vt_t(h, i0, nb, st); }
 void launch_hetrd_wide_vt(const HetrdArgsT<float> &h, int i0, int nb, hipStream_t st) { launch_hetrd_wide_vt_t(h, i0, nb, st); }
+// resident tridiagonalisation: tile dimension NTM and tiles per wave MT for
+// the order (FP64 up to 160: the panel's reflectors share the LDS)
+template <class R, int NTM, int MT>
+static void launch_hetrd_res_q(const HetrdArgsT<R> &h, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_hetrd_res<R, NTM, MT>), dim3(nb), dim3(RZ_THREADS), 0, st, h);
+}
+int hetrd_resident_max_n(bool fp32) { return fp32 ? 288 : 160; }
+// (callers keep h.n <= hetrd_resident_max_n: the plans choose the resident
+// form only there, and tbk_plan_set_tridiag refuses it beyond)
+void launch_hetrd_resident(const HetrdArgsT<float> &h, int nb, hipStream_t st)
+{
+    if (h.n > hetrd_resident_max_n(true)) return;
+    if (h.n <= 128) launch_hetrd_res_q<float, 8, 5>(h, nb, st);
+    else if (h.n <= 192) launch_hetrd_res_q<float, 12, 10>(h, nb, st);
+    else if (h.n <= 256) launch_hetrd_res_q<float, 16, 17>(h, nb, st);
+    else launch_hetrd_res_q<float, 18, 22>(h, nb, st);
+}
+void launch_hetrd_resident(const HetrdArgs &h, int nb, hipStream_t st)
+{
+    if (h.n > hetrd_resident_max_n(false)) return;
+    if (h.n <= 128) launch_hetrd_res_q<double, 8, 5>(h, nb, st);
+    else launch_hetrd_res_q<double, 10, 7>(h, nb, st);
+}
 void launch_hetrd_panel(const HetrdArgs &h, int i0, int nb, hipStream_t st) { launch_hetrd_panel_t(h, i0, nb, st); }
 void launch_hetrd_finish(const HetrdArgs &h, int nb, hipStream_t st) { launch_hetrd_finish_t(h, nb, st); }
 void launch_hetrd_panel(const HetrdArgsT<float> &h, int i0, int nb, hipStream_t st) { launch_hetrd_panel_t(h, i0, nb, st); }

@@ -101,6 +101,9 @@ struct Workspace {
     // two-stage tridiagonalisation (band reduction + bulge chasing) instead
     // of the one-stage panels
     bool two_stage = false;
+    // resident tridiagonalisation (the matrix held on chip, one workgroup per
+    // item) instead of the one-stage panels: the default where the order fits
+    bool resident = false;
     double2 *wv = nullptr, *wcol = nullptr, *wcarry = nullptr, *wydot = nullptr, *wyax = nullptr,
             *wudot = nullptr, *wscal = nullptr;
     float2 *wv32 = nullptr, *wcol32 = nullptr, *wcarry32 = nullptr, *wydot32 = nullptr, *wyax32 = nullptr,
@@ -113,6 +116,7 @@ struct Workspace {
     {
         n = n_; b = b_;
         two_stage = n >= kTwoStageMinN && two_stage_fits(n);
+        resident = n <= tbk::hetrd_resident_max_n(false);
         const size_t nn = (size_t)n * n * b, np = (size_t)n * kNB * b;
         int rc = TBK_OK;
         rc |= dalloc(&A, nn); rc |= dalloc(&S, nn); rc |= dalloc(&Y, nn);
@@ -136,6 +140,7 @@ struct Workspace {
     int alloc_mixed()
     {
         const size_t nn = (size_t)n * n * b;
+        resident = n <= tbk::hetrd_resident_max_n(true);
         int rc = TBK_OK;
         rc |= dalloc(&A32, nn); rc |= dalloc(&VT32, nn); rc |= dalloc(&Y32, nn); rc |= dalloc(&X32, nn);
         rc |= dalloc(&P32, (size_t)n * 96 * b); rc |= dalloc(&T32, tstride() * b);
@@ -167,6 +172,7 @@ struct Workspace {
     {
         n = n_; b = b_;
         two_stage = n >= kTwoStageMinN && two_stage_fits(n);
+        resident = n <= tbk::hetrd_resident_max_n(false);
         size_t off = 0;
         auto take = [&](auto *&p, size_t count) {
             typedef std::remove_reference_t<decltype(*p)> T;
@@ -286,6 +292,12 @@ void hetrd(Workspace &ws, int nb, hipStream_t st, StageTimer &tm)
     const int n = ws.n;
     const size_t ss = ws.stride(), sp = (size_t)n * 96;
     tbk::HetrdArgs h{n, ss, ws.A, ws.Vp, ws.Wp, ws.Tf, ws.VT, ws.d, ws.e, ws.tstride()};
+    if (ws.resident) {
+        tm.begin("hetrd_resident");
+        tbk::launch_hetrd_resident(h, nb, st);
+        tm.end();
+        return;
+    }
     for (int i0 = 0; i0 < n - 1; i0 += kNB) {
         const int pw = std::min(kNB, n - 1 - i0);
         tm.begin("hetrd_panel");
@@ -497,6 +509,11 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
         tm.begin("hetrd_bulge");
         tbk::launch_bulge(tbk::BulgeArgsT<float>{n, ws.Y32, ss, ws.X32, ss, ws.d, ws.e, (size_t)n, ws.px, (size_t)n},
                           nb, st);
+        tm.end();
+    } else if (ws.resident) {
+        tbk::HetrdArgsT<float> h{n, ss, ws.A32, ws.P32, nullptr, ws.T32, ws.VT32, ws.d, ws.e, ws.tstride()};
+        tm.begin("hetrd_resident");
+        tbk::launch_hetrd_resident(h, nb, st);
         tm.end();
     } else {
         tbk::HetrdArgsT<float> h{n, ss, ws.A32, ws.P32, nullptr, ws.T32, ws.VT32, ws.d, ws.e, ws.tstride()};
@@ -982,7 +999,8 @@ static int plan_create_from(int device, int nstruct, const tbk_prepared *const *
     }
     // Few items of large order: one workgroup per item would leave most CUs
     // idle in the tridiagonalisation, so it runs over tiles instead.
-    if (!rc && N >= kWideMinN && p->batch < kWideMaxBatch) rc = p->ws.alloc_wide(precision == TBK_MIXED);
+    if (!rc && N >= kWideMinN && p->batch < kWideMaxBatch && !p->ws.resident)
+        rc = p->ws.alloc_wide(precision == TBK_MIXED);
     if (!rc && hipStreamSynchronize(p->stream) != hipSuccess) rc = TBK_ERR_HIP;   // uploads done
     if (rc) { plan_free(p); return rc == TBK_ERR_HIP ? TBK_ERR_HIP : TBK_ERR_NOMEM; }
     p->bytes = g_dev_bytes;
@@ -1237,6 +1255,7 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
             return TBK_ERR_NOMEM;
         }
         p->ws2.two_stage = p->ws.two_stage;
+        if (p->ws.two_stage || !p->ws.resident) p->ws2.resident = false;
         p->bytes += g_dev_bytes - before;
     }
     if (mode == 2 || mode == 4) ensure_deriv_buffers(p);
@@ -1497,11 +1516,17 @@ extern "C" int tbk_plan_projections(tbk_plan *p, int k, double eigcut, int *ntot
 
 extern "C" int tbk_plan_set_tridiag(tbk_plan *p, int stages)
 {
-    if (!p || stages < 0 || stages > 2) return TBK_ERR_ARG;
-    const bool two = stages == 0 ? p->N >= kTwoStageMinN && two_stage_fits(p->N) : stages == 2;
+    if (!p || stages < 0 || stages > 3) return TBK_ERR_ARG;
+    const bool fits_res = p->N <= tbk::hetrd_resident_max_n(p->precision == TBK_MIXED);
+    const bool res = stages == 0 ? fits_res : stages == 3;
+    const bool two = stages == 0 ? !res && p->N >= kTwoStageMinN && two_stage_fits(p->N) : stages == 2;
     if (two && !two_stage_fits(p->N)) return TBK_ERR_ARG;
+    if (res && !fits_res) return TBK_ERR_ARG;
     p->ws.two_stage = two;
     p->ws2.two_stage = two;
+    p->ws.resident = res;
+    // the projections' eigendecomposition of S runs in FP64
+    p->ws2.resident = !two && (stages == 0 || stages == 3) && p->N <= tbk::hetrd_resident_max_n(false);
     if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
     return TBK_OK;
 }

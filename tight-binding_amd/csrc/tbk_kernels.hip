@@ -2795,6 +2795,7 @@ template <class R, int NTM, int MT>
 __global__ __launch_bounds__(RZ_THREADS, 1) void k_hetrd_res(HetrdArgsT<R> h)
 {
     typedef typename cplx<R>::type C;
+    typedef R v4 __attribute__((ext_vector_type(4)));
     typedef ResLds<R, NTM> L_t;
     constexpr int LM = L_t::LM;
     static_assert(LM <= RZ_WAVES * MT, "every tile needs a register slot");
@@ -2805,6 +2806,9 @@ __global__ __launch_bounds__(RZ_THREADS, 1) void k_hetrd_res(HetrdArgsT<R> h)
     const int kb = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int c = lane & 15, g = lane >> 4;
+    // row within its tile of this lane's accumulator item i (the MFMA
+    // accumulator layout of the precision; column c for every item)
+    auto rowoff = [&](int i) { return mfma_row(R(0), lane, i); };
     C *A = h.a + (size_t)kb * h.stride;
     double *dd = h.d + (size_t)kb * N, *ee = h.e + (size_t)kb * N;
     const C one = cmake<C, R>(R(1), R(0));
@@ -2819,20 +2823,21 @@ __global__ __launch_bounds__(RZ_THREADS, 1) void k_hetrd_res(HetrdArgsT<R> h)
         tJ[k] = J;
         tI[k] = J >= 0 ? J + t - rz_cnt(NT, J + 1) : -1;
     }
-    C x[MT][4];
+    // the lower triangle, real and imaginary parts as MFMA accumulators
+    // (the upper triangle of a diagonal tile from the lower one); every
+    // load is issued unconditionally from a clamped address
+    v4 xr[MT], xi[MT];
 #pragma unroll
     for (int k = 0; k < MT; ++k) {
         const int cc = 16 * tJ[k] + c;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int r = 16 * tI[k] + 4 * g + i;
-            C v = czero<C>();
-            if (tI[k] >= 0 && r < N && cc < N) {
-                if (r > cc) v = A[(size_t)cc * N + r];
-                else if (r < cc) v = zconj(A[(size_t)r * N + cc]);
-                else v = cmake<C, R>(A[(size_t)cc * N + r].x, R(0));
-            }
-            x[k][i] = v;
+            const int r = 16 * tI[k] + rowoff(i);
+            const bool ok = tI[k] >= 0 && r < N && cc < N;
+            const size_t off = !ok ? 0 : (r >= cc ? (size_t)cc * N + r : (size_t)r * N + cc);
+            const C a = A[off];
+            xr[k][i] = ok ? a.x : R(0);
+            xi[k][i] = !ok || r == cc ? R(0) : (r > cc ? a.y : -a.y);
         }
     }
     for (int r = tid; r < 16 * NTM; r += RZ_THREADS) {
@@ -2841,8 +2846,12 @@ __global__ __launch_bounds__(RZ_THREADS, 1) void k_hetrd_res(HetrdArgsT<R> h)
     __syncthreads();
 
     for (int j = 0; j < N; ++j) {
-        // (U) the previous column's rank-2 update on tile columns >= j / 16;
-        // column j into colb
+        // (U) the previous column's rank-2 update A -= v w^H + w v^H on tile
+        // columns >= j / 16, two MFMAs per tile with K = 4:
+        //   Re: [-v.x, -v.y, -w.x, -w.y]_r . [w.x, w.y, v.x, v.y]_c
+        //   Im: [-v.y,  v.x, -w.y,  w.x]_r . [w.x, w.y, v.x, v.y]_c
+        // (lane l supplies row l % 16 of the first operand and column l % 16
+        // of the second, component l / 16); column j into colb
         const int Ju = j >> 4, Pu = rz_cnt(NT, Ju), jc = j & 15;
 #pragma unroll
         for (int k = 0; k < MT; ++k) {
@@ -2853,16 +2862,17 @@ __global__ __launch_bounds__(RZ_THREADS, 1) void k_hetrd_res(HetrdArgsT<R> h)
                 int I = tI[k], J = tJ[k];
                 asm volatile("" : "+s"(I), "+s"(J));
                 if (j > 0) {
-                    const C vJc = zconj(S.vb[16 * J + c]), wJc = zconj(S.wb[16 * J + c]);
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int r = 16 * I + 4 * g + i;
-                        x[k][i] = zsub(x[k][i], zadd(zmul(S.vb[r], wJc), zmul(S.wb[r], vJc)));
-                    }
+                    const C pv = g < 2 ? S.vb[16 * I + c] : S.wb[16 * I + c];
+                    const C qv = g < 2 ? S.wb[16 * J + c] : S.vb[16 * J + c];
+                    const R are = (g & 1) ? -pv.y : -pv.x;
+                    const R aim = (g & 1) ? pv.x : -pv.y;
+                    const R bq = (g & 1) ? qv.y : qv.x;
+                    xr[k] = mfma16(are, bq, xr[k]);
+                    xi[k] = mfma16(aim, bq, xi[k]);
                 }
                 if (J == Ju && c == jc) {
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) S.colb[16 * I + 4 * g + i] = x[k][i];
+                    for (int i = 0; i < 4; ++i) S.colb[16 * I + rowoff(i)] = cmake<C, R>(xr[k][i], xi[k][i]);
                 }
             }
         }
@@ -2901,11 +2911,17 @@ __global__ __launch_bounds__(RZ_THREADS, 1) void k_hetrd_res(HetrdArgsT<R> h)
         {
             const int l = tid >> 4, part = tid & 15;
             if (l < pcol) {
-                C acc = czero<C>();
-                for (int r = j + 1 + part; r < N; r += 16) acc = zadd(acc, zcmul(S.Vp[l][r], S.vb[r]));
-                acc.x = row16_sum(acc.x);
-                acc.y = row16_sum(acc.y);
-                if (part == 0) S.Gc[l] = acc;
+                C a0 = czero<C>(), a1 = czero<C>();
+                int r = j + 1 + part;
+                for (; r + 16 < N; r += 32) {
+                    a0 = zadd(a0, zcmul(S.Vp[l][r], S.vb[r]));
+                    a1 = zadd(a1, zcmul(S.Vp[l][r + 16], S.vb[r + 16]));
+                }
+                if (r < N) a0 = zadd(a0, zcmul(S.Vp[l][r], S.vb[r]));
+                a0 = zadd(a0, a1);
+                a0.x = row16_sum(a0.x);
+                a0.y = row16_sum(a0.y);
+                if (part == 0) S.Gc[l] = a0;
             }
         }
         const int Jh = (j + 1) >> 4, Ph = rz_cnt(NT, Jh);
@@ -2922,16 +2938,18 @@ __global__ __launch_bounds__(RZ_THREADS, 1) void k_hetrd_res(HetrdArgsT<R> h)
                 C q = czero<C>();
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const C vI = S.vb[16 * I + 4 * g + i];
-                    const C pr = zmul(x[k][i], vJ);
+                    const C vI = S.vb[16 * I + rowoff(i)];
+                    const C x = cmake<C, R>(xr[k][i], xi[k][i]);
+                    const C pr = zmul(x, vJ);
                     vals[2 * i] = pr.x;
                     vals[2 * i + 1] = pr.y;
                     vv += vI.x * pr.x + vI.y * pr.y;         // Re(conj(v_I) A v_J)
-                    q = zadd(q, zcmul(x[k][i], vI));          // conj(A) v_I
+                    q = zadd(q, zcmul(x, vI));                // conj(A) v_I
                 }
                 vav += I == J ? vv : R(2) * vv;
                 const R rs = row16_sum8<R>(vals, c);
-                if ((c & 1) == 0) reinterpret_cast<R *>(&S.slotR[t][4 * g + (c >> 2)])[(c >> 1) & 1] = rs;
+                if ((c & 1) == 0)
+                    reinterpret_cast<R *>(&S.slotR[t][rowoff(c >> 2)])[(c >> 1) & 1] = rs;
                 if (I != J) {
                     q.x = xor32_sum(xor16_sum(q.x));
                     q.y = xor32_sum(xor16_sum(q.y));
@@ -2944,6 +2962,7 @@ __global__ __launch_bounds__(RZ_THREADS, 1) void k_hetrd_res(HetrdArgsT<R> h)
         __syncthreads();
         // (W) w = tau y - |tau|^2 (v^H A v) / 2 v; the WY column
         // T[0:pcol, pcol] = -tau T[0:pcol, 0:pcol] G, T[pcol, pcol] = tau
+        // (16 threads per row)
         {
             R s = 0;
 #pragma unroll
@@ -2962,17 +2981,22 @@ __global__ __launch_bounds__(RZ_THREADS, 1) void k_hetrd_res(HetrdArgsT<R> h)
                 }
                 S.wb[r] = wr;
             }
-            if (tid < pcol) {
-                C acc = czero<C>();
-                for (int q = tid; q < pcol; ++q) acc = zadd(acc, zmul(S.Ts[tid][q], S.Gc[q]));
-                S.Ts[tid][pcol] = zmul(cmake<C, R>(-tau.x, -tau.y), acc);
-            } else if (tid == pcol) {
-                S.Ts[pcol][pcol] = tau;
+            const int l = tid >> 4, part = tid & 15;
+            C acc = czero<C>();
+            for (int q = l + part; q < pcol; q += 16) acc = zadd(acc, zmul(S.Ts[l][q], S.Gc[q]));
+            acc.x = row16_sum(acc.x);
+            acc.y = row16_sum(acc.y);
+            if (part == 0) {
+                if (l < pcol) S.Ts[l][pcol] = zmul(cmake<C, R>(-tau.x, -tau.y), acc);
+                else if (l == pcol) S.Ts[pcol][pcol] = tau;
             }
         }
         __syncthreads();
         if (pcol == NB - 1 || j == N - 2) {
-            // the panel is complete: T and V T^H (rows > i0) of its reflectors
+            // the panel is complete: T, and V T^H (rows > i0) on the matrix
+            // cores, 16 x 16 output tiles (rows 16 rt, columns 16 ct) with
+            // K = pw in steps of 4; conj(T)^T entries outside 0 <= cc <= l < pw
+            // are zeros
             const int pw = pcol + 1;
             C *Tp = h.t + (size_t)kb * h.tstride + (size_t)(i0 / NB) * NB * NB;
             for (int q = tid; q < NB * NB; q += RZ_THREADS) {
@@ -2980,13 +3004,27 @@ __global__ __launch_bounds__(RZ_THREADS, 1) void k_hetrd_res(HetrdArgsT<R> h)
                 Tp[q] = (row <= col && col < pw) ? S.Ts[row][col] : czero<C>();
             }
             C *VT = h.vt + (size_t)kb * h.stride + (size_t)i0 * N;
-            for (int q = tid; q < N * NB; q += RZ_THREADS) {
-                const int r = q % N, cc = q / N;
-                if (i0 + cc >= N) continue;
-                C acc = czero<C>();
-                if (cc < pw && r > i0)
-                    for (int l = cc; l < pw; ++l) acc = zadd(acc, zmul(S.Vp[l][r], zconj(S.Ts[cc][l])));
-                VT[(size_t)cc * N + r] = acc;
+            const int ntile = NT * 2;
+            for (int ot = wv; ot < ntile; ot += RZ_WAVES) {
+                const int rt = ot >> 1, ct = ot & 1;
+                v4 p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
+                for (int k0 = 0; k0 < pw; k0 += 4) {
+                    const int l = k0 + g;
+                    const C a = l < pw ? S.Vp[l][16 * rt + c] : czero<C>();
+                    const int cc = 16 * ct + c;
+                    const C t = (l < pw && cc <= l) ? S.Ts[cc][l] : czero<C>();
+                    const R br = t.x, bi = -t.y;                 // conj(T[cc][l])
+                    p1 = mfma16(a.x, br, p1);
+                    p2 = mfma16(a.y, bi, p2);
+                    p3 = mfma16(a.x + a.y, br + bi, p3);
+                }
+                const int cc = 16 * ct + c;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int r = 16 * rt + rowoff(i);
+                    if (r < N && i0 + cc < N)
+                        VT[(size_t)cc * N + r] = cmake<C, R>(p1[i] - p2[i], p3[i] - p1[i] - p2[i]);
+                }
             }
             // (the next column rewrites Vp and Ts only after its first barrier)
         }

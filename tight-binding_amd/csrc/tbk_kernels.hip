@@ -2729,6 +2729,34 @@ __device__ __forceinline__ R row16_sum8(R (&v)[8], int c)
 
 __host__ __device__ constexpr int rz_cnt(int nt, int x) { return (nt - x) * (nt - x + 1) / 2; }
 
+// 1 / x: FP32 from the hardware reciprocal refined by one Newton step
+// (within an ulp; the mixed method's FP32 reduction is refined in FP64
+// afterwards), FP64 divided
+__device__ __forceinline__ float rz_rcp(float x)
+{
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(fmaf(-x, r, 1.0f), r, r);
+}
+__device__ __forceinline__ double rz_rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float rz_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double rz_sqrt(double x) { return sqrt(x); }
+// larfg_coef with two reciprocals (1 / beta and 1 / |alpha - beta|^2)
+template <class R, class C>
+__device__ __forceinline__ void rz_larfg(C alpha, R xn2, C &tau, R &beta, C &scale)
+{
+    if (xn2 == R(0) && alpha.y == R(0)) {
+        tau = czero<C>();
+        beta = alpha.x;
+        scale = czero<C>();
+        return;
+    }
+    beta = -copysign(rz_sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
+    const R rb = rz_rcp(beta);
+    tau = cmake<C, R>((beta - alpha.x) * rb, -alpha.y * rb);
+    const R dx = alpha.x - beta, dy = alpha.y, rd = rz_rcp(dx * dx + dy * dy);
+    scale = cmake<C, R>(dx * rd, -dy * rd);
+}
+
 // v + v(lane ^ 32) and v + v(lane ^ 16) through gfx950's lane-half swaps
 // (identical in both lanes of a pair: the addition commutes)
 __device__ __forceinline__ float xor32_sum(float v)
@@ -2885,14 +2913,16 @@ __global__ __launch_bounds__(RZ_THREADS, 1) void k_hetrd_res(HetrdArgsT<R> h)
         const int pcol = j % NB, i0 = j - pcol;
         // (R) reflector of x = colb[j+1 ..]: H^H x = beta e1, H = I - tau v v^H
         R xs = 0;
-        for (int r = j + 2 + lane; r < N; r += 64) {
-            const C a = S.colb[r];
-            xs += a.x * a.x + a.y * a.y;
+#pragma unroll
+        for (int q = 0; q < (16 * NTM + 63) / 64; ++q) {
+            const int r = j + 2 + lane + 64 * q;
+            const C a = S.colb[r < N ? r : 0];
+            xs += r < N ? a.x * a.x + a.y * a.y : R(0);
         }
         xs = wave_sum_fast(xs);
         C tau, scale;
         R beta;
-        larfg_coef<R, C>(S.colb[j + 1], xs, tau, beta, scale);
+        rz_larfg<R, C>(S.colb[j + 1], xs, tau, beta, scale);
         if (tid == 0) { dd[j] = (double)S.colb[j].x; ee[j] = (double)beta; }
         {
             const int r = 40 * wv + lane;

@@ -11,9 +11,14 @@ timed region starts.
 Both precisions are measured (FP64, and mixed: FP32 tridiagonalisation
 and back-transform with FP64 Ogita-Aishima refinement); the headline value
 is --precision (default mixed).  Prints one JSON line with
-  roofline      k_hetrd_panel, the dominant kernel: algorithmic bytes per
-                launch over its average duration (HIP events on the plan's
-                stream around each launch) against the HBM peak;
+  roofline      the headline step's Householder reduction: k_hetrd_res
+                (resident tridiagonalisation, one launch per step) as 16/3 N^3
+                flops per k over the launch's event time against the matrix
+                peak of the precision, or k_hetrd_panel (one-stage panels) as
+                algorithmic bytes per launch over its average duration (HIP
+                events on the plan's stream around each launch) against the
+                HBM peak; traffic from the committed counter summaries;
+  roofline_panel_fp64  k_hetrd_panel of the FP64 step (its dominant kernel);
   roofline_build  the H/S build: 2 * 16 * N^2 bytes per k written over the
                 build stage's event time, against the HBM peak;
   roofline_solve  per precision, the eigensolve's LAPACK flop count (22.67 N^3
@@ -52,11 +57,11 @@ FP32_PEAK_TFLOPS = 157.3     # MI355X dense FP32 matrix
 HBM_PEAK_GBS = 8000.0
 
 
-def measured_traffic(precision: str):
-    """HBM bytes per k_hetrd_panel launch from the committed rocprofv3
-    --pmc summary (FETCH_SIZE and WRITE_SIZE, each scaled by what the same
-    counter reads for a 1 GiB copy kernel), or None."""
-    path = os.path.join(ROOT, "profiles", "c2_hetrd_panel_traffic.txt")
+def measured_traffic(precision: str, profile: str = "c2_hetrd_panel_traffic.txt"):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by what the same counter
+    reads for a 1 GiB copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", profile)
     try:
         for line in open(path):
             if line.startswith(f"{precision}_traffic_bytes_per_launch:"):
@@ -64,6 +69,44 @@ def measured_traffic(precision: str):
     except OSError:
         pass
     return None
+
+
+def panel_roofline(n, nk, stages, precision):
+    """k_hetrd_panel (one-stage Householder panels, HBM-bound Hermitian
+    products): algorithmic bytes per launch over its average launch time
+    (HIP events on the plan's stream around each launch)."""
+    csize = 16 if precision == "fp64" else 8
+    panel_ms = stages.get("hetrd_panel", float("nan"))
+    launches = (n - 2) // 32 + 1
+    bytes_step = hetrd_panel_bytes(n, csize) * nk
+    achieved = bytes_step / (panel_ms / 1e3) / 1e9
+    return {"kernel": "k_hetrd_panel", "precision": precision, "bound": "hbm", "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(precision),
+            "traffic_source": "profiles/c2_hetrd_panel_traffic.txt (rocprofv3 --pmc FETCH_SIZE, "
+                              "WRITE_SIZE of the same step, scaled by a 1 GiB copy); not measured in this run",
+            "launches_per_step": launches, "avg_launch_ms": panel_ms / launches,
+            "algorithmic_bytes_per_launch": bytes_step / launches}
+
+
+def resident_roofline(n, nk, stages, precision):
+    """k_hetrd_res (the resident tridiagonalisation: the whole reduction of
+    each k-point in one workgroup, the matrix held in registers; one launch
+    per step): the Householder reduction's 16/3 N^3 real flops per k over
+    the launch's event time, against the matrix peak of the precision (the
+    rank-2 updates run on the matrix cores, the Hermitian products on the
+    vector units)."""
+    ms = stages.get("hetrd_resident", float("nan"))
+    flops = 16.0 / 3.0 * n ** 3 * nk
+    tf = flops / (ms / 1e3) / 1e12
+    peak = FP64_PEAK_TFLOPS if precision == "fp64" else FP32_PEAK_TFLOPS
+    traffic = measured_traffic(precision, "c2_hetrd_res_traffic.txt")
+    return {"kernel": "k_hetrd_res", "precision": precision, "bound": "mfma", "achieved": tf, "peak": peak,
+            "unit": "TFLOP/s", "frac": tf / peak, "traffic": traffic,
+            "traffic_source": "profiles/c2_hetrd_res_traffic.txt (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of "
+                              "the same step, scaled by a 1 GiB copy); not measured in this run",
+            "hbm_gbs": (traffic / (ms / 1e3) / 1e9) if traffic else None,
+            "launches_per_step": 1, "avg_launch_ms": ms, "algorithmic_flops_per_launch": flops}
 
 
 def hetrd_panel_bytes(n: int, csize: int) -> float:
@@ -87,7 +130,8 @@ def hetrd_panel_bytes(n: int, csize: int) -> float:
 # at order N: ZPOTRF 4/3, ZHEGST 4, ZHETRD 16/3, apply Q 8 and the
 # triangular back-transform 4 (in N^3 real flops).
 SOLVE_FLOP_PER_N3 = 4.0 / 3.0 + 4.0 + 16.0 / 3.0 + 8.0 + 4.0
-SOLVE_STAGES = ("potrf_trtri", "reduce_gemm", "hetrd_panel", "hetrd_update", "hetrd_band", "hetrd_bulge", "stein",
+SOLVE_STAGES = ("potrf_trtri", "reduce_gemm", "hetrd_panel", "hetrd_update", "hetrd_resident", "hetrd_band",
+                "hetrd_bulge", "stein",
                 "backtransform", "refine")
 
 
@@ -421,20 +465,14 @@ def main():
     pplain.close()
     print(f"# plain fp64: {plain_rate:.1f} k-solves/s", file=sys.stderr, flush=True)
 
-    # ---- roofline of the dominant kernel: k_hetrd_panel (HBM-bound
-    # Hermitian products), HIP events around each of its launches on the
-    # plan's stream -----------------------------------------------------------
-    csize = 16 if args.precision == "fp64" else 8
-    panel_ms = head["stages_ms_per_step"].get("hetrd_panel", float("nan"))
-    launches = (n - 2) // 32 + 1
-    bytes_step = hetrd_panel_bytes(n, csize) * len(kpts)
-    achieved = bytes_step / (panel_ms / 1e3) / 1e9
-    roofline = {"kernel": "k_hetrd_panel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.precision),
-                "traffic_source": "profiles/c2_hetrd_panel_traffic.txt (rocprofv3 --pmc FETCH_SIZE, "
-                                  "WRITE_SIZE of the same step, scaled by a 1 GiB copy); not measured in this run",
-                "launches_per_step": launches, "avg_launch_ms": panel_ms / launches,
-                "algorithmic_bytes_per_launch": bytes_step / launches}
+    # ---- roofline of the headline step's Householder reduction: the
+    # resident kernel where the order allows it (one launch, flops), else the
+    # one-stage panels (HBM); the FP64 step's panels beside it
+    hs = head["stages_ms_per_step"]
+    roofline = (resident_roofline(n, len(kpts), hs, args.precision) if "hetrd_resident" in hs
+                else panel_roofline(n, len(kpts), hs, args.precision))
+    roofline_panel_fp64 = (panel_roofline(n, len(kpts), runs["fp64"]["stages_ms_per_step"], "fp64")
+                           if "hetrd_panel" in runs["fp64"]["stages_ms_per_step"] else None)
     # ---- the H/S build (k_phases + k_build_hs, HBM-bound): H and S written
     # once, 2 * 16 * N^2 bytes per k, over the build stage's event time
     build_ms = head["stages_ms_per_step"].get("build_hs", float("nan"))
@@ -489,6 +527,7 @@ def main():
         "plan_create_ms": {k: r["plan_create_ms"] for k, r in runs.items()},
         "result": {k: r["result"] for k, r in runs.items()},
         "roofline": roofline,
+        "roofline_panel_fp64": roofline_panel_fp64,
         "roofline_build": roofline_build,
         "roofline_solve": roofline_solve,
         "serving": serving,

@@ -28,8 +28,8 @@ GOLD = os.path.join(ROOT, "tests", "golden", "spin.json")
 # mixed precision against the reference's results, about ten times the
 # largest error measured: E_F, energies and moment (measured 7.9e-13 Ry),
 # pressure (measured 1.6e-12 Ry/bohr^3 = 2.3e-7 kbar, non-collinear fcc4_flip)
-MIX_SPIN = 1e-11
-MIX_SPIN_P = 2e-11
+MIX_SPIN = 1e-12
+MIX_SPIN_P = 1e-12
 EXE = os.path.join(ROOT, "oracle", "_ref", "static_tbk")
 EX1 = os.path.join(ROOT, "oracle", "_ref", "ex1")
 

@@ -1374,11 +1374,13 @@ extern "C" int tbk_plan_result(const tbk_plan *p, int s, double *efermi, double 
     return TBK_OK;
 }
 
-// GEMM autotuner: one mode-4 pass records the plan's GEMM shapes, then each
-// shape not yet in the table is timed with every tile (one warm-up and
-// `reps` launches on the plan's stream) and the fastest tile is kept for the
-// rest of the process.  The timing launches overwrite workspace contents, so
-// the plan's results are undefined until its next run.
+// GEMM autotuner: one mode-4 pass records the plan's GEMM shapes (the
+// shapes this thread launches during the pass), then each shape not yet in
+// the table is timed with every tile (one warm-up and `reps` launches on the
+// plan's stream) and the fastest tile is kept for the rest of the process.
+// The timing launches overwrite workspace contents, so the plan's results
+// are undefined until its next run; launches of other plans meanwhile only
+// disturb the timings (a tile choice changes speed, never results).
 extern "C" int tbk_plan_autotune(tbk_plan *p, int reps, int *ntuned)
 {
     if (!p || reps < 1) return TBK_ERR_ARG;

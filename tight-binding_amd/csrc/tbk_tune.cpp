@@ -14,8 +14,11 @@ typedef std::tuple<int, int, int, int, int, int, int, int> Key;   // prec, opa, 
 std::mutex g_mu;
 std::map<Key, int> g_table;
 int g_forced = -1;
-bool g_record = false;
-std::vector<GemmShape> g_recorded;
+// shapes recorded by the autotuner's pass: per host thread, since the pass
+// launches from the calling thread, so that launches other threads make
+// meanwhile (other plans, the request loop's workers) are not recorded with it
+thread_local bool g_record = false;
+thread_local std::vector<GemmShape> g_recorded;
 
 const int kBM[kGemmTiles] = {64, 96, 128, 32};
 const int kBN[kGemmTiles] = {64, 96, 64, 32};
@@ -66,14 +69,12 @@ bool gemm_tuned(const GemmShape &s)
 
 void gemm_record(bool on)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
     g_record = on;
     if (on) g_recorded.clear();
 }
 
 void gemm_note(const GemmShape &s)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
     if (!g_record) return;
     for (const GemmShape &r : g_recorded)
         if (r.prec == s.prec && r.opa == s.opa && r.opb == s.opb && r.m == s.m && r.n == s.n &&
@@ -82,10 +83,6 @@ void gemm_note(const GemmShape &s)
     g_recorded.push_back(s);
 }
 
-std::vector<GemmShape> gemm_recorded()
-{
-    std::lock_guard<std::mutex> lk(g_mu);
-    return g_recorded;
-}
+std::vector<GemmShape> gemm_recorded() { return g_recorded; }
 
 }  // namespace tbk

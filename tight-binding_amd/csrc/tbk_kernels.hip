@@ -3502,21 +3502,23 @@ __global__ __launch_bounds__(64) void k_q2_tfactors(Q2ArgsT<R> g)
 //   W = V^H Y_rows (16 x 16),  W <- T W,  Y_rows -= V W  (32 rows)
 // on the matrix cores, complex products as three real ones.  A workgroup
 // stages each diamond's V (rows padded to 32) and T in LDS, double-buffered
-// (one barrier per diamond); every wave holds kQ2Tiles tiles of 16 columns
-// of Y.  The K index of every product is laid out like the rows of an
+// (one barrier per diamond); every wave holds q2_tiles<R>() tiles of 16
+// columns of Y (FP64: one, with three workgroups per CU, so that more
+// workgroups hide the latency of each diamond's chain of products; FP32:
+// two, two per CU).  The K index of every product is laid out like the rows of an
 // accumulator (slot q of lane group g4 is row mfma_row(q)), so each
 // accumulator is the next product's B operand in the same lane, and the 32
 // rows of Y a wave holds per tile (two halves of 16) are both B operand and
 // update target.  Consecutive diamonds of a block overlap by 16 rows: the
 // lower half moves up and one new half is loaded.
 constexpr int kQ2Waves = 4;
-constexpr int kQ2Tiles = 2;
+template <class R> constexpr int q2_tiles() { return sizeof(R) == 8 ? 1 : 2; }
 template <class R>
-__global__ __launch_bounds__(64 * kQ2Waves, 2) void k_apply_q2(Q2ArgsT<R> g)
+__global__ __launch_bounds__(64 * kQ2Waves, (sizeof(R) == 8 ? 3 : 2)) void k_apply_q2(Q2ArgsT<R> g)
 {
     typedef typename cplx<R>::type C;
     typedef R v4 __attribute__((ext_vector_type(4)));
-    constexpr int NT = kQ2Tiles;
+    constexpr int NT = q2_tiles<R>();
     __shared__ C Vs[2][32][TB + 1];
     __shared__ C Ts[2][TB][TB + 1];
     const int n = g.n, kb = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -3914,7 +3916,7 @@ template <class R>
 static void launch_q2_t(const Q2ArgsT<R> &g, int nb, hipStream_t st)
 {
     hipLaunchKernelGGL((k_q2_tfactors<R>), dim3(band_diamonds(g.n), nb), dim3(64), 0, st, g);
-    constexpr int cols = 16 * kQ2Waves * kQ2Tiles;
+    constexpr int cols = 16 * kQ2Waves * q2_tiles<R>();
     hipLaunchKernelGGL((k_apply_q2<R>), dim3((g.n + cols - 1) / cols, nb), dim3(64 * kQ2Waves), 0, st, g);
 }
 void launch_q2(const Q2ArgsT<double> &g, int nb, hipStream_t st) { launch_q2_t(g, nb, st); }

@@ -2710,18 +2710,15 @@ __device__ __forceinline__ R row16_sum8(R (&v)[8], int c)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         R lo = v[q], hi = v[q + 4];
-        asm volatile("" : "+v"(lo), "+v"(hi));
         v[q] = (b3 ? hi : lo) + dpp_mov<0x128>(b3 ? lo : hi);
     }
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         R lo = v[q], hi = v[q + 2];
-        asm volatile("" : "+v"(lo), "+v"(hi));
         v[q] = (b2 ? hi : lo) + dpp_mov<0x141>(b2 ? lo : hi);
     }
     {
         R lo = v[0], hi = v[1];
-        asm volatile("" : "+v"(lo), "+v"(hi));
         v[0] = (b1 ? hi : lo) + dpp_mov<0x1B>(b1 ? lo : hi);
     }
     return v[0] + dpp_mov<0xB1>(v[0]);
@@ -2964,7 +2961,6 @@ __global__ __launch_bounds__(RZ_THREADS, 1) void k_hetrd_res(HetrdArgsT<R> h)
                 asm volatile("" : "+s"(I), "+s"(J));
                 const C vJ = S.vb[16 * J + c];
                 R vals[8];
-                R vv = 0;
                 C q = czero<C>();
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -2973,13 +2969,19 @@ __global__ __launch_bounds__(RZ_THREADS, 1) void k_hetrd_res(HetrdArgsT<R> h)
                     const C pr = zmul(x, vJ);
                     vals[2 * i] = pr.x;
                     vals[2 * i + 1] = pr.y;
-                    vv += vI.x * pr.x + vI.y * pr.y;         // Re(conj(v_I) A v_J)
                     q = zadd(q, zcmul(x, vI));                // conj(A) v_I
                 }
-                vav += I == J ? vv : R(2) * vv;
+                // lane c holds component (c >> 1) & 1 of the sum of row
+                // rowoff(c >> 2) (in lanes c and c ^ 1); v^H A v takes
+                // Re(conj(v_r) (A_IJ v_J)_r) from the even lanes, twice below
+                // the diagonal (the tile's mirror)
                 const R rs = row16_sum8<R>(vals, c);
-                if ((c & 1) == 0)
+                const int rrow = 16 * I + rowoff(c >> 2);
+                const R vcomp = reinterpret_cast<const R *>(&S.vb[rrow])[(c >> 1) & 1];
+                if ((c & 1) == 0) {
                     reinterpret_cast<R *>(&S.slotR[t][rowoff(c >> 2)])[(c >> 1) & 1] = rs;
+                    vav += (I == J ? R(1) : R(2)) * (vcomp * rs);
+                }
                 if (I != J) {
                     q.x = xor32_sum(xor16_sum(q.x));
                     q.y = xor32_sum(xor16_sum(q.y));

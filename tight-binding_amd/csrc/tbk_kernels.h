@@ -243,24 +243,34 @@ void launch_q2(const Q2ArgsT<float> &g, int nb, hipStream_t st);
 void launch_rows_c32_to_c64(int n, int r0, int r1, const float2 *y32, double2 *y64, size_t stride, int nb,
                             hipStream_t st);
 int oa_max_n();                  // largest N of the refinement kernels
-int oa_max_cluster();            // largest cluster the refinement rotates
-// The refinement step of the mixed method, on T = Y^H C Y and G = Y^H Y
-// (both rotated in place): eigenvalues (ascending) into w, then F = P (I + E)
-// with sorted columns by launch_oa_fmat.  The other arrays are scratch:
-// lam, nu, w, perm, perm2, cinfo, clist [nb][n], tau, ncl [nb],
-// pb [nb][n][oa_max_cluster()].
+// The refinement step of the mixed method on C (FP64) and the approximate
+// eigenvectors Y (FP64 copy Y, FP32 copy Y32 of the FP32 back-transform):
+// the caller forms X = C Y (FP64), then launch_oa_prepare (Rayleigh
+// quotients, residuals R32 = X - Y diag(lambda), clusters, rotated
+// clusters), K32 = Y32^H R32 (FP32), launch_oa_eigen (eigenvalues,
+// ascending, into w), and for the vectors launch_oa_emat (E32, columns in
+// the final order), D32 = Y32 E32 (FP32) and launch_oa_final (Yr, the
+// refined vectors normalised).  Columns j >= mw[k] are not refined: they
+// keep theta's eigenvalues and form no vector.  lam, rn, perm, perm2,
+// cinfo, clist [nb][n], tau, ncl [nb] are scratch; R32 and D32 may share
+// storage.
 struct OaArgs {
     int n;
     size_t stride;
-    const int *nact;
-    double2 *T, *G;
-    double *lam, *nu, *tau, *w;
+    const int *nact;               // [nb] active order (padding after); null: n
+    const int *mw;                 // [nb] refined columns; null: n
+    const double *theta;           // [nb][n] eigenvalues of the FP32 tridiagonal
+    double2 *X, *Y;                // rotated in place over the clusters
+    float2 *Y32, *R32, *K32, *E32, *D32;
+    double2 *Yr;
+    double *lam, *rn, *tau, *w;
     int *perm, *perm2, *cinfo, *clist, *ncl;
-    double2 *pb;
 };
-void launch_oa_refine(const OaArgs &a, int nb, hipStream_t st);
+void launch_oa_prepare(const OaArgs &a, int nb, hipStream_t st);
+void launch_oa_eigen(const OaArgs &a, int nb, hipStream_t st);
 // columns j >= cut[k] are not formed (cut may be null)
-void launch_oa_fmat(const OaArgs &a, double2 *f, int nb, hipStream_t st, const int *cut = nullptr);
+void launch_oa_emat(const OaArgs &a, int nb, hipStream_t st, const int *cut = nullptr);
+void launch_oa_final(const OaArgs &a, int nb, hipStream_t st, const int *cut = nullptr);
 void launch_coldot(int n, const double2 *z, const double2 *x, const double2 *y, size_t stride,
                    double2 *px, double2 *py, int nb, hipStream_t st, const int *cut = nullptr);
 void launch_perturb(const PerturbArgs &p, hipStream_t st);

@@ -1705,147 +1705,219 @@ __global__ __launch_bounds__(256) void k_vt_to_columns(int N, const double *Vt, 
 // Mixed precision: the refinement step
 // ---------------------------------------------------------------------------
 
-// One refinement step for the Hermitian eigenproblem of C with approximate
-// eigenvectors Y (Ogita & Aishima's step, RefSyEv, extended to complex
-// Hermitian, with clusters rotated exactly): on entry T = Y^H C Y and
-// G = Y^H Y, both triangles stored.  With R = I - G:
-//  1. lambda_i = T_ii / G_ii; tau = kOaRho * max_{i != j} |T_ij - lambda_j G_ij|;
-//     clusters = runs of the ascending lambda whose consecutive gaps are
-//     below tau (k_oa_stats).
-//  2. Each cluster c of 2 .. kOaMaxCl states is rotated by Rayleigh-Ritz:
-//     L = I + R_cc / 2 (G_cc^{-1/2} to second order), M = L T_cc L = Q L Q^H
-//     by cyclic Jacobi, P_c = L Q; its states take the eigenvalues of M
-//     (k_oa_rr).  T and G are carried into the rotated basis in place,
-//     T <- P^H T P, G <- P^H G P, P block-diagonal (k_oa_xform).
-//  3. Between states of different clusters the first-order step
-//     E_ij = (T_ij - lambda_j G_ij) / (lambda_j - lambda_i), |E_ij| <= 1 / kOaRho;
-//     within a cluster E_ij = R_ij / 2; the diagonal
-//     E_jj = R_jj / 2 + nu_j, nu_j = sum_{i != j} Re(conj(E_ij) R_ij) - |E_ij|^2 / 2,
-//     normalises the refined vector to second order, and
-//     lambda'_j = lambda_j + sum_{i not in j's cluster} |T_ij - lambda_j G_ij|^2 / (lambda_j - lambda_i)
-//     is its Rayleigh quotient to second order (k_oa_second); lambda' is
-//     sorted (k_oa_sort).
-//  4. F = P (I + E) with its columns in that order (k_oa_fmat): Y F is the
-//     refined, sorted eigenvector matrix.
-// Ogita & Aishima's step leaves every pair closer than its global bound
-// delta = 2 (||T - diag lambda||_F + max|lambda| ||R||_F) unrotated and
-// reports the Rayleigh quotients of the unrefined vectors; at order ~1000 with
-// FP32 vectors delta is ~1e-3, so most neighbouring levels fall inside it.
+// One refinement step for the Hermitian eigenproblem of C from the
+// approximate eigenvectors Y of the FP32 reduction (Ogita & Aishima's step,
+// RefSyEv, extended to complex Hermitian, in residual form, with clusters
+// rotated exactly).  Columns j < mw are refined (mw: a provisional
+// occupation window, or all columns); the others keep the FP32
+// tridiagonal's eigenvalues theta and are not formed.
+//  1. X = C Y (FP64 GEMM, by the caller).  Per column the Rayleigh quotient
+//     lambda_j = Re(y_j^H x_j) / y_j^H y_j, the residual
+//     r_j = x_j - lambda_j y_j (FP64, stored in FP32) and its norm
+//     (k_oa_resid).
+//  2. tau = kOaRhoR max_j ||r_j||, so that every pair further apart than tau
+//     has |y_i^H r_j| / |lambda_j - lambda_i| <= 1 / kOaRhoR; clusters = runs
+//     of the ascending lambda whose consecutive gaps are below tau
+//     (k_oa_clusters).
+//  3. Each cluster of 2 .. kOaMaxCl states is rotated by Rayleigh-Ritz on
+//     T_cc = Y_c^H X_c and G_cc = Y_c^H Y_c (FP64 dots): L = I + (I - G_cc) / 2
+//     (G_cc^{-1/2} to second order), M = L T_cc L = Q diag Q^H by cyclic
+//     Jacobi, P = L Q; Y_c, X_c <- Y_c P, X_c P in FP64, their FP32 copies
+//     and residuals rewritten, lambda_c <- eig(M) (k_oa_rr).
+//  4. K = Y^H R (FP32 GEMM, by the caller): K_ij = y_i^H r_j =
+//     (Y^H C Y)_ij - lambda_j (Y^H Y)_ij, the numerator of the first-order
+//     step.  It is of the size of the FP32 reduction's errors, so FP32
+//     products carry it to ~1e-7 of itself.
+//  5. lambda'_j = lambda_j + sum_{i first order} |K_ij|^2 / (lambda_j - lambda_i),
+//     the Rayleigh quotient of the refined vector to second order
+//     (k_oa_second), sorted (k_oa_sort).
+//  6. E_ij = K_ij / (lambda_j - lambda_i) between states of different
+//     clusters further apart than tau; within a run of close states too long
+//     to rotate E_ij = -(Y^H Y)_ij / 2 (FP64 dots: the symmetric
+//     orthogonalisation); 0 within a rotated cluster, on the diagonal and
+//     for pairs closer than tau across the window's edge (k_oa_emat, columns
+//     in the final order).
+//  7. D = Y E (FP32 GEMM, by the caller); y'_j = y_j + d_j, normalised in
+//     FP64 (k_oa_final).
+// Only step 1's product and the Rayleigh-Ritz dots run in FP64 matrix
+// arithmetic; Ogita & Aishima's form takes Y^H C Y and Y^H Y as two further
+// FP64 products of order N^3, whose cancellation the residual form moves
+// into the O(N^2) residual.
 constexpr int kOaMax = 1536;
 constexpr int kOaMaxCl = 16;
-constexpr double kOaRho = 100.0;
-constexpr int kOaSlots = 8;                  // workgroups per item in the cluster kernels
+// ||r_j|| bounds every |y_i^H r_j| and exceeds the largest by about the
+// square root of the number of states r_j spreads over: 10 here puts the
+// clusters about where 100 max |y_i^H r_j| would.  Larger values lengthen the
+// runs beyond kOaMaxCl, which stay unrotated (at the 108-atom cell, order
+// 972: largest eigenvalue error 4.2e-12 Ry at 10, 9.7e-11 at 30, 3.4e-10 at
+// 100).
+constexpr double kOaRhoR = 10.0;
+constexpr int kOaSlots = 8;                  // workgroups per item in the cluster kernel
 
-__device__ __forceinline__ bool oa_first_order(int bi, int bc, double li, double lc, double tau)
+// Sum over the 64 lanes of a wave; every lane receives the same value
+// (each exchange adds the same two operands in both lanes).
+__device__ __forceinline__ double wave_allsum(double v)
 {
-    return bi != bc && fabs(lc - li) > tau;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
-__global__ __launch_bounds__(1024) void k_oa_stats(OaArgs a)
+__device__ __forceinline__ int oa_active(const OaArgs &a, int kb) { return a.nact ? a.nact[kb] : a.n; }
+// refined active columns
+__device__ __forceinline__ int oa_refined(const OaArgs &a, int kb)
 {
-    __shared__ double lam[kOaMax], lsort[kOaMax];
+    return min(oa_active(a, kb), a.mw ? a.mw[kb] : a.n);
+}
+// states i and c (cinfo entries ci, cc) are coupled at first order
+__device__ __forceinline__ bool oa_first_order(int ci, int cc, double li, double lc, double tau)
+{
+    return (ci & 0xffff) != (cc & 0xffff) && fabs(lc - li) > tau;
+}
+
+// One wave per column (four per workgroup).
+__global__ __launch_bounds__(256) void k_oa_resid(OaArgs a)
+{
+    const int N = a.n, kb = blockIdx.y, lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (j >= N) return;
+    double *lam = a.lam + (size_t)kb * N;
+    double *rn = a.rn + (size_t)kb * N;
+    if (a.mw && j >= a.mw[kb]) {                      // not refined
+        if (lane == 0) { lam[j] = a.theta[(size_t)kb * N + j]; rn[j] = 0.0; }
+        return;
+    }
+    const double2 *x = a.X + (size_t)kb * a.stride + (size_t)j * N;
+    const double2 *y = a.Y + (size_t)kb * a.stride + (size_t)j * N;
+    float2 *r = a.R32 + (size_t)kb * a.stride + (size_t)j * N;
+    double t = 0.0, g = 0.0;
+    for (int i = lane; i < N; i += 64) {
+        const double2 xv = x[i], yv = y[i];
+        t += yv.x * xv.x + yv.y * xv.y;
+        g += yv.x * yv.x + yv.y * yv.y;
+    }
+    t = wave_allsum(t);
+    g = wave_allsum(g);
+    const double l = t / g;
+    double rr = 0.0;
+    for (int i = lane; i < N; i += 64) {
+        const double2 xv = x[i], yv = y[i];
+        const double rx = xv.x - l * yv.x, ry = xv.y - l * yv.y;
+        r[i] = make_float2((float)rx, (float)ry);
+        rr += rx * rx + ry * ry;
+    }
+    rr = wave_allsum(rr);
+    if (lane == 0) { lam[j] = l; rn[j] = sqrt(rr); }
+}
+
+// tau, the ascending order of the refined lambda and the clusters.  cinfo:
+// bits 0-15 the cluster (the sorted position of its first state; an
+// unrefined state is its own, numbered by its column), 16-23 the member's
+// place, 24-31 the cluster's size if it is rotated, else 1.
+__global__ __launch_bounds__(1024) void k_oa_clusters(OaArgs a)
+{
+    __shared__ double lsort[kOaMax];
     __shared__ int idx[kOaMax], cinf[kOaMax];
     __shared__ double red[16];
     __shared__ int ncl_s;
     const int N = a.n, kb = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int nw = (int)(blockDim.x >> 6);
-    const double2 *Tk = a.T + (size_t)kb * a.stride;
-    const double2 *Gk = a.G + (size_t)kb * a.stride;
-    const int Na = a.nact ? a.nact[kb] : N;          // padding (decoupled) after Na
+    const int Ma = oa_refined(a, kb);
+    const double *lam = a.lam + (size_t)kb * N;
+    const double *rn = a.rn + (size_t)kb * N;
+    double mx = 0.0;
     for (int i = tid; i < N; i += blockDim.x) {
-        const double l = Tk[(size_t)i * N + i].x / Gk[(size_t)i * N + i].x;
-        lam[i] = l; lsort[i] = l; idx[i] = i;
+        if (i < Ma) mx = fmax(mx, rn[i]);
+        lsort[i] = lam[i];
+        idx[i] = i;
+        cinf[i] = i | (1 << 24);
     }
-    __syncthreads();
-    // largest coupling over the active off-diagonal pairs: element (r, c),
-    // r > c, of the lower triangles gives |T_rc - lambda_c G_rc| and, by
-    // Hermitian symmetry, |T_cr - lambda_r G_cr| = |T_rc - lambda_r G_rc|
-    double cm = 0.0;
-    for (int c = wv; c < Na; c += nw) {
-        const double lc = lam[c];
-        for (int r0 = c + 1; r0 < Na; r0 += 128) {
-            double2 tv[2], gv[2];
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                const int r = r0 + 64 * u + lane;
-                tv[u] = r < Na ? Tk[(size_t)c * N + r] : make_double2(0.0, 0.0);
-                gv[u] = r < Na ? Gk[(size_t)c * N + r] : make_double2(0.0, 0.0);
-            }
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                const int r = r0 + 64 * u + lane;
-                if (r >= Na) continue;
-                const double lr = lam[r];
-                const double ax = tv[u].x - lc * gv[u].x, ay = tv[u].y - lc * gv[u].y;
-                const double bx = tv[u].x - lr * gv[u].x, by = tv[u].y - lr * gv[u].y;
-                cm = fmax(cm, fmax(ax * ax + ay * ay, bx * bx + by * by));
-            }
-        }
-    }
-    for (int off = 32; off > 0; off >>= 1) cm = fmax(cm, __shfl_xor(cm, off, 64));
-    if (lane == 0) red[wv] = cm;
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    if (lane == 0) red[wv] = mx;
     __syncthreads();
     if (tid == 0) {
-        double c2 = 0.0;
-        for (int q = 0; q < nw; ++q) c2 = fmax(c2, red[q]);
-        const double tau = kOaRho * sqrt(c2);
+        double m2 = 0.0;
+        for (int q = 0; q < nw; ++q) m2 = fmax(m2, red[q]);
+        const double tau = kOaRhoR * m2;
         a.tau[kb] = tau;
-        // stable insertion sort of the active part (the input is nearly sorted)
-        for (int i = 1; i < Na; ++i) {
+        // stable insertion sort of the refined part (nearly sorted already)
+        for (int i = 1; i < Ma; ++i) {
             const double x = lsort[i];
             const int xi = idx[i];
             int k = i - 1;
             while (k >= 0 && lsort[k] > x) { lsort[k + 1] = lsort[k]; idx[k + 1] = idx[k]; --k; }
             lsort[k + 1] = x; idx[k + 1] = xi;
         }
-        // clusters: runs with consecutive gaps below tau; those of 2 ..
-        // kOaMaxCl states are rotated, longer runs stay unrotated
+        // runs with consecutive gaps below tau; those of 2 .. kOaMaxCl
+        // states are rotated, longer runs are orthogonalised only
         int ncl = 0, s0 = 0;
         int *clist = a.clist + (size_t)kb * N;
-        while (s0 < Na) {
+        while (s0 < Ma) {
             int e = s0 + 1;
-            while (e < Na && lsort[e] - lsort[e - 1] < tau) ++e;
+            while (e < Ma && lsort[e] - lsort[e - 1] < tau) ++e;
             const int m = e - s0;
             const bool rot = m >= 2 && m <= kOaMaxCl;
             if (rot) clist[ncl++] = s0;
             for (int q = 0; q < m; ++q) cinf[idx[s0 + q]] = s0 | (rot ? (q << 16) | (m << 24) : (1 << 24));
             s0 = e;
         }
-        for (int i = Na; i < N; ++i) cinf[i] = i | (1 << 24);
         ncl_s = ncl;
     }
     __syncthreads();
     if (tid == 0) a.ncl[kb] = ncl_s;
     for (int i = tid; i < N; i += blockDim.x) {
         a.perm[(size_t)kb * N + i] = idx[i];
-        a.lam[(size_t)kb * N + i] = lam[i];
         a.cinfo[(size_t)kb * N + i] = cinf[i];
     }
 }
 
-// Rayleigh-Ritz in each rotated cluster: one wave per cluster, the m x m
-// matrices in LDS.
+// Rayleigh-Ritz in each rotated cluster (one wave per cluster, the m x m
+// matrices in LDS), then the cluster's columns of Y, X, Y32 and R32 in the
+// rotated basis.  Clusters own disjoint columns, so the clusters of an item
+// run in parallel.
 __global__ __launch_bounds__(64) void k_oa_rr(OaArgs a)
 {
     __shared__ double2 Am[kOaMaxCl][kOaMaxCl + 1], Qm[kOaMaxCl][kOaMaxCl + 1], Lm[kOaMaxCl][kOaMaxCl + 1];
+    __shared__ double2 Pm[kOaMaxCl][kOaMaxCl + 1];
     __shared__ double ev[kOaMaxCl];
-    __shared__ int ord[kOaMaxCl];
+    __shared__ int ord[kOaMaxCl], mem[kOaMaxCl];
     const int N = a.n, kb = blockIdx.y, lane = threadIdx.x;
-    const double2 *Tk = a.T + (size_t)kb * a.stride;
-    const double2 *Gk = a.G + (size_t)kb * a.stride;
     const int *perm = a.perm + (size_t)kb * N;
     const int ncl = a.ncl[kb];
+    double2 *Xk = a.X + (size_t)kb * a.stride, *Yk = a.Y + (size_t)kb * a.stride;
+    float2 *Y32k = a.Y32 + (size_t)kb * a.stride, *R32k = a.R32 + (size_t)kb * a.stride;
     for (int ci = blockIdx.x; ci < ncl; ci += gridDim.x) {
         const int b = a.clist[(size_t)kb * N + ci];
         const int m = (a.cinfo[(size_t)kb * N + perm[b]] >> 24) & 0xff;
-        // T_cc into Am, L = I + (I - G_cc) / 2 into Lm
-        for (int t = lane; t < m * m; t += 64) {
-            const int q = t % m, k = t / m;
-            const size_t e = (size_t)perm[b + k] * N + perm[b + q];      // element (c_q, c_k)
-            const double2 tv = Tk[e], gv = Gk[e];
-            Am[q][k] = tv;
-            Lm[q][k] = q == k ? make_double2(1.5 - 0.5 * gv.x, 0.0) : make_double2(-0.5 * gv.x, -0.5 * gv.y);
-        }
+        if (lane < m) mem[lane] = perm[b + lane];
+        __syncthreads();
+        // T_cc = Y_c^H X_c into Am, L = I + (I - Y_c^H Y_c) / 2 into Lm:
+        // the upper triangle by dots over the rows, mirrored
+        for (int q = 0; q < m; ++q)
+            for (int k = q; k < m; ++k) {
+                const double2 *yq = Yk + (size_t)mem[q] * N;
+                const double2 *xk = Xk + (size_t)mem[k] * N, *yk = Yk + (size_t)mem[k] * N;
+                double tr = 0.0, ti = 0.0, gr = 0.0, gi = 0.0;
+                for (int i = lane; i < N; i += 64) {
+                    const double2 u = yq[i], xv = xk[i], yv = yk[i];
+                    tr += u.x * xv.x + u.y * xv.y;
+                    ti += u.x * xv.y - u.y * xv.x;
+                    gr += u.x * yv.x + u.y * yv.y;
+                    gi += u.x * yv.y - u.y * yv.x;
+                }
+                tr = wave_allsum(tr); ti = wave_allsum(ti);
+                gr = wave_allsum(gr); gi = wave_allsum(gi);
+                if (lane == 0) {
+                    if (q == k) {
+                        Am[q][q] = make_double2(tr, 0.0);
+                        Lm[q][q] = make_double2(1.5 - 0.5 * gr, 0.0);
+                    } else {
+                        Am[q][k] = make_double2(tr, ti);
+                        Am[k][q] = make_double2(tr, -ti);
+                        Lm[q][k] = make_double2(-0.5 * gr, -0.5 * gi);
+                        Lm[k][q] = make_double2(-0.5 * gr, 0.5 * gi);
+                    }
+                }
+            }
         __syncthreads();
         // M = L T L: Qm = T L, then Am = L Qm, Hermitian part
         for (int t = lane; t < m * m; t += 64) {
@@ -1960,103 +2032,71 @@ __global__ __launch_bounds__(64) void k_oa_rr(OaArgs a)
             }
         }
         __syncthreads();
-        // P = L Q (columns in ascending order) into the rows b .. b + m - 1
-        // of pb; the eigenvalues go to the members in sorted order
-        double2 *pb = a.pb + (size_t)kb * N * kOaMaxCl;
+        // P = L Q (columns in ascending order of their eigenvalues)
         for (int t = lane; t < m * m; t += 64) {
             const int q = t % m, k = t / m;
             double2 acc = make_double2(0.0, 0.0);
             for (int p = 0; p < m; ++p) acc = zadd(acc, zmul(Lm[q][p], Qm[p][ord[k]]));
-            pb[(size_t)(b + q) * kOaMaxCl + k] = acc;
+            Pm[q][k] = acc;
         }
-        if (lane < m) a.lam[(size_t)kb * N + perm[b + lane]] = ev[lane];
         __syncthreads();
-    }
-}
-
-// T <- T P (rows = 0) or T <- P^H T (rows = 1) over each rotated cluster's
-// columns (rows), and the same for G.  Clusters own disjoint columns
-// (rows), so the clusters of an item run in parallel within one pass.
-__global__ __launch_bounds__(256) void k_oa_xform(OaArgs a, int rows)
-{
-    __shared__ double2 P[kOaMaxCl][kOaMaxCl];
-    __shared__ int mem[kOaMaxCl];
-    const int N = a.n, kb = blockIdx.y, tid = threadIdx.x;
-    const int *perm = a.perm + (size_t)kb * N;
-    const double2 *pb = a.pb + (size_t)kb * N * kOaMaxCl;
-    const int ncl = a.ncl[kb];
-    for (int ci = blockIdx.x; ci < ncl; ci += gridDim.x) {
-        const int b = a.clist[(size_t)kb * N + ci];
-        const int m = (a.cinfo[(size_t)kb * N + perm[b]] >> 24) & 0xff;
-        for (int t = tid; t < m * m; t += blockDim.x) P[t % m][t / m] = pb[(size_t)(b + t % m) * kOaMaxCl + t / m];
-        if (tid < m) mem[tid] = perm[b + tid];
-        __syncthreads();
-        for (int mat = 0; mat < 2; ++mat) {
-            double2 *M = (mat == 0 ? a.T : a.G) + (size_t)kb * a.stride;
-            for (int i = tid; i < N; i += blockDim.x) {
-                double2 v[kOaMaxCl];
+        for (int i = lane; i < N; i += 64) {
+            double2 v[kOaMaxCl], u[kOaMaxCl];
+#pragma unroll
+            for (int q = 0; q < kOaMaxCl; ++q)
+                if (q < m) {
+                    v[q] = Yk[(size_t)mem[q] * N + i];
+                    u[q] = Xk[(size_t)mem[q] * N + i];
+                }
+#pragma unroll
+            for (int k = 0; k < kOaMaxCl; ++k) {
+                if (k >= m) break;
+                double2 yh = make_double2(0.0, 0.0), xh = make_double2(0.0, 0.0);
 #pragma unroll
                 for (int q = 0; q < kOaMaxCl; ++q)
-                    if (q < m) v[q] = rows ? M[(size_t)i * N + mem[q]] : M[(size_t)mem[q] * N + i];
-#pragma unroll
-                for (int k = 0; k < kOaMaxCl; ++k) {
-                    if (k >= m) break;
-                    double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-                    for (int q = 0; q < kOaMaxCl; ++q)
-                        if (q < m) acc = zadd(acc, rows ? zmul(zconj(P[q][k]), v[q]) : zmul(v[q], P[q][k]));
-                    if (rows) M[(size_t)i * N + mem[k]] = acc;
-                    else M[(size_t)mem[k] * N + i] = acc;
-                }
+                    if (q < m) {
+                        yh = zadd(yh, zmul(v[q], Pm[q][k]));
+                        xh = zadd(xh, zmul(u[q], Pm[q][k]));
+                    }
+                const size_t e = (size_t)mem[k] * N + i;
+                Yk[e] = yh;
+                Xk[e] = xh;
+                Y32k[e] = make_float2((float)yh.x, (float)yh.y);
+                R32k[e] = make_float2((float)(xh.x - ev[k] * yh.x), (float)(xh.y - ev[k] * yh.y));
             }
         }
+        if (lane < m) a.lam[(size_t)kb * N + mem[lane]] = ev[lane];
         __syncthreads();
     }
 }
 
-// Second-order terms per column c (one wave per column): lambda'_c into w
-// (unsorted), nu_c.
+// Second-order eigenvalues, one wave per refined column: lambda'_c into w
+// (unsorted; unrefined columns keep lambda).
 __global__ __launch_bounds__(1024) void k_oa_second(OaArgs a)
 {
     const int N = a.n, kb = blockIdx.y, lane = threadIdx.x & 63;
     const int c = blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
-    const int Na = a.nact ? a.nact[kb] : N;
     if (c >= N) return;
+    const int Na = oa_active(a, kb), Ma = oa_refined(a, kb);
     const double *lam = a.lam + (size_t)kb * N;
-    const int *cinfo = a.cinfo + (size_t)kb * N;
-    const double lc = lam[c], tau = a.tau[kb];
-    if (c >= Na) {
-        if (lane == 0) { a.w[(size_t)kb * N + c] = lc; a.nu[(size_t)kb * N + c] = 0.0; }
+    const double lc = lam[c];
+    if (c >= Ma) {
+        if (lane == 0) a.w[(size_t)kb * N + c] = lc;
         return;
     }
-    const double2 *Tc = a.T + (size_t)kb * a.stride + (size_t)c * N;
-    const double2 *Gc = a.G + (size_t)kb * a.stride + (size_t)c * N;
-    const int bc = cinfo[c] & 0xffff;
-    double corr = 0.0, nu = 0.0;
+    const int *cinfo = a.cinfo + (size_t)kb * N;
+    const int cc = cinfo[c];
+    const double tau = a.tau[kb];
+    const float2 *Kc = a.K32 + (size_t)kb * a.stride + (size_t)c * N;
+    double corr = 0.0;
     for (int i = lane; i < Na; i += 64) {
-        if (i == c) continue;
-        const double2 g = Gc[i];
-        const double rx = -g.x, ry = -g.y;                          // R_ic
         const double li = lam[i];
-        if (oa_first_order(cinfo[i] & 0xffff, bc, li, lc, tau)) {
-            const double2 t = Tc[i];
-            const double mx = t.x - lc * g.x, my = t.y - lc * g.y;   // T_ic - lambda_c G_ic
-            const double den = lc - li;
-            const double ex = mx / den, ey = my / den;
-            corr += (mx * mx + my * my) / den;
-            nu += ex * rx + ey * ry - 0.5 * (ex * ex + ey * ey);
-        } else {
-            nu += 0.375 * (rx * rx + ry * ry);                       // E_ic = R_ic / 2
-        }
+        if (i == c || !oa_first_order(cinfo[i], cc, li, lc, tau)) continue;
+        const float2 k = Kc[i];
+        corr += ((double)k.x * k.x + (double)k.y * k.y) / (lc - li);
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        corr += __shfl_xor(corr, o, 64);
-        nu += __shfl_xor(nu, o, 64);
-    }
-    if (lane == 0) {
-        a.w[(size_t)kb * N + c] = lc + corr;
-        a.nu[(size_t)kb * N + c] = nu;
-    }
+    corr = wave_allsum(corr);
+    if (lane == 0) a.w[(size_t)kb * N + c] = lc + corr;
 }
 
 // lambda' sorted (from the order of the first sort, which it nearly keeps):
@@ -2066,7 +2106,7 @@ __global__ __launch_bounds__(256) void k_oa_sort(OaArgs a)
     __shared__ double ls[kOaMax];
     __shared__ int ix[kOaMax];
     const int N = a.n, kb = blockIdx.x, tid = threadIdx.x;
-    const int Na = a.nact ? a.nact[kb] : N;
+    const int Na = oa_active(a, kb);
     double *w = a.w + (size_t)kb * N;
     const int *perm = a.perm + (size_t)kb * N;
     for (int s = tid; s < N; s += blockDim.x) {
@@ -2090,48 +2130,62 @@ __global__ __launch_bounds__(256) void k_oa_sort(OaArgs a)
     }
 }
 
-// Element (i, c) of I + E in the rotated basis.
-__device__ __forceinline__ double2 oa_ehat(const double2 *Tc, const double2 *Gc, const double *lam,
-                                           const int *cinfo, const double *nu, int i, int c, double tau)
-{
-    const double2 g = Gc[i];
-    if (i == c) return make_double2(1.0 + 0.5 * (1.0 - g.x) + nu[c], 0.0);
-    const double lc = lam[c], li = lam[i];
-    if (oa_first_order(cinfo[i] & 0xffff, cinfo[c] & 0xffff, li, lc, tau)) {
-        const double2 t = Tc[i];
-        const double den = lc - li;
-        return make_double2((t.x - lc * g.x) / den, (t.y - lc * g.y) / den);
-    }
-    return make_double2(-0.5 * g.x, -0.5 * g.y);
-}
-
-// F = P (I + E) with columns in the final order: F[:, j] = (P (I + E))[:, perm2_j].
-__global__ __launch_bounds__(256) void k_oa_fmat(OaArgs a, double2 *F, const int *cut)
+// E (FP32) with columns in the final order: E[:, j] is the step of the
+// refined column perm2[j].
+__global__ __launch_bounds__(256) void k_oa_emat(OaArgs a, const int *cut)
 {
     const int N = a.n, kb = blockIdx.y;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)N * N) return;
-    const int r = (int)(t % N), j = (int)(t / N);
+    const int i = (int)(t % N), j = (int)(t / N);
     if (cut && j >= cut[kb]) return;                  // column outside the occupation window
     const int c = a.perm2[(size_t)kb * N + j];
-    const double *lam = a.lam + (size_t)kb * N;
-    const int *cinfo = a.cinfo + (size_t)kb * N;
-    const double *nu = a.nu + (size_t)kb * N;
-    const double tau = a.tau[kb];
-    const double2 *Tc = a.T + (size_t)kb * a.stride + (size_t)c * N;
-    const double2 *Gc = a.G + (size_t)kb * a.stride + (size_t)c * N;
-    const int ci = cinfo[r], m = (ci >> 24) & 0xff;
-    double2 v;
-    if (m < 2) {
-        v = oa_ehat(Tc, Gc, lam, cinfo, nu, r, c, tau);
-    } else {
-        const int b = ci & 0xffff, q = (ci >> 16) & 0xff;
-        const int *perm = a.perm + (size_t)kb * N;
-        const double2 *prow = a.pb + ((size_t)kb * N + b + q) * kOaMaxCl;
-        v = make_double2(0.0, 0.0);
-        for (int k = 0; k < m; ++k) v = zadd(v, zmul(prow[k], oa_ehat(Tc, Gc, lam, cinfo, nu, perm[b + k], c, tau)));
+    const int Na = oa_active(a, kb), Ma = oa_refined(a, kb);
+    float2 e = make_float2(0.f, 0.f);
+    if (c < Ma && i < Na && i != c) {
+        const int *cinfo = a.cinfo + (size_t)kb * N;
+        const double *lam = a.lam + (size_t)kb * N;
+        const int ci = cinfo[i], cc = cinfo[c];
+        const double li = lam[i], lc = lam[c];
+        if (oa_first_order(ci, cc, li, lc, a.tau[kb])) {
+            const float2 k = a.K32[(size_t)kb * a.stride + (size_t)c * N + i];
+            const double den = lc - li;
+            e = make_float2((float)(k.x / den), (float)(k.y / den));
+        } else if ((ci & 0xffff) == (cc & 0xffff) && ((cc >> 24) & 0xff) < 2) {
+            // a run too long to rotate: -(Y^H Y)_ic / 2
+            const double2 *yi = a.Y + (size_t)kb * a.stride + (size_t)i * N;
+            const double2 *yc = a.Y + (size_t)kb * a.stride + (size_t)c * N;
+            double gr = 0.0, gi = 0.0;
+            for (int r = 0; r < N; ++r) {
+                const double2 u = yi[r], v = yc[r];
+                gr += u.x * v.x + u.y * v.y;
+                gi += u.x * v.y - u.y * v.x;
+            }
+            e = make_float2((float)(-0.5 * gr), (float)(-0.5 * gi));
+        }
     }
-    F[(size_t)kb * a.stride + t] = v;
+    a.E32[(size_t)kb * a.stride + t] = e;
+}
+
+// y'_j = y_c + d_j (c = perm2[j]), normalised; one wave per column.
+__global__ __launch_bounds__(256) void k_oa_final(OaArgs a, const int *cut)
+{
+    const int N = a.n, kb = blockIdx.y, lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (j >= N || (cut && j >= cut[kb])) return;
+    const int c = a.perm2[(size_t)kb * N + j];
+    const double2 *y = a.Y + (size_t)kb * a.stride + (size_t)c * N;
+    const float2 *d = a.D32 + (size_t)kb * a.stride + (size_t)j * N;
+    double2 *o = a.Yr + (size_t)kb * a.stride + (size_t)j * N;
+    double ss = 0.0;
+    for (int i = lane; i < N; i += 64) {
+        const double vx = y[i].x + (double)d[i].x, vy = y[i].y + (double)d[i].y;
+        ss += vx * vx + vy * vy;
+    }
+    ss = wave_allsum(ss);
+    const double sc = 1.0 / sqrt(ss);
+    for (int i = lane; i < N; i += 64)
+        o[i] = make_double2(sc * (y[i].x + (double)d[i].x), sc * (y[i].y + (double)d[i].y));
 }
 
 // ---------------------------------------------------------------------------
@@ -3931,20 +3985,25 @@ void launch_rows_c32_to_c64(int n, int r0, int r1, const float2 *y32, double2 *y
     hipLaunchKernelGGL(k_rows_c32_to_c64, dim3(gx, nb), dim3(256), 0, st, n, r0, r1, y32, y64, stride);
 }
 int oa_max_n() { return kOaMax; }
-int oa_max_cluster() { return kOaMaxCl; }
-void launch_oa_refine(const OaArgs &a, int nb, hipStream_t st)
+void launch_oa_prepare(const OaArgs &a, int nb, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_oa_stats, dim3(nb), dim3(1024), 0, st, a);
+    hipLaunchKernelGGL(k_oa_resid, dim3((a.n + 3) / 4, nb), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_oa_clusters, dim3(nb), dim3(1024), 0, st, a);
     hipLaunchKernelGGL(k_oa_rr, dim3(kOaSlots, nb), dim3(64), 0, st, a);
-    hipLaunchKernelGGL(k_oa_xform, dim3(kOaSlots, nb), dim3(256), 0, st, a, 0);
-    hipLaunchKernelGGL(k_oa_xform, dim3(kOaSlots, nb), dim3(256), 0, st, a, 1);
+}
+void launch_oa_eigen(const OaArgs &a, int nb, hipStream_t st)
+{
     hipLaunchKernelGGL(k_oa_second, dim3((a.n + 15) / 16, nb), dim3(1024), 0, st, a);
     hipLaunchKernelGGL(k_oa_sort, dim3(nb), dim3(256), 0, st, a);
 }
-void launch_oa_fmat(const OaArgs &a, double2 *f, int nb, hipStream_t st, const int *cut)
+void launch_oa_emat(const OaArgs &a, int nb, hipStream_t st, const int *cut)
 {
-    hipLaunchKernelGGL(k_oa_fmat, dim3((unsigned)(((size_t)a.n * a.n + 255) / 256), nb), dim3(256), 0, st, a, f,
+    hipLaunchKernelGGL(k_oa_emat, dim3((unsigned)(((size_t)a.n * a.n + 255) / 256), nb), dim3(256), 0, st, a,
                        cut);
+}
+void launch_oa_final(const OaArgs &a, int nb, hipStream_t st, const int *cut)
+{
+    hipLaunchKernelGGL(k_oa_final, dim3((a.n + 3) / 4, nb), dim3(256), 0, st, a, cut);
 }
 void launch_coldot(int N, const double2 *Z, const double2 *X, const double2 *Y, size_t stride,
                    double2 *px, double2 *py, int nb, hipStream_t st, const int *cut)

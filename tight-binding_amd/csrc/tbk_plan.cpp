@@ -93,9 +93,8 @@ struct Workspace {
     float2 *A32 = nullptr, *P32 = nullptr, *T32 = nullptr, *VT32 = nullptr, *Y32 = nullptr,
            *X32 = nullptr;
     // the refinement step's scratch (tbk::OaArgs)
-    double *lam = nullptr, *nu = nullptr, *tau = nullptr;
+    double *lam = nullptr, *rn = nullptr, *tau = nullptr;
     int *perm = nullptr, *perm2 = nullptr, *cinfo = nullptr, *clist = nullptr, *ncl = nullptr;
-    double2 *pb = nullptr;
     // multi-workgroup tridiagonalisation (few items of large order)
     bool wide = false;
     // two-stage tridiagonalisation (band reduction + bulge chasing) instead
@@ -144,10 +143,9 @@ struct Workspace {
         int rc = TBK_OK;
         rc |= dalloc(&A32, nn); rc |= dalloc(&VT32, nn); rc |= dalloc(&Y32, nn); rc |= dalloc(&X32, nn);
         rc |= dalloc(&P32, (size_t)n * 96 * b); rc |= dalloc(&T32, tstride() * b);
-        rc |= dalloc(&lam, (size_t)n * b); rc |= dalloc(&nu, (size_t)n * b); rc |= dalloc(&tau, (size_t)b);
+        rc |= dalloc(&lam, (size_t)n * b); rc |= dalloc(&rn, (size_t)n * b); rc |= dalloc(&tau, (size_t)b);
         rc |= dalloc(&perm, (size_t)n * b); rc |= dalloc(&perm2, (size_t)n * b); rc |= dalloc(&cinfo, (size_t)n * b);
         rc |= dalloc(&clist, (size_t)n * b); rc |= dalloc(&ncl, (size_t)b);
-        rc |= dalloc(&pb, (size_t)n * tbk::oa_max_cluster() * b);
         return rc ? TBK_ERR_NOMEM : TBK_OK;
     }
     int alloc_wide(bool fp32)
@@ -206,8 +204,7 @@ struct Workspace {
         dfree(px); dfree(py); dfree(d); dfree(e); dfree(w); dfree(Vt); dfree(scratch);
         dfree(info); dfree(sinfo);
         dfree(A32); dfree(P32); dfree(T32); dfree(VT32); dfree(Y32); dfree(X32);
-        dfree(lam); dfree(nu); dfree(tau); dfree(perm); dfree(perm2); dfree(cinfo); dfree(clist); dfree(ncl);
-        dfree(pb);
+        dfree(lam); dfree(rn); dfree(tau); dfree(perm); dfree(perm2); dfree(cinfo); dfree(clist); dfree(ncl);
     }
 };
 
@@ -564,22 +561,24 @@ void solve_batch_mixed(Workspace &ws, int nb, const int *nact, double *w_out, hi
     }
     tm.end();
     tm.begin("refine");
-    // X = C Y, Z = Y^H X (= Y^H C Y), VT = Y^H Y (Hermitian: lower triangles
-    // computed, mirrored into the upper ones)
+    // X = C Y in FP64; Rayleigh quotients, residuals (FP32, in X32) and
+    // rotated clusters; K = Y^H R in FP32 (in A32, free after the
+    // reduction); the eigenvalues into w_out
     tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.Y, n, ss, ws.X, n, ss), 0, 0, nb, st);
-    tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.Z, n, ss, 5), 1, 0, nb, st);
-    tbk::launch_zgemm(gemm(n, n, n, ws.Y, n, ss, ws.Y, n, ss, ws.VT, n, ss, 5), 1, 0, nb, st);
-    // eigenvalues into w_out, then X = P (I + E) with sorted columns;
-    // VT = Y X; Z = U^{-1} VT
-    const tbk::OaArgs oa{n, ss, nact, ws.Z, ws.VT, ws.lam, ws.nu, ws.tau, w_out,
-                         ws.perm, ws.perm2, ws.cinfo, ws.clist, ws.ncl, ws.pb};
-    tbk::launch_oa_refine(oa, nb, st);
+    const tbk::OaArgs oa{n, ss, nact, nullptr, ws.w, ws.X, ws.Y, ws.Y32, ws.X32, ws.A32, ws.VT32, ws.X32, ws.VT,
+                         ws.lam, ws.rn, ws.tau, w_out, ws.perm, ws.perm2, ws.cinfo, ws.clist, ws.ncl};
+    tbk::launch_oa_prepare(oa, nb, st);
+    tbk::launch_cgemm(cgemm(n, n, n, ws.Y32, n, ss, ws.X32, n, ss, ws.A32, n, ss), 1, 0, nb, st);
+    tbk::launch_oa_eigen(oa, nb, st);
     // the eigenvalues are final: occupation window, then only the columns
-    // it keeps of F, Y F and U^{-1} (Y F)
+    // it keeps of E, D = Y E (FP32, in X32), the refined Y' (in VT) and
+    // Z = U^{-1} Y'
     if (wnd.cut) tbk::launch_fermi_window_group(wnd.fargs, wnd.nstruct, st);
     if (wnd.vectors) {
-        tbk::launch_oa_fmat(oa, ws.X, nb, st, wnd.cut);
-        tbk::launch_zgemm(with_ncol(gemm(n, n, n, ws.Y, n, ss, ws.X, n, ss, ws.VT, n, ss), wnd.cut), 0, 0, nb, st);
+        tbk::launch_oa_emat(oa, nb, st, wnd.cut);
+        tbk::launch_cgemm(with_ncol(cgemm(n, n, n, ws.Y32, n, ss, ws.VT32, n, ss, ws.X32, n, ss), wnd.cut), 0, 0,
+                          nb, st);
+        tbk::launch_oa_final(oa, nb, st, wnd.cut);
         tbk::launch_zgemm(with_ncol(gemm(n, n, n, ws.S, n, ss, ws.VT, n, ss, ws.Z, n, ss, 1), wnd.cut), 0, 0, nb, st);
     }
     tm.end();

@@ -322,17 +322,32 @@ __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a, int nb)
 // ---------------------------------------------------------------------------
 constexpr int NB = 32;
 
-// In-place upper Cholesky of a w x w block held in LDS (row stride 33).
-__device__ bool lds_potrf(double2 (*D)[NB + 1], int w)
+// The panel kernel's 256 threads own the elements of the upper triangle of
+// the NB x NB block, column by column (element q = c (c + 1) / 2 + r, r <= c),
+// q = tid, tid + 256, tid + 512: fixed places, so that no step divides.
+constexpr int kPotrfThreads = 256;
+constexpr int kPotrfOwn = (NB * (NB + 1) / 2 + kPotrfThreads - 1) / kPotrfThreads;
+__device__ __forceinline__ void upper_place(int q, int &r, int &c)
 {
-    // Right-looking on the unscaled rows: step j subtracts
-    // conj(D[j][r]) D[j][c] / p_j from the upper triangle of the trailing
-    // block (rows > j only, so row j is read and never written within the
-    // step: one barrier per column); the pivots p_j are kept and every row
-    // is scaled by 1 / sqrt(p_j) at the end.
+    int cc = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
+    while (cc * (cc + 1) / 2 > q) --cc;
+    while ((cc + 1) * (cc + 2) / 2 <= q) ++cc;
+    c = cc;
+    r = q - cc * (cc + 1) / 2;
+}
+
+// In-place upper Cholesky of a w x w block held in LDS (row stride 33).
+// Right-looking on the unscaled rows: step j subtracts
+// conj(D[j][r]) D[j][c] / p_j from the upper triangle of the trailing block
+// (rows > j only, so row j is read and never written within the step: one
+// barrier per column); the pivots p_j are kept and every row is scaled by
+// 1 / sqrt(p_j) at the end.
+__device__ bool lds_potrf(double2 (*D)[NB + 1], int w, const int (&orow)[kPotrfOwn], const int (&ocol)[kPotrfOwn])
+{
     __shared__ double piv[NB];
     __shared__ int bad;
     if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
     for (int j = 0; j < w; ++j) {
         const double p = D[j][j].x;
         const bool ok = p > 0.0;
@@ -341,15 +356,17 @@ __device__ bool lds_potrf(double2 (*D)[NB + 1], int w)
             piv[j] = ok ? p : 1.0;
             if (!ok) bad = 1;
         }
-        const int m = w - j - 1;
-        for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
-            const int r = j + 1 + t % m, c = j + 1 + t / m;
-            if (r <= c) D[r][c] = zsub(D[r][c], zscale(pinv, zcmul(D[j][r], D[j][c])));
+#pragma unroll
+        for (int s = 0; s < kPotrfOwn; ++s) {
+            const int r = orow[s], c = ocol[s];
+            if (r > j && c < w) D[r][c] = zsub(D[r][c], zscale(pinv, zcmul(D[j][r], D[j][c])));
         }
         __syncthreads();
     }
-    for (int t = threadIdx.x; t < w * w; t += blockDim.x) {
-        const int r = t % w, c = t / w;
+#pragma unroll
+    for (int s = 0; s < kPotrfOwn; ++s) {
+        const int r = orow[s], c = ocol[s];
+        if (c >= w) continue;
         if (r < c) D[r][c] = zscale(1.0 / sqrt(piv[r]), D[r][c]);
         else if (r == c) D[r][r] = make_double2(sqrt(piv[r]), 0.0);
     }
@@ -357,27 +374,28 @@ __device__ bool lds_potrf(double2 (*D)[NB + 1], int w)
     return bad == 0;
 }
 
-// In-place inverse of a w x w upper triangular block in LDS: rows of
-// X = U^{-1} from the last up, X[i][j] = -(sum_{k=i+1..j} U[i][k] X[k][j]) /
-// U[i][i], X[i][i] = 1 / U[i][i]; row i of X goes to the scratch block E
-// (rows below i of E are final, row i of D still holds U), one barrier per
-// row; E is copied back at the end.
-__device__ void lds_trtri(double2 (*D)[NB + 1], int w)
+// Inverse X = U^{-1} of the w x w upper triangular block in LDS into E, by
+// rows from the last up: X[i][i] = 1 / U[i][i], X[i][j] = -acc_ij / U[i][i]
+// with acc_ij = sum_{k=i+1..j} U[i][k] X[k][j] gathered in the owners'
+// registers as the rows k below are finished (one barrier per row).
+__device__ void lds_trtri(const double2 (*D)[NB + 1], double2 (*E)[NB + 1], int w, const int (&orow)[kPotrfOwn],
+                          const int (&ocol)[kPotrfOwn])
 {
-    __shared__ double2 E[NB][NB + 1];
-    for (int i = w - 1; i >= 0; --i) {
-        const int j = i + (int)threadIdx.x;
-        if (j < w) {
-            const double dinv = 1.0 / D[i][i].x;
-            double2 acc = zzero();
-            for (int k = i + 1; k <= j; ++k) acc = zadd(acc, zmul(D[i][k], E[k][j]));
-            E[i][j] = j == i ? make_double2(dinv, 0.0) : zscale(-dinv, acc);
-        }
+    double2 acc[kPotrfOwn];
+#pragma unroll
+    for (int s = 0; s < kPotrfOwn; ++s) acc[s] = zzero();
+    for (int k = w - 1; k >= 0; --k) {
+        const double dinv = 1.0 / D[k][k].x;
+#pragma unroll
+        for (int s = 0; s < kPotrfOwn; ++s)
+            if (orow[s] == k && ocol[s] < w)
+                E[k][ocol[s]] = ocol[s] == k ? make_double2(dinv, 0.0) : zscale(-dinv, acc[s]);
         __syncthreads();
-    }
-    for (int t = threadIdx.x; t < w * w; t += blockDim.x) {
-        const int r = t % w, c = t / w;
-        if (r <= c) D[r][c] = E[r][c];
+#pragma unroll
+        for (int s = 0; s < kPotrfOwn; ++s) {
+            const int r = orow[s], c = ocol[s];
+            if (r < k && k <= c && c < w) acc[s] = zadd(acc[s], zmul(D[r][k], E[k][c]));
+        }
     }
     __syncthreads();
 }
@@ -389,10 +407,9 @@ __device__ void lds_trtri(double2 (*D)[NB + 1], int w)
 // blocks of U^{-1} as GEMMs with U11^{-1}, and the trailing update
 // S22 -= U12^H U12.  The strict lower triangle of the panel's columns is
 // cleared (the GEMMs read whole tiles).  info[k] = first failing column + 1.
-__global__ __launch_bounds__(256) void k_potrf_panel(int N, double2 *S, size_t stride, int j0,
-                                                     int *info)
+__global__ __launch_bounds__(kPotrfThreads) void k_potrf_panel(int N, double2 *S, size_t stride, int j0, int *info)
 {
-    __shared__ double2 D[NB][NB + 1];
+    __shared__ double2 D[NB][NB + 1], E[NB][NB + 1];
     const int kb = blockIdx.x, tid = threadIdx.x;
     double2 *A = S + (size_t)kb * stride;
     if (j0 == 0) {
@@ -401,24 +418,29 @@ __global__ __launch_bounds__(256) void k_potrf_panel(int N, double2 *S, size_t s
         return;                                      // uniform per block
     }
     const int w = min(NB, N - j0);
-    for (int t = tid; t < w * w; t += blockDim.x) {
-        const int r = t % w, c = t / w;
-        D[r][c] = A[(size_t)(j0 + c) * N + j0 + r];
+    int orow[kPotrfOwn], ocol[kPotrfOwn];
+#pragma unroll
+    for (int s = 0; s < kPotrfOwn; ++s) {
+        const int q = tid + kPotrfThreads * s;
+        if (q < NB * (NB + 1) / 2) upper_place(q, orow[s], ocol[s]);
+        else { orow[s] = NB; ocol[s] = NB; }         // no element
     }
+    // eight threads per column of the block, each every eighth row
+    const int cc = tid >> 3, r8 = tid & 7;
+    if (cc < w)
+        for (int r = r8; r < w; r += 8) D[r][cc] = A[(size_t)(j0 + cc) * N + j0 + r];
     __syncthreads();
-    if (!lds_potrf(D, w)) {
+    if (!lds_potrf(D, w, orow, ocol)) {
         if (tid == 0) info[kb] = j0 + 1;
         return;
     }
-    lds_trtri(D, w);
-    for (int t = tid; t < w * N; t += blockDim.x) {   // write U11^{-1}, clear below
-        const int r = t % N, c = t / N;
-        if (r >= j0 && r - j0 <= c) {
-            if (r - j0 < w) A[(size_t)(j0 + c) * N + r] = D[r - j0][c];
-        } else if (r > j0 + c) {
-            A[(size_t)(j0 + c) * N + r] = zzero();
+    lds_trtri(D, E, w, orow, ocol);
+    // U11^{-1} into the block, the panel's columns cleared below it
+    if (cc < w)
+        for (int r = j0 + r8; r < N; r += 8) {
+            if (r - j0 <= cc) A[(size_t)(j0 + cc) * N + r] = E[r - j0][cc];
+            else A[(size_t)(j0 + cc) * N + r] = zzero();
         }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1786,22 +1808,25 @@ __global__ __launch_bounds__(256) void k_oa_resid(OaArgs a)
         if (lane == 0) { lam[j] = a.theta[(size_t)kb * N + j]; rn[j] = 0.0; }
         return;
     }
+    // Y in FP64 holds the FP32 values of Y32 until the clusters are rotated
     const double2 *x = a.X + (size_t)kb * a.stride + (size_t)j * N;
-    const double2 *y = a.Y + (size_t)kb * a.stride + (size_t)j * N;
+    const float2 *y = a.Y32 + (size_t)kb * a.stride + (size_t)j * N;
     float2 *r = a.R32 + (size_t)kb * a.stride + (size_t)j * N;
     double t = 0.0, g = 0.0;
     for (int i = lane; i < N; i += 64) {
-        const double2 xv = x[i], yv = y[i];
-        t += yv.x * xv.x + yv.y * xv.y;
-        g += yv.x * yv.x + yv.y * yv.y;
+        const double2 xv = x[i];
+        const double yx = y[i].x, yy = y[i].y;
+        t += yx * xv.x + yy * xv.y;
+        g += yx * yx + yy * yy;
     }
     t = wave_allsum(t);
     g = wave_allsum(g);
     const double l = t / g;
     double rr = 0.0;
     for (int i = lane; i < N; i += 64) {
-        const double2 xv = x[i], yv = y[i];
-        const double rx = xv.x - l * yv.x, ry = xv.y - l * yv.y;
+        const double2 xv = x[i];
+        const double yx = y[i].x, yy = y[i].y;
+        const double rx = xv.x - l * yx, ry = xv.y - l * yy;
         r[i] = make_float2((float)rx, (float)ry);
         rr += rx * rx + ry * ry;
     }
@@ -2413,8 +2438,16 @@ __device__ double fermi_level(const FermiArgs &f, double *s1, double *s2, int &i
         double ns = 0.0, dns = 0.0;
         for (int k = tid; k < nk; k += blockDim.x) {
             const double *ev = f.esk + (size_t)k * ld;
-            double a = 0.0, b = 0.0;
-            for (int i = 0; i < nv; ++i) {
+            // the levels of a k are ascending: those below mu - 20 kT (full,
+            // 1 each) are counted by bisection, the sum of their ones being
+            // exact, and the walk starts at the first one above
+            int lo = 0, hi = nv;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (ev[mid] < mum20) lo = mid + 1; else hi = mid;
+            }
+            double a = (double)lo, b = 0.0;
+            for (int i = lo; i < nv; ++i) {
                 const double x = ev[i];
                 if (x < mum20) a = a + 1.0;
                 else if (x < mup20) {

@@ -318,35 +318,39 @@ void hetrd(Workspace &ws, int nb, hipStream_t st, StageTimer &tm)
     tm.end();
 }
 
-// S = U^H U and U^{-1} in place (lower triangle cleared): panel kernels for
-// the diagonal blocks, GEMMs for the O(n^3) updates.
-void potrf_trtri(Workspace &ws, int nb, hipStream_t st)
+// S = U^H U and U^{-1} in place (lower triangle cleared), recursively over
+// the diagonal blocks of panels p0 .. p1-1 (kNB columns each): the leading
+// half's factor and inverse, U12 = U11^{-H} S12 (into Y), S22 -= U12^H U12,
+// the trailing half's, then (U^{-1})_12 = -U11^{-1} U12 U22^{-1} (through Z);
+// a single panel is one panel kernel.  The GEMMs of the top levels have
+// inner dimensions of half the order, so the trailing matrix is read and
+// written once per level rather than once per panel.  Y and Z are scratch.
+void potrf_rec(Workspace &ws, int p0, int p1, int nb, hipStream_t st)
 {
     const int n = ws.n;
     const size_t ss = ws.stride();
-    for (int j0 = 0; j0 < n; j0 += kNB) {
-        const int w = std::min(kNB, n - j0), s0 = j0 + w, m = n - s0;
-        tbk::launch_potrf_panel(n, ws.S, ss, j0, ws.info, nb, st);   // S[J, J] <- U_JJ^{-1}
-        if (m > 0) {
-            const double2 *dinv = ws.S + (size_t)j0 * n + j0;
-            double2 *u12 = ws.S + (size_t)s0 * n + j0;
-            // U12 = U_JJ^{-H} S12 in place: every output tile covers all w
-            // rows and reads only its own columns of S12 before it writes
-            tbk::launch_zgemm(gemm(w, m, w, dinv, n, ss, u12, n, ss, u12, n, ss), 1, 0, nb, st);
-            // S22 -= U12^H U12 on the tiles touching the upper triangle (the
-            // factorisation reads only that; the panel kernel clears below)
-            tbk::launch_zgemm(gemm(m, m, w, u12, n, ss, u12, n, ss, ws.S + (size_t)s0 * n + s0, n, ss,
-                                   7, double2{-1.0, 0.0}, double2{1.0, 0.0}), 1, 0, nb, st);
-        }
+    if (p1 - p0 == 1) {
+        tbk::launch_potrf_panel(n, ws.S, ss, p0 * kNB, ws.info, nb, st);   // S[J, J] <- U_JJ^{-1}
+        return;
     }
-    // U^{-1} by block columns J: Y = Uinv[0:j0, 0:j0] U[0:j0, J], then
-    // Uinv[0:j0, J] = -Y U_JJ^{-1} (the diagonal block already holds U_JJ^{-1})
-    for (int j0 = kNB; j0 < n; j0 += kNB) {
-        const int w = std::min(kNB, n - j0);
-        tbk::launch_zgemm(gemm(j0, w, j0, ws.S, n, ss, ws.S + (size_t)j0 * n, n, ss, ws.Y, n, ss, 1), 0, 0, nb, st);
-        tbk::launch_zgemm(gemm(j0, w, w, ws.Y, n, ss, ws.S + (size_t)j0 * n + j0, n, ss, ws.S + (size_t)j0 * n, n,
-                               ss, 0, double2{-1.0, 0.0}), 0, 0, nb, st);
-    }
+    const int pm = p0 + (p1 - p0) / 2;
+    const int a = p0 * kNB, m = pm * kNB, b = std::min(p1 * kNB, n), n1 = m - a, n2 = b - m;
+    const size_t o11 = (size_t)a * n + a, o12 = (size_t)m * n + a, o22 = (size_t)m * n + m;
+    potrf_rec(ws, p0, pm, nb, st);
+    tbk::launch_zgemm(gemm(n1, n2, n1, ws.S + o11, n, ss, ws.S + o12, n, ss, ws.Y + o12, n, ss, 2), 1, 0, nb, st);
+    // on the tiles touching the upper triangle (the factorisation reads only
+    // that; the panel kernels clear below)
+    tbk::launch_zgemm(gemm(n2, n2, n1, ws.Y + o12, n, ss, ws.Y + o12, n, ss, ws.S + o22, n, ss, 7,
+                           double2{-1.0, 0.0}, double2{1.0, 0.0}), 1, 0, nb, st);
+    potrf_rec(ws, pm, p1, nb, st);
+    tbk::launch_zgemm(gemm(n1, n2, n1, ws.S + o11, n, ss, ws.Y + o12, n, ss, ws.Z + o12, n, ss, 1), 0, 0, nb, st);
+    tbk::launch_zgemm(gemm(n1, n2, n2, ws.Z + o12, n, ss, ws.S + o22, n, ss, ws.S + o12, n, ss, 3,
+                           double2{-1.0, 0.0}), 0, 0, nb, st);
+}
+
+void potrf_trtri(Workspace &ws, int nb, hipStream_t st)
+{
+    potrf_rec(ws, 0, (ws.n + kNB - 1) / kNB, nb, st);
 }
 
 tbk::GemmArgsT<float> cgemm(int m, int n, int k, const float2 *a, int lda, size_t sa, const float2 *b,

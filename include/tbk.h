@@ -598,7 +598,8 @@ int tbk_debug_gemm_time(int device, int precision, int tile, int m, int n,
  * upper triangular; 4 only the tiles touching the lower triangle of C;
  * 5 (m == n) the lower triangle of the product and its conjugate transpose
  * above the diagonal; 6 = 5 with op(A) lower triangular; 7 only the tiles
- * touching the upper triangle of C. */
+ * touching the upper triangle of C; 9 (opa = 0, k == m) A Hermitian, read
+ * as tril(A, -1) + diag(A) / 2 (its upper triangle is not read). */
 int tbk_debug_zgemm(int device, int precision, int tile, int m, int n, int k,
                     int opa, int opb, int tri, const void *a, int lda,
                     const void *b, int ldb, void *c, int ldc);

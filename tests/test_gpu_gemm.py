@@ -85,3 +85,14 @@ def test_gemm_triangular_variants(tile):
     ref = U.conj().T @ X
     herm = np.tril(ref) + np.tril(ref, -1).conj().T
     assert np.abs(c - herm).max() < 1e-11
+    # tri 9: a full Hermitian A read as L = tril(A, -1) + diag(A) / 2, whatever
+    # its upper triangle holds; then A = L + L^H (the pressure's products)
+    H = X + X.conj().T
+    Hs = np.tril(H) + np.triu(rand(rng, n, n), 1)          # the upper part must not be read
+    Lh = np.tril(H, -1) + np.diag(np.diag(H)) / 2
+    for m2 in (n, 37):
+        c = run(0, tile, n, m2, n, 0, 0, 9, Hs, X[:, :m2])
+        assert np.abs(c - Lh @ X[:, :m2]).max() < 1e-11
+        quad = 2 * np.einsum("ij,ij->j", X[:, :m2].conj(), c).real
+        want = np.einsum("ij,ij->j", X[:, :m2].conj(), H @ X[:, :m2]).real
+        assert np.abs(quad - want).max() < 1e-9 * np.abs(want).max()

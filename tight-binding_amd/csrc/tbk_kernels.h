@@ -58,7 +58,8 @@ struct GemmArgsT {
                                    //   (Hermitian C; alpha real, beta 0); 6 = 5 with
                                    //   A lower triangular as in 2; 7 upper triangle of C only;
                                    //   8 = 5 with beta (C = alpha A B + beta C on the
-                                   //   lower triangle, mirrored)
+                                   //   lower triangle, mirrored); 9 A (not transposed)
+                                   //   Hermitian, read as tril(A, -1) + diag(A) / 2
     const int *ncol = nullptr;     // [nb] per item: column tiles at or beyond ncol[k]
                                    // are skipped (C left as it was); null: all
     // optional copy of every element written to C, in the other precision
@@ -154,6 +155,8 @@ struct PerturbArgs {
     double eigcut;
     const int *nact;               // [nb] active order of each item (null: n)
     const double *w;
+    // x = L_dH z, y = L_dS z with L = tril(M, -1) + diag(M) / 2 (GEMM tri 9):
+    // z_a^H M z_b = z_a^H (L z_b) + conj(z_b^H (L z_a)); px, py their column dots
     const double2 *px, *py, *z, *x, *y;
     double *desk;
     int *ncut, *err;               // ncut [k0+nb], err [k0+nb] (2: > 9 degenerate)

@@ -460,7 +460,10 @@ __global__ __launch_bounds__(kPotrfThreads) void k_potrf_panel(int N, double2 *S
 // and C receives the lower triangle of the product and its conjugate
 // transpose above the diagonal (a Hermitian C from its lower half; alpha
 // real, beta 0); 6 = 5 with op(A) lower triangular as in 2; 7 = only tiles
-// touching the upper triangle of C are computed.
+// touching the upper triangle of C are computed; 9 = op(A) (not transposed)
+// is read as L = tril(A, -1) + diag(A) / 2 of a Hermitian A (elements above
+// the diagonal as zeros, the diagonal halved; k beyond the row tile
+// skipped), so that A = L + L^H and z^H A z = 2 Re(z^H L z).
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -510,7 +513,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
     const int wm = (wv & 1) * (BM / 2), wn = (wv >> 1) * (BN / 2);
     int kbeg = 0, kend = g.k;
     if (g.tri == 1) kbeg = bm;
-    if (g.tri == 2 || g.tri == 6) kend = min(g.k, bm + BM);
+    if (g.tri == 2 || g.tri == 6 || g.tri == 9) kend = min(g.k, bm + BM);
     if (g.tri == 3) kend = min(g.k, bn + BN);
     kbeg = (kbeg / BK) * BK;
     v4 acc1[MI][NJ], acc2[MI][NJ], acc3[MI][NJ];
@@ -532,6 +535,10 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<R> g)
             if (gm < g.m && gk < kend) {
                 if (OPA == 0) v = A[(size_t)gk * g.lda + gm];
                 else { v = A[(size_t)gm * g.lda + gk]; if (OPA == 1) v.y = -v.y; }
+                if (OPA == 0 && g.tri == 9 && gk >= gm) {
+                    if (gk > gm) v = czero<C>();
+                    else { v.x *= R(0.5); v.y *= R(0.5); }
+                }
             }
             ra[u] = v;
         }
@@ -2341,7 +2348,7 @@ __global__ void k_perturb(PerturbArgs p)
             // 1 x 1 pencil, so the reference's perturb works on a zero vector
             desk[start] = 0.0;
         } else if (num == 1) {
-            desk[start] = px[start].x - es * py[start].x;
+            desk[start] = 2.0 * (px[start].x - es * py[start].x);
         } else {
             double2 hd[81];
             for (int b = 0; b < num; ++b)
@@ -2349,14 +2356,16 @@ __global__ void k_perturb(PerturbArgs p)
                     const int ja = start + a, jb = start + b;
                     double2 v;
                     if (a == b) {
-                        v = make_double2(px[ja].x - es * py[ja].x, 0.0);
+                        v = make_double2(2.0 * (px[ja].x - es * py[ja].x), 0.0);
                     } else {
+                        // z_a^H M z_b = z_a^H (L z_b) + conj(z_b^H (L z_a))
                         double sr = 0, si = 0;
                         for (int i = 0; i < N; ++i) {
-                            const double2 z = Z[(size_t)ja * LD + i];
-                            const double2 q = zsub(X[(size_t)jb * LD + i], zscale(es, Y[(size_t)jb * LD + i]));
-                            const double2 t = zcmul(z, q);
-                            sr += t.x; si += t.y;
+                            const double2 za = Z[(size_t)ja * LD + i], zb = Z[(size_t)jb * LD + i];
+                            const double2 qb = zsub(X[(size_t)jb * LD + i], zscale(es, Y[(size_t)jb * LD + i]));
+                            const double2 qa = zsub(X[(size_t)ja * LD + i], zscale(es, Y[(size_t)ja * LD + i]));
+                            const double2 t = zcmul(za, qb), u = zcmul(zb, qa);
+                            sr += t.x + u.x; si += t.y - u.y;
                         }
                         v = make_double2(sr, si);
                     }

@@ -1215,10 +1215,12 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
                 tbk::launch_build_hs(build_args(p, k0, true), p->maxnat, nb, st);
                 if (p->noncol) tbk::launch_spinor_expand(build_args(p, k0, true), p->maxnorb, nb, st);
             }
-            tbk::launch_zgemm(with_ncol(gemm(n, n, n, dH, n, ss, p->ws.Z, n, ss, p->ws.X, n, ss), wnd.cut), 0, 0,
-                              nb, st);
-            tbk::launch_zgemm(with_ncol(gemm(n, n, n, dS, n, ss, p->ws.Z, n, ss, p->ws.Y, n, ss), wnd.cut), 0, 0,
-                              nb, st);
+            // X = L_dH Z, Y = L_dS Z over the lower triangles (half the
+            // products; k_perturb adds the conjugate halves)
+            tbk::launch_zgemm(with_ncol(gemm(n, n, n, dH, n, ss, p->ws.Z, n, ss, p->ws.X, n, ss, 9), wnd.cut), 0,
+                              0, nb, st);
+            tbk::launch_zgemm(with_ncol(gemm(n, n, n, dS, n, ss, p->ws.Z, n, ss, p->ws.Y, n, ss, 9), wnd.cut), 0,
+                              0, nb, st);
             tbk::launch_coldot(n, p->ws.Z, p->ws.X, p->ws.Y, ss, p->ws.px, p->ws.py, nb, st, wnd.cut);
             tbk::PerturbArgs pa;
             pa.cut = wnd.cut;
@@ -1640,8 +1642,9 @@ extern "C" int tbk_zhegv(int device, int n, const double *h, const double *s, do
     return rc;
 }
 
-// perturb.f:1-221 for one k on the device: X = dH Z, Y = dS Z (GEMMs), the
-// column dots, then the degenerate groups (k_coldot, k_perturb).
+// perturb.f:1-221 for one k on the device: X = L_dH Z, Y = L_dS Z over the
+// lower triangles (GEMMs), the column dots, then the degenerate groups
+// (k_coldot, k_perturb).
 extern "C" int tbk_perturb(int device, int n, const double *dh, const double *ds, const double *z,
                            const double *eig, double eigcut, int *ntot, double *deig)
 {
@@ -1662,8 +1665,8 @@ extern "C" int tbk_perturb(int device, int n, const double *dh, const double *ds
         hipMemcpyAsync(ws.w, eig, n * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) {
         rc = TBK_ERR_HIP;
     } else {
-        tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.Z, n, ss, ws.X, n, ss), 0, 0, 1, st);
-        tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Z, n, ss, ws.Y, n, ss), 0, 0, 1, st);
+        tbk::launch_zgemm(gemm(n, n, n, ws.A, n, ss, ws.Z, n, ss, ws.X, n, ss, 9), 0, 0, 1, st);
+        tbk::launch_zgemm(gemm(n, n, n, ws.S, n, ss, ws.Z, n, ss, ws.Y, n, ss, 9), 0, 0, 1, st);
         tbk::launch_coldot(n, ws.Z, ws.X, ws.Y, ss, ws.px, ws.py, 1, st);
         tbk::PerturbArgs pa;
         pa.n = n; pa.nb = 1; pa.k0 = 0; pa.stride = ss; pa.eigcut = eigcut; pa.nact = nullptr;
@@ -1829,7 +1832,9 @@ extern "C" int tbk_debug_zgemm(int device, int precision, int tile, int m, int n
                                int tri, const void *a, int lda, const void *b, int ldb, void *c, int ldc)
 {
     if (tile >= tbk::kGemmTiles || opa < 0 || opa > 1 || opb < 0 || opb > 1) return TBK_ERR_ARG;
-    if (tri < 0 || tri > 7 || (tri >= 5 && tri <= 6 && m != n) || m <= 0 || n <= 0 || k <= 0) return TBK_ERR_ARG;
+    if (tri < 0 || tri > 9 || tri == 8 || (tri >= 5 && tri <= 6 && m != n) || (tri == 9 && (opa != 0 || k != m)) ||
+        m <= 0 || n <= 0 || k <= 0)
+        return TBK_ERR_ARG;
     HIP_TRY(hipSetDevice(device));
     if (precision == TBK_MIXED)
         return debug_gemm_t<float2, tbk::GemmArgsT<float>>(tile, m, n, k, opa, opb, tri, a, lda, b, ldb, c, ldc);

@@ -414,20 +414,32 @@ void apply_q(int n, int nb, typename tbk::cplx_t<R>::type *A, typename tbk::cplx
         const int c0 = g0 * pw;
         const int wlast = std::min(pw, n - off - (g1 - 1) * pw);
         const int W = (g1 - 1 - g0) * pw + wlast;      // columns of the group
+        // Column c of V (and of V T^H, a combination of the reflectors c and
+        // later) is zero above row c + off: with rows counted from c0 + off,
+        // op(A) of the first product of each pair below is upper triangular
+        // (tri 1: k below the row tile skipped) and V's block of the second
+        // lower trapezoidal (tri 2: k beyond the row tile skipped)
         for (int p = g1 - 2; p >= g0; --p) {            // merge V T^H blocks
             const int cp = p * pw, cr = cp + pw, wr = c0 + W - cr;
             const int rr0 = cr + off, k = n - rr0;       // rows where V_rest can be non-zero
-            launch_gemm_any(gemm_t<R>(wr, pw, k, A + (size_t)cr * n + rr0, n, ss, VT + (size_t)cp * n + rr0, n,
-                                      ss, M, wr, sm), 1, 0, nb, st);
-            launch_gemm_any(gemm_t<R>(k, pw, wr, VT + (size_t)cr * n + rr0, n, ss, M, wr, sm,
-                                      VT + (size_t)cp * n + rr0, n, ss, R(-1), R(1)), 0, 0, nb, st);
+            tbk::GemmArgsT<R> g1a = gemm_t<R>(wr, pw, k, A + (size_t)cr * n + rr0, n, ss, VT + (size_t)cp * n + rr0,
+                                              n, ss, M, wr, sm);
+            g1a.tri = 1;
+            launch_gemm_any(g1a, 1, 0, nb, st);
+            tbk::GemmArgsT<R> g1b = gemm_t<R>(k, pw, wr, VT + (size_t)cr * n + rr0, n, ss, M, wr, sm,
+                                              VT + (size_t)cp * n + rr0, n, ss, R(-1), R(1));
+            g1b.tri = 2;
+            launch_gemm_any(g1b, 0, 0, nb, st);
         }
         const int r0 = c0 + off, m = n - r0;
         // X[0:W, :] = (V_c T_c^H)^H Y[r0:, :];  Y[r0:, :] -= V_c X
-        launch_gemm_any(with_ncol(gemm_t<R>(W, n, m, VT + (size_t)c0 * n + r0, n, ss, Y + r0, n, ss, X, n, ss), ncol),
-                        1, 0, nb, st);
+        tbk::GemmArgsT<R> gx = with_ncol(gemm_t<R>(W, n, m, VT + (size_t)c0 * n + r0, n, ss, Y + r0, n, ss, X, n, ss),
+                                         ncol);
+        gx.tri = 1;
+        launch_gemm_any(gx, 1, 0, nb, st);
         tbk::GemmArgsT<R> g = with_ncol(gemm_t<R>(m, n, W, A + (size_t)c0 * n + r0, n, ss, X, n, ss, Y + r0, n, ss,
                                                   R(-1), R(1)), ncol);
+        g.tri = 2;
         if (g0 == 0 && ycx) {          // the last group: Y rows r0.. also in the other precision
             g.cx = ycx + r0; g.ldcx = n; g.scx = ss;
         }

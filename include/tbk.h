@@ -347,6 +347,19 @@ int tbk_plan_create_noncol(int device, const tbk_structure_desc *st,
  * plans. */
 int tbk_plan_magnetization(const tbk_plan *plan, double *smag);
 
+/* Spin directions of a non-collinear plan (replaces the lspdir output of
+ * bande.f:1484-1561 with diag.f:458-523): on = 1 makes the following runs
+ * form every state's spin on each atom, (-2 Re, 2 Im, |z_dn|^2 - |z_up|^2)
+ * of conj(z_up) z_dn summed over the atom's orbitals, and its
+ * occupation-weighted sum over the k-points (states below E_F + 20 kT,
+ * Fermi-Dirac weights within 20 kT); eigenvectors are formed in every mode.
+ * TBK_ERR_ARG for a plan that is not non-collinear.  Memory: 24 N maxnat
+ * bytes per item. */
+int tbk_plan_set_spin_directions(tbk_plan *plan, int on);
+/* After such a run: spdir[3*i + l], component l of atom i's spin vector,
+ * of structure s (natoms of it). */
+int tbk_plan_spin_directions(tbk_plan *plan, int s, double *spdir);
+
 /* Run the whole bande k-loop + Fermi + sums on the device.  Replaces
  * bande(mode,...,efermi,earg1,earg2,...,iberr), bande.f:1, for the
  * collinear case (spin-polarised through tbk_plan_create_spin).  Modes 1-4,

@@ -6,7 +6,9 @@ with its mesh line aligned, ex2 as published (where both stop in kptgen), a
 SKIN walking every lattice menu and strain option of setlat, and ex1 with a
 two-channel parameter file, spin-flipped and non-collinear; and the QLMT /
 QAPW files of modes 6, 8, 9 and 10 (Lowdin projections on the device,
-diag.f:526-758) on ex1, ex2 and the spin inputs.
+diag.f:526-758) on ex1, ex2 and the spin inputs; and the SPNDIR file of the
+spin-direction output (diag.f:458-523, bande.f:1484-1561) on a 20-atom
+non-collinear cell, against the reference program built for it (msdir = 1).
 
 A line matches when its label and volume fields are the same text and every
 number agrees within one unit of its last printed digit (the FP64 device
@@ -118,6 +120,21 @@ def same_qfile(mine, ref):
     return True
 
 
+def same_spndir(mine, ref):
+    """SPNDIR (bande.f:1556-1557, (a20,i5,4f10.6,2f12.6)): label and atom
+    verbatim, the spin vector, its length and (theta, phi) within one unit
+    of their last digit."""
+    a, b = mine.splitlines(), ref.splitlines()
+    assert len(a) == len(b), (len(a), len(b))
+    for x, y in zip(a, b):
+        assert x[:25] == y[:25], (x, y)
+        u, v = x[25:].split(), y[25:].split()
+        assert len(u) == len(v) == 6, (x, y)
+        for p, q in zip(u, v):
+            assert _close(float(p), float(q), q), (x, y)
+    return True
+
+
 @pytest.mark.parametrize("name", SC.STATIC_CASES)
 def test_native_program_writes_the_reference_skeng(tmp_path, name):
     if not (os.path.isdir(EX1) and os.path.isdir(EX2)):
@@ -142,4 +159,4 @@ def test_native_program_writes_the_reference_skeng(tmp_path, name):
     if "qtext" in want:
         mine = (tmp_path / want["qfile"]).read_text()
         print(f"  {want['qfile']}: {len(mine.splitlines())} lines, {sum(x == y for x, y in zip(mine.splitlines(), want['qtext'].splitlines()))} identical", flush=True)
-        assert same_qfile(mine, want["qtext"])
+        assert (same_spndir if want["qfile"] == "SPNDIR" else same_qfile)(mine, want["qtext"])

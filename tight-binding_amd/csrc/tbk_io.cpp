@@ -729,6 +729,10 @@ namespace {
 std::string fmt_f(double x, int w, int d)
 {
     char buf[128];
+    if (std::isnan(x)) {                      // Fortran writes NaN right-justified
+        std::snprintf(buf, sizeof(buf), "%*s", w, "NaN");
+        return std::string(buf);
+    }
     std::snprintf(buf, sizeof(buf), "%*.*f", w, d, x);
     std::string s(buf);
     if ((int)s.size() > w) s.assign(w, '*');
@@ -896,6 +900,7 @@ extern "C" int tbk_static_run(const char *dir, int device, int precision, int *n
     FILE *out = rc ? nullptr : std::fopen((d + "/SKENG").c_str(), "w");
     if (!rc && !out) rc = TBK_ERR_ARG;
     FILE *qf = nullptr;                                          // static.f:757-778
+    FILE *spf = nullptr;                                         // SPNDIR (unit 42), static.f:1010-1017
     if (!rc && qmode) {
         qf = std::fopen((d + (mode == 8 ? "/QAPW" : "/QLMT")).c_str(), "w");
         if (!qf) rc = TBK_ERR_ARG;
@@ -912,7 +917,21 @@ extern "C" int tbk_static_run(const char *dir, int device, int precision, int *n
         if (r) { rc = r; break; }
         bool noncol = st.lnoncol != 0;
         if (noncol && ps.jspins == 1) noncol = false;              // static.f:982-996
-        if (noncol && (st.ltorque || st.lspdir)) { rc = TBK_ERR_ARG; break; }   // outputs outside this path
+        // static.f:983-1004 turns ltorque off with the non-collinear run but
+        // leaves lspdir on, and diag.f:268-283 then stops; refused here
+        if (!noncol && st.lspdir) { rc = TBK_ERR_ARG; break; }
+        const bool spdir = noncol && st.lspdir != 0;
+        if (noncol && st.ltorque) {
+            // static.f:1012 opens SKDERIV; bande computes the spin-angle
+            // derivatives of spinham (bande.f:897-901) and writes nothing there
+            FILE *fd = std::fopen((d + "/SKDERIV").c_str(), "a");
+            if (!fd) { rc = TBK_ERR_ARG; break; }
+            std::fclose(fd);
+        }
+        if (spdir && !spf) {
+            spf = std::fopen((d + "/SPNDIR").c_str(), "w");
+            if (!spf) { rc = TBK_ERR_ARG; break; }
+        }
         const int natx = noncol ? 2 * st.natoms : st.natoms;
         if (qf && mode != 8) {        // static.f:1127-1137, 1152-1157: the structure's header lines
             std::fprintf(qf, "%s%s%s%s     SPINS, WIND., SC. WIND.\n", fmt_i(1, 5).c_str(), fmt_i(1, 5).c_str(),
@@ -938,7 +957,24 @@ extern "C" int tbk_static_run(const char *dir, int device, int precision, int *n
             rc = tbk_plan_create_spin(device, &sd, param.data() + npard, flipos.data(), st.nk, kp.data(),
                                       wt.data(), 0, precision, &p);
         if (!rc && (mode == 6 || mode == 10)) rc = tbk_plan_set_window(p, 0);
+        if (!rc && spdir) rc = tbk_plan_set_spin_directions(p, 1);
         if (!rc) rc = tbk_plan_run(p, mode, tkb, eigcut, &ef, &e1, &e2);
+        if (!rc && spdir) {         // bande.f:1541-1560: (a20,i5,4f10.6,2f12.6) per atom
+            std::vector<double> sv(3 * (size_t)st.natoms);
+            rc = tbk_plan_spin_directions(p, 0, sv.data());
+            const double rad2deg = 180.0 / 3.141592653589793116;
+            std::string lab(st.label);
+            lab.resize(20, ' ');
+            for (int i = 0; !rc && i < st.natoms; ++i) {
+                const double x = sv[3 * i], y = sv[3 * i + 1], z = sv[3 * i + 2];
+                const double smagl = std::sqrt(x * x + y * y + z * z);
+                const double theta = rad2deg * std::acos(z / smagl), phi = rad2deg * std::atan2(y, x);
+                std::fprintf(spf, "%s%s%s%s%s%s%s%s\n", lab.c_str(), fmt_i(i + 1, 5).c_str(), fmt_f(x, 10, 6).c_str(),
+                             fmt_f(y, 10, 6).c_str(), fmt_f(z, 10, 6).c_str(), fmt_f(smagl, 10, 6).c_str(),
+                             fmt_f(theta, 12, 6).c_str(), fmt_f(phi, 12, 6).c_str());
+            }
+            std::fflush(spf);
+        }
         const bool with_smag = ps.jspins == 2 && !noncol;
         if (!rc && with_smag) rc = tbk_plan_magnetization(p, &smag);
         if (!rc && qf) rc = write_qlmt(qf, p, mode, eigcut, st, kkind.data(), ps, noncol, natx, kp.data(), wt.data());
@@ -953,6 +989,7 @@ extern "C" int tbk_static_run(const char *dir, int device, int precision, int *n
     }
     if (out) std::fclose(out);
     if (qf) std::fclose(qf);
+    if (spf) std::fclose(spf);
     tbk_skin_close(skin);
     return rc;
 }

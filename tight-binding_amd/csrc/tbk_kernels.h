@@ -174,6 +174,7 @@ struct FermiArgs {
     const int *ncut;
     double *out;                   // efermi, earg1, earg2, esum, entropy, iters, smag
     int *cut = nullptr;            // [nk] occupation window per k (k_fermi_window)
+    int k0 = 0;                    // first item of the group in the plan
     // > 0: items [0, khalf) are spin up and [khalf, nk) spin down of one
     // structure (collinear, el = 1); out[6] = up - down electrons
     int khalf = 0;
@@ -284,6 +285,23 @@ void launch_fermi_sums_group(const FermiArgs *f_dev, int nstruct, hipStream_t st
 // E_F + 20 kT, moved up to the next spectral gap (no state above it enters
 // the Fermi sums; no group of close eigenvalues straddles it).
 void launch_fermi_window_group(const FermiArgs *f_dev, int nstruct, hipStream_t st);
+// Spin directions of non-collinear structures (diag.f:458-523,
+// bande.f:1484-1521): per state j of each item and atom a of its structure
+// spev[item][j][a][0..2] = (-2 Re, 2 Im, |z_dn|^2 - |z_up|^2) of conj(z_up) z_dn
+// summed over the atom's orbitals (states at or beyond cut[k] zero); then
+// per structure the occupation-weighted sum over k and states into
+// out[s][a][0..2] (one workgroup per Fermi group, E_F from its out[0]).
+struct SpinArgs {
+    int n, maxnat, k0;
+    size_t stride;
+    const StructDev *sd;
+    const int *item_s, *nact, *cut;
+    const double2 *z;
+    double *spev;                  // [items][n][maxnat][3]
+};
+void launch_spin_expect(const SpinArgs &a, int nb, hipStream_t st);
+void launch_spin_sum(const FermiArgs *f_dev, const StructDev *sd, int nstruct, const double *spev, int n,
+                     int maxnat, double *out, hipStream_t st);
 void launch_copy_int(const int *src, int *dst, int n, hipStream_t st);
 // Lowdin projections: S = I on every item; T[i, j] *= sqrt(lam[k][i]);
 // proj[k][e] = |Y[k][e]|^2 over the n x n slabs of nb items

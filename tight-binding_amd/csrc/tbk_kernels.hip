@@ -2626,6 +2626,69 @@ __global__ __launch_bounds__(1024) void k_fermi_window_group(const FermiArgs *fa
     }
 }
 
+// Spin expectation of each state on each atom (diag.f:458-523): one wave
+// per state, lanes over atoms.  The reference sums over kbas(kkind(matom))
+// orbitals of every atom (diag.f:488), the kind of its last atom slot; this
+// sums over each atom's own orbitals, the same where the structure fills
+// the reference's atom dimension (for fewer atoms that slot's kind is 0 and
+// the bound reads the word before kbas, the COMMON's number of kinds).
+__global__ __launch_bounds__(256) void k_spin_expect(SpinArgs a)
+{
+    const int kb = blockIdx.y, lane = threadIdx.x & 63, N = a.n;
+    const int j = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (j >= N) return;
+    const StructDev &S = a.sd[a.item_s[kb]];
+    const int Na = a.nact ? a.nact[kb] : N;
+    const bool on = S.spin && j < Na && (!a.cut || j < a.cut[kb]);
+    const double2 *z = a.z + (size_t)kb * a.stride + (size_t)j * N;
+    double *sp = a.spev + ((size_t)(a.k0 + kb) * N + j) * a.maxnat * 3;
+    for (int at = lane; at < S.natoms; at += 64) {
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        if (on) {
+            const int b0 = S.ksk[at], nb = S.nbas[at], nv = S.nv;
+            for (int ib = b0; ib < b0 + nb; ++ib) {
+                const double2 u = z[ib], d = z[ib + nv];
+                const double cr = u.x * d.x + u.y * d.y, ci = u.x * d.y - u.y * d.x;   // conj(u) d
+                sx = sx - cr - cr;
+                sy = sy + (ci + ci);
+                sz = sz + ((d.x * d.x + d.y * d.y) - (u.x * u.x + u.y * u.y));
+            }
+        }
+        sp[3 * at] = sx;
+        sp[3 * at + 1] = sy;
+        sp[3 * at + 2] = sz;
+    }
+}
+
+// The occupation-weighted sums of bande.f:1501-1521 (states below
+// E_F + 20 kT, weight sfac w_k, Fermi-Dirac within 20 kT of E_F), k and
+// states in order; one workgroup per structure, threads over (atom, axis).
+__global__ __launch_bounds__(256) void k_spin_sum(const FermiArgs *fa, const StructDev *sdv, const double *spev,
+                                                  int n, int maxnat, double *out)
+{
+    const FermiArgs &f = fa[blockIdx.x];
+    const StructDev &S = sdv[blockIdx.x];
+    const double ef = f.out[0], tkb = f.tkb, nearz = 1e-5;
+    const double tkbi = tkb < nearz ? 1.0 / nearz : 1.0 / tkb;
+    const double efp = ef + 20.0 * tkb, efm = ef - 20.0 * tkb;
+    for (int t = threadIdx.x; t < 3 * S.natoms; t += blockDim.x) {
+        const int at = t / 3, l = t % 3;
+        double acc = 0.0;
+        for (int k = 0; k < f.nk; ++k) {
+            const double *ev = f.esk + (size_t)k * f.ld;
+            const double wt = f.el * f.w[k];
+            const double *sp = spev + (size_t)(f.k0 + k) * n * maxnat * 3 + 3 * at + l;
+            for (int i = 0; i < f.nv; ++i) {
+                const double e = ev[i];
+                if (!(e < efp)) continue;
+                const double fbe = e > efm ? wt * (1.0 / (1.0 + exp(tkbi * (e - ef)))) : wt;
+                acc = acc + fbe * sp[(size_t)i * maxnat * 3];
+            }
+        }
+        out[((size_t)blockIdx.x * maxnat + at) * 3 + l] = acc;
+    }
+}
+
 // addspins.f:68-96 on dense matrices: with H, S the spatial matrices in the
 // leading nv x nv block of an item of order N = 2 nv (the rest decoupled
 // padding from k_build_hs), the spin-down block becomes H - diag(d), S; the
@@ -4083,6 +4146,15 @@ void launch_fermi_window_group(const FermiArgs *f_dev, int nstruct, hipStream_t 
 void launch_fermi_sums_group(const FermiArgs *f_dev, int nstruct, hipStream_t st, int ef_known)
 {
     hipLaunchKernelGGL(k_fermi_sums_group, dim3(nstruct), dim3(1024), 0, st, f_dev, ef_known);
+}
+void launch_spin_expect(const SpinArgs &a, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_spin_expect, dim3((a.n + 3) / 4, nb), dim3(256), 0, st, a);
+}
+void launch_spin_sum(const FermiArgs *f_dev, const StructDev *sd, int nstruct, const double *spev, int n,
+                     int maxnat, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_spin_sum, dim3(nstruct), dim3(256), 0, st, f_dev, sd, spev, n, maxnat, out);
 }
 void launch_copy16(const double2 *src, double2 *dst, size_t n, hipStream_t st)
 {

@@ -703,6 +703,10 @@ struct tbk_plan {
     // solve; off by default: at order 288 the one pass takes 1.5 ms against
     // 2 x 0.6 ms for the two
     bool fused_build = false;
+    // spin directions of non-collinear structures (bande.f:1484-1561):
+    // spev [nk][N][maxnat][3] per state, spdir [nstruct][maxnat][3]
+    bool spdir = false;
+    double *spev = nullptr, *spdir_out = nullptr;
     std::vector<int> info_host, err_host, status;
     std::vector<double> out_host;
     Workspace ws;
@@ -738,6 +742,7 @@ static void plan_free(tbk_plan *p)
     dfree(p->cut);
     dfree(p->ph);
     dfree(p->proj);
+    dfree(p->spev); dfree(p->spdir_out);
     p->ws.release();
     p->ws2.release();
     if (p->stream) {
@@ -1118,6 +1123,25 @@ extern "C" int tbk_plan_magnetization(const tbk_plan *p, double *smag)
     return TBK_OK;
 }
 
+extern "C" int tbk_plan_set_spin_directions(tbk_plan *p, int on)
+{
+    if (!p || !p->noncol) return TBK_ERR_ARG;
+    p->spdir = on != 0;
+    if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+    return TBK_OK;
+}
+
+extern "C" int tbk_plan_spin_directions(tbk_plan *p, int s, double *spdir)
+{
+    if (!p || !spdir || s < 0 || s >= p->nstruct || !p->spdir || !p->spdir_out) return TBK_ERR_ARG;
+    HIP_TRY(hipSetDevice(p->device));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    const size_t per = (size_t)p->maxnat * 3;
+    HIP_TRY(hipMemcpy(spdir, p->spdir_out + (size_t)s * per, (size_t)p->sh[s].natoms * 3 * sizeof(double),
+                      hipMemcpyDeviceToHost));
+    return TBK_OK;
+}
+
 extern "C" int tbk_plan_create(int device, const tbk_structure_desc *st, int nk,
                                const double *kpts, const double *weights, int batch,
                                int precision, tbk_plan **out)
@@ -1178,7 +1202,7 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
     // 8 and 9 only eigenvalues
     const bool qproj = mode == 6 || mode == 10;
     const bool win = p->window && lpress && p->batch >= p->nk;
-    const bool vecs = lpress || qproj || (!p->window && mode <= 4);
+    const bool vecs = lpress || qproj || (!p->window && mode <= 4) || p->spdir;
     const Window wnd{win ? p->fargs : nullptr, p->nfermi, win ? p->cut : nullptr, vecs};
     p->win_last = win;
     p->vec_last = vecs;
@@ -1202,6 +1226,11 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
         else
             solve_batch(p->ws, nb, nact, p->esk + (size_t)k0 * n, st, p->timer, wnd);
         tbk::launch_copy_int(p->ws.info, p->info_all + k0, nb, st);
+        if (p->spdir) {               // each state's spin on each atom (diag.f:458-523)
+            tbk::SpinArgs sa{n, p->maxnat, k0, ss, p->sdev, p->item_s + k0, nact, wnd.cut ? wnd.cut + k0 : nullptr,
+                             p->ws.Z, p->spev};
+            tbk::launch_spin_expect(sa, nb, st);
+        }
         if (qproj) {
             // S(k) again, into ws2.A (H into ws2.S, replaced by I), its
             // eigenpairs S = V diag(lambda) V^H, then Y = V (lambda^{1/2} (V^H Z))
@@ -1247,6 +1276,7 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
     }
     p->timer.begin("fermi_sums");
     tbk::launch_fermi_sums_group(p->fargs, p->nfermi, st, win ? 1 : 0);
+    if (p->spdir) tbk::launch_spin_sum(p->fargs, p->sdev, p->nfermi, p->spev, n, p->maxnat, p->spdir_out, st);
     p->timer.end();
 }
 
@@ -1276,6 +1306,16 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
         p->bytes += g_dev_bytes - before;
     }
     if (mode == 2 || mode == 4) ensure_deriv_buffers(p);
+    if (p->spdir && !p->spev) {
+        AllocStream as(p->stream);
+        const size_t before = g_dev_bytes;
+        const size_t per = (size_t)p->maxnat * 3;
+        if (dalloc(&p->spev, (size_t)p->nk * p->N * per) || dalloc(&p->spdir_out, (size_t)p->nstruct * per)) {
+            dfree(p->spev); dfree(p->spdir_out);
+            return TBK_ERR_NOMEM;
+        }
+        p->bytes += g_dev_bytes - before;
+    }
     p->last_mode = mode;
     const bool params_changed =
         !(p->g_mode == mode && p->g_tkb == tkb && p->g_eigcut == eigcut && p->g_window == p->window);
@@ -1290,6 +1330,7 @@ extern "C" int tbk_plan_launch(tbk_plan *p, int mode, double tkb, double eigcut)
             f.esk = p->esk + (size_t)h.k0 * p->N; f.desk = p->desk + (size_t)h.k0 * p->N;
             f.w = p->wts + h.k0; f.ncut = p->ncut + h.k0; f.out = p->out + 8 * (size_t)i;
             f.cut = p->cut + h.k0;
+            f.k0 = h.k0;
             if (p->noncol) f.el = 1.0;          // one electron per spinor state (bande.f:1069)
             if (p->spin) {          // both channels' items follow each other
                 f.nk = p->sh[0].nk + p->sh[1].nk;

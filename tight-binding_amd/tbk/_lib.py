@@ -30,6 +30,7 @@ EXPORTS = [
     "tbk_plan_stage_times", "tbk_plan_set_timing", "tbk_plan_autotune", "tbk_gemm_tile",
     "tbk_plan_create_group", "tbk_plan_result", "tbk_plan_structure_eigenvalues",
     "tbk_plan_create_spin", "tbk_plan_magnetization", "tbk_plan_create_noncol",
+    "tbk_plan_set_spin_directions", "tbk_plan_spin_directions",
     "tbk_plan_set_graph", "tbk_plan_set_window", "tbk_plan_set_fused_build", "tbk_plan_set_tridiag", "tbk_plan_order", "tbk_plan_nstruct", "tbk_plan_device_bytes",
     "tbk_multi_create", "tbk_multi_launch", "tbk_multi_wait", "tbk_multi_run", "tbk_multi_result",
     "tbk_multi_eigenvalues", "tbk_multi_ngroups", "tbk_multi_group_order", "tbk_multi_set_graph",
@@ -125,6 +126,8 @@ def lib():
         L.tbk_plan_create_spin.argtypes = [C.c_int, C.POINTER(StructureDesc), d, vp, C.c_int, d, d,
                                             C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.tbk_plan_magnetization.argtypes = [vp, dp]
+        L.tbk_plan_set_spin_directions.argtypes = [vp, C.c_int]
+        L.tbk_plan_spin_directions.argtypes = [vp, C.c_int, dp]
         L.tbk_plan_create_noncol.argtypes = [C.c_int, C.POINTER(StructureDesc), d, d, C.c_int, d, d,
                                               C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.tbk_plan_result.argtypes = [vp, C.c_int, dp, dp, dp]

@@ -370,6 +370,19 @@ class NonCollinearPlan(Plan):
         if graph:
             L.tbk_plan_set_graph(h, 1)
 
+    def set_spin_directions(self, on: bool = True):
+        """Form each atom's spin vector in the following runs (the lspdir
+        output of bande.f:1484-1561)."""
+        check(lib().tbk_plan_set_spin_directions(self.h, 1 if on else 0), "tbk_plan_set_spin_directions")
+
+    def spin_directions(self) -> np.ndarray:
+        """[natoms, 3]: each atom's spin vector after a run with
+        set_spin_directions(True)."""
+        out = np.zeros(3 * self.st.natoms)
+        check(lib().tbk_plan_spin_directions(self.h, 0, out.ctypes.data_as(C.POINTER(C.c_double))),
+              "tbk_plan_spin_directions")
+        return out.reshape(-1, 3)
+
 
 class MultiPlan:
     """Many structures of uneven orbital count, grouped by padded order

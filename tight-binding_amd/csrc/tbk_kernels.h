@@ -38,6 +38,9 @@ struct BuildArgs {
     const double2 *ph;             // [nb][maxpairs] Bloch phases
     double2 *h, *s;
     double2 *dh = nullptr, *ds = nullptr;   // non-null: H, S, dH/dV, dS/dV in one pass (deriv unused)
+    // 1: the strictly upper triangle is not written (for consumers that
+    // read only the lower triangle and the diagonal, GEMM tri 9)
+    int lower_only = 0;
 };
 
 template <class R> struct cplx_t;

@@ -263,7 +263,8 @@ __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a, int nb)
                 if (live) {
                     v[m] = (ia == ja && b == o) ? make_double2(diag_value(SD, m, acc[u][2 * m], cls), 0.0)
                                                 : make_double2(acc[u][2 * m], acc[u][2 * m + 1]);
-                    out[m][(size_t)kb * a.stride + (size_t)(c0 + b) * N + r0 + o] = v[m];
+                    if (!a.lower_only || (ia == ja && b == o))
+                        out[m][(size_t)kb * a.stride + (size_t)(c0 + b) * N + r0 + o] = v[m];
                 }
                 if (il < 3) tr[m][il][e] = v[m];
             }
@@ -311,7 +312,7 @@ __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a, int nb)
             const double2 v = (ia == ja && b == o) ? make_double2(diag_value(SD, m, acc[2 * m], cls), 0.0)
                                                    : make_double2(acc[2 * m], acc[2 * m + 1]);
             double2 *M = out[m] + (size_t)kb * a.stride;
-            M[(size_t)(c0 + b) * N + r0 + o] = v;
+            if (!a.lower_only || (ia == ja && b == o)) M[(size_t)(c0 + b) * N + r0 + o] = v;
             if (ia < ja || b > o) M[(size_t)(r0 + o) * N + c0 + b] = make_double2(v.x, -v.y);   // conjugate partner
         }
     }

@@ -1253,8 +1253,11 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
         if (lpress) {
             p->timer.begin("pressure");
             if (!fused) {
-                tbk::launch_build_hs(build_args(p, k0, true), p->maxnat, nb, st);
-                if (p->noncol) tbk::launch_spinor_expand(build_args(p, k0, true), p->maxnorb, nb, st);
+                // the products below read the lower triangles only
+                tbk::BuildArgs db = build_args(p, k0, true);
+                db.lower_only = 1;
+                tbk::launch_build_hs(db, p->maxnat, nb, st);
+                if (p->noncol) tbk::launch_spinor_expand(db, p->maxnorb, nb, st);
             }
             // X = L_dH Z, Y = L_dS Z over the lower triangles (half the
             // products; k_perturb adds the conjugate halves)

@@ -41,6 +41,9 @@ struct BuildArgs {
     // 1: the strictly upper triangle is not written (for consumers that
     // read only the lower triangle and the diagonal, GEMM tri 9)
     int lower_only = 0;
+    // 1: the strictly lower triangle of S is not written (the Cholesky
+    // reads the upper triangle only)
+    int s_upper_only = 0;
 };
 
 template <class R> struct cplx_t;

@@ -273,6 +273,7 @@ __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a, int nb)
                 // partner of upper element (o, b) = (omega, beta): row c0 + beta, column r0 + omega
 #pragma unroll
                 for (int m = 0; m < NM; ++m) {
+                    if (m == 1 && a.s_upper_only) continue;
                     const double2 pv = tr[m][il][omega + 9 * beta];
                     out[m][(size_t)kb * a.stride + (size_t)(r0 + omega) * N + c0 + beta] = make_double2(pv.x, -pv.y);
                 }
@@ -313,7 +314,8 @@ __global__ __launch_bounds__(256) void k_build_hs(BuildArgs a, int nb)
                                                    : make_double2(acc[2 * m], acc[2 * m + 1]);
             double2 *M = out[m] + (size_t)kb * a.stride;
             if (!a.lower_only || (ia == ja && b == o)) M[(size_t)(c0 + b) * N + r0 + o] = v;
-            if (ia < ja || b > o) M[(size_t)(r0 + o) * N + c0 + b] = make_double2(v.x, -v.y);   // conjugate partner
+            if ((ia < ja || b > o) && !(m == 1 && a.s_upper_only))
+                M[(size_t)(r0 + o) * N + c0 + b] = make_double2(v.x, -v.y);   // conjugate partner
         }
     }
 }

@@ -1214,7 +1214,11 @@ static void enqueue_run(tbk_plan *p, int mode, double eigcut)
         const int nb = std::min(p->batch, p->nk - k0);
         p->timer.begin("build_hs");
         tbk::launch_phases(p->sdev, p->item_s + k0, p->kp + 3 * (size_t)k0, p->maxpairs, p->ph, nb, st);
-        tbk::launch_build_hs(build_args(p, k0, false, fused), p->maxnat, nb, st);
+        {
+            tbk::BuildArgs hb = build_args(p, k0, false, fused);
+            hb.s_upper_only = fused ? 0 : 1;
+            tbk::launch_build_hs(hb, p->maxnat, nb, st);
+        }
         if (p->noncol) {
             tbk::launch_spinor_expand(build_args(p, k0, false), p->maxnorb, nb, st);
             if (fused) tbk::launch_spinor_expand(deriv_args(p, k0), p->maxnorb, nb, st);

@@ -1398,6 +1398,14 @@ __global__ __launch_bounds__(256) void k_stebz(SteinArgs s)
     const TriBounds tb = tri_bounds(d, e, e2, N, red4);
     const double eps = DBL_EPSILON * 0.5;
     if (blockIdx.x == 0 && tid == 0) s.info[kb] = 0;
+    // one shared step: the counts at the 256 interior points of a uniform
+    // grid over the Gershgorin interval, one per thread, bracket every
+    // eigenvalue to 1/257 of it (about five trisection steps)
+    __shared__ int gcnt[256];
+    const int G = (int)blockDim.x;
+    const double gh = (tb.gu - tb.gl) / (double)(G + 1);
+    gcnt[tid] = sturm_count(d, e2, N, tb.gl + (double)(tid + 1) * gh, tb.pivmin);
+    __syncthreads();
     const int gt = blockIdx.x * blockDim.x + tid;
     const int j = gt >> 1, side = gt & 1;
     if (j >= LD) return;
@@ -1406,6 +1414,20 @@ __global__ __launch_bounds__(256) void k_stebz(SteinArgs s)
         return;
     }
     double a = tb.gl, b = tb.gu;
+    {
+        // the first grid point whose count exceeds j; the bracket is kept
+        // only if the counts around it confirm it (they are monotone but
+        // for rounding)
+        int lo = 0, hi = G;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (gcnt[mid] >= j + 1) hi = mid; else lo = mid + 1;
+        }
+        if ((lo == 0 || gcnt[lo - 1] < j + 1) && (lo == G || gcnt[lo] >= j + 1)) {
+            if (lo > 0) a = tb.gl + (double)lo * gh;
+            if (lo < G) b = tb.gl + (double)(lo + 1) * gh;
+        }
+    }
     const double atol = s.abstol_rel * tb.onenrm;
     for (int it = 0; it < 200; ++it) {
         const double mid = 0.5 * (a + b);

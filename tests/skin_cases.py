@@ -158,7 +158,7 @@ def kpoint_blocks(group_fcc, kfile):
 # ---------------------------------------------------------------------------
 STATIC_CASES = ("ex1", "ex1_mode1", "ex1_mode2", "ex1_mode3", "ex2_aligned", "ex2_published", "lattices",
                 "spin_flip", "noncol", "ex1_qlmt6", "ex1_qlmt10", "ex1_qlmt9", "ex1_qapw8", "ex2_qlmt6",
-                "ex2_qlmt10", "spin_qlmt6", "noncol_qlmt6", "noncol_spdir")
+                "ex2_qlmt10", "spin_qlmt6", "noncol_qlmt6", "noncol_spdir", "noncol20")
 # cases run by the reference program built for the spin-direction output
 # (oracle/Makefile spdir: msdir = 1), and the files they add
 SPDIR_CASES = {"noncol_spdir": "SPNDIR"}
@@ -191,9 +191,9 @@ def _mode(text, mode):
     return "\n".join(lines) + "\n"
 
 
-def spdir_structure():
+def spdir_structure(flag="S"):
     """A 20-atom Pd cell for the spin-direction output (input1.f: spin flag
-    'S', non-collinear with SPNDIR): five conventional fcc cells stacked
+    'S', non-collinear with SPNDIR; 'N' plain non-collinear): five conventional fcc cells stacked
     along z (simple tetragonal, a = 7.35, c = 5a), every atom with its own
     spin direction (theta, phi in degrees), on an explicit list of 8
     k-points with the trivial group.  Twenty atoms fill the reference
@@ -207,8 +207,9 @@ def spdir_structure():
             t = 4 * n + i
             theta, phi = 15.0 + 7.0 * t, 20.0 + 13.0 * t
             atoms.append(f" 1  {x:.3f}  {y:.3f}  {(z + n) / 5:.3f}  0 0 0  {theta:.1f} {phi:.1f}")
-    block = structure_block((5, f"{a} {5 * a}"), (0, None), atoms, flags="FS")
-    return ("Palladium spin directions", "pd20 spdir", 0.0, block, kblock_explicit(K8, W8))
+    block = structure_block((5, f"{a} {5 * a}"), (0, None), atoms, flags="F" + flag)
+    return ("Palladium spin directions", "pd20 spdir" if flag == "S" else "pd20 noncol", 0.0, block,
+            kblock_explicit(K8, W8))
 
 
 def build_static_case(name, work, ex1_dir, ex2_dir):
@@ -233,13 +234,13 @@ def build_static_case(name, work, ex1_dir, ex2_dir):
     for f in ("pd_par", "fcc.08", "bcc.08", "spcgrp.fcc", "spcgrp.bcc"):
         shutil.copy(os.path.join(ex1_dir, f), os.path.join(work, f))
     skin = open(os.path.join(ex1_dir, "SKIN")).read()
-    if name == "noncol_spdir":                 # two-channel pd_par, one 20-atom structure
+    if name in ("noncol_spdir", "noncol20"):   # two-channel pd_par, one 20-atom structure
         par = open(os.path.join(ex1_dir, "pd_par")).read()
         pdn = spin_case.spin_down_values(spin_case.read_channel(par))
         os.remove(os.path.join(work, "pd_par"))
         open(os.path.join(work, "pd_par"), "w").write(spin_case.nm_text(par, pdn))
         write_trivial_group(os.path.join(work, "trivial.grp"))
-        write_skin(os.path.join(work, "SKIN"), "pd_par", [spdir_structure()])
+        write_skin(os.path.join(work, "SKIN"), "pd_par", [spdir_structure("S" if name == "noncol_spdir" else "N")])
         return
     if name == "ex1":
         out = skin

@@ -130,3 +130,26 @@ def test_device_perturb_matches_oracle(n, degen):
     print(f"  ntot {len(got)} vs {len(ref)}, max|d deig| = {np.abs(got - ref).max():.2e}", flush=True)
     assert len(got) == len(ref)
     assert np.abs(got - ref).max() < 1e-11 * scale
+
+
+@pytest.mark.skipif(not os.path.exists(EXE) or not os.path.isdir(INPUTS) or not os.path.isdir(INPUTS2),
+                    reason="drop-in program not built (oracle/Makefile dropin)")
+def test_reference_program_with_device_bande_twenty_spins(tmp_path):
+    """A 20-atom non-collinear cell whose atoms all point different ways
+    (tests/skin_cases.py noncol20): the reference program through the device
+    bande_ (every atom's spinat through the drop-in) writes the SKENG line
+    of the unmodified program (tests/golden/static.json)."""
+    import skin_cases as SC
+    want = json.load(open(os.path.join(ROOT, "tests", "golden", "static.json")))["noncol20"]["skeng"]
+    SC.build_static_case("noncol20", tmp_path, INPUTS, INPUTS2)
+    print("case: static_tbk on a 20-atom non-collinear cell", flush=True)
+    r = subprocess.run("ulimit -s unlimited; " + EXE, shell=True, cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300, env=dict(os.environ, TBK_DEVICE="0"))
+    assert r.returncode == 0, (r.stdout[-1000:], r.stderr[-2000:])
+    got = [l.rstrip("\n") for l in open(tmp_path / "SKENG") if l.strip()]
+    assert len(got) == len(want) == 1
+    a, b = got[0].split(), want[0].split()
+    assert a[:2] == b[:2]
+    worst = max(abs(float(x) - float(y)) for x, y in zip(a[2:], b[2:]))
+    print(f"  {got[0]}\n  {want[0]}\n  worst |SKENG - reference program| = {worst:.1e}", flush=True)
+    assert worst <= 1.01e-9

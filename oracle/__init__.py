@@ -111,6 +111,7 @@ def ref():
         L.ref_setpar_spin.argtypes = [ip, dp, _d, ip, _i, _d, _d, _d, _d, _d, _d, _d, ip]
         L.ref_setpar_nc.argtypes = [dp, _d, ip, _d, _d, _d, _d, _d, _d, _d, _d, _d, ip]
         L.ref_bande_nc.argtypes = [ip, ip, _d, _d, dp, dp, dp, dp, _d, ip, _d, dp, dp, dp, ip]
+        L.ref_setham_nc.argtypes = [_d, ip, _d, _d, _c, _c, ip]
         L.ref_bande_spin.argtypes = [ip, ip, _d, _d, dp, dp, dp, dp, _d, ip, _i, dp, dp, dp, dp, ip]
         L.ref_kptin.argtypes = [_d, C.c_char_p, ip, C.c_char_p, ip, ip, ip, _d, _d, _d, _d, _d,
                                 dp, ip, ip, _d, _d, _d, ip]
@@ -331,6 +332,22 @@ class RefSetup:
         if ierr.value:
             raise RuntimeError(f"ref_setpar_nc ierr={ierr.value}")
         return out
+
+    def hs_noncol(self, ak, spinat, spin_onsite, deriv=False):
+        """The reference's spinor H(k), S(k) (ideriv 0) or their volume
+        derivatives (ideriv 1): spinham + setham/sethamv + addspins, as
+        bande.f:897-962 and 1024-1034 build them. spin_onsite is
+        setpar_noncol's spinos (dspnos for deriv), which must run first;
+        returns the packed upper triangles of order 2 nv."""
+        n2 = 2 * self.nv
+        hp = np.zeros(n2 * (n2 + 1) // 2, np.complex128); sp = np.zeros_like(hp)
+        ierr = C.c_int(0)
+        ref().ref_setham_nc(np.asarray(ak, float), C.byref(C.c_int(int(deriv))),
+                            np.ascontiguousarray(np.asarray(spinat, float).ravel()),
+                            np.ascontiguousarray(np.asarray(spin_onsite, float)), hp, sp, C.byref(ierr))
+        if ierr.value:
+            raise RuntimeError(f"ref_setham_nc ierr={ierr.value}")
+        return hp, sp
 
     def bande_noncol(self, kpts, weights, param_dn, spinat, mode=4, tkb=0.005, eigcut=10.0):
         """The reference's bande with lnoncol (jspins = 2): param(:,2) =

@@ -334,6 +334,71 @@ c
       return
       end
 c
+c     ref_setham_nc: the spinor H(k), S(k) of bande.f:897-962 (ideriv 0)
+c     or their volume derivatives of bande.f:905-907, 1024-1034 (ideriv
+c     1) at one k: spinham on the spins spn (3,natoms) and the on-site
+c     spin terms spo (spinos or dspnos of ref_setpar_nc, which must run
+c     first), setham / sethamv into the packed nv triangle, and addspins
+c     to the packed 2nv triangle, copied out as (2nv)(2nv+1)/2 entries.
+      subroutine ref_setham_nc(ak,ideriv,spn,spo,hpk,spk,ierr)
+     $     bind(C,name='ref_setham_nc')
+      implicit real*8 (a-h,o-z)
+      include 'P1'
+      real*8 ak(3),spn(3,*),spo(4,*)
+      integer ideriv,ierr
+      complex*16 hpk(*),spk(*)
+      common /struc/ nv
+      common /codes/posn(matom,3),kkind(matom),natoms
+      common /parinfo/ valence(mkind),kinds,kbas(mkind)
+      logical jsover
+      common /lia/ npts,jsover
+      common /refstate/ paros(4,matom),hop(mpkind,mpair),
+     $     over(mpkind,mpair),dparos(4,matom),dhop(mpkind,mpair),
+     $     dover(mpkind,mpair),den(mkind,matom,mspin)
+      complex*16 hmat(mh*(mh+1)/2),smat(mh*(mh+1)/2)
+      real*8 spinat(3,matom),spinos(4,matom)
+      real*8 dspth(3,matom),dspphi(3,matom)
+      real*8 spndiag(mh/mcol),spndth(mh/mcol)
+      complex*16 spnmix(mh/mcol),spnmxth(mh/mcol),spndph(mh/mcol)
+      ierr = 0
+      if(mcol.lt.2) then
+         ierr = 10
+         return
+      end if
+      do i = 1,matom
+         do j = 1,3
+            spinat(j,i) = 0d0
+            dspth(j,i) = 0d0
+            dspphi(j,i) = 0d0
+         end do
+         do j = 1,4
+            spinos(j,i) = 0d0
+         end do
+      end do
+      do i = 1,natoms
+         do j = 1,3
+            spinat(j,i) = spn(j,i)
+         end do
+         do j = 1,4
+            spinos(j,i) = spo(j,i)
+         end do
+      end do
+      call spinham(natoms,kkind,kbas,spinat,spinos,
+     $     spndiag,spnmix,.false.,dspth,dspphi,
+     $     spndth,spnmxth,spndph)
+      if(ideriv.eq.0) then
+         call setham(jsover,ak,paros,hop,over,hmat,smat,.false.)
+      else
+         call sethamv(jsover,ak,dparos,dhop,dover,hmat,smat,.false.)
+      end if
+      call addspins(nv,spndiag,spnmix,hmat,smat,.false.)
+      do i = 1,(2*nv)*(2*nv+1)/2
+         hpk(i) = hmat(i)
+         spk(i) = smat(i)
+      end do
+      return
+      end
+c
 c     diag with packed inputs (copied); eigenvectors out as z(nv,nv).
 c
       subroutine ref_diag(mode,hpk,spk,eo,zo,ioverr)

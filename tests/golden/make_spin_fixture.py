@@ -8,6 +8,9 @@ atoms, k-points) and the reference's E_F, earg1, earg2 and smag per mode,
 from oracle/_ref/libnrlref.so (the reference's sources compiled where they
 lie, NRLtb/ex1's P1 with mspin = 2) through its bande, on the CPU.  The
 spin-up channel is the Pd set of tight-binding_amd/tbk/data/pd_par.json.
+Also writes tests/golden/spin_hs.npz: the reference's spinor H(k), S(k) and
+their volume derivatives (spinham + setham/sethamv + addspins) of the 4-atom
+non-collinear case at one general k-point, packed upper triangles.
 Run in a tree where the reference is built:
     make -C oracle ref && python tests/golden/make_spin_fixture.py
 """
@@ -108,6 +111,23 @@ def noncol_cases():
     return out[0::2]
 
 
+# a k-point off every symmetry element (the units of the cases' k-points)
+GENERAL_K = [0.137, -0.291, 0.413]
+
+
+def spinor_matrices():
+    """The reference's spinor H, S, dH/dV, dS/dV (packed upper triangles of
+    order 2 nv) of the 4-atom non-collinear case, whose spins point along
+    z, x, y and (1,1,1), at GENERAL_K."""
+    c, st = [x for x in noncol_cases() if len(x[0]["posn"]) == 4][0]
+    R = oracle.RefSetup(st)
+    par = R.setpar_noncol(np.array(c["param_dn"]))
+    h, s = R.hs_noncol(GENERAL_K, c["spinat"], par["spinos"], deriv=False)
+    dh, ds = R.hs_noncol(GENERAL_K, c["spinat"], par["dspnos"], deriv=True)
+    return dict(name=np.array(c["name"]), k=np.array(GENERAL_K), nv2=np.array(2 * st.nv),
+                h=h, s=s, dh=dh, ds=ds)
+
+
 def main():
     res = []
     for c in cases():
@@ -139,6 +159,10 @@ def main():
                tkb=0.005, eigcut=10.0, nd=[4, 4, 4], cases=res, ex1_magnetic=ex1, noncol=nc)
     path = os.path.join(HERE, "spin.json")
     json.dump(out, open(path, "w"), indent=1)
+    print(f"wrote {path}")
+    mats = spinor_matrices()
+    path = os.path.join(HERE, "spin_hs.npz")
+    np.savez_compressed(path, **mats)
     print(f"wrote {path}")
 
 

@@ -238,6 +238,30 @@ def test_noncollinear_plan_vs_reference(idx):
     m.close()
 
 
+def test_noncollinear_spinor_matrices_vs_reference():
+    """The device's spinor H(k), S(k) and dH/dV, dS/dV of order 2 nv, element
+    by element, against the reference's spinham + setham/sethamv + addspins
+    (tests/golden/spin_hs.npz: 4 atoms, spins along z, x, y and (1,1,1), one
+    general k-point)."""
+    from test_spin import GOLD_HS, packed_upper_full
+    gh = np.load(GOLD_HS)
+    c = [c for c in load()["noncol"] if c["name"] == str(gh["name"])][0]
+    st = structure(c)
+    n2 = int(gh["nv2"])
+    print(f"case: non-collinear spinor H/S {c['name']} at k = {list(gh['k'])}, order {n2}", flush=True)
+    plan = NonCollinearPlan(st, c["param_dn"], c["spinat"], [list(gh["k"])], [1.0])
+    try:
+        assert plan.nv == n2
+        for deriv, hk, sk in ((False, "h", "s"), (True, "dh", "ds")):
+            h, s = plan.hs(0, deriv)
+            dh = float(np.abs(h - packed_upper_full(gh[hk], n2)).max())
+            ds = float(np.abs(s - packed_upper_full(gh[sk], n2)).max())
+            print(f"  deriv={deriv}: |dH|={dh:.1e} |dS|={ds:.1e}", flush=True)
+            assert dh < 1e-14 and ds < 1e-14, (deriv, dh, ds)
+    finally:
+        plan.close()
+
+
 def test_noncollinear_rotation_and_collinear_limit():
     """A common rotation of every spin leaves the spectrum; spins along z
     with channels differing only in decay and on-site values reproduce the

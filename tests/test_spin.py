@@ -209,3 +209,65 @@ def test_reference_noncollinear_invariants():
         r = R.bande_noncol(np.array(nc["kpts"]), np.array(nc["weights"]), np.array(nc["param_dn"]),
                            np.array(nc["spinat"]), mode=4)
         assert r.pop("err") == 0 and r == nc["modes"]["4"], nc["name"]
+
+
+GOLD_HS = os.path.join(ROOT, "tests", "golden", "spin_hs.npz")
+
+
+def packed_upper_full(ap, n):
+    """Full Hermitian matrix from a packed upper triangle ((i, j), i <= j,
+    at j(j+1)/2 + i, 0-based), indexed [row, col]."""
+    a = np.zeros((n, n), np.complex128)
+    iu = np.triu_indices(n)
+    order = np.argsort(iu[1] * (n + 1) + iu[0], kind="stable")   # column-major packed order
+    rows, cols = iu[0][order], iu[1][order]
+    a[rows, cols] = ap
+    a[cols, rows] = np.conj(ap)
+    return a
+
+
+def test_spinor_matrix_fixture_structure():
+    """The reference's spinor H, S and dH/dV, dS/dV (spin_hs.npz, 4 atoms
+    with spins along z, x, y and (1,1,1)) have addspins.f's form: S and
+    dS/dV block-diagonal with equal blocks, the spin terms on-site and
+    diagonal in the orbital, and per orbital (Re m, Im m, d) parallel to its
+    atom's spin, where d = (H_uu - H_dd)/2 and m = H_ud on the diagonal."""
+    g = np.load(GOLD_HS)
+    n2 = int(g["nv2"]); nv = n2 // 2
+    spins = np.array([c for c in json.load(open(GOLD))["noncol"] if c["name"] == str(g["name"])][0]["spinat"])
+    for hk, sk in (("h", "s"), ("dh", "ds")):
+        h, s = packed_upper_full(g[hk], n2), packed_upper_full(g[sk], n2)
+        assert np.array_equal(s[:nv, nv:], np.zeros((nv, nv))) and np.array_equal(s[:nv, :nv], s[nv:, nv:])
+        huu, hdd, hud = h[:nv, :nv], h[nv:, nv:], h[:nv, nv:]
+        d = np.diag(huu - hdd).real / 2
+        assert np.array_equal(huu - hdd, np.diag(np.diag(huu - hdd)))
+        assert np.array_equal(hud, np.diag(np.diag(hud)))
+        m = np.diag(hud)
+        per = nv // len(spins)
+        for i in range(nv):
+            v = np.array([m[i].real, m[i].imag, d[i]])
+            u = spins[i // per] / np.linalg.norm(spins[i // per])
+            assert np.linalg.norm(np.cross(v, u)) <= 1e-14 * max(1.0, np.linalg.norm(v)), (hk, i)
+        assert np.abs(d).max() > 0 and np.abs(m.imag).max() > 0, hk   # the y spin gives Im m
+
+
+@needs_ref
+def test_reference_reproduces_spinor_matrices():
+    """spin_hs.npz is what the reference's spinham + setham/sethamv +
+    addspins give now, and the mean of its spin blocks is the reference's
+    own single-spin setham on the averaged channel."""
+    g = np.load(GOLD_HS)
+    c = [c for c in json.load(open(GOLD))["noncol"] if c["name"] == str(g["name"])][0]
+    st = structure(c)
+    R = oracle.RefSetup(st)
+    par = R.setpar_noncol(np.array(c["param_dn"]))
+    h, s = R.hs_noncol(g["k"], c["spinat"], par["spinos"])
+    dh, ds = R.hs_noncol(g["k"], c["spinat"], par["dspnos"], deriv=True)
+    for a, b in ((h, "h"), (s, "s"), (dh, "dh"), (ds, "ds")):
+        assert np.array_equal(a, g[b]), b
+    nv = st.nv
+    for deriv, hk in ((False, "h"), (True, "dh")):
+        hp, _ = R.hs(g["k"], deriv)
+        h0 = packed_upper_full(hp, nv)
+        full = packed_upper_full(g[hk], 2 * nv)
+        assert np.abs((full[:nv, :nv] + full[nv:, nv:]) / 2 - h0).max() <= 1e-15 * max(1.0, np.abs(h0).max())

@@ -18,7 +18,7 @@ import pytest
 
 import spin_case
 from conftest import ROOT
-from tbk._lib import TBK_MIXED
+from tbk._lib import TBK_FP64, TBK_MIXED
 from tbk import structures as S
 from tbk.api import NonCollinearPlan, Plan, SpinPlan
 
@@ -260,6 +260,33 @@ def test_noncollinear_spinor_matrices_vs_reference():
             assert dh < 1e-14 and ds < 1e-14, (deriv, dh, ds)
     finally:
         plan.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_noncollinear_c2_vs_reference(prec):
+    """The C2 structure (32 atoms, spinor order 576) with a spin direction
+    per atom, 2x2x2 mesh, mode 4, against the reference's lnoncol bande
+    built with matom = 128 (tests/golden/c2_noncol.json). FP64 within the
+    small cases' 1e-9 (measured 3.6e-14 on E(T)); mixed within MIX_SPIN for
+    E_F and MIX_SPIN_P for P(T), and 1e-11 Ry for the 32-atom cell's E(T)
+    (measured 3.0e-12 Ry, 9.5e-14 per atom; P(T) 2.3e-13 Ry/bohr^3)."""
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "c2_noncol.json")))
+    st = S.config_c2(g["seed"])
+    pdn = spin_case.spin_down_values(st.params.param, seed=g["dn_seed"], n_changed=g["dn_changed"])
+    print(f"case: C2 non-collinear ({st.natoms} atoms, order {2 * st.nv}, {len(g['kpts'])} k), {prec}",
+          flush=True)
+    plan = NonCollinearPlan(st, pdn, g["spinat"], g["kpts"], g["weights"],
+                            precision=TBK_MIXED if prec == "mixed" else TBK_FP64)
+    try:
+        r = plan.run(g["mode"], g["tkb"], g["eigcut"])
+    finally:
+        plan.close()
+    err = {key: abs(r[key] - g["result"][key]) for key in ("efermi", "earg1", "earg2")}
+    print(f"  |device - reference| = {err}", flush=True)
+    tol = dict(efermi=1e-9, earg1=1e-9, earg2=1e-9) if prec == "fp64" else \
+        dict(efermi=MIX_SPIN, earg1=1e-11, earg2=MIX_SPIN_P)
+    for key in err:
+        assert err[key] <= tol[key], (key, err[key], tol[key])
 
 
 def test_noncollinear_rotation_and_collinear_limit():
